@@ -1,0 +1,159 @@
+/* librc — dependency-aware radiance cache on AMD Instinct MI355X (gfx950).
+ *
+ * C ABI of the hot path of the prototype (cuda/poc_gpu.cu): BVH ray traversal, the cell
+ * hash grid with 64-bit keys, the per-entry cache update / verification / invalidation,
+ * per-entry light-dependency masks and transit-grid DDA masks, behind the prototype's
+ * scene / event / method interface. The prototype exposes no library API — it is a CLI
+ * whose Runner::run (cuda/poc_gpu.cu:826-1016) drives the kernels; every entry point
+ * below names the reference function or block it replaces.
+ *
+ * Conventions: plain pointers and sizes; pointers named d_* are device pointers (HIP),
+ * `stream` is a hipStream_t passed as void* (NULL = the handle's own stream). Every call
+ * returns an rc_status; rc_last_error() gives the message of the last failure on the
+ * calling thread. Handles are thread-compatible, not thread-safe. Nothing calls exit().
+ */
+#ifndef RC_H
+#define RC_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum rc_status {
+    RC_OK = 0,
+    RC_ERR_ARG = 1,      /* invalid argument / shape */
+    RC_ERR_HIP = 2,      /* HIP runtime error (launch, allocation, copy) */
+    RC_ERR_NODEV = 3,    /* no usable GPU */
+    RC_ERR_STATE = 4,    /* call out of order (e.g. frame before begin) */
+    RC_ERR_SCENE = 5     /* scene preparation failed */
+} rc_status;
+
+typedef struct rc_scene rc_scene;
+typedef struct rc_cache rc_cache;
+
+/* Method flags: the prototype's Method struct (cuda/poc_gpu.cu:803-805) plus the SDK bench's
+ * flush / targeted-invalidation switches (windows/run.sh:98-103,128; rtxgi_bench.patch:363-386,
+ * :1028-1050). The four north-star methods are: no cache (rc_render with no_cache=1),
+ * plain (mode 0), flush + snapshots (flush=1, snaps=1), targeted (targeted=1, deps=1). */
+typedef struct rc_method {
+    int mode;            /* 0: floor(budget)+Bernoulli rays per entry, 1: adaptive share (:445-454) */
+    float nmax, budget;  /* history cap in frames, rays per entry */
+    int deps;            /* record dependency masks in k_trace2 */
+    int verify;          /* occluder events open a verification window (k_decide) */
+    int oracle;          /* shrink history by the true reference change (:676-681) */
+    int track_every;     /* masks recorded every n-th frame */
+    int rescale;         /* per-light channel rescale on intensity change (:875-882) */
+    uint64_t resp_mask;  /* lights blended with the responsive cap (SHaRC emulation) */
+    float nmax_resp;
+    int ddgi; float ddgi_h, ddgi_thr; int ddgi_clamp;
+    int memory, ramp, wake_verify, snaps, snap_verify;
+    int flush;           /* event to a state not in a snapshot slot clears the whole cache */
+    int targeted;        /* dependency-targeted invalidation at the event */
+} rc_method;
+
+#ifndef RC_CAMERA_DEFINED
+#define RC_CAMERA_DEFINED
+typedef struct rc_camera {
+    float pos[3], fwd[3], right[3], up[3];
+    float tanx, tany;
+    int32_t W, H, spp, bounces, cache_bounce, pad0;
+} rc_camera;
+#endif
+
+typedef struct rc_scene_info {
+    int Np, npst, nbox, nst, nnodes, nwnodes, NS, nl, NR, Nq, hsize, vx, vy, vz, nw, bvh_depth, wide_depth, has_ref;
+    float h, lo[3], ext[3], diag, sun_len;
+    rc_camera cam;
+} rc_scene_info;
+
+typedef struct rc_frame_stats {
+    float ms_swap, ms_trace, ms_decide, ms_update, ms_event;   /* GPU time per phase (:948-1002) */
+    double err;            /* relative RMS vs the converged reference, -1 without one (:1005-1010) */
+    int event;             /* an event block ran this frame */
+    int invalidated;       /* entries reset by targeted invalidation / verification opened */
+    long long rays;        /* rays traced by the entry pass (sum of k) */
+} rc_frame_stats;
+
+/* host arrays of a scene (rc_scene_array); state-dependent arrays take a state index */
+enum {
+    RC_A_ST = 0, RC_A_STN, RC_A_BN, RC_A_WN, RC_A_HK, RC_A_HV, RC_A_P, RC_A_N, RC_A_ALB, RC_A_REGION,
+    RC_A_L, RC_A_LTYPE, RC_A_INTEN, RC_A_Q, RC_A_DIMS, RC_A_OFF, RC_A_T, RC_A_TQ, RC_A_TALB,
+    RC_A_EDIR, RC_A_SV, RC_A_LM, RC_A_OCC, RC_A_EM, RC_A_REF, RC_A_VALID, RC_A_PW, RC_A_ORDER, RC_A_OBJBOX,
+    RC_A_MESH_V, RC_A_MESH_ALB, RC_A_MESH_DS, RC_A_COUNT
+};
+/* cache arrays (rc_cache_get / rc_cache_set) */
+enum {
+    RC_C_LC = 0, RC_C_M, RC_C_N, RC_C_KK, RC_C_PENDING, RC_C_CUR, RC_C_PREV, RC_C_LCUR, RC_C_LPREV,
+    RC_C_SUMPR, RC_C_NPEND, RC_C_AGE, RC_C_STALE, RC_C_C0, RC_C_RSTAT, RC_C_LNEW, RC_C_DSTAT,
+    RC_C_COUNT
+};
+
+const char* rc_last_error(void);
+int rc_version(void);
+/* number of visible GPUs (0 on a machine without one; never fails) */
+int rc_device_count(void);
+
+/* ---- scenes (cuda/poc_gpu.cu:751-801 load, cuda/prep_mesh.cuh:303-531 prep) ---- */
+/* Procedural box room: 6 walls + 2 cubes (~target_tris), 2 point lights, states {both on, light 1 off}. */
+rc_status rc_scene_box_room(uint64_t seed, int target_tris, rc_scene** out);
+/* Procedural street of nboxes boxes (~target_tris), nlamps point lamps (+ sun), states
+ * {base, night (sun 0), lamp 0 off, 12x10 m occluder inserted}; camera W x H x spp, bounces. */
+rc_status rc_scene_street(uint64_t seed, int nboxes, int target_tris, int nlamps, int sun, int target_points, int table_log2,
+                          int W, int H, int spp, int bounces, rc_scene** out);
+void rc_scene_destroy(rc_scene* s);
+rc_status rc_scene_info_get(const rc_scene* s, rc_scene_info* out);
+/* byte size / host copy of a scene array (state: state index, or state pair a*NS+b for OCC,
+ * (a*NS+b)*nl+l for EM). Device-produced arrays (EDIR, SV, LM, REF, VALID, PW) need upload. */
+size_t rc_scene_array_bytes(const rc_scene* s, int which, int state);
+rc_status rc_scene_array(const rc_scene* s, int which, int state, void* dst, size_t bytes);
+/* Device upload of the scene and the per-state build (k_build, cuda/poc_gpu.cu:369-391).
+ * trav: 0 = binary BVH in the prototype's order, 1 = binary closest-hit, 2 = 4-wide closest-hit. */
+rc_status rc_scene_upload(rc_scene* s, int trav);
+rc_status rc_scene_set_trav(rc_scene* s, int trav);
+/* Converged per-state reference by fresh-ray Jacobi iterations, two chains, until the chains'
+ * relative RMS difference <= target or max_iter (cuda/prep_mesh.cuh:454-520). noise_out[NS]. */
+rc_status rc_scene_reference(rc_scene* s, int K, int warm, int block, int max_iter, float target, float* noise_out, int* iters_out);
+
+/* ---- cache + runner (Runner, cuda/poc_gpu.cu:807-1016) ---- */
+rc_status rc_cache_create(rc_scene* s, const rc_method* m, rc_cache** out);
+void rc_cache_destroy(rc_cache* c);
+/* run() prologue: warm start (converged Pw when present), n = nmax, masks cleared (:828-844) */
+rc_status rc_cache_begin(rc_cache* c, int state0, uint32_t seed);
+/* one frame of run() in state `state`: event block if the state changed, bank swap, trace,
+ * decide, update, optional error (:851-1011). timed != 0 brackets phases with HIP events. */
+rc_status rc_cache_frame(rc_cache* c, int state, int want_err, int timed, rc_frame_stats* out);
+/* camera query pass over the current cache into d_img[W*H] (no_cache: plain path tracing).
+ * rays_out[2] = extension rays, shadow rays (may be NULL). */
+rc_status rc_cache_render(rc_cache* c, const rc_camera* cam, int no_cache, uint32_t seed, float* d_img, long long* rays_out);
+size_t rc_cache_array_bytes(const rc_cache* c, int which);
+rc_status rc_cache_get(const rc_cache* c, int which, void* host_dst, size_t bytes);
+rc_status rc_cache_set(rc_cache* c, int which, const void* host_src, size_t bytes);
+rc_status rc_cache_sync(rc_cache* c);
+
+/* ---- kernel level (device pointers) ---- */
+/* closest hit of n rays: t (inf on a miss), q (>= 0 quad of a dynamic triangle, -2-j static j,
+ * -1 none) (trace_full, cuda/poc_gpu.cu:340-344) */
+rc_status rc_trace(rc_scene* s, int state, const float* d_O, const float* d_D, int n, float* d_t, int* d_q, int trav, void* stream);
+/* GPU insert of n 64-bit keys (0 = empty, never inserted) with values into a linear-probing
+ * table of hmask+1 slots; HK must start zeroed and HV at INT32_MAX. d_fail[0] += keys left
+ * out after 64 probes. Same slot function and probe order as hash_find (:263-272). */
+rc_status rc_hash_insert(uint64_t* d_HK, int32_t* d_HV, uint32_t hmask, const uint64_t* d_keys, const int32_t* d_vals, int n,
+                         int32_t* d_fail, void* stream);
+/* lookup of n keys: value, -1 when absent (hash_find, :263-272) */
+rc_status rc_hash_query(const uint64_t* d_HK, const int32_t* d_HV, uint32_t hmask, const uint64_t* d_keys, int n, int32_t* d_out,
+                        void* stream);
+/* cache entry of n hit points with facing normals, 26-neighbour fallback (hash_point, :274-287) */
+rc_status rc_hash_point(rc_scene* s, const float* d_H, const float* d_N, int n, int32_t* d_out, void* stream);
+/* transit-grid DDA mask (nw words) of n segments (dda_reg, :171-190) */
+rc_status rc_dda_mark(rc_scene* s, const float* d_A, const float* d_B, int n, uint32_t* d_out, void* stream);
+/* DDA over a gx*gy*gz bit volume (same stepping rule), OR-ing crossed voxels into volume
+ * seg_vol[i] (NULL: volume 0) of d_vol, words_per_vol 32-bit words each. */
+rc_status rc_dda_volume(rc_scene* s, int gx, int gy, int gz, const float* d_A, const float* d_B, const int32_t* d_seg_vol, int n,
+                        uint32_t* d_vol, long long words_per_vol, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RC_H */

@@ -1,0 +1,714 @@
+// C ABI of librc (include/rc.h): scene handles, device upload, converged references, the cache
+// runner (per-frame kernel sequence and event block of Runner::run, cuda/poc_gpu.cu:826-1016)
+// and the kernel-level entry points.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/rc.h"
+#include "rc_kernels.h"
+#include "rc_scene.h"
+
+static thread_local std::string g_err;
+
+static rc_status fail(rc_status st, const char* fmt, ...) {
+    char buf[512];
+    va_list a;
+    va_start(a, fmt);
+    vsnprintf(buf, sizeof buf, fmt, a);
+    va_end(a);
+    g_err = buf;
+    return st;
+}
+#define HIPCK(x)                                                                                        \
+    do {                                                                                                \
+        hipError_t e_ = (x);                                                                            \
+        if (e_ != hipSuccess) return fail(RC_ERR_HIP, "%s: %s (%s:%d)", #x, hipGetErrorString(e_), __FILE__, __LINE__); \
+    } while (0)
+
+extern "C" const char* rc_last_error(void) { return g_err.c_str(); }
+extern "C" int rc_version(void) { return 1; }
+extern "C" int rc_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) { (void)hipGetLastError(); return 0; }
+    return n;
+}
+
+template <class T> static hipError_t dput(T** d, const std::vector<T>& v, std::vector<void*>& owned) {
+    size_t bytes = std::max<size_t>(v.size() * sizeof(T), 16);
+    hipError_t e = hipMalloc((void**)d, bytes);
+    if (e != hipSuccess) return e;
+    owned.push_back(*d);
+    if (!v.empty()) e = hipMemcpy(*d, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice);
+    return e;
+}
+template <class T> static hipError_t dalloc(T** d, size_t n, std::vector<void*>& owned) {
+    hipError_t e = hipMalloc((void**)d, std::max<size_t>(n * sizeof(T), 16));
+    if (e != hipSuccess) return e;
+    owned.push_back(*d);
+    return hipMemset(*d, 0, std::max<size_t>(n * sizeof(T), 16));
+}
+
+struct rc_scene {
+    rc::Scene H;
+    bool uploaded = false;
+    int trav = 2;
+    hipStream_t stream = nullptr;
+    rc_dscene ds{};
+    std::vector<rc_dstate> dst;
+    std::vector<int*> dOcc, dEm;
+    std::vector<void*> owned;
+    // converged references (host + device)
+    bool has_ref = false;
+    std::vector<std::vector<float>> ref, Pw;
+    std::vector<std::vector<uint8_t>> valid;
+    ~rc_scene() {
+        for (void* p : owned) (void)hipFree(p);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+// ------------------------------------------------------------------------------------ scenes
+static rc_status make_scene(const rc::SceneDesc& d, rc_scene** out) {
+    if (!out) return fail(RC_ERR_ARG, "out is NULL");
+    rc_scene* s = new rc_scene();
+    std::string e = rc::prepare(d, s->H);
+    if (!e.empty()) { delete s; return fail(RC_ERR_SCENE, "scene preparation: %s", e.c_str()); }
+    *out = s;
+    return RC_OK;
+}
+
+extern "C" rc_status rc_scene_box_room(uint64_t seed, int target_tris, rc_scene** out) {
+    if (target_tris < 12) return fail(RC_ERR_ARG, "target_tris must be >= 12");
+    return make_scene(rc::gen_box_room(seed, target_tris), out);
+}
+
+extern "C" rc_status rc_scene_street(uint64_t seed, int nboxes, int target_tris, int nlamps, int sun, int target_points, int table_log2,
+                                     int W, int H, int spp, int bounces, rc_scene** out) {
+    if (nboxes < 1 || target_tris < 12 || nlamps < 1 || nlamps + (sun ? 1 : 0) > RC_MAX_LIGHTS || target_points < 16 || W < 1 ||
+        H < 1 || spp < 1 || bounces < 0 || table_log2 < 0 || table_log2 > 30)
+        return fail(RC_ERR_ARG, "rc_scene_street: argument out of range");
+    return make_scene(rc::gen_street(seed, nboxes, target_tris, nlamps, sun != 0, target_points, table_log2, W, H, spp, bounces), out);
+}
+
+extern "C" void rc_scene_destroy(rc_scene* s) { delete s; }
+
+extern "C" rc_status rc_scene_info_get(const rc_scene* s, rc_scene_info* o) {
+    if (!s || !o) return fail(RC_ERR_ARG, "NULL argument");
+    const rc::Scene& H = s->H;
+    memset(o, 0, sizeof *o);
+    o->Np = H.Np; o->npst = H.npst; o->nbox = H.nbox; o->nst = H.nst; o->nnodes = (int)H.BN.size(); o->nwnodes = (int)H.WN.size();
+    o->NS = H.NS; o->nl = H.nl; o->NR = H.NR; o->Nq = H.Nq; o->hsize = (int)H.HK.size();
+    o->vx = H.vx; o->vy = H.vy; o->vz = H.vz; o->nw = H.nw; o->bvh_depth = H.bvh_depth; o->wide_depth = H.wide_depth;
+    o->has_ref = s->has_ref ? 1 : 0;
+    o->h = H.h; o->diag = H.diag; o->sun_len = H.sun_len;
+    for (int a = 0; a < 3; ++a) { o->lo[a] = H.lo[a]; o->ext[a] = H.ext[a]; }
+    memcpy(&o->cam, &H.cam, sizeof(rc_camera));
+    return RC_OK;
+}
+
+// host view of one array: pointer + bytes (device-produced ones are fetched separately)
+static bool host_array(const rc_scene* s, int which, int st, const void** p, size_t* bytes) {
+    const rc::Scene& H = s->H;
+    auto ok_state = [&](int n) { return st >= 0 && st < n; };
+#define RET(vec) do { *p = (vec).data(); *bytes = (vec).size() * sizeof((vec)[0]); return true; } while (0)
+    switch (which) {
+        case RC_A_ST: RET(H.ST);
+        case RC_A_STN: RET(H.STN);
+        case RC_A_BN: RET(H.BN);
+        case RC_A_WN: RET(H.WN);
+        case RC_A_HK: RET(H.HK);
+        case RC_A_HV: RET(H.HV);
+        case RC_A_ALB: RET(H.A);
+        case RC_A_REGION: RET(H.region);
+        case RC_A_L: RET(H.L);
+        case RC_A_LTYPE: RET(H.ltype);
+        case RC_A_DIMS: RET(H.dims);
+        case RC_A_OFF: RET(H.off);
+        case RC_A_TALB: RET(H.Talb);
+        case RC_A_ORDER: RET(H.order);
+        case RC_A_MESH_V: RET(H.mesh.V);
+        case RC_A_MESH_ALB: RET(H.mesh.alb);
+        case RC_A_MESH_DS: RET(H.mesh.ds);
+        case RC_A_P: if (!ok_state(H.NS)) return false; RET(H.P[st]);
+        case RC_A_N: if (!ok_state(H.NS)) return false; RET(H.N[st]);
+        case RC_A_INTEN: if (!ok_state(H.NS)) return false; RET(H.inten[st]);
+        case RC_A_Q: if (!ok_state(H.NS)) return false; RET(H.Q[st]);
+        case RC_A_T: if (!ok_state(H.NS)) return false; RET(H.T[st]);
+        case RC_A_TQ: if (!ok_state(H.NS)) return false; RET(H.TQ[st]);
+        case RC_A_OBJBOX: if (!ok_state(H.NS)) return false; RET(H.objbox[st]);
+        case RC_A_OCC: if (!ok_state(H.NS * H.NS)) return false; RET(H.occ[st]);
+        case RC_A_EM: if (!ok_state(H.NS * H.NS * H.nl)) return false; RET(H.em[st]);
+        case RC_A_REF: if (!s->has_ref || !ok_state(H.NS)) return false; RET(s->ref[st]);
+        case RC_A_VALID: if (!s->has_ref || !ok_state(H.NS)) return false; RET(s->valid[st]);
+        case RC_A_PW: if (!s->has_ref || !ok_state(H.NS)) return false; RET(s->Pw[st]);
+        default: return false;
+    }
+#undef RET
+}
+
+static bool device_array(const rc_scene* s, int which, int st, const void** dp, size_t* bytes) {
+    if (!s->uploaded || st < 0 || st >= s->H.NS) return false;
+    const rc_dstate& d = s->dst[st];
+    size_t Np = (size_t)s->H.Np;
+    switch (which) {
+        case RC_A_EDIR: *dp = d.Edir; *bytes = Np * s->H.nl * sizeof(float); return true;
+        case RC_A_SV: *dp = d.SV; *bytes = Np * s->H.nw * sizeof(uint32_t); return true;
+        case RC_A_LM: *dp = d.LM; *bytes = Np * sizeof(uint64_t); return true;
+        default: return false;
+    }
+}
+
+extern "C" size_t rc_scene_array_bytes(const rc_scene* s, int which, int state) {
+    if (!s) return 0;
+    const void* p; size_t b = 0;
+    if (host_array(s, which, state, &p, &b)) return b;
+    if (device_array(s, which, state, &p, &b)) return b;
+    return 0;
+}
+
+extern "C" rc_status rc_scene_array(const rc_scene* s, int which, int state, void* dst, size_t bytes) {
+    if (!s || (!dst && bytes)) return fail(RC_ERR_ARG, "NULL argument");
+    const void* p; size_t b = 0;
+    if (host_array(s, which, state, &p, &b)) {
+        if (bytes != b) return fail(RC_ERR_ARG, "array %d: %zu bytes requested, %zu available", which, bytes, b);
+        if (b) memcpy(dst, p, b);
+        return RC_OK;
+    }
+    if (device_array(s, which, state, &p, &b)) {
+        if (bytes != b) return fail(RC_ERR_ARG, "array %d: %zu bytes requested, %zu available", which, bytes, b);
+        HIPCK(hipStreamSynchronize(s->stream));
+        HIPCK(hipMemcpy(dst, p, b, hipMemcpyDeviceToHost));
+        return RC_OK;
+    }
+    return fail(RC_ERR_ARG, "array %d (state %d) not available%s", which, state, s->uploaded ? "" : " (scene not uploaded)");
+}
+
+static rc_status need_device() {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n < 1) { (void)hipGetLastError(); return fail(RC_ERR_NODEV, "no HIP device visible"); }
+    return RC_OK;
+}
+
+extern "C" rc_status rc_scene_set_trav(rc_scene* s, int trav) {
+    if (!s || trav < 0 || trav > 2) return fail(RC_ERR_ARG, "trav must be 0, 1 or 2");
+    s->trav = trav;
+    return RC_OK;
+}
+
+extern "C" rc_status rc_scene_upload(rc_scene* s, int trav) {
+    if (!s) return fail(RC_ERR_ARG, "NULL scene");
+    if (trav < 0 || trav > 2) return fail(RC_ERR_ARG, "trav must be 0, 1 or 2");
+    if (s->uploaded) return fail(RC_ERR_STATE, "scene already uploaded");
+    rc_status st = need_device();
+    if (st != RC_OK) return st;
+    s->trav = trav;
+    rc::Scene& H = s->H;
+    HIPCK(hipStreamCreate(&s->stream));
+    rc_dscene& d = s->ds;
+    float *dST, *dSTN, *dL, *dA; rc_bvh_node* dBN; rc_wnode* dWN; uint64_t* dHK; int *dHV, *dLT, *dR;
+    HIPCK(dput(&dST, H.ST, s->owned)); HIPCK(dput(&dSTN, H.STN, s->owned)); HIPCK(dput(&dBN, H.BN, s->owned));
+    HIPCK(dput(&dWN, H.WN, s->owned)); HIPCK(dput(&dHK, H.HK, s->owned)); HIPCK(dput(&dHV, H.HV, s->owned));
+    HIPCK(dput(&dL, H.L, s->owned)); HIPCK(dput(&dLT, H.ltype, s->owned)); HIPCK(dput(&dA, H.A, s->owned));
+    HIPCK(dput(&dR, H.region, s->owned));
+    d.ST = dST; d.STN = dSTN; d.BN = dBN; d.WN = dWN; d.HK = dHK; d.HV = dHV; d.L = dL; d.ltype = dLT; d.Alb = dA; d.region = dR;
+    d.hmask = (uint32_t)(H.HK.size() - 1); d.hcell = H.h;
+    for (int a = 0; a < 3; ++a) { d.lo[a] = H.lo[a]; d.ext[a] = H.ext[a]; }
+    d.sun_len = H.sun_len; d.nst = H.nst; d.nl = H.nl; d.Np = H.Np; d.NR = H.NR;
+    d.vx = H.vx; d.vy = H.vy; d.vz = H.vz; d.nw = H.nw;
+    int *dDims, *dOff;
+    HIPCK(dput(&dDims, H.dims, s->owned)); HIPCK(dput(&dOff, H.off, s->owned));
+    s->dst.resize(H.NS);
+    for (int k = 0; k < H.NS; ++k) {
+        rc_dstate& x = s->dst[k];
+        memset(&x, 0, sizeof x);
+        float *P, *N, *Q, *T, *I, *E; int* TQ; uint32_t* SV; uint64_t* LM;
+        HIPCK(dput(&P, H.P[k], s->owned)); HIPCK(dput(&N, H.N[k], s->owned)); HIPCK(dput(&Q, H.Q[k], s->owned));
+        HIPCK(dput(&T, H.T[k], s->owned)); HIPCK(dput(&TQ, H.TQ[k], s->owned)); HIPCK(dput(&I, H.inten[k], s->owned));
+        HIPCK(dalloc(&E, (size_t)H.nl * H.Np, s->owned)); HIPCK(dalloc(&SV, (size_t)H.nw * H.Np, s->owned));
+        HIPCK(dalloc(&LM, (size_t)H.Np, s->owned));
+        x.P = P; x.N = N; x.Q = Q; x.dims = dDims; x.off = dOff; x.T = T; x.TQ = TQ; x.Edir = E; x.SV = SV; x.LM = LM; x.inten = I;
+        x.nt = H.nt[k];
+    }
+    for (auto& v : H.occ) { int* p; std::vector<int> w = v.empty() ? std::vector<int>{0} : v; HIPCK(dput(&p, w, s->owned)); s->dOcc.push_back(p); }
+    for (auto& v : H.em) { int* p; std::vector<int> w = v.empty() ? std::vector<int>{0} : v; HIPCK(dput(&p, w, s->owned)); s->dEm.push_back(p); }
+    for (int k = 0; k < H.NS; ++k) HIPCK(rc_launch_build(trav, &s->ds, &s->dst[k], s->stream));
+    HIPCK(hipStreamSynchronize(s->stream));
+    s->uploaded = true;
+    return RC_OK;
+}
+
+// Converged reference per state (cuda/prep_mesh.cuh:454-520): warm iterations, then blocks
+// of iterations averaged per chain until the two chains agree within `target` (relative RMS
+// over valid points) or max_iter. Points whose rays hit within 2 cm more than 30 % of the
+// time are not valid (inside geometry). Pw = per-light mean of both chains, ref = sum.
+extern "C" rc_status rc_scene_reference(rc_scene* s, int K, int warm, int block, int max_iter, float target, float* noise_out, int* iters_out) {
+    if (!s || K < 1 || warm < 0 || block < 1 || max_iter < block) return fail(RC_ERR_ARG, "rc_scene_reference: bad arguments");
+    if (!s->uploaded) return fail(RC_ERR_STATE, "scene not uploaded");
+    rc::Scene& H = s->H;
+    const size_t Np = H.Np, nl = H.nl;
+    std::vector<void*> tmp;
+    float *Lout, *closec, *chain[2], *acc[2];
+    auto cleanup = [&]() { for (void* p : tmp) (void)hipFree(p); };
+#define TCK(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) { cleanup(); return fail(RC_ERR_HIP, "%s: %s", #x, hipGetErrorString(e_)); } } while (0)
+    TCK(dalloc(&Lout, nl * Np, tmp)); TCK(dalloc(&closec, Np, tmp));
+    for (int c = 0; c < 2; ++c) { TCK(dalloc(&chain[c], nl * Np, tmp)); TCK(dalloc(&acc[c], nl * Np, tmp)); }
+    s->ref.assign(H.NS, {}); s->Pw.assign(H.NS, {}); s->valid.assign(H.NS, {});
+    std::vector<float> a0(nl * Np), a1(nl * Np), cc(Np);
+    for (int si = 0; si < H.NS; ++si) {
+        TCK(hipMemsetAsync(closec, 0, Np * 4, s->stream));
+        for (int c = 0; c < 2; ++c) { TCK(hipMemsetAsync(chain[c], 0, nl * Np * 4, s->stream)); TCK(hipMemsetAsync(acc[c], 0, nl * Np * 4, s->stream)); }
+        int it_done = 0, T = 0; double nz = 1.0;
+        std::vector<float> ref(Np), Pw(nl * Np); std::vector<uint8_t> val(Np);
+        while (true) {
+            int n_it = it_done == 0 ? warm + block : block;
+            for (int c = 0; c < 2; ++c) {
+                float* a = chain[c];
+                float* b = Lout;
+                for (int k = 0; k < n_it; ++k) {
+                    int it = it_done + k;
+                    uint32_t seed = (uint32_t)rc::mix64(0x51ull + 7919ull * si + 104729ull * c + 31ull * it);
+                    TCK(rc_launch_ref(s->trav, &s->ds, &s->dst[si], a, b, acc[c], closec, seed, K, it >= warm, c == 0, s->stream));
+                    std::swap(a, b);
+                }
+                if (a != chain[c]) TCK(hipMemcpyAsync(chain[c], a, nl * Np * 4, hipMemcpyDeviceToDevice, s->stream));
+            }
+            it_done += n_it;
+            T = it_done - warm;
+            TCK(hipStreamSynchronize(s->stream));
+            TCK(hipMemcpy(a0.data(), acc[0], a0.size() * 4, hipMemcpyDeviceToHost));
+            TCK(hipMemcpy(a1.data(), acc[1], a1.size() * 4, hipMemcpyDeviceToHost));
+            TCK(hipMemcpy(cc.data(), closec, Np * 4, hipMemcpyDeviceToHost));
+            double d2 = 0, rs = 0; int nv = 0;
+            for (size_t i = 0; i < Np; ++i) {
+                float e0 = 0, e1 = 0;
+                for (size_t l = 0; l < nl; ++l) {
+                    float x0 = a0[nl * i + l] / T, x1 = a1[nl * i + l] / T;
+                    Pw[nl * i + l] = 0.5f * (x0 + x1); e0 += x0; e1 += x1;
+                }
+                ref[i] = 0.5f * (e0 + e1);
+                val[i] = cc[i] / ((double)K * it_done) < 0.3;
+                if (val[i]) { d2 += 0.25 * (e0 - e1) * (e0 - e1); rs += ref[i]; nv++; }
+            }
+            nz = nv && rs > 0 ? std::sqrt(d2 / nv) / (rs / nv) : 0.0;
+            if (nz <= target || T >= max_iter) break;
+        }
+        // points of absent objects are never valid
+        for (size_t i = (size_t)H.npst; i < Np; ++i) {
+            int q = -1;
+            for (int qq = 0; qq < H.Nq; ++qq) if ((int)i >= H.off[qq] && (int)i < H.off[qq] + H.dims[2 * qq] * H.dims[2 * qq + 1]) q = qq;
+            if (q >= 0 && !H.present[si][q / 6]) val[i] = 0;
+        }
+        s->ref[si] = ref; s->Pw[si] = Pw; s->valid[si] = val;
+        if (noise_out) noise_out[si] = (float)nz;
+        if (iters_out) iters_out[si] = T;
+    }
+    cleanup();
+#undef TCK
+    // device copies for the error kernel and the warm start
+    for (int si = 0; si < H.NS; ++si) {
+        float *r, *pw; uint8_t* v;
+        HIPCK(dput(&r, s->ref[si], s->owned)); HIPCK(dput(&pw, s->Pw[si], s->owned)); HIPCK(dput(&v, s->valid[si], s->owned));
+        s->dst[si].ref = r; s->dst[si].Pw = pw; s->dst[si].valid = v;
+    }
+    s->has_ref = true;
+    return RC_OK;
+}
+
+// ------------------------------------------------------------------------------------ cache
+struct rc_cache {
+    rc_scene* S = nullptr;
+    rc_method M{};
+    rc_dcache c{};
+    uint64_t* lnew = nullptr;
+    float* d_query_img = nullptr;
+    unsigned long long* d_counters = nullptr;
+    std::vector<void*> owned;
+    hipEvent_t ev[8]{};
+    // runner state
+    bool begun = false;
+    int prev_state = -1;
+    int f = 0;
+    uint32_t seed = 0;
+    float nmax = 0.f;
+    int pend_left = 0;
+    uint64_t dormant = 0;
+    std::vector<float> ilearn;
+    int nslots = 0, slot_next = 0, last_event_f = 0;
+    int slot_state[RC_KSNAP];
+    rc_params p{};
+    ~rc_cache() {
+        for (auto& e : ev) if (e) (void)hipEventDestroy(e);
+        for (void* q : owned) (void)hipFree(q);
+    }
+};
+
+extern "C" rc_status rc_cache_create(rc_scene* s, const rc_method* m, rc_cache** out) {
+    if (!s || !m || !out) return fail(RC_ERR_ARG, "NULL argument");
+    if (!s->uploaded) return fail(RC_ERR_STATE, "scene not uploaded");
+    if (m->mode < 0 || m->mode > 1 || !(m->nmax > 0.f) || m->budget < 0.f || m->track_every < 1)
+        return fail(RC_ERR_ARG, "method: mode must be 0/1, nmax > 0, budget >= 0, track_every >= 1");
+    rc_cache* c = new rc_cache();
+    c->S = s;
+    c->M = *m;
+    const rc::Scene& H = s->H;
+    const size_t Np = H.Np, nl = H.nl, nw = H.nw;
+    rc_dcache& d = c->c;
+#define CK2(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) { delete c; return fail(RC_ERR_HIP, "%s: %s", #x, hipGetErrorString(e_)); } } while (0)
+    CK2(dalloc(&d.Lc, nl * Np, c->owned)); CK2(dalloc(&d.Lc2, nl * Np, c->owned)); CK2(dalloc(&d.m, nl * Np, c->owned));
+    CK2(dalloc(&d.n, Np, c->owned)); CK2(dalloc(&d.c0, Np, c->owned)); CK2(dalloc(&d.kk, Np, c->owned));
+    CK2(dalloc(&d.pending, Np, c->owned));
+    CK2(dalloc(&d.cur, nw * Np, c->owned)); CK2(dalloc(&d.prev, nw * Np, c->owned));
+    CK2(dalloc(&d.lcur, Np, c->owned)); CK2(dalloc(&d.lprev, Np, c->owned)); CK2(dalloc(&c->lnew, Np, c->owned));
+    CK2(dalloc(&d.rstat, 4 * (size_t)H.NR, c->owned)); CK2(dalloc(&d.dstat, 3 * (size_t)H.NR, c->owned));
+    CK2(dalloc(&d.stale, Np, c->owned)); CK2(dalloc(&d.age, Np, c->owned));
+    if (m->snaps) CK2(dalloc(&d.snap, (size_t)RC_KSNAP * nl * Np, c->owned));
+    CK2(dalloc(&d.sumpr, 2, c->owned)); CK2(dalloc(&d.npend, 2, c->owned)); CK2(dalloc(&d.err, 3, c->owned));
+    CK2(dalloc(&c->d_counters, 2, c->owned));
+    for (auto& e : c->ev) CK2(hipEventCreate(&e));
+#undef CK2
+    *out = c;
+    return RC_OK;
+}
+
+extern "C" void rc_cache_destroy(rc_cache* c) { delete c; }
+
+extern "C" rc_status rc_cache_begin(rc_cache* c, int state0, uint32_t seed) {
+    if (!c) return fail(RC_ERR_ARG, "NULL cache");
+    rc_scene* S = c->S;
+    const rc::Scene& H = S->H;
+    if (state0 < 0 || state0 >= H.NS) return fail(RC_ERR_ARG, "state %d out of range", state0);
+    hipStream_t st = S->stream;
+    const size_t Np = H.Np, nl = H.nl, nw = H.nw;
+    rc_dcache& d = c->c;
+    if (S->has_ref) HIPCK(hipMemcpyAsync(d.Lc, S->dst[state0].Pw, nl * Np * 4, hipMemcpyDeviceToDevice, st));
+    else HIPCK(hipMemsetAsync(d.Lc, 0, nl * Np * 4, st));
+    c->nmax = c->M.nmax;
+    std::vector<float> nm(Np, c->nmax), ag(Np, 1e4f);
+    HIPCK(hipMemcpyAsync(d.n, nm.data(), Np * 4, hipMemcpyHostToDevice, st));
+    HIPCK(hipMemcpyAsync(d.age, ag.data(), Np * 4, hipMemcpyHostToDevice, st));
+    HIPCK(hipMemsetAsync(d.pending, 0, Np, st));
+    HIPCK(hipMemsetAsync(d.cur, 0, nw * Np * 4, st)); HIPCK(hipMemsetAsync(d.prev, 0, nw * Np * 4, st));
+    HIPCK(hipMemsetAsync(d.lcur, 0, Np * 8, st)); HIPCK(hipMemsetAsync(d.lprev, 0, Np * 8, st));
+    HIPCK(hipMemsetAsync(c->lnew, 0, Np * 8, st));
+    HIPCK(hipMemsetAsync(d.stale, 0, Np * 8, st));
+    HIPCK(hipMemsetAsync(d.npend, 0, 8, st));
+    HIPCK(hipMemsetAsync(d.sumpr, 0, 16, st));
+    HIPCK(rc_launch_sumpr(&d, (int)Np, st));   // sum of 1/(n+1) at n = nmax
+    HIPCK(hipStreamSynchronize(st));
+    c->begun = true;
+    c->prev_state = state0;
+    c->f = 0;
+    c->seed = seed;
+    c->pend_left = 0;
+    c->dormant = 0;
+    c->ilearn.assign(nl, 0.f);
+    c->nslots = 0; c->slot_next = 0; c->last_event_f = 0;
+    for (int k = 0; k < RC_KSNAP; ++k) c->slot_state[k] = -1;
+    rc_params& p = c->p;
+    memset(&p, 0, sizeof p);
+    p.Np = (int)Np; p.NR = H.NR; p.mode = c->M.mode; p.budget = c->M.budget; p.nmax = c->nmax; p.seed = seed;
+    p.deps = c->M.deps; p.resp_mask = c->M.resp_mask; p.nmax_resp = c->M.nmax_resp;
+    p.ddgi = c->M.ddgi; p.ddgi_h = c->M.ddgi_h; p.ddgi_thr = c->M.ddgi_thr; p.ddgi_clamp = c->M.ddgi_clamp;
+    p.ramp = c->M.ramp; p.rays = 1.f; p.nl = (int)nl;
+    return RC_OK;
+}
+
+static float ms_between(hipEvent_t a, hipEvent_t b) { float ms = 0.f; (void)hipEventElapsedTime(&ms, a, b); return ms; }
+
+// event block of Runner::run for the transition e -> si (cuda/poc_gpu.cu:856-943)
+static rc_status run_event(rc_cache* c, int e, int si, int& invalidated) {
+    rc_scene* S = c->S;
+    const rc::Scene& H = S->H;
+    const rc_method& M = c->M;
+    hipStream_t st = S->stream;
+    rc_dcache& d = c->c;
+    const int Np = H.Np, nl = H.nl;
+    const int pe = e * H.NS + si;
+    const int f = c->f;
+    if (M.snaps && f - c->last_event_f >= 20) {   // snapshot of the state being left
+        int k = -1;
+        for (int a = 0; a < c->nslots; ++a) if (c->slot_state[a] == e) k = a;
+        if (k < 0) { if (c->nslots < RC_KSNAP) k = c->nslots++; else { k = c->slot_next; c->slot_next = (c->slot_next + 1) % RC_KSNAP; } }
+        c->slot_state[k] = e;
+        HIPCK(hipMemcpyAsync(d.snap + (size_t)k * Np * nl, d.Lc, (size_t)Np * nl * 4, hipMemcpyDeviceToDevice, st));
+    }
+    if (M.oracle && S->has_ref) HIPCK(rc_launch_oracle_method(&d, Np, S->dst[e].ref, S->dst[si].ref, st));
+    if (M.rescale && !M.deps) {
+        for (int l = 0; l < nl; ++l) {
+            float a = H.inten[e][l], b = H.inten[si][l];
+            if (a == b) continue;
+            bool dm = l < 64 && ((c->dormant >> l) & 1ull);
+            if (M.memory && l < 64 && a > 0 && b == 0) { c->dormant |= 1ull << l; c->ilearn[l] = a; }
+            else if (M.memory && a == 0 && b > 0 && dm) { HIPCK(rc_launch_rescale(&d, Np, nl, l, b / c->ilearn[l], st)); c->dormant &= ~(1ull << l); }
+            else if (a > 0) HIPCK(rc_launch_rescale(&d, Np, nl, l, b / a, st));
+        }
+    }
+    if (M.deps && !M.targeted) {
+        uint64_t woke = 0;
+        for (int l = 0; l < nl; ++l) {
+            float a = H.inten[e][l], b = H.inten[si][l];
+            if (a == b) continue;
+            bool dm = l < 64 && ((c->dormant >> l) & 1ull);
+            if (M.memory && l < 64 && a > 0 && b == 0) { c->dormant |= 1ull << l; c->ilearn[l] = a; }
+            else if (M.memory && a == 0 && b > 0 && dm) {
+                HIPCK(rc_launch_rescale(&d, Np, nl, l, b / c->ilearn[l], st));
+                c->dormant &= ~(1ull << l);
+                woke |= 1ull << l;
+                if (M.wake_verify) {
+                    HIPCK(hipMemsetAsync(d.npend, 0, 8, st));
+                    HIPCK(hipMemsetAsync(d.rstat, 0, 4 * (size_t)H.NR * 8, st));
+                    HIPCK(rc_launch_wake(&d, Np, nl, l, 1, c->dormant, st));
+                    HIPCK(hipMemcpyAsync(d.npend, d.npend + 1, 4, hipMemcpyDeviceToDevice, st));
+                    c->pend_left = RC_VERIF_FRAMES;
+                } else HIPCK(rc_launch_wake(&d, Np, nl, l, 0, c->dormant, st));
+            } else if (a > 0) HIPCK(rc_launch_rescale(&d, Np, nl, l, b / a, st));
+        }
+        for (int l = 0; l < nl; ++l) {   // new light without memory: touched entries restart
+            int ix = pe * nl + l;
+            if (!H.em[ix].empty() && !(l < 64 && ((woke >> l) & 1ull)))
+                HIPCK(rc_launch_touched(&d, Np, nl, S->dEm[ix], (int)H.em[ix].size(), 0, c->dormant, st));
+        }
+        if (!H.occ[pe].empty()) {
+            if (M.verify) {
+                HIPCK(hipMemsetAsync(d.npend, 0, 8, st));
+                HIPCK(hipMemsetAsync(d.rstat, 0, 4 * (size_t)H.NR * 8, st));
+                HIPCK(rc_launch_touched(&d, Np, nl, S->dOcc[pe], (int)H.occ[pe].size(), 1, c->dormant, st));
+                HIPCK(hipMemcpyAsync(d.npend, d.npend + 1, 4, hipMemcpyDeviceToDevice, st));
+                c->pend_left = RC_VERIF_FRAMES;
+            } else if (c->dormant) HIPCK(rc_launch_touched(&d, Np, nl, S->dOcc[pe], (int)H.occ[pe].size(), 2, c->dormant, st));
+        }
+    }
+    int hit = -1;
+    if (M.snaps)
+        for (int a = 0; a < c->nslots; ++a) if (c->slot_state[a] == si) hit = a;
+    if (M.targeted) {   // resolve-time dependency invalidation (rtxgi_bench.patch:368-386, :1028-1050)
+        uint64_t inv = 0;
+        for (int l = 0; l < nl; ++l) if (H.inten[e][l] != H.inten[si][l]) inv |= 1ull << (l < 63 ? l : 63);
+        int nv = (int)H.occ[pe].size();
+        if (inv || nv) {
+            HIPCK(hipMemsetAsync(d.npend + 1, 0, 4, st));
+            HIPCK(rc_launch_invalidate(&d, Np, nl, inv, S->dOcc[pe], nv, H.nw, st));
+            int h2 = 0;
+            HIPCK(hipMemcpyAsync(&h2, d.npend + 1, 4, hipMemcpyDeviceToHost, st));
+            HIPCK(hipStreamSynchronize(st));
+            HIPCK(hipMemsetAsync(d.npend + 1, 0, 4, st));
+            invalidated = h2;
+        }
+    }
+    if (M.flush && hit < 0) HIPCK(rc_launch_flush(&d, Np, nl, st));   // new state: clear (rtxgi_bench.patch:363-366)
+    if (M.snaps && hit >= 0) {   // revisited state: restore its snapshot
+        HIPCK(hipMemcpyAsync(d.Lc, d.snap + (size_t)hit * Np * nl, (size_t)Np * nl * 4, hipMemcpyDeviceToDevice, st));
+        if (M.snap_verify) {
+            HIPCK(hipMemsetAsync(d.npend, 0, 8, st));
+            HIPCK(hipMemsetAsync(d.rstat, 0, 4 * (size_t)H.NR * 8, st));
+            HIPCK(rc_launch_restored(&d, Np, nl, c->nmax, c->dormant, 1, st));
+            HIPCK(hipMemcpyAsync(d.npend, d.npend + 1, 4, hipMemcpyDeviceToDevice, st));
+            c->pend_left = RC_VERIF_FRAMES;
+        } else {
+            HIPCK(rc_launch_restored(&d, Np, nl, c->nmax, c->dormant, 0, st));
+            HIPCK(hipMemsetAsync(d.npend, 0, 8, st));
+            c->pend_left = 0;
+        }
+    }
+    c->last_event_f = f;
+    if (M.mode == 1) {   // ray allocation must see this frame's invalidation
+        HIPCK(hipMemsetAsync(d.sumpr, 0, 8, st));
+        HIPCK(rc_launch_sumpr(&d, Np, st));
+    }
+    return RC_OK;
+}
+
+extern "C" rc_status rc_cache_frame(rc_cache* c, int si, int want_err, int timed, rc_frame_stats* out) {
+    if (!c) return fail(RC_ERR_ARG, "NULL cache");
+    if (!c->begun) return fail(RC_ERR_STATE, "rc_cache_begin not called");
+    rc_scene* S = c->S;
+    const rc::Scene& H = S->H;
+    if (si < 0 || si >= H.NS) return fail(RC_ERR_ARG, "state %d out of range", si);
+    const rc_method& M = c->M;
+    hipStream_t st = S->stream;
+    rc_dcache& d = c->c;
+    const int Np = H.Np, nl = H.nl, nw = H.nw;
+    rc_params& p = c->p;
+    rc_frame_stats fs;
+    memset(&fs, 0, sizeof fs);
+    fs.err = -1.0;
+    p.frame = (uint32_t)c->f;
+    if (c->f > 0 && c->prev_state != si) {
+        if (timed) HIPCK(hipEventRecord(c->ev[4], st));
+        rc_status r = run_event(c, c->prev_state, si, fs.invalidated);
+        if (r != RC_OK) return r;
+        fs.event = 1;
+        if (timed) { HIPCK(hipEventRecord(c->ev[5], st)); HIPCK(hipEventSynchronize(c->ev[5])); fs.ms_event = ms_between(c->ev[4], c->ev[5]); }
+    }
+    p.pend_left = c->pend_left;
+    p.dormant = c->dormant;
+    if (M.ddgi) HIPCK(hipMemsetAsync(d.dstat, 0, 3 * (size_t)H.NR * 8, st));
+    if (timed) HIPCK(hipEventRecord(c->ev[0], st));
+    if (M.deps && c->f % 32 == 0) {   // dependency banks: swap every 32 frames, clear the new current bank
+        std::swap(d.cur, d.prev);
+        std::swap(d.lcur, d.lprev);
+        HIPCK(hipMemsetAsync(d.cur, 0, (size_t)nw * Np * 4, st));
+        HIPCK(hipMemsetAsync(d.lcur, 0, (size_t)Np * 8, st));
+    }
+    if (timed) HIPCK(hipEventRecord(c->ev[1], st));
+    bool track = M.deps && (c->f % M.track_every == 0);
+    HIPCK(rc_launch_trace2(S->trav, track ? 1 : 0, &S->ds, &S->dst[si], &d, &p, c->lnew, st));
+    if (timed) HIPCK(hipEventRecord(c->ev[2], st));
+    if (c->pend_left > 0) {
+        HIPCK(hipMemsetAsync(d.npend + 1, 0, 4, st));
+        HIPCK(rc_launch_decide(&S->ds, &d, &p, st));
+        HIPCK(hipMemcpyAsync(d.npend, d.npend + 1, 4, hipMemcpyDeviceToDevice, st));
+        c->pend_left--;
+    }
+    if (timed) HIPCK(hipEventRecord(c->ev[3], st));
+    HIPCK(hipMemsetAsync(d.sumpr + 1, 0, 8, st));
+    HIPCK(rc_launch_update(&S->ds, &d, &p, track ? c->lnew : nullptr, st));
+    HIPCK(hipMemcpyAsync(d.sumpr, d.sumpr + 1, 8, hipMemcpyDeviceToDevice, st));
+    std::swap(d.Lc, d.Lc2);
+    if (timed) {
+        HIPCK(hipEventRecord(c->ev[6], st));
+        HIPCK(hipEventSynchronize(c->ev[6]));
+        fs.ms_swap = ms_between(c->ev[0], c->ev[1]);
+        fs.ms_trace = ms_between(c->ev[1], c->ev[2]);
+        fs.ms_decide = ms_between(c->ev[2], c->ev[3]);
+        fs.ms_update = ms_between(c->ev[3], c->ev[6]);
+    }
+    if (want_err && S->has_ref) {
+        HIPCK(hipMemsetAsync(d.err, 0, 24, st));
+        HIPCK(rc_launch_error(&d, Np, nl, S->dst[si].ref, S->dst[si].valid, c->dormant, st));
+        double h3[3];
+        HIPCK(hipMemcpyAsync(h3, d.err, 24, hipMemcpyDeviceToHost, st));
+        HIPCK(hipStreamSynchronize(st));
+        fs.err = h3[2] > 0 ? std::sqrt(h3[0] / h3[2]) / (h3[1] / h3[2]) : 0.0;
+    }
+    c->prev_state = si;
+    c->f++;
+    if (out) *out = fs;
+    return RC_OK;
+}
+
+extern "C" rc_status rc_cache_render(rc_cache* c, const rc_camera* cam, int no_cache, uint32_t seed, float* d_img, long long* rays_out) {
+    if (!c || !cam || !d_img) return fail(RC_ERR_ARG, "NULL argument");
+    if (!c->begun) return fail(RC_ERR_STATE, "rc_cache_begin not called");
+    if (cam->W < 1 || cam->H < 1 || cam->spp < 1 || cam->bounces < 0 || cam->cache_bounce < 0 || (long long)cam->W * cam->H > (1ll << 30))
+        return fail(RC_ERR_ARG, "camera: bad size / spp / bounces");
+    rc_scene* S = c->S;
+    hipStream_t st = S->stream;
+    int si = c->prev_state;
+    unsigned long long* ctr = nullptr;
+    if (rays_out) { HIPCK(hipMemsetAsync(c->d_counters, 0, 16, st)); ctr = c->d_counters; }
+    HIPCK(rc_launch_query(S->trav, &S->ds, &S->dst[si], no_cache ? nullptr : c->c.Lc, cam, seed, (uint32_t)c->f, c->dormant, d_img, ctr, st));
+    if (rays_out) {
+        unsigned long long h2[2];
+        HIPCK(hipMemcpyAsync(h2, c->d_counters, 16, hipMemcpyDeviceToHost, st));
+        HIPCK(hipStreamSynchronize(st));
+        rays_out[0] = (long long)h2[0]; rays_out[1] = (long long)h2[1];
+    }
+    return RC_OK;
+}
+
+static bool cache_array(const rc_cache* c, int which, void** dp, size_t* bytes) {
+    const rc::Scene& H = c->S->H;
+    const size_t Np = H.Np, nl = H.nl, nw = H.nw, NR = H.NR;
+    const rc_dcache& d = c->c;
+    switch (which) {
+        case RC_C_LC: *dp = d.Lc; *bytes = nl * Np * 4; return true;
+        case RC_C_M: *dp = d.m; *bytes = nl * Np * 4; return true;
+        case RC_C_N: *dp = d.n; *bytes = Np * 4; return true;
+        case RC_C_KK: *dp = d.kk; *bytes = Np * 4; return true;
+        case RC_C_PENDING: *dp = d.pending; *bytes = Np; return true;
+        case RC_C_CUR: *dp = d.cur; *bytes = nw * Np * 4; return true;
+        case RC_C_PREV: *dp = d.prev; *bytes = nw * Np * 4; return true;
+        case RC_C_LCUR: *dp = d.lcur; *bytes = Np * 8; return true;
+        case RC_C_LPREV: *dp = d.lprev; *bytes = Np * 8; return true;
+        case RC_C_SUMPR: *dp = d.sumpr; *bytes = 16; return true;
+        case RC_C_NPEND: *dp = d.npend; *bytes = 8; return true;
+        case RC_C_AGE: *dp = d.age; *bytes = Np * 4; return true;
+        case RC_C_STALE: *dp = d.stale; *bytes = Np * 8; return true;
+        case RC_C_C0: *dp = d.c0; *bytes = Np * 4; return true;
+        case RC_C_RSTAT: *dp = d.rstat; *bytes = 4 * NR * 8; return true;
+        case RC_C_LNEW: *dp = c->lnew; *bytes = Np * 8; return true;
+        case RC_C_DSTAT: *dp = d.dstat; *bytes = 3 * NR * 8; return true;
+        default: return false;
+    }
+}
+
+extern "C" size_t rc_cache_array_bytes(const rc_cache* c, int which) {
+    void* p; size_t b = 0;
+    if (!c || !cache_array(c, which, &p, &b)) return 0;
+    return b;
+}
+extern "C" rc_status rc_cache_get(const rc_cache* c, int which, void* host_dst, size_t bytes) {
+    void* p; size_t b = 0;
+    if (!c || !cache_array(c, which, &p, &b)) return fail(RC_ERR_ARG, "unknown cache array %d", which);
+    if (bytes != b) return fail(RC_ERR_ARG, "cache array %d: %zu bytes requested, %zu available", which, bytes, b);
+    HIPCK(hipStreamSynchronize(c->S->stream));
+    HIPCK(hipMemcpy(host_dst, p, b, hipMemcpyDeviceToHost));
+    return RC_OK;
+}
+extern "C" rc_status rc_cache_set(rc_cache* c, int which, const void* host_src, size_t bytes) {
+    void* p; size_t b = 0;
+    if (!c || !cache_array(c, which, &p, &b)) return fail(RC_ERR_ARG, "unknown cache array %d", which);
+    if (bytes != b) return fail(RC_ERR_ARG, "cache array %d: %zu bytes given, %zu expected", which, bytes, b);
+    HIPCK(hipStreamSynchronize(c->S->stream));
+    HIPCK(hipMemcpy(p, host_src, b, hipMemcpyHostToDevice));
+    return RC_OK;
+}
+extern "C" rc_status rc_cache_sync(rc_cache* c) {
+    if (!c) return fail(RC_ERR_ARG, "NULL cache");
+    HIPCK(hipStreamSynchronize(c->S->stream));
+    return RC_OK;
+}
+
+// ------------------------------------------------------------------------------------ kernel level
+static hipStream_t pick(rc_scene* s, void* stream) { return stream ? (hipStream_t)stream : (s ? s->stream : nullptr); }
+
+extern "C" rc_status rc_trace(rc_scene* s, int state, const float* d_O, const float* d_D, int n, float* d_t, int* d_q, int trav, void* stream) {
+    if (!s || n < 0 || (n > 0 && (!d_O || !d_D || !d_t || !d_q))) return fail(RC_ERR_ARG, "rc_trace: bad arguments");
+    if (!s->uploaded) return fail(RC_ERR_STATE, "scene not uploaded");
+    if (state < 0 || state >= s->H.NS || trav < 0 || trav > 2) return fail(RC_ERR_ARG, "rc_trace: state / trav out of range");
+    HIPCK(rc_launch_trace_rays(trav, &s->ds, &s->dst[state], d_O, d_D, n, d_t, d_q, pick(s, stream)));
+    return RC_OK;
+}
+
+extern "C" rc_status rc_hash_insert(uint64_t* d_HK, int32_t* d_HV, uint32_t hmask, const uint64_t* d_keys, const int32_t* d_vals, int n,
+                                    int32_t* d_fail, void* stream) {
+    if (n < 0 || (n > 0 && (!d_HK || !d_HV || !d_keys || !d_vals || !d_fail))) return fail(RC_ERR_ARG, "rc_hash_insert: bad arguments");
+    if ((hmask & (hmask + 1)) != 0) return fail(RC_ERR_ARG, "rc_hash_insert: table size must be a power of two");
+    HIPCK(rc_launch_hash_insert((unsigned long long*)d_HK, d_HV, hmask, (const unsigned long long*)d_keys, d_vals, n, d_fail, (hipStream_t)stream));
+    return RC_OK;
+}
+extern "C" rc_status rc_hash_query(const uint64_t* d_HK, const int32_t* d_HV, uint32_t hmask, const uint64_t* d_keys, int n, int32_t* d_out,
+                                   void* stream) {
+    if (n < 0 || (n > 0 && (!d_HK || !d_HV || !d_keys || !d_out))) return fail(RC_ERR_ARG, "rc_hash_query: bad arguments");
+    if ((hmask & (hmask + 1)) != 0) return fail(RC_ERR_ARG, "rc_hash_query: table size must be a power of two");
+    HIPCK(rc_launch_hash_query((const unsigned long long*)d_HK, d_HV, hmask, (const unsigned long long*)d_keys, n, d_out, (hipStream_t)stream));
+    return RC_OK;
+}
+extern "C" rc_status rc_hash_point(rc_scene* s, const float* d_H, const float* d_N, int n, int32_t* d_out, void* stream) {
+    if (!s || n < 0 || (n > 0 && (!d_H || !d_N || !d_out))) return fail(RC_ERR_ARG, "rc_hash_point: bad arguments");
+    if (!s->uploaded) return fail(RC_ERR_STATE, "scene not uploaded");
+    HIPCK(rc_launch_hash_point(&s->ds, d_H, d_N, n, d_out, pick(s, stream)));
+    return RC_OK;
+}
+extern "C" rc_status rc_dda_mark(rc_scene* s, const float* d_A, const float* d_B, int n, uint32_t* d_out, void* stream) {
+    if (!s || n < 0 || (n > 0 && (!d_A || !d_B || !d_out))) return fail(RC_ERR_ARG, "rc_dda_mark: bad arguments");
+    if (!s->uploaded) return fail(RC_ERR_STATE, "scene not uploaded");
+    HIPCK(rc_launch_dda_segments(&s->ds, d_A, d_B, n, d_out, pick(s, stream)));
+    return RC_OK;
+}
+extern "C" rc_status rc_dda_volume(rc_scene* s, int gx, int gy, int gz, const float* d_A, const float* d_B, const int32_t* d_seg_vol, int n,
+                                   uint32_t* d_vol, long long words_per_vol, void* stream) {
+    if (!s || n < 0 || gx < 1 || gy < 1 || gz < 1 || (n > 0 && (!d_A || !d_B || !d_vol))) return fail(RC_ERR_ARG, "rc_dda_volume: bad arguments");
+    if (words_per_vol < ((long long)gx * gy * gz + 31) / 32) return fail(RC_ERR_ARG, "rc_dda_volume: volume smaller than the grid");
+    if (!s->uploaded) return fail(RC_ERR_STATE, "scene not uploaded");
+    HIPCK(rc_launch_dda_volume(&s->ds, gx, gy, gz, d_A, d_B, d_seg_vol, n, d_vol, words_per_vol, pick(s, stream)));
+    return RC_OK;
+}

@@ -1,0 +1,350 @@
+// Device primitives of the cache frame: RNG, cosine sampling, ray/triangle and ray/box
+// tests, BVH traversal, the 64-bit cell-key hash grid and the transit-grid DDA.
+//
+// Every function keeps the prototype's operation order so that a scalar CPU build of the
+// same arithmetic (compiled with -ffp-contract=off, as this file is) gives identical bits.
+// Two deliberate departures from the prototype, both needed for CPU/GPU bit equality:
+//   * rsqrtf in cosine_dir (cuda/poc_gpu.cu:363) becomes 1/sqrtf (both correctly rounded);
+//   * cosf/sinf (cuda/poc_gpu.cu:365) become rc_sincos below: Cody-Waite reduction by pi/2
+//     (two-constant split) and Taylor polynomials of degree 9 (sin) / 10 (cos) on
+//     [-pi/4, pi/4], evaluated in a fixed order.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "rc_types.h"
+
+struct f3 { float x, y, z; };
+__device__ __forceinline__ f3 mk3(float x, float y, float z) { return {x, y, z}; }
+__device__ __forceinline__ f3 operator+(f3 a, f3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ f3 operator-(f3 a, f3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ f3 operator*(f3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+__device__ __forceinline__ float dot3(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ f3 cross3(f3 a, f3 b) {
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+__device__ __forceinline__ f3 ld3(const float* p, int64_t i) { return {p[3 * i], p[3 * i + 1], p[3 * i + 2]}; }
+
+// ---------------------------------------------------------------- RNG (cuda/poc_gpu.cu:127-135)
+__device__ __forceinline__ uint32_t rc_hash32(uint32_t x) {
+    x ^= x >> 16; x *= 0x7feb352dU; x ^= x >> 15; x *= 0x846ca68bU; x ^= x >> 16; return x;
+}
+__device__ __forceinline__ float rc_rnd(uint32_t& s) {
+    s = s * 747796405u + 2891336453u;
+    uint32_t w = ((s >> ((s >> 28u) + 4u)) ^ s) * 277803737u;
+    w = (w >> 22u) ^ w;
+    return (float)(w >> 8) * (1.0f / 16777216.0f);
+}
+// per-point / per-path seed chain (cuda/poc_gpu.cu:442)
+__device__ __forceinline__ uint32_t rc_seed(uint32_t seed, uint32_t frame, uint32_t i) {
+    return rc_hash32(seed * 0x9E3779B9u ^ rc_hash32(frame * 0x85EBCA6Bu ^ rc_hash32(i + 1u)));
+}
+
+// ---------------------------------------------------------------- sin / cos
+__device__ __forceinline__ void rc_sincos(float x, float& s, float& c) {
+    const float TWO_OVER_PI = 0.636619772367581f;
+    const float PIO2_HI = 1.5703125f;              // 201/128, exact product with small q
+    const float PIO2_LO = 4.83826794897e-4f;       // pi/2 - PIO2_HI
+    float qf = floorf(x * TWO_OVER_PI + 0.5f);
+    int q = (int)qf;
+    float y = (x - qf * PIO2_HI) - qf * PIO2_LO;
+    float y2 = y * y;
+    float sp = y + y * y2 * (-1.66666667e-1f + y2 * (8.33333333e-3f + y2 * (-1.98412698e-4f + y2 * 2.75573192e-6f)));
+    float cp = 1.0f + y2 * (-0.5f + y2 * (4.16666667e-2f + y2 * (-1.38888889e-3f + y2 * (2.48015873e-5f + y2 * -2.75573192e-7f))));
+    switch (q & 3) {
+        case 0: s = sp; c = cp; break;
+        case 1: s = cp; c = -sp; break;
+        case 2: s = -sp; c = -cp; break;
+        default: s = -cp; c = sp; break;
+    }
+}
+
+// cosine-weighted hemisphere direction around N (cuda/poc_gpu.cu:360-366)
+__device__ __forceinline__ f3 rc_cosine_dir(f3 N, float u1, float u2) {
+    float r = sqrtf(u1), phi = 6.28318530718f * u2;
+    f3 a = fabsf(N.x) > 0.9f ? mk3(0.f, 1.f, 0.f) : mk3(1.f, 0.f, 0.f);
+    f3 T = cross3(N, a);
+    T = T * (1.0f / sqrtf(dot3(T, T)));
+    f3 B = cross3(N, T);
+    float sn, cs;
+    rc_sincos(phi, sn, cs);
+    return T * (r * cs) + B * (r * sn) + N * sqrtf(fmaxf(0.f, 1.f - u1));
+}
+
+// ---------------------------------------------------------------- triangles
+// Dynamic triangles held in LDS: closest hit, strict '<' so the first of equal hits wins
+// (cuda/poc_gpu.cu:200-220). q = quad id of the hit triangle, -1 if none.
+__device__ __forceinline__ bool rc_trace_lds(f3 o, f3 d, const float* T, const int* TQ, int nt, float& tmin, int& q) {
+    tmin = __int_as_float(0x7f800000); q = -1;
+    for (int j = 0; j < nt; ++j) {
+        f3 v0 = mk3(T[9 * j], T[9 * j + 1], T[9 * j + 2]);
+        f3 e1 = mk3(T[9 * j + 3], T[9 * j + 4], T[9 * j + 5]);
+        f3 e2 = mk3(T[9 * j + 6], T[9 * j + 7], T[9 * j + 8]);
+        f3 p = cross3(d, e2);
+        float det = dot3(e1, p);
+        if (fabsf(det) <= 1e-9f) continue;
+        float inv = 1.f / det;
+        f3 tv = o - v0;
+        float u = dot3(tv, p) * inv;
+        if (u < 0.f) continue;
+        f3 qv = cross3(tv, e1);
+        float v = dot3(d, qv) * inv;
+        if (v < 0.f || u + v > 1.f) continue;
+        float t = dot3(e2, qv) * inv;
+        if (t > 1e-4f && t < tmin) { tmin = t; q = TQ[j]; }
+    }
+    return q >= 0;
+}
+
+// Static-triangle test of the BVH leaves (cuda/poc_gpu.cu:305-321). Returns t or -1.
+__device__ __forceinline__ float rc_tri_static(f3 o, f3 d, const float* __restrict__ Tp) {
+    f3 v0 = mk3(Tp[0], Tp[1], Tp[2]);
+    f3 e1 = mk3(Tp[3], Tp[4], Tp[5]);
+    f3 e2 = mk3(Tp[6], Tp[7], Tp[8]);
+    f3 p = cross3(d, e2);
+    float det = dot3(e1, p);
+    if (fabsf(det) <= 1e-12f) return -1.f;
+    float iv = 1.f / det;
+    f3 tv = o - v0;
+    float u = dot3(tv, p) * iv;
+    if (u < 0.f || u > 1.f) return -1.f;
+    f3 qv = cross3(tv, e1);
+    float v = dot3(d, qv) * iv;
+    if (v < 0.f || u + v > 1.f) return -1.f;
+    return dot3(e2, qv) * iv;
+}
+
+// slab entry distance, 1e30 on a miss (cuda/poc_gpu.cu:288-295)
+__device__ __forceinline__ float rc_aabb_enter(const float* lo, const float* hi, f3 o, f3 inv, float tmax) {
+    float tx1 = (lo[0] - o.x) * inv.x, tx2 = (hi[0] - o.x) * inv.x;
+    float ty1 = (lo[1] - o.y) * inv.y, ty2 = (hi[1] - o.y) * inv.y;
+    float tz1 = (lo[2] - o.z) * inv.z, tz2 = (hi[2] - o.z) * inv.z;
+    float tn = fmaxf(fmaxf(fminf(tx1, tx2), fminf(ty1, ty2)), fmaxf(fminf(tz1, tz2), 0.f));
+    float tf = fminf(fminf(fmaxf(tx1, tx2), fmaxf(ty1, ty2)), fminf(fmaxf(tz1, tz2), tmax));
+    return tn <= tf ? tn : 1e30f;
+}
+
+__device__ __forceinline__ f3 rc_inv_dir(f3 d) {
+    return mk3(fabsf(d.x) > 1e-12f ? 1.f / d.x : copysignf(1e30f, d.x),
+               fabsf(d.y) > 1e-12f ? 1.f / d.y : copysignf(1e30f, d.y),
+               fabsf(d.z) > 1e-12f ? 1.f / d.z : copysignf(1e30f, d.z));
+}
+
+// Binary-BVH traversal in the prototype's order: nearest child first (ties: left), a
+// 64-entry stack whose overflowing pushes are dropped, strict '<' on t so the first
+// triangle found among equal hits wins. q = -2 - triangle index (cuda/poc_gpu.cu:297-338).
+__device__ __forceinline__ void rc_trace_bvh_ref(f3 o, f3 d, const rc_dscene& s, float& tmin, int& q) {
+    f3 inv = rc_inv_dir(d);
+    int stack[64]; int sp = 0, node = 0;
+    while (true) {
+        const rc_bvh_node nd = s.BN[node];
+        if (nd.cnt > 0) {
+            for (int j = nd.lf; j < nd.lf + nd.cnt; ++j) {
+                float t = rc_tri_static(o, d, s.ST + 9 * (int64_t)j);
+                if (t > 1e-4f && t < tmin) { tmin = t; q = -2 - j; }
+            }
+        } else {
+            const rc_bvh_node L = s.BN[nd.lf], R = s.BN[nd.lf + 1];
+            float tl = rc_aabb_enter(L.lo, L.hi, o, inv, tmin), tr = rc_aabb_enter(R.lo, R.hi, o, inv, tmin);
+            int first = nd.lf, second = nd.lf + 1;
+            if (tr < tl) { float x = tl; tl = tr; tr = x; first = nd.lf + 1; second = nd.lf; }
+            if (tl < 1e30f) {
+                if (tr < 1e30f && sp < 64) stack[sp++] = second;
+                node = first; continue;
+            }
+        }
+        if (sp == 0) break;
+        node = stack[--sp];
+    }
+}
+
+// Order-independent closest hit: smallest t, ties between static triangles go to the lowest
+// triangle index, a dynamic-triangle hit keeps its ties with static ones. Any traversal order
+// (binary or wide BVH) that never culls a box holding the closest hit returns the same (t, q).
+__device__ __forceinline__ bool rc_closer(float t, int j, float tmin, int q) {
+    return t > 1e-4f && (t < tmin || (t == tmin && q <= -2 && j < -2 - q));
+}
+
+__device__ __forceinline__ void rc_trace_bvh_closest(f3 o, f3 d, const rc_dscene& s, float& tmin, int& q) {
+    f3 inv = rc_inv_dir(d);
+    int stack[64]; int sp = 0, node = 0;
+    while (true) {
+        const rc_bvh_node nd = s.BN[node];
+        if (nd.cnt > 0) {
+            for (int j = nd.lf; j < nd.lf + nd.cnt; ++j) {
+                float t = rc_tri_static(o, d, s.ST + 9 * (int64_t)j);
+                if (rc_closer(t, j, tmin, q)) { tmin = t; q = -2 - j; }
+            }
+        } else {
+            const rc_bvh_node L = s.BN[nd.lf], R = s.BN[nd.lf + 1];
+            float tl = rc_aabb_enter(L.lo, L.hi, o, inv, tmin), tr = rc_aabb_enter(R.lo, R.hi, o, inv, tmin);
+            int first = nd.lf, second = nd.lf + 1;
+            if (tr < tl) { float x = tl; tl = tr; tr = x; first = nd.lf + 1; second = nd.lf; }
+            if (tl < 1e30f) {
+                if (tr < 1e30f) stack[sp++] = second;   // depth <= 56 by construction
+                node = first; continue;
+            }
+        }
+        // pop, skipping entries whose box now starts beyond the current hit is left to the
+        // box test of their parent: popped nodes are re-tested below when they are inner.
+        if (sp == 0) break;
+        node = stack[--sp];
+    }
+}
+
+// 4-wide BVH traversal with the order-independent closest-hit rule. Leaf children are
+// intersected as soon as their node is visited; inner children are pushed far-to-near with
+// their entry distance and skipped on pop once that distance exceeds the current hit.
+__device__ __forceinline__ void rc_trace_wide_closest(f3 o, f3 d, const rc_dscene& s, float& tmin, int& q) {
+    f3 inv = rc_inv_dir(d);
+    int stk[RC_WSTACK]; float stt[RC_WSTACK];
+    int sp = 0, cur = 0;
+    while (true) {
+        const rc_wnode* nd = s.WN + cur;
+        const float4* nb = (const float4*)nd;
+        float4 lx = nb[0], ly = nb[1], lz = nb[2], hx = nb[3], hy = nb[4], hz = nb[5];
+        int4 ch = ((const int4*)nd)[6], cc = ((const int4*)nd)[7];
+        float lox[4] = {lx.x, lx.y, lx.z, lx.w}, loy[4] = {ly.x, ly.y, ly.z, ly.w}, loz[4] = {lz.x, lz.y, lz.z, lz.w};
+        float hix[4] = {hx.x, hx.y, hx.z, hx.w}, hiy[4] = {hy.x, hy.y, hy.z, hy.w}, hiz[4] = {hz.x, hz.y, hz.z, hz.w};
+        int chv[4] = {ch.x, ch.y, ch.z, ch.w}, ccv[4] = {cc.x, cc.y, cc.z, cc.w};
+        float tc[4]; int ic[4]; int nh = 0;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            if (chv[c] == -1) continue;
+            float lo[3] = {lox[c], loy[c], loz[c]}, hi[3] = {hix[c], hiy[c], hiz[c]};
+            float te = rc_aabb_enter(lo, hi, o, inv, tmin);
+            if (te >= 1e30f) continue;
+            if (chv[c] <= -2) {
+                int first = -2 - chv[c], cnt = ccv[c];
+                for (int j = first; j < first + cnt; ++j) {
+                    float t = rc_tri_static(o, d, s.ST + 9 * (int64_t)j);
+                    if (rc_closer(t, j, tmin, q)) { tmin = t; q = -2 - j; }
+                }
+            } else {
+                tc[nh] = te; ic[nh] = chv[c]; ++nh;
+            }
+        }
+        // sort the inner hits by entry distance (at most 4)
+        for (int a = 1; a < nh; ++a) {
+            float tv = tc[a]; int iv = ic[a]; int b = a - 1;
+            while (b >= 0 && tc[b] > tv) { tc[b + 1] = tc[b]; ic[b + 1] = ic[b]; --b; }
+            tc[b + 1] = tv; ic[b + 1] = iv;
+        }
+        int next = -1; float next_t = 0.f;
+        for (int a = nh - 1; a >= 0; --a) {
+            if (tc[a] > tmin) continue;
+            if (next >= 0) { stk[sp] = next; stt[sp] = next_t; ++sp; }
+            next = ic[a]; next_t = tc[a];
+        }
+        // 'next' holds the nearest surviving child; the pushes above stored the farther ones
+        if (next >= 0) { cur = next; continue; }
+        bool found = false;
+        while (sp > 0) {
+            --sp;
+            if (stt[sp] <= tmin) { cur = stk[sp]; found = true; break; }
+        }
+        if (!found) break;
+    }
+}
+
+// ---------------------------------------------------------------- hash grid (cuda/poc_gpu.cu:251-287)
+__device__ __forceinline__ int rc_normal_bucket(float nx, float ny, float nz) {
+    float ax = fabsf(nx), ay = fabsf(ny), az = fabsf(nz);
+    if (ax >= ay && ax >= az) return nx > 0.f ? 0 : 1;
+    if (ay >= az) return ny > 0.f ? 2 : 3;
+    return nz > 0.f ? 4 : 5;
+}
+__device__ __forceinline__ uint64_t rc_cell_key(int cx, int cy, int cz, int b) {
+    return ((((uint64_t)(cx & 0xFFFFF) << 20 | (uint64_t)(cy & 0xFFFFF)) << 20 | (uint64_t)(cz & 0xFFFFF)) << 3 | (uint64_t)b) + 1ull;
+}
+__device__ __forceinline__ uint64_t rc_mix64(uint64_t x) {
+    x ^= x >> 30; x *= 0xbf58476d1ce4e5b9ull; x ^= x >> 27; x *= 0x94d049bb133111ebull; x ^= x >> 31; return x;
+}
+__device__ __forceinline__ int rc_hash_find(const uint64_t* __restrict__ HK, const int* __restrict__ HV, uint32_t hmask, uint64_t key) {
+    uint32_t h = (uint32_t)rc_mix64(key) & hmask;
+    for (int probe = 0; probe < 64; ++probe) {
+        uint64_t k = HK[h];
+        if (k == key) return HV[h];
+        if (k == 0ull) return -1;
+        h = (h + 1) & hmask;
+    }
+    return -1;
+}
+__device__ __forceinline__ int rc_hash_point(const rc_dscene& s, f3 H, f3 n) {
+    int b = rc_normal_bucket(n.x, n.y, n.z);
+    float ih = 1.f / s.hcell;
+    int cx = (int)floorf((H.x - s.lo[0]) * ih), cy = (int)floorf((H.y - s.lo[1]) * ih), cz = (int)floorf((H.z - s.lo[2]) * ih);
+    int p = rc_hash_find(s.HK, s.HV, s.hmask, rc_cell_key(cx, cy, cz, b));
+    if (p >= 0) return p;
+    for (int dd = 0; dd < 27; ++dd) {
+        int dz = dd / 9 - 1, dy = (dd / 3) % 3 - 1, dx = dd % 3 - 1;
+        if (dx == 0 && dy == 0 && dz == 0) continue;
+        p = rc_hash_find(s.HK, s.HV, s.hmask, rc_cell_key(cx + dx, cy + dy, cz + dz, b));
+        if (p >= 0) return p;
+    }
+    return -1;
+}
+
+// cache point of a quad hit (cuda/poc_gpu.cu:239-246)
+__device__ __forceinline__ int rc_point_index(f3 H, int q, const rc_dstate& st) {
+    f3 o = ld3(st.Q + 12 * q, 0), u = ld3(st.Q + 12 * q, 1), v = ld3(st.Q + 12 * q, 2);
+    f3 dd = H - o;
+    float a = dot3(dd, u) / dot3(u, u), b = dot3(dd, v) / dot3(v, v);
+    int nu = st.dims[2 * q], nv = st.dims[2 * q + 1];
+    int iu = min(max((int)(a * nu), 0), nu - 1), iv = min(max((int)(b * nv), 0), nv - 1);
+    return st.off[q] + iv * nu + iu;
+}
+
+// hit -> cache entry; true when the hit does not count (cuda/poc_gpu.cu:346-358)
+__device__ __forceinline__ bool rc_resolve_hit(const rc_dscene& s, const rc_dstate& st, bool hit, f3 H, f3 D, int q, int i, int& pi) {
+    if (!hit) { pi = i; return true; }
+    if (q >= 0) { pi = rc_point_index(H, q, st); return dot3(ld3(st.Q + 12 * q, 3), D) > 0.f; }
+    const float* n4 = s.STN + 4 * (int64_t)(-2 - q);
+    f3 n = mk3(n4[0], n4[1], n4[2]);
+    if (dot3(n, D) > 0.f) {
+        if (!(__float_as_int(n4[3]) & 1)) { pi = i; return true; }
+        n = n * -1.f;
+    }
+    pi = rc_hash_point(s, H, n);
+    if (pi < 0) { pi = i; return true; }
+    return false;
+}
+
+// ---------------------------------------------------------------- transit grid (cuda/poc_gpu.cu:137-197)
+__device__ __forceinline__ int rc_vox_of(const rc_dscene& s, f3 p) {
+    int i = min(max((int)((p.x - s.lo[0]) / s.ext[0] * s.vx), 0), s.vx - 1);
+    int j = min(max((int)((p.y - s.lo[1]) / s.ext[1] * s.vy), 0), s.vy - 1);
+    int k = min(max((int)((p.z - s.lo[2]) / s.ext[2] * s.vz), 0), s.vz - 1);
+    return (k * s.vy + j) * s.vx + i;
+}
+__device__ __forceinline__ void rc_set_bit(uint32_t (&m)[RC_MAX_NW], int v) {
+    uint32_t bit = 1u << (v & 31); int w0 = v >> 5;
+#pragma unroll
+    for (int w = 0; w < RC_MAX_NW; ++w) m[w] |= (w == w0) ? bit : 0u;
+}
+// Amanatides-Woo walk of segment a->b over the vx*vy*vz transit grid, OR-ing the voxels
+// crossed into m (cuda/poc_gpu.cu:171-190).
+__device__ __forceinline__ void rc_dda_reg(const rc_dscene& s, f3 a, f3 b, uint32_t (&m)[RC_MAX_NW]) {
+    const int VX = s.vx, VY = s.vy, VZ = s.vz;
+    const float sx = VX / s.ext[0], sy = VY / s.ext[1], sz = VZ / s.ext[2];
+    float ax = (a.x - s.lo[0]) * sx, ay = (a.y - s.lo[1]) * sy, az = (a.z - s.lo[2]) * sz;
+    float bx = (b.x - s.lo[0]) * sx, by = (b.y - s.lo[1]) * sy, bz = (b.z - s.lo[2]) * sz;
+    int ix = min(max((int)ax, 0), VX - 1), iy = min(max((int)ay, 0), VY - 1), iz = min(max((int)az, 0), VZ - 1);
+    int ex = min(max((int)bx, 0), VX - 1), ey = min(max((int)by, 0), VY - 1), ez = min(max((int)bz, 0), VZ - 1);
+    float dx = bx - ax, dy = by - ay, dz = bz - az;
+    int stx = dx > 0 ? 1 : -1, sty = dy > 0 ? 1 : -1, stz = dz > 0 ? 1 : -1;
+    float tdx = dx != 0 ? fabsf(1.f / dx) : 1e30f, tdy = dy != 0 ? fabsf(1.f / dy) : 1e30f, tdz = dz != 0 ? fabsf(1.f / dz) : 1e30f;
+    float tmx = dx != 0 ? ((dx > 0 ? ((float)(ix + 1) - ax) : (ax - (float)ix)) * tdx) : 1e30f;
+    float tmy = dy != 0 ? ((dy > 0 ? ((float)(iy + 1) - ay) : (ay - (float)iy)) * tdy) : 1e30f;
+    float tmz = dz != 0 ? ((dz > 0 ? ((float)(iz + 1) - az) : (az - (float)iz)) * tdz) : 1e30f;
+    for (int g = 0; g < VX + VY + VZ; ++g) {
+        rc_set_bit(m, (iz * VY + iy) * VX + ix);
+        if (ix == ex && iy == ey && iz == ez) break;
+        if (tmx < tmy && tmx < tmz) { ix += stx; tmx += tdx; if (ix < 0 || ix >= VX) break; }
+        else if (tmy < tmz) { iy += sty; tmy += tdy; if (iy < 0 || iy >= VY) break; }
+        else { iz += stz; tmz += tdz; if (iz < 0 || iz >= VZ) break; }
+    }
+}
+// 20 point samples along the segment (cuda/poc_gpu.cu:192-197)
+__device__ __forceinline__ void rc_rast_seg(const rc_dscene& s, f3 a, f3 b, uint32_t (&m)[RC_MAX_NW]) {
+    for (int k = 0; k < 20; ++k) rc_set_bit(m, rc_vox_of(s, a + (b - a) * ((float)k / 19.f)));
+}

@@ -1,0 +1,34 @@
+// Host-side launchers of the HIP kernels (rc_kernels.hip). All take device pointers and a
+// stream; they return the launch status.
+#pragma once
+#include <hip/hip_runtime.h>
+#include "rc_types.h"
+
+hipError_t rc_launch_trace_rays(int trav, const rc_dscene* s, const rc_dstate* st, const float* O, const float* D, int n,
+                                float* t, int* q, hipStream_t stream);
+hipError_t rc_launch_build(int trav, const rc_dscene* s, const rc_dstate* st, hipStream_t stream);
+hipError_t rc_launch_trace2(int trav, int deps, const rc_dscene* s, const rc_dstate* st, const rc_dcache* c, const rc_params* p,
+                            uint64_t* lnew, hipStream_t stream);
+hipError_t rc_launch_decide(const rc_dscene* s, const rc_dcache* c, const rc_params* p, hipStream_t stream);
+hipError_t rc_launch_update(const rc_dscene* s, const rc_dcache* c, const rc_params* p, const uint64_t* lnew, hipStream_t stream);
+hipError_t rc_launch_sumpr(const rc_dcache* c, int Np, hipStream_t stream);
+hipError_t rc_launch_restored(const rc_dcache* c, int Np, int nl, float nmax, uint64_t dormant, int verify, hipStream_t stream);
+hipError_t rc_launch_rescale(const rc_dcache* c, int Np, int nl, int l, float ratio, hipStream_t stream);
+hipError_t rc_launch_touched(const rc_dcache* c, int Np, int nl, const int* V, int nv, int kind, uint64_t dormant, hipStream_t stream);
+hipError_t rc_launch_wake(const rc_dcache* c, int Np, int nl, int l, int verify, uint64_t dormant, hipStream_t stream);
+hipError_t rc_launch_oracle_method(const rc_dcache* c, int Np, const float* rold, const float* rnew, hipStream_t stream);
+hipError_t rc_launch_invalidate(const rc_dcache* c, int Np, int nl, uint64_t inv_lights, const int* V, int nv, int nw, hipStream_t stream);
+hipError_t rc_launch_flush(const rc_dcache* c, int Np, int nl, hipStream_t stream);
+hipError_t rc_launch_error(const rc_dcache* c, int Np, int nl, const float* ref, const uint8_t* valid, uint64_t dormant, hipStream_t stream);
+hipError_t rc_launch_ref(int trav, const rc_dscene* s, const rc_dstate* st, const float* Lin, float* Lout, float* acc, float* closec,
+                         uint32_t seed, int K, int accumulate, int count_close, hipStream_t stream);
+hipError_t rc_launch_query(int trav, const rc_dscene* s, const rc_dstate* st, const float* Lc, const rc_camera* cam, uint32_t seed,
+                           uint32_t frame, uint64_t dormant, float* img, unsigned long long* counters, hipStream_t stream);
+hipError_t rc_launch_hash_insert(unsigned long long* HK, int* HV, uint32_t hmask, const unsigned long long* keys, const int* vals,
+                                 int n, int* fail, hipStream_t stream);
+hipError_t rc_launch_hash_query(const unsigned long long* HK, const int* HV, uint32_t hmask, const unsigned long long* keys, int n,
+                                int* out, hipStream_t stream);
+hipError_t rc_launch_hash_point(const rc_dscene* s, const float* H, const float* Nn, int n, int* out, hipStream_t stream);
+hipError_t rc_launch_dda_segments(const rc_dscene* s, const float* A, const float* B, int n, uint32_t* out, hipStream_t stream);
+hipError_t rc_launch_dda_volume(const rc_dscene* s, int gx, int gy, int gz, const float* A, const float* B, const int* seg_vol, int n,
+                                uint32_t* vol, long long words_per_vol, hipStream_t stream);

@@ -1,0 +1,770 @@
+// Frame kernels of the dependency-aware radiance cache for gfx950 (wave64).
+//
+// One thread per cache entry for the entry passes (build / trace / update / event kernels),
+// one thread per pixel for the camera query pass. Reductions that feed control flow use
+// int64 fixed point so that the result does not depend on the order waves finish in;
+// wave-level partial sums use the full 64-lane width.
+#include "rc_device.h"
+#include "rc_kernels.h"
+
+#define RC_TB 256
+
+// Reduction of a 64-bit integer over the 64 lanes of a wave (any order gives the same sum).
+__device__ __forceinline__ long long rc_wave_sum_i64(long long v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    return v;
+}
+__device__ __forceinline__ double rc_wave_sum_f64(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    return v;
+}
+
+__device__ __forceinline__ long long rc_fx(float v, float scale) { return (long long)(v * scale); }
+
+template <int TRAV>
+__device__ __forceinline__ void rc_trace_static(f3 o, f3 d, const rc_dscene& s, float& tmin, int& q);
+
+template <> __device__ __forceinline__ void rc_trace_static<0>(f3 o, f3 d, const rc_dscene& s, float& tmin, int& q) {
+    rc_trace_bvh_ref(o, d, s, tmin, q);
+}
+template <> __device__ __forceinline__ void rc_trace_static<1>(f3 o, f3 d, const rc_dscene& s, float& tmin, int& q) {
+    rc_trace_bvh_closest(o, d, s, tmin, q);
+}
+template <> __device__ __forceinline__ void rc_trace_static<2>(f3 o, f3 d, const rc_dscene& s, float& tmin, int& q) {
+    rc_trace_wide_closest(o, d, s, tmin, q);
+}
+
+// Dynamic triangles from LDS, then the static BVH carrying tmin (cuda/poc_gpu.cu:340-344).
+template <int TRAV>
+__device__ __forceinline__ bool rc_trace_full(f3 o, f3 d, const float* T, const int* TQ, int nt, const rc_dscene& s, float& tmin, int& q) {
+    bool h = rc_trace_lds(o, d, T, TQ, nt, tmin, q);
+    if (s.nst > 0) { rc_trace_static<TRAV>(o, d, s, tmin, q); h = q != -1; }
+    return h;
+}
+
+__device__ __forceinline__ void rc_load_dyn(const rc_dstate& st, float* sh, int* shq) {
+    for (int j = threadIdx.x; j < 9 * st.nt; j += blockDim.x) sh[j] = st.T[j];
+    for (int j = threadIdx.x; j < st.nt; j += blockDim.x) shq[j] = st.TQ[j];
+    __syncthreads();
+}
+
+// ------------------------------------------------------------------------------ rays (kernel ABI)
+template <int TRAV>
+__global__ void __launch_bounds__(RC_TB) k_trace_rays(rc_dscene s, rc_dstate st, const float* __restrict__ O,
+                                                     const float* __restrict__ D, int n, float* tout, int* qout) {
+    extern __shared__ float sh[];
+    int* shq = (int*)(sh + 9 * st.nt);
+    rc_load_dyn(st, sh, shq);
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float t; int q;
+    rc_trace_full<TRAV>(ld3(O, i), ld3(D, i), sh, shq, st.nt, s, t, q);
+    tout[i] = t; qout[i] = q;
+}
+
+// ------------------------------------------------------------------------------ state build
+// Direct light per entry and light, static transit mask of the shadow segments and the
+// entry's light-visibility mask (cuda/poc_gpu.cu:369-391). Directional lights (the sun) use
+// no distance falloff and a shadow segment of length s.sun_len.
+template <int TRAV>
+__global__ void __launch_bounds__(RC_TB) k_build(rc_dscene s, rc_dstate st) {
+    extern __shared__ float sh[];
+    int* shq = (int*)(sh + 9 * st.nt);
+    rc_load_dyn(st, sh, shq);
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= s.Np) return;
+    f3 P = ld3(st.P, i), N = ld3(st.N, i), O = P + N * 1e-3f;
+    uint32_t m[RC_MAX_NW] = {0u, 0u, 0u, 0u};
+    uint64_t lm = 0ull;
+    for (int l = 0; l < s.nl; ++l) {
+        float r; f3 D;
+        if (s.ltype[l] == 1) { D = ld3(s.L, l); r = s.sun_len; }
+        else { f3 dv = ld3(s.L, l) - P; r = sqrtf(dot3(dv, dv)); D = dv * (1.f / r); }
+        float c = dot3(N, D), t; int q;
+        rc_trace_full<TRAV>(O, D, sh, shq, st.nt, s, t, q);
+        bool vis = t >= r - 1e-3f;
+        float e;
+        if (s.ltype[l] == 1) e = st.inten[l] * fmaxf(c, 0.f) * (vis ? 1.f : 0.f);
+        else { float rr = fmaxf(r, 0.05f); e = st.inten[l] * fmaxf(c, 0.f) / (rr * rr) * (vis ? 1.f : 0.f); }
+        st.Edir[(int64_t)s.nl * i + l] = e;
+        if (vis && c > 0.f) lm |= 1ull << (l < 63 ? l : 63);
+        rc_rast_seg(s, O, O + D * fminf(t, r), m);
+    }
+    for (int w = 0; w < s.nw; ++w) st.SV[(int64_t)s.nw * i + w] = m[w];
+    st.LM[i] = lm;
+}
+
+// ------------------------------------------------------------------------------ frame: trace
+__device__ __forceinline__ float rc_est_of(const float* Lc, int64_t i, int nl, uint64_t dormant) {
+    float e = 0.f;
+    for (int l = 0; l < nl; ++l) if (!(l < 64 && ((dormant >> l) & 1ull))) e += Lc[(int64_t)nl * i + l];
+    return e;
+}
+__device__ __forceinline__ bool rc_is_dormant(uint64_t dormant, int l) { return l < 64 && ((dormant >> l) & 1ull); }
+
+// k rays per entry, cache read at the hit (multi-bounce through the cache), per-light sample
+// means in c.m, dependency masks (cuda/poc_gpu.cu:433-530). The per-light accumulator is the
+// thread's own row of c.m (rays outer, lights inner) so any light count runs in one kernel.
+// Light-dependency masks (deps on): own visible lights | hit's visible lights | hit's recorded
+// light deps (cache-terminated path inherits them), staged in lnew and merged by k_update.
+template <bool DEPS, int TRAV>
+__global__ void __launch_bounds__(RC_TB) k_trace2(rc_dscene s, rc_dstate st, rc_dcache c, rc_params p, uint64_t* lnew) {
+    extern __shared__ float sh[];
+    int* shq = (int*)(sh + 9 * st.nt);
+    rc_load_dyn(st, sh, shq);
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= p.Np) return;
+    const int nl = p.nl;
+    uint32_t rs = rc_seed(p.seed, p.frame, (uint32_t)i);
+    int k;
+    bool pend = p.pend_left > 0 && c.pending[i];
+    if (p.mode == 0) {
+        float kb = floorf(p.budget);
+        k = (int)kb + (rc_rnd(rs) < p.budget - kb ? 1 : 0);
+    } else {
+        float sumpr = (float)((double)c.sumpr[0] * (1.0 / 4294967296.0));
+        float pr = 1.f / (c.n[i] + 1.f);
+        float rrest = (float)(p.Np - c.npend[0]);
+        float e = rrest * (0.5f / (float)p.Np + 0.5f * pr / sumpr) * p.rays;
+        float eb = floorf(e);
+        k = (pend ? 1 : 0) + (int)eb + (rc_rnd(rs) < e - eb ? 1 : 0);
+    }
+    f3 P = ld3(st.P, i), N = ld3(st.N, i), O = P + N * 1e-3f;
+    uint32_t m[RC_MAX_NW] = {0u, 0u, 0u, 0u};
+    uint64_t lm = DEPS ? st.LM[i] : 0ull;
+    float c0 = pend ? c.c0[i] : 0.f;
+    int rg = pend ? s.region[i] : 0;
+    float sd = 0.f, sq = 0.f;
+    float* mrow = c.m + (int64_t)nl * i;
+    for (int r = 0; r < k; ++r) {
+        float u1 = rc_rnd(rs), u2 = rc_rnd(rs);
+        f3 D = rc_cosine_dir(N, u1, u2);
+        float t; int q;
+        bool hit = rc_trace_full<TRAV>(O, D, sh, shq, st.nt, s, t, q);
+        f3 H = hit ? O + D * t : O;
+        int pi;
+        bool back = rc_resolve_hit(s, st, hit, H, D, q, i, pi);
+        float val = 0.f;
+        if (!back) {
+            float A = s.Alb[pi];
+            const float* ed = st.Edir + (int64_t)nl * pi;
+            const float* lc = c.Lc + (int64_t)nl * pi;
+            for (int l = 0; l < nl; ++l) {
+                float lv = rc_is_dormant(p.dormant, l) ? 0.f : lc[l];
+                float v = A * (ed[l] + lv);
+                if (r == 0) mrow[l] = v; else mrow[l] += v;
+                val += v;
+            }
+        } else if (r == 0) {
+            for (int l = 0; l < nl; ++l) mrow[l] = 0.f;
+        }
+        if (DEPS) {
+            int src = back ? i : pi;
+            const uint32_t* sv = st.SV + (int64_t)s.nw * src;
+#pragma unroll
+            for (int w = 0; w < RC_MAX_NW; ++w) if (w < s.nw) m[w] |= sv[w];
+            rc_dda_reg(s, O, H, m);
+            if (!back) lm |= st.LM[pi] | c.lcur[pi] | c.lprev[pi];
+        }
+        if (pend) { float dv = val - c0; sd += dv; sq += dv * dv; }
+    }
+    if (pend && k > 0) {
+        atomicAdd((unsigned long long*)&c.rstat[4 * rg + 0], (unsigned long long)rc_fx(sd, RC_FX_STAT));
+        atomicAdd((unsigned long long*)&c.rstat[4 * rg + 1], (unsigned long long)rc_fx(sq, RC_FX_STAT));
+        atomicAdd((unsigned long long*)&c.rstat[4 * rg + 2], (unsigned long long)rc_fx(c0 * (float)k, RC_FX_STAT));
+        atomicAdd((unsigned long long*)&c.rstat[4 * rg + 3], (unsigned long long)rc_fx((float)k, RC_FX_STAT));
+    }
+    c.kk[i] = k;
+    float ik = k > 0 ? 1.f / (float)k : 0.f;
+    float sa = 0.f;
+    if (p.ddgi && k > 0)
+        for (int l = 0; l < nl; ++l) sa += mrow[l];   // sum of the per-light accumulators
+    if (k == 0) {
+        for (int l = 0; l < nl; ++l) mrow[l] = 0.f;
+    } else if (k > 1) {
+        for (int l = 0; l < nl; ++l) mrow[l] = mrow[l] * ik;
+    }
+    if (p.ddgi && k > 0) {
+        int rgd = s.region[i];
+        atomicAdd((unsigned long long*)&c.dstat[3 * rgd + 0], (unsigned long long)rc_fx(sa * ik, RC_FX_STAT));
+        atomicAdd((unsigned long long*)&c.dstat[3 * rgd + 1], (unsigned long long)rc_fx(rc_est_of(c.Lc, i, nl, p.dormant), RC_FX_STAT));
+        atomicAdd((unsigned long long*)&c.dstat[3 * rgd + 2], (unsigned long long)rc_fx(1.f, RC_FX_STAT));
+    }
+    if (DEPS) {
+        if (k > 0) {
+#pragma unroll
+            for (int w = 0; w < RC_MAX_NW; ++w) {
+                if (w >= s.nw) break;
+                uint32_t* cw = c.cur + (int64_t)w * p.Np + i;
+                uint32_t old = *cw;
+                if ((old | m[w]) != old) *cw = old | m[w];
+            }
+        }
+        lnew[i] = k > 0 ? lm : 0ull;
+    }
+}
+
+// verification decision per pending entry (cuda/poc_gpu.cu:532-555)
+__global__ void __launch_bounds__(RC_TB) k_decide(rc_dscene s, rc_dcache c, rc_params p) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= p.Np) return;
+    if (!c.pending[i]) return;
+    const long long* r4 = c.rstat + 4 * s.region[i];
+    const float isc = 1.0f / RC_FX_STAT;
+    float cnt = (float)r4[3] * isc;
+    bool done = false;
+    if (cnt >= 8.f) {
+        float mean = ((float)r4[0] * isc) / cnt;
+        float se = sqrtf(fmaxf(((float)r4[1] * isc) / cnt - mean * mean, 0.f) / cnt);
+        float base = ((float)r4[2] * isc) / cnt;
+        float rel = mean / base;
+        if (fabsf(mean) > 2.f * se && fabsf(rel) > 0.03f) {
+            float tgt = fminf(fmaxf(0.5f / fabsf(rel), 1.f) * p.rays, p.nmax);
+            c.n[i] = fminf(c.n[i], tgt);
+            c.pending[i] = 0;
+            done = true;
+        }
+    }
+    if (!done) {
+        if (p.pend_left <= 1) c.pending[i] = 0;
+        else atomicAdd(&c.npend[1], 1);
+    }
+}
+
+// EMA update of the per-light channels + history count (cuda/poc_gpu.cu:562-602), with the
+// light-dependency merge of this frame's samples and the next frame's sum of 1/(n+1).
+__global__ void __launch_bounds__(RC_TB) k_update(rc_dscene s, rc_dcache c, rc_params p, const uint64_t* lnew) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    long long pr = 0;
+    const int nl = p.nl;
+    if (i < p.Np) {
+        int k = c.kk[i];
+        float n = c.n[i], nn = n + (float)k;
+        float nmx = p.nmax;
+        if (p.ramp) { float a = c.age[i]; nmx = fminf(p.nmax, (4.f + 0.5f * a) * p.rays); c.age[i] = fminf(a + 1.f, 1e4f); }
+        float alpha = k > 0 ? fminf((float)k / fminf(fmaxf(nn, 1.f), nmx), 1.f) : 0.f;
+        const float* a_ = c.Lc + (int64_t)nl * i;
+        const float* m_ = c.m + (int64_t)nl * i;
+        float* o_ = c.Lc2 + (int64_t)nl * i;
+        if (p.ddgi) {
+            const long long* d3 = c.dstat + 3 * s.region[i];
+            const float isc = 1.0f / RC_FX_STAT;
+            float scale = 1.f;
+            float h = p.ddgi_h;
+            float d2 = (float)d3[2] * isc;
+            if (d2 > 0.f) {
+                float mn = ((float)d3[0] * isc) / d2, mc = ((float)d3[1] * isc) / d2;
+                float rel = fabsf(mn - mc) / fmaxf(mc, 1e-3f);
+                if (rel > p.ddgi_thr) h = fmaxf(0.f, h - 0.75f);
+                if (p.ddgi_clamp && rel > 0.10f) scale = 0.25f;
+            }
+            float w = k > 0 ? (1.f - h) * scale : 0.f;
+            for (int l = 0; l < nl; ++l) {
+                float a = a_[l];
+                o_[l] = rc_is_dormant(p.dormant, l) ? a : a + w * (m_[l] - a);
+            }
+        } else {
+            float alpha_r = k > 0 ? fminf((float)k / fminf(fmaxf(nn, 1.f), p.nmax_resp), 1.f) : 0.f;
+            for (int l = 0; l < nl; ++l) {
+                float a = a_[l];
+                float al = (l < 64 && ((p.resp_mask >> l) & 1ull)) ? alpha_r : alpha;
+                if (rc_is_dormant(p.dormant, l)) al = 0.f;
+                o_[l] = a + al * (m_[l] - a);
+            }
+        }
+        n = fminf(nn, nmx);
+        c.n[i] = n;
+        pr = rc_fx(1.f / (n + 1.f), RC_FX_PR);
+        if (lnew) { uint64_t x = lnew[i]; if (x) c.lcur[i] |= x; }
+    }
+    pr = rc_wave_sum_i64(pr);
+    if ((threadIdx.x & 63) == 0) atomicAdd((unsigned long long*)&c.sumpr[1], (unsigned long long)pr);
+}
+
+__global__ void __launch_bounds__(RC_TB) k_sumpr(rc_dcache c, int Np) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    long long pr = i < Np ? rc_fx(1.f / (c.n[i] + 1.f), RC_FX_PR) : 0;
+    pr = rc_wave_sum_i64(pr);
+    if ((threadIdx.x & 63) == 0) atomicAdd((unsigned long long*)&c.sumpr[0], (unsigned long long)pr);
+}
+
+// restored snapshot: full confidence, optionally verified (cuda/poc_gpu.cu:612-617)
+__global__ void __launch_bounds__(RC_TB) k_restored(rc_dcache c, int Np, int nl, float nmax, uint64_t dormant, int verify) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= Np) return;
+    c.n[i] = nmax; c.age[i] = 1e4f; c.pending[i] = 0;
+    if (verify) { c.pending[i] = 1; c.c0[i] = rc_est_of(c.Lc, i, nl, dormant); atomicAdd(&c.npend[1], 1); }
+}
+
+// light-intensity change a -> b on channel l: linear rescale (cuda/poc_gpu.cu:637-640)
+__global__ void __launch_bounds__(RC_TB) k_rescale(rc_dcache c, int Np, int nl, int l, float ratio) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < Np) c.Lc[(int64_t)nl * i + l] *= ratio;
+}
+
+// Event invalidation from the transit-grid banks (cuda/poc_gpu.cu:642-660).
+// kind 0: emitter appeared -> reset history; 1: occluder -> verification; 2: stale marking only.
+__global__ void __launch_bounds__(RC_TB) k_touched(rc_dcache c, int Np, int nl, const int* V, int nv, int kind, uint64_t dormant) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= Np) return;
+    bool t = false;
+    for (int j = 0; j < nv && !t; ++j) {
+        int v = V[j];
+        int64_t o = (int64_t)(v >> 5) * Np + i;
+        t = ((c.cur[o] | c.prev[o]) >> (v & 31)) & 1u;
+    }
+    if (!t) return;
+    if (kind == 0) { c.n[i] = 0.f; c.age[i] = 0.f; }
+    else if (kind == 1) {
+        c.pending[i] = 1;
+        c.c0[i] = rc_est_of(c.Lc, i, nl, dormant);
+        atomicAdd(&c.npend[1], 1);
+    }
+    if (kind >= 1 && dormant) c.stale[i] |= dormant;
+}
+
+// light wakes from dormancy (cuda/poc_gpu.cu:663-674)
+__global__ void __launch_bounds__(RC_TB) k_wake(rc_dcache c, int Np, int nl, int l, int verify, uint64_t dormant) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= Np) return;
+    if ((c.stale[i] >> l) & 1ull) {
+        if (verify) {
+            c.pending[i] = 1;
+            c.c0[i] = rc_est_of(c.Lc, i, nl, dormant);
+            atomicAdd(&c.npend[1], 1);
+        } else { c.Lc[(int64_t)nl * i + l] = 0.f; c.n[i] = 0.f; c.age[i] = 0.f; }
+        c.stale[i] &= ~(1ull << l);
+    }
+}
+
+// informed upper bound: shrink history by the true reference change (cuda/poc_gpu.cu:676-681)
+__global__ void __launch_bounds__(RC_TB) k_oracle_method(rc_dcache c, int Np, const float* rold, const float* rnew) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= Np) return;
+    float rel = fabsf(rnew[i] - rold[i]) / (rold[i] + 1e-2f);
+    c.n[i] *= 1.f - fminf(fmaxf(4.f * rel, 0.f), 1.f);
+}
+
+// Dependency-targeted invalidation (resolve-time, rtxgi_bench.patch:1028-1050): an entry whose
+// recorded dependencies intersect the event (changed lights, or event voxels in the transit
+// banks) restarts from zero; an entry with no recorded dependency is reset too (unknown ->
+// conservative). Returns the number of invalidated entries in c.npend[1].
+__global__ void __launch_bounds__(RC_TB) k_invalidate(rc_dcache c, int Np, int nl, uint64_t inv_lights,
+                                                     const int* V, int nv, int nw) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int hit = 0;
+    if (i < Np) {
+        uint64_t ld = c.lcur[i] | c.lprev[i];
+        bool known = ld != 0ull;
+        for (int w = 0; w < nw && !known; ++w) known = (c.cur[(int64_t)w * Np + i] | c.prev[(int64_t)w * Np + i]) != 0u;
+        bool t = !known || (ld & inv_lights) != 0ull;
+        for (int j = 0; j < nv && !t; ++j) {
+            int v = V[j];
+            int64_t o = (int64_t)(v >> 5) * Np + i;
+            t = ((c.cur[o] | c.prev[o]) >> (v & 31)) & 1u;
+        }
+        if (t) {
+            c.n[i] = 0.f; c.age[i] = 0.f;
+            float* row = c.Lc + (int64_t)nl * i;
+            for (int l = 0; l < nl; ++l) row[l] = 0.f;
+            hit = 1;
+        }
+    }
+    long long h = rc_wave_sum_i64((long long)hit);
+    if ((threadIdx.x & 63) == 0 && h) atomicAdd(&c.npend[1], (int)h);
+}
+
+// whole-cache flush (event to a new state, rtxgi_bench.patch:363-366)
+__global__ void __launch_bounds__(RC_TB) k_flush(rc_dcache c, int Np, int nl) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= Np) return;
+    c.n[i] = 0.f; c.age[i] = 0.f;
+    float* row = c.Lc + (int64_t)nl * i;
+    for (int l = 0; l < nl; ++l) row[l] = 0.f;
+}
+
+// relative RMS of the estimate against the converged reference (cuda/poc_gpu.cu:683-696)
+__global__ void __launch_bounds__(RC_TB) k_error(rc_dcache c, int Np, int nl, const float* ref, const uint8_t* valid, uint64_t dormant) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double d2 = 0, rs = 0, cnt = 0;
+    if (i < Np && valid[i]) {
+        float e = rc_est_of(c.Lc, i, nl, dormant) - ref[i];
+        d2 = (double)e * e; rs = ref[i]; cnt = 1;
+    }
+    d2 = rc_wave_sum_f64(d2); rs = rc_wave_sum_f64(rs); cnt = rc_wave_sum_f64(cnt);
+    if ((threadIdx.x & 63) == 0) { atomicAdd(&c.err[0], d2); atomicAdd(&c.err[1], rs); atomicAdd(&c.err[2], cnt); }
+}
+
+// One fresh-ray Jacobi iteration Lout = mean_K A (Edir + Lin) of the converged reference
+// (cuda/prep_mesh.cuh:182-213).
+template <int TRAV>
+__global__ void __launch_bounds__(RC_TB) k_ref(rc_dscene s, rc_dstate st, const float* Lin, float* Lout, float* acc,
+                                              float* closec, uint32_t seed, int K, int accumulate, int count_close) {
+    extern __shared__ float sh[];
+    int* shq = (int*)(sh + 9 * st.nt);
+    rc_load_dyn(st, sh, shq);
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= s.Np) return;
+    const int nl = s.nl;
+    uint32_t rs = rc_hash32(seed ^ rc_hash32((uint32_t)i + 1u));
+    f3 P = ld3(st.P, i), N = ld3(st.N, i), O = P + N * 1e-3f;
+    float* out = Lout + (int64_t)nl * i;
+    for (int l = 0; l < nl; ++l) out[l] = 0.f;
+    int close = 0;
+    for (int r = 0; r < K; ++r) {
+        float u1 = rc_rnd(rs), u2 = rc_rnd(rs);
+        f3 D = rc_cosine_dir(N, u1, u2);
+        float t; int q;
+        bool hit = rc_trace_full<TRAV>(O, D, sh, shq, st.nt, s, t, q);
+        if (hit && t < 0.02f) close++;
+        f3 H = hit ? O + D * t : O;
+        int pi;
+        if (rc_resolve_hit(s, st, hit, H, D, q, i, pi)) continue;
+        float A = s.Alb[pi];
+        const float* ed = st.Edir + (int64_t)nl * pi;
+        const float* li = Lin + (int64_t)nl * pi;
+        for (int l = 0; l < nl; ++l) out[l] += A * (ed[l] + li[l]);
+    }
+    for (int l = 0; l < nl; ++l) {
+        float v = out[l] / (float)K;
+        out[l] = v;
+        if (accumulate) acc[(int64_t)nl * i + l] += v;
+    }
+    if (count_close) closec[i] += (float)close;
+}
+
+// ------------------------------------------------------------------------------ camera query
+// Facing normal and entry of a hit (rc_resolve_hit plus the normal the path continues with).
+__device__ __forceinline__ bool rc_resolve_hit_n(const rc_dscene& s, const rc_dstate& st, f3 H, f3 D, int q, int& pi, f3& n) {
+    if (q >= 0) {
+        pi = rc_point_index(H, q, st);
+        n = ld3(st.Q + 12 * q, 3);
+        return dot3(n, D) > 0.f;
+    }
+    const float* n4 = s.STN + 4 * (int64_t)(-2 - q);
+    n = mk3(n4[0], n4[1], n4[2]);
+    if (dot3(n, D) > 0.f) {
+        if (!(__float_as_int(n4[3]) & 1)) { pi = -1; return true; }
+        n = n * -1.f;
+    }
+    pi = rc_hash_point(s, H, n);
+    return pi < 0;
+}
+
+// One-light next-event estimate at H (normal n): light chosen uniformly, weight nl.
+template <int TRAV>
+__device__ __forceinline__ float rc_nee(const rc_dscene& s, const rc_dstate& st, const float* sh, const int* shq,
+                                        f3 H, f3 n, uint32_t& rs, uint64_t dormant, int& shadow_rays) {
+    const int nl = s.nl;
+    float u = rc_rnd(rs);
+    int l = min((int)(u * (float)nl), nl - 1);
+    float I = st.inten[l];
+    if (!(I > 0.f) || rc_is_dormant(dormant, l)) return 0.f;
+    float r; f3 D;
+    if (s.ltype[l] == 1) { D = ld3(s.L, l); r = s.sun_len; }
+    else { f3 dv = ld3(s.L, l) - H; r = sqrtf(dot3(dv, dv)); D = dv * (1.f / r); }
+    float c = dot3(n, D);
+    if (!(c > 0.f)) return 0.f;
+    f3 O = H + n * 1e-3f;
+    float t; int q;
+    rc_trace_full<TRAV>(O, D, sh, shq, st.nt, s, t, q);
+    ++shadow_rays;
+    if (!(t >= r - 1e-3f)) return 0.f;
+    if (s.ltype[l] == 1) return (float)nl * (I * c);
+    float rr = fmaxf(r, 0.05f);
+    return (float)nl * (I * c / (rr * rr));
+}
+
+// Full cached radiance of entry e: A (Edir + Lc) summed over the active light channels, the
+// value a cache-terminated ray reads (cuda/poc_gpu.cu:470-478).
+__device__ __forceinline__ float rc_cached(const rc_dscene& s, const rc_dstate& st, const float* Lc, int e, uint64_t dormant) {
+    const int nl = s.nl;
+    float A = s.Alb[e], val = 0.f;
+    const float* ed = st.Edir + (int64_t)nl * e;
+    const float* lc = Lc + (int64_t)nl * e;
+    for (int l = 0; l < nl; ++l) {
+        float lv = rc_is_dormant(dormant, l) ? 0.f : lc[l];
+        val += A * (ed[l] + lv);
+    }
+    return val;
+}
+
+// Camera pass: per pixel, spp paths. Each path: primary hit x0, one-light NEE at every
+// explicit vertex, cosine bounces; at bounce `cache_bounce` the path ends on the cached
+// radiance of the hit entry (Lc == nullptr: no cache, the path ends after `bounces` bounces).
+template <int TRAV>
+__global__ void __launch_bounds__(RC_TB) k_query(rc_dscene s, rc_dstate st, const float* Lc, rc_camera cam, uint32_t seed,
+                                                uint32_t frame, uint64_t dormant, float* img, unsigned long long* counters) {
+    extern __shared__ float sh[];
+    int* shq = (int*)(sh + 9 * st.nt);
+    rc_load_dyn(st, sh, shq);
+    int pix = blockIdx.x * blockDim.x + threadIdx.x;
+    int ext_rays = 0, sh_rays = 0;
+    if (pix < cam.W * cam.H) {
+        int px = pix % cam.W, py = pix / cam.W;
+        f3 pos = mk3(cam.pos[0], cam.pos[1], cam.pos[2]);
+        f3 fw = mk3(cam.fwd[0], cam.fwd[1], cam.fwd[2]);
+        f3 rt = mk3(cam.right[0], cam.right[1], cam.right[2]);
+        f3 up = mk3(cam.up[0], cam.up[1], cam.up[2]);
+        float sum = 0.f;
+        for (int sp = 0; sp < cam.spp; ++sp) {
+            uint32_t rs = rc_seed(seed, frame, (uint32_t)(pix * cam.spp + sp));
+            float ju = rc_rnd(rs), jv = rc_rnd(rs);
+            float sx = (2.f * ((float)px + ju) / (float)cam.W - 1.f) * cam.tanx;
+            float sy = (1.f - 2.f * ((float)py + jv) / (float)cam.H) * cam.tany;
+            f3 d = fw + rt * sx + up * sy;
+            d = d * (1.f / sqrtf(dot3(d, d)));
+            f3 o = pos;
+            float thr = 1.f, rad = 0.f;
+            for (int b = 0; ; ++b) {
+                float t; int q;
+                bool hit = rc_trace_full<TRAV>(o, d, sh, shq, st.nt, s, t, q);
+                ++ext_rays;
+                if (!hit) break;
+                f3 H = o + d * t, n;
+                int e;
+                if (rc_resolve_hit_n(s, st, H, d, q, e, n)) break;
+                if (Lc && b == cam.cache_bounce) { rad += thr * rc_cached(s, st, Lc, e, dormant); break; }
+                float A = s.Alb[e];
+                rad += thr * (A * rc_nee<TRAV>(s, st, sh, shq, H, n, rs, dormant, sh_rays));
+                if (b >= cam.bounces) break;
+                thr = thr * A;
+                float u1 = rc_rnd(rs), u2 = rc_rnd(rs);
+                d = rc_cosine_dir(n, u1, u2);
+                o = H + n * 1e-3f;
+            }
+            sum += rad;
+        }
+        img[pix] = sum / (float)cam.spp;
+    }
+    if (counters) {
+        long long a = rc_wave_sum_i64(ext_rays), b2 = rc_wave_sum_i64(sh_rays);
+        if ((threadIdx.x & 63) == 0) {
+            atomicAdd(&counters[0], (unsigned long long)a);
+            atomicAdd(&counters[1], (unsigned long long)b2);
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------ hash grid insert
+// Insert keys with 64-bit CAS and linear probing (same home slot mix64(key) & hmask and probe
+// order as the lookup, cuda/poc_gpu.cu:263-272). Lanes of a wave holding the same key elect
+// the lowest lane (ballot over the match mask) so each distinct key probes once per wave; the
+// value stored for a key is the minimum inserted (atomicMin), independent of arrival order.
+// fail[0] counts keys that found no slot within 64 probes.
+__global__ void __launch_bounds__(RC_TB) k_hash_insert(unsigned long long* HK, int* HV, uint32_t hmask,
+                                                      const unsigned long long* keys, const int* vals, int n, int* fail) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    bool active = i < n;
+    unsigned long long key = active ? keys[i] : 0ull;
+    int val = active ? vals[i] : 0x7fffffff;
+    active = active && key != 0ull;
+    // wave-level dedup: the lowest lane with a given key inserts the min value of its group
+    int lane = threadIdx.x & 63;
+    bool leader = active;
+    int best = val;
+    for (int o = 0; o < 64; ++o) {
+        unsigned long long ko = __shfl(key, o, 64);
+        int vo = __shfl(val, o, 64);
+        bool ao = __shfl((int)active, o, 64) != 0;
+        if (active && ao && ko == key) {
+            if (o < lane) leader = false;
+            best = min(best, vo);
+        }
+    }
+    if (!leader) return;
+    uint32_t h = (uint32_t)rc_mix64(key) & hmask;
+    for (int probe = 0; probe < 64; ++probe) {
+        unsigned long long prev = atomicCAS(HK + h, 0ull, key);
+        if (prev == 0ull || prev == key) { atomicMin(HV + h, best); return; }
+        h = (h + 1) & hmask;
+    }
+    atomicAdd(fail, 1);
+}
+
+__global__ void __launch_bounds__(RC_TB) k_hash_query(const unsigned long long* HK, const int* HV, uint32_t hmask,
+                                                     const unsigned long long* keys, int n, int* out) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    out[i] = rc_hash_find((const uint64_t*)HK, HV, hmask, (uint64_t)keys[i]);
+}
+
+// cache entry of hit points (position + facing normal), the lookup of the cache-read path
+__global__ void __launch_bounds__(RC_TB) k_hash_point(rc_dscene s, const float* H, const float* Nn, int n, int* out) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    out[i] = rc_hash_point(s, ld3(H, i), ld3(Nn, i));
+}
+
+// ------------------------------------------------------------------------------ DDA kernels
+// Transit-grid masks of segments (a_i -> b_i), one thread per segment.
+__global__ void __launch_bounds__(RC_TB) k_dda_segments(rc_dscene s, const float* A, const float* B, int n, uint32_t* out) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t m[RC_MAX_NW] = {0u, 0u, 0u, 0u};
+    rc_dda_reg(s, ld3(A, i), ld3(B, i), m);
+    for (int w = 0; w < s.nw; ++w) out[(int64_t)s.nw * i + w] = m[w];
+}
+
+// Fine voxel volume: Amanatides-Woo over a gx*gy*gz bit volume (stepping rule of the transit
+// grid, cuda/poc_gpu.cu:171-190), OR-ing each crossed voxel's bit with a read-before-atomic
+// so a voxel already set costs one load. vol has ceil(gx*gy*gz/32) words per volume; seg_vol
+// selects the volume of each segment (e.g. its light). Steps are bounded by gx+gy+gz.
+__global__ void __launch_bounds__(RC_TB) k_dda_volume(rc_dscene s, int gx, int gy, int gz, const float* A, const float* B,
+                                                     const int* seg_vol, int n, uint32_t* vol, long long words_per_vol) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    f3 a = ld3(A, i), b = ld3(B, i);
+    uint32_t* V = vol + (seg_vol ? (int64_t)seg_vol[i] * words_per_vol : 0);
+    const float sx = gx / s.ext[0], sy = gy / s.ext[1], sz = gz / s.ext[2];
+    float ax = (a.x - s.lo[0]) * sx, ay = (a.y - s.lo[1]) * sy, az = (a.z - s.lo[2]) * sz;
+    float bx = (b.x - s.lo[0]) * sx, by = (b.y - s.lo[1]) * sy, bz = (b.z - s.lo[2]) * sz;
+    int ix = min(max((int)ax, 0), gx - 1), iy = min(max((int)ay, 0), gy - 1), iz = min(max((int)az, 0), gz - 1);
+    int ex = min(max((int)bx, 0), gx - 1), ey = min(max((int)by, 0), gy - 1), ez = min(max((int)bz, 0), gz - 1);
+    float dx = bx - ax, dy = by - ay, dz = bz - az;
+    int stx = dx > 0 ? 1 : -1, sty = dy > 0 ? 1 : -1, stz = dz > 0 ? 1 : -1;
+    float tdx = dx != 0 ? fabsf(1.f / dx) : 1e30f, tdy = dy != 0 ? fabsf(1.f / dy) : 1e30f, tdz = dz != 0 ? fabsf(1.f / dz) : 1e30f;
+    float tmx = dx != 0 ? ((dx > 0 ? ((float)(ix + 1) - ax) : (ax - (float)ix)) * tdx) : 1e30f;
+    float tmy = dy != 0 ? ((dy > 0 ? ((float)(iy + 1) - ay) : (ay - (float)iy)) * tdy) : 1e30f;
+    float tmz = dz != 0 ? ((dz > 0 ? ((float)(iz + 1) - az) : (az - (float)iz)) * tdz) : 1e30f;
+    for (int g = 0; g < gx + gy + gz; ++g) {
+        int64_t v = ((int64_t)iz * gy + iy) * gx + ix;
+        uint32_t bit = 1u << (v & 31);
+        uint32_t* w = V + (v >> 5);
+        if (!(*w & bit)) atomicOr(w, bit);
+        if (ix == ex && iy == ey && iz == ez) break;
+        if (tmx < tmy && tmx < tmz) { ix += stx; tmx += tdx; if (ix < 0 || ix >= gx) break; }
+        else if (tmy < tmz) { iy += sty; tmy += tdy; if (iy < 0 || iy >= gy) break; }
+        else { iz += stz; tmz += tdz; if (iz < 0 || iz >= gz) break; }
+    }
+}
+
+// ------------------------------------------------------------------------------ launchers
+static inline int rc_grid(long long n) { return (int)((n + RC_TB - 1) / RC_TB); }
+static inline size_t rc_shm(const rc_dstate& st) { return (size_t)st.nt * 10 * sizeof(float); }
+
+hipError_t rc_launch_trace_rays(int trav, const rc_dscene* s, const rc_dstate* st, const float* O, const float* D, int n,
+                                float* t, int* q, hipStream_t stream) {
+    if (n <= 0) return hipSuccess;
+    dim3 g(rc_grid(n)), b(RC_TB);
+    size_t shm = rc_shm(*st);
+    if (trav == 0) hipLaunchKernelGGL(k_trace_rays<0>, g, b, shm, stream, *s, *st, O, D, n, t, q);
+    else if (trav == 1) hipLaunchKernelGGL(k_trace_rays<1>, g, b, shm, stream, *s, *st, O, D, n, t, q);
+    else hipLaunchKernelGGL(k_trace_rays<2>, g, b, shm, stream, *s, *st, O, D, n, t, q);
+    return hipGetLastError();
+}
+
+hipError_t rc_launch_build(int trav, const rc_dscene* s, const rc_dstate* st, hipStream_t stream) {
+    if (s->Np <= 0) return hipSuccess;
+    dim3 g(rc_grid(s->Np)), b(RC_TB);
+    size_t shm = rc_shm(*st);
+    if (trav == 0) hipLaunchKernelGGL(k_build<0>, g, b, shm, stream, *s, *st);
+    else if (trav == 1) hipLaunchKernelGGL(k_build<1>, g, b, shm, stream, *s, *st);
+    else hipLaunchKernelGGL(k_build<2>, g, b, shm, stream, *s, *st);
+    return hipGetLastError();
+}
+
+hipError_t rc_launch_trace2(int trav, int deps, const rc_dscene* s, const rc_dstate* st, const rc_dcache* c, const rc_params* p,
+                            uint64_t* lnew, hipStream_t stream) {
+    if (p->Np <= 0) return hipSuccess;
+    dim3 g(rc_grid(p->Np)), b(RC_TB);
+    size_t shm = rc_shm(*st);
+#define RC_T2(D_, T_) hipLaunchKernelGGL((k_trace2<D_, T_>), g, b, shm, stream, *s, *st, *c, *p, lnew)
+    if (deps) { if (trav == 0) RC_T2(true, 0); else if (trav == 1) RC_T2(true, 1); else RC_T2(true, 2); }
+    else { if (trav == 0) RC_T2(false, 0); else if (trav == 1) RC_T2(false, 1); else RC_T2(false, 2); }
+#undef RC_T2
+    return hipGetLastError();
+}
+
+hipError_t rc_launch_decide(const rc_dscene* s, const rc_dcache* c, const rc_params* p, hipStream_t stream) {
+    hipLaunchKernelGGL(k_decide, dim3(rc_grid(p->Np)), dim3(RC_TB), 0, stream, *s, *c, *p);
+    return hipGetLastError();
+}
+hipError_t rc_launch_update(const rc_dscene* s, const rc_dcache* c, const rc_params* p, const uint64_t* lnew, hipStream_t stream) {
+    hipLaunchKernelGGL(k_update, dim3(rc_grid(p->Np)), dim3(RC_TB), 0, stream, *s, *c, *p, lnew);
+    return hipGetLastError();
+}
+hipError_t rc_launch_sumpr(const rc_dcache* c, int Np, hipStream_t stream) {
+    hipLaunchKernelGGL(k_sumpr, dim3(rc_grid(Np)), dim3(RC_TB), 0, stream, *c, Np);
+    return hipGetLastError();
+}
+hipError_t rc_launch_restored(const rc_dcache* c, int Np, int nl, float nmax, uint64_t dormant, int verify, hipStream_t stream) {
+    hipLaunchKernelGGL(k_restored, dim3(rc_grid(Np)), dim3(RC_TB), 0, stream, *c, Np, nl, nmax, dormant, verify);
+    return hipGetLastError();
+}
+hipError_t rc_launch_rescale(const rc_dcache* c, int Np, int nl, int l, float ratio, hipStream_t stream) {
+    hipLaunchKernelGGL(k_rescale, dim3(rc_grid(Np)), dim3(RC_TB), 0, stream, *c, Np, nl, l, ratio);
+    return hipGetLastError();
+}
+hipError_t rc_launch_touched(const rc_dcache* c, int Np, int nl, const int* V, int nv, int kind, uint64_t dormant, hipStream_t stream) {
+    hipLaunchKernelGGL(k_touched, dim3(rc_grid(Np)), dim3(RC_TB), 0, stream, *c, Np, nl, V, nv, kind, dormant);
+    return hipGetLastError();
+}
+hipError_t rc_launch_wake(const rc_dcache* c, int Np, int nl, int l, int verify, uint64_t dormant, hipStream_t stream) {
+    hipLaunchKernelGGL(k_wake, dim3(rc_grid(Np)), dim3(RC_TB), 0, stream, *c, Np, nl, l, verify, dormant);
+    return hipGetLastError();
+}
+hipError_t rc_launch_oracle_method(const rc_dcache* c, int Np, const float* rold, const float* rnew, hipStream_t stream) {
+    hipLaunchKernelGGL(k_oracle_method, dim3(rc_grid(Np)), dim3(RC_TB), 0, stream, *c, Np, rold, rnew);
+    return hipGetLastError();
+}
+hipError_t rc_launch_invalidate(const rc_dcache* c, int Np, int nl, uint64_t inv_lights, const int* V, int nv, int nw, hipStream_t stream) {
+    hipLaunchKernelGGL(k_invalidate, dim3(rc_grid(Np)), dim3(RC_TB), 0, stream, *c, Np, nl, inv_lights, V, nv, nw);
+    return hipGetLastError();
+}
+hipError_t rc_launch_flush(const rc_dcache* c, int Np, int nl, hipStream_t stream) {
+    hipLaunchKernelGGL(k_flush, dim3(rc_grid(Np)), dim3(RC_TB), 0, stream, *c, Np, nl);
+    return hipGetLastError();
+}
+hipError_t rc_launch_error(const rc_dcache* c, int Np, int nl, const float* ref, const uint8_t* valid, uint64_t dormant, hipStream_t stream) {
+    hipLaunchKernelGGL(k_error, dim3(rc_grid(Np)), dim3(RC_TB), 0, stream, *c, Np, nl, ref, valid, dormant);
+    return hipGetLastError();
+}
+hipError_t rc_launch_ref(int trav, const rc_dscene* s, const rc_dstate* st, const float* Lin, float* Lout, float* acc, float* closec,
+                         uint32_t seed, int K, int accumulate, int count_close, hipStream_t stream) {
+    dim3 g(rc_grid(s->Np)), b(RC_TB);
+    size_t shm = rc_shm(*st);
+    if (trav == 0) hipLaunchKernelGGL(k_ref<0>, g, b, shm, stream, *s, *st, Lin, Lout, acc, closec, seed, K, accumulate, count_close);
+    else if (trav == 1) hipLaunchKernelGGL(k_ref<1>, g, b, shm, stream, *s, *st, Lin, Lout, acc, closec, seed, K, accumulate, count_close);
+    else hipLaunchKernelGGL(k_ref<2>, g, b, shm, stream, *s, *st, Lin, Lout, acc, closec, seed, K, accumulate, count_close);
+    return hipGetLastError();
+}
+hipError_t rc_launch_query(int trav, const rc_dscene* s, const rc_dstate* st, const float* Lc, const rc_camera* cam, uint32_t seed,
+                           uint32_t frame, uint64_t dormant, float* img, unsigned long long* counters, hipStream_t stream) {
+    dim3 g(rc_grid((long long)cam->W * cam->H)), b(RC_TB);
+    size_t shm = rc_shm(*st);
+    if (trav == 0) hipLaunchKernelGGL(k_query<0>, g, b, shm, stream, *s, *st, Lc, *cam, seed, frame, dormant, img, counters);
+    else if (trav == 1) hipLaunchKernelGGL(k_query<1>, g, b, shm, stream, *s, *st, Lc, *cam, seed, frame, dormant, img, counters);
+    else hipLaunchKernelGGL(k_query<2>, g, b, shm, stream, *s, *st, Lc, *cam, seed, frame, dormant, img, counters);
+    return hipGetLastError();
+}
+hipError_t rc_launch_hash_insert(unsigned long long* HK, int* HV, uint32_t hmask, const unsigned long long* keys, const int* vals,
+                                 int n, int* fail, hipStream_t stream) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_hash_insert, dim3(rc_grid(n)), dim3(RC_TB), 0, stream, HK, HV, hmask, keys, vals, n, fail);
+    return hipGetLastError();
+}
+hipError_t rc_launch_hash_query(const unsigned long long* HK, const int* HV, uint32_t hmask, const unsigned long long* keys, int n,
+                                int* out, hipStream_t stream) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_hash_query, dim3(rc_grid(n)), dim3(RC_TB), 0, stream, HK, HV, hmask, keys, n, out);
+    return hipGetLastError();
+}
+hipError_t rc_launch_hash_point(const rc_dscene* s, const float* H, const float* Nn, int n, int* out, hipStream_t stream) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_hash_point, dim3(rc_grid(n)), dim3(RC_TB), 0, stream, *s, H, Nn, n, out);
+    return hipGetLastError();
+}
+hipError_t rc_launch_dda_segments(const rc_dscene* s, const float* A, const float* B, int n, uint32_t* out, hipStream_t stream) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_dda_segments, dim3(rc_grid(n)), dim3(RC_TB), 0, stream, *s, A, B, n, out);
+    return hipGetLastError();
+}
+hipError_t rc_launch_dda_volume(const rc_dscene* s, int gx, int gy, int gz, const float* A, const float* B, const int* seg_vol, int n,
+                                uint32_t* vol, long long words_per_vol, hipStream_t stream) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_dda_volume, dim3(rc_grid(n)), dim3(RC_TB), 0, stream, *s, gx, gy, gz, A, B, seg_vol, n, vol, words_per_vol);
+    return hipGetLastError();
+}

@@ -1,0 +1,609 @@
+// Host-side scene preparation (see rc_scene.h).
+#include "rc_scene.h"
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <unordered_map>
+
+namespace rc {
+
+uint64_t mix64(uint64_t x) {
+    x ^= x >> 30; x *= 0xbf58476d1ce4e5b9ull; x ^= x >> 27; x *= 0x94d049bb133111ebull; x ^= x >> 31; return x;
+}
+uint64_t cell_key(int cx, int cy, int cz, int b) {
+    return ((((uint64_t)(cx & 0xFFFFF) << 20 | (uint64_t)(cy & 0xFFFFF)) << 20 | (uint64_t)(cz & 0xFFFFF)) << 3 | (uint64_t)b) + 1ull;
+}
+int normal_bucket(float nx, float ny, float nz) {
+    float ax = std::fabs(nx), ay = std::fabs(ny), az = std::fabs(nz);
+    if (ax >= ay && ax >= az) return nx > 0.f ? 0 : 1;
+    if (ay >= az) return ny > 0.f ? 2 : 3;
+    return nz > 0.f ? 4 : 5;
+}
+void decode_key(uint64_t k, int& cx, int& cy, int& cz, int& b) {
+    k -= 1ull;
+    b = (int)(k & 7ull); cz = (int)((k >> 3) & 0xFFFFFull); cy = (int)((k >> 23) & 0xFFFFFull); cx = (int)((k >> 43) & 0xFFFFFull);
+}
+uint64_t spread3(uint64_t x) {   // 21 bits -> every third bit (Morton interleave)
+    x &= 0x1fffffull;
+    x = (x | x << 32) & 0x1f00000000ffffull;
+    x = (x | x << 16) & 0x1f0000ff0000ffull;
+    x = (x | x << 8) & 0x100f00f00f00f00full;
+    x = (x | x << 4) & 0x10c30c30c30c30c3ull;
+    x = (x | x << 2) & 0x1249249249249249ull;
+    return x;
+}
+
+void Mesh::add_tri(const float* a, const float* b, const float* c, float albedo, bool dsided) {
+    V.insert(V.end(), a, a + 3); V.insert(V.end(), b, b + 3); V.insert(V.end(), c, c + 3);
+    alb.push_back(albedo); ds.push_back(dsided ? 1 : 0);
+}
+void Mesh::add_quad_grid(const float o[3], const float u[3], const float v[3], int nu, int nv, float albedo, bool dsided) {
+    for (int j = 0; j < nv; ++j)
+        for (int i = 0; i < nu; ++i) {
+            float p00[3], p10[3], p11[3], p01[3];
+            for (int a = 0; a < 3; ++a) {
+                float fu0 = u[a] * ((float)i / nu), fu1 = u[a] * ((float)(i + 1) / nu);
+                float fv0 = v[a] * ((float)j / nv), fv1 = v[a] * ((float)(j + 1) / nv);
+                p00[a] = o[a] + fu0 + fv0; p10[a] = o[a] + fu1 + fv0; p11[a] = o[a] + fu1 + fv1; p01[a] = o[a] + fu0 + fv1;
+            }
+            add_tri(p00, p10, p11, albedo, dsided);
+            add_tri(p00, p11, p01, albedo, dsided);
+        }
+}
+
+// ------------------------------------------------------------------------------------ BVH
+static float area_of(const float* lo, const float* hi) {
+    float d0 = hi[0] - lo[0], d1 = hi[1] - lo[1], d2 = hi[2] - lo[2];
+    return d0 < 0 ? 0.f : 2.f * (d0 * d1 + d1 * d2 + d2 * d0);
+}
+
+void build_bvh(const Mesh& M, std::vector<rc_bvh_node>& nodes, std::vector<int>& order, int& depth_out) {
+    const int n = M.nt();
+    order.resize(n);
+    std::iota(order.begin(), order.end(), 0);
+    std::vector<float> cen(3 * (size_t)n), tlo(3 * (size_t)n), thi(3 * (size_t)n);
+    for (int t = 0; t < n; ++t)
+        for (int a = 0; a < 3; ++a) {
+            const float* v = &M.V[9 * (size_t)t];
+            float x0 = v[a], x1 = v[3 + a], x2 = v[6 + a];
+            float mn = std::min(x0, std::min(x1, x2)), mx = std::max(x0, std::max(x1, x2));
+            tlo[3 * (size_t)t + a] = mn; thi[3 * (size_t)t + a] = mx;
+            cen[3 * (size_t)t + a] = 0.5f * (mn + mx);
+        }
+    nodes.clear();
+    nodes.reserve(2 * (size_t)n + 1);
+    nodes.push_back(rc_bvh_node{});
+    struct Item { int node, start, count, depth; };
+    std::vector<Item> work{{0, 0, n, 0}};
+    const int NB = 16;
+    depth_out = 0;
+    while (!work.empty()) {
+        Item it = work.back(); work.pop_back();
+        depth_out = std::max(depth_out, it.depth);
+        float nlo[3] = {1e30f, 1e30f, 1e30f}, nhi[3] = {-1e30f, -1e30f, -1e30f};
+        float clo[3] = {1e30f, 1e30f, 1e30f}, chi[3] = {-1e30f, -1e30f, -1e30f};
+        for (int k = it.start; k < it.start + it.count; ++k) {
+            int t = order[k];
+            for (int a = 0; a < 3; ++a) {
+                nlo[a] = std::min(nlo[a], tlo[3 * (size_t)t + a]); nhi[a] = std::max(nhi[a], thi[3 * (size_t)t + a]);
+                clo[a] = std::min(clo[a], cen[3 * (size_t)t + a]); chi[a] = std::max(chi[a], cen[3 * (size_t)t + a]);
+            }
+        }
+        rc_bvh_node& nd = nodes[it.node];
+        for (int a = 0; a < 3; ++a) { nd.lo[a] = nlo[a]; nd.hi[a] = nhi[a]; }
+        int ax = 0;
+        for (int a = 1; a < 3; ++a) if (chi[a] - clo[a] > chi[ax] - clo[ax]) ax = a;
+        const float ext = chi[ax] - clo[ax];
+        auto make_leaf = [&]() { nodes[it.node].lf = it.start; nodes[it.node].cnt = it.count; };
+        if ((it.count <= 4 || it.depth >= 56 || ext < 1e-7f) && (it.count <= 64 || it.depth >= 56)) { make_leaf(); continue; }
+        int mid = -1;
+        if (ext >= 1e-7f) {
+            int bc[NB] = {0};
+            float blo[NB][3], bhi[NB][3];
+            for (int b = 0; b < NB; ++b) for (int a = 0; a < 3; ++a) { blo[b][a] = 1e30f; bhi[b][a] = -1e30f; }
+            auto bin_of = [&](int t) { return std::min(NB - 1, (int)((cen[3 * (size_t)t + ax] - clo[ax]) / ext * NB)); };
+            for (int k = it.start; k < it.start + it.count; ++k) {
+                int t = order[k], b = bin_of(t);
+                bc[b]++;
+                for (int a = 0; a < 3; ++a) { blo[b][a] = std::min(blo[b][a], tlo[3 * (size_t)t + a]); bhi[b][a] = std::max(bhi[b][a], thi[3 * (size_t)t + a]); }
+            }
+            // prefix / suffix sweeps over the bins
+            float plo[NB][3], phi[NB][3], slo[NB][3], shi[NB][3];
+            int pc[NB], sc[NB];
+            for (int b = 0; b < NB; ++b)
+                for (int a = 0; a < 3; ++a) {
+                    plo[b][a] = std::min(b ? plo[b - 1][a] : 1e30f, blo[b][a]); phi[b][a] = std::max(b ? phi[b - 1][a] : -1e30f, bhi[b][a]);
+                }
+            for (int b = 0; b < NB; ++b) pc[b] = (b ? pc[b - 1] : 0) + bc[b];
+            for (int b = NB - 1; b >= 0; --b) {
+                for (int a = 0; a < 3; ++a) {
+                    slo[b][a] = std::min(b < NB - 1 ? slo[b + 1][a] : 1e30f, blo[b][a]); shi[b][a] = std::max(b < NB - 1 ? shi[b + 1][a] : -1e30f, bhi[b][a]);
+                }
+                sc[b] = (b < NB - 1 ? sc[b + 1] : 0) + bc[b];
+            }
+            float best = 1e30f; int bs = -1;
+            for (int s = 1; s < NB; ++s) {
+                int nl = pc[s - 1], nr = sc[s];
+                if (!nl || !nr) continue;
+                float c = nl * area_of(plo[s - 1], phi[s - 1]) + nr * area_of(slo[s], shi[s]);
+                if (c < best) { best = c; bs = s; }
+            }
+            float leaf_cost = it.count * area_of(nlo, nhi);
+            if (bs > 0 && !(it.count <= 4 && best >= leaf_cost)) {
+                auto pm = std::partition(order.begin() + it.start, order.begin() + it.start + it.count, [&](int t) { return bin_of(t) < bs; });
+                mid = (int)(pm - order.begin());
+            } else if (it.count <= 4) { make_leaf(); continue; }
+        }
+        if (mid <= it.start || mid >= it.start + it.count) {
+            mid = it.start + it.count / 2;
+            std::nth_element(order.begin() + it.start, order.begin() + mid, order.begin() + it.start + it.count,
+                             [&](int a, int b) { return cen[3 * (size_t)a + ax] < cen[3 * (size_t)b + ax]; });
+        }
+        int left = (int)nodes.size();
+        nodes.push_back(rc_bvh_node{});
+        nodes.push_back(rc_bvh_node{});
+        nodes[it.node].lf = left; nodes[it.node].cnt = 0;
+        work.push_back({left, it.start, mid - it.start, it.depth + 1});
+        work.push_back({left + 1, mid, it.start + it.count - mid, it.depth + 1});
+    }
+}
+
+void build_wide(const std::vector<rc_bvh_node>& bn, std::vector<rc_wnode>& wn, int& depth_out) {
+    wn.clear();
+    depth_out = 0;
+    struct Job { int bnode, wslot, depth; };
+    std::vector<Job> jobs;
+    wn.push_back(rc_wnode{});
+    jobs.push_back({0, 0, 1});
+    while (!jobs.empty()) {
+        Job jb = jobs.back(); jobs.pop_back();
+        depth_out = std::max(depth_out, jb.depth);
+        std::vector<int> kids;
+        if (bn[jb.bnode].cnt > 0) kids.push_back(jb.bnode);
+        else { kids.push_back(bn[jb.bnode].lf); kids.push_back(bn[jb.bnode].lf + 1); }
+        while (kids.size() < 4) {   // open the inner child with the largest surface area
+            int best = -1; float ba = -1.f;
+            for (size_t k = 0; k < kids.size(); ++k) {
+                const rc_bvh_node& c = bn[kids[k]];
+                if (c.cnt > 0) continue;
+                float a = area_of(c.lo, c.hi);
+                if (a > ba) { ba = a; best = (int)k; }
+            }
+            if (best < 0) break;
+            int b = kids[best];
+            kids[best] = bn[b].lf;
+            kids.insert(kids.begin() + best + 1, bn[b].lf + 1);
+        }
+        rc_wnode w;
+        for (int c = 0; c < 4; ++c) {
+            w.lox[c] = w.loy[c] = w.loz[c] = 1e30f;
+            w.hix[c] = w.hiy[c] = w.hiz[c] = -1e30f;
+            w.child[c] = -1; w.ccount[c] = 0;
+        }
+        std::vector<Job> sub;
+        for (size_t k = 0; k < kids.size(); ++k) {
+            const rc_bvh_node& c = bn[kids[k]];
+            w.lox[k] = c.lo[0]; w.loy[k] = c.lo[1]; w.loz[k] = c.lo[2];
+            w.hix[k] = c.hi[0]; w.hiy[k] = c.hi[1]; w.hiz[k] = c.hi[2];
+            if (c.cnt > 0) { w.child[k] = -2 - c.lf; w.ccount[k] = c.cnt; }
+            else {
+                int slot = (int)wn.size();
+                wn.push_back(rc_wnode{});
+                w.child[k] = slot;
+                sub.push_back({kids[k], slot, jb.depth + 1});
+            }
+        }
+        wn[jb.wslot] = w;
+        for (auto& s : sub) jobs.push_back(s);
+    }
+}
+
+// ------------------------------------------------------------------------------------ cells
+void sample_cells(const Mesh& M, float h, const float lo[3], std::vector<uint64_t>& keys_out, std::vector<CellRec>& recs_out) {
+    std::unordered_map<uint64_t, CellRec> cells;
+    cells.reserve((size_t)M.nt() * 4);
+    for (int t = 0; t < M.nt(); ++t) {
+        const float* v = &M.V[9 * (size_t)t];
+        float e1[3] = {v[3] - v[0], v[4] - v[1], v[5] - v[2]}, e2[3] = {v[6] - v[0], v[7] - v[1], v[8] - v[2]};
+        float cr[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
+        float l = std::sqrt(cr[0] * cr[0] + cr[1] * cr[1] + cr[2] * cr[2]);
+        if (l < 1e-12f) continue;
+        float ar = 0.5f * l, n[3] = {cr[0] / l, cr[1] / l, cr[2] / l};
+        int ns = (int)std::min(1e6, std::max(1.0, std::ceil(ar / (h * h) * 4.0)));
+        HostRng r((uint64_t)t);
+        int sides = M.ds[t] ? 2 : 1;
+        for (int k = 0; k < ns; ++k) {
+            float a = 1.f / 3.f, b = 1.f / 3.f;
+            if (k > 0) { a = r.u(); b = r.u(); if (a + b > 1.f) { a = 1.f - a; b = 1.f - b; } }
+            float P[3] = {v[0] + e1[0] * a + e2[0] * b, v[1] + e1[1] * a + e2[1] * b, v[2] + e1[2] * a + e2[2] * b};
+            int c[3]; float d2 = 0.f;
+            for (int x = 0; x < 3; ++x) {
+                c[x] = (int)std::floor((P[x] - lo[x]) / h);
+                float cc = lo[x] + ((float)c[x] + 0.5f) * h;
+                d2 += (P[x] - cc) * (P[x] - cc);
+            }
+            for (int sd = 0; sd < sides; ++sd) {
+                float nn[3] = {sd ? -n[0] : n[0], sd ? -n[1] : n[1], sd ? -n[2] : n[2]};
+                uint64_t key = cell_key(c[0], c[1], c[2], normal_bucket(nn[0], nn[1], nn[2]));
+                auto ins = cells.try_emplace(key, CellRec{{0, 0, 0}, 1e30f, {0, 0, 1}, 0.0, 0.0});
+                CellRec& rec = ins.first->second;
+                if (d2 < rec.d2) { rec.d2 = d2; for (int x = 0; x < 3; ++x) { rec.p[x] = P[x]; rec.n[x] = nn[x]; } }
+                rec.asum += (double)M.alb[t] * ar / ns;
+                rec.wsum += (double)ar / ns;
+            }
+        }
+    }
+    std::vector<std::pair<uint64_t, uint64_t>> mk;
+    mk.reserve(cells.size());
+    for (auto& kv : cells) {
+        int cx, cy, cz, b;
+        decode_key(kv.first, cx, cy, cz, b);
+        mk.push_back({(spread3((uint64_t)cx) | spread3((uint64_t)cy) << 1 | spread3((uint64_t)cz) << 2) * 8 + (uint64_t)b, kv.first});
+    }
+    std::sort(mk.begin(), mk.end());
+    keys_out.resize(mk.size());
+    recs_out.resize(mk.size());
+    for (size_t i = 0; i < mk.size(); ++i) { keys_out[i] = mk[i].second; recs_out[i] = cells[mk[i].second]; }
+}
+
+void build_hash(const std::vector<uint64_t>& keys, int log2, std::vector<uint64_t>& HK, std::vector<int>& HV) {
+    size_t HS = (size_t)1 << log2;
+    HK.assign(HS, 0ull);
+    HV.assign(HS, -1);
+    for (size_t i = 0; i < keys.size(); ++i) {
+        size_t hh = (size_t)mix64(keys[i]) & (HS - 1);
+        while (HK[hh] != 0ull) hh = (hh + 1) & (HS - 1);
+        HK[hh] = keys[i];
+        HV[hh] = (int)i;
+    }
+}
+
+// ------------------------------------------------------------------------------------ prepare
+static void box_faces(const float c[3], const float e[3], float F[6][12]) {
+    // o, u, v, n per face; u x v points along the outward normal n
+    const float X = 2 * e[0], Y = 2 * e[1], Z = 2 * e[2];
+    const float G[6][12] = {
+        {c[0] + e[0], c[1] - e[1], c[2] - e[2], 0, Y, 0, 0, 0, Z, 1, 0, 0},
+        {c[0] - e[0], c[1] - e[1], c[2] - e[2], 0, 0, Z, 0, Y, 0, -1, 0, 0},
+        {c[0] - e[0], c[1] + e[1], c[2] - e[2], 0, 0, Z, X, 0, 0, 0, 1, 0},
+        {c[0] - e[0], c[1] - e[1], c[2] - e[2], X, 0, 0, 0, 0, Z, 0, -1, 0},
+        {c[0] - e[0], c[1] - e[1], c[2] + e[2], X, 0, 0, 0, Y, 0, 0, 0, 1},
+        {c[0] - e[0], c[1] - e[1], c[2] - e[2], 0, Y, 0, X, 0, 0, 0, 0, -1}};
+    memcpy(F, G, sizeof G);
+}
+
+static int host_vox(const Scene& S, const float* p) {
+    int i = std::min(std::max((int)((p[0] - S.lo[0]) / S.ext[0] * S.vx), 0), S.vx - 1);
+    int j = std::min(std::max((int)((p[1] - S.lo[1]) / S.ext[1] * S.vy), 0), S.vy - 1);
+    int k = std::min(std::max((int)((p[2] - S.lo[2]) / S.ext[2] * S.vz), 0), S.vz - 1);
+    return (k * S.vy + j) * S.vx + i;
+}
+
+std::string prepare(const SceneDesc& d, Scene& S) {
+    const Mesh& M = d.mesh;
+    if (M.nt() <= 0) return "scene has no static triangles";
+    if (d.nl <= 0 || d.nl > RC_MAX_LIGHTS) return "light count out of range (1..72)";
+    if (d.inten.empty()) return "scene has no states";
+    for (auto& v : d.inten) if ((int)v.size() != d.nl) return "state intensity count differs from the light count";
+    if (d.vx * d.vy * d.vz % 32 != 0 || d.vx * d.vy * d.vz > 32 * RC_MAX_NW) return "transit grid must hold 32..128 bits in whole words";
+    S = Scene();
+    S.mesh = d.mesh;
+    S.vx = d.vx; S.vy = d.vy; S.vz = d.vz; S.nw = d.vx * d.vy * d.vz / 32;
+    S.nl = d.nl; S.L = d.L; S.ltype = d.ltype; S.cam = d.cam;
+    S.NS = (int)d.inten.size();
+    S.inten = d.inten;
+    // bounds over static vertices and present dynamic boxes, padded (cuda/prep_mesh.cuh:306-310)
+    float hi[3];
+    for (int a = 0; a < 3; ++a) { S.lo[a] = 1e30f; hi[a] = -1e30f; }
+    for (size_t k = 0; k < M.V.size(); ++k) { int a = k % 3; S.lo[a] = std::min(S.lo[a], M.V[k]); hi[a] = std::max(hi[a], M.V[k]); }
+    for (auto& st : d.objects)
+        for (auto& b : st)
+            if (b.present)
+                for (int a = 0; a < 3; ++a) { S.lo[a] = std::min(S.lo[a], b.c[a] - b.e[a]); hi[a] = std::max(hi[a], b.c[a] + b.e[a]); }
+    float dg = 0.f;
+    for (int a = 0; a < 3; ++a) dg += (hi[a] - S.lo[a]) * (hi[a] - S.lo[a]);
+    S.diag = std::sqrt(dg);
+    for (int a = 0; a < 3; ++a) { float pad = 0.01f * S.diag + 0.05f; S.lo[a] -= pad; hi[a] += pad; S.ext[a] = hi[a] - S.lo[a]; }
+    S.sun_len = 2.f * S.diag;
+
+    // 1. BVH + triangles in leaf order (v0, e1, e2) with normal + double-sided flag
+    build_bvh(M, S.BN, S.order, S.bvh_depth);
+    build_wide(S.BN, S.WN, S.wide_depth);
+    if (3 * S.wide_depth > RC_WSTACK) return "wide BVH too deep for the traversal stack";
+    S.nst = M.nt();
+    S.ST.resize(9 * (size_t)S.nst); S.STN.resize(4 * (size_t)S.nst); S.Talb.resize(S.nst);
+    for (int j = 0; j < S.nst; ++j) {
+        const float* v = &M.V[9 * (size_t)S.order[j]];
+        float* T = &S.ST[9 * (size_t)j];
+        for (int x = 0; x < 3; ++x) { T[x] = v[x]; T[3 + x] = v[3 + x] - v[x]; T[6 + x] = v[6 + x] - v[x]; }
+        float cr[3] = {T[4] * T[8] - T[5] * T[7], T[5] * T[6] - T[3] * T[8], T[3] * T[7] - T[4] * T[6]};
+        float l = std::sqrt(cr[0] * cr[0] + cr[1] * cr[1] + cr[2] * cr[2]);
+        l = l > 0 ? l : 1.f;
+        int fl = M.ds[S.order[j]] ? 1 : 0;
+        S.STN[4 * (size_t)j] = cr[0] / l; S.STN[4 * (size_t)j + 1] = cr[1] / l; S.STN[4 * (size_t)j + 2] = cr[2] / l;
+        memcpy(&S.STN[4 * (size_t)j + 3], &fl, 4);
+        S.Talb[j] = M.alb[S.order[j]];
+    }
+
+    // 2. cache cells (h fixed, or adjusted toward the target count: cuda/prep_mesh.cuh:338-348)
+    std::vector<uint64_t> keys; std::vector<CellRec> recs;
+    if (d.h > 0.f) {
+        S.h = d.h;
+        sample_cells(M, S.h, S.lo, keys, recs);
+    } else {
+        if (d.target_points <= 0) return "either h or target_points must be set";
+        double atot = 0;
+        for (int t = 0; t < M.nt(); ++t) {
+            const float* v = &M.V[9 * (size_t)t];
+            double e1[3] = {v[3] - v[0], v[4] - v[1], v[5] - v[2]}, e2[3] = {v[6] - v[0], v[7] - v[1], v[8] - v[2]};
+            double cr[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
+            atot += 0.5 * std::sqrt(cr[0] * cr[0] + cr[1] * cr[1] + cr[2] * cr[2]) * (M.ds[t] ? 2 : 1);
+        }
+        S.h = (float)std::sqrt(atot / d.target_points);
+        for (int it = 0; it < 5; ++it) {
+            sample_cells(M, S.h, S.lo, keys, recs);
+            double ratio = (double)keys.size() / d.target_points;
+            if (std::fabs(ratio - 1.0) < 0.08 || it == 4) break;
+            S.h *= (float)std::min(2.0, std::max(0.5, std::sqrt(ratio)));
+        }
+    }
+    S.npst = (int)keys.size();
+    int log2 = 10;
+    while (((size_t)1 << log2) < 2 * (size_t)S.npst) ++log2;
+    if (d.table_log2 > 0) {
+        if (d.table_log2 < log2) return "hash table too small for the cache cells (needs load <= 1/2)";
+        log2 = d.table_log2;
+    }
+    build_hash(keys, log2, S.HK, S.HV);
+    std::unordered_map<uint64_t, int> zone;
+    for (int i = 0; i < S.npst; ++i) {
+        const CellRec& c = recs[i];
+        for (int x = 0; x < 3; ++x) { S.Pst.push_back(c.p[x]); S.Nst.push_back(c.n[x]); }
+        S.A.push_back((float)(c.wsum > 0 ? c.asum / c.wsum : 0.5));
+        int cx, cy, cz, b;
+        decode_key(keys[i], cx, cy, cz, b);
+        auto z = zone.try_emplace(cell_key(cx >> 2, cy >> 2, cz >> 2, b), (int)zone.size());
+        S.region.push_back(z.first->second);
+    }
+    S.NR = (int)zone.size();
+
+    // 3. dynamic boxes -> quads with point grids (cuda/prep_mesh.cuh:215-272)
+    const int nobj = d.objects.empty() ? 0 : (int)d.objects[0].size();
+    for (auto& st : d.objects) if ((int)st.size() != nobj) return "every state must list the same objects";
+    if (!d.objects.empty() && (int)d.objects.size() != S.NS) return "objects must be given for every state";
+    S.nbox = 0;
+    S.Nq = 6 * nobj;
+    int npt = S.npst;
+    for (int o = 0; o < nobj; ++o) {
+        const DynBox& b0 = d.objects[0][o];
+        float F[6][12];
+        box_faces(b0.c, b0.e, F);
+        for (int q = 0; q < 6; ++q) {
+            float lu = std::sqrt(F[q][3] * F[q][3] + F[q][4] * F[q][4] + F[q][5] * F[q][5]);
+            float lv = std::sqrt(F[q][6] * F[q][6] + F[q][7] * F[q][7] + F[q][8] * F[q][8]);
+            int nu = std::max(2, (int)std::lround(lu / S.h)), nv = std::max(2, (int)std::lround(lv / S.h));
+            S.dims.push_back(nu); S.dims.push_back(nv); S.off.push_back(npt);
+            int tu = (nu + 3) / 4, tv = (nv + 3) / 4;
+            for (int iv = 0; iv < nv; ++iv)
+                for (int iu = 0; iu < nu; ++iu) { S.A.push_back(d.dyn_albedo); S.region.push_back(S.NR + (iv / 4) * tu + iu / 4); }
+            S.NR += tu * tv;
+            npt += nu * nv;
+        }
+    }
+    S.nbox = npt - S.npst;
+    S.Np = npt;
+    for (int s = 0; s < S.NS; ++s) {
+        std::vector<float> P(S.Pst), N(S.Nst), Q, T; std::vector<int> TQ; std::vector<uint8_t> pres; std::vector<float> ob;
+        for (int o = 0; o < nobj; ++o) {
+            const DynBox& b = d.objects[s][o];
+            float c[3] = {b.c[0], b.c[1], b.c[2]};
+            if (!b.present) { for (int a = 0; a < 3; ++a) c[a] = S.lo[a] - 1e4f; }   // parked far outside the scene
+            pres.push_back(b.present ? 1 : 0);
+            for (int a = 0; a < 3; ++a) ob.push_back(c[a] - b.e[a]);
+            for (int a = 0; a < 3; ++a) ob.push_back(c[a] + b.e[a]);
+            float F[6][12];
+            box_faces(c, b.e, F);
+            for (int q = 0; q < 6; ++q) {
+                int qi = 6 * o + q, nu = S.dims[2 * qi], nv = S.dims[2 * qi + 1];
+                Q.insert(Q.end(), F[q], F[q] + 12);
+                for (int iv = 0; iv < nv; ++iv)
+                    for (int iu = 0; iu < nu; ++iu)
+                        for (int x = 0; x < 3; ++x) {
+                            P.push_back(F[q][x] + F[q][3 + x] * ((float)iu + 0.5f) / (float)nu + F[q][6 + x] * ((float)iv + 0.5f) / (float)nv);
+                            N.push_back(F[q][9 + x]);
+                        }
+                if (b.present) {
+                    const float* oo = F[q]; const float* u = oo + 3; const float* v = oo + 6;
+                    float uv[3] = {u[0] + v[0], u[1] + v[1], u[2] + v[2]};
+                    T.insert(T.end(), oo, oo + 3); T.insert(T.end(), u, u + 3); T.insert(T.end(), uv, uv + 3);
+                    T.insert(T.end(), oo, oo + 3); T.insert(T.end(), uv, uv + 3); T.insert(T.end(), v, v + 3);
+                    TQ.push_back(qi); TQ.push_back(qi);
+                }
+            }
+        }
+        S.P.push_back(P); S.N.push_back(N); S.Q.push_back(Q); S.T.push_back(T); S.TQ.push_back(TQ);
+        S.nt.push_back((int)TQ.size()); S.present.push_back(pres); S.objbox.push_back(ob);
+    }
+
+    // 4. events per ordered state pair: transit voxels overlapped by every object that moves,
+    // appears or disappears (both placements), and per light the voxel of a light turning on
+    // (a directional light turning on touches every voxel).
+    const int NV = S.vx * S.vy * S.vz;
+    for (int a = 0; a < S.NS; ++a)
+        for (int b = 0; b < S.NS; ++b) {
+            std::vector<char> mark(NV, 0);
+            if (a != b)
+                for (int o = 0; o < nobj; ++o) {
+                    const DynBox &A_ = d.objects[a][o], &B_ = d.objects[b][o];
+                    bool same = A_.present == B_.present && (!A_.present || (A_.c[0] == B_.c[0] && A_.c[1] == B_.c[1] && A_.c[2] == B_.c[2]));
+                    if (same) continue;
+                    for (const DynBox* X : {&A_, &B_}) {
+                        if (!X->present) continue;
+                        float plo[3] = {X->c[0] - X->e[0], X->c[1] - X->e[1], X->c[2] - X->e[2]};
+                        float phi[3] = {X->c[0] + X->e[0], X->c[1] + X->e[1], X->c[2] + X->e[2]};
+                        int v0 = host_vox(S, plo), v1 = host_vox(S, phi);
+                        int i0 = v0 % S.vx, j0 = (v0 / S.vx) % S.vy, k0 = v0 / (S.vx * S.vy);
+                        int i1 = v1 % S.vx, j1 = (v1 / S.vx) % S.vy, k1 = v1 / (S.vx * S.vy);
+                        for (int k = k0; k <= k1; ++k) for (int j = j0; j <= j1; ++j) for (int i = i0; i <= i1; ++i) mark[(k * S.vy + j) * S.vx + i] = 1;
+                    }
+                }
+            std::vector<int> occ;
+            for (int v = 0; v < NV; ++v) if (mark[v]) occ.push_back(v);
+            S.occ.push_back(occ);
+            for (int l = 0; l < S.nl; ++l) {
+                std::vector<int> em;
+                if (a != b && S.inten[a][l] == 0.f && S.inten[b][l] > 0.f) {
+                    if (S.ltype[l] == 1) for (int v = 0; v < NV; ++v) em.push_back(v);
+                    else em.push_back(host_vox(S, &S.L[3 * l]));
+                }
+                S.em.push_back(em);
+            }
+        }
+    return "";
+}
+
+// ------------------------------------------------------------------------------------ generators
+static void face_grid(Mesh& M, const float F[12], float cell, float albedo) {
+    float lu = std::sqrt(F[3] * F[3] + F[4] * F[4] + F[5] * F[5]), lv = std::sqrt(F[6] * F[6] + F[7] * F[7] + F[8] * F[8]);
+    int nu = std::max(1, (int)std::lround(lu / cell)), nv = std::max(1, (int)std::lround(lv / cell));
+    M.add_quad_grid(F, F + 3, F + 6, nu, nv, albedo, false);
+}
+static int face_tris(const float F[12], float cell) {
+    float lu = std::sqrt(F[3] * F[3] + F[4] * F[4] + F[5] * F[5]), lv = std::sqrt(F[6] * F[6] + F[7] * F[7] + F[8] * F[8]);
+    return 2 * std::max(1, (int)std::lround(lu / cell)) * std::max(1, (int)std::lround(lv / cell));
+}
+// cell size whose face subdivision gives about `target` triangles (bisection on log scale)
+static float fit_cell(const std::vector<std::array<float, 12>>& faces, int target) {
+    float lo = 1e-3f, hi = 1e3f;
+    for (int it = 0; it < 60; ++it) {
+        float mid = std::sqrt(lo * hi);
+        long long n = 0;
+        for (auto& f : faces) n += face_tris(f.data(), mid);
+        if (n > target) lo = mid; else hi = mid;
+    }
+    return hi;
+}
+
+static void look_at(rc_camera& c, const float pos[3], const float at[3], float hfov_deg, int W, int H) {
+    float f[3] = {at[0] - pos[0], at[1] - pos[1], at[2] - pos[2]};
+    float fl = std::sqrt(f[0] * f[0] + f[1] * f[1] + f[2] * f[2]);
+    for (int a = 0; a < 3; ++a) f[a] /= fl;
+    float upw[3] = {0, 0, 1};
+    float r[3] = {f[1] * upw[2] - f[2] * upw[1], f[2] * upw[0] - f[0] * upw[2], f[0] * upw[1] - f[1] * upw[0]};
+    float rl = std::sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+    for (int a = 0; a < 3; ++a) r[a] /= rl;
+    float u[3] = {r[1] * f[2] - r[2] * f[1], r[2] * f[0] - r[0] * f[2], r[0] * f[1] - r[1] * f[0]};
+    for (int a = 0; a < 3; ++a) { c.pos[a] = pos[a]; c.fwd[a] = f[a]; c.right[a] = r[a]; c.up[a] = u[a]; }
+    c.tanx = std::tan(0.5f * hfov_deg * 3.14159265f / 180.f);
+    c.tany = c.tanx * (float)H / (float)W;
+    c.W = W; c.H = H;
+}
+
+SceneDesc gen_box_room(uint64_t seed, int target_tris) {
+    SceneDesc d;
+    HostRng r(seed);
+    const float X = 8.f, Y = 4.f, Z = 3.f;
+    std::vector<std::array<float, 12>> faces;
+    std::vector<float> falb;
+    const float walls[6][12] = {
+        {0, 0, 0, X, 0, 0, 0, Y, 0, 0, 0, 1},    // floor
+        {0, 0, Z, 0, Y, 0, X, 0, 0, 0, 0, -1},   // ceiling
+        {0, 0, 0, 0, 0, Z, X, 0, 0, 0, 1, 0},    // y = 0
+        {0, Y, 0, X, 0, 0, 0, 0, Z, 0, -1, 0},   // y = Y
+        {0, 0, 0, 0, Y, 0, 0, 0, Z, 1, 0, 0},    // x = 0
+        {X, 0, 0, 0, 0, Z, 0, Y, 0, -1, 0, 0}};  // x = X
+    for (auto& w : walls) { std::array<float, 12> a; memcpy(a.data(), w, 48); faces.push_back(a); falb.push_back(0.2f + 0.6f * r.u()); }
+    const float cubes[2][6] = {{2.5f, 1.5f, 0.5f, 0.5f, 0.5f, 0.5f}, {5.5f, 2.6f, 0.7f, 0.7f, 0.7f, 0.7f}};
+    for (auto& cb : cubes) {
+        float F[6][12];
+        box_faces(cb, cb + 3, F);
+        for (int q = 0; q < 5; ++q) {   // the bottom face rests on the floor
+            std::array<float, 12> a; memcpy(a.data(), F[q], 48); faces.push_back(a); falb.push_back(0.2f + 0.6f * r.u());
+        }
+    }
+    float cell = fit_cell(faces, target_tris);
+    for (size_t k = 0; k < faces.size(); ++k) face_grid(d.mesh, faces[k].data(), cell, falb[k]);
+    d.nl = 2;
+    for (int l = 0; l < 2; ++l) {
+        d.L.push_back(0.5f + 7.f * r.u()); d.L.push_back(0.5f + 3.f * r.u()); d.L.push_back(1.5f + 1.3f * r.u());
+        d.ltype.push_back(0);
+    }
+    d.inten = {{50.f, 50.f}, {50.f, 0.f}};
+    d.h = 0.05f;
+    d.table_log2 = 18;
+    d.vx = 8; d.vy = 4; d.vz = 4;
+    float pos[3] = {0.4f, 0.6f, 2.2f}, at[3] = {6.5f, 2.8f, 0.6f};
+    look_at(d.cam, pos, at, 75.f, 256, 256);
+    d.cam.spp = 1; d.cam.bounces = 1; d.cam.cache_bounce = 1;
+    return d;
+}
+
+SceneDesc gen_street(uint64_t seed, int nboxes, int target_tris, int nlamps, bool sun, int target_points, int table_log2,
+                     int W, int H, int spp, int bounces) {
+    SceneDesc d;
+    HostRng r(seed);
+    const float pitch = 16.f;
+    int cols = (int)std::ceil(std::sqrt(nboxes * 1.25)), rows = (nboxes + cols - 1) / cols;
+    std::vector<std::array<float, 12>> faces;
+    std::vector<float> falb;
+    float gx = cols * pitch, gy = rows * pitch;
+    {   // ground
+        std::array<float, 12> g = {-pitch, -pitch, 0, gx + 2 * pitch, 0, 0, 0, gy + 2 * pitch, 0, 0, 0, 1};
+        faces.push_back(g); falb.push_back(0.35f);
+    }
+    for (int b = 0; b < nboxes; ++b) {
+        int ci = b % cols, ri = b / cols;
+        float e[3] = {0.5f * (0.5f + 9.5f * r.u()), 0.5f * (0.5f + 9.5f * r.u()), 0.5f * (0.5f + 9.5f * r.u())};
+        float c[3];
+        for (int a = 0; a < 2; ++a) {
+            float base = (a == 0 ? ci : ri) * pitch + 3.f;
+            float room = 10.f - 2.f * e[a];
+            c[a] = base + e[a] + room * r.u();
+        }
+        c[2] = e[2];
+        float alb = 0.2f + 0.6f * r.u();
+        float F[6][12];
+        box_faces(c, e, F);
+        for (int q = 0; q < 5; ++q) { std::array<float, 12> a; memcpy(a.data(), F[q], 48); faces.push_back(a); falb.push_back(alb); }
+    }
+    float cell = fit_cell(faces, target_tris);
+    for (size_t k = 0; k < faces.size(); ++k) face_grid(d.mesh, faces[k].data(), cell, falb[k]);
+    // lamps at street crossings (x = i*pitch, y = j*pitch), lamp 0 at the crossing nearest the middle
+    int cx0 = cols / 2, cy0 = rows / 2;
+    std::vector<std::pair<int, int>> used{{cx0, cy0}};
+    while ((int)used.size() < nlamps) {
+        int i = (int)(r.u() * (cols + 1)), j = (int)(r.u() * (rows + 1));
+        if (std::find(used.begin(), used.end(), std::make_pair(i, j)) == used.end()) used.push_back({i, j});
+    }
+    for (auto& ij : used) { d.L.push_back(ij.first * pitch); d.L.push_back(ij.second * pitch); d.L.push_back(6.f); d.ltype.push_back(0); }
+    if (sun) {
+        float s[3] = {0.45f, 0.3f, 0.84f};
+        float sl = std::sqrt(s[0] * s[0] + s[1] * s[1] + s[2] * s[2]);
+        for (int a = 0; a < 3; ++a) d.L.push_back(s[a] / sl);
+        d.ltype.push_back(1);
+    }
+    d.nl = (int)d.ltype.size();
+    std::vector<float> base(d.nl, 60.f);
+    if (sun) base[d.nl - 1] = 1.0f;
+    std::vector<float> night = base, lamp0 = base, occl = base;
+    if (sun) night[d.nl - 1] = 0.f;
+    lamp0[0] = 0.f;
+    d.inten = {base, night, lamp0, occl};
+    // occluder: a 12 m x 10 m wall (0.4 m thick) standing on the street next to lamp 0, present in state 3
+    DynBox wall{{cx0 * pitch + 4.f, cy0 * pitch, 5.f}, {0.2f, 6.f, 5.f}, false};
+    d.objects.resize(4, std::vector<DynBox>{wall});
+    d.objects[3][0].present = true;
+    d.h = 0.f;
+    d.target_points = target_points;
+    d.table_log2 = table_log2;
+    d.vx = 8; d.vy = 4; d.vz = 4;
+    float at[3] = {cx0 * pitch + 2.f, cy0 * pitch + 1.f, 0.5f}, pos[3] = {cx0 * pitch - 22.f, cy0 * pitch - 18.f, 14.f};
+    look_at(d.cam, pos, at, 70.f, W, H);
+    d.cam.spp = spp; d.cam.bounces = bounces; d.cam.cache_bounce = 1;
+    return d;
+}
+
+}  // namespace rc

@@ -1,0 +1,122 @@
+// Plain-old-data layouts shared by the host C++ (g++) and the HIP kernels (hipcc).
+// Field meanings follow the prototype's DevState / Cache / Params structs
+// (cuda/poc_gpu.cu:222-237, :394-409, :411-422); the light count, transit-grid
+// size and channel count are runtime values here instead of -DNLIGHTS/-DGX
+// compile-time constants (cuda/build.sh:12).
+#pragma once
+#include <stdint.h>
+
+#define RC_MAX_LIGHTS 72      // light channels per cache entry (64 lamps + sun fit)
+#define RC_MAX_NW 4           // transit-grid mask words per entry (128 bits, cuda/build.sh:12)
+#define RC_KSNAP 6            // snapshot slots (cuda/poc_gpu.cu:423)
+#define RC_VERIF_FRAMES 4     // verification window (cuda/poc_gpu.cu:115)
+
+// Fixed-point scales of the order-independent reductions. Float sums that feed
+// control flow (ray allocation, verification, DDGI blend) are accumulated as
+// int64 so that every summation order gives the same bits.
+#define RC_FX_PR 4294967296.0f    // sum of 1/(n+1)           (2^32)
+#define RC_FX_STAT 1048576.0f     // per-region sample stats   (2^20)
+
+// One BVH node of the binary tree: the prototype's two float4 {lo.xyz, lf} {hi.xyz, cnt}
+// (cuda/prep_mesh.cuh:39-40). cnt > 0: leaf with triangles lf..lf+cnt-1; else children lf, lf+1.
+typedef struct rc_bvh_node {
+    float lo[3]; int32_t lf;
+    float hi[3]; int32_t cnt;
+} rc_bvh_node;
+
+// 4-wide BVH node (128 B = two 64-byte lines): child boxes as fp32 SoA so the four slab
+// tests of a node read whole float4s. child[c] >= 0: inner node; child[c] <= -2: leaf holding
+// triangles first = -2 - child[c], count = ccount[c] (same triangle order and indices as the
+// binary BVH); child[c] == -1: empty slot.
+typedef struct rc_wnode {
+    float lox[4], loy[4], loz[4], hix[4];   // 64 B
+    float hiy[4], hiz[4];                   // 32 B
+    int32_t child[4];                       // 16 B
+    int32_t ccount[4];                      // 16 B
+} rc_wnode;
+#define RC_WSTACK 64          // wide-traversal stack; the builder bounds depth so 3 pushes/level fit
+
+// Per-state device view of the scene (cuda/poc_gpu.cu:222-237).
+typedef struct rc_dstate {
+    const float* P;          // [3 Np] cache point positions (quad points move with the state)
+    const float* N;          // [3 Np] cache point normals
+    const float* Q;          // [12 Nq] quads: o, u, v, n
+    const int32_t* dims;     // [2 Nq] points per quad side
+    const int32_t* off;      // [Nq] first cache point of the quad
+    const float* T;          // [9 nt] dynamic triangles v0, e1, e2
+    const int32_t* TQ;       // [nt] quad of each dynamic triangle
+    float* Edir;             // [NL Np] direct light per entry and light
+    uint32_t* SV;            // [NW Np] static shadow-ray transit mask (AoS)
+    uint64_t* LM;            // [Np] light-visibility mask of the entry (bit min(l,63))
+    const float* ref;        // [Np] converged reference (optional)
+    const uint8_t* valid;    // [Np]
+    const float* Pw;         // [NL Np] converged per-light reference (optional)
+    const float* inten;      // [NL] light intensities of the state
+    int32_t nt;              // dynamic triangles in this state
+    int32_t pad0;
+} rc_dstate;
+
+// Static scene data on the device (format-3 part of cuda/poc_gpu.cu:231-236).
+typedef struct rc_dscene {
+    const float* ST;         // [9 nst] static triangles v0, e1, e2 (BVH leaf order)
+    const float* STN;        // [4 nst] geometric normal + flags (w bit 0 = double sided)
+    const rc_bvh_node* BN;   // binary BVH nodes
+    const rc_wnode* WN;      // 4-wide BVH nodes
+    const uint64_t* HK;      // hash keys (0 = empty)
+    const int32_t* HV;       // entry of each key
+    const float* L;          // [3 NL] light positions (point) or unit direction to the light (sun)
+    const int32_t* ltype;    // [NL] 0 = point, 1 = directional
+    const float* Alb;        // [Np] entry albedo
+    const int32_t* region;   // [Np] verification zone
+    uint32_t hmask;          // table size - 1
+    float hcell;             // cache cell size (m)
+    float lo[3], ext[3];     // transit-grid / hash-grid bounds
+    float sun_len;           // shadow-ray length toward a directional light
+    int32_t nst, nl, Np, NR;
+    int32_t vx, vy, vz, nw;  // transit grid (vx*vy*vz = 32 nw bits per entry)
+} rc_dscene;
+
+// Cache buffers (cuda/poc_gpu.cu:394-409) without the v5-B projection buffers (out of scope).
+typedef struct rc_dcache {
+    float *Lc, *Lc2, *m, *n, *c0;
+    int32_t* kk;
+    uint8_t* pending;
+    uint32_t *cur, *prev;        // [NW][Np] SoA transit-grid dependency banks
+    uint64_t *lcur, *lprev;      // [Np] light-dependency banks
+    long long* rstat;            // [4 NR] fixed point (RC_FX_STAT)
+    long long* dstat;            // [3 NR] fixed point (RC_FX_STAT)
+    uint64_t* stale;             // [Np] bit l = dormant channel l is out of date here (l < 64)
+    float* snap;                 // [KSNAP NL Np]
+    float* age;                  // [Np]
+    long long* sumpr;            // [2] fixed point (RC_FX_PR): current, next
+    int32_t* npend;              // [2]
+    double* err;                 // [3]
+} rc_dcache;
+
+// Per-frame parameters (cuda/poc_gpu.cu:411-422).
+typedef struct rc_params {
+    int32_t Np, NR, mode;        // mode 0 = fixed budget, 1 = adaptive allocation
+    float budget, nmax;
+    uint32_t seed, frame;
+    int32_t pend_left, deps;
+    uint64_t resp_mask;          // lights blended with the responsive history cap
+    float nmax_resp;
+    int32_t ddgi; float ddgi_h, ddgi_thr; int32_t ddgi_clamp;
+    uint64_t dormant;            // dormant light channels (frozen, excluded from estimates)
+    int32_t ramp;
+    float rays;
+    int32_t nl;
+    int32_t pad0;
+} rc_params;
+
+// Pinhole camera of the query pass. Paths: primary hit, one-light NEE per explicit vertex,
+// cosine bounces; with a cache the path ends on the cached radiance at bounce cache_bounce,
+// without one it ends after `bounces` bounces.
+#ifndef RC_CAMERA_DEFINED
+#define RC_CAMERA_DEFINED
+typedef struct rc_camera {
+    float pos[3], fwd[3], right[3], up[3];
+    float tanx, tany;
+    int32_t W, H, spp, bounces, cache_bounce, pad0;
+} rc_camera;
+#endif
