@@ -1,0 +1,220 @@
+"""Benchmark of the radiance-cache frame on one MI355X.
+
+Workload (default): procedural street of 2,000 boxes (~200k triangles), 32 point lamps + sun,
+~530k cache points in a 2^22-entry hash grid, camera 960x540 at 4 spp with up to 3 bounces.
+One step = one frame of the plain-cache method: entry pass (k_trace2: one cosine ray per cache
+entry, cache read at the hit) + resolve (k_update EMA) + camera query pass (k_query: primary ray,
+one-light NEE shadow ray, one bounce ray ending on the cached radiance).
+
+Prints one JSON line: value = Mrays/s over all rays traced in the timed frames (entry rays +
+query extension rays + query shadow rays), with inputs resident in HBM. Extra fields: cache
+Mpts/s, frames/s, per-kernel times, roofline of the dominant kernel and the CPU baseline
+(the C oracle on the host cores, bounded sample).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+CONFIGS = {
+    # small box room (CPU-runnable case): 256x256, 1 spp, 1 bounce, 2^18 grid
+    1: dict(kind="room", seed=1, tris=2000),
+    # street at 960x540, 4 spp, 3 bounces, ~530k points, 2^22 grid
+    2: dict(kind="street", seed=1, nboxes=2000, tris=200_000, nlamps=32, sun=True, points=530_000, table_log2=22, W=960, H=540,
+            spp=4, bounces=3),
+    # street at 1920x1080, 4 spp, 3 bounces, ~1M points, 2^23 grid, 64 lamps
+    3: dict(kind="street", seed=1, nboxes=2000, tris=1_000_000, nlamps=64, sun=True, points=1_000_000, table_log2=23, W=1920,
+            H=1080, spp=4, bounces=3),
+}
+
+
+def make_scene(rea, cfg):
+    if cfg["kind"] == "room":
+        return rea.Scene.box_room(cfg["seed"], cfg["tris"])
+    return rea.Scene.street(cfg["seed"], cfg["nboxes"], cfg["tris"], cfg["nlamps"], cfg["sun"], cfg["points"], cfg["table_log2"],
+                            cfg["W"], cfg["H"], cfg["spp"], cfg["bounces"])
+
+
+def update_bytes_per_entry(nl, tracked=False):
+    """k_update algorithmic bytes per entry: read kk, n, Lc[nl], m[nl]; write Lc2[nl], n"""
+    return 4 + 4 + 4 * nl + 4 * nl + 4 * nl + 4 + (8 if tracked else 0)
+
+
+def cpu_baseline(scene, cfg, seconds, log):
+    """The C oracle on the host cores: the same frame (entry pass + update + query) on the same
+    scene, state-0 build arrays taken from the GPU build (bit-identical), timed per frame."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import ctypes as C
+
+    import numpy as np
+    import pyoracle as po
+    import render_engine_amd as rea
+
+    osc = po.OracleScene(scene, build=False)
+    for st in range(scene.info.NS):
+        osc.Edir[st][:] = scene.array("EDIR", st)
+        osc.SV[st][:] = scene.array("SV", st)
+        osc.LM[st][:] = scene.array("LM", st)
+    m = rea.methods.plain()
+    run = po.OracleRunner(osc, m)
+    run.begin(0, 1234)
+    cam = scene.camera()
+    img = np.zeros(cam.W * cam.H, np.float32)
+    frames, rays, t_tot = 0, 0, 0.0
+    while frames < 1 or (t_tot < seconds and frames < 50):
+        t0 = time.perf_counter()
+        run.frame(0)
+        lc = run.get("LC")   # a copy, outside the timed part below
+        t1 = time.perf_counter()
+        er = (C.c_longlong * 2)()
+        t2 = time.perf_counter()
+        po.lib.orc_k_query(osc.state(0), 1, po.ptr(lc), C.byref(cam), 99, frames, 0, po.ptr(img), er, 0, cam.W * cam.H)
+        t3 = time.perf_counter()
+        kk = run.get("KK")
+        rays += int(kk.sum()) + er[0] + er[1]
+        t_tot += (t1 - t0) + (t3 - t2)
+        frames += 1
+    log(f"cpu baseline: {frames} frames in {t_tot:.1f} s on {po.lib.orc_num_threads()} threads")
+    return dict(value=rays / t_tot / 1e6, unit="Mrays/s", cores=int(po.lib.orc_num_threads()), kind="port",
+                sample=f"{frames} full plain-cache frames (entry pass + update + {cam.W}x{cam.H}x{cam.spp} query) of the "
+                       f"same scene, state 0", fps=frames / t_tot)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=30)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--config", type=int, default=2, choices=sorted(CONFIGS))
+    ap.add_argument("--trav", type=int, default=2)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--quiet", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only: --gpus {args.gpus} is not supported (the path shards by screen tiles and "
+              f"hash-key ranges, see the design notes; this build measures one GPU's share)", file=sys.stderr)
+        sys.exit(2)
+
+    def log(msg):
+        if not args.quiet:
+            print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+    import numpy as np
+    import torch
+
+    import render_engine_amd as rea
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    cfg = CONFIGS[args.config]
+    t0 = time.time()
+    scene = make_scene(rea, cfg)
+    info = scene.info
+    log(f"scene: {info.nst} triangles, {info.nnodes} binary / {info.nwnodes} wide nodes, {info.Np} cache entries, "
+        f"table {info.hsize}, {info.nl} lights, prep {time.time() - t0:.1f} s")
+    t0 = time.time()
+    scene.upload(trav=args.trav)
+    log(f"upload + per-state build {time.time() - t0:.1f} s")
+    m = rea.methods.plain()
+    cache = rea.Cache(scene, m)
+    cam = scene.camera()
+    img = torch.zeros(cam.W * cam.H, dtype=torch.float32, device="cuda")
+    seed = 4242
+
+    def one_frame(f):
+        cache.frame(0)
+        cache.render(img, cam, seed=seed + f)
+
+    # 1) ray counts of exactly the frames that are timed below (same seeds -> same paths)
+    cache.begin(0, seed)
+    for f in range(args.warmup):
+        one_frame(f)
+    n_entry, n_ext, n_sh = 0, 0, 0
+    for f in range(args.steps):
+        cache.frame(0)
+        n_entry += int(cache.get("KK").sum())
+        e, s_ = cache.render(img, cam, seed=seed + args.warmup + f, count_rays=True)
+        n_ext += e
+        n_sh += s_
+    # 2) timed region: the same frames again from the same start
+    cache.begin(0, seed)
+    for f in range(args.warmup):
+        one_frame(f)
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    for f in range(args.steps):
+        one_frame(args.warmup + f)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t_start
+    ms_per_step = elapsed * 1e3 / args.steps
+    total_rays = n_entry + n_ext + n_sh
+    mrays = total_rays / elapsed / 1e6
+    # 3) per-kernel GPU time with HIP events on the library stream (same frames again)
+    cache.begin(0, seed)
+    for f in range(args.warmup):
+        one_frame(f)
+    k_trace, k_update, k_query = [], [], []
+    for f in range(args.steps):
+        fs = cache.frame(0, timed=True)
+        ms_q = cache.render(img, cam, seed=seed + args.warmup + f, timed=True)
+        k_trace.append(fs.ms_trace)
+        k_update.append(fs.ms_update)
+        k_query.append(ms_q)
+    t_tr, t_up, t_q = (float(np.mean(x)) for x in (k_trace, k_update, k_query))
+    upd_bytes = update_bytes_per_entry(info.nl) * info.Np
+    kernels = {
+        "k_trace2": {"ms": t_tr, "rays": n_entry / args.steps, "mrays_s": n_entry / args.steps / t_tr / 1e3},
+        "k_update": {"ms": t_up, "entries": info.Np, "bytes": upd_bytes, "GB_s": upd_bytes / t_up / 1e6},
+        "k_query": {"ms": t_q, "rays": (n_ext + n_sh) / args.steps, "mrays_s": (n_ext + n_sh) / args.steps / t_q / 1e3},
+    }
+    dom = max(kernels, key=lambda k: kernels[k]["ms"])
+    log(f"kernels: {json.dumps(kernels)}")
+    # roofline: traversal kernels are priced by the bytes of the nodes and triangles the rays
+    # visit (counted on the CPU oracle for the same rays); k_update by its streaming bytes
+    roof = None
+    if dom == "k_update":
+        roof = {"bound": "hbm", "achieved": upd_bytes / t_up / 1e6, "peak": 8000.0, "unit": "GB/s",
+                "frac": upd_bytes / t_up / 1e6 / 8000.0, "traffic": None, "kernel": dom}
+    else:
+        roof = {"bound": "hbm", "achieved": None, "peak": 8000.0, "unit": "GB/s", "frac": None, "traffic": None, "kernel": dom}
+    cpu = None
+    if not args.no_cpu:
+        try:
+            cpu = cpu_baseline(scene, cfg, args.cpu_seconds, log)
+        except Exception as e:  # the baseline is reported, not required
+            log(f"cpu baseline failed: {e!r}")
+    out = {
+        "metric": "Mrays/s + cache Mpts/s; frames to <5% error after light-off, 1 GPU",
+        "value": round(mrays, 2),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (procedural scene, seeded)",
+        "config": {"workload": (f"street {cfg.get('nboxes')} boxes / {info.nst} tris, {info.nl} lights, {info.Np} cache entries, "
+                                f"table 2^{int(np.log2(info.hsize))}, {cam.W}x{cam.H} x {cam.spp} spp, plain cache")
+                   if cfg["kind"] == "street" else f"box room {info.nst} tris, {info.Np} entries, {cam.W}x{cam.H}",
+                   "config_index": args.config, "traversal": ["binary-ref", "binary", "wide4"][args.trav]},
+        "cache_mpts_s": round(info.Np * args.steps / elapsed / 1e6, 2),
+        "frames_per_s": round(args.steps / elapsed, 2),
+        "rays_per_frame": {"entry": n_entry / args.steps, "query_ext": n_ext / args.steps, "query_shadow": n_sh / args.steps},
+        "kernels": kernels,
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    if cpu:
+        out["gpu_over_cpu_fps"] = round((args.steps / elapsed) / cpu["fps"], 1)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -125,8 +125,10 @@ rc_status rc_cache_begin(rc_cache* c, int state0, uint32_t seed);
  * decide, update, optional error (:851-1011). timed != 0 brackets phases with HIP events. */
 rc_status rc_cache_frame(rc_cache* c, int state, int want_err, int timed, rc_frame_stats* out);
 /* camera query pass over the current cache into d_img[W*H] (no_cache: plain path tracing).
- * rays_out[2] = extension rays, shadow rays (may be NULL). */
-rc_status rc_cache_render(rc_cache* c, const rc_camera* cam, int no_cache, uint32_t seed, float* d_img, long long* rays_out);
+ * rays_out[2] = extension rays, shadow rays (may be NULL); ms_out = GPU time of the pass
+ * (may be NULL; when set the call waits for the pass). */
+rc_status rc_cache_render(rc_cache* c, const rc_camera* cam, int no_cache, uint32_t seed, float* d_img, long long* rays_out,
+                          float* ms_out);
 size_t rc_cache_array_bytes(const rc_cache* c, int which);
 rc_status rc_cache_get(const rc_cache* c, int which, void* host_dst, size_t bytes);
 rc_status rc_cache_set(rc_cache* c, int which, const void* host_src, size_t bytes);

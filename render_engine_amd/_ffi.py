@@ -79,7 +79,8 @@ _SIGS = {
     "rc_cache_destroy": (None, [_P]),
     "rc_cache_begin": (C.c_int, [_P, C.c_int, C.c_uint32]),
     "rc_cache_frame": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.POINTER(FrameStats)]),
-    "rc_cache_render": (C.c_int, [_P, C.POINTER(Camera), C.c_int, C.c_uint32, _P, C.POINTER(C.c_longlong)]),
+    "rc_cache_render": (C.c_int, [_P, C.POINTER(Camera), C.c_int, C.c_uint32, _P, C.POINTER(C.c_longlong),
+                                 C.POINTER(C.c_float)]),
     "rc_cache_array_bytes": (C.c_size_t, [_P, C.c_int]),
     "rc_cache_get": (C.c_int, [_P, C.c_int, _P, C.c_size_t]),
     "rc_cache_set": (C.c_int, [_P, C.c_int, _P, C.c_size_t]),
@@ -95,6 +96,13 @@ EXPORTS = sorted(_SIGS)
 
 
 def load(path=LIB_PATH):
+    # torch bundles its own HIP runtime (torch/lib/libamdhip64.so, soname libamdhip64.so.7) and
+    # loads it by file name. Importing torch first makes librc's libamdhip64.so.7 dependency bind
+    # to that same runtime, so device pointers and streams are shared by one HIP instance.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     if not os.path.exists(path):
         raise RcError(f"librc.so not found at {path}: build it first (python -c 'import __graft_entry__ as g; g.build()')")
     lib = C.CDLL(path)

@@ -594,7 +594,8 @@ extern "C" rc_status rc_cache_frame(rc_cache* c, int si, int want_err, int timed
     return RC_OK;
 }
 
-extern "C" rc_status rc_cache_render(rc_cache* c, const rc_camera* cam, int no_cache, uint32_t seed, float* d_img, long long* rays_out) {
+extern "C" rc_status rc_cache_render(rc_cache* c, const rc_camera* cam, int no_cache, uint32_t seed, float* d_img, long long* rays_out,
+                                     float* ms_out) {
     if (!c || !cam || !d_img) return fail(RC_ERR_ARG, "NULL argument");
     if (!c->begun) return fail(RC_ERR_STATE, "rc_cache_begin not called");
     if (cam->W < 1 || cam->H < 1 || cam->spp < 1 || cam->bounces < 0 || cam->cache_bounce < 0 || (long long)cam->W * cam->H > (1ll << 30))
@@ -604,7 +605,13 @@ extern "C" rc_status rc_cache_render(rc_cache* c, const rc_camera* cam, int no_c
     int si = c->prev_state;
     unsigned long long* ctr = nullptr;
     if (rays_out) { HIPCK(hipMemsetAsync(c->d_counters, 0, 16, st)); ctr = c->d_counters; }
+    if (ms_out) HIPCK(hipEventRecord(c->ev[0], st));
     HIPCK(rc_launch_query(S->trav, &S->ds, &S->dst[si], no_cache ? nullptr : c->c.Lc, cam, seed, (uint32_t)c->f, c->dormant, d_img, ctr, st));
+    if (ms_out) {
+        HIPCK(hipEventRecord(c->ev[1], st));
+        HIPCK(hipEventSynchronize(c->ev[1]));
+        *ms_out = ms_between(c->ev[0], c->ev[1]);
+    }
     if (rays_out) {
         unsigned long long h2[2];
         HIPCK(hipMemcpyAsync(h2, c->d_counters, 16, hipMemcpyDeviceToHost, st));

@@ -1,0 +1,230 @@
+"""GPU parity: the HIP library through its C ABI against the CPU oracle, bit for bit on hit
+ids, keys, masks and cache values; radiance within 1e-4 relative RMS of the FP64 twin."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import render_engine_amd as rea
+import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def room(torch_gpu):
+    s = rea.Scene.box_room(seed=1, target_tris=2000).upload(trav=rea.TRAV_WIDE)
+    return s, po.OracleScene(s)
+
+
+@pytest.fixture(scope="module")
+def street(torch_gpu):
+    s = rea.Scene.street(seed=3, nboxes=150, target_tris=12000, nlamps=6, sun=True, target_points=30000, table_log2=0,
+                         W=96, H=64, spp=2, bounces=3).upload(trav=rea.TRAV_WIDE)
+    return s, po.OracleScene(s)
+
+
+def _rays(scene, n, seed, st=0):
+    rng = np.random.default_rng(seed)
+    i = scene.info
+    lo, ext = np.array(list(i.lo)), np.array(list(i.ext))
+    O = (lo + ext * (0.05 + 0.9 * rng.random((n, 3)))).astype(np.float32)
+    P = scene.array("P", st).reshape(-1, 3)
+    pick = rng.integers(0, len(P), n // 2)
+    O[: n // 2] = P[pick] + scene.array("N", st).reshape(-1, 3)[pick] * 1e-3
+    D = rng.normal(size=(n, 3))
+    D /= np.linalg.norm(D, axis=1, keepdims=True)
+    # a few axis-aligned and grazing directions
+    D[:64] = np.eye(3)[rng.integers(0, 3, 64)] * rng.choice([-1, 1], (64, 1))
+    return O, D.astype(np.float32)
+
+
+def _gpu_trace(torch, scene, O, D, trav, st=0):
+    Ot = torch.from_numpy(O).cuda()
+    Dt = torch.from_numpy(D).cuda()
+    t = torch.empty(len(O), dtype=torch.float32, device="cuda")
+    q = torch.empty(len(O), dtype=torch.int32, device="cuda")
+    rea.trace(scene, Ot, Dt, t, q, state=st, trav=trav)
+    torch.cuda.synchronize()
+    return t.cpu().numpy(), q.cpu().numpy()
+
+
+@pytest.mark.parametrize("which", ["room", "street"])
+def test_trace_bit_exact(torch_gpu, which, request):
+    scene, osc = request.getfixturevalue(which)
+    for st in range(scene.info.NS):
+        O, D = _rays(scene, 20000, 10 + st, st)
+        t0, q0 = osc.trace(O, D, st, trav=0)
+        t1, q1 = osc.trace(O, D, st, trav=1)
+        for trav, (te, qe) in ((0, (t0, q0)), (1, (t1, q1)), (2, (t1, q1))):
+            tg, qg = _gpu_trace(torch_gpu, scene, O, D, trav, st)
+            assert np.array_equal(qg, qe), f"trav {trav} state {st}: {np.sum(qg != qe)} hit ids differ"
+            assert np.array_equal(tg.view(np.uint32), te.view(np.uint32)), f"trav {trav} state {st}: t differs"
+
+
+@pytest.mark.parametrize("which", ["room", "street"])
+def test_build_bit_exact(which, request):
+    scene, osc = request.getfixturevalue(which)
+    for st in range(scene.info.NS):
+        assert np.array_equal(scene.array("EDIR", st).view(np.uint32), osc.Edir[st].view(np.uint32))
+        assert np.array_equal(scene.array("SV", st), osc.SV[st])
+        assert np.array_equal(scene.array("LM", st), osc.LM[st])
+
+
+def test_hash_point_and_dda(torch_gpu, street):
+    torch = torch_gpu
+    scene, osc = street
+    rng = np.random.default_rng(5)
+    P = scene.array("P", 0).reshape(-1, 3)
+    N = scene.array("N", 0).reshape(-1, 3)
+    pick = rng.integers(0, len(P), 5000)
+    H = (P[pick] + rng.normal(scale=scene.info.h, size=(5000, 3))).astype(np.float32)
+    Nn = N[pick].astype(np.float32)
+    out = torch.empty(5000, dtype=torch.int32, device="cuda")
+    rea.hash_point(scene, torch.from_numpy(H).cuda(), torch.from_numpy(Nn).cuda(), out)
+    exp = np.array([po.lib.orc_hash_point(osc.state(0), po.ptr(H[k]), po.ptr(Nn[k])) for k in range(5000)], np.int32)
+    assert np.array_equal(out.cpu().numpy(), exp)
+    Aseg = H
+    Bseg = P[rng.integers(0, len(P), 5000)].astype(np.float32)
+    dm = torch.empty(5000 * scene.info.nw, dtype=torch.int32, device="cuda")
+    rea.dda_mark(scene, torch.from_numpy(Aseg).cuda(), torch.from_numpy(Bseg).cuda(), dm)
+    em = np.zeros(5000 * scene.info.nw, np.uint32)
+    po.lib.orc_dda_segments(osc.state(0), po.ptr(Aseg), po.ptr(Bseg), 5000, po.ptr(em))
+    assert np.array_equal(dm.cpu().numpy().view(np.uint32), em)
+
+
+def test_hash_insert_lookup(torch_gpu, street):
+    torch = torch_gpu
+    scene, _ = street
+    HKh = scene.array("HK")
+    keys = HKh[HKh != 0]
+    HVh = scene.array("HV")[HKh != 0]
+    rng = np.random.default_rng(1)
+    perm = rng.permutation(len(keys))
+    # duplicates: every key inserted twice with two values, the smaller must win
+    k2 = np.concatenate([keys[perm], keys])
+    v2 = np.concatenate([HVh[perm] + 1000000, HVh]).astype(np.int32)
+    size = len(HKh)
+    HK = torch.zeros(size, dtype=torch.int64, device="cuda")
+    HV = torch.full((size,), 2**31 - 1, dtype=torch.int32, device="cuda")
+    fail = torch.zeros(1, dtype=torch.int32, device="cuda")
+    rea.hash_insert(HK, HV, torch.from_numpy(k2.view(np.int64)).cuda(), torch.from_numpy(v2).cuda(), fail)
+    absent = (keys[:2000] ^ np.uint64(1 << 60)).astype(np.uint64)
+    qk = np.concatenate([keys, absent])
+    out = torch.empty(len(qk), dtype=torch.int32, device="cuda")
+    rea.hash_query(HK, HV, torch.from_numpy(qk.view(np.int64)).cuda(), out)
+    o = out.cpu().numpy()
+    assert int(fail.item()) == 0
+    assert np.array_equal(o[: len(keys)], HVh)
+    assert np.all(o[len(keys):] == -1)
+    hk = HK.cpu().numpy().view(np.uint64)
+    assert np.array_equal(np.sort(hk[hk != 0]), np.sort(keys))
+    # linear-probing invariant: no empty slot between a key's home slot and its slot
+    mask = size - 1
+    occ = hk != 0
+    for slot in np.nonzero(occ)[0][:5000]:
+        home = int(po.lib.orc_mix64(int(hk[slot]))) & mask
+        k = home
+        while k != slot:
+            assert occ[k]
+            k = (k + 1) & mask
+
+
+RUN_METHODS = [
+    ("plain", lambda: rea.methods.plain()),
+    ("targeted", lambda: rea.methods.targeted()),
+    ("flush_snapshots", lambda: rea.methods.flush_snapshots()),
+] + [(n, (lambda m=m: m)) for n, m in rea.methods.timing_list()] + [
+    (n, (lambda m=m: m)) for n, m in rea.methods.scenario_list(dyn=0b11) if "Oracle" not in n]
+
+SEQ = {"room": [0] * 22 + [1] * 22 + [0] * 22, "street": [0] * 21 + [3] * 12 + [2] * 12 + [0] * 12 + [1] * 6}
+
+
+@pytest.mark.parametrize("name,mk", RUN_METHODS, ids=[n for n, _ in RUN_METHODS])
+@pytest.mark.parametrize("which", ["room", "street"])
+def test_runner_bit_exact(torch_gpu, which, name, mk, request):
+    scene, osc = request.getfixturevalue(which)
+    m = mk()
+    cache = rea.Cache(scene, m)
+    orun = po.OracleRunner(osc, m)
+    seq = SEQ[which]
+    cache.begin(seq[0], 77)
+    orun.begin(seq[0], 77)
+    for f, st in enumerate(seq):
+        fs = cache.frame(st)
+        _, inv = orun.frame(st)
+        assert fs.invalidated == inv, f"frame {f}: invalidated {fs.invalidated} vs {inv}"
+        for arr in ("LC", "N", "KK", "PENDING", "CUR", "LCUR", "SUMPR", "NPEND", "AGE", "STALE"):
+            g, e = cache.get(arr), orun.get(arr)
+            if g.dtype == np.float32:
+                g, e = g.view(np.uint32), e.view(np.uint32)
+            assert np.array_equal(g, e), f"{name} frame {f} state {st}: {arr} differs in {np.sum(g != e)} entries"
+
+
+def test_query_bit_exact(torch_gpu, street):
+    torch = torch_gpu
+    scene, osc = street
+    m = rea.methods.plain()
+    cache = rea.Cache(scene, m)
+    orun = po.OracleRunner(osc, m)
+    cache.begin(0, 9)
+    orun.begin(0, 9)
+    for _ in range(3):
+        cache.frame(0)
+        orun.frame(0)
+    cam = scene.camera()
+    img = torch.zeros(cam.W * cam.H, dtype=torch.float32, device="cuda")
+    for no_cache in (0, 1):
+        rays = cache.render(img, cam, no_cache=bool(no_cache), seed=4, count_rays=True)
+        lc = orun.get("LC")
+        exp = np.zeros(cam.W * cam.H, np.float32)
+        er = (C.c_longlong * 2)()
+        po.lib.orc_k_query(osc.state(0), 1, None if no_cache else po.ptr(lc), C.byref(cam), 4, 3, 0, po.ptr(exp), er, 0,
+                           cam.W * cam.H)
+        got = img.cpu().numpy()
+        assert np.array_equal(got.view(np.uint32), exp.view(np.uint32)), f"no_cache={no_cache}: {np.sum(got != exp)} pixels differ"
+        assert tuple(rays) == (er[0], er[1])
+
+
+def test_radiance_vs_fp64(torch_gpu, room):
+    """FP32 cache radiance after several frames within 1e-4 relative RMS of the FP64 twin
+    driven by the same rays and hits."""
+    scene, osc = room
+    i = scene.info
+    m = rea.methods.plain()
+    cache = rea.Cache(scene, m)
+    cache.begin(0, 21)
+    s = osc.states[0]
+    E64 = np.zeros(i.Np * i.nl)
+    po.lib.orc_k_build_f64(C.byref(s), 1, po.ptr(E64))
+    # FP32 oracle state that the GPU must equal, plus the FP64 twin
+    orun = po.OracleRunner(osc, m)
+    orun.begin(0, 21)
+    L64 = orun.get("LC").astype(np.float64)
+    kmax = 4
+    for f in range(8):
+        n_before = orun.get("N")
+        # record this frame's hits with a standalone oracle trace2 on a copy of the runner state
+        hits = np.full(i.Np * kmax, -2, np.int32)
+        c = po.lib.orc_runner_cache(orun.h).contents
+        p = po.OrcParams()
+        p.Np, p.NR, p.mode, p.budget, p.nmax, p.seed, p.frame = i.Np, i.NR, 0, 1.0, m.nmax, 21, f
+        p.nmax_resp, p.rays, p.nl = m.nmax_resp, 1.0, i.nl
+        mcopy = np.zeros(i.Np * i.nl, np.float32)
+        kk = np.zeros(i.Np, np.int32)
+        tmp = po.OrcCache()
+        for fld, _ in po.OrcCache._fields_:
+            setattr(tmp, fld, getattr(c, fld))
+        tmp.m = mcopy.ctypes.data
+        tmp.kk = kk.ctypes.data
+        po.lib.orc_k_trace2(C.byref(s), 1, 0, C.byref(tmp), C.byref(p), None, po.ptr(hits), kmax)
+        m64 = np.zeros(i.Np * i.nl)
+        po.lib.orc_k_trace2_f64(C.byref(s), po.ptr(E64), po.ptr(L64), po.ptr(kk), po.ptr(hits), kmax, 0, po.ptr(m64))
+        out64 = np.zeros_like(L64)
+        po.lib.orc_k_update_f64(C.byref(s), po.ptr(n_before), po.ptr(kk), C.byref(p), po.ptr(L64), po.ptr(m64), po.ptr(out64))
+        L64 = out64
+        cache.frame(0)
+        orun.frame(0)
+    g = cache.get("LC").astype(np.float64)
+    rel = np.sqrt(np.mean((g - L64) ** 2)) / np.mean(np.abs(L64))
+    assert rel < 1e-4, f"relative RMS vs FP64 = {rel:.3e}"

