@@ -40,25 +40,62 @@ def make_scene(rea, cfg):
 
 
 def update_bytes_per_entry(nl, tracked=False):
-    """k_update algorithmic bytes per entry: read kk, n, Lc[nl], m[nl]; write Lc2[nl], n"""
-    return 4 + 4 + 4 * nl + 4 * nl + 4 * nl + 4 + (8 if tracked else 0)
+    """k_update algorithmic bytes per entry: read kk, n, albedo, Lc[nl], m[nl], Edir[nl];
+    write Lc2[nl], n, R"""
+    return 4 + 4 + 4 + 16 * nl + 4 + 4 + (8 if tracked else 0)
 
 
-def cpu_baseline(scene, cfg, seconds, log):
-    """The C oracle on the host cores: the same frame (entry pass + update + query) on the same
-    scene, state-0 build arrays taken from the GPU build (bit-identical), timed per frame."""
+def oracle_scene(scene):
+    """the CPU oracle's view of the scene, with the GPU-built (bit-identical) state arrays"""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import ctypes as C
-
-    import numpy as np
     import pyoracle as po
-    import render_engine_amd as rea
 
     osc = po.OracleScene(scene, build=False)
     for st in range(scene.info.NS):
         osc.Edir[st][:] = scene.array("EDIR", st)
         osc.SV[st][:] = scene.array("SV", st)
         osc.LM[st][:] = scene.array("LM", st)
+    return osc
+
+
+def query_bytes(osc, R, cam, seed, frame, log):
+    """Algorithmic bytes of one query launch: per ray 32 B per binary-BVH node visited and 36 B
+    per triangle tested, 8 B per hash probe, 4 B per resolved-radiance read, plus the image;
+    counted by the CPU oracle on the same paths over 1/8 of the rows (8 bands) and scaled."""
+    import ctypes as C
+
+    import numpy as np
+    import pyoracle as po
+
+    W, H = cam.W, cam.H
+    band = max(1, H // 64)
+    img = np.zeros(W * H, np.float32)
+    po.lib.orc_count_enable(1)
+    sampled = 0
+    for b in range(8):
+        r0 = b * H // 8
+        r1 = min(H, r0 + band)
+        po.lib.orc_k_query(osc.state(0), 1, po.ptr(R), C.byref(cam), seed, frame, 0, po.ptr(img), None, r0 * W, r1 * W)
+        sampled += (r1 - r0) * W
+    cnt = np.zeros(4, np.int64)
+    po.lib.orc_count_get(po.ptr(cnt))
+    po.lib.orc_count_enable(0)
+    scale = W * H / sampled
+    per = dict(nodes=cnt[0] * scale, tris=cnt[1] * scale, probes=cnt[2] * scale, resolved=cnt[3] * scale)
+    total = 32 * per["nodes"] + 36 * per["tris"] + 8 * per["probes"] + 4 * per["resolved"] + 4 * W * H
+    log(f"query work per launch (oracle count, 1/{scale:.0f} of rows): {json.dumps({k: round(v) for k, v in per.items()})}")
+    return float(total), per
+
+
+def cpu_baseline(osc, scene, seconds, log):
+    """The C oracle on the host cores: the same frame (entry pass + update + query) on the same
+    scene in state 0, timed per frame."""
+    import ctypes as C
+
+    import numpy as np
+    import pyoracle as po
+    import render_engine_amd as rea
+
     m = rea.methods.plain()
     run = po.OracleRunner(osc, m)
     run.begin(0, 1234)
@@ -68,14 +105,13 @@ def cpu_baseline(scene, cfg, seconds, log):
     while frames < 1 or (t_tot < seconds and frames < 50):
         t0 = time.perf_counter()
         run.frame(0)
-        lc = run.get("LC")   # a copy, outside the timed part below
         t1 = time.perf_counter()
+        R = run.get("R")   # a copy, outside the timed parts
         er = (C.c_longlong * 2)()
         t2 = time.perf_counter()
-        po.lib.orc_k_query(osc.state(0), 1, po.ptr(lc), C.byref(cam), 99, frames, 0, po.ptr(img), er, 0, cam.W * cam.H)
+        po.lib.orc_k_query(osc.state(0), 1, po.ptr(R), C.byref(cam), 99, frames, 0, po.ptr(img), er, 0, cam.W * cam.H)
         t3 = time.perf_counter()
-        kk = run.get("KK")
-        rays += int(kk.sum()) + er[0] + er[1]
+        rays += int(run.get("KK").sum()) + er[0] + er[1]
         t_tot += (t1 - t0) + (t3 - t2)
         frames += 1
     log(f"cpu baseline: {frames} frames in {t_tot:.1f} s on {po.lib.orc_num_threads()} threads")
@@ -173,18 +209,24 @@ def main():
     }
     dom = max(kernels, key=lambda k: kernels[k]["ms"])
     log(f"kernels: {json.dumps(kernels)}")
-    # roofline: traversal kernels are priced by the bytes of the nodes and triangles the rays
-    # visit (counted on the CPU oracle for the same rays); k_update by its streaming bytes
-    roof = None
+    osc = oracle_scene(scene)
+    R = cache.get("R")
+    qb, qwork = query_bytes(osc, R, cam, seed + args.warmup + args.steps - 1, args.warmup + args.steps, log)
+    kernels["k_query"].update({"alg_bytes": qb, "GB_s": qb / t_q / 1e6, "work": qwork})
     if dom == "k_update":
-        roof = {"bound": "hbm", "achieved": upd_bytes / t_up / 1e6, "peak": 8000.0, "unit": "GB/s",
-                "frac": upd_bytes / t_up / 1e6 / 8000.0, "traffic": None, "kernel": dom}
+        ach = upd_bytes / t_up / 1e6
+    elif dom == "k_query":
+        ach = qb / t_q / 1e6
     else:
-        roof = {"bound": "hbm", "achieved": None, "peak": 8000.0, "unit": "GB/s", "frac": None, "traffic": None, "kernel": dom}
+        ach = None
+    roof = {"bound": "hbm", "achieved": ach, "peak": 8000.0, "unit": "GB/s", "frac": (ach / 8000.0) if ach else None,
+            "traffic": None, "kernel": dom,
+            "basis": ("k_query: 32 B/binary node + 36 B/triangle + 8 B/hash probe + 4 B/resolved read per path, oracle-counted"
+                      if dom == "k_query" else "k_update: 12 B + 12 B x lights per entry")}
     cpu = None
     if not args.no_cpu:
         try:
-            cpu = cpu_baseline(scene, cfg, args.cpu_seconds, log)
+            cpu = cpu_baseline(osc, scene, args.cpu_seconds, log)
         except Exception as e:  # the baseline is reported, not required
             log(f"cpu baseline failed: {e!r}")
     out = {

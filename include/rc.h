@@ -83,11 +83,11 @@ enum {
     RC_A_EDIR, RC_A_SV, RC_A_LM, RC_A_OCC, RC_A_EM, RC_A_REF, RC_A_VALID, RC_A_PW, RC_A_ORDER, RC_A_OBJBOX,
     RC_A_MESH_V, RC_A_MESH_ALB, RC_A_MESH_DS, RC_A_COUNT
 };
-/* cache arrays (rc_cache_get / rc_cache_set) */
+/* cache arrays (rc_cache_get / rc_cache_set); M is SoA [nl][Np], LC / EDIR rows are AoS [Np][nl] */
 enum {
     RC_C_LC = 0, RC_C_M, RC_C_N, RC_C_KK, RC_C_PENDING, RC_C_CUR, RC_C_PREV, RC_C_LCUR, RC_C_LPREV,
     RC_C_SUMPR, RC_C_NPEND, RC_C_AGE, RC_C_STALE, RC_C_C0, RC_C_RSTAT, RC_C_LNEW, RC_C_DSTAT,
-    RC_C_COUNT
+    RC_C_R, RC_C_COUNT
 };
 
 const char* rc_last_error(void);

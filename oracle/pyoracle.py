@@ -25,7 +25,7 @@ class OrcScene(C.Structure):
 
 
 class OrcCache(C.Structure):
-    _fields_ = [(n, _P) for n in ("Lc", "Lc2", "m", "n", "c0", "kk", "pending", "cur", "prev", "lcur", "lprev", "rstat",
+    _fields_ = [(n, _P) for n in ("Lc", "Lc2", "n", "c0", "m", "R", "kk", "pending", "cur", "prev", "lcur", "lprev", "rstat",
                                   "dstat", "stale", "age", "sumpr", "npend", "err")]
 
 
@@ -79,6 +79,9 @@ def _load():
     lib.orc_k_trace2.argtypes = [C.POINTER(OrcScene), C.c_int, C.c_int, C.POINTER(OrcCache), C.POINTER(OrcParams), _P, _P, C.c_int]
     lib.orc_k_update.argtypes = [C.POINTER(OrcScene), C.POINTER(OrcCache), C.POINTER(OrcParams), _P]
     lib.orc_k_ref.argtypes = [C.POINTER(OrcScene), C.c_int, _P, _P, _P, _P, C.c_uint32, C.c_int, C.c_int, C.c_int]
+    lib.orc_k_resolve.argtypes = [C.POINTER(OrcScene), _P, _P, C.c_uint64]
+    lib.orc_count_enable.argtypes = [C.c_int]
+    lib.orc_count_get.argtypes = [_P]
     lib.orc_k_query.argtypes = [C.POINTER(OrcScene), C.c_int, _P, _P, C.c_uint32, C.c_uint32, C.c_uint64, _P, _P, C.c_int, C.c_int]
     lib.orc_k_build_f64.argtypes = [C.POINTER(OrcScene), C.c_int, _P]
     lib.orc_k_trace2_f64.argtypes = [C.POINTER(OrcScene), _P, _P, _P, _P, C.c_int, C.c_uint64, _P]
@@ -193,11 +196,11 @@ class OracleRunner:
         spec = {"LC": (np.float32, nl * Np), "M": (np.float32, nl * Np), "N": (np.float32, Np), "KK": (np.int32, Np),
                 "PENDING": (np.uint8, Np), "CUR": (np.uint32, nw * Np), "PREV": (np.uint32, nw * Np), "LCUR": (np.uint64, Np),
                 "LPREV": (np.uint64, Np), "SUMPR": (np.int64, 2), "NPEND": (np.int32, 2), "AGE": (np.float32, Np),
-                "STALE": (np.uint64, Np), "C0": (np.float32, Np)}
+                "STALE": (np.uint64, Np), "C0": (np.float32, Np), "R": (np.float32, Np)}
         dt, n = spec[name]
         field = {"LC": "Lc", "M": "m", "N": "n", "KK": "kk", "PENDING": "pending", "CUR": "cur", "PREV": "prev",
                  "LCUR": "lcur", "LPREV": "lprev", "SUMPR": "sumpr", "NPEND": "npend", "AGE": "age", "STALE": "stale",
-                 "C0": "c0"}[name]
+                 "C0": "c0", "R": "R"}[name]
         addr = getattr(c, field)
         buf = (C.c_char * (n * np.dtype(dt).itemsize)).from_address(addr)
         return np.frombuffer(buf, dt).copy()

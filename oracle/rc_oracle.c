@@ -27,6 +27,12 @@ static inline int imin(int a, int b) { return a < b ? a : b; }
 static inline int imax(int a, int b) { return a > b ? a : b; }
 static inline long long fx(float v, float scale) { return (long long)(v * scale); }
 
+static int g_count = 0;
+static long long g_cnt[4];
+#define CNT(k, v) do { if (g_count) __atomic_fetch_add(&g_cnt[k], (long long)(v), __ATOMIC_RELAXED); } while (0)
+void orc_count_enable(int on) { g_count = on; if (on) memset(g_cnt, 0, sizeof g_cnt); }
+void orc_count_get(long long out[4]) { for (int k = 0; k < 4; ++k) out[k] = g_cnt[k]; }
+
 int orc_num_threads(void) {
 #ifdef _OPENMP
     return omp_get_max_threads();
@@ -115,6 +121,7 @@ int orc_hash_find(const uint64_t* HK, const int32_t* HV, uint32_t hmask, uint64_
     uint32_t h = (uint32_t)orc_mix64(key) & hmask;
     for (int probe = 0; probe < 64; ++probe) {
         uint64_t k = HK[h];
+        CNT(2, 1);
         if (k == key) return HV[h];
         if (k == 0ull) return -1;
         h = (h + 1) & hmask;
@@ -318,7 +325,9 @@ static void bvh_closest(const orc_scene* s, v3 o, v3 d, float* tmin, int* q) {
     int stack[128]; int sp = 0, node = 0;
     for (;;) {
         const orc_node* nd = &s->BN[node];
+        CNT(0, 1);
         if (nd->cnt > 0) {
+            CNT(1, nd->cnt);
             for (int j = nd->lf; j < nd->lf + nd->cnt; ++j) {
                 float t = tri_static(o, d, s->ST + 9 * (size_t)j);
                 if (closer(t, j, *tmin, *q)) { *tmin = t; *q = -2 - j; }
@@ -521,7 +530,8 @@ void orc_k_trace2(const orc_scene* s, int trav, int deps, const orc_cache* c, co
         float c0 = pend ? c->c0[i] : 0.f;
         int rg = pend ? s->region[i] : 0;
         float sd = 0.f, sq = 0.f;
-        float* mrow = c->m + (size_t)nl * i;
+        float* mcol = c->m + i;              /* SoA: light l at mcol[l * Np] */
+        const size_t ms = (size_t)p->Np;
         for (int r = 0; r < k; ++r) {
             float u1 = orc_rnd(&rs), u2 = orc_rnd(&rs);
             v3 D = cosine_dir(N, u1, u2);
@@ -539,11 +549,11 @@ void orc_k_trace2(const orc_scene* s, int trav, int deps, const orc_cache* c, co
                 for (int l = 0; l < nl; ++l) {
                     float lv = dormant_l(p->dormant, l) ? 0.f : lc[l];
                     float v = A * (ed[l] + lv);
-                    if (r == 0) mrow[l] = v; else mrow[l] += v;
+                    if (r == 0) mcol[l * ms] = v; else mcol[l * ms] += v;
                     val += v;
                 }
             } else if (r == 0) {
-                for (int l = 0; l < nl; ++l) mrow[l] = 0.f;
+                for (int l = 0; l < nl; ++l) mcol[l * ms] = 0.f;
             }
             if (deps) {
                 int src = back ? i : pi;
@@ -570,9 +580,9 @@ void orc_k_trace2(const orc_scene* s, int trav, int deps, const orc_cache* c, co
         float ik = k > 0 ? 1.f / (float)k : 0.f;
         float sa = 0.f;
         if (p->ddgi && k > 0)
-            for (int l = 0; l < nl; ++l) sa += mrow[l];
-        if (k == 0) { for (int l = 0; l < nl; ++l) mrow[l] = 0.f; }
-        else if (k > 1) { for (int l = 0; l < nl; ++l) mrow[l] = mrow[l] * ik; }
+            for (int l = 0; l < nl; ++l) sa += mcol[l * ms];
+        if (k == 0) { for (int l = 0; l < nl; ++l) mcol[l * ms] = 0.f; }
+        else if (k > 1) { for (int l = 0; l < nl; ++l) mcol[l * ms] = mcol[l * ms] * ik; }
         if (p->ddgi && k > 0) {
             int rgd = s->region[i];
             long long d0 = fx(sa * ik, ORC_FX_STAT), d1 = fx(est_of(c->Lc, i, nl, p->dormant), ORC_FX_STAT), d2 = fx(1.f, ORC_FX_STAT);
@@ -623,6 +633,7 @@ void orc_k_decide(const orc_scene* s, const orc_cache* c, const orc_params* p) {
 
 void orc_k_update(const orc_scene* s, const orc_cache* c, const orc_params* p, const uint64_t* lnew) {   /* :562-602 */
     const int nl = p->nl;
+    const size_t Np = (size_t)p->Np;
     long long tot = 0;
 #pragma omp parallel for schedule(static) reduction(+ : tot)
     for (int i = 0; i < p->Np; ++i) {
@@ -631,9 +642,11 @@ void orc_k_update(const orc_scene* s, const orc_cache* c, const orc_params* p, c
         float nmx = p->nmax;
         if (p->ramp) { float a = c->age[i]; nmx = fminf(p->nmax, (4.f + 0.5f * a) * p->rays); c->age[i] = fminf(a + 1.f, 1e4f); }
         float alpha = k > 0 ? fminf((float)k / fminf(fmaxf(nn, 1.f), nmx), 1.f) : 0.f;
+        float alpha_r = k > 0 ? fminf((float)k / fminf(fmaxf(nn, 1.f), p->nmax_resp), 1.f) : 0.f;
         const float* a_ = c->Lc + (size_t)nl * i;
-        const float* m_ = c->m + (size_t)nl * i;
         float* o_ = c->Lc2 + (size_t)nl * i;
+        const float* ed = s->Edir + (size_t)nl * i;
+        float A = s->Alb[i], val = 0.f;
         if (p->ddgi) {
             const long long* d3 = c->dstat + 3 * s->region[i];
             const float isc = 1.0f / ORC_FX_STAT;
@@ -645,22 +658,43 @@ void orc_k_update(const orc_scene* s, const orc_cache* c, const orc_params* p, c
                 if (p->ddgi_clamp && rel > 0.10f) scale = 0.25f;
             }
             float w = k > 0 ? (1.f - h) * scale : 0.f;
-            for (int l = 0; l < nl; ++l) { float a = a_[l]; o_[l] = dormant_l(p->dormant, l) ? a : a + w * (m_[l] - a); }
-        } else {
-            float alpha_r = k > 0 ? fminf((float)k / fminf(fmaxf(nn, 1.f), p->nmax_resp), 1.f) : 0.f;
             for (int l = 0; l < nl; ++l) {
-                float a = a_[l];
+                float a = a_[l], mm = c->m[l * Np + i];
+                int dm = dormant_l(p->dormant, l);
+                float o = dm ? a : a + w * (mm - a);
+                o_[l] = o;
+                val += A * (ed[l] + (dm ? 0.f : o));
+            }
+        } else {
+            for (int l = 0; l < nl; ++l) {
+                float a = a_[l], mm = c->m[l * Np + i];
                 float al = (l < 64 && ((p->resp_mask >> l) & 1ull)) ? alpha_r : alpha;
-                if (dormant_l(p->dormant, l)) al = 0.f;
-                o_[l] = a + al * (m_[l] - a);
+                int dm = dormant_l(p->dormant, l);
+                if (dm) al = 0.f;
+                float o = a + al * (mm - a);
+                o_[l] = o;
+                val += A * (ed[l] + (dm ? 0.f : o));
             }
         }
+        c->R[i] = val;
         n = fminf(nn, nmx);
         c->n[i] = n;
         tot += fx(1.f / (n + 1.f), ORC_FX_PR);
         if (lnew) { uint64_t x = lnew[i]; if (x) c->lcur[i] |= x; }
     }
     c->sumpr[1] += tot;
+}
+
+void orc_k_resolve(const orc_scene* s, const float* Lc, float* R, uint64_t dormant) {
+    const int nl = s->nl;
+    for (int i = 0; i < s->Np; ++i) {
+        float A = s->Alb[i], val = 0.f;
+        for (int l = 0; l < nl; ++l) {
+            float lv = dormant_l(dormant, l) ? 0.f : Lc[(size_t)nl * i + l];
+            val += A * (s->Edir[(size_t)nl * i + l] + lv);
+        }
+        R[i] = val;
+    }
 }
 
 void orc_k_sumpr(const orc_cache* c, int Np) {
@@ -795,15 +829,7 @@ static float nee(const orc_scene* s, int trav, v3 H, v3 n, uint32_t* rs, uint64_
     float rr = fmaxf(r, 0.05f);
     return (float)nl * (I * c / (rr * rr));
 }
-static float cached(const orc_scene* s, const float* Lc, int e, uint64_t dormant) {
-    const int nl = s->nl;
-    float A = s->Alb[e], val = 0.f;
-    const float* ed = s->Edir + (size_t)nl * e;
-    const float* lc = Lc + (size_t)nl * e;
-    for (int l = 0; l < nl; ++l) { float lv = dormant_l(dormant, l) ? 0.f : lc[l]; val += A * (ed[l] + lv); }
-    return val;
-}
-void orc_k_query(const orc_scene* s, int trav, const float* Lc, const orc_camera* cam, uint32_t seed, uint32_t frame,
+void orc_k_query(const orc_scene* s, int trav, const float* R, const orc_camera* cam, uint32_t seed, uint32_t frame,
                  uint64_t dormant, float* img, long long* rays, int pix0, int pix1) {
     long long ext_tot = 0, sh_tot = 0;
     if (pix1 > cam->W * cam->H) pix1 = cam->W * cam->H;
@@ -831,7 +857,7 @@ void orc_k_query(const orc_scene* s, int trav, const float* Lc, const orc_camera
                 v3 H = add(o, mul(d, t)), n;
                 int e;
                 if (resolve_hit_n(s, H, d, q, &e, &n)) break;
-                if (Lc && b == cam->cache_bounce) { rad += thr * cached(s, Lc, e, dormant); break; }
+                if (R && b == cam->cache_bounce) { CNT(3, 1); rad += thr * R[e]; break; }
                 float A = s->Alb[e];
                 rad += thr * (A * nee(s, trav, H, n, &rs, dormant, &sh));
                 if (b >= cam->bounces) break;
@@ -947,6 +973,7 @@ orc_runner* orc_runner_create(const orc_scene* states, int NS, const orc_method*
     size_t Np = s->Np, nl = s->nl, nw = s->nw, NR = s->NR;
     orc_cache* c = &r->c;
     c->Lc = (float*)calloc(nl * Np, 4); c->Lc2 = (float*)calloc(nl * Np, 4); c->m = (float*)calloc(nl * Np, 4);
+    c->R = (float*)calloc(Np, 4);
     c->n = (float*)calloc(Np, 4); c->c0 = (float*)calloc(Np, 4); c->kk = (int32_t*)calloc(Np, 4);
     c->pending = (uint8_t*)calloc(Np, 1);
     c->cur = (uint32_t*)calloc(nw * Np, 4); c->prev = (uint32_t*)calloc(nw * Np, 4);
@@ -961,7 +988,7 @@ orc_runner* orc_runner_create(const orc_scene* states, int NS, const orc_method*
 void orc_runner_destroy(orc_runner* r) {
     if (!r) return;
     orc_cache* c = &r->c;
-    free(c->Lc); free(c->Lc2); free(c->m); free(c->n); free(c->c0); free(c->kk); free(c->pending); free(c->cur); free(c->prev);
+    free(c->Lc); free(c->Lc2); free(c->m); free(c->R); free(c->n); free(c->c0); free(c->kk); free(c->pending); free(c->cur); free(c->prev);
     free(c->lcur); free(c->lprev); free(c->rstat); free(c->dstat); free(c->stale); free(c->age); free(c->sumpr); free(c->npend);
     free(c->err); free(r->lnew); free(r->snap); free(r);
 }

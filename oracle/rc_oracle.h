@@ -43,7 +43,9 @@ typedef struct orc_scene {
 } orc_scene;
 
 typedef struct orc_cache {
-    float *Lc, *Lc2, *m, *n, *c0;
+    float *Lc, *Lc2, *n, *c0;
+    float* m;        /* [nl][Np] SoA sample means */
+    float* R;        /* [Np] resolved radiance A * sum_l (Edir_l + Lc_l) after the update */
     int32_t* kk;
     uint8_t* pending;
     uint32_t *cur, *prev;
@@ -123,9 +125,15 @@ void orc_k_flush(const orc_cache* c, int Np, int nl);
 void orc_k_error(const orc_cache* c, int Np, int nl, const float* ref, const uint8_t* valid, uint64_t dormant);
 void orc_k_ref(const orc_scene* s, int trav, const float* Lin, float* Lout, float* acc, float* closec, uint32_t seed, int K,
                int accumulate, int count_close);
-/* camera pass (Lc NULL: no cache); rays[2] += extension, shadow rays */
-void orc_k_query(const orc_scene* s, int trav, const float* Lc, const orc_camera* cam, uint32_t seed, uint32_t frame,
+/* resolved radiance of every entry from channels Lc */
+void orc_k_resolve(const orc_scene* s, const float* Lc, float* R, uint64_t dormant);
+/* camera pass (R NULL: no cache); rays[2] += extension, shadow rays */
+void orc_k_query(const orc_scene* s, int trav, const float* R, const orc_camera* cam, uint32_t seed, uint32_t frame,
                  uint64_t dormant, float* img, long long* rays, int pix0, int pix1);
+/* traversal-work counters (binary nodes visited, triangles tested, hash probes, cache rows
+ * read) accumulated by the closest-hit traversal and the lookups while enabled */
+void orc_count_enable(int on);
+void orc_count_get(long long out[4]);
 
 /* ---- FP64 radiance twin ---- */
 /* direct light in double from the FP32 build's visibility (Edir > 0 or lit geometry) */

@@ -25,7 +25,8 @@ _DT = {"ST": np.float32, "STN": np.float32, "HK": np.uint64, "HV": np.int32, "P"
        "MESH_DS": np.uint8}
 _CDT = {"LC": np.float32, "M": np.float32, "N": np.float32, "KK": np.int32, "PENDING": np.uint8, "CUR": np.uint32,
         "PREV": np.uint32, "LCUR": np.uint64, "LPREV": np.uint64, "SUMPR": np.int64, "NPEND": np.int32, "AGE": np.float32,
-        "STALE": np.uint64, "C0": np.float32, "RSTAT": np.int64, "LNEW": np.uint64, "DSTAT": np.int64}
+        "STALE": np.uint64, "C0": np.float32, "RSTAT": np.int64, "LNEW": np.uint64, "DSTAT": np.int64,
+        "R": np.float32}
 
 
 def device_count():

@@ -55,7 +55,7 @@ A_NAMES = ["ST", "STN", "BN", "WN", "HK", "HV", "P", "N", "ALB", "REGION", "L", 
            "MESH_DS"]
 A = {n: i for i, n in enumerate(A_NAMES)}
 C_NAMES = ["LC", "M", "N", "KK", "PENDING", "CUR", "PREV", "LCUR", "LPREV", "SUMPR", "NPEND", "AGE", "STALE", "C0", "RSTAT",
-           "LNEW", "DSTAT"]
+           "LNEW", "DSTAT", "R"]
 CA = {n: i for i, n in enumerate(C_NAMES)}
 
 # exported symbols and their signatures: (restype, argtypes)

@@ -212,16 +212,21 @@ extern "C" rc_status rc_scene_upload(rc_scene* s, int trav) {
     rc::Scene& H = s->H;
     HIPCK(hipStreamCreate(&s->stream));
     rc_dscene& d = s->ds;
-    float *dST, *dSTN, *dL, *dA; rc_bvh_node* dBN; rc_wnode* dWN; uint64_t* dHK; int *dHV, *dLT, *dR;
-    HIPCK(dput(&dST, H.ST, s->owned)); HIPCK(dput(&dSTN, H.STN, s->owned)); HIPCK(dput(&dBN, H.BN, s->owned));
+    float *dST, *dST4, *dSTN, *dL, *dA; rc_bvh_node* dBN; rc_wnode* dWN; uint64_t* dHK; int *dHV, *dLT, *dR;
+    std::vector<float> st4((size_t)12 * H.nst, 0.f);
+    for (int j = 0; j < H.nst; ++j)
+        for (int r = 0; r < 3; ++r)
+            for (int x = 0; x < 3; ++x) st4[12 * (size_t)j + 4 * r + x] = H.ST[9 * (size_t)j + 3 * r + x];
+    HIPCK(dput(&dST, H.ST, s->owned)); HIPCK(dput(&dST4, st4, s->owned)); HIPCK(dput(&dSTN, H.STN, s->owned)); HIPCK(dput(&dBN, H.BN, s->owned));
     HIPCK(dput(&dWN, H.WN, s->owned)); HIPCK(dput(&dHK, H.HK, s->owned)); HIPCK(dput(&dHV, H.HV, s->owned));
     HIPCK(dput(&dL, H.L, s->owned)); HIPCK(dput(&dLT, H.ltype, s->owned)); HIPCK(dput(&dA, H.A, s->owned));
     HIPCK(dput(&dR, H.region, s->owned));
-    d.ST = dST; d.STN = dSTN; d.BN = dBN; d.WN = dWN; d.HK = dHK; d.HV = dHV; d.L = dL; d.ltype = dLT; d.Alb = dA; d.region = dR;
+    d.ST = dST; d.ST4 = dST4; d.STN = dSTN; d.BN = dBN; d.WN = dWN; d.HK = dHK; d.HV = dHV; d.L = dL; d.ltype = dLT; d.Alb = dA; d.region = dR;
     d.hmask = (uint32_t)(H.HK.size() - 1); d.hcell = H.h;
     for (int a = 0; a < 3; ++a) { d.lo[a] = H.lo[a]; d.ext[a] = H.ext[a]; }
     d.sun_len = H.sun_len; d.nst = H.nst; d.nl = H.nl; d.Np = H.Np; d.NR = H.NR;
     d.vx = H.vx; d.vy = H.vy; d.vz = H.vz; d.nw = H.nw;
+    d.wstack = 3 * H.wide_depth + 1;
     int *dDims, *dOff;
     HIPCK(dput(&dDims, H.dims, s->owned)); HIPCK(dput(&dOff, H.off, s->owned));
     s->dst.resize(H.NS);
@@ -333,6 +338,7 @@ struct rc_cache {
     hipEvent_t ev[8]{};
     // runner state
     bool begun = false;
+    bool r_valid = false;         // R matches Lc of the last frame
     int prev_state = -1;
     int f = 0;
     uint32_t seed = 0;
@@ -363,6 +369,7 @@ extern "C" rc_status rc_cache_create(rc_scene* s, const rc_method* m, rc_cache**
 #define CK2(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) { delete c; return fail(RC_ERR_HIP, "%s: %s", #x, hipGetErrorString(e_)); } } while (0)
     CK2(dalloc(&d.Lc, nl * Np, c->owned)); CK2(dalloc(&d.Lc2, nl * Np, c->owned)); CK2(dalloc(&d.m, nl * Np, c->owned));
     CK2(dalloc(&d.n, Np, c->owned)); CK2(dalloc(&d.c0, Np, c->owned)); CK2(dalloc(&d.kk, Np, c->owned));
+    CK2(dalloc(&d.R, Np, c->owned));
     CK2(dalloc(&d.pending, Np, c->owned));
     CK2(dalloc(&d.cur, nw * Np, c->owned)); CK2(dalloc(&d.prev, nw * Np, c->owned));
     CK2(dalloc(&d.lcur, Np, c->owned)); CK2(dalloc(&d.lprev, Np, c->owned)); CK2(dalloc(&c->lnew, Np, c->owned));
@@ -403,6 +410,7 @@ extern "C" rc_status rc_cache_begin(rc_cache* c, int state0, uint32_t seed) {
     HIPCK(rc_launch_sumpr(&d, (int)Np, st));   // sum of 1/(n+1) at n = nmax
     HIPCK(hipStreamSynchronize(st));
     c->begun = true;
+    c->r_valid = false;
     c->prev_state = state0;
     c->f = 0;
     c->seed = seed;
@@ -569,9 +577,10 @@ extern "C" rc_status rc_cache_frame(rc_cache* c, int si, int want_err, int timed
     }
     if (timed) HIPCK(hipEventRecord(c->ev[3], st));
     HIPCK(hipMemsetAsync(d.sumpr + 1, 0, 8, st));
-    HIPCK(rc_launch_update(&S->ds, &d, &p, track ? c->lnew : nullptr, st));
+    HIPCK(rc_launch_update(&S->ds, &S->dst[si], &d, &p, track ? c->lnew : nullptr, st));
     HIPCK(hipMemcpyAsync(d.sumpr, d.sumpr + 1, 8, hipMemcpyDeviceToDevice, st));
     std::swap(d.Lc, d.Lc2);
+    c->r_valid = true;
     if (timed) {
         HIPCK(hipEventRecord(c->ev[6], st));
         HIPCK(hipEventSynchronize(c->ev[6]));
@@ -605,8 +614,12 @@ extern "C" rc_status rc_cache_render(rc_cache* c, const rc_camera* cam, int no_c
     int si = c->prev_state;
     unsigned long long* ctr = nullptr;
     if (rays_out) { HIPCK(hipMemsetAsync(c->d_counters, 0, 16, st)); ctr = c->d_counters; }
+    if (!no_cache && !c->r_valid) {   // no frame yet: resolve the warm-start channels
+        HIPCK(rc_launch_resolve(&S->ds, &S->dst[si], c->c.Lc, c->c.R, c->dormant, st));
+        c->r_valid = true;
+    }
     if (ms_out) HIPCK(hipEventRecord(c->ev[0], st));
-    HIPCK(rc_launch_query(S->trav, &S->ds, &S->dst[si], no_cache ? nullptr : c->c.Lc, cam, seed, (uint32_t)c->f, c->dormant, d_img, ctr, st));
+    HIPCK(rc_launch_query(S->trav, &S->ds, &S->dst[si], no_cache ? nullptr : c->c.R, cam, seed, (uint32_t)c->f, c->dormant, d_img, ctr, st));
     if (ms_out) {
         HIPCK(hipEventRecord(c->ev[1], st));
         HIPCK(hipEventSynchronize(c->ev[1]));
@@ -643,6 +656,7 @@ static bool cache_array(const rc_cache* c, int which, void** dp, size_t* bytes) 
         case RC_C_RSTAT: *dp = d.rstat; *bytes = 4 * NR * 8; return true;
         case RC_C_LNEW: *dp = c->lnew; *bytes = Np * 8; return true;
         case RC_C_DSTAT: *dp = d.dstat; *bytes = 3 * NR * 8; return true;
+        case RC_C_R: *dp = d.R; *bytes = Np * 4; return true;
         default: return false;
     }
 }

@@ -95,11 +95,21 @@ __device__ __forceinline__ bool rc_trace_lds(f3 o, f3 d, const float* T, const i
     return q >= 0;
 }
 
-// Static-triangle test of the BVH leaves (cuda/poc_gpu.cu:305-321). Returns t or -1.
-__device__ __forceinline__ float rc_tri_static(f3 o, f3 d, const float* __restrict__ Tp) {
-    f3 v0 = mk3(Tp[0], Tp[1], Tp[2]);
-    f3 e1 = mk3(Tp[3], Tp[4], Tp[5]);
-    f3 e2 = mk3(Tp[6], Tp[7], Tp[8]);
+// slab entry distance of one child box of a wide node (same arithmetic as rc_aabb_enter)
+__device__ __forceinline__ float rc_box_enter(float lx, float ly, float lz, float hx, float hy, float hz, f3 o, f3 inv, float tmax) {
+    float tx1 = (lx - o.x) * inv.x, tx2 = (hx - o.x) * inv.x;
+    float ty1 = (ly - o.y) * inv.y, ty2 = (hy - o.y) * inv.y;
+    float tz1 = (lz - o.z) * inv.z, tz2 = (hz - o.z) * inv.z;
+    float tn = fmaxf(fmaxf(fminf(tx1, tx2), fminf(ty1, ty2)), fmaxf(fminf(tz1, tz2), 0.f));
+    float tf = fminf(fminf(fmaxf(tx1, tx2), fmaxf(ty1, ty2)), fminf(fmaxf(tz1, tz2), tmax));
+    return tn <= tf ? tn : 1e30f;
+}
+
+// Static-triangle test of the BVH leaves (cuda/poc_gpu.cu:305-321), reading the 48-byte
+// aligned copy (v0, e1, e2 as three float4: three 16-byte loads). Returns t or -1.
+__device__ __forceinline__ float rc_tri_static4(f3 o, f3 d, const float4* __restrict__ T4) {
+    float4 a = T4[0], b = T4[1], c = T4[2];
+    f3 v0 = mk3(a.x, a.y, a.z), e1 = mk3(b.x, b.y, b.z), e2 = mk3(c.x, c.y, c.z);
     f3 p = cross3(d, e2);
     float det = dot3(e1, p);
     if (fabsf(det) <= 1e-12f) return -1.f;
@@ -139,7 +149,7 @@ __device__ __forceinline__ void rc_trace_bvh_ref(f3 o, f3 d, const rc_dscene& s,
         const rc_bvh_node nd = s.BN[node];
         if (nd.cnt > 0) {
             for (int j = nd.lf; j < nd.lf + nd.cnt; ++j) {
-                float t = rc_tri_static(o, d, s.ST + 9 * (int64_t)j);
+                float t = rc_tri_static4(o, d, (const float4*)s.ST4 + 3 * (int64_t)j);
                 if (t > 1e-4f && t < tmin) { tmin = t; q = -2 - j; }
             }
         } else {
@@ -171,7 +181,7 @@ __device__ __forceinline__ void rc_trace_bvh_closest(f3 o, f3 d, const rc_dscene
         const rc_bvh_node nd = s.BN[node];
         if (nd.cnt > 0) {
             for (int j = nd.lf; j < nd.lf + nd.cnt; ++j) {
-                float t = rc_tri_static(o, d, s.ST + 9 * (int64_t)j);
+                float t = rc_tri_static4(o, d, (const float4*)s.ST4 + 3 * (int64_t)j);
                 if (rc_closer(t, j, tmin, q)) { tmin = t; q = -2 - j; }
             }
         } else {
@@ -191,56 +201,70 @@ __device__ __forceinline__ void rc_trace_bvh_closest(f3 o, f3 d, const rc_dscene
     }
 }
 
-// 4-wide BVH traversal with the order-independent closest-hit rule. Leaf children are
-// intersected as soon as their node is visited; inner children are pushed far-to-near with
-// their entry distance and skipped on pop once that distance exceeds the current hit.
-__device__ __forceinline__ void rc_trace_wide_closest(f3 o, f3 d, const rc_dscene& s, float& tmin, int& q) {
+__device__ __forceinline__ void rc_cswap(float& ta, int& ia, float& tb, int& ib) {
+    if (tb < ta) { float t = ta; ta = tb; tb = t; int i = ia; ia = ib; ib = i; }
+}
+
+// Stack entries of the wide traversal are 32-bit: node index << 8 | an 8-bit code of the
+// node's entry distance t. The code keeps t's exponent and top 3 mantissa bits (t >= 2^-15),
+// and decodes to a lower bound of t (0 for code 0), so culling a popped node whose bound
+// exceeds the current hit never drops a node the exact distance would have kept.
+__device__ __forceinline__ uint32_t rc_tcode(float t) {
+    int c = (int)(__float_as_uint(t) >> 20) - (112 << 3) + 1;
+    return c < 1 ? 0u : (c > 255 ? 255u : (uint32_t)c);
+}
+__device__ __forceinline__ float rc_tlb(uint32_t code) {
+    return code == 0u ? 0.f : __uint_as_float(((code - 1u) + (112u << 3)) << 20);
+}
+
+// 4-wide BVH traversal with the order-independent closest-hit rule. Each node's four child
+// boxes are tested straight from its six float4 rows; leaf children are intersected at once;
+// inner children are ordered with a 4-input sorting network and pushed far-to-near. The
+// stack lives in LDS, one column per lane (lstk[level * 256 + lane]); the scene's wstack
+// levels (3 per wide level) bound its depth.
+__device__ __forceinline__ void rc_trace_wide_closest(f3 o, f3 d, const rc_dscene& s, float& tmin, int& q, uint32_t* lstk) {
+    uint32_t* stk = lstk + (threadIdx.x & 255);
     f3 inv = rc_inv_dir(d);
-    int stk[RC_WSTACK]; float stt[RC_WSTACK];
     int sp = 0, cur = 0;
     while (true) {
-        const rc_wnode* nd = s.WN + cur;
-        const float4* nb = (const float4*)nd;
-        float4 lx = nb[0], ly = nb[1], lz = nb[2], hx = nb[3], hy = nb[4], hz = nb[5];
-        int4 ch = ((const int4*)nd)[6], cc = ((const int4*)nd)[7];
-        float lox[4] = {lx.x, lx.y, lx.z, lx.w}, loy[4] = {ly.x, ly.y, ly.z, ly.w}, loz[4] = {lz.x, lz.y, lz.z, lz.w};
-        float hix[4] = {hx.x, hx.y, hx.z, hx.w}, hiy[4] = {hy.x, hy.y, hy.z, hy.w}, hiz[4] = {hz.x, hz.y, hz.z, hz.w};
-        int chv[4] = {ch.x, ch.y, ch.z, ch.w}, ccv[4] = {cc.x, cc.y, cc.z, cc.w};
-        float tc[4]; int ic[4]; int nh = 0;
-#pragma unroll
-        for (int c = 0; c < 4; ++c) {
-            if (chv[c] == -1) continue;
-            float lo[3] = {lox[c], loy[c], loz[c]}, hi[3] = {hix[c], hiy[c], hiz[c]};
-            float te = rc_aabb_enter(lo, hi, o, inv, tmin);
-            if (te >= 1e30f) continue;
-            if (chv[c] <= -2) {
-                int first = -2 - chv[c], cnt = ccv[c];
-                for (int j = first; j < first + cnt; ++j) {
-                    float t = rc_tri_static(o, d, s.ST + 9 * (int64_t)j);
-                    if (rc_closer(t, j, tmin, q)) { tmin = t; q = -2 - j; }
-                }
-            } else {
-                tc[nh] = te; ic[nh] = chv[c]; ++nh;
-            }
+        const float4* nb = (const float4*)(s.WN + cur);
+        const float4 lx = nb[0], ly = nb[1], lz = nb[2], hx = nb[3], hy = nb[4], hz = nb[5];
+        const int4 ch = ((const int4*)nb)[6], cc = ((const int4*)nb)[7];
+        float t0 = ch.x != -1 ? rc_box_enter(lx.x, ly.x, lz.x, hx.x, hy.x, hz.x, o, inv, tmin) : 1e30f;
+        float t1 = ch.y != -1 ? rc_box_enter(lx.y, ly.y, lz.y, hx.y, hy.y, hz.y, o, inv, tmin) : 1e30f;
+        float t2 = ch.z != -1 ? rc_box_enter(lx.z, ly.z, lz.z, hx.z, hy.z, hz.z, o, inv, tmin) : 1e30f;
+        float t3 = ch.w != -1 ? rc_box_enter(lx.w, ly.w, lz.w, hx.w, hy.w, hz.w, o, inv, tmin) : 1e30f;
+        int i0 = ch.x, i1 = ch.y, i2 = ch.z, i3 = ch.w;
+        // leaves first (their hits tighten tmin before the inner children are ordered)
+#define RC_LEAF(tc, ic, cnt)                                                              \
+        if (tc < 1e30f && ic <= -2) {                                                     \
+            const int first = -2 - ic;                                                    \
+            for (int j = first; j < first + (cnt); ++j) {                                 \
+                float t = rc_tri_static4(o, d, (const float4*)s.ST4 + 3 * (int64_t)j);    \
+                if (rc_closer(t, j, tmin, q)) { tmin = t; q = -2 - j; }                   \
+            }                                                                             \
+            tc = 1e30f;                                                                   \
         }
-        // sort the inner hits by entry distance (at most 4)
-        for (int a = 1; a < nh; ++a) {
-            float tv = tc[a]; int iv = ic[a]; int b = a - 1;
-            while (b >= 0 && tc[b] > tv) { tc[b + 1] = tc[b]; ic[b + 1] = ic[b]; --b; }
-            tc[b + 1] = tv; ic[b + 1] = iv;
-        }
+        RC_LEAF(t0, i0, cc.x) RC_LEAF(t1, i1, cc.y) RC_LEAF(t2, i2, cc.z) RC_LEAF(t3, i3, cc.w)
+#undef RC_LEAF
+        // sorting network on (t, node): ascending entry distance, misses (1e30) last
+        rc_cswap(t0, i0, t1, i1); rc_cswap(t2, i2, t3, i3);
+        rc_cswap(t0, i0, t2, i2); rc_cswap(t1, i1, t3, i3);
+        rc_cswap(t1, i1, t2, i2);
         int next = -1; float next_t = 0.f;
-        for (int a = nh - 1; a >= 0; --a) {
-            if (tc[a] > tmin) continue;
-            if (next >= 0) { stk[sp] = next; stt[sp] = next_t; ++sp; }
-            next = ic[a]; next_t = tc[a];
+#define RC_PUSH_OR_NEXT(tc, ic)                                                           \
+        if (tc < 1e30f && tc <= tmin) {                                                   \
+            if (next >= 0) { stk[sp * 256] = ((uint32_t)next << 8) | rc_tcode(next_t); ++sp; } \
+            next = ic; next_t = tc;                                                       \
         }
-        // 'next' holds the nearest surviving child; the pushes above stored the farther ones
+        RC_PUSH_OR_NEXT(t3, i3) RC_PUSH_OR_NEXT(t2, i2) RC_PUSH_OR_NEXT(t1, i1) RC_PUSH_OR_NEXT(t0, i0)
+#undef RC_PUSH_OR_NEXT
         if (next >= 0) { cur = next; continue; }
         bool found = false;
         while (sp > 0) {
             --sp;
-            if (stt[sp] <= tmin) { cur = stk[sp]; found = true; break; }
+            uint32_t e = stk[sp * 256];
+            if (rc_tlb(e & 255u) <= tmin) { cur = (int)(e >> 8); found = true; break; }
         }
         if (!found) break;
     }

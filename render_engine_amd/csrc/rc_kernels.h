@@ -10,7 +10,9 @@ hipError_t rc_launch_build(int trav, const rc_dscene* s, const rc_dstate* st, hi
 hipError_t rc_launch_trace2(int trav, int deps, const rc_dscene* s, const rc_dstate* st, const rc_dcache* c, const rc_params* p,
                             uint64_t* lnew, hipStream_t stream);
 hipError_t rc_launch_decide(const rc_dscene* s, const rc_dcache* c, const rc_params* p, hipStream_t stream);
-hipError_t rc_launch_update(const rc_dscene* s, const rc_dcache* c, const rc_params* p, const uint64_t* lnew, hipStream_t stream);
+hipError_t rc_launch_update(const rc_dscene* s, const rc_dstate* st, const rc_dcache* c, const rc_params* p, const uint64_t* lnew,
+                            hipStream_t stream);
+hipError_t rc_launch_resolve(const rc_dscene* s, const rc_dstate* st, const float* Lc, float* R, uint64_t dormant, hipStream_t stream);
 hipError_t rc_launch_sumpr(const rc_dcache* c, int Np, hipStream_t stream);
 hipError_t rc_launch_restored(const rc_dcache* c, int Np, int nl, float nmax, uint64_t dormant, int verify, hipStream_t stream);
 hipError_t rc_launch_rescale(const rc_dcache* c, int Np, int nl, int l, float ratio, hipStream_t stream);
@@ -22,7 +24,7 @@ hipError_t rc_launch_flush(const rc_dcache* c, int Np, int nl, hipStream_t strea
 hipError_t rc_launch_error(const rc_dcache* c, int Np, int nl, const float* ref, const uint8_t* valid, uint64_t dormant, hipStream_t stream);
 hipError_t rc_launch_ref(int trav, const rc_dscene* s, const rc_dstate* st, const float* Lin, float* Lout, float* acc, float* closec,
                          uint32_t seed, int K, int accumulate, int count_close, hipStream_t stream);
-hipError_t rc_launch_query(int trav, const rc_dscene* s, const rc_dstate* st, const float* Lc, const rc_camera* cam, uint32_t seed,
+hipError_t rc_launch_query(int trav, const rc_dscene* s, const rc_dstate* st, const float* R, const rc_camera* cam, uint32_t seed,
                            uint32_t frame, uint64_t dormant, float* img, unsigned long long* counters, hipStream_t stream);
 hipError_t rc_launch_hash_insert(unsigned long long* HK, int* HV, uint32_t hmask, const unsigned long long* keys, const int* vals,
                                  int n, int* fail, hipStream_t stream);

@@ -23,32 +23,32 @@ __device__ __forceinline__ double rc_wave_sum_f64(double v) {
 
 __device__ __forceinline__ long long rc_fx(float v, float scale) { return (long long)(v * scale); }
 
+// Static geometry per traversal mode: 0 binary in the prototype's order, 1 binary closest-hit,
+// 2 wide closest-hit (LDS stack).
 template <int TRAV>
-__device__ __forceinline__ void rc_trace_static(f3 o, f3 d, const rc_dscene& s, float& tmin, int& q);
-
-template <> __device__ __forceinline__ void rc_trace_static<0>(f3 o, f3 d, const rc_dscene& s, float& tmin, int& q) {
-    rc_trace_bvh_ref(o, d, s, tmin, q);
-}
-template <> __device__ __forceinline__ void rc_trace_static<1>(f3 o, f3 d, const rc_dscene& s, float& tmin, int& q) {
-    rc_trace_bvh_closest(o, d, s, tmin, q);
-}
-template <> __device__ __forceinline__ void rc_trace_static<2>(f3 o, f3 d, const rc_dscene& s, float& tmin, int& q) {
-    rc_trace_wide_closest(o, d, s, tmin, q);
+__device__ __forceinline__ void rc_trace_static(f3 o, f3 d, const rc_dscene& s, float& tmin, int& q, uint32_t* lstk) {
+    if (TRAV == 0) rc_trace_bvh_ref(o, d, s, tmin, q);
+    else if (TRAV == 1) rc_trace_bvh_closest(o, d, s, tmin, q);
+    else rc_trace_wide_closest(o, d, s, tmin, q, lstk);
 }
 
 // Dynamic triangles from LDS, then the static BVH carrying tmin (cuda/poc_gpu.cu:340-344).
 template <int TRAV>
-__device__ __forceinline__ bool rc_trace_full(f3 o, f3 d, const float* T, const int* TQ, int nt, const rc_dscene& s, float& tmin, int& q) {
+__device__ __forceinline__ bool rc_trace_full(f3 o, f3 d, const float* T, const int* TQ, int nt, const rc_dscene& s, float& tmin, int& q,
+                                              uint32_t* lstk) {
     bool h = rc_trace_lds(o, d, T, TQ, nt, tmin, q);
-    if (s.nst > 0) { rc_trace_static<TRAV>(o, d, s, tmin, q); h = q != -1; }
+    if (s.nst > 0) { rc_trace_static<TRAV>(o, d, s, tmin, q, lstk); h = q != -1; }
     return h;
 }
 
+// Dynamic LDS of the tracing kernels: nt triangles (9 floats) + their quad ids, then the
+// traversal stack (wstack levels x 256 lanes) at a 16-byte aligned offset.
 __device__ __forceinline__ void rc_load_dyn(const rc_dstate& st, float* sh, int* shq) {
     for (int j = threadIdx.x; j < 9 * st.nt; j += blockDim.x) sh[j] = st.T[j];
     for (int j = threadIdx.x; j < st.nt; j += blockDim.x) shq[j] = st.TQ[j];
     __syncthreads();
 }
+__device__ __forceinline__ uint32_t* rc_lstack(float* sh, int nt) { return (uint32_t*)(sh + ((10 * nt + 3) & ~3)); }
 
 // ------------------------------------------------------------------------------ rays (kernel ABI)
 template <int TRAV>
@@ -56,11 +56,12 @@ __global__ void __launch_bounds__(RC_TB) k_trace_rays(rc_dscene s, rc_dstate st,
                                                      const float* __restrict__ D, int n, float* tout, int* qout) {
     extern __shared__ float sh[];
     int* shq = (int*)(sh + 9 * st.nt);
+    uint32_t* lstk = rc_lstack(sh, st.nt);
     rc_load_dyn(st, sh, shq);
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     float t; int q;
-    rc_trace_full<TRAV>(ld3(O, i), ld3(D, i), sh, shq, st.nt, s, t, q);
+    rc_trace_full<TRAV>(ld3(O, i), ld3(D, i), sh, shq, st.nt, s, t, q, lstk);
     tout[i] = t; qout[i] = q;
 }
 
@@ -72,6 +73,7 @@ template <int TRAV>
 __global__ void __launch_bounds__(RC_TB) k_build(rc_dscene s, rc_dstate st) {
     extern __shared__ float sh[];
     int* shq = (int*)(sh + 9 * st.nt);
+    uint32_t* lstk = rc_lstack(sh, st.nt);
     rc_load_dyn(st, sh, shq);
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= s.Np) return;
@@ -83,7 +85,7 @@ __global__ void __launch_bounds__(RC_TB) k_build(rc_dscene s, rc_dstate st) {
         if (s.ltype[l] == 1) { D = ld3(s.L, l); r = s.sun_len; }
         else { f3 dv = ld3(s.L, l) - P; r = sqrtf(dot3(dv, dv)); D = dv * (1.f / r); }
         float c = dot3(N, D), t; int q;
-        rc_trace_full<TRAV>(O, D, sh, shq, st.nt, s, t, q);
+        rc_trace_full<TRAV>(O, D, sh, shq, st.nt, s, t, q, lstk);
         bool vis = t >= r - 1e-3f;
         float e;
         if (s.ltype[l] == 1) e = st.inten[l] * fmaxf(c, 0.f) * (vis ? 1.f : 0.f);
@@ -113,6 +115,7 @@ template <bool DEPS, int TRAV>
 __global__ void __launch_bounds__(RC_TB) k_trace2(rc_dscene s, rc_dstate st, rc_dcache c, rc_params p, uint64_t* lnew) {
     extern __shared__ float sh[];
     int* shq = (int*)(sh + 9 * st.nt);
+    uint32_t* lstk = rc_lstack(sh, st.nt);
     rc_load_dyn(st, sh, shq);
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= p.Np) return;
@@ -137,12 +140,13 @@ __global__ void __launch_bounds__(RC_TB) k_trace2(rc_dscene s, rc_dstate st, rc_
     float c0 = pend ? c.c0[i] : 0.f;
     int rg = pend ? s.region[i] : 0;
     float sd = 0.f, sq = 0.f;
-    float* mrow = c.m + (int64_t)nl * i;
+    float* mcol = c.m + i;   // SoA: light l of entry i at mcol[l * Np], coalesced across lanes
+    const int64_t ms = p.Np;
     for (int r = 0; r < k; ++r) {
         float u1 = rc_rnd(rs), u2 = rc_rnd(rs);
         f3 D = rc_cosine_dir(N, u1, u2);
         float t; int q;
-        bool hit = rc_trace_full<TRAV>(O, D, sh, shq, st.nt, s, t, q);
+        bool hit = rc_trace_full<TRAV>(O, D, sh, shq, st.nt, s, t, q, lstk);
         f3 H = hit ? O + D * t : O;
         int pi;
         bool back = rc_resolve_hit(s, st, hit, H, D, q, i, pi);
@@ -154,11 +158,11 @@ __global__ void __launch_bounds__(RC_TB) k_trace2(rc_dscene s, rc_dstate st, rc_
             for (int l = 0; l < nl; ++l) {
                 float lv = rc_is_dormant(p.dormant, l) ? 0.f : lc[l];
                 float v = A * (ed[l] + lv);
-                if (r == 0) mrow[l] = v; else mrow[l] += v;
+                if (r == 0) mcol[l * ms] = v; else mcol[l * ms] += v;
                 val += v;
             }
         } else if (r == 0) {
-            for (int l = 0; l < nl; ++l) mrow[l] = 0.f;
+            for (int l = 0; l < nl; ++l) mcol[l * ms] = 0.f;
         }
         if (DEPS) {
             int src = back ? i : pi;
@@ -180,11 +184,11 @@ __global__ void __launch_bounds__(RC_TB) k_trace2(rc_dscene s, rc_dstate st, rc_
     float ik = k > 0 ? 1.f / (float)k : 0.f;
     float sa = 0.f;
     if (p.ddgi && k > 0)
-        for (int l = 0; l < nl; ++l) sa += mrow[l];   // sum of the per-light accumulators
+        for (int l = 0; l < nl; ++l) sa += mcol[l * ms];   // sum of the per-light accumulators
     if (k == 0) {
-        for (int l = 0; l < nl; ++l) mrow[l] = 0.f;
+        for (int l = 0; l < nl; ++l) mcol[l * ms] = 0.f;
     } else if (k > 1) {
-        for (int l = 0; l < nl; ++l) mrow[l] = mrow[l] * ik;
+        for (int l = 0; l < nl; ++l) mcol[l * ms] = mcol[l * ms] * ik;
     }
     if (p.ddgi && k > 0) {
         int rgd = s.region[i];
@@ -233,54 +237,98 @@ __global__ void __launch_bounds__(RC_TB) k_decide(rc_dscene s, rc_dcache c, rc_p
     }
 }
 
+#define RC_UT 64   // entries per k_update block
+
 // EMA update of the per-light channels + history count (cuda/poc_gpu.cu:562-602), with the
-// light-dependency merge of this frame's samples and the next frame's sum of 1/(n+1).
-__global__ void __launch_bounds__(RC_TB) k_update(rc_dscene s, rc_dcache c, rc_params p, const uint64_t* lnew) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    long long pr = 0;
-    const int nl = p.nl;
-    if (i < p.Np) {
-        int k = c.kk[i];
-        float n = c.n[i], nn = n + (float)k;
-        float nmx = p.nmax;
-        if (p.ramp) { float a = c.age[i]; nmx = fminf(p.nmax, (4.f + 0.5f * a) * p.rays); c.age[i] = fminf(a + 1.f, 1e4f); }
-        float alpha = k > 0 ? fminf((float)k / fminf(fmaxf(nn, 1.f), nmx), 1.f) : 0.f;
-        const float* a_ = c.Lc + (int64_t)nl * i;
-        const float* m_ = c.m + (int64_t)nl * i;
-        float* o_ = c.Lc2 + (int64_t)nl * i;
-        if (p.ddgi) {
-            const long long* d3 = c.dstat + 3 * s.region[i];
-            const float isc = 1.0f / RC_FX_STAT;
-            float scale = 1.f;
-            float h = p.ddgi_h;
-            float d2 = (float)d3[2] * isc;
-            if (d2 > 0.f) {
-                float mn = ((float)d3[0] * isc) / d2, mc = ((float)d3[1] * isc) / d2;
-                float rel = fabsf(mn - mc) / fmaxf(mc, 1e-3f);
-                if (rel > p.ddgi_thr) h = fmaxf(0.f, h - 0.75f);
-                if (p.ddgi_clamp && rel > 0.10f) scale = 0.25f;
-            }
-            float w = k > 0 ? (1.f - h) * scale : 0.f;
-            for (int l = 0; l < nl; ++l) {
-                float a = a_[l];
-                o_[l] = rc_is_dormant(p.dormant, l) ? a : a + w * (m_[l] - a);
-            }
-        } else {
+// light-dependency merge of this frame's samples, the next frame's sum of 1/(n+1), and the
+// resolved radiance R = A * sum_l (Edir_l + Lc_l) of the updated entry (the value the query
+// pass reads, one float instead of 2 NL). A block owns RC_UT entries: per-entry factors are
+// computed by its first wave into LDS; the per-light blend then runs over the tile's AoS rows
+// in flat order (coalesced Lc / Edir / Lc2), reading the SoA sample means from an LDS copy.
+__global__ void __launch_bounds__(RC_TB) k_update(rc_dscene s, rc_dstate st, rc_dcache c, rc_params p, const uint64_t* lnew) {
+    __shared__ float s_al[RC_UT], s_alr[RC_UT], s_A[RC_UT];
+    extern __shared__ float sm[];                 // [nl][RC_UT] sample means, then [RC_UT][nl] terms
+    const int nl = p.nl, Np = p.Np;
+    const int base = blockIdx.x * RC_UT;
+    const int T = min(RC_UT, Np - base);
+    const int t = threadIdx.x;
+    float* s_m = sm;
+    float* s_v = sm + nl * RC_UT;
+    if (t < RC_UT) {
+        long long pr = 0;
+        if (t < T) {
+            const int i = base + t;
+            int k = c.kk[i];
+            float n = c.n[i], nn = n + (float)k;
+            float nmx = p.nmax;
+            if (p.ramp) { float a = c.age[i]; nmx = fminf(p.nmax, (4.f + 0.5f * a) * p.rays); c.age[i] = fminf(a + 1.f, 1e4f); }
+            float alpha = k > 0 ? fminf((float)k / fminf(fmaxf(nn, 1.f), nmx), 1.f) : 0.f;
             float alpha_r = k > 0 ? fminf((float)k / fminf(fmaxf(nn, 1.f), p.nmax_resp), 1.f) : 0.f;
-            for (int l = 0; l < nl; ++l) {
-                float a = a_[l];
-                float al = (l < 64 && ((p.resp_mask >> l) & 1ull)) ? alpha_r : alpha;
-                if (rc_is_dormant(p.dormant, l)) al = 0.f;
-                o_[l] = a + al * (m_[l] - a);
+            if (p.ddgi) {
+                const long long* d3 = c.dstat + 3 * s.region[i];
+                const float isc = 1.0f / RC_FX_STAT;
+                float scale = 1.f;
+                float h = p.ddgi_h;
+                float d2 = (float)d3[2] * isc;
+                if (d2 > 0.f) {
+                    float mn = ((float)d3[0] * isc) / d2, mc = ((float)d3[1] * isc) / d2;
+                    float rel = fabsf(mn - mc) / fmaxf(mc, 1e-3f);
+                    if (rel > p.ddgi_thr) h = fmaxf(0.f, h - 0.75f);
+                    if (p.ddgi_clamp && rel > 0.10f) scale = 0.25f;
+                }
+                alpha = k > 0 ? (1.f - h) * scale : 0.f;   // DDGI blend weight
             }
+            s_al[t] = alpha; s_alr[t] = alpha_r; s_A[t] = s.Alb[i];
+            n = fminf(nn, nmx);
+            c.n[i] = n;
+            pr = rc_fx(1.f / (n + 1.f), RC_FX_PR);
+            if (lnew) { uint64_t x = lnew[i]; if (x) c.lcur[i] |= x; }
         }
-        n = fminf(nn, nmx);
-        c.n[i] = n;
-        pr = rc_fx(1.f / (n + 1.f), RC_FX_PR);
-        if (lnew) { uint64_t x = lnew[i]; if (x) c.lcur[i] |= x; }
+        pr = rc_wave_sum_i64(pr);
+        if (t == 0) atomicAdd((unsigned long long*)&c.sumpr[1], (unsigned long long)pr);
     }
-    pr = rc_wave_sum_i64(pr);
-    if ((threadIdx.x & 63) == 0) atomicAdd((unsigned long long*)&c.sumpr[1], (unsigned long long)pr);
+    for (int e = t; e < nl * RC_UT; e += RC_TB) {
+        int l = e / RC_UT, j = e % RC_UT;
+        s_m[e] = j < T ? c.m[(int64_t)l * Np + base + j] : 0.f;
+    }
+    __syncthreads();
+    const int64_t g0 = (int64_t)base * nl;
+    for (int e = t; e < T * nl; e += RC_TB) {
+        int j = e / nl, l = e - j * nl;
+        float a = c.Lc[g0 + e];
+        float mm = s_m[l * RC_UT + j];
+        bool dm = rc_is_dormant(p.dormant, l);
+        float o;
+        if (p.ddgi) o = dm ? a : a + s_al[j] * (mm - a);
+        else {
+            float al = (l < 64 && ((p.resp_mask >> l) & 1ull)) ? s_alr[j] : s_al[j];
+            if (dm) al = 0.f;
+            o = a + al * (mm - a);
+        }
+        c.Lc2[g0 + e] = o;
+        float lv = dm ? 0.f : o;
+        s_v[e] = s_A[j] * (st.Edir[g0 + e] + lv);
+    }
+    __syncthreads();
+    if (t < T) {
+        float val = 0.f;
+        for (int l = 0; l < nl; ++l) val += s_v[t * nl + l];
+        c.R[base + t] = val;
+    }
+}
+
+// resolved radiance from the current channels (entries not yet updated this run)
+__global__ void __launch_bounds__(RC_TB) k_resolve(rc_dscene s, rc_dstate st, const float* Lc, float* R, int nl, uint64_t dormant) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= s.Np) return;
+    float A = s.Alb[i], val = 0.f;
+    const float* ed = st.Edir + (int64_t)nl * i;
+    const float* lc = Lc + (int64_t)nl * i;
+    for (int l = 0; l < nl; ++l) {
+        float lv = rc_is_dormant(dormant, l) ? 0.f : lc[l];
+        val += A * (ed[l] + lv);
+    }
+    R[i] = val;
 }
 
 __global__ void __launch_bounds__(RC_TB) k_sumpr(rc_dcache c, int Np) {
@@ -404,6 +452,7 @@ __global__ void __launch_bounds__(RC_TB) k_ref(rc_dscene s, rc_dstate st, const 
                                               float* closec, uint32_t seed, int K, int accumulate, int count_close) {
     extern __shared__ float sh[];
     int* shq = (int*)(sh + 9 * st.nt);
+    uint32_t* lstk = rc_lstack(sh, st.nt);
     rc_load_dyn(st, sh, shq);
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= s.Np) return;
@@ -417,7 +466,7 @@ __global__ void __launch_bounds__(RC_TB) k_ref(rc_dscene s, rc_dstate st, const 
         float u1 = rc_rnd(rs), u2 = rc_rnd(rs);
         f3 D = rc_cosine_dir(N, u1, u2);
         float t; int q;
-        bool hit = rc_trace_full<TRAV>(O, D, sh, shq, st.nt, s, t, q);
+        bool hit = rc_trace_full<TRAV>(O, D, sh, shq, st.nt, s, t, q, lstk);
         if (hit && t < 0.02f) close++;
         f3 H = hit ? O + D * t : O;
         int pi;
@@ -456,7 +505,7 @@ __device__ __forceinline__ bool rc_resolve_hit_n(const rc_dscene& s, const rc_ds
 // One-light next-event estimate at H (normal n): light chosen uniformly, weight nl.
 template <int TRAV>
 __device__ __forceinline__ float rc_nee(const rc_dscene& s, const rc_dstate& st, const float* sh, const int* shq,
-                                        f3 H, f3 n, uint32_t& rs, uint64_t dormant, int& shadow_rays) {
+                                        f3 H, f3 n, uint32_t& rs, uint64_t dormant, int& shadow_rays, uint32_t* lstk) {
     const int nl = s.nl;
     float u = rc_rnd(rs);
     int l = min((int)(u * (float)nl), nl - 1);
@@ -469,7 +518,7 @@ __device__ __forceinline__ float rc_nee(const rc_dscene& s, const rc_dstate& st,
     if (!(c > 0.f)) return 0.f;
     f3 O = H + n * 1e-3f;
     float t; int q;
-    rc_trace_full<TRAV>(O, D, sh, shq, st.nt, s, t, q);
+    rc_trace_full<TRAV>(O, D, sh, shq, st.nt, s, t, q, lstk);
     ++shadow_rays;
     if (!(t >= r - 1e-3f)) return 0.f;
     if (s.ltype[l] == 1) return (float)nl * (I * c);
@@ -477,67 +526,78 @@ __device__ __forceinline__ float rc_nee(const rc_dscene& s, const rc_dstate& st,
     return (float)nl * (I * c / (rr * rr));
 }
 
-// Full cached radiance of entry e: A (Edir + Lc) summed over the active light channels, the
-// value a cache-terminated ray reads (cuda/poc_gpu.cu:470-478).
-__device__ __forceinline__ float rc_cached(const rc_dscene& s, const rc_dstate& st, const float* Lc, int e, uint64_t dormant) {
-    const int nl = s.nl;
-    float A = s.Alb[e], val = 0.f;
-    const float* ed = st.Edir + (int64_t)nl * e;
-    const float* lc = Lc + (int64_t)nl * e;
-    for (int l = 0; l < nl; ++l) {
-        float lv = rc_is_dormant(dormant, l) ? 0.f : lc[l];
-        val += A * (ed[l] + lv);
+// One camera path: primary hit x0, one-light NEE at every explicit vertex, cosine bounces; at
+// bounce `cache_bounce` the path ends on the resolved radiance R of the hit entry (the value a
+// cache-terminated ray reads, A (Edir + Lc) over the active lights, cuda/poc_gpu.cu:470-478);
+// R == nullptr: no cache, the path ends after `bounces` bounces.
+template <int TRAV>
+__device__ __forceinline__ float rc_path(const rc_dscene& s, const rc_dstate& st, const float* sh, const int* shq, const float* R,
+                                         const rc_camera& cam, int px, int py, uint32_t& rs, uint64_t dormant, int& ext_rays,
+                                         int& sh_rays, uint32_t* lstk) {
+    float ju = rc_rnd(rs), jv = rc_rnd(rs);
+    float sx = (2.f * ((float)px + ju) / (float)cam.W - 1.f) * cam.tanx;
+    float sy = (1.f - 2.f * ((float)py + jv) / (float)cam.H) * cam.tany;
+    f3 d = mk3(cam.fwd[0], cam.fwd[1], cam.fwd[2]) + mk3(cam.right[0], cam.right[1], cam.right[2]) * sx +
+           mk3(cam.up[0], cam.up[1], cam.up[2]) * sy;
+    d = d * (1.f / sqrtf(dot3(d, d)));
+    f3 o = mk3(cam.pos[0], cam.pos[1], cam.pos[2]);
+    float thr = 1.f, rad = 0.f;
+    for (int b = 0; ; ++b) {
+        float t; int q;
+        bool hit = rc_trace_full<TRAV>(o, d, sh, shq, st.nt, s, t, q, lstk);
+        ++ext_rays;
+        if (!hit) break;
+        f3 H = o + d * t, n;
+        int e;
+        if (rc_resolve_hit_n(s, st, H, d, q, e, n)) break;
+        if (R && b == cam.cache_bounce) { rad += thr * R[e]; break; }
+        float A = s.Alb[e];
+        rad += thr * (A * rc_nee<TRAV>(s, st, sh, shq, H, n, rs, dormant, sh_rays, lstk));
+        if (b >= cam.bounces) break;
+        thr = thr * A;
+        float u1 = rc_rnd(rs), u2 = rc_rnd(rs);
+        d = rc_cosine_dir(n, u1, u2);
+        o = H + n * 1e-3f;
     }
-    return val;
+    return rad;
 }
 
-// Camera pass: per pixel, spp paths. Each path: primary hit x0, one-light NEE at every
-// explicit vertex, cosine bounces; at bounce `cache_bounce` the path ends on the cached
-// radiance of the hit entry (Lc == nullptr: no cache, the path ends after `bounces` bounces).
-template <int TRAV>
-__global__ void __launch_bounds__(RC_TB) k_query(rc_dscene s, rc_dstate st, const float* Lc, rc_camera cam, uint32_t seed,
+// Camera pass, one thread per path (spp divides 64: the spp paths of a pixel sit in adjacent
+// lanes and are summed in sample order by the pixel's first lane), or one thread per pixel
+// looping over its samples (any spp). Both sum ((r0 + r1) + r2) + ... and divide by spp.
+template <int TRAV, bool PER_PATH>
+__global__ void __launch_bounds__(RC_TB) k_query(rc_dscene s, rc_dstate st, const float* R, rc_camera cam, uint32_t seed,
                                                 uint32_t frame, uint64_t dormant, float* img, unsigned long long* counters) {
     extern __shared__ float sh[];
     int* shq = (int*)(sh + 9 * st.nt);
+    uint32_t* lstk = rc_lstack(sh, st.nt);
     rc_load_dyn(st, sh, shq);
-    int pix = blockIdx.x * blockDim.x + threadIdx.x;
+    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
     int ext_rays = 0, sh_rays = 0;
-    if (pix < cam.W * cam.H) {
-        int px = pix % cam.W, py = pix / cam.W;
-        f3 pos = mk3(cam.pos[0], cam.pos[1], cam.pos[2]);
-        f3 fw = mk3(cam.fwd[0], cam.fwd[1], cam.fwd[2]);
-        f3 rt = mk3(cam.right[0], cam.right[1], cam.right[2]);
-        f3 up = mk3(cam.up[0], cam.up[1], cam.up[2]);
+    const int npix = cam.W * cam.H;
+    if (PER_PATH) {
+        const int spp = cam.spp;
+        const bool live = tid < npix * spp;
+        float rad = 0.f;
+        if (live) {
+            int pix = tid / spp;
+            uint32_t rs = rc_seed(seed, frame, (uint32_t)tid);
+            rad = rc_path<TRAV>(s, st, sh, shq, R, cam, pix % cam.W, pix / cam.W, rs, dormant, ext_rays, sh_rays, lstk);
+        }
+        const int lane = threadIdx.x & 63, first = lane - lane % spp;
+        float sum = rad;
+        for (int k = 1; k < spp; ++k) {
+            float rk = __shfl(rad, first + k, 64);
+            if (lane == first) sum += rk;
+        }
+        if (live && lane == first) img[tid / spp] = sum / (float)spp;
+    } else if (tid < npix) {
         float sum = 0.f;
         for (int sp = 0; sp < cam.spp; ++sp) {
-            uint32_t rs = rc_seed(seed, frame, (uint32_t)(pix * cam.spp + sp));
-            float ju = rc_rnd(rs), jv = rc_rnd(rs);
-            float sx = (2.f * ((float)px + ju) / (float)cam.W - 1.f) * cam.tanx;
-            float sy = (1.f - 2.f * ((float)py + jv) / (float)cam.H) * cam.tany;
-            f3 d = fw + rt * sx + up * sy;
-            d = d * (1.f / sqrtf(dot3(d, d)));
-            f3 o = pos;
-            float thr = 1.f, rad = 0.f;
-            for (int b = 0; ; ++b) {
-                float t; int q;
-                bool hit = rc_trace_full<TRAV>(o, d, sh, shq, st.nt, s, t, q);
-                ++ext_rays;
-                if (!hit) break;
-                f3 H = o + d * t, n;
-                int e;
-                if (rc_resolve_hit_n(s, st, H, d, q, e, n)) break;
-                if (Lc && b == cam.cache_bounce) { rad += thr * rc_cached(s, st, Lc, e, dormant); break; }
-                float A = s.Alb[e];
-                rad += thr * (A * rc_nee<TRAV>(s, st, sh, shq, H, n, rs, dormant, sh_rays));
-                if (b >= cam.bounces) break;
-                thr = thr * A;
-                float u1 = rc_rnd(rs), u2 = rc_rnd(rs);
-                d = rc_cosine_dir(n, u1, u2);
-                o = H + n * 1e-3f;
-            }
-            sum += rad;
+            uint32_t rs = rc_seed(seed, frame, (uint32_t)(tid * cam.spp + sp));
+            sum += rc_path<TRAV>(s, st, sh, shq, R, cam, tid % cam.W, tid / cam.W, rs, dormant, ext_rays, sh_rays, lstk);
         }
-        img[pix] = sum / (float)cam.spp;
+        img[tid] = sum / (float)cam.spp;
     }
     if (counters) {
         long long a = rc_wave_sum_i64(ext_rays), b2 = rc_wave_sum_i64(sh_rays);
@@ -643,13 +703,17 @@ __global__ void __launch_bounds__(RC_TB) k_dda_volume(rc_dscene s, int gx, int g
 
 // ------------------------------------------------------------------------------ launchers
 static inline int rc_grid(long long n) { return (int)((n + RC_TB - 1) / RC_TB); }
-static inline size_t rc_shm(const rc_dstate& st) { return (size_t)st.nt * 10 * sizeof(float); }
+// dynamic LDS of a tracing kernel: dynamic triangles + (wide traversal) the lane stacks
+static inline size_t rc_shm(const rc_dstate& st, const rc_dscene& s, int trav) {
+    size_t tri = (((size_t)st.nt * 10 + 3) & ~(size_t)3) * sizeof(float);
+    return tri + (trav == 2 ? (size_t)s.wstack * 256 * sizeof(uint32_t) : 0);
+}
 
 hipError_t rc_launch_trace_rays(int trav, const rc_dscene* s, const rc_dstate* st, const float* O, const float* D, int n,
                                 float* t, int* q, hipStream_t stream) {
     if (n <= 0) return hipSuccess;
     dim3 g(rc_grid(n)), b(RC_TB);
-    size_t shm = rc_shm(*st);
+    size_t shm = rc_shm(*st, *s, trav);
     if (trav == 0) hipLaunchKernelGGL(k_trace_rays<0>, g, b, shm, stream, *s, *st, O, D, n, t, q);
     else if (trav == 1) hipLaunchKernelGGL(k_trace_rays<1>, g, b, shm, stream, *s, *st, O, D, n, t, q);
     else hipLaunchKernelGGL(k_trace_rays<2>, g, b, shm, stream, *s, *st, O, D, n, t, q);
@@ -659,7 +723,7 @@ hipError_t rc_launch_trace_rays(int trav, const rc_dscene* s, const rc_dstate* s
 hipError_t rc_launch_build(int trav, const rc_dscene* s, const rc_dstate* st, hipStream_t stream) {
     if (s->Np <= 0) return hipSuccess;
     dim3 g(rc_grid(s->Np)), b(RC_TB);
-    size_t shm = rc_shm(*st);
+    size_t shm = rc_shm(*st, *s, trav);
     if (trav == 0) hipLaunchKernelGGL(k_build<0>, g, b, shm, stream, *s, *st);
     else if (trav == 1) hipLaunchKernelGGL(k_build<1>, g, b, shm, stream, *s, *st);
     else hipLaunchKernelGGL(k_build<2>, g, b, shm, stream, *s, *st);
@@ -670,7 +734,7 @@ hipError_t rc_launch_trace2(int trav, int deps, const rc_dscene* s, const rc_dst
                             uint64_t* lnew, hipStream_t stream) {
     if (p->Np <= 0) return hipSuccess;
     dim3 g(rc_grid(p->Np)), b(RC_TB);
-    size_t shm = rc_shm(*st);
+    size_t shm = rc_shm(*st, *s, trav);
 #define RC_T2(D_, T_) hipLaunchKernelGGL((k_trace2<D_, T_>), g, b, shm, stream, *s, *st, *c, *p, lnew)
     if (deps) { if (trav == 0) RC_T2(true, 0); else if (trav == 1) RC_T2(true, 1); else RC_T2(true, 2); }
     else { if (trav == 0) RC_T2(false, 0); else if (trav == 1) RC_T2(false, 1); else RC_T2(false, 2); }
@@ -682,8 +746,17 @@ hipError_t rc_launch_decide(const rc_dscene* s, const rc_dcache* c, const rc_par
     hipLaunchKernelGGL(k_decide, dim3(rc_grid(p->Np)), dim3(RC_TB), 0, stream, *s, *c, *p);
     return hipGetLastError();
 }
-hipError_t rc_launch_update(const rc_dscene* s, const rc_dcache* c, const rc_params* p, const uint64_t* lnew, hipStream_t stream) {
-    hipLaunchKernelGGL(k_update, dim3(rc_grid(p->Np)), dim3(RC_TB), 0, stream, *s, *c, *p, lnew);
+hipError_t rc_launch_update(const rc_dscene* s, const rc_dstate* st, const rc_dcache* c, const rc_params* p, const uint64_t* lnew,
+                            hipStream_t stream) {
+    if (p->Np <= 0) return hipSuccess;
+    dim3 g((p->Np + RC_UT - 1) / RC_UT), b(RC_TB);
+    size_t shm = (size_t)2 * p->nl * RC_UT * sizeof(float);
+    hipLaunchKernelGGL(k_update, g, b, shm, stream, *s, *st, *c, *p, lnew);
+    return hipGetLastError();
+}
+hipError_t rc_launch_resolve(const rc_dscene* s, const rc_dstate* st, const float* Lc, float* R, uint64_t dormant, hipStream_t stream) {
+    if (s->Np <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_resolve, dim3(rc_grid(s->Np)), dim3(RC_TB), 0, stream, *s, *st, Lc, R, s->nl, dormant);
     return hipGetLastError();
 }
 hipError_t rc_launch_sumpr(const rc_dcache* c, int Np, hipStream_t stream) {
@@ -725,19 +798,22 @@ hipError_t rc_launch_error(const rc_dcache* c, int Np, int nl, const float* ref,
 hipError_t rc_launch_ref(int trav, const rc_dscene* s, const rc_dstate* st, const float* Lin, float* Lout, float* acc, float* closec,
                          uint32_t seed, int K, int accumulate, int count_close, hipStream_t stream) {
     dim3 g(rc_grid(s->Np)), b(RC_TB);
-    size_t shm = rc_shm(*st);
+    size_t shm = rc_shm(*st, *s, trav);
     if (trav == 0) hipLaunchKernelGGL(k_ref<0>, g, b, shm, stream, *s, *st, Lin, Lout, acc, closec, seed, K, accumulate, count_close);
     else if (trav == 1) hipLaunchKernelGGL(k_ref<1>, g, b, shm, stream, *s, *st, Lin, Lout, acc, closec, seed, K, accumulate, count_close);
     else hipLaunchKernelGGL(k_ref<2>, g, b, shm, stream, *s, *st, Lin, Lout, acc, closec, seed, K, accumulate, count_close);
     return hipGetLastError();
 }
-hipError_t rc_launch_query(int trav, const rc_dscene* s, const rc_dstate* st, const float* Lc, const rc_camera* cam, uint32_t seed,
+hipError_t rc_launch_query(int trav, const rc_dscene* s, const rc_dstate* st, const float* R, const rc_camera* cam, uint32_t seed,
                            uint32_t frame, uint64_t dormant, float* img, unsigned long long* counters, hipStream_t stream) {
-    dim3 g(rc_grid((long long)cam->W * cam->H)), b(RC_TB);
-    size_t shm = rc_shm(*st);
-    if (trav == 0) hipLaunchKernelGGL(k_query<0>, g, b, shm, stream, *s, *st, Lc, *cam, seed, frame, dormant, img, counters);
-    else if (trav == 1) hipLaunchKernelGGL(k_query<1>, g, b, shm, stream, *s, *st, Lc, *cam, seed, frame, dormant, img, counters);
-    else hipLaunchKernelGGL(k_query<2>, g, b, shm, stream, *s, *st, Lc, *cam, seed, frame, dormant, img, counters);
+    const bool per_path = cam->spp <= 64 && 64 % cam->spp == 0;
+    long long n = (long long)cam->W * cam->H * (per_path ? cam->spp : 1);
+    dim3 g(rc_grid(n)), b(RC_TB);
+    size_t shm = rc_shm(*st, *s, trav);
+#define RC_Q(T_, P_) hipLaunchKernelGGL((k_query<T_, P_>), g, b, shm, stream, *s, *st, R, *cam, seed, frame, dormant, img, counters)
+    if (per_path) { if (trav == 0) RC_Q(0, true); else if (trav == 1) RC_Q(1, true); else RC_Q(2, true); }
+    else { if (trav == 0) RC_Q(0, false); else if (trav == 1) RC_Q(1, false); else RC_Q(2, false); }
+#undef RC_Q
     return hipGetLastError();
 }
 hipError_t rc_launch_hash_insert(unsigned long long* HK, int* HV, uint32_t hmask, const unsigned long long* keys, const int* vals,

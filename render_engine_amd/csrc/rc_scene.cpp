@@ -312,7 +312,8 @@ std::string prepare(const SceneDesc& d, Scene& S) {
     // 1. BVH + triangles in leaf order (v0, e1, e2) with normal + double-sided flag
     build_bvh(M, S.BN, S.order, S.bvh_depth);
     build_wide(S.BN, S.WN, S.wide_depth);
-    if (3 * S.wide_depth > RC_WSTACK) return "wide BVH too deep for the traversal stack";
+    if (3 * S.wide_depth + 1 > RC_WSTACK) return "wide BVH too deep for the LDS traversal stack";
+    if (S.WN.size() >= (1u << 24)) return "wide BVH has more than 2^24 nodes";
     S.nst = M.nt();
     S.ST.resize(9 * (size_t)S.nst); S.STN.resize(4 * (size_t)S.nst); S.Talb.resize(S.nst);
     for (int j = 0; j < S.nst; ++j) {

@@ -34,7 +34,7 @@ typedef struct rc_wnode {
     int32_t child[4];                       // 16 B
     int32_t ccount[4];                      // 16 B
 } rc_wnode;
-#define RC_WSTACK 64          // wide-traversal stack; the builder bounds depth so 3 pushes/level fit
+#define RC_WSTACK 48          // most LDS stack levels per lane (48 KB per 256-lane block)
 
 // Per-state device view of the scene (cuda/poc_gpu.cu:222-237).
 typedef struct rc_dstate {
@@ -59,6 +59,7 @@ typedef struct rc_dstate {
 // Static scene data on the device (format-3 part of cuda/poc_gpu.cu:231-236).
 typedef struct rc_dscene {
     const float* ST;         // [9 nst] static triangles v0, e1, e2 (BVH leaf order)
+    const float* ST4;        // [12 nst] the same triangles as 48-byte rows (v0,0 e1,0 e2,0)
     const float* STN;        // [4 nst] geometric normal + flags (w bit 0 = double sided)
     const rc_bvh_node* BN;   // binary BVH nodes
     const rc_wnode* WN;      // 4-wide BVH nodes
@@ -74,11 +75,14 @@ typedef struct rc_dscene {
     float sun_len;           // shadow-ray length toward a directional light
     int32_t nst, nl, Np, NR;
     int32_t vx, vy, vz, nw;  // transit grid (vx*vy*vz = 32 nw bits per entry)
+    int32_t wstack, pad0;    // LDS stack levels of the wide traversal (3 per wide-BVH level)
 } rc_dscene;
 
 // Cache buffers (cuda/poc_gpu.cu:394-409) without the v5-B projection buffers (out of scope).
 typedef struct rc_dcache {
-    float *Lc, *Lc2, *m, *n, *c0;
+    float *Lc, *Lc2, *n, *c0;
+    float* m;                    // [NL][Np] SoA: this frame's per-light sample means
+    float* R;                    // [Np] resolved radiance A (sum Edir + sum Lc) read by the query pass
     int32_t* kk;
     uint8_t* pending;
     uint32_t *cur, *prev;        // [NW][Np] SoA transit-grid dependency banks

@@ -154,7 +154,7 @@ def test_runner_bit_exact(torch_gpu, which, name, mk, request):
         fs = cache.frame(st)
         _, inv = orun.frame(st)
         assert fs.invalidated == inv, f"frame {f}: invalidated {fs.invalidated} vs {inv}"
-        for arr in ("LC", "N", "KK", "PENDING", "CUR", "LCUR", "SUMPR", "NPEND", "AGE", "STALE"):
+        for arr in ("LC", "M", "R", "N", "KK", "PENDING", "CUR", "LCUR", "SUMPR", "NPEND", "AGE", "STALE"):
             g, e = cache.get(arr), orun.get(arr)
             if g.dtype == np.float32:
                 g, e = g.view(np.uint32), e.view(np.uint32)
@@ -176,10 +176,10 @@ def test_query_bit_exact(torch_gpu, street):
     img = torch.zeros(cam.W * cam.H, dtype=torch.float32, device="cuda")
     for no_cache in (0, 1):
         rays = cache.render(img, cam, no_cache=bool(no_cache), seed=4, count_rays=True)
-        lc = orun.get("LC")
+        R = orun.get("R")
         exp = np.zeros(cam.W * cam.H, np.float32)
         er = (C.c_longlong * 2)()
-        po.lib.orc_k_query(osc.state(0), 1, None if no_cache else po.ptr(lc), C.byref(cam), 4, 3, 0, po.ptr(exp), er, 0,
+        po.lib.orc_k_query(osc.state(0), 1, None if no_cache else po.ptr(R), C.byref(cam), 4, 3, 0, po.ptr(exp), er, 0,
                            cam.W * cam.H)
         got = img.cpu().numpy()
         assert np.array_equal(got.view(np.uint32), exp.view(np.uint32)), f"no_cache={no_cache}: {np.sum(got != exp)} pixels differ"
@@ -210,7 +210,7 @@ def test_radiance_vs_fp64(torch_gpu, room):
         p = po.OrcParams()
         p.Np, p.NR, p.mode, p.budget, p.nmax, p.seed, p.frame = i.Np, i.NR, 0, 1.0, m.nmax, 21, f
         p.nmax_resp, p.rays, p.nl = m.nmax_resp, 1.0, i.nl
-        mcopy = np.zeros(i.Np * i.nl, np.float32)
+        mcopy = np.zeros(i.Np * i.nl, np.float32)   # SoA sample means of this frame
         kk = np.zeros(i.Np, np.int32)
         tmp = po.OrcCache()
         for fld, _ in po.OrcCache._fields_:
@@ -228,3 +228,24 @@ def test_radiance_vs_fp64(torch_gpu, room):
     g = cache.get("LC").astype(np.float64)
     rel = np.sqrt(np.mean((g - L64) ** 2)) / np.mean(np.abs(L64))
     assert rel < 1e-4, f"relative RMS vs FP64 = {rel:.3e}"
+
+
+def test_query_any_spp(torch_gpu, room):
+    """spp that does not divide 64 takes the per-pixel loop kernel; same result as the oracle"""
+    torch = torch_gpu
+    scene, osc = room
+    m = rea.methods.plain()
+    cache = rea.Cache(scene, m)
+    orun = po.OracleRunner(osc, m)
+    cache.begin(0, 2)
+    orun.begin(0, 2)
+    cam = scene.camera()
+    for spp, nb in ((3, 1), (5, 2), (1, 0)):
+        cam.spp, cam.bounces = spp, nb
+        img = torch.zeros(cam.W * cam.H, dtype=torch.float32, device="cuda")
+        cache.render(img, cam, seed=8)   # before any frame: resolved from the warm-start channels
+        R = np.zeros(osc.Np, np.float32)
+        po.lib.orc_k_resolve(osc.state(0), po.ptr(orun.get("LC")), po.ptr(R), 0)
+        exp = np.zeros(cam.W * cam.H, np.float32)
+        po.lib.orc_k_query(osc.state(0), 1, po.ptr(R), C.byref(cam), 8, 0, 0, po.ptr(exp), None, 0, cam.W * cam.H)
+        assert np.array_equal(img.cpu().numpy().view(np.uint32), exp.view(np.uint32)), f"spp {spp}"

@@ -247,7 +247,7 @@ def test_runner_ema_closed_form(room):
     run.begin(0, 3)
     lc0 = run.get("LC").copy()
     run.frame(0)
-    M = run.get("M")   # this frame's sample means
+    M = run.get("M").reshape(osc.nl, osc.Np).T.ravel()   # this frame's sample means (SoA -> AoS)
     lc1 = run.get("LC")
     assert np.allclose(lc1, lc0 + (M - lc0) / 8.0, rtol=1e-6, atol=1e-7)
     assert np.all(run.get("N") == 8.0)
@@ -288,7 +288,7 @@ def test_rescale_linearity(room):
     # one frame in state 1 (light 1 off): channel 1 is rescaled by 0 before this frame's update
     lc_before = run.get("LC").reshape(-1, 2)
     run.frame(1)
-    M = run.get("M").reshape(-1, 2)
+    M = run.get("M").reshape(2, -1).T
     lc = run.get("LC").reshape(-1, 2)
     # EMA from a zeroed channel 1: L = 0 + (m - 0) / 16
     assert np.allclose(lc[:, 1], M[:, 1] / 16.0, rtol=1e-6, atol=1e-9)
@@ -314,3 +314,16 @@ def test_scenario_parsing_and_region_metric():
     reg = sc.light_off_region(on, off)
     assert reg[:, :18].all() and not reg[:, 32:].any()
     assert sc.frames_to_below([0.3, 0.1, 0.04, 0.02]) == 2
+
+
+def test_resolved_radiance(room):
+    """R after an update = A * sum_l (Edir_l + Lc_l) of the new channels"""
+    scene, osc = room
+    run = po.OracleRunner(osc, rea.methods.plain())
+    run.begin(0, 9)
+    run.frame(0)
+    lc = run.get("LC").reshape(-1, osc.nl).astype(np.float64)
+    E = osc.Edir[0].reshape(-1, osc.nl).astype(np.float64)
+    A = scene.array("ALB").astype(np.float64)
+    R = run.get("R")
+    assert np.allclose(R, A * (E + lc).sum(1), rtol=2e-6, atol=1e-7)
