@@ -120,6 +120,63 @@ def cpu_baseline(osc, scene, seconds, log):
                        f"same scene, state 0", fps=frames / t_tot)
 
 
+def quality(rea, scene, cam, log, pre=20, post=60, ref_spp_rounds=16):
+    """Light-off events against converged references (cuda/prep_mesh.cuh:454-520 solve, then the
+    same query at 64 spp): per method, the region-luminance bias sum(est)/sum(ref) - 1 over the
+    pixels where the switched-off light carried > 50 % of the converged radiance (9x9 blur at
+    960 px, windows/sondes.py:30-47), per frame after the event; frames until |bias| < 5 %."""
+    import numpy as np
+    import torch
+
+    from render_engine_amd import scenarios as sc
+
+    t0 = time.time()
+    noise, iters = scene.reference(K=32, warm=8, block=32, max_iter=256, target=0.03)
+    log(f"converged references: self-noise {[round(x, 4) for x in noise]}, iterations {iters}, {time.time() - t0:.1f} s")
+    img = torch.zeros(cam.W * cam.H, dtype=torch.float32, device="cuda")
+    refs = {}
+    for st in range(scene.info.NS):
+        c = rea.Cache(scene, rea.methods.plain())
+        c.begin(st, 1)
+        acc = np.zeros(cam.W * cam.H)
+        for k in range(ref_spp_rounds):
+            c.render(img, cam, seed=100000 + k)
+            acc += img.cpu().numpy()
+        refs[st] = (acc / ref_spp_rounds).reshape(cam.H, cam.W)
+    events = {"lamp 0 off": (0, 2), "night (sun off)": (0, 1)} if scene.info.NS >= 3 else {"light 1 off": (0, 1)}
+    out = {}
+    for name, (a, b) in events.items():
+        region = sc.light_off_region(refs[a], refs[b])
+        ev = {"region_pixels": int(region.sum()), "methods": {}}
+        if region.sum() == 0:
+            out[name] = ev
+            continue
+        for mname, m in (("plain", rea.methods.plain()), ("flush_snapshots", rea.methods.flush_snapshots()),
+                         ("targeted", rea.methods.targeted())):
+            c = rea.Cache(scene, m)
+            c.begin(a, 777)
+            biases, errs, inval = [], [], 0
+            for f in range(pre + post):
+                st = a if f < pre else b
+                fs = c.frame(st, want_err=True)
+                inval += fs.invalidated
+                c.render(img, cam, seed=5000 + f)
+                if f >= pre:
+                    est = img.cpu().numpy().reshape(cam.H, cam.W)
+                    biases.append(sc.region_bias(est, refs[b], region))
+                    errs.append(fs.err)
+            ev["methods"][mname] = {
+                "frames_to_5pct": sc.frames_to_below(biases, 0.05),
+                "bias_first_10": [round(x, 4) for x in biases[:10]],
+                "bias_mean_30": round(float(np.mean(np.abs(biases[:30]))), 4),
+                "entry_relrms_first_5": [round(x, 4) for x in errs[:5]],
+                "entries_invalidated": inval,
+            }
+        out[name] = ev
+        log(f"{name}: {json.dumps(ev)}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -129,6 +186,7 @@ def main():
     ap.add_argument("--trav", type=int, default=2)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-quality", action="store_true")
     ap.add_argument("--quiet", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -229,6 +287,12 @@ def main():
             cpu = cpu_baseline(osc, scene, args.cpu_seconds, log)
         except Exception as e:  # the baseline is reported, not required
             log(f"cpu baseline failed: {e!r}")
+    qual = None
+    if not args.no_quality:
+        try:
+            qual = quality(rea, scene, cam, log)
+        except Exception as e:
+            log(f"quality phase failed: {e!r}")
     out = {
         "metric": "Mrays/s + cache Mpts/s; frames to <5% error after light-off, 1 GPU",
         "value": round(mrays, 2),
@@ -252,6 +316,7 @@ def main():
         "kernels": kernels,
         "roofline": roof,
         "cpu_baseline": cpu,
+        "light_off": qual,
     }
     if cpu:
         out["gpu_over_cpu_fps"] = round((args.steps / elapsed) / cpu["fps"], 1)

@@ -270,6 +270,66 @@ __device__ __forceinline__ void rc_trace_wide_closest(f3 o, f3 d, const rc_dscen
     }
 }
 
+// Any-hit test of the static geometry for a shadow segment: true when some triangle is hit at
+// 1e-4 < t < tlim. Equivalent to the closest-hit visibility test "closest t >= tlim" used by the
+// direct-light estimate, but stops at the first blocker. Boxes are culled against tlim.
+__device__ __forceinline__ bool rc_occluded_wide(f3 o, f3 d, const rc_dscene& s, float tlim, uint32_t* lstk) {
+    uint32_t* stk = lstk + (threadIdx.x & 255);
+    f3 inv = rc_inv_dir(d);
+    int sp = 0, cur = 0;
+    while (true) {
+        const float4* nb = (const float4*)(s.WN + cur);
+        const float4 lx = nb[0], ly = nb[1], lz = nb[2], hx = nb[3], hy = nb[4], hz = nb[5];
+        const int4 ch = ((const int4*)nb)[6], cc = ((const int4*)nb)[7];
+        int nxt = -1;
+#define RC_ANY(cx, lxx, lyx, lzx, hxx, hyx, hzx, cnt)                                      \
+        if (cx != -1) {                                                                    \
+            float te = rc_box_enter(lxx, lyx, lzx, hxx, hyx, hzx, o, inv, tlim);           \
+            if (te < 1e30f) {                                                              \
+                if (cx <= -2) {                                                            \
+                    const int first = -2 - cx;                                             \
+                    for (int j = first; j < first + (cnt); ++j) {                          \
+                        float t = rc_tri_static4(o, d, (const float4*)s.ST4 + 3 * (int64_t)j); \
+                        if (t > 1e-4f && t < tlim) return true;                            \
+                    }                                                                      \
+                } else if (nxt < 0) nxt = cx;                                              \
+                else { stk[sp * 256] = (uint32_t)cx << 8; ++sp; }                          \
+            }                                                                              \
+        }
+        RC_ANY(ch.x, lx.x, ly.x, lz.x, hx.x, hy.x, hz.x, cc.x)
+        RC_ANY(ch.y, lx.y, ly.y, lz.y, hx.y, hy.y, hz.y, cc.y)
+        RC_ANY(ch.z, lx.z, ly.z, lz.z, hx.z, hy.z, hz.z, cc.z)
+        RC_ANY(ch.w, lx.w, ly.w, lz.w, hx.w, hy.w, hz.w, cc.w)
+#undef RC_ANY
+        if (nxt >= 0) { cur = nxt; continue; }
+        if (sp == 0) return false;
+        --sp;
+        cur = (int)(stk[sp * 256] >> 8);
+    }
+}
+
+// the same test against the LDS-held dynamic triangles (cuda/poc_gpu.cu:200-220 arithmetic)
+__device__ __forceinline__ bool rc_occluded_lds(f3 o, f3 d, const float* T, int nt, float tlim) {
+    for (int j = 0; j < nt; ++j) {
+        f3 v0 = mk3(T[9 * j], T[9 * j + 1], T[9 * j + 2]);
+        f3 e1 = mk3(T[9 * j + 3], T[9 * j + 4], T[9 * j + 5]);
+        f3 e2 = mk3(T[9 * j + 6], T[9 * j + 7], T[9 * j + 8]);
+        f3 p = cross3(d, e2);
+        float det = dot3(e1, p);
+        if (fabsf(det) <= 1e-9f) continue;
+        float inv = 1.f / det;
+        f3 tv = o - v0;
+        float u = dot3(tv, p) * inv;
+        if (u < 0.f) continue;
+        f3 qv = cross3(tv, e1);
+        float v = dot3(d, qv) * inv;
+        if (v < 0.f || u + v > 1.f) continue;
+        float t = dot3(e2, qv) * inv;
+        if (t > 1e-4f && t < tlim) return true;
+    }
+    return false;
+}
+
 // ---------------------------------------------------------------- hash grid (cuda/poc_gpu.cu:251-287)
 __device__ __forceinline__ int rc_normal_bucket(float nx, float ny, float nz) {
     float ax = fabsf(nx), ay = fabsf(ny), az = fabsf(nz);
