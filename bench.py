@@ -277,8 +277,14 @@ def main():
         ach = qb / t_q / 1e6
     else:
         ach = None
+    traffic = None
+    pmc = os.path.join(ROOT, "profiles", "pmc_street960x540.json")
+    if args.config == 2 and os.path.exists(pmc):   # HBM bytes per launch measured by rocprofv3 PMC on this workload
+        for kname, v in json.load(open(pmc))["kernels"].items():
+            if kname.startswith(dom):
+                traffic = v["traffic_bytes"]
     roof = {"bound": "hbm", "achieved": ach, "peak": 8000.0, "unit": "GB/s", "frac": (ach / 8000.0) if ach else None,
-            "traffic": None, "kernel": dom,
+            "traffic": traffic, "traffic_source": "profiles/pmc_street960x540.json" if traffic else None, "kernel": dom,
             "basis": ("k_query: 32 B/binary node + 36 B/triangle + 8 B/hash probe + 4 B/resolved read per path, oracle-counted"
                       if dom == "k_query" else "k_update: 12 B + 12 B x lights per entry")}
     cpu = None
