@@ -55,6 +55,8 @@ def oracle_scene(scene):
         osc.Edir[st][:] = scene.array("EDIR", st)
         osc.SV[st][:] = scene.array("SV", st)
         osc.LM[st][:] = scene.array("LM", st)
+        if scene.info.fine > 0:
+            osc.VOL[st][:] = scene.array("VOL", st)
     return osc
 
 

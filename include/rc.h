@@ -63,7 +63,7 @@ typedef struct rc_camera {
 #endif
 
 typedef struct rc_scene_info {
-    int Np, npst, nbox, nst, nnodes, nwnodes, NS, nl, NR, Nq, hsize, vx, vy, vz, nw, bvh_depth, wide_depth, has_ref;
+    int Np, npst, nbox, nst, nnodes, nwnodes, NS, nl, NR, Nq, hsize, vx, vy, vz, nw, bvh_depth, wide_depth, has_ref, fine;
     float h, lo[3], ext[3], diag, sun_len;
     rc_camera cam;
 } rc_scene_info;
@@ -81,7 +81,7 @@ enum {
     RC_A_ST = 0, RC_A_STN, RC_A_BN, RC_A_WN, RC_A_HK, RC_A_HV, RC_A_P, RC_A_N, RC_A_ALB, RC_A_REGION,
     RC_A_L, RC_A_LTYPE, RC_A_INTEN, RC_A_Q, RC_A_DIMS, RC_A_OFF, RC_A_T, RC_A_TQ, RC_A_TALB,
     RC_A_EDIR, RC_A_SV, RC_A_LM, RC_A_OCC, RC_A_EM, RC_A_REF, RC_A_VALID, RC_A_PW, RC_A_ORDER, RC_A_OBJBOX,
-    RC_A_MESH_V, RC_A_MESH_ALB, RC_A_MESH_DS, RC_A_COUNT
+    RC_A_MESH_V, RC_A_MESH_ALB, RC_A_MESH_DS, RC_A_VOL, RC_A_OCCF, RC_A_COUNT
 };
 /* cache arrays (rc_cache_get / rc_cache_set); M is SoA [nl][Np], LC / EDIR rows are AoS [Np][nl] */
 enum {
@@ -112,6 +112,11 @@ rc_status rc_scene_array(const rc_scene* s, int which, int state, void* dst, siz
  * trav: 0 = binary BVH in the prototype's order, 1 = binary closest-hit, 2 = 4-wide closest-hit. */
 rc_status rc_scene_upload(rc_scene* s, int trav);
 rc_status rc_scene_set_trav(rc_scene* s, int trav);
+/* Fine voxel grid g^3 of the per-light volumes (before upload; 0 disables). Every lit shadow segment
+ * of the state build is DDA-marked into its light's volume (the stepping rule of dda_reg,
+ * cuda/poc_gpu.cu:171-190); a geometry event then affects the lights whose volume, before or
+ * after, holds a voxel of the moving object, and targeted invalidation resets their entries. */
+rc_status rc_scene_set_fine_grid(rc_scene* s, int g);
 /* Converged per-state reference by fresh-ray Jacobi iterations, two chains, until the chains'
  * relative RMS difference <= target or max_iter (cuda/prep_mesh.cuh:454-520). noise_out[NS]. */
 rc_status rc_scene_reference(rc_scene* s, int K, int warm, int block, int max_iter, float target, float* noise_out, int* iters_out);

@@ -20,7 +20,7 @@ class OrcScene(C.Structure):
         ("sun_len", C.c_float), ("nst", C.c_int32), ("nl", C.c_int32), ("Np", C.c_int32), ("NR", C.c_int32),
         ("vx", C.c_int32), ("vy", C.c_int32), ("vz", C.c_int32), ("nw", C.c_int32),
         ("P", _P), ("N", _P), ("Q", _P), ("dims", _P), ("off", _P), ("T", _P), ("TQ", _P), ("nt", C.c_int32),
-        ("Edir", _P), ("SV", _P), ("LM", _P), ("inten", _P),
+        ("Edir", _P), ("SV", _P), ("LM", _P), ("inten", _P), ("VOL", _P), ("fg", C.c_int32),
     ]
 
 
@@ -87,7 +87,7 @@ def _load():
     lib.orc_k_trace2_f64.argtypes = [C.POINTER(OrcScene), _P, _P, _P, _P, C.c_int, C.c_uint64, _P]
     lib.orc_k_update_f64.argtypes = [C.POINTER(OrcScene), _P, _P, C.POINTER(OrcParams), _P, _P, _P]
     lib.orc_runner_create.restype = _P
-    lib.orc_runner_create.argtypes = [_P, C.c_int, C.POINTER(OrcMethod), C.c_int, _P, _P, _P, _P, _P, _P, _P]
+    lib.orc_runner_create.argtypes = [_P, C.c_int, C.POINTER(OrcMethod), C.c_int, _P, _P, _P, _P, _P, _P, _P, _P, _P]
     lib.orc_runner_destroy.argtypes = [_P]
     lib.orc_runner_begin.argtypes = [_P, C.c_int, C.c_uint32]
     lib.orc_runner_frame.restype = C.c_double
@@ -132,9 +132,13 @@ class OracleScene:
         self.Edir = [np.zeros(i.Np * i.nl, np.float32) for _ in range(i.NS)]
         self.SV = [np.zeros(i.Np * i.nw, np.uint32) for _ in range(i.NS)]
         self.LM = [np.zeros(i.Np, np.uint64) for _ in range(i.NS)]
+        self.fg = i.fine
+        wpv = (i.fine ** 3 + 31) // 32 if i.fine > 0 else 0
+        self.VOL = [np.zeros(i.nl * wpv, np.uint32) for _ in range(i.NS)]
         for st in range(i.NS):
             s = self.states[st]
             s.Edir, s.SV, s.LM = ptr(self.Edir[st]), ptr(self.SV[st]), ptr(self.LM[st])
+            s.VOL, s.fg = (ptr(self.VOL[st]) if wpv else None), i.fine
             if build:
                 lib.orc_k_build(C.byref(s), trav, s.Edir, s.SV, s.LM)
         occ = [g("OCC", k) for k in range(i.NS * i.NS)]
@@ -143,6 +147,9 @@ class OracleScene:
         self.occ_off = np.concatenate([[0], np.cumsum([len(x) for x in occ])]).astype(np.int64)
         self.em_flat = np.concatenate(em + [np.zeros(1, np.int32)]).astype(np.int32)
         self.em_off = np.concatenate([[0], np.cumsum([len(x) for x in em])]).astype(np.int64)
+        occf = [g("OCCF", k) for k in range(i.NS * i.NS)]
+        self.occf_flat = np.concatenate(occf + [np.zeros(1, np.int32)]).astype(np.int32)
+        self.occf_off = np.concatenate([[0], np.cumsum([len(x) for x in occf])]).astype(np.int64)
 
     def state(self, st):
         return C.byref(self.states[st])
@@ -175,7 +182,8 @@ class OracleRunner:
         self.m = method_from(method)
         self.h = lib.orc_runner_create(C.cast(osc.states, _P), osc.NS, C.byref(self.m), trav, ptr(osc.occ_flat), ptr(osc.occ_off),
                                        ptr(osc.em_flat), ptr(osc.em_off), C.cast(self.pw, _P) if self.pw else None,
-                                       C.cast(self.rf, _P) if self.rf else None, C.cast(self.vd, _P) if self.vd else None)
+                                       C.cast(self.rf, _P) if self.rf else None, C.cast(self.vd, _P) if self.vd else None,
+                                       ptr(osc.occf_flat), ptr(osc.occf_off))
 
     def __del__(self):
         if getattr(self, "h", None):

@@ -444,6 +444,8 @@ static void dda_walk(const orc_scene* s, int gx, int gy, int gz, v3 a, v3 b, orc
     }
 }
 static void set_mask_bit(void* ctx, long long v) { uint32_t* m = (uint32_t*)ctx; m[v >> 5] |= 1u << (v & 31); }
+static void set_mask_bit_atomic(void* ctx, long long v) { uint32_t* m = (uint32_t*)ctx; __atomic_fetch_or(&m[v >> 5], 1u << (v & 31), __ATOMIC_RELAXED); }
+static long long fine_words(int g) { return ((long long)g * g * g + 31) / 32; }
 void orc_dda_reg(const orc_scene* s, const float a[3], const float b[3], uint32_t* m) {
     dda_walk(s, s->vx, s->vy, s->vz, V3(a[0], a[1], a[2]), V3(b[0], b[1], b[2]), set_mask_bit, m);
 }
@@ -489,8 +491,12 @@ void orc_k_build(const orc_scene* s, int trav, float* Edir, uint32_t* SV, uint64
             if (s->ltype[l] == 1) e = s->inten[l] * fmaxf(c, 0.f) * (vis ? 1.f : 0.f);
             else { float rr = fmaxf(r, 0.05f); e = s->inten[l] * fmaxf(c, 0.f) / (rr * rr) * (vis ? 1.f : 0.f); }
             Edir[(size_t)s->nl * i + l] = e;
-            if (vis && c > 0.f) lm |= 1ull << (l < 63 ? l : 63);
-            rast_seg(s, O, add(O, mul(D, fminf(t, r))), m);
+            v3 end = add(O, mul(D, fminf(t, r)));
+            if (vis && c > 0.f) {
+                lm |= 1ull << (l < 63 ? l : 63);
+                if (s->fg > 0 && s->VOL) dda_walk(s, s->fg, s->fg, s->fg, O, end, set_mask_bit_atomic, s->VOL + (long long)l * fine_words(s->fg));
+            }
+            rast_seg(s, O, end, m);
         }
         for (int w = 0; w < s->nw; ++w) SV[(size_t)s->nw * i + w] = m[w];
         LM[i] = lm;
@@ -953,6 +959,7 @@ void orc_k_update_f64(const orc_scene* s, const float* n_before, const int32_t* 
 struct orc_runner {
     const orc_scene* st; int NS; orc_method M; int trav;
     const int32_t* occ; const int64_t* occ_off; const int32_t* em; const int64_t* em_off;
+    const int32_t* occf; const int64_t* occf_off;
     const float* const* Pw; const float* const* ref; const uint8_t* const* valid;
     orc_cache c; orc_params p;
     uint64_t* lnew; float* snap;
@@ -964,8 +971,10 @@ struct orc_runner {
 
 orc_runner* orc_runner_create(const orc_scene* states, int NS, const orc_method* m, int trav, const int32_t* occ_flat,
                               const int64_t* occ_off, const int32_t* em_flat, const int64_t* em_off, const float* const* Pw,
-                              const float* const* ref, const uint8_t* const* valid) {
+                              const float* const* ref, const uint8_t* const* valid, const int32_t* occf_flat,
+                              const int64_t* occf_off) {
     orc_runner* r = (orc_runner*)calloc(1, sizeof(orc_runner));
+    r->occf = occf_flat; r->occf_off = occf_off;
     r->st = states; r->NS = NS; r->M = *m; r->trav = trav;
     r->occ = occ_flat; r->occ_off = occ_off; r->em = em_flat; r->em_off = em_off;
     r->Pw = Pw; r->ref = ref; r->valid = valid;
@@ -1085,6 +1094,17 @@ static void event_block(orc_runner* r, int e, int si, int* invalidated) {
         uint64_t inv = 0;
         for (int l = 0; l < nl; ++l) if (ie[l] != is[l]) inv |= 1ull << (l < 63 ? l : 63);
         int nv = (int)(r->occ_off[pe + 1] - r->occ_off[pe]);
+        const orc_scene *sa = &r->st[e], *sb = &r->st[si];
+        int nf = r->occf ? (int)(r->occf_off[pe + 1] - r->occf_off[pe]) : 0;
+        if (s->fg > 0 && nf > 0 && sa->VOL && sb->VOL) {   /* lights whose lit segments cross the object */
+            const long long wpv = fine_words(s->fg);
+            for (int l = 0; l < nl; ++l)
+                for (int j = 0; j < nf; ++j) {
+                    int v = r->occf[r->occf_off[pe] + j];
+                    uint32_t w = sa->VOL[l * wpv + (v >> 5)] | sb->VOL[l * wpv + (v >> 5)];
+                    if ((w >> (v & 31)) & 1u) { inv |= 1ull << (l < 63 ? l : 63); break; }
+                }
+        }
         if (inv || nv) {
             c->npend[1] = 0;
             orc_k_invalidate(c, Np, nl, inv, r->occ + r->occ_off[pe], nv, s->nw);

@@ -40,6 +40,7 @@ typedef struct orc_scene {
     const float* P; const float* N; const float* Q; const int32_t* dims; const int32_t* off;
     const float* T; const int32_t* TQ; int32_t nt;
     const float* Edir; const uint32_t* SV; const uint64_t* LM; const float* inten;
+    uint32_t* VOL; int32_t fg;       /* per-light fine volumes [nl][ceil(fg^3/32)] (fg = 0: none) */
 } orc_scene;
 
 typedef struct orc_cache {
@@ -110,6 +111,7 @@ void orc_dda_volume(const orc_scene* s, int gx, int gy, int gz, const float* A, 
                     uint32_t* vol, long long words_per_vol);
 
 /* ---- frame kernels (one call = the whole grid) ---- */
+/* also marks each lit shadow segment into s->VOL[l] when s->fg > 0 */
 void orc_k_build(const orc_scene* s, int trav, float* Edir, uint32_t* SV, uint64_t* LM);
 void orc_k_trace2(const orc_scene* s, int trav, int deps, const orc_cache* c, const orc_params* p, uint64_t* lnew, int32_t* hits, int kmax);
 void orc_k_decide(const orc_scene* s, const orc_cache* c, const orc_params* p);
@@ -157,7 +159,8 @@ typedef struct orc_method {
  * library's (occ[a*NS+b], em[(a*NS+b)*nl+l]) given as offsets into one flat int array. */
 orc_runner* orc_runner_create(const orc_scene* states, int NS, const orc_method* m, int trav, const int32_t* occ_flat,
                               const int64_t* occ_off, const int32_t* em_flat, const int64_t* em_off, const float* const* Pw,
-                              const float* const* ref, const uint8_t* const* valid);
+                              const float* const* ref, const uint8_t* const* valid, const int32_t* occf_flat,
+                              const int64_t* occf_off);
 void orc_runner_destroy(orc_runner* r);
 void orc_runner_begin(orc_runner* r, int state0, uint32_t seed);
 /* one frame; returns the relative RMS error (or -1 without a reference) */

@@ -36,7 +36,7 @@ class Camera(C.Structure):
 class SceneInfo(C.Structure):
     _fields_ = [(n, C.c_int) for n in (
         "Np", "npst", "nbox", "nst", "nnodes", "nwnodes", "NS", "nl", "NR", "Nq", "hsize", "vx", "vy", "vz", "nw",
-        "bvh_depth", "wide_depth", "has_ref")] + [
+        "bvh_depth", "wide_depth", "has_ref", "fine")] + [
         ("h", C.c_float), ("lo", C.c_float * 3), ("ext", C.c_float * 3), ("diag", C.c_float), ("sun_len", C.c_float),
         ("cam", Camera),
     ]
@@ -52,7 +52,7 @@ class FrameStats(C.Structure):
 # scene arrays (enum order of include/rc.h)
 A_NAMES = ["ST", "STN", "BN", "WN", "HK", "HV", "P", "N", "ALB", "REGION", "L", "LTYPE", "INTEN", "Q", "DIMS", "OFF", "T",
            "TQ", "TALB", "EDIR", "SV", "LM", "OCC", "EM", "REF", "VALID", "PW", "ORDER", "OBJBOX", "MESH_V", "MESH_ALB",
-           "MESH_DS"]
+           "MESH_DS", "VOL", "OCCF"]
 A = {n: i for i, n in enumerate(A_NAMES)}
 C_NAMES = ["LC", "M", "N", "KK", "PENDING", "CUR", "PREV", "LCUR", "LPREV", "SUMPR", "NPEND", "AGE", "STALE", "C0", "RSTAT",
            "LNEW", "DSTAT", "R"]
@@ -73,6 +73,7 @@ _SIGS = {
     "rc_scene_array": (C.c_int, [_P, C.c_int, C.c_int, _P, C.c_size_t]),
     "rc_scene_upload": (C.c_int, [_P, C.c_int]),
     "rc_scene_set_trav": (C.c_int, [_P, C.c_int]),
+    "rc_scene_set_fine_grid": (C.c_int, [_P, C.c_int]),
     "rc_scene_reference": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_float, C.POINTER(C.c_float),
                                      C.POINTER(C.c_int)]),
     "rc_cache_create": (C.c_int, [_P, C.POINTER(Method), C.POINTER(_P)]),

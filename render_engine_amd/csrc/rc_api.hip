@@ -40,6 +40,8 @@ extern "C" int rc_device_count(void) {
     return n;
 }
 
+static inline size_t fine_words(int g) { return ((size_t)g * g * g + 31) / 32; }
+
 template <class T> static hipError_t dput(T** d, const std::vector<T>& v, std::vector<void*>& owned) {
     size_t bytes = std::max<size_t>(v.size() * sizeof(T), 16);
     hipError_t e = hipMalloc((void**)d, bytes);
@@ -62,7 +64,8 @@ struct rc_scene {
     hipStream_t stream = nullptr;
     rc_dscene ds{};
     std::vector<rc_dstate> dst;
-    std::vector<int*> dOcc, dEm;
+    std::vector<int*> dOcc, dEm, dOccF;
+    unsigned long long* d_lights = nullptr;   // event light set (k_vol_lights)
     std::vector<void*> owned;
     // converged references (host + device)
     bool has_ref = false;
@@ -107,7 +110,7 @@ extern "C" rc_status rc_scene_info_get(const rc_scene* s, rc_scene_info* o) {
     o->NS = H.NS; o->nl = H.nl; o->NR = H.NR; o->Nq = H.Nq; o->hsize = (int)H.HK.size();
     o->vx = H.vx; o->vy = H.vy; o->vz = H.vz; o->nw = H.nw; o->bvh_depth = H.bvh_depth; o->wide_depth = H.wide_depth;
     o->has_ref = s->has_ref ? 1 : 0;
-    o->h = H.h; o->diag = H.diag; o->sun_len = H.sun_len;
+    o->h = H.h; o->diag = H.diag; o->sun_len = H.sun_len; o->fine = H.fine;
     for (int a = 0; a < 3; ++a) { o->lo[a] = H.lo[a]; o->ext[a] = H.ext[a]; }
     memcpy(&o->cam, &H.cam, sizeof(rc_camera));
     return RC_OK;
@@ -144,6 +147,7 @@ static bool host_array(const rc_scene* s, int which, int st, const void** p, siz
         case RC_A_TQ: if (!ok_state(H.NS)) return false; RET(H.TQ[st]);
         case RC_A_OBJBOX: if (!ok_state(H.NS)) return false; RET(H.objbox[st]);
         case RC_A_OCC: if (!ok_state(H.NS * H.NS)) return false; RET(H.occ[st]);
+        case RC_A_OCCF: if (!ok_state(H.NS * H.NS)) return false; RET(H.occ_fine[st]);
         case RC_A_EM: if (!ok_state(H.NS * H.NS * H.nl)) return false; RET(H.em[st]);
         case RC_A_REF: if (!s->has_ref || !ok_state(H.NS)) return false; RET(s->ref[st]);
         case RC_A_VALID: if (!s->has_ref || !ok_state(H.NS)) return false; RET(s->valid[st]);
@@ -161,6 +165,7 @@ static bool device_array(const rc_scene* s, int which, int st, const void** dp, 
         case RC_A_EDIR: *dp = d.Edir; *bytes = Np * s->H.nl * sizeof(float); return true;
         case RC_A_SV: *dp = d.SV; *bytes = Np * s->H.nw * sizeof(uint32_t); return true;
         case RC_A_LM: *dp = d.LM; *bytes = Np * sizeof(uint64_t); return true;
+        case RC_A_VOL: if (!d.VOL) return false; *dp = d.VOL; *bytes = (size_t)s->H.nl * fine_words(s->H.fine) * 4; return true;
         default: return false;
     }
 }
@@ -196,6 +201,13 @@ static rc_status need_device() {
     return RC_OK;
 }
 
+extern "C" rc_status rc_scene_set_fine_grid(rc_scene* s, int g) {
+    if (!s || g < 0 || g > 1024) return fail(RC_ERR_ARG, "fine grid must be 0..1024");
+    if (s->uploaded) return fail(RC_ERR_STATE, "fine grid must be set before upload");
+    rc::fine_events(s->H, g);
+    return RC_OK;
+}
+
 extern "C" rc_status rc_scene_set_trav(rc_scene* s, int trav) {
     if (!s || trav < 0 || trav > 2) return fail(RC_ERR_ARG, "trav must be 0, 1 or 2");
     s->trav = trav;
@@ -227,6 +239,7 @@ extern "C" rc_status rc_scene_upload(rc_scene* s, int trav) {
     d.sun_len = H.sun_len; d.nst = H.nst; d.nl = H.nl; d.Np = H.Np; d.NR = H.NR;
     d.vx = H.vx; d.vy = H.vy; d.vz = H.vz; d.nw = H.nw;
     d.wstack = 3 * H.wide_depth + 1;
+    d.fg = H.fine;
     int *dDims, *dOff;
     HIPCK(dput(&dDims, H.dims, s->owned)); HIPCK(dput(&dOff, H.off, s->owned));
     s->dst.resize(H.NS);
@@ -238,11 +251,13 @@ extern "C" rc_status rc_scene_upload(rc_scene* s, int trav) {
         HIPCK(dput(&T, H.T[k], s->owned)); HIPCK(dput(&TQ, H.TQ[k], s->owned)); HIPCK(dput(&I, H.inten[k], s->owned));
         HIPCK(dalloc(&E, (size_t)H.nl * H.Np, s->owned)); HIPCK(dalloc(&SV, (size_t)H.nw * H.Np, s->owned));
         HIPCK(dalloc(&LM, (size_t)H.Np, s->owned));
+        if (H.fine > 0) { uint32_t* V; HIPCK(dalloc(&V, (size_t)H.nl * fine_words(H.fine), s->owned)); x.VOL = V; }
         x.P = P; x.N = N; x.Q = Q; x.dims = dDims; x.off = dOff; x.T = T; x.TQ = TQ; x.Edir = E; x.SV = SV; x.LM = LM; x.inten = I;
         x.nt = H.nt[k];
     }
     for (auto& v : H.occ) { int* p; std::vector<int> w = v.empty() ? std::vector<int>{0} : v; HIPCK(dput(&p, w, s->owned)); s->dOcc.push_back(p); }
     for (auto& v : H.em) { int* p; std::vector<int> w = v.empty() ? std::vector<int>{0} : v; HIPCK(dput(&p, w, s->owned)); s->dEm.push_back(p); }
+    for (auto& v : H.occ_fine) { int* p; std::vector<int> w = v.empty() ? std::vector<int>{0} : v; HIPCK(dput(&p, w, s->owned)); s->dOccF.push_back(p); }
     for (int k = 0; k < H.NS; ++k) HIPCK(rc_launch_build(trav, &s->ds, &s->dst[k], s->stream));
     HIPCK(hipStreamSynchronize(s->stream));
     s->uploaded = true;
@@ -500,6 +515,16 @@ static rc_status run_event(rc_cache* c, int e, int si, int& invalidated) {
         uint64_t inv = 0;
         for (int l = 0; l < nl; ++l) if (H.inten[e][l] != H.inten[si][l]) inv |= 1ull << (l < 63 ? l : 63);
         int nv = (int)H.occ[pe].size();
+        if (H.fine > 0 && !H.occ_fine[pe].empty()) {   // lights whose lit segments cross the object (before or after)
+            if (!S->d_lights) { HIPCK(hipMalloc((void**)&S->d_lights, 8)); S->owned.push_back(S->d_lights); }
+            HIPCK(hipMemsetAsync(S->d_lights, 0, 8, st));
+            HIPCK(rc_launch_vol_lights(nl, S->dst[e].VOL, S->dst[si].VOL, (long long)fine_words(H.fine), S->dOccF[pe],
+                                       (int)H.occ_fine[pe].size(), S->d_lights, st));
+            unsigned long long lights = 0;
+            HIPCK(hipMemcpyAsync(&lights, S->d_lights, 8, hipMemcpyDeviceToHost, st));
+            HIPCK(hipStreamSynchronize(st));
+            inv |= lights;
+        }
         if (inv || nv) {
             HIPCK(hipMemsetAsync(d.npend + 1, 0, 4, st));
             HIPCK(rc_launch_invalidate(&d, Np, nl, inv, S->dOcc[pe], nv, H.nw, st));

@@ -432,3 +432,29 @@ __device__ __forceinline__ void rc_dda_reg(const rc_dscene& s, f3 a, f3 b, uint3
 __device__ __forceinline__ void rc_rast_seg(const rc_dscene& s, f3 a, f3 b, uint32_t (&m)[RC_MAX_NW]) {
     for (int k = 0; k < 20; ++k) rc_set_bit(m, rc_vox_of(s, a + (b - a) * ((float)k / 19.f)));
 }
+
+// Amanatides-Woo walk of segment a->b over a g^3 voxel grid on the scene bounds (the stepping rule
+// of rc_dda_reg), setting each crossed voxel's bit in V; a bit already set costs one load.
+__device__ __forceinline__ void rc_dda_volume_mark(const rc_dscene& s, int gx, int gy, int gz, f3 a, f3 b, uint32_t* V) {
+    const float sx = gx / s.ext[0], sy = gy / s.ext[1], sz = gz / s.ext[2];
+    float ax = (a.x - s.lo[0]) * sx, ay = (a.y - s.lo[1]) * sy, az = (a.z - s.lo[2]) * sz;
+    float bx = (b.x - s.lo[0]) * sx, by = (b.y - s.lo[1]) * sy, bz = (b.z - s.lo[2]) * sz;
+    int ix = min(max((int)ax, 0), gx - 1), iy = min(max((int)ay, 0), gy - 1), iz = min(max((int)az, 0), gz - 1);
+    int ex = min(max((int)bx, 0), gx - 1), ey = min(max((int)by, 0), gy - 1), ez = min(max((int)bz, 0), gz - 1);
+    float dx = bx - ax, dy = by - ay, dz = bz - az;
+    int stx = dx > 0 ? 1 : -1, sty = dy > 0 ? 1 : -1, stz = dz > 0 ? 1 : -1;
+    float tdx = dx != 0 ? fabsf(1.f / dx) : 1e30f, tdy = dy != 0 ? fabsf(1.f / dy) : 1e30f, tdz = dz != 0 ? fabsf(1.f / dz) : 1e30f;
+    float tmx = dx != 0 ? ((dx > 0 ? ((float)(ix + 1) - ax) : (ax - (float)ix)) * tdx) : 1e30f;
+    float tmy = dy != 0 ? ((dy > 0 ? ((float)(iy + 1) - ay) : (ay - (float)iy)) * tdy) : 1e30f;
+    float tmz = dz != 0 ? ((dz > 0 ? ((float)(iz + 1) - az) : (az - (float)iz)) * tdz) : 1e30f;
+    for (int g = 0; g < gx + gy + gz; ++g) {
+        int64_t v = ((int64_t)iz * gy + iy) * gx + ix;
+        uint32_t bit = 1u << (v & 31);
+        uint32_t* w = V + (v >> 5);
+        if (!(*w & bit)) atomicOr(w, bit);
+        if (ix == ex && iy == ey && iz == ez) break;
+        if (tmx < tmy && tmx < tmz) { ix += stx; tmx += tdx; if (ix < 0 || ix >= gx) break; }
+        else if (tmy < tmz) { iy += sty; tmy += tdy; if (iy < 0 || iy >= gy) break; }
+        else { iz += stz; tmz += tdz; if (iz < 0 || iz >= gz) break; }
+    }
+}

@@ -34,3 +34,5 @@ hipError_t rc_launch_hash_point(const rc_dscene* s, const float* H, const float*
 hipError_t rc_launch_dda_segments(const rc_dscene* s, const float* A, const float* B, int n, uint32_t* out, hipStream_t stream);
 hipError_t rc_launch_dda_volume(const rc_dscene* s, int gx, int gy, int gz, const float* A, const float* B, const int* seg_vol, int n,
                                 uint32_t* vol, long long words_per_vol, hipStream_t stream);
+hipError_t rc_launch_vol_lights(int nl, const uint32_t* Va, const uint32_t* Vb, long long words_per_vol, const int* vox, int nvox,
+                                unsigned long long* out, hipStream_t stream);

@@ -91,8 +91,12 @@ __global__ void __launch_bounds__(RC_TB) k_build(rc_dscene s, rc_dstate st) {
         if (s.ltype[l] == 1) e = st.inten[l] * fmaxf(c, 0.f) * (vis ? 1.f : 0.f);
         else { float rr = fmaxf(r, 0.05f); e = st.inten[l] * fmaxf(c, 0.f) / (rr * rr) * (vis ? 1.f : 0.f); }
         st.Edir[(int64_t)s.nl * i + l] = e;
-        if (vis && c > 0.f) lm |= 1ull << (l < 63 ? l : 63);
-        rc_rast_seg(s, O, O + D * fminf(t, r), m);
+        f3 end = O + D * fminf(t, r);
+        if (vis && c > 0.f) {
+            lm |= 1ull << (l < 63 ? l : 63);
+            if (s.fg > 0) rc_dda_volume_mark(s, s.fg, s.fg, s.fg, O, end, st.VOL + (int64_t)l * (((int64_t)s.fg * s.fg * s.fg + 31) >> 5));
+        }
+        rc_rast_seg(s, O, end, m);
     }
     for (int w = 0; w < s.nw; ++w) st.SV[(int64_t)s.nw * i + w] = m[w];
     st.LM[i] = lm;
@@ -681,29 +685,24 @@ __global__ void __launch_bounds__(RC_TB) k_dda_volume(rc_dscene s, int gx, int g
                                                      const int* seg_vol, int n, uint32_t* vol, long long words_per_vol) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    f3 a = ld3(A, i), b = ld3(B, i);
     uint32_t* V = vol + (seg_vol ? (int64_t)seg_vol[i] * words_per_vol : 0);
-    const float sx = gx / s.ext[0], sy = gy / s.ext[1], sz = gz / s.ext[2];
-    float ax = (a.x - s.lo[0]) * sx, ay = (a.y - s.lo[1]) * sy, az = (a.z - s.lo[2]) * sz;
-    float bx = (b.x - s.lo[0]) * sx, by = (b.y - s.lo[1]) * sy, bz = (b.z - s.lo[2]) * sz;
-    int ix = min(max((int)ax, 0), gx - 1), iy = min(max((int)ay, 0), gy - 1), iz = min(max((int)az, 0), gz - 1);
-    int ex = min(max((int)bx, 0), gx - 1), ey = min(max((int)by, 0), gy - 1), ez = min(max((int)bz, 0), gz - 1);
-    float dx = bx - ax, dy = by - ay, dz = bz - az;
-    int stx = dx > 0 ? 1 : -1, sty = dy > 0 ? 1 : -1, stz = dz > 0 ? 1 : -1;
-    float tdx = dx != 0 ? fabsf(1.f / dx) : 1e30f, tdy = dy != 0 ? fabsf(1.f / dy) : 1e30f, tdz = dz != 0 ? fabsf(1.f / dz) : 1e30f;
-    float tmx = dx != 0 ? ((dx > 0 ? ((float)(ix + 1) - ax) : (ax - (float)ix)) * tdx) : 1e30f;
-    float tmy = dy != 0 ? ((dy > 0 ? ((float)(iy + 1) - ay) : (ay - (float)iy)) * tdy) : 1e30f;
-    float tmz = dz != 0 ? ((dz > 0 ? ((float)(iz + 1) - az) : (az - (float)iz)) * tdz) : 1e30f;
-    for (int g = 0; g < gx + gy + gz; ++g) {
-        int64_t v = ((int64_t)iz * gy + iy) * gx + ix;
-        uint32_t bit = 1u << (v & 31);
-        uint32_t* w = V + (v >> 5);
-        if (!(*w & bit)) atomicOr(w, bit);
-        if (ix == ex && iy == ey && iz == ez) break;
-        if (tmx < tmy && tmx < tmz) { ix += stx; tmx += tdx; if (ix < 0 || ix >= gx) break; }
-        else if (tmy < tmz) { iy += sty; tmy += tdy; if (iy < 0 || iy >= gy) break; }
-        else { iz += stz; tmz += tdz; if (iz < 0 || iz >= gz) break; }
+    rc_dda_volume_mark(s, gx, gy, gz, ld3(A, i), ld3(B, i), V);
+}
+
+// Lights affected by a geometry event: light l is affected when its fine volume in the state
+// before (Va) or after (Vb) the event has a bit among the event's fine voxels. One block per
+// light; the result is ORed into out[0] as bit min(l, 63).
+__global__ void __launch_bounds__(RC_TB) k_vol_lights(const uint32_t* Va, const uint32_t* Vb, long long words_per_vol,
+                                                     const int* vox, int nvox, unsigned long long* out) {
+    const int l = blockIdx.x;
+    const uint32_t* a = Va + (int64_t)l * words_per_vol;
+    const uint32_t* b = Vb + (int64_t)l * words_per_vol;
+    int hit = 0;
+    for (int j = threadIdx.x; j < nvox && !hit; j += blockDim.x) {
+        int v = vox[j];
+        hit = ((a[v >> 5] | b[v >> 5]) >> (v & 31)) & 1u;
     }
+    if (__syncthreads_or(hit) && threadIdx.x == 0) atomicOr(out, 1ull << (l < 63 ? l : 63));
 }
 
 // ------------------------------------------------------------------------------ launchers
@@ -847,5 +846,12 @@ hipError_t rc_launch_dda_volume(const rc_dscene* s, int gx, int gy, int gz, cons
                                 uint32_t* vol, long long words_per_vol, hipStream_t stream) {
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_dda_volume, dim3(rc_grid(n)), dim3(RC_TB), 0, stream, *s, gx, gy, gz, A, B, seg_vol, n, vol, words_per_vol);
+    return hipGetLastError();
+}
+
+hipError_t rc_launch_vol_lights(int nl, const uint32_t* Va, const uint32_t* Vb, long long words_per_vol, const int* vox, int nvox,
+                                unsigned long long* out, hipStream_t stream) {
+    if (nl <= 0 || nvox <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_vol_lights, dim3(nl), dim3(RC_TB), 0, stream, Va, Vb, words_per_vol, vox, nvox, out);
     return hipGetLastError();
 }

@@ -429,6 +429,8 @@ std::string prepare(const SceneDesc& d, Scene& S) {
         S.nt.push_back((int)TQ.size()); S.present.push_back(pres); S.objbox.push_back(ob);
     }
 
+    S.objects = d.objects;
+    fine_events(S, d.fine);
     // 4. events per ordered state pair: transit voxels overlapped by every object that moves,
     // appears or disappears (both placements), and per light the voxel of a light turning on
     // (a directional light turning on touches every voxel).
@@ -464,6 +466,35 @@ std::string prepare(const SceneDesc& d, Scene& S) {
             }
         }
     return "";
+}
+
+void fine_events(Scene& S, int g) {
+    S.fine = g;
+    S.occ_fine.assign((size_t)S.NS * S.NS, {});
+    if (g <= 0 || S.objects.empty()) return;
+    const int nobj = (int)S.objects[0].size();
+    auto cell = [&](float x, int a) { return std::min(std::max((int)((x - S.lo[a]) / S.ext[a] * g), 0), g - 1); };
+    for (int a = 0; a < S.NS; ++a)
+        for (int b = 0; b < S.NS; ++b) {
+            if (a == b) continue;
+            std::vector<int64_t> vs;
+            for (int o = 0; o < nobj; ++o) {
+                const DynBox &A_ = S.objects[a][o], &B_ = S.objects[b][o];
+                bool same = A_.present == B_.present && (!A_.present || (A_.c[0] == B_.c[0] && A_.c[1] == B_.c[1] && A_.c[2] == B_.c[2]));
+                if (same) continue;
+                for (const DynBox* X : {&A_, &B_}) {
+                    if (!X->present) continue;
+                    int i0 = cell(X->c[0] - X->e[0], 0), i1 = cell(X->c[0] + X->e[0], 0);
+                    int j0 = cell(X->c[1] - X->e[1], 1), j1 = cell(X->c[1] + X->e[1], 1);
+                    int k0 = cell(X->c[2] - X->e[2], 2), k1 = cell(X->c[2] + X->e[2], 2);
+                    for (int k = k0; k <= k1; ++k) for (int j = j0; j <= j1; ++j) for (int i = i0; i <= i1; ++i)
+                        vs.push_back(((int64_t)k * g + j) * g + i);
+                }
+            }
+            std::sort(vs.begin(), vs.end());
+            vs.erase(std::unique(vs.begin(), vs.end()), vs.end());
+            for (int64_t v : vs) S.occ_fine[(size_t)a * S.NS + b].push_back((int)v);
+        }
 }
 
 // ------------------------------------------------------------------------------------ generators
@@ -535,6 +566,7 @@ SceneDesc gen_box_room(uint64_t seed, int target_tris) {
     d.inten = {{50.f, 50.f}, {50.f, 0.f}};
     d.h = 0.05f;
     d.table_log2 = 18;
+    d.fine = 64;
     d.vx = 8; d.vy = 4; d.vz = 4;
     float pos[3] = {0.4f, 0.6f, 2.2f}, at[3] = {6.5f, 2.8f, 0.6f};
     look_at(d.cam, pos, at, 75.f, 256, 256);
@@ -599,6 +631,7 @@ SceneDesc gen_street(uint64_t seed, int nboxes, int target_tris, int nlamps, boo
     d.objects[3][0].present = true;
     d.h = 0.f;
     d.target_points = target_points;
+    d.fine = target_points > 800000 ? 256 : 128;
     d.table_log2 = table_log2;
     d.vx = 8; d.vy = 4; d.vz = 4;
     float at[3] = {cx0 * pitch + 2.f, cy0 * pitch + 1.f, 0.5f}, pos[3] = {cx0 * pitch - 22.f, cy0 * pitch - 18.f, 14.f};

@@ -48,6 +48,7 @@ struct SceneDesc {
     int vx = 8, vy = 4, vz = 4;     // transit grid (cuda/build.sh:12)
     rc_camera cam{};
     float dyn_albedo = 0.7f;
+    int fine = 0;                   // fine voxel grid fine^3 for per-light volumes (0: none)
 };
 
 struct Scene {
@@ -83,7 +84,14 @@ struct Scene {
     int vx = 8, vy = 4, vz = 4, nw = 4;
     rc_camera cam{};
     std::vector<std::vector<float>> objbox;          // per state, per object: lo3 hi3 (for fine voxel events)
+    std::vector<std::vector<DynBox>> objects;        // per state, per object
+    int fine = 0;
+    std::vector<std::vector<int>> occ_fine;          // per ordered state pair: fine voxels of moving objects
 };
+
+// Fine-grid voxels (g^3 over the bounds) overlapped by every object that moves, appears or
+// disappears between states a and b (both placements), for every ordered pair.
+void fine_events(Scene& S, int g);
 
 // Binned-SAH binary BVH over the mesh (16 bins, leaves of <= 4 triangles or when SAH says so,
 // median split fallback, depth <= 56; cuda/prep_mesh.cuh:42-114).

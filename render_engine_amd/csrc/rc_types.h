@@ -52,6 +52,7 @@ typedef struct rc_dstate {
     const uint8_t* valid;    // [Np]
     const float* Pw;         // [NL Np] converged per-light reference (optional)
     const float* inten;      // [NL] light intensities of the state
+    uint32_t* VOL;           // [NL][fg^3 / 32] per-light fine voxel volumes of the lit shadow segments
     int32_t nt;              // dynamic triangles in this state
     int32_t pad0;
 } rc_dstate;
@@ -75,7 +76,8 @@ typedef struct rc_dscene {
     float sun_len;           // shadow-ray length toward a directional light
     int32_t nst, nl, Np, NR;
     int32_t vx, vy, vz, nw;  // transit grid (vx*vy*vz = 32 nw bits per entry)
-    int32_t wstack, pad0;    // LDS stack levels of the wide traversal (3 per wide-BVH level)
+    int32_t wstack;          // LDS stack levels of the wide traversal (3 per wide-BVH level)
+    int32_t fg;              // fine voxel grid fg^3 over the bounds (0: no fine volumes)
 } rc_dscene;
 
 // Cache buffers (cuda/poc_gpu.cu:394-409) without the v5-B projection buffers (out of scope).

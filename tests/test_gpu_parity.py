@@ -69,6 +69,8 @@ def test_build_bit_exact(which, request):
         assert np.array_equal(scene.array("EDIR", st).view(np.uint32), osc.Edir[st].view(np.uint32))
         assert np.array_equal(scene.array("SV", st), osc.SV[st])
         assert np.array_equal(scene.array("LM", st), osc.LM[st])
+        if scene.info.fine > 0:
+            assert np.array_equal(scene.array("VOL", st), osc.VOL[st]), f"fine volumes differ in state {st}"
 
 
 def test_hash_point_and_dda(torch_gpu, street):

@@ -327,3 +327,57 @@ def test_resolved_radiance(room):
     A = scene.array("ALB").astype(np.float64)
     R = run.get("R")
     assert np.allclose(R, A * (E + lc).sum(1), rtol=2e-6, atol=1e-7)
+
+
+def test_fine_volumes_hold_lit_segments(street):
+    """each lit shadow segment marks the voxel of its entry and of the light in the light's volume"""
+    scene, osc = street
+    i = scene.info
+    g = i.fine
+    assert g > 0
+    wpv = (g ** 3 + 31) // 32
+    lo, ext = np.array(list(i.lo), np.float64), np.array(list(i.ext), np.float64)
+    vox = lambda p: tuple(np.clip(((p - lo) / ext * g).astype(int), 0, g - 1))
+    P = scene.array("P", 0).reshape(-1, 3)
+    N = scene.array("N", 0).reshape(-1, 3)
+    L = scene.array("L").reshape(-1, 3)
+    lt = scene.array("LTYPE")
+    LM = osc.LM[0]
+    rng = np.random.default_rng(4)
+    checked = 0
+    for e in rng.integers(0, i.npst, 400):
+        for l in range(i.nl):
+            if lt[l] != 0 or not ((int(LM[e]) >> l) & 1):
+                continue
+            V = osc.VOL[0][l * wpv:(l + 1) * wpv]
+            for p in (P[e] + 1e-3 * N[e], L[l]):
+                x, y, z = vox(p.astype(np.float64))
+                v = (z * g + y) * g + x
+                assert (V[v >> 5] >> (v & 31)) & 1
+            checked += 1
+    assert checked > 50
+
+
+def test_occluder_event_light_set(street):
+    """inserting the wall next to lamp 0 affects lamp 0 and the sun (their lit segments cross it);
+    the targeted method invalidates every entry that recorded one of them"""
+    scene, osc = street
+    i = scene.info
+    ns = i.NS
+    occf = scene.array("OCCF", 0 * ns + 3)
+    assert len(occf) > 0
+    wpv = (i.fine ** 3 + 31) // 32
+    hit = set()
+    for l in range(i.nl):
+        V = osc.VOL[0][l * wpv:(l + 1) * wpv] | osc.VOL[3][l * wpv:(l + 1) * wpv]
+        if any((V[v >> 5] >> (v & 31)) & 1 for v in occf):
+            hit.add(l)
+    assert 0 in hit and (i.nl - 1) in hit
+    run = po.OracleRunner(osc, rea.methods.targeted())
+    run.begin(0, 5)
+    for _ in range(3):
+        run.frame(0)
+    deps = run.get("LCUR") | run.get("LPREV")
+    mask = sum(1 << min(l, 63) for l in hit)
+    _, inv = run.frame(3)
+    assert inv >= int(((deps & np.uint64(mask)) != 0).sum())
