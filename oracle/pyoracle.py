@@ -185,9 +185,9 @@ class OracleRunner:
                                        C.cast(self.rf, _P) if self.rf else None, C.cast(self.vd, _P) if self.vd else None,
                                        ptr(osc.occf_flat), ptr(osc.occf_off))
 
-    def __del__(self):
+    def __del__(self, _destroy=lib.orc_runner_destroy):
         if getattr(self, "h", None):
-            lib.orc_runner_destroy(self.h)
+            _destroy(self.h)
             self.h = None
 
     def begin(self, st, seed):

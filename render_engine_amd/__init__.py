@@ -70,6 +70,27 @@ class Scene:
                                   bounces, C.byref(h)))
         return cls(h)
 
+    @classmethod
+    def load(cls, path):
+        """prototype scene file (format 2 / 3); .gz files are decompressed to a temporary file"""
+        import gzip
+        import os
+        import shutil
+        import tempfile
+        tmp = None
+        if path.endswith(".gz"):
+            fd, tmp = tempfile.mkstemp(suffix=".bin")
+            with gzip.open(path, "rb") as src, os.fdopen(fd, "wb") as dst:
+                shutil.copyfileobj(src, dst)
+            path = tmp
+        try:
+            h = C.c_void_p()
+            check(lib.rc_scene_load(path.encode(), C.byref(h)))
+        finally:
+            if tmp:
+                os.unlink(tmp)
+        return cls(h)
+
     def __del__(self, _destroy=lib.rc_scene_destroy):
         h = getattr(self, "_h", None)
         if h is not None and h.value:

@@ -67,6 +67,7 @@ _SIGS = {
     "rc_scene_box_room": (C.c_int, [C.c_uint64, C.c_int, C.POINTER(_P)]),
     "rc_scene_street": (C.c_int, [C.c_uint64, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                   C.c_int, C.c_int, C.POINTER(_P)]),
+    "rc_scene_load": (C.c_int, [C.c_char_p, C.POINTER(_P)]),
     "rc_scene_destroy": (None, [_P]),
     "rc_scene_info_get": (C.c_int, [_P, C.POINTER(SceneInfo)]),
     "rc_scene_array_bytes": (C.c_size_t, [_P, C.c_int, C.c_int]),

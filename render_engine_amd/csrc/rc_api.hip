@@ -100,6 +100,17 @@ extern "C" rc_status rc_scene_street(uint64_t seed, int nboxes, int target_tris,
     return make_scene(rc::gen_street(seed, nboxes, target_tris, nlamps, sun != 0, target_points, table_log2, W, H, spp, bounces), out);
 }
 
+extern "C" rc_status rc_scene_load(const char* path, rc_scene** out) {
+    if (!path || !out) return fail(RC_ERR_ARG, "NULL argument");
+    rc_scene* s = new rc_scene();
+    rc::FileRefs R;
+    std::string e = rc::load_scene_file(path, s->H, R);
+    if (!e.empty()) { delete s; return fail(RC_ERR_SCENE, "scene file %s: %s", path, e.c_str()); }
+    if (R.has) { s->has_ref = true; s->ref = R.ref; s->Pw = R.Pw; s->valid = R.valid; }
+    *out = s;
+    return RC_OK;
+}
+
 extern "C" void rc_scene_destroy(rc_scene* s) { delete s; }
 
 extern "C" rc_status rc_scene_info_get(const rc_scene* s, rc_scene_info* o) {
@@ -234,7 +245,7 @@ extern "C" rc_status rc_scene_upload(rc_scene* s, int trav) {
     HIPCK(dput(&dL, H.L, s->owned)); HIPCK(dput(&dLT, H.ltype, s->owned)); HIPCK(dput(&dA, H.A, s->owned));
     HIPCK(dput(&dR, H.region, s->owned));
     d.ST = dST; d.ST4 = dST4; d.STN = dSTN; d.BN = dBN; d.WN = dWN; d.HK = dHK; d.HV = dHV; d.L = dL; d.ltype = dLT; d.Alb = dA; d.region = dR;
-    d.hmask = (uint32_t)(H.HK.size() - 1); d.hcell = H.h;
+    d.hmask = H.HK.empty() ? 0u : (uint32_t)(H.HK.size() - 1); d.hcell = H.h;
     for (int a = 0; a < 3; ++a) { d.lo[a] = H.lo[a]; d.ext[a] = H.ext[a]; }
     d.sun_len = H.sun_len; d.nst = H.nst; d.nl = H.nl; d.Np = H.Np; d.NR = H.NR;
     d.vx = H.vx; d.vy = H.vy; d.vz = H.vz; d.nw = H.nw;
@@ -260,6 +271,13 @@ extern "C" rc_status rc_scene_upload(rc_scene* s, int trav) {
     for (auto& v : H.occ_fine) { int* p; std::vector<int> w = v.empty() ? std::vector<int>{0} : v; HIPCK(dput(&p, w, s->owned)); s->dOccF.push_back(p); }
     for (int k = 0; k < H.NS; ++k) HIPCK(rc_launch_build(trav, &s->ds, &s->dst[k], s->stream));
     HIPCK(hipStreamSynchronize(s->stream));
+    if (s->has_ref) {   // references read from a scene file
+        for (int si = 0; si < H.NS; ++si) {
+            float *r, *pw; uint8_t* v;
+            HIPCK(dput(&r, s->ref[si], s->owned)); HIPCK(dput(&pw, s->Pw[si], s->owned)); HIPCK(dput(&v, s->valid[si], s->owned));
+            s->dst[si].ref = r; s->dst[si].Pw = pw; s->dst[si].valid = v;
+        }
+    }
     s->uploaded = true;
     return RC_OK;
 }

@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <array>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <numeric>
 #include <unordered_map>
@@ -495,6 +496,124 @@ void fine_events(Scene& S, int g) {
             vs.erase(std::unique(vs.begin(), vs.end()), vs.end());
             for (int64_t v : vs) S.occ_fine[(size_t)a * S.NS + b].push_back((int)v);
         }
+}
+
+// ------------------------------------------------------------------------------------ scene files
+namespace {
+struct Reader {
+    FILE* f = nullptr;
+    bool ok = true;
+    template <class T> void rd(std::vector<T>& v, size_t n) {
+        v.resize(n);
+        if (!ok) return;
+        if (n && fread(v.data(), sizeof(T), n, f) != n) ok = false;
+    }
+    template <class T> T one() { T x{}; if (ok && fread(&x, sizeof(T), 1, f) != 1) ok = false; return x; }
+    ~Reader() { if (f) fclose(f); }
+};
+}  // namespace
+
+// fine 16x8x6 event voxels of a box scene -> transit voxels they overlap (cuda/poc_gpu.cu:720-735)
+static std::vector<int> fold_to_transit(const std::vector<int>& fine, int VX, int VY, int VZ) {
+    const int FX = 16, FY = 8, FZ = 6;
+    const float X1 = 8.f, Y1 = 4.f, Z1 = 3.f;
+    std::vector<char> mark((size_t)VX * VY * VZ, 0);
+    for (int v : fine) {
+        int i = v % FX, j = (v / FX) % FY, k = v / (FX * FY);
+        float x0 = i * X1 / FX, x1 = (i + 1) * X1 / FX, y0 = j * Y1 / FY, y1 = (j + 1) * Y1 / FY;
+        float z0 = k * Z1 / FZ, z1 = (k + 1) * Z1 / FZ;
+        int a0 = std::min((int)(x0 / X1 * VX + 1e-4f), VX - 1), a1 = std::min((int)(x1 / X1 * VX - 1e-4f), VX - 1);
+        int b0 = std::min((int)(y0 / Y1 * VY + 1e-4f), VY - 1), b1 = std::min((int)(y1 / Y1 * VY - 1e-4f), VY - 1);
+        int c0 = std::min((int)(z0 / Z1 * VZ + 1e-4f), VZ - 1), c1 = std::min((int)(z1 / Z1 * VZ - 1e-4f), VZ - 1);
+        for (int a = a0; a <= a1; ++a) for (int b = b0; b <= b1; ++b) for (int c = c0; c <= c1; ++c) mark[((size_t)c * VY + b) * VX + a] = 1;
+    }
+    std::vector<int> out;
+    for (size_t v = 0; v < mark.size(); ++v) if (mark[v]) out.push_back((int)v);
+    return out;
+}
+
+std::string load_scene_file(const char* path, Scene& S, FileRefs& R) {
+    Reader in;
+    in.f = fopen(path, "rb");
+    if (!in.f) return std::string("cannot open ") + path;
+    std::vector<int> h;
+    in.rd(h, 8);
+    if (!in.ok) return "truncated header";
+    const int Np = h[0], Nq = h[1], Nt = h[2], NS = h[3], NL = h[4], NR = h[5], has_ref = h[6], fmt = h[7];
+    if (fmt != 2 && fmt != 3) return "scene format " + std::to_string(fmt) + " not supported (2 or 3)";
+    if (Np <= 0 || Nq < 0 || Nt < 0 || NS <= 0 || NL <= 0 || NL > RC_MAX_LIGHTS || NR <= 0) return "bad header values";
+    S = Scene();
+    S.vx = 8; S.vy = 4; S.vz = 4; S.nw = 4;
+    S.Np = Np; S.NS = NS; S.nl = NL; S.NR = NR; S.Nq = Nq;
+    S.lo[0] = S.lo[1] = S.lo[2] = 0.f; S.ext[0] = 8.f; S.ext[1] = 4.f; S.ext[2] = 3.f;
+    if (fmt == 3) {
+        std::vector<float> lo, ext, hc; std::vector<int> m;
+        in.rd(lo, 3); in.rd(ext, 3); in.rd(hc, 1); in.rd(m, 4);
+        if (!in.ok) return "truncated mesh header";
+        for (int a = 0; a < 3; ++a) { S.lo[a] = lo[a]; S.ext[a] = ext[a]; }
+        S.h = hc[0];
+        S.nst = m[0]; S.npst = m[3];
+        if (m[2] <= 0 || (m[2] & (m[2] - 1))) return "hash table size must be a power of two";
+        in.rd(S.ST, 9 * (size_t)m[0]); in.rd(S.STN, 4 * (size_t)m[0]);
+        std::vector<float> bn; in.rd(bn, 8 * (size_t)m[1]);
+        S.BN.resize(m[1]);
+        if (in.ok && m[1]) memcpy(S.BN.data(), bn.data(), bn.size() * 4);
+        in.rd(S.HK, m[2]); in.rd(S.HV, m[2]);
+        if (!in.ok) return "truncated mesh data";
+        build_wide(S.BN, S.WN, S.wide_depth);
+        if (3 * S.wide_depth + 1 > RC_WSTACK) return "wide BVH too deep for the LDS traversal stack";
+        S.order.resize(S.nst); std::iota(S.order.begin(), S.order.end(), 0);
+        S.Talb.assign(S.nst, 0.5f);
+    }
+    float dg = std::sqrt(S.ext[0] * S.ext[0] + S.ext[1] * S.ext[1] + S.ext[2] * S.ext[2]);
+    S.diag = dg; S.sun_len = 2.f * dg;
+    in.rd(S.L, 3 * (size_t)NL); in.rd(S.A, Np); in.rd(S.region, Np);
+    S.ltype.assign(NL, 0);
+    if (!in.ok) return "truncated light / albedo / region data";
+    R = FileRefs(); R.has = has_ref != 0;
+    for (int st = 0; st < NS; ++st) {
+        std::vector<float> inten, P, N, Q, T; std::vector<int> dims, off, TQ;
+        in.rd(inten, NL); in.rd(P, 3 * (size_t)Np); in.rd(N, 3 * (size_t)Np); in.rd(Q, 12 * (size_t)Nq);
+        in.rd(dims, 2 * (size_t)Nq); in.rd(off, Nq); in.rd(T, 9 * (size_t)Nt); in.rd(TQ, Nt);
+        if (!in.ok) return "truncated state " + std::to_string(st);
+        if (st == 0) { S.dims = dims; S.off = off; }
+        else if (dims != S.dims || off != S.off) return "quad grids differ between states";
+        for (int q = 0; q < Nq; ++q)
+            if (dims[2 * q] <= 0 || dims[2 * q + 1] <= 0 || off[q] < 0 || (long long)off[q] + (long long)dims[2 * q] * dims[2 * q + 1] > Np)
+                return "quad point grid out of range";
+        for (int j = 0; j < Nt; ++j) if (TQ[j] < 0 || TQ[j] >= Nq) return "triangle quad id out of range";
+        S.inten.push_back(inten); S.P.push_back(P); S.N.push_back(N); S.Q.push_back(Q); S.T.push_back(T); S.TQ.push_back(TQ);
+        S.nt.push_back(Nt);
+        S.present.push_back({});
+        S.objbox.push_back({});
+        if (has_ref) {
+            std::vector<float> ref, Pw; std::vector<uint8_t> valid;
+            in.rd(ref, Np); in.rd(valid, Np); in.rd(Pw, (size_t)NL * Np);
+            if (!in.ok) return "truncated references of state " + std::to_string(st);
+            R.ref.push_back(ref); R.valid.push_back(valid); R.Pw.push_back(Pw);
+        }
+    }
+    const int NV = S.vx * S.vy * S.vz;
+    for (int e = 0; e < NS * NS; ++e) {
+        int no = in.one<int>();
+        std::vector<int> o; in.rd(o, no < 0 ? 0 : no);
+        if (!in.ok || no < 0) return "truncated event list";
+        std::vector<int> oc = fmt == 3 ? o : fold_to_transit(o, S.vx, S.vy, S.vz);
+        for (int v : oc) if (v < 0 || v >= NV) return "event voxel out of range";
+        S.occ.push_back(oc);
+        for (int l = 0; l < NL; ++l) {
+            int ne = in.one<int>();
+            std::vector<int> m; in.rd(m, ne < 0 ? 0 : ne);
+            if (!in.ok || ne < 0) return "truncated emitter list";
+            std::vector<int> em = fmt == 3 ? m : fold_to_transit(m, S.vx, S.vy, S.vz);
+            for (int v : em) if (v < 0 || v >= NV) return "event voxel out of range";
+            S.em.push_back(em);
+        }
+    }
+    S.occ_fine.assign((size_t)NS * NS, {});
+    S.fine = 0;
+    S.cam = rc_camera{};
+    return "";
 }
 
 // ------------------------------------------------------------------------------------ generators

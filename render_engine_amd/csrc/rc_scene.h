@@ -110,6 +110,13 @@ void build_hash(const std::vector<uint64_t>& keys, int log2, std::vector<uint64_
 // Full preparation of a scene description (bounds, BVH, cells, table, zones, quads, events).
 std::string prepare(const SceneDesc& d, Scene& S);
 
+// Reader of the prototype's scene files (cuda/poc_gpu.cu:751-801): format 2 (box scenes: all
+// geometry dynamic, bounds (0,0,0)-(8,4,3), event voxels on a 16x8x6 grid folded onto the transit
+// grid) and format 3 (mesh scenes: static triangles + binary BVH + cell hash table). Converged
+// references (ref, valid, per-light Pw) are returned when the file holds them.
+struct FileRefs { bool has = false; std::vector<std::vector<float>> ref, Pw; std::vector<std::vector<uint8_t>> valid; };
+std::string load_scene_file(const char* path, Scene& S, FileRefs& R);
+
 // Procedural generators.
 SceneDesc gen_box_room(uint64_t seed, int target_tris);
 SceneDesc gen_street(uint64_t seed, int nboxes, int target_tris, int nlamps, bool sun, int target_points, int table_log2,
