@@ -186,6 +186,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--config", type=int, default=2, choices=sorted(CONFIGS))
     ap.add_argument("--trav", type=int, default=2)
+    ap.add_argument("--render", choices=["staged", "fused"], default="fused",
+                    help="camera-pass schedule (same image): staged ray lists or one thread per path")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-quality", action="store_true")
@@ -217,6 +219,7 @@ def main():
     log(f"upload + per-state build {time.time() - t0:.1f} s")
     m = rea.methods.plain()
     cache = rea.Cache(scene, m)
+    cache.set_render_mode(rea.RENDER_STAGED if args.render == "staged" else rea.RENDER_FUSED)
     cam = scene.camera()
     img = torch.zeros(cam.W * cam.H, dtype=torch.float32, device="cuda")
     seed = 4242
@@ -317,7 +320,8 @@ def main():
         "config": {"workload": (f"street {cfg.get('nboxes')} boxes / {info.nst} tris, {info.nl} lights, {info.Np} cache entries, "
                                 f"table 2^{int(np.log2(info.hsize))}, {cam.W}x{cam.H} x {cam.spp} spp, plain cache")
                    if cfg["kind"] == "street" else f"box room {info.nst} tris, {info.Np} entries, {cam.W}x{cam.H}",
-                   "config_index": args.config, "traversal": ["binary-ref", "binary", "wide4"][args.trav]},
+                   "config_index": args.config, "traversal": ["binary-ref", "binary", "wide4"][args.trav],
+                   "render": args.render if args.trav == 2 else "fused"},
         "cache_mpts_s": round(info.Np * args.steps / elapsed / 1e6, 2),
         "frames_per_s": round(args.steps / elapsed, 2),
         "rays_per_frame": {"entry": n_entry / args.steps, "query_ext": n_ext / args.steps, "query_shadow": n_sh / args.steps},

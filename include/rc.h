@@ -137,6 +137,12 @@ rc_status rc_cache_frame(rc_cache* c, int state, int want_err, int timed, rc_fra
  * (may be NULL; when set the call waits for the pass). */
 rc_status rc_cache_render(rc_cache* c, const rc_camera* cam, int no_cache, uint32_t seed, float* d_img, long long* rays_out,
                           float* ms_out);
+/* Schedule of the camera pass (same image bit for bit): RC_RENDER_FUSED (default) runs each path
+ * in one thread; RC_RENDER_STAGED (4-wide BVH scenes, trav 2) runs generate / trace / shade /
+ * shadow-trace stages, the live rays of each 1024-path segment compacted between stages. The
+ * staged workspace (~100 B per path) is kept by the cache. */
+enum { RC_RENDER_FUSED = 0, RC_RENDER_STAGED = 1 };
+rc_status rc_cache_set_render_mode(rc_cache* c, int mode);
 size_t rc_cache_array_bytes(const rc_cache* c, int which);
 rc_status rc_cache_get(const rc_cache* c, int which, void* host_dst, size_t bytes);
 rc_status rc_cache_set(rc_cache* c, int which, const void* host_src, size_t bytes);

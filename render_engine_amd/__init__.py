@@ -16,6 +16,7 @@ __all__ = ["Scene", "Cache", "Method", "Camera", "RcError", "methods", "scenario
            "hash_query", "hash_point", "dda_mark", "dda_volume", "TRAV_REF", "TRAV_BINARY", "TRAV_WIDE"]
 
 TRAV_REF, TRAV_BINARY, TRAV_WIDE = 0, 1, 2
+RENDER_FUSED, RENDER_STAGED = 0, 1
 
 _DT = {"ST": np.float32, "STN": np.float32, "HK": np.uint64, "HV": np.int32, "P": np.float32, "N": np.float32,
        "ALB": np.float32, "REGION": np.int32, "L": np.float32, "LTYPE": np.int32, "INTEN": np.float32, "Q": np.float32,
@@ -182,6 +183,11 @@ class Cache:
         if timed:
             out.append(ms.value)
         return out[0] if len(out) == 1 else (tuple(out) if out else None)
+
+    def set_render_mode(self, mode):
+        """RENDER_FUSED (default): one thread per path; RENDER_STAGED (4-wide BVH): stage kernels over
+        per-segment compacted ray lists. Both give the same image bit for bit."""
+        check(lib.rc_cache_set_render_mode(self._h, int(mode)))
 
     def get(self, name):
         which = CA[name]

@@ -87,6 +87,7 @@ _SIGS = {
     "rc_cache_get": (C.c_int, [_P, C.c_int, _P, C.c_size_t]),
     "rc_cache_set": (C.c_int, [_P, C.c_int, _P, C.c_size_t]),
     "rc_cache_sync": (C.c_int, [_P]),
+    "rc_cache_set_render_mode": (C.c_int, [_P, C.c_int]),
     "rc_trace": (C.c_int, [_P, C.c_int, _P, _P, C.c_int, _P, _P, C.c_int, _P]),
     "rc_hash_insert": (C.c_int, [_P, _P, C.c_uint32, _P, _P, C.c_int, _P, _P]),
     "rc_hash_query": (C.c_int, [_P, _P, C.c_uint32, _P, C.c_int, _P, _P]),

@@ -382,9 +382,18 @@ struct rc_cache {
     int nslots = 0, slot_next = 0, last_event_f = 0;
     int slot_state[RC_KSNAP];
     rc_params p{};
+    int render_mode = RC_RENDER_FUSED;
+    rc_qwork qw{};                // staged camera-pass workspace, sized for qw_paths paths
+    long long qw_paths = 0;
+    std::vector<void*> qw_owned;
+    void free_qwork() {
+        for (void* q : qw_owned) (void)hipFree(q);
+        qw_owned.clear(); qw = rc_qwork{}; qw_paths = 0;
+    }
     ~rc_cache() {
         for (auto& e : ev) if (e) (void)hipEventDestroy(e);
         for (void* q : owned) (void)hipFree(q);
+        free_qwork();
     }
 };
 
@@ -646,12 +655,39 @@ extern "C" rc_status rc_cache_frame(rc_cache* c, int si, int want_err, int timed
     return RC_OK;
 }
 
+// Staged camera-pass workspace for npath paths (grown on demand, kept between calls).
+static hipError_t qwork_reserve(rc_cache* c, long long npath) {
+    if (npath <= c->qw_paths) return hipSuccess;
+    c->free_qwork();
+    rc_qwork& w = c->qw;
+    const size_t n = (size_t)npath;
+    hipError_t e;
+#define QA(ptr, bytes) do { void* p_ = nullptr; e = hipMalloc(&p_, (bytes)); if (e != hipSuccess) { c->free_qwork(); return e; } \
+        c->qw_owned.push_back(p_); ptr = decltype(ptr)(p_); } while (0)
+    QA(w.ro, 16 * n); QA(w.rd, 16 * n); QA(w.so, 16 * n); QA(w.sd, 16 * n);
+    QA(w.sc, 4 * n); QA(w.ht, 4 * n); QA(w.hq, 4 * n);
+    QA(w.rs, 4 * n); QA(w.thr, 4 * n); QA(w.rad, 4 * n);
+    const size_t nseg = (n + 1023) / 1024;
+    QA(w.idx, 4 * nseg * 1024); QA(w.sidx, 4 * nseg * 1024); QA(w.seg, 8 * nseg);
+#undef QA
+    c->qw_paths = npath;
+    return hipSuccess;
+}
+
+extern "C" rc_status rc_cache_set_render_mode(rc_cache* c, int mode) {
+    if (!c) return fail(RC_ERR_ARG, "NULL argument");
+    if (mode != RC_RENDER_FUSED && mode != RC_RENDER_STAGED) return fail(RC_ERR_ARG, "render mode must be 0 (fused) or 1 (staged)");
+    c->render_mode = mode;
+    return RC_OK;
+}
+
 extern "C" rc_status rc_cache_render(rc_cache* c, const rc_camera* cam, int no_cache, uint32_t seed, float* d_img, long long* rays_out,
                                      float* ms_out) {
     if (!c || !cam || !d_img) return fail(RC_ERR_ARG, "NULL argument");
     if (!c->begun) return fail(RC_ERR_STATE, "rc_cache_begin not called");
     if (cam->W < 1 || cam->H < 1 || cam->spp < 1 || cam->bounces < 0 || cam->cache_bounce < 0 || (long long)cam->W * cam->H > (1ll << 30))
         return fail(RC_ERR_ARG, "camera: bad size / spp / bounces");
+    if ((long long)cam->W * cam->H * cam->spp > (1ll << 30)) return fail(RC_ERR_ARG, "camera: W * H * spp above 2^30 paths");
     rc_scene* S = c->S;
     hipStream_t st = S->stream;
     int si = c->prev_state;
@@ -661,8 +697,15 @@ extern "C" rc_status rc_cache_render(rc_cache* c, const rc_camera* cam, int no_c
         HIPCK(rc_launch_resolve(&S->ds, &S->dst[si], c->c.Lc, c->c.R, c->dormant, st));
         c->r_valid = true;
     }
+    const bool staged = c->render_mode == RC_RENDER_STAGED && S->trav == 2;
+    if (staged) HIPCK(qwork_reserve(c, (long long)cam->W * cam->H * cam->spp));
     if (ms_out) HIPCK(hipEventRecord(c->ev[0], st));
-    HIPCK(rc_launch_query(S->trav, &S->ds, &S->dst[si], no_cache ? nullptr : c->c.R, cam, seed, (uint32_t)c->f, c->dormant, d_img, ctr, st));
+    if (staged)
+        HIPCK(rc_launch_query_staged(&S->ds, &S->dst[si], no_cache ? nullptr : c->c.R, cam, seed, (uint32_t)c->f, c->dormant, d_img, ctr,
+                                     &c->qw, st));
+    else
+        HIPCK(rc_launch_query(S->trav, &S->ds, &S->dst[si], no_cache ? nullptr : c->c.R, cam, seed, (uint32_t)c->f, c->dormant, d_img, ctr,
+                              st));
     if (ms_out) {
         HIPCK(hipEventRecord(c->ev[1], st));
         HIPCK(hipEventSynchronize(c->ev[1]));

@@ -210,6 +210,7 @@ __device__ __forceinline__ void rc_cswap(float& ta, int& ia, float& tb, int& ib)
 // and decodes to a lower bound of t (0 for code 0), so culling a popped node whose bound
 // exceeds the current hit never drops a node the exact distance would have kept.
 __device__ __forceinline__ uint32_t rc_tcode(float t) {
+    if (!(t > 0.f)) return 0u;   // +0, -0: lower bound 0
     int c = (int)(__float_as_uint(t) >> 20) - (112 << 3) + 1;
     return c < 1 ? 0u : (c > 255 ? 255u : (uint32_t)c);
 }
@@ -217,95 +218,138 @@ __device__ __forceinline__ float rc_tlb(uint32_t code) {
     return code == 0u ? 0.f : __uint_as_float(((code - 1u) + (112u << 3)) << 20);
 }
 
-// 4-wide BVH traversal with the order-independent closest-hit rule. Each node's four child
-// boxes are tested straight from its six float4 rows; leaf children are intersected at once;
-// inner children are ordered with a 4-input sorting network and pushed far-to-near. The
-// stack lives in LDS, one column per lane (lstk[level * 256 + lane]); the scene's wstack
-// levels (3 per wide level) bound its depth.
-__device__ __forceinline__ void rc_trace_wide_closest(f3 o, f3 d, const rc_dscene& s, float& tmin, int& q, uint32_t* lstk) {
-    uint32_t* stk = lstk + (threadIdx.x & 255);
-    f3 inv = rc_inv_dir(d);
-    int sp = 0, cur = 0;
-    while (true) {
-        const float4* nb = (const float4*)(s.WN + cur);
-        const float4 lx = nb[0], ly = nb[1], lz = nb[2], hx = nb[3], hy = nb[4], hz = nb[5];
-        const int4 ch = ((const int4*)nb)[6], cc = ((const int4*)nb)[7];
-        float t0 = ch.x != -1 ? rc_box_enter(lx.x, ly.x, lz.x, hx.x, hy.x, hz.x, o, inv, tmin) : 1e30f;
-        float t1 = ch.y != -1 ? rc_box_enter(lx.y, ly.y, lz.y, hx.y, hy.y, hz.y, o, inv, tmin) : 1e30f;
-        float t2 = ch.z != -1 ? rc_box_enter(lx.z, ly.z, lz.z, hx.z, hy.z, hz.z, o, inv, tmin) : 1e30f;
-        float t3 = ch.w != -1 ? rc_box_enter(lx.w, ly.w, lz.w, hx.w, hy.w, hz.w, o, inv, tmin) : 1e30f;
-        int i0 = ch.x, i1 = ch.y, i2 = ch.z, i3 = ch.w;
-        // leaves first (their hits tighten tmin before the inner children are ordered)
-#define RC_LEAF(tc, ic, cnt)                                                              \
-        if (tc < 1e30f && ic <= -2) {                                                     \
-            const int first = -2 - ic;                                                    \
-            for (int j = first; j < first + (cnt); ++j) {                                 \
-                float t = rc_tri_static4(o, d, (const float4*)s.ST4 + 3 * (int64_t)j);    \
-                if (rc_closer(t, j, tmin, q)) { tmin = t; q = -2 - j; }                   \
-            }                                                                             \
-            tc = 1e30f;                                                                   \
-        }
-        RC_LEAF(t0, i0, cc.x) RC_LEAF(t1, i1, cc.y) RC_LEAF(t2, i2, cc.z) RC_LEAF(t3, i3, cc.w)
-#undef RC_LEAF
-        // sorting network on (t, node): ascending entry distance, misses (1e30) last
-        rc_cswap(t0, i0, t1, i1); rc_cswap(t2, i2, t3, i3);
-        rc_cswap(t0, i0, t2, i2); rc_cswap(t1, i1, t3, i3);
-        rc_cswap(t1, i1, t2, i2);
-        int next = -1; float next_t = 0.f;
-#define RC_PUSH_OR_NEXT(tc, ic)                                                           \
-        if (tc < 1e30f && tc <= tmin) {                                                   \
-            if (next >= 0) { stk[sp * 256] = ((uint32_t)next << 8) | rc_tcode(next_t); ++sp; } \
-            next = ic; next_t = tc;                                                       \
-        }
-        RC_PUSH_OR_NEXT(t3, i3) RC_PUSH_OR_NEXT(t2, i2) RC_PUSH_OR_NEXT(t1, i1) RC_PUSH_OR_NEXT(t0, i0)
-#undef RC_PUSH_OR_NEXT
-        if (next >= 0) { cur = next; continue; }
-        bool found = false;
-        while (sp > 0) {
-            --sp;
-            uint32_t e = stk[sp * 256];
-            if (rc_tlb(e & 255u) <= tmin) { cur = (int)(e >> 8); found = true; break; }
-        }
-        if (!found) break;
-    }
+// Per-ray constants of the wide traversal: origin, direction, reciprocal direction and the
+// rows of a node that hold each axis's near planes (lo when the direction is positive, hi
+// when negative; node rows: 0-2 lo x/y/z, 3-5 hi x/y/z, 6 children, 7 leaf counts).
+struct rc_wray { f3 o, d, inv; int nx, ny, nz; };
+__device__ __forceinline__ rc_wray rc_wray_make(f3 o, f3 d) {
+    rc_wray r;
+    r.o = o; r.d = d; r.inv = rc_inv_dir(d);
+    r.nx = r.inv.x < 0.f ? 3 : 0; r.ny = r.inv.y < 0.f ? 4 : 1; r.nz = r.inv.z < 0.f ? 5 : 2;
+    return r;
 }
 
-// Any-hit test of the static geometry for a shadow segment: true when some triangle is hit at
-// 1e-4 < t < tlim. Equivalent to the closest-hit visibility test "closest t >= tlim" used by the
-// direct-light estimate, but stops at the first blocker. Boxes are culled against tlim.
+// Entry distances of a node's four child boxes against [0, tmax] (1e30: missed or empty
+// slot), branch-free. With lo <= hi, (lo - o) * inv and (hi - o) * inv are ordered by the sign
+// of inv, so the near-plane value is exactly the per-axis minimum of rc_box_enter and the
+// result equals rc_box_enter's bit for bit.
+__device__ __forceinline__ void rc_child_enter(const rc_wray& r, const float4* nb, int4 ch, float tmax, float& t0, float& t1, float& t2,
+                                               float& t3) {
+    const float4 nx = nb[r.nx], ny = nb[r.ny], nz = nb[r.nz];
+    const float4 fx = nb[3 - r.nx], fy = nb[5 - r.ny], fz = nb[7 - r.nz];
+#define RC_SLAB(tc, c, id)                                                                                      \
+    {                                                                                                           \
+        const float tn = fmaxf(fmaxf((nx.c - r.o.x) * r.inv.x, (ny.c - r.o.y) * r.inv.y), fmaxf((nz.c - r.o.z) * r.inv.z, 0.f)); \
+        const float tf = fminf(fminf((fx.c - r.o.x) * r.inv.x, (fy.c - r.o.y) * r.inv.y), fminf((fz.c - r.o.z) * r.inv.z, tmax)); \
+        tc = (id != -1 && tn <= tf) ? tn : 1e30f;                                                               \
+    }
+    RC_SLAB(t0, x, ch.x) RC_SLAB(t1, y, ch.y) RC_SLAB(t2, z, ch.z) RC_SLAB(t3, w, ch.w)
+#undef RC_SLAB
+}
+
+// Leaf children of a node as up to four triangle ranges [a, b), walked by one flat loop so a
+// wave iterates as often as its busiest lane (not once per child slot). Empty ranges: a = b.
+#define RC_LEAF_RANGE(tc, ic, cnt, a, b)          \
+    if (tc < 1e30f && ic <= -2) {                 \
+        a = -2 - ic; b = a + (cnt); tc = 1e30f;   \
+    }
+#define RC_LEAF_WALK(BODY)                                                              \
+    for (;;) {                                                                          \
+        if (a0 >= b0) {                                                                 \
+            if (a1 >= b1 && a2 >= b2 && a3 >= b3) break;                                \
+            a0 = a1; b0 = b1; a1 = a2; b1 = b2; a2 = a3; b2 = b3; a3 = 0; b3 = 0;       \
+            continue;                                                                   \
+        }                                                                               \
+        const int j = a0++;                                                             \
+        const float t = rc_tri_static4(r.o, r.d, (const float4*)s.ST4 + 3 * (int64_t)j); \
+        BODY                                                                            \
+    }
+
+// One node of the 4-wide closest-hit traversal (order-independent rule): leaf children are
+// intersected at once, inner children ordered by a 4-input sorting network and pushed
+// far-to-near, the nearest visited next. The stack lives in LDS, one column per lane
+// (stk[level * 256]), 32-bit entries node << 8 | code(t). Returns true when finished.
+__device__ __forceinline__ bool rc_closest_node(const rc_wray& r, const rc_dscene& s, float& tmin, int& q, int& cur, int& sp, uint32_t* stk) {
+    const float4* nb = (const float4*)(s.WN + cur);
+    const int4 ch = ((const int4*)nb)[6], cc = ((const int4*)nb)[7];
+    float t0, t1, t2, t3;
+    rc_child_enter(r, nb, ch, tmin, t0, t1, t2, t3);
+    int i0 = ch.x, i1 = ch.y, i2 = ch.z, i3 = ch.w;
+    int a0 = 0, b0 = 0, a1 = 0, b1 = 0, a2 = 0, b2 = 0, a3 = 0, b3 = 0;
+    RC_LEAF_RANGE(t0, i0, cc.x, a0, b0) RC_LEAF_RANGE(t1, i1, cc.y, a1, b1)
+    RC_LEAF_RANGE(t2, i2, cc.z, a2, b2) RC_LEAF_RANGE(t3, i3, cc.w, a3, b3)
+    RC_LEAF_WALK(if (rc_closer(t, j, tmin, q)) { tmin = t; q = -2 - j; })
+    // sorting network on (t, node): ascending entry distance, misses (1e30) last
+    rc_cswap(t0, i0, t1, i1); rc_cswap(t2, i2, t3, i3);
+    rc_cswap(t0, i0, t2, i2); rc_cswap(t1, i1, t3, i3);
+    rc_cswap(t1, i1, t2, i2);
+    int next = -1; float next_t = 0.f;
+#define RC_PUSH_OR_NEXT(tc, ic)                                                           \
+    if (tc < 1e30f && tc <= tmin) {                                                       \
+        if (next >= 0) { stk[sp * 256] = ((uint32_t)next << 8) | rc_tcode(next_t); ++sp; } \
+        next = ic; next_t = tc;                                                           \
+    }
+    RC_PUSH_OR_NEXT(t3, i3) RC_PUSH_OR_NEXT(t2, i2) RC_PUSH_OR_NEXT(t1, i1) RC_PUSH_OR_NEXT(t0, i0)
+#undef RC_PUSH_OR_NEXT
+    if (next >= 0) { cur = next; return false; }
+    while (sp > 0) {
+        --sp;
+        const uint32_t e = stk[sp * 256];
+        if (rc_tlb(e & 255u) <= tmin) { cur = (int)(e >> 8); return false; }
+    }
+    return true;
+}
+
+// 4-wide BVH closest hit of the static geometry (rc_closest_node until the stack is empty);
+// the stack depth is bounded by the scene's wstack levels (3 per wide level).
+__device__ __forceinline__ void rc_trace_wide_closest(f3 o, f3 d, const rc_dscene& s, float& tmin, int& q, uint32_t* lstk) {
+    uint32_t* stk = lstk + (threadIdx.x & 255);
+    const rc_wray r = rc_wray_make(o, d);
+    int sp = 0, cur = 0;
+    while (!rc_closest_node(r, s, tmin, q, cur, sp, stk)) {}
+}
+
+// One node of the any-hit test of the static geometry for a shadow segment (a hit at
+// 1e-4 < t < tlim blocks it; equivalent to the closest-hit test "closest t >= tlim" of the
+// direct-light estimate, but stops at the first blocker). Boxes are culled against tlim; leaf
+// children are tested at once and inner children visited near-first, so blockers next to the
+// shading point, the likely ones, are found early; the answer does not depend on the order.
+// Returns true when finished (blocked set on a blocker).
+__device__ __forceinline__ bool rc_any_node(const rc_wray& r, const rc_dscene& s, float tlim, int& cur, int& sp, uint32_t* stk, bool& blocked) {
+    const float4* nb = (const float4*)(s.WN + cur);
+    const int4 ch = ((const int4*)nb)[6], cc = ((const int4*)nb)[7];
+    float t0, t1, t2, t3;
+    rc_child_enter(r, nb, ch, tlim, t0, t1, t2, t3);
+    int i0 = ch.x, i1 = ch.y, i2 = ch.z, i3 = ch.w;
+    int a0 = 0, b0 = 0, a1 = 0, b1 = 0, a2 = 0, b2 = 0, a3 = 0, b3 = 0;
+    RC_LEAF_RANGE(t0, i0, cc.x, a0, b0) RC_LEAF_RANGE(t1, i1, cc.y, a1, b1)
+    RC_LEAF_RANGE(t2, i2, cc.z, a2, b2) RC_LEAF_RANGE(t3, i3, cc.w, a3, b3)
+    RC_LEAF_WALK(if (t > 1e-4f && t < tlim) { blocked = true; return true; })
+    rc_cswap(t0, i0, t1, i1); rc_cswap(t2, i2, t3, i3);
+    rc_cswap(t0, i0, t2, i2); rc_cswap(t1, i1, t3, i3);
+    rc_cswap(t1, i1, t2, i2);
+    int next = -1;
+#define RC_ANYPUSH(tc, ic)                                                                \
+    if (tc < 1e30f) {                                                                     \
+        if (next >= 0) { stk[sp * 256] = (uint32_t)next << 8; ++sp; }                     \
+        next = ic;                                                                        \
+    }
+    RC_ANYPUSH(t3, i3) RC_ANYPUSH(t2, i2) RC_ANYPUSH(t1, i1) RC_ANYPUSH(t0, i0)
+#undef RC_ANYPUSH
+    if (next >= 0) { cur = next; return false; }
+    if (sp == 0) return true;
+    --sp;
+    cur = (int)(stk[sp * 256] >> 8);
+    return false;
+}
+
 __device__ __forceinline__ bool rc_occluded_wide(f3 o, f3 d, const rc_dscene& s, float tlim, uint32_t* lstk) {
     uint32_t* stk = lstk + (threadIdx.x & 255);
-    f3 inv = rc_inv_dir(d);
+    const rc_wray r = rc_wray_make(o, d);
     int sp = 0, cur = 0;
-    while (true) {
-        const float4* nb = (const float4*)(s.WN + cur);
-        const float4 lx = nb[0], ly = nb[1], lz = nb[2], hx = nb[3], hy = nb[4], hz = nb[5];
-        const int4 ch = ((const int4*)nb)[6], cc = ((const int4*)nb)[7];
-        int nxt = -1;
-#define RC_ANY(cx, lxx, lyx, lzx, hxx, hyx, hzx, cnt)                                      \
-        if (cx != -1) {                                                                    \
-            float te = rc_box_enter(lxx, lyx, lzx, hxx, hyx, hzx, o, inv, tlim);           \
-            if (te < 1e30f) {                                                              \
-                if (cx <= -2) {                                                            \
-                    const int first = -2 - cx;                                             \
-                    for (int j = first; j < first + (cnt); ++j) {                          \
-                        float t = rc_tri_static4(o, d, (const float4*)s.ST4 + 3 * (int64_t)j); \
-                        if (t > 1e-4f && t < tlim) return true;                            \
-                    }                                                                      \
-                } else if (nxt < 0) nxt = cx;                                              \
-                else { stk[sp * 256] = (uint32_t)cx << 8; ++sp; }                          \
-            }                                                                              \
-        }
-        RC_ANY(ch.x, lx.x, ly.x, lz.x, hx.x, hy.x, hz.x, cc.x)
-        RC_ANY(ch.y, lx.y, ly.y, lz.y, hx.y, hy.y, hz.y, cc.y)
-        RC_ANY(ch.z, lx.z, ly.z, lz.z, hx.z, hy.z, hz.z, cc.z)
-        RC_ANY(ch.w, lx.w, ly.w, lz.w, hx.w, hy.w, hz.w, cc.w)
-#undef RC_ANY
-        if (nxt >= 0) { cur = nxt; continue; }
-        if (sp == 0) return false;
-        --sp;
-        cur = (int)(stk[sp * 256] >> 8);
-    }
+    bool blocked = false;
+    while (!rc_any_node(r, s, tlim, cur, sp, stk, blocked)) {}
+    return blocked;
 }
 
 // the same test against the LDS-held dynamic triangles (cuda/poc_gpu.cu:200-220 arithmetic)

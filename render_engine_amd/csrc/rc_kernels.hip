@@ -617,6 +617,182 @@ __global__ void __launch_bounds__(RC_TB) k_query(rc_dscene s, rc_dstate st, cons
     }
 }
 
+// ------------------------------------------------------------------------------ wavefront query
+// The camera pass as stages over compacted ray lists: generate primary rays, trace a batch,
+// shade hits (cache read, next-event candidate, bounce), trace the shadow batch (any hit),
+// add the visible next-event terms, repeat per bounce, then sum each pixel's samples. Every
+// path performs exactly the fused path's operations in the same order (same RNG draws, same
+// float expressions), so the image is bit-identical to k_query's; only the scheduling differs.
+
+// Staged pass state is indexed by path id p (= pixel * spp + sample). The paths are split in
+// segments of RC_QB consecutive ids; after each stage the rays still alive in a segment are
+// compacted, in id order, into that segment's slice of an id list (idx: extension rays, sidx:
+// shadow rays; counts in seg[0..nblk) and seg[nblk..2 nblk)). Compaction stays inside a block,
+// so it needs no global atomics or scan; the trace kernels run one thread per list slot and
+// waves past a segment's count exit at once. Only the scheduling differs from the fused pass.
+#define RC_QB 1024
+
+// Wave-ballot compaction inside a block: rank of each lane among the block's lanes with want
+// (in thread order) and the block total.
+__device__ __forceinline__ int rc_block_rank(bool want, int* s_w, int& total) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    const unsigned long long m = __ballot(want);
+    if (lane == 0) s_w[wv] = __popcll(m);
+    __syncthreads();
+    int before = 0;
+    total = 0;
+    for (int k = 0; k < nw; ++k) { const int c = s_w[k]; before += k < wv ? c : 0; total += c; }
+    __syncthreads();
+    return before + __popcll(m & ((1ull << lane) - 1ull));
+}
+
+// primary rays of every path (the jitter draws of rc_path)
+__global__ void __launch_bounds__(RC_TB) k_q_gen(rc_camera cam, uint32_t seed, uint32_t frame, int npath, rc_qwork w,
+                                                unsigned long long* stat) {
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p == 0 && stat) atomicAdd(stat, (unsigned long long)npath);
+    if (p >= npath) return;
+    int pix = p / cam.spp;
+    int px = pix % cam.W, py = pix / cam.W;
+    uint32_t rs = rc_seed(seed, frame, (uint32_t)p);
+    float ju = rc_rnd(rs), jv = rc_rnd(rs);
+    float sx = (2.f * ((float)px + ju) / (float)cam.W - 1.f) * cam.tanx;
+    float sy = (1.f - 2.f * ((float)py + jv) / (float)cam.H) * cam.tany;
+    f3 d = mk3(cam.fwd[0], cam.fwd[1], cam.fwd[2]) + mk3(cam.right[0], cam.right[1], cam.right[2]) * sx +
+           mk3(cam.up[0], cam.up[1], cam.up[2]) * sy;
+    d = d * (1.f / sqrtf(dot3(d, d)));
+    ((float4*)w.ro)[p] = make_float4(cam.pos[0], cam.pos[1], cam.pos[2], 0.f);
+    ((float4*)w.rd)[p] = make_float4(d.x, d.y, d.z, 0.f);
+    w.rs[p] = rs; w.thr[p] = 1.f; w.rad[p] = 0.f;
+}
+
+// Trace one list slot per thread: ids == nullptr is the identity list of all npath paths.
+// Closest: hit (t, q) of path p into ht / hq. ANY: the shadow ray of path p; when the light is
+// visible the path's next-event term is added to its radiance (one shadow ray per path per
+// bounce, after the bounce's shading, so each path's additions keep the fused order).
+template <bool ANY>
+__global__ void __launch_bounds__(RC_TB) k_q_trace(rc_dscene s, rc_dstate st, rc_qwork w, const int* __restrict__ ids,
+                                                  const int* __restrict__ segcnt, int npath) {
+    const int seg = blockIdx.x / (RC_QB / RC_TB);
+    const int local = (blockIdx.x % (RC_QB / RC_TB)) * RC_TB + threadIdx.x;
+    const int cnt = ids ? segcnt[seg] : min(RC_QB, npath - seg * RC_QB);
+    if ((blockIdx.x % (RC_QB / RC_TB)) * RC_TB >= cnt) return;   // whole block past the segment's count
+    extern __shared__ float sh[];
+    int* shq = (int*)(sh + 9 * st.nt);
+    uint32_t* lstk = rc_lstack(sh, st.nt);
+    rc_load_dyn(st, sh, shq);
+    if (local >= cnt) return;
+    const int p = ids ? ids[seg * RC_QB + local] : seg * RC_QB + local;
+    if (ANY) {
+        const float4 a = ((const float4*)w.so)[p], b = ((const float4*)w.sd)[p];
+        const f3 o = mk3(a.x, a.y, a.z), d = mk3(b.x, b.y, b.z);
+        const float tlim = a.w;
+        if (rc_occluded_lds(o, d, sh, st.nt, tlim) || (s.nst > 0 && rc_occluded_wide(o, d, s, tlim, lstk))) return;
+        w.rad[p] = w.rad[p] + w.sc[p];
+    } else {
+        const float4 a = ((const float4*)w.ro)[p], b = ((const float4*)w.rd)[p];
+        float t; int q;
+        rc_trace_full<2>(mk3(a.x, a.y, a.z), mk3(b.x, b.y, b.z), sh, shq, st.nt, s, t, q, lstk);
+        w.ht[p] = t; w.hq[p] = q;
+    }
+}
+
+// Shade the hits of bounce b over a segment's list: cache read at the cache bounce, else the
+// next-event candidate (shadow ray + its term, rc_nee up to its shadow ray) and the bounce ray.
+// The segment's new extension and shadow lists are written in place, in id order.
+__global__ void __launch_bounds__(RC_QB) k_q_shade(rc_dscene s, rc_dstate st, const float* R, rc_camera cam, uint64_t dormant, int b,
+                                                  int npath, int nblk, rc_qwork w) {
+    __shared__ int s_w[RC_QB / 64];
+    const int seg = blockIdx.x, k = threadIdx.x;
+    const int cnt = b == 0 ? min(RC_QB, npath - seg * RC_QB) : w.seg[seg];
+    int p = -1;
+    if (k < cnt) p = b == 0 ? seg * RC_QB + k : w.idx[seg * RC_QB + k];
+    bool emit_sh = false, emit_b = false;
+    if (p >= 0) {
+        const float4 a = ((const float4*)w.ro)[p], c4 = ((const float4*)w.rd)[p];
+        const f3 o = mk3(a.x, a.y, a.z), d = mk3(c4.x, c4.y, c4.z);
+        const float t = w.ht[p];
+        const int q = w.hq[p];
+        if (q != -1) {
+            f3 H = o + d * t, nrm;
+            int e;
+            if (!rc_resolve_hit_n(s, st, H, d, q, e, nrm)) {
+                float thr = w.thr[p];
+                if (R && b == cam.cache_bounce) {
+                    w.rad[p] = w.rad[p] + thr * R[e];
+                } else {
+                    uint32_t rs = w.rs[p];
+                    const int nl = s.nl;
+                    const float A = s.Alb[e];
+                    const float u = rc_rnd(rs);
+                    const int l = min((int)(u * (float)nl), nl - 1);
+                    const float I = st.inten[l];
+                    if (I > 0.f && !rc_is_dormant(dormant, l)) {
+                        float r; f3 D;
+                        if (s.ltype[l] == 1) { D = ld3(s.L, l); r = s.sun_len; }
+                        else { f3 dv = ld3(s.L, l) - H; r = sqrtf(dot3(dv, dv)); D = dv * (1.f / r); }
+                        const float cth = dot3(nrm, D);
+                        if (cth > 0.f) {
+                            const f3 O = H + nrm * 1e-3f;
+                            float v;
+                            if (s.ltype[l] == 1) v = (float)nl * (I * cth);
+                            else { float rr = fmaxf(r, 0.05f); v = (float)nl * (I * cth / (rr * rr)); }
+                            w.sc[p] = thr * (A * v);
+                            ((float4*)w.so)[p] = make_float4(O.x, O.y, O.z, r - 1e-3f);
+                            ((float4*)w.sd)[p] = make_float4(D.x, D.y, D.z, 0.f);
+                            emit_sh = true;
+                        }
+                    }
+                    if (b < cam.bounces) {
+                        thr = thr * A;
+                        const float u1 = rc_rnd(rs), u2 = rc_rnd(rs);
+                        const f3 nd = rc_cosine_dir(nrm, u1, u2);
+                        const f3 no = H + nrm * 1e-3f;
+                        ((float4*)w.ro)[p] = make_float4(no.x, no.y, no.z, 0.f);
+                        ((float4*)w.rd)[p] = make_float4(nd.x, nd.y, nd.z, 0.f);
+                        w.thr[p] = thr;
+                        emit_b = true;
+                    }
+                    w.rs[p] = rs;
+                }
+            }
+        }
+    }
+    int ns, nb2;
+    const int rs_ = rc_block_rank(emit_sh, s_w, ns);
+    const int rb_ = rc_block_rank(emit_b, s_w, nb2);   // ranks read the old list first: p is in a register
+    if (emit_sh) w.sidx[seg * RC_QB + rs_] = p;
+    if (emit_b) w.idx[seg * RC_QB + rb_] = p;
+    if (k == 0) { w.seg[nblk + seg] = ns; w.seg[seg] = nb2; }
+}
+
+// rays in the lists (ray counters): sum of the segment counts
+__global__ void __launch_bounds__(RC_TB) k_q_count(const int* segcnt, int nblk, unsigned long long* stat) {
+    __shared__ long long s_p[RC_TB / 64];
+    long long v = 0;
+    for (int i = threadIdx.x; i < nblk; i += blockDim.x) v += segcnt[i];
+    v = rc_wave_sum_i64(v);
+    if ((threadIdx.x & 63) == 0) s_p[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        long long t = 0;
+        for (int k = 0; k < RC_TB / 64; ++k) t += s_p[k];
+        atomicAdd(stat, (unsigned long long)t);
+    }
+}
+
+// pixel = (((r0 + r1) + r2) + ...) / spp over the pixel's paths in sample order
+__global__ void __launch_bounds__(RC_TB) k_q_pixels(rc_camera cam, rc_qwork w, float* img) {
+    int pix = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pix >= cam.W * cam.H) return;
+    const float* r = w.rad + (int64_t)pix * cam.spp;
+    // the fused pass starts from the first sample when spp divides 64, else from 0
+    const bool per_path = cam.spp <= 64 && 64 % cam.spp == 0;
+    float sum = per_path ? r[0] : 0.f + r[0];
+    for (int k = 1; k < cam.spp; ++k) sum += r[k];
+    img[pix] = sum / (float)cam.spp;
+}
+
 // ------------------------------------------------------------------------------ hash grid insert
 // Insert keys with 64-bit CAS and linear probing (same home slot mix64(key) & hmask and probe
 // order as the lookup, cuda/poc_gpu.cu:263-272). Lanes of a wave holding the same key elect
@@ -853,5 +1029,27 @@ hipError_t rc_launch_vol_lights(int nl, const uint32_t* Va, const uint32_t* Vb, 
                                 unsigned long long* out, hipStream_t stream) {
     if (nl <= 0 || nvox <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_vol_lights, dim3(nl), dim3(RC_TB), 0, stream, Va, Vb, words_per_vol, vox, nvox, out);
+    return hipGetLastError();
+}
+
+hipError_t rc_launch_query_staged(const rc_dscene* s, const rc_dstate* st, const float* R, const rc_camera* cam, uint32_t seed,
+                                  uint32_t frame, uint64_t dormant, float* img, unsigned long long* counters, const rc_qwork* w,
+                                  hipStream_t stream) {
+    const int npath = cam->W * cam->H * cam->spp;
+    const size_t shm = rc_shm(*st, *s, 2);
+    const int nblk = (npath + RC_QB - 1) / RC_QB;
+    const dim3 tgrid(nblk * (RC_QB / RC_TB));
+    hipLaunchKernelGGL(k_q_gen, dim3(rc_grid(npath)), dim3(RC_TB), 0, stream, *cam, seed, frame, npath, *w, counters);
+    const int last = R ? min(cam->cache_bounce, cam->bounces) : cam->bounces;
+    for (int b = 0; b <= last; ++b) {
+        if (b > 0 && counters) hipLaunchKernelGGL(k_q_count, dim3(1), dim3(RC_TB), 0, stream, (const int*)w->seg, nblk, counters);
+        hipLaunchKernelGGL(k_q_trace<false>, tgrid, dim3(RC_TB), shm, stream, *s, *st, *w, b == 0 ? (const int*)nullptr : (const int*)w->idx,
+                           (const int*)w->seg, npath);
+        hipLaunchKernelGGL(k_q_shade, dim3(nblk), dim3(RC_QB), 0, stream, *s, *st, R, *cam, dormant, b, npath, nblk, *w);
+        if (R && b == cam->cache_bounce) break;   // cache read: no shadow or bounce rays follow
+        if (counters) hipLaunchKernelGGL(k_q_count, dim3(1), dim3(RC_TB), 0, stream, (const int*)w->seg + nblk, nblk, counters + 1);
+        hipLaunchKernelGGL(k_q_trace<true>, tgrid, dim3(RC_TB), shm, stream, *s, *st, *w, (const int*)w->sidx, (const int*)w->seg + nblk, npath);
+    }
+    hipLaunchKernelGGL(k_q_pixels, dim3(rc_grid((long long)cam->W * cam->H)), dim3(RC_TB), 0, stream, *cam, *w, img);
     return hipGetLastError();
 }

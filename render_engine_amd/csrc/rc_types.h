@@ -126,3 +126,14 @@ typedef struct rc_camera {
     int32_t W, H, spp, bounces, cache_bounce, pad0;
 } rc_camera;
 #endif
+
+// Workspace of the staged camera pass, indexed by path id (W*H*spp paths, segments of 1024).
+typedef struct rc_qwork {
+    void* ro; void* rd;            // float4 extension ray of each path: origin, direction
+    void* so; void* sd;            // float4 shadow ray: origin + tlim, direction
+    float* sc;                     // next-event term of the shadow ray
+    float* ht; int32_t* hq;        // hit of the extension ray
+    uint32_t* rs; float* thr; float* rad;   // RNG state, throughput, radiance
+    int32_t* idx; int32_t* sidx;   // per-segment compacted id lists: extension rays, shadow rays
+    int32_t* seg;                  // [2 * segments]: extension-list counts, then shadow-list counts
+} rc_qwork;
