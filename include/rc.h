@@ -66,6 +66,7 @@ typedef struct rc_scene_info {
     int Np, npst, nbox, nst, nnodes, nwnodes, NS, nl, NR, Nq, hsize, vx, vy, vz, nw, bvh_depth, wide_depth, has_ref, fine;
     float h, lo[3], ext[3], diag, sun_len;
     rc_camera cam;
+    int nmat, pad1;        /* materials (albedo table size) */
 } rc_scene_info;
 
 typedef struct rc_frame_stats {
@@ -81,7 +82,7 @@ enum {
     RC_A_ST = 0, RC_A_STN, RC_A_BN, RC_A_WN, RC_A_HK, RC_A_HV, RC_A_P, RC_A_N, RC_A_ALB, RC_A_REGION,
     RC_A_L, RC_A_LTYPE, RC_A_INTEN, RC_A_Q, RC_A_DIMS, RC_A_OFF, RC_A_T, RC_A_TQ, RC_A_TALB,
     RC_A_EDIR, RC_A_SV, RC_A_LM, RC_A_OCC, RC_A_EM, RC_A_REF, RC_A_VALID, RC_A_PW, RC_A_ORDER, RC_A_OBJBOX,
-    RC_A_MESH_V, RC_A_MESH_ALB, RC_A_MESH_DS, RC_A_VOL, RC_A_OCCF, RC_A_COUNT
+    RC_A_MESH_V, RC_A_MESH_ALB, RC_A_MESH_DS, RC_A_VOL, RC_A_OCCF, RC_A_MAT, RC_A_MATALB, RC_A_COUNT
 };
 /* cache arrays (rc_cache_get / rc_cache_set); M is SoA [nl][Np], LC / EDIR rows are AoS [Np][nl] */
 enum {
@@ -168,6 +169,27 @@ rc_status rc_dda_mark(rc_scene* s, const float* d_A, const float* d_B, int n, ui
  * seg_vol[i] (NULL: volume 0) of d_vol, words_per_vol 32-bit words each. */
 rc_status rc_dda_volume(rc_scene* s, int gx, int gy, int gz, const float* d_A, const float* d_B, const int32_t* d_seg_vol, int n,
                         uint32_t* d_vol, long long words_per_vol, void* stream);
+
+/* ---- adjoint of the cached radiance (extension) ----
+ * Model: the cache's transport with fixed rays. G = unit-intensity direct light of the state's
+ * geometry (cuda/poc_gpu.cu:369-391 at I = 1), E_i = sum_l I_l G_il, A_i = alb[mat_i] (material table
+ * of the scene), L^(0) = 0, L^(b+1)_i = (1/K) sum over entry i's K rays of bounce b that reach a valid
+ * entry h of A_h (E_h + L^(b)_h) (rays of the converged-reference iteration, cuda/prep_mesh.cuh:181-213,
+ * seed_b = hash32(seed + b * 0x9E3779B9)), cached radiance R_i = A_i (E_i + L^(B)_i). The backward
+ * re-traces each bounce's rays instead of storing paths. No counterpart in the reference: its
+ * extension for gradient-driven invalidation (callers: autograd bindings, the gradient-mask method). */
+typedef struct rc_adjoint rc_adjoint;
+rc_status rc_adjoint_create(rc_scene* s, int state, int K, int bounces, uint32_t seed, rc_adjoint** out);
+void rc_adjoint_destroy(rc_adjoint* a);
+/* d_I[nl] emission, d_alb[nmat] material albedos (device) -> d_R[Np] (device; may be NULL) */
+rc_status rc_adjoint_forward(rc_adjoint* a, const float* d_I, const float* d_alb, float* d_R, void* stream);
+/* gradient of J = sum_i gR_i R_i at the last forward: d_gI[nl], d_galb[nmat] (device, overwritten) */
+rc_status rc_adjoint_backward(rc_adjoint* a, const float* d_gR, float* d_gI, float* d_galb, void* stream);
+/* host copies: G [Np][nl] f32, SG [Np] u64, SR (support of R at the last forward: bit min(l, 63) when
+ * dR_i/dI_l != 0) [Np] u64, LB (L^(B)) [Np] f32, DA / DE (adjoints of A / E at the last backward) [Np] f32 */
+enum { RC_ADJ_G = 0, RC_ADJ_SG, RC_ADJ_SR, RC_ADJ_LB, RC_ADJ_DA, RC_ADJ_DE, RC_ADJ_COUNT };
+size_t rc_adjoint_array_bytes(const rc_adjoint* a, int which);
+rc_status rc_adjoint_get(const rc_adjoint* a, int which, void* host_dst, size_t bytes);
 
 #ifdef __cplusplus
 }

@@ -94,6 +94,12 @@ def _load():
     lib.orc_runner_frame.argtypes = [_P, C.c_int, C.c_int, C.POINTER(C.c_int)]
     lib.orc_runner_cache.restype = C.POINTER(OrcCache); lib.orc_runner_cache.argtypes = [_P]
     lib.orc_num_threads.restype = C.c_int
+    lib.orc_adj_seed.restype = C.c_uint32; lib.orc_adj_seed.argtypes = [C.c_uint32, C.c_int]
+    lib.orc_adj_unit.argtypes = [C.POINTER(OrcScene), C.c_int, _P, _P, _P]
+    lib.orc_adj_hits.argtypes = [C.POINTER(OrcScene), C.c_int, C.c_int, C.c_int, C.c_uint32, _P]
+    lib.orc_adj_forward.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, _P, _P, _P, _P, _P, _P, _P, _P, _P]
+    lib.orc_adj_grad64.restype = C.c_double
+    lib.orc_adj_grad64.argtypes = [C.c_int] * 5 + [_P] * 9
     return lib
 
 
@@ -109,6 +115,7 @@ class OracleScene:
     build arrays (Edir, SV, LM) come from the oracle's own k_build unless given."""
 
     def __init__(self, scene, build=True, trav=1):
+        self.scene = scene
         self.info = i = scene.info
         self.keep = {}
         g = lambda name, st=0: self.keep.setdefault((name, st), scene.array(name, st))
@@ -212,3 +219,37 @@ class OracleRunner:
         addr = getattr(c, field)
         buf = (C.c_char * (n * np.dtype(dt).itemsize)).from_address(addr)
         return np.frombuffer(buf, dt).copy()
+
+
+class OracleAdjoint:
+    """CPU adjoint of the cached radiance on state `st` of an OracleScene: unit direct light
+    (FP32 and FP64), the hit entry of every ray of the B fixed-seed bounces, then the FP32 forward
+    (the kernels' operation order) and the FP64 forward + reverse-mode gradient."""
+
+    def __init__(self, osc, st, K, B, seed, trav=1):
+        i = osc.info
+        self.Np, self.nl, self.K, self.B = i.Np, i.nl, K, B
+        self.nmat = i.nmat
+        self.mat = np.ascontiguousarray(osc.scene.array("MAT"), np.int32)
+        self.G = np.zeros(i.Np * i.nl, np.float32)
+        self.SG = np.zeros(i.Np, np.uint64)
+        self.G64 = np.zeros(i.Np * i.nl, np.float64)
+        lib.orc_adj_unit(osc.state(st), trav, ptr(self.G), ptr(self.SG), ptr(self.G64))
+        self.hits = np.zeros(max(B, 1) * i.Np * K, np.int32)
+        if B > 0:
+            lib.orc_adj_hits(osc.state(st), trav, K, B, seed, ptr(self.hits))
+
+    def forward32(self, I, alb):
+        I = np.ascontiguousarray(I, np.float32); alb = np.ascontiguousarray(alb, np.float32)
+        R = np.zeros(self.Np, np.float32); SR = np.zeros(self.Np, np.uint64); LB = np.zeros(self.Np, np.float32)
+        lib.orc_adj_forward(self.Np, self.nl, self.K, self.B, ptr(self.hits), ptr(self.mat), ptr(alb), ptr(I), ptr(self.G),
+                            ptr(self.SG), ptr(R), ptr(SR), ptr(LB))
+        return R, SR, LB
+
+    def grad64(self, I, alb, g):
+        I = np.ascontiguousarray(I, np.float64); alb = np.ascontiguousarray(alb, np.float64)
+        g = np.ascontiguousarray(g, np.float64)
+        R = np.zeros(self.Np); dI = np.zeros(self.nl); dalb = np.zeros(self.nmat)
+        J = lib.orc_adj_grad64(self.Np, self.nl, self.nmat, self.K, self.B, ptr(self.hits), ptr(self.mat), ptr(alb), ptr(I),
+                               ptr(self.G64), ptr(g), ptr(R), ptr(dI), ptr(dalb))
+        return J, R, dI, dalb

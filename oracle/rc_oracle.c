@@ -1172,3 +1172,169 @@ double orc_runner_frame(orc_runner* r, int si, int want_err, int* invalidated) {
     r->f++;
     return err;
 }
+
+/* ------------------------------------------------------------------ adjoint of the cached radiance */
+uint32_t orc_adj_seed(uint32_t seed, int b) { return orc_hash32(seed + (uint32_t)b * 0x9E3779B9u); }
+
+void orc_adj_unit(const orc_scene* s, int trav, float* G, uint64_t* SG, double* G64) {
+    const int nl = s->nl;
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int i = 0; i < s->Np; ++i) {
+        v3 P = ld(s->P, i), N = ld(s->N, i), O = add(P, mul(N, 1e-3f));
+        uint64_t m = 0;
+        for (int l = 0; l < nl; ++l) {
+            float r; v3 D;
+            if (s->ltype[l] == 1) { D = ld(s->L, l); r = s->sun_len; }
+            else { v3 dv = sub(ld(s->L, l), P); r = sqrtf(dot(dv, dv)); D = mul(dv, 1.f / r); }
+            float c = dot(N, D), g = 0.f;
+            double g64 = 0.0;
+            if (c > 0.f) {
+                float t; int q;
+                trace_full(s, trav, O, D, &t, &q);
+                if (t >= r - 1e-3f) {   /* visible: closest hit beyond the light (k_build) */
+                    if (s->ltype[l] == 1) g = c;
+                    else { float rr = fmaxf(r, 0.05f); g = c / (rr * rr); }
+                    if (G64) {   /* the same geometry in double */
+                        double d[3], r64, c64;
+                        if (s->ltype[l] == 1) {
+                            double n = sqrt((double)s->L[3 * l] * s->L[3 * l] + (double)s->L[3 * l + 1] * s->L[3 * l + 1] +
+                                            (double)s->L[3 * l + 2] * s->L[3 * l + 2]);
+                            for (int a = 0; a < 3; ++a) d[a] = s->L[3 * l + a] / n;
+                            c64 = N.x * d[0] + N.y * d[1] + N.z * d[2];
+                            g64 = c64 > 0 ? c64 : 0.0;
+                        } else {
+                            const float* pp = s->P + 3 * (size_t)i;
+                            for (int a = 0; a < 3; ++a) d[a] = (double)s->L[3 * l + a] - (double)pp[a];
+                            r64 = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+                            c64 = (N.x * d[0] + N.y * d[1] + N.z * d[2]) / r64;
+                            double rr = r64 > 0.05 ? r64 : 0.05;
+                            g64 = c64 > 0 ? c64 / (rr * rr) : 0.0;
+                        }
+                    }
+                }
+            }
+            G[(size_t)nl * i + l] = g;
+            if (G64) G64[(size_t)nl * i + l] = g64;
+            if (g > 0.f) m |= 1ull << (l < 63 ? l : 63);
+        }
+        SG[i] = m;
+    }
+}
+
+void orc_adj_hits(const orc_scene* s, int trav, int K, int B, uint32_t seed, int32_t* hits) {
+    for (int b = 0; b < B; ++b) {
+        const uint32_t sb = orc_adj_seed(seed, b);
+#pragma omp parallel for schedule(dynamic, 64)
+        for (int i = 0; i < s->Np; ++i) {
+            uint32_t rs = orc_hash32(sb ^ orc_hash32((uint32_t)i + 1u));
+            v3 P = ld(s->P, i), N = ld(s->N, i), O = add(P, mul(N, 1e-3f));
+            for (int k = 0; k < K; ++k) {
+                float u1 = orc_rnd(&rs), u2 = orc_rnd(&rs);
+                v3 D = cosine_dir(N, u1, u2);
+                float t; int q;
+                int hit = trace_full(s, trav, O, D, &t, &q);
+                v3 H = hit ? add(O, mul(D, t)) : O;
+                int pi;
+                hits[((size_t)b * s->Np + i) * K + k] = resolve_hit(s, hit, H, D, q, i, &pi) ? -1 : pi;
+            }
+        }
+    }
+}
+
+void orc_adj_forward(int Np, int nl, int K, int B, const int32_t* hits, const int32_t* mat, const float* alb, const float* I,
+                     const float* G, const uint64_t* SG, float* R, uint64_t* SR, float* LB) {
+    float* A = (float*)malloc(sizeof(float) * Np);
+    float* E = (float*)malloc(sizeof(float) * Np);
+    float* L0 = (float*)calloc(Np, sizeof(float));
+    float* L1 = (float*)calloc(Np, sizeof(float));
+    uint64_t* S0 = (uint64_t*)calloc(Np, sizeof(uint64_t));
+    uint64_t* S1 = (uint64_t*)calloc(Np, sizeof(uint64_t));
+    for (int i = 0; i < Np; ++i) {
+        A[i] = alb[mat[i]];
+        float e = 0.f;
+        for (int l = 0; l < nl; ++l) e += I[l] * G[(size_t)nl * i + l];
+        E[i] = e;
+    }
+    for (int b = 0; b < B; ++b) {
+#pragma omp parallel for schedule(static)
+        for (int i = 0; i < Np; ++i) {
+            float acc = 0.f;
+            uint64_t m = 0;
+            for (int k = 0; k < K; ++k) {
+                int h = hits[((size_t)b * Np + i) * K + k];
+                if (h < 0) continue;
+                acc += A[h] * (E[h] + (b > 0 ? L0[h] : 0.f));
+                if (A[h] > 0.f) m |= SG[h] | (b > 0 ? S0[h] : 0ull);
+            }
+            L1[i] = acc / (float)K;
+            S1[i] = m;
+        }
+        float* tl = L0; L0 = L1; L1 = tl;
+        uint64_t* ts = S0; S0 = S1; S1 = ts;
+    }
+    for (int i = 0; i < Np; ++i) {
+        float L = B > 0 ? L0[i] : 0.f;
+        R[i] = A[i] * (E[i] + L);
+        SR[i] = A[i] > 0.f ? (SG[i] | (B > 0 ? S0[i] : 0ull)) : 0ull;
+        if (LB) LB[i] = L;
+    }
+    free(A); free(E); free(L0); free(L1); free(S0); free(S1);
+}
+
+double orc_adj_grad64(int Np, int nl, int nmat, int K, int B, const int32_t* hits, const int32_t* mat, const double* alb,
+                      const double* I, const double* G64, const double* g, double* R64, double* dI, double* dalb) {
+    double* A = (double*)malloc(sizeof(double) * Np);
+    double* E = (double*)malloc(sizeof(double) * Np);
+    double* L = (double*)calloc((size_t)(B + 1) * Np, sizeof(double));   /* L^(0..B) */
+    for (int i = 0; i < Np; ++i) {
+        A[i] = alb[mat[i]];
+        double e = 0.0;
+        for (int l = 0; l < nl; ++l) e += I[l] * G64[(size_t)nl * i + l];
+        E[i] = e;
+    }
+    for (int b = 0; b < B; ++b) {
+        const double* Lp = L + (size_t)b * Np;
+        double* Ln = L + (size_t)(b + 1) * Np;
+        for (int i = 0; i < Np; ++i) {
+            double acc = 0.0;
+            for (int k = 0; k < K; ++k) {
+                int h = hits[((size_t)b * Np + i) * K + k];
+                if (h >= 0) acc += A[h] * (E[h] + Lp[h]);
+            }
+            Ln[i] = acc / K;
+        }
+    }
+    const double* LBp = L + (size_t)B * Np;
+    double J = 0.0;
+    for (int i = 0; i < Np; ++i) { R64[i] = A[i] * (E[i] + LBp[i]); J += g[i] * R64[i]; }
+    /* reverse mode */
+    double* dA = (double*)malloc(sizeof(double) * Np);
+    double* dE = (double*)malloc(sizeof(double) * Np);
+    double* lam = (double*)malloc(sizeof(double) * Np);
+    double* lam2 = (double*)malloc(sizeof(double) * Np);
+    for (int i = 0; i < Np; ++i) { dA[i] = g[i] * (E[i] + LBp[i]); dE[i] = g[i] * A[i]; lam[i] = g[i] * A[i]; }
+    for (int b = B - 1; b >= 0; --b) {
+        const double* Lp = L + (size_t)b * Np;
+        for (int i = 0; i < Np; ++i) lam2[i] = 0.0;
+        for (int i = 0; i < Np; ++i) {
+            double w = lam[i] / K;
+            if (w == 0.0) continue;
+            for (int k = 0; k < K; ++k) {
+                int h = hits[((size_t)b * Np + i) * K + k];
+                if (h < 0) continue;
+                dA[h] += w * (E[h] + Lp[h]);
+                dE[h] += w * A[h];
+                lam2[h] += w * A[h];
+            }
+        }
+        double* t = lam; lam = lam2; lam2 = t;
+    }
+    for (int l = 0; l < nl; ++l) dI[l] = 0.0;
+    for (int m = 0; m < nmat; ++m) dalb[m] = 0.0;
+    for (int i = 0; i < Np; ++i) {
+        for (int l = 0; l < nl; ++l) dI[l] += dE[i] * G64[(size_t)nl * i + l];
+        dalb[mat[i]] += dA[i];
+    }
+    free(A); free(E); free(L); free(dA); free(dE); free(lam); free(lam2);
+    return J;
+}

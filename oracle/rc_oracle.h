@@ -148,6 +148,24 @@ void orc_k_trace2_f64(const orc_scene* s, const double* Edir64, const double* Lc
 void orc_k_update_f64(const orc_scene* s, const float* n_before, const int32_t* kk, const orc_params* p, const double* Lc64,
                       const double* m64, double* Lc64_out);
 
+/* ---- adjoint of the cached radiance (per-light emission, per-material albedo) ----
+ * Model: unit-intensity direct light G[Np][nl] (the k_build geometry at I = 1), E_i = sum_l I_l G_il,
+ * A_i = alb[mat_i], L^(0) = 0, L^(b+1)_i = (1/K) sum over the K rays of entry i at bounce b that hit a
+ * valid entry h of A_h (E_h + L^(b)_h), R_i = A_i (E_i + L^(B)_i). Rays: seed_b = hash32(seed + b * 0x9E3779B9),
+ * per entry rs = hash32(seed_b ^ hash32(i + 1)), then the k_ref ray rule (prep_mesh.cuh:181-213). */
+uint32_t orc_adj_seed(uint32_t seed, int b);
+/* G and its support SG (bit min(l, 63) when G_il > 0); G64 (optional) the same in double from the
+ * same visibility decisions */
+void orc_adj_unit(const orc_scene* s, int trav, float* G, uint64_t* SG, double* G64);
+/* hit entry of every ray, hits[(b * Np + i) * K + k], -1 when the ray does not count */
+void orc_adj_hits(const orc_scene* s, int trav, int K, int B, uint32_t seed, int32_t* hits);
+/* FP32 forward in the kernels' operation order: R, support SR, L^(B) (LB may be NULL) */
+void orc_adj_forward(int Np, int nl, int K, int B, const int32_t* hits, const int32_t* mat, const float* alb, const float* I,
+                     const float* G, const uint64_t* SG, float* R, uint64_t* SR, float* LB);
+/* FP64 forward R64 and reverse-mode gradient of J = sum_i g_i R_i: dI[nl], dalb[nmat]; returns J */
+double orc_adj_grad64(int Np, int nl, int nmat, int K, int B, const int32_t* hits, const int32_t* mat, const double* alb,
+                      const double* I, const double* G64, const double* g, double* R64, double* dI, double* dalb);
+
 /* ---- runner (cuda/poc_gpu.cu:826-1016) on host buffers ---- */
 typedef struct orc_runner orc_runner;
 typedef struct orc_method {

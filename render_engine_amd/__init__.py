@@ -23,7 +23,8 @@ _DT = {"ST": np.float32, "STN": np.float32, "HK": np.uint64, "HV": np.int32, "P"
        "DIMS": np.int32, "OFF": np.int32, "T": np.float32, "TQ": np.int32, "TALB": np.float32, "EDIR": np.float32,
        "SV": np.uint32, "LM": np.uint64, "OCC": np.int32, "EM": np.int32, "REF": np.float32, "VALID": np.uint8,
        "PW": np.float32, "ORDER": np.int32, "OBJBOX": np.float32, "MESH_V": np.float32, "MESH_ALB": np.float32,
-       "MESH_DS": np.uint8, "VOL": np.uint32, "OCCF": np.int32}
+       "MESH_DS": np.uint8, "VOL": np.uint32, "OCCF": np.int32,
+       "MAT": np.int32, "MATALB": np.float32}
 _CDT = {"LC": np.float32, "M": np.float32, "N": np.float32, "KK": np.int32, "PENDING": np.uint8, "CUR": np.uint32,
         "PREV": np.uint32, "LCUR": np.uint64, "LPREV": np.uint64, "SUMPR": np.int64, "NPEND": np.int32, "AGE": np.float32,
         "STALE": np.uint64, "C0": np.float32, "RSTAT": np.int64, "LNEW": np.uint64, "DSTAT": np.int64,

@@ -38,7 +38,7 @@ class SceneInfo(C.Structure):
         "Np", "npst", "nbox", "nst", "nnodes", "nwnodes", "NS", "nl", "NR", "Nq", "hsize", "vx", "vy", "vz", "nw",
         "bvh_depth", "wide_depth", "has_ref", "fine")] + [
         ("h", C.c_float), ("lo", C.c_float * 3), ("ext", C.c_float * 3), ("diag", C.c_float), ("sun_len", C.c_float),
-        ("cam", Camera),
+        ("cam", Camera), ("nmat", C.c_int), ("pad1", C.c_int),
     ]
 
 
@@ -52,7 +52,7 @@ class FrameStats(C.Structure):
 # scene arrays (enum order of include/rc.h)
 A_NAMES = ["ST", "STN", "BN", "WN", "HK", "HV", "P", "N", "ALB", "REGION", "L", "LTYPE", "INTEN", "Q", "DIMS", "OFF", "T",
            "TQ", "TALB", "EDIR", "SV", "LM", "OCC", "EM", "REF", "VALID", "PW", "ORDER", "OBJBOX", "MESH_V", "MESH_ALB",
-           "MESH_DS", "VOL", "OCCF"]
+           "MESH_DS", "VOL", "OCCF", "MAT", "MATALB"]
 A = {n: i for i, n in enumerate(A_NAMES)}
 C_NAMES = ["LC", "M", "N", "KK", "PENDING", "CUR", "PREV", "LCUR", "LPREV", "SUMPR", "NPEND", "AGE", "STALE", "C0", "RSTAT",
            "LNEW", "DSTAT", "R"]
@@ -88,6 +88,12 @@ _SIGS = {
     "rc_cache_set": (C.c_int, [_P, C.c_int, _P, C.c_size_t]),
     "rc_cache_sync": (C.c_int, [_P]),
     "rc_cache_set_render_mode": (C.c_int, [_P, C.c_int]),
+    "rc_adjoint_create": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_uint32, C.POINTER(_P)]),
+    "rc_adjoint_destroy": (None, [_P]),
+    "rc_adjoint_forward": (C.c_int, [_P, _P, _P, _P, _P]),
+    "rc_adjoint_backward": (C.c_int, [_P, _P, _P, _P, _P]),
+    "rc_adjoint_array_bytes": (C.c_size_t, [_P, C.c_int]),
+    "rc_adjoint_get": (C.c_int, [_P, C.c_int, _P, C.c_size_t]),
     "rc_trace": (C.c_int, [_P, C.c_int, _P, _P, C.c_int, _P, _P, C.c_int, _P]),
     "rc_hash_insert": (C.c_int, [_P, _P, C.c_uint32, _P, _P, C.c_int, _P, _P]),
     "rc_hash_query": (C.c_int, [_P, _P, C.c_uint32, _P, C.c_int, _P, _P]),

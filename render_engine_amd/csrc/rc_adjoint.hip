@@ -1,0 +1,265 @@
+// Adjoint of the cached radiance with respect to per-light emission and per-material albedo.
+//
+// Differentiable model: the cache's own transport with fixed rays. With the unit-intensity
+// direct light G[i][l] (visibility x cosine / distance^2: the k_build geometry at I = 1,
+// cuda/poc_gpu.cu:369-391), E_i = sum_l I_l G_il and A_i = alb[mat_i], L^(0) = 0 and, for the
+// bounces b = 0..B-1,
+//     L^(b+1)_i = (1/K) sum over the K rays of entry i at bounce b that reach a valid entry h
+//                 of A_h (E_h + L^(b)_h),
+// and the cached radiance is R_i = A_i (E_i + L^(B)_i). The rays are those of the converged-
+// reference iteration (cuda/prep_mesh.cuh:181-213): seed_b = hash32(seed + b * 0x9E3779B9),
+// rs = hash32(seed_b ^ hash32(i + 1)), cosine directions from the entry.
+//
+// Reverse mode for J = sum_i g_i R_i keeps the per-bounce radiance L^(b) (one float per entry
+// and bounce) and re-traces each bounce's rays from their seeds instead of storing paths; the
+// adjoint of a ray's contribution is scattered to its hit entry with hardware float atomics.
+// The final reductions (per light over entries, per material over its entries) run in a fixed
+// order. The 64-bit support masks (bit min(l, 63) when dR_i/dI_l != 0) propagate with the
+// forward: all terms are non-negative, so the nonzero pattern is the union over the rays.
+#include "rc_device.h"
+#include "rc_kernels.h"
+#include "rc_trace.h"
+
+// G[i][l] and its support SG[i]: the shadow-ray visibility of k_build (closest hit beyond
+// r - 1e-3, equivalently no hit in (1e-4, r - 1e-3)), at unit intensity; rays are skipped
+// where the cosine is not positive (G = 0 there).
+template <int TRAV>
+__global__ void __launch_bounds__(RC_TB) k_adj_unit(rc_dscene s, rc_dstate st, float* G, uint64_t* SG) {
+    extern __shared__ float sh[];
+    int* shq = (int*)(sh + 9 * st.nt);
+    uint32_t* lstk = rc_lstack(sh, st.nt);
+    rc_load_dyn(st, sh, shq);
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= s.Np) return;
+    const int nl = s.nl;
+    const f3 P = ld3(st.P, i), N = ld3(st.N, i), O = P + N * 1e-3f;
+    uint64_t m = 0ull;
+    for (int l = 0; l < nl; ++l) {
+        float r; f3 D;
+        if (s.ltype[l] == 1) { D = ld3(s.L, l); r = s.sun_len; }
+        else { f3 dv = ld3(s.L, l) - P; r = sqrtf(dot3(dv, dv)); D = dv * (1.f / r); }
+        const float c = dot3(N, D);
+        float g = 0.f;
+        if (c > 0.f) {
+            bool vis;
+            if (TRAV == 2) {
+                const float tlim = r - 1e-3f;
+                vis = !(rc_occluded_lds(O, D, sh, st.nt, tlim) || (s.nst > 0 && rc_occluded_wide(O, D, s, tlim, lstk)));
+            } else {
+                float t; int q;
+                rc_trace_full<TRAV>(O, D, sh, shq, st.nt, s, t, q, lstk);
+                vis = t >= r - 1e-3f;
+            }
+            if (vis) {
+                if (s.ltype[l] == 1) g = c;
+                else { const float rr = fmaxf(r, 0.05f); g = c / (rr * rr); }
+            }
+        }
+        G[(int64_t)nl * i + l] = g;
+        if (g > 0.f) m |= 1ull << (l < 63 ? l : 63);
+    }
+    SG[i] = m;
+}
+
+__global__ void __launch_bounds__(RC_TB) k_adj_zero(int n, float* x) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = 0.f;
+}
+
+// A_i = alb[mat_i], E_i = sum_l I_l G_il (lights in order)
+__global__ void __launch_bounds__(RC_TB) k_adj_setup(int Np, int nl, const int* mat, const float* alb, const float* I, const float* G,
+                                                    float* A, float* E) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= Np) return;
+    A[i] = alb[mat[i]];
+    const float* g = G + (int64_t)nl * i;
+    float e = 0.f;
+    for (int l = 0; l < nl; ++l) e += I[l] * g[l];
+    E[i] = e;
+}
+
+// the ray rule of the adjoint bounces: hit entry h of ray k of entry i, or -1
+template <int TRAV>
+__device__ __forceinline__ int rc_adj_ray(const rc_dscene& s, const rc_dstate& st, const float* sh, const int* shq, uint32_t* lstk,
+                                          int i, f3 O, f3 N, uint32_t& rs) {
+    const float u1 = rc_rnd(rs), u2 = rc_rnd(rs);
+    const f3 D = rc_cosine_dir(N, u1, u2);
+    float t; int q;
+    const bool hit = rc_trace_full<TRAV>(O, D, sh, shq, st.nt, s, t, q, lstk);
+    const f3 H = hit ? O + D * t : O;
+    int pi;
+    return rc_resolve_hit(s, st, hit, H, D, q, i, pi) ? -1 : pi;
+}
+
+// one forward bounce: Lout_i = (1/K) sum A_h (E_h + Lin_h), Sout_i = union of SG_h | Sin_h over
+// hits with A_h > 0 (Lin, Sin NULL: zero)
+template <int TRAV>
+__global__ void __launch_bounds__(RC_TB) k_adj_fwd(rc_dscene s, rc_dstate st, const float* A, const float* E, const uint64_t* SG,
+                                                  const float* Lin, const uint64_t* Sin, float* Lout, uint64_t* Sout, uint32_t seed, int K) {
+    extern __shared__ float sh[];
+    int* shq = (int*)(sh + 9 * st.nt);
+    uint32_t* lstk = rc_lstack(sh, st.nt);
+    rc_load_dyn(st, sh, shq);
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= s.Np) return;
+    uint32_t rs = rc_hash32(seed ^ rc_hash32((uint32_t)i + 1u));
+    const f3 P = ld3(st.P, i), N = ld3(st.N, i), O = P + N * 1e-3f;
+    float acc = 0.f;
+    uint64_t m = 0ull;
+    for (int k = 0; k < K; ++k) {
+        const int h = rc_adj_ray<TRAV>(s, st, sh, shq, lstk, i, O, N, rs);
+        if (h < 0) continue;
+        const float Ah = A[h];
+        acc += Ah * (E[h] + (Lin ? Lin[h] : 0.f));
+        if (Ah > 0.f) m |= SG[h] | (Sin ? Sin[h] : 0ull);
+    }
+    Lout[i] = acc / (float)K;
+    Sout[i] = m;
+}
+
+// R_i = A_i (E_i + L_i), SR_i = support of R_i
+__global__ void __launch_bounds__(RC_TB) k_adj_out(int Np, const float* A, const float* E, const float* L, const uint64_t* SG,
+                                                  const uint64_t* S, float* R, uint64_t* SR) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= Np) return;
+    const float a = A[i];
+    if (R) R[i] = a * (E[i] + (L ? L[i] : 0.f));
+    SR[i] = a > 0.f ? (SG[i] | (S ? S[i] : 0ull)) : 0ull;
+}
+
+// cotangent g on R: dA_i = g_i (E_i + L^(B)_i), dE_i = g_i A_i, adjoint of L^(B)_i = g_i A_i
+__global__ void __launch_bounds__(RC_TB) k_adj_bwd_init(int Np, const float* g, const float* A, const float* E, const float* L,
+                                                       float* dA, float* dE, float* lam) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= Np) return;
+    const float gi = g[i], a = A[i];
+    dA[i] = gi * (E[i] + (L ? L[i] : 0.f));
+    dE[i] = gi * a;
+    lam[i] = gi * a;
+}
+
+// one reverse bounce: re-trace the rays of bounce b; each hit h receives w = lam_i / K times
+// (E_h + L^(b)_h) on dA, A_h on dE and (b > 0) A_h on the adjoint of L^(b)
+template <int TRAV>
+__global__ void __launch_bounds__(RC_TB) k_adj_bwd(rc_dscene s, rc_dstate st, const float* A, const float* E, const float* Lp,
+                                                  const float* lam_in, float* lam_out, float* dA, float* dE, uint32_t seed, int K) {
+    extern __shared__ float sh[];
+    int* shq = (int*)(sh + 9 * st.nt);
+    uint32_t* lstk = rc_lstack(sh, st.nt);
+    rc_load_dyn(st, sh, shq);
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= s.Np) return;
+    const float w = lam_in[i] / (float)K;
+    if (w == 0.f) return;   // no contribution: the rays need not be traced
+    uint32_t rs = rc_hash32(seed ^ rc_hash32((uint32_t)i + 1u));
+    const f3 P = ld3(st.P, i), N = ld3(st.N, i), O = P + N * 1e-3f;
+    for (int k = 0; k < K; ++k) {
+        const int h = rc_adj_ray<TRAV>(s, st, sh, shq, lstk, i, O, N, rs);
+        if (h < 0) continue;
+        const float Ah = A[h];
+        unsafeAtomicAdd(dA + h, w * (E[h] + (Lp ? Lp[h] : 0.f)));
+        unsafeAtomicAdd(dE + h, w * Ah);
+        if (lam_out) unsafeAtomicAdd(lam_out + h, w * Ah);
+    }
+}
+
+// block sum in a fixed order (wave shuffles, then the waves in order)
+__device__ __forceinline__ float rc_block_sum(float v, float* s_w) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) s_w[wv] = v;
+    __syncthreads();
+    float t = 0.f;
+    for (int k = 0; k < (int)(blockDim.x >> 6); ++k) t += s_w[k];
+    return t;
+}
+
+// per-block partial sums of dE_i G_il for every light: part[block][l]
+__global__ void __launch_bounds__(RC_TB) k_adj_dI_part(int Np, int nl, const float* dE, const float* G, float* part) {
+    __shared__ float s_w[RC_TB / 64];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const float de = i < Np ? dE[i] : 0.f;
+    const float* g = G + (int64_t)nl * (i < Np ? i : 0);
+    for (int l = 0; l < nl; ++l) {
+        const float t = rc_block_sum(i < Np ? de * g[l] : 0.f, s_w);
+        if (threadIdx.x == 0) part[(int64_t)blockIdx.x * nl + l] = t;
+    }
+}
+
+// dI_l = sum over blocks of part[block][l] (one block per light)
+__global__ void __launch_bounds__(RC_TB) k_adj_dI_sum(int nblk, int nl, const float* part, float* dI) {
+    __shared__ float s_w[RC_TB / 64];
+    const int l = blockIdx.x;
+    float v = 0.f;
+    for (int b = threadIdx.x; b < nblk; b += blockDim.x) v += part[(int64_t)b * nl + l];
+    const float t = rc_block_sum(v, s_w);
+    if (threadIdx.x == 0) dI[l] = t;
+}
+
+// dalb_m = sum of dA over the entries of material m (perm: entries sorted by material, moff:
+// offsets); one wave per material
+__global__ void __launch_bounds__(RC_TB) k_adj_dalb(int nmat, const float* dA, const int* perm, const int* moff, float* dalb) {
+    const int m = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, lane = threadIdx.x & 63;
+    if (m >= nmat) return;
+    float v = 0.f;
+    for (int j = moff[m] + lane; j < moff[m + 1]; j += 64) v += dA[perm[j]];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    if (lane == 0) dalb[m] = v;
+}
+
+// ------------------------------------------------------------------------------ launchers
+#define RC_ADJ_T(KER, ...)                                                                         \
+    do {                                                                                            \
+        if (trav == 0) hipLaunchKernelGGL(KER<0>, g, b, shm, stream, __VA_ARGS__);                 \
+        else if (trav == 1) hipLaunchKernelGGL(KER<1>, g, b, shm, stream, __VA_ARGS__);            \
+        else hipLaunchKernelGGL(KER<2>, g, b, shm, stream, __VA_ARGS__);                           \
+    } while (0)
+
+hipError_t rc_launch_adj_unit(int trav, const rc_dscene* s, const rc_dstate* st, float* G, uint64_t* SG, hipStream_t stream) {
+    const dim3 g(rc_grid(s->Np)), b(RC_TB);
+    const size_t shm = rc_shm(*st, *s, trav);
+    RC_ADJ_T(k_adj_unit, *s, *st, G, SG);
+    return hipGetLastError();
+}
+
+hipError_t rc_launch_adj_forward(int trav, const rc_dscene* s, const rc_dstate* st, const rc_adjbuf* w, const float* I,
+                                 const float* alb, int K, int B, uint32_t seed, float* R, hipStream_t stream) {
+    const int Np = s->Np;
+    const dim3 g(rc_grid(Np)), b(RC_TB);
+    const size_t shm = rc_shm(*st, *s, trav);
+    hipLaunchKernelGGL(k_adj_setup, g, b, 0, stream, Np, s->nl, w->mat, alb, I, w->G, w->A, w->E);
+    for (int k = 0; k < B; ++k) {
+        const uint32_t sb = rc_hash32(seed + (uint32_t)k * 0x9E3779B9u);
+        RC_ADJ_T(k_adj_fwd, *s, *st, w->A, w->E, w->SG, k > 0 ? w->L + (size_t)(k - 1) * Np : (const float*)nullptr,
+                 k > 0 ? w->S[(k - 1) & 1] : (const uint64_t*)nullptr, w->L + (size_t)k * Np, w->S[k & 1], sb, K);
+    }
+    hipLaunchKernelGGL(k_adj_out, g, b, 0, stream, Np, w->A, w->E, B > 0 ? w->L + (size_t)(B - 1) * Np : (const float*)nullptr, w->SG,
+                       B > 0 ? w->S[(B - 1) & 1] : (const uint64_t*)nullptr, R, w->SR);
+    return hipGetLastError();
+}
+
+hipError_t rc_launch_adj_backward(int trav, const rc_dscene* s, const rc_dstate* st, const rc_adjbuf* w, const float* gR, int K,
+                                  int B, uint32_t seed, float* dI, float* dalb, hipStream_t stream) {
+    const int Np = s->Np, nl = s->nl;
+    const dim3 g(rc_grid(Np)), b(RC_TB);
+    const size_t shm = rc_shm(*st, *s, trav);
+    hipLaunchKernelGGL(k_adj_bwd_init, g, b, 0, stream, Np, gR, w->A, w->E, B > 0 ? w->L + (size_t)(B - 1) * Np : (const float*)nullptr,
+                       w->dA, w->dE, w->lam[B & 1]);
+    for (int k = B - 1; k >= 0; --k) {
+        const uint32_t sb = rc_hash32(seed + (uint32_t)k * 0x9E3779B9u);
+        float* lam_out = k > 0 ? w->lam[k & 1] : nullptr;
+        // the adjoint of L^(k) starts at zero: a fill kernel (no memset node in a captured graph)
+        if (lam_out) hipLaunchKernelGGL(k_adj_zero, g, b, 0, stream, Np, lam_out);
+        RC_ADJ_T(k_adj_bwd, *s, *st, w->A, w->E, k > 0 ? w->L + (size_t)(k - 1) * Np : (const float*)nullptr, w->lam[(k + 1) & 1],
+                 lam_out, w->dA, w->dE, sb, K);
+    }
+    const int nblk = rc_grid(Np);
+    hipLaunchKernelGGL(k_adj_dI_part, dim3(nblk), b, 0, stream, Np, nl, w->dE, w->G, w->part);
+    hipLaunchKernelGGL(k_adj_dI_sum, dim3(nl), b, 0, stream, nblk, nl, w->part, dI);
+    hipLaunchKernelGGL(k_adj_dalb, dim3(rc_grid((long long)w->nmat * 64)), b, 0, stream, w->nmat, w->dA, w->perm, w->moff, dalb);
+    return hipGetLastError();
+}
+#undef RC_ADJ_T

@@ -124,6 +124,7 @@ extern "C" rc_status rc_scene_info_get(const rc_scene* s, rc_scene_info* o) {
     o->h = H.h; o->diag = H.diag; o->sun_len = H.sun_len; o->fine = H.fine;
     for (int a = 0; a < 3; ++a) { o->lo[a] = H.lo[a]; o->ext[a] = H.ext[a]; }
     memcpy(&o->cam, &H.cam, sizeof(rc_camera));
+    o->nmat = H.nmat;
     return RC_OK;
 }
 
@@ -150,6 +151,8 @@ static bool host_array(const rc_scene* s, int which, int st, const void** p, siz
         case RC_A_MESH_V: RET(H.mesh.V);
         case RC_A_MESH_ALB: RET(H.mesh.alb);
         case RC_A_MESH_DS: RET(H.mesh.ds);
+        case RC_A_MAT: RET(H.mat);
+        case RC_A_MATALB: RET(H.matalb);
         case RC_A_P: if (!ok_state(H.NS)) return false; RET(H.P[st]);
         case RC_A_N: if (!ok_state(H.NS)) return false; RET(H.N[st]);
         case RC_A_INTEN: if (!ok_state(H.NS)) return false; RET(H.inten[st]);
@@ -817,5 +820,103 @@ extern "C" rc_status rc_dda_volume(rc_scene* s, int gx, int gy, int gz, const fl
     if (words_per_vol < ((long long)gx * gy * gz + 31) / 32) return fail(RC_ERR_ARG, "rc_dda_volume: volume smaller than the grid");
     if (!s->uploaded) return fail(RC_ERR_STATE, "scene not uploaded");
     HIPCK(rc_launch_dda_volume(&s->ds, gx, gy, gz, d_A, d_B, d_seg_vol, n, d_vol, words_per_vol, pick(s, stream)));
+    return RC_OK;
+}
+
+// ------------------------------------------------------------------------------------ adjoint
+struct rc_adjoint {
+    rc_scene* S = nullptr;
+    int state = 0, K = 1, B = 0;
+    uint32_t seed = 0;
+    rc_adjbuf w{};
+    bool forward_done = false, backward_done = false;
+    std::vector<void*> owned;
+    ~rc_adjoint() { for (void* p : owned) (void)hipFree(p); }
+};
+
+extern "C" rc_status rc_adjoint_create(rc_scene* s, int state, int K, int bounces, uint32_t seed, rc_adjoint** out) {
+    if (!s || !out) return fail(RC_ERR_ARG, "NULL argument");
+    if (!s->uploaded) return fail(RC_ERR_STATE, "scene not uploaded");
+    const rc::Scene& H = s->H;
+    if (state < 0 || state >= H.NS || K < 1 || K > 1024 || bounces < 0 || bounces > 16)
+        return fail(RC_ERR_ARG, "rc_adjoint_create: state in [0, NS), 1 <= K <= 1024, 0 <= bounces <= 16");
+    if (H.nmat < 1 || (int)H.mat.size() != H.Np) return fail(RC_ERR_SCENE, "scene has no material table");
+    for (int m : H.mat) if (m < 0 || m >= H.nmat) return fail(RC_ERR_SCENE, "entry material out of range");
+    rc_adjoint* a = new rc_adjoint();
+    a->S = s; a->state = state; a->K = K; a->B = bounces; a->seed = seed;
+    rc_adjbuf& w = a->w;
+    const size_t Np = H.Np, nl = H.nl, nblk = (Np + 255) / 256;
+#define AK(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) { delete a; return fail(RC_ERR_HIP, "%s: %s", #x, hipGetErrorString(e_)); } } while (0)
+    AK(dalloc(&w.G, Np * nl, a->owned)); AK(dalloc(&w.SG, Np, a->owned)); AK(dalloc(&w.mat, Np, a->owned));
+    AK(dalloc(&w.A, Np, a->owned)); AK(dalloc(&w.E, Np, a->owned)); AK(dalloc(&w.L, Np * (size_t)std::max(bounces, 1), a->owned));
+    AK(dalloc(&w.S[0], Np, a->owned)); AK(dalloc(&w.S[1], Np, a->owned)); AK(dalloc(&w.SR, Np, a->owned));
+    AK(dalloc(&w.dA, Np, a->owned)); AK(dalloc(&w.dE, Np, a->owned));
+    AK(dalloc(&w.lam[0], Np, a->owned)); AK(dalloc(&w.lam[1], Np, a->owned));
+    AK(dalloc(&w.part, nblk * nl, a->owned));
+    AK(dalloc(&w.perm, Np, a->owned)); AK(dalloc(&w.moff, (size_t)H.nmat + 1, a->owned));
+    w.nmat = H.nmat;
+    // entries grouped by material (counting sort, entry order kept inside a material)
+    std::vector<int> moff(H.nmat + 1, 0), perm(Np);
+    for (int m : H.mat) ++moff[m + 1];
+    for (int m = 0; m < H.nmat; ++m) moff[m + 1] += moff[m];
+    {
+        std::vector<int> pos(moff.begin(), moff.end() - 1);
+        for (size_t i = 0; i < Np; ++i) perm[pos[H.mat[i]]++] = (int)i;
+    }
+    AK(hipMemcpyAsync(w.mat, H.mat.data(), Np * 4, hipMemcpyHostToDevice, s->stream));
+    AK(hipMemcpyAsync(w.perm, perm.data(), Np * 4, hipMemcpyHostToDevice, s->stream));
+    AK(hipMemcpyAsync(w.moff, moff.data(), moff.size() * 4, hipMemcpyHostToDevice, s->stream));
+    AK(rc_launch_adj_unit(s->trav, &s->ds, &s->dst[state], w.G, w.SG, s->stream));
+    AK(hipStreamSynchronize(s->stream));
+#undef AK
+    *out = a;
+    return RC_OK;
+}
+
+extern "C" void rc_adjoint_destroy(rc_adjoint* a) { delete a; }
+
+extern "C" rc_status rc_adjoint_forward(rc_adjoint* a, const float* d_I, const float* d_alb, float* d_R, void* stream) {
+    if (!a || !d_I || !d_alb) return fail(RC_ERR_ARG, "NULL argument");
+    rc_scene* s = a->S;
+    HIPCK(rc_launch_adj_forward(s->trav, &s->ds, &s->dst[a->state], &a->w, d_I, d_alb, a->K, a->B, a->seed, d_R, (hipStream_t)stream));
+    a->forward_done = true;
+    return RC_OK;
+}
+
+extern "C" rc_status rc_adjoint_backward(rc_adjoint* a, const float* d_gR, float* d_gI, float* d_galb, void* stream) {
+    if (!a || !d_gR || !d_gI || !d_galb) return fail(RC_ERR_ARG, "NULL argument");
+    if (!a->forward_done) return fail(RC_ERR_STATE, "rc_adjoint_backward before rc_adjoint_forward");
+    rc_scene* s = a->S;
+    HIPCK(rc_launch_adj_backward(s->trav, &s->ds, &s->dst[a->state], &a->w, d_gR, a->K, a->B, a->seed, d_gI, d_galb, (hipStream_t)stream));
+    a->backward_done = true;
+    return RC_OK;
+}
+
+static bool adj_array(const rc_adjoint* a, int which, const void** dp, size_t* bytes) {
+    const size_t Np = a->S->H.Np, nl = a->S->H.nl;
+    switch (which) {
+        case RC_ADJ_G: *dp = a->w.G; *bytes = Np * nl * 4; return true;
+        case RC_ADJ_SG: *dp = a->w.SG; *bytes = Np * 8; return true;
+        case RC_ADJ_SR: if (!a->forward_done) return false; *dp = a->w.SR; *bytes = Np * 8; return true;
+        case RC_ADJ_LB: if (!a->forward_done || a->B < 1) return false; *dp = a->w.L + (size_t)(a->B - 1) * Np; *bytes = Np * 4; return true;
+        case RC_ADJ_DA: if (!a->backward_done) return false; *dp = a->w.dA; *bytes = Np * 4; return true;
+        case RC_ADJ_DE: if (!a->backward_done) return false; *dp = a->w.dE; *bytes = Np * 4; return true;
+        default: return false;
+    }
+}
+
+extern "C" size_t rc_adjoint_array_bytes(const rc_adjoint* a, int which) {
+    const void* p; size_t b = 0;
+    if (!a || !adj_array(a, which, &p, &b)) return 0;
+    return b;
+}
+
+extern "C" rc_status rc_adjoint_get(const rc_adjoint* a, int which, void* host_dst, size_t bytes) {
+    const void* p; size_t b = 0;
+    if (!a || !host_dst) return fail(RC_ERR_ARG, "NULL argument");
+    if (!adj_array(a, which, &p, &b)) return fail(RC_ERR_ARG, "rc_adjoint_get: unknown array or not computed yet");
+    if (bytes != b) return fail(RC_ERR_ARG, "rc_adjoint_get: %zu bytes given, %zu expected", bytes, b);
+    HIPCK(hipDeviceSynchronize());
+    HIPCK(hipMemcpy(host_dst, p, b, hipMemcpyDeviceToHost));
     return RC_OK;
 }

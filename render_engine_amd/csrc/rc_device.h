@@ -25,7 +25,7 @@ __device__ __forceinline__ f3 cross3(f3 a, f3 b) {
 __device__ __forceinline__ f3 ld3(const float* p, int64_t i) { return {p[3 * i], p[3 * i + 1], p[3 * i + 2]}; }
 
 // ---------------------------------------------------------------- RNG (cuda/poc_gpu.cu:127-135)
-__device__ __forceinline__ uint32_t rc_hash32(uint32_t x) {
+__host__ __device__ __forceinline__ uint32_t rc_hash32(uint32_t x) {
     x ^= x >> 16; x *= 0x7feb352dU; x ^= x >> 15; x *= 0x846ca68bU; x ^= x >> 16; return x;
 }
 __device__ __forceinline__ float rc_rnd(uint32_t& s) {

@@ -39,3 +39,8 @@ hipError_t rc_launch_vol_lights(int nl, const uint32_t* Va, const uint32_t* Vb, 
 hipError_t rc_launch_query_staged(const rc_dscene* s, const rc_dstate* st, const float* R, const rc_camera* cam, uint32_t seed,
                                   uint32_t frame, uint64_t dormant, float* img, unsigned long long* counters, const rc_qwork* w,
                                   hipStream_t stream);
+hipError_t rc_launch_adj_unit(int trav, const rc_dscene* s, const rc_dstate* st, float* G, uint64_t* SG, hipStream_t stream);
+hipError_t rc_launch_adj_forward(int trav, const rc_dscene* s, const rc_dstate* st, const rc_adjbuf* w, const float* I,
+                                 const float* alb, int K, int B, uint32_t seed, float* R, hipStream_t stream);
+hipError_t rc_launch_adj_backward(int trav, const rc_dscene* s, const rc_dstate* st, const rc_adjbuf* w, const float* gR, int K,
+                                  int B, uint32_t seed, float* dI, float* dalb, hipStream_t stream);

@@ -6,49 +6,7 @@
 // wave-level partial sums use the full 64-lane width.
 #include "rc_device.h"
 #include "rc_kernels.h"
-
-#define RC_TB 256
-
-// Reduction of a 64-bit integer over the 64 lanes of a wave (any order gives the same sum).
-__device__ __forceinline__ long long rc_wave_sum_i64(long long v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
-    return v;
-}
-__device__ __forceinline__ double rc_wave_sum_f64(double v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
-    return v;
-}
-
-__device__ __forceinline__ long long rc_fx(float v, float scale) { return (long long)(v * scale); }
-
-// Static geometry per traversal mode: 0 binary in the prototype's order, 1 binary closest-hit,
-// 2 wide closest-hit (LDS stack).
-template <int TRAV>
-__device__ __forceinline__ void rc_trace_static(f3 o, f3 d, const rc_dscene& s, float& tmin, int& q, uint32_t* lstk) {
-    if (TRAV == 0) rc_trace_bvh_ref(o, d, s, tmin, q);
-    else if (TRAV == 1) rc_trace_bvh_closest(o, d, s, tmin, q);
-    else rc_trace_wide_closest(o, d, s, tmin, q, lstk);
-}
-
-// Dynamic triangles from LDS, then the static BVH carrying tmin (cuda/poc_gpu.cu:340-344).
-template <int TRAV>
-__device__ __forceinline__ bool rc_trace_full(f3 o, f3 d, const float* T, const int* TQ, int nt, const rc_dscene& s, float& tmin, int& q,
-                                              uint32_t* lstk) {
-    bool h = rc_trace_lds(o, d, T, TQ, nt, tmin, q);
-    if (s.nst > 0) { rc_trace_static<TRAV>(o, d, s, tmin, q, lstk); h = q != -1; }
-    return h;
-}
-
-// Dynamic LDS of the tracing kernels: nt triangles (9 floats) + their quad ids, then the
-// traversal stack (wstack levels x 256 lanes) at a 16-byte aligned offset.
-__device__ __forceinline__ void rc_load_dyn(const rc_dstate& st, float* sh, int* shq) {
-    for (int j = threadIdx.x; j < 9 * st.nt; j += blockDim.x) sh[j] = st.T[j];
-    for (int j = threadIdx.x; j < st.nt; j += blockDim.x) shq[j] = st.TQ[j];
-    __syncthreads();
-}
-__device__ __forceinline__ uint32_t* rc_lstack(float* sh, int nt) { return (uint32_t*)(sh + ((10 * nt + 3) & ~3)); }
+#include "rc_trace.h"
 
 // ------------------------------------------------------------------------------ rays (kernel ABI)
 template <int TRAV>
@@ -882,12 +840,6 @@ __global__ void __launch_bounds__(RC_TB) k_vol_lights(const uint32_t* Va, const 
 }
 
 // ------------------------------------------------------------------------------ launchers
-static inline int rc_grid(long long n) { return (int)((n + RC_TB - 1) / RC_TB); }
-// dynamic LDS of a tracing kernel: dynamic triangles + (wide traversal) the lane stacks
-static inline size_t rc_shm(const rc_dstate& st, const rc_dscene& s, int trav) {
-    size_t tri = (((size_t)st.nt * 10 + 3) & ~(size_t)3) * sizeof(float);
-    return tri + (trav == 2 ? (size_t)s.wstack * 256 * sizeof(uint32_t) : 0);
-}
 
 hipError_t rc_launch_trace_rays(int trav, const rc_dscene* s, const rc_dstate* st, const float* O, const float* D, int n,
                                 float* t, int* q, hipStream_t stream) {

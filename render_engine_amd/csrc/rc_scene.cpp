@@ -37,11 +37,11 @@ uint64_t spread3(uint64_t x) {   // 21 bits -> every third bit (Morton interleav
     return x;
 }
 
-void Mesh::add_tri(const float* a, const float* b, const float* c, float albedo, bool dsided) {
+void Mesh::add_tri(const float* a, const float* b, const float* c, float albedo, bool dsided, int material) {
     V.insert(V.end(), a, a + 3); V.insert(V.end(), b, b + 3); V.insert(V.end(), c, c + 3);
-    alb.push_back(albedo); ds.push_back(dsided ? 1 : 0);
+    alb.push_back(albedo); ds.push_back(dsided ? 1 : 0); mat.push_back(material);
 }
-void Mesh::add_quad_grid(const float o[3], const float u[3], const float v[3], int nu, int nv, float albedo, bool dsided) {
+void Mesh::add_quad_grid(const float o[3], const float u[3], const float v[3], int nu, int nv, float albedo, bool dsided, int material) {
     for (int j = 0; j < nv; ++j)
         for (int i = 0; i < nu; ++i) {
             float p00[3], p10[3], p11[3], p01[3];
@@ -50,8 +50,8 @@ void Mesh::add_quad_grid(const float o[3], const float u[3], const float v[3], i
                 float fv0 = v[a] * ((float)j / nv), fv1 = v[a] * ((float)(j + 1) / nv);
                 p00[a] = o[a] + fu0 + fv0; p10[a] = o[a] + fu1 + fv0; p11[a] = o[a] + fu1 + fv1; p01[a] = o[a] + fu0 + fv1;
             }
-            add_tri(p00, p10, p11, albedo, dsided);
-            add_tri(p00, p11, p01, albedo, dsided);
+            add_tri(p00, p10, p11, albedo, dsided, material);
+            add_tri(p00, p11, p01, albedo, dsided, material);
         }
 }
 
@@ -229,9 +229,13 @@ void sample_cells(const Mesh& M, float h, const float lo[3], std::vector<uint64_
             for (int sd = 0; sd < sides; ++sd) {
                 float nn[3] = {sd ? -n[0] : n[0], sd ? -n[1] : n[1], sd ? -n[2] : n[2]};
                 uint64_t key = cell_key(c[0], c[1], c[2], normal_bucket(nn[0], nn[1], nn[2]));
-                auto ins = cells.try_emplace(key, CellRec{{0, 0, 0}, 1e30f, {0, 0, 1}, 0.0, 0.0});
+                auto ins = cells.try_emplace(key, CellRec{{0, 0, 0}, 1e30f, {0, 0, 1}, 0.0, 0.0, -1});
                 CellRec& rec = ins.first->second;
-                if (d2 < rec.d2) { rec.d2 = d2; for (int x = 0; x < 3; ++x) { rec.p[x] = P[x]; rec.n[x] = nn[x]; } }
+                if (d2 < rec.d2) {
+                    rec.d2 = d2;
+                    for (int x = 0; x < 3; ++x) { rec.p[x] = P[x]; rec.n[x] = nn[x]; }
+                    rec.mat = (size_t)t < M.mat.size() ? M.mat[t] : -1;
+                }
                 rec.asum += (double)M.alb[t] * ar / ns;
                 rec.wsum += (double)ar / ns;
             }
@@ -292,6 +296,26 @@ std::string prepare(const SceneDesc& d, Scene& S) {
     if (d.vx * d.vy * d.vz % 32 != 0 || d.vx * d.vy * d.vz > 32 * RC_MAX_NW) return "transit grid must hold 32..128 bits in whole words";
     S = Scene();
     S.mesh = d.mesh;
+    {   // material table and per-triangle materials
+        Mesh& Mm = S.mesh;
+        Mm.mat.resize(Mm.nt(), -1);
+        S.matalb = d.matalb;
+        bool derive = S.matalb.empty();
+        for (int t = 0; t < Mm.nt() && !derive; ++t)
+            if (Mm.mat[t] < 0 || Mm.mat[t] >= (int)S.matalb.size()) derive = true;
+        if (derive) {   // the distinct albedos, sorted, are the materials
+            std::vector<float> u(Mm.alb);
+            if (!d.objects.empty()) u.push_back(d.dyn_albedo);
+            std::sort(u.begin(), u.end());
+            u.erase(std::unique(u.begin(), u.end()), u.end());
+            S.matalb = u;
+            for (int t = 0; t < Mm.nt(); ++t) Mm.mat[t] = (int)(std::lower_bound(u.begin(), u.end(), Mm.alb[t]) - u.begin());
+        }
+        S.nmat = (int)S.matalb.size();
+    }
+    const int dyn_mat = (d.dyn_mat >= 0 && d.dyn_mat < S.nmat)
+        ? d.dyn_mat : (int)(std::lower_bound(S.matalb.begin(), S.matalb.end(), d.dyn_albedo) - S.matalb.begin());
+    if (!d.objects.empty() && (dyn_mat >= S.nmat || S.matalb[dyn_mat] != d.dyn_albedo)) return "dynamic objects' albedo is not a material";
     S.vx = d.vx; S.vy = d.vy; S.vz = d.vz; S.nw = d.vx * d.vy * d.vz / 32;
     S.nl = d.nl; S.L = d.L; S.ltype = d.ltype; S.cam = d.cam;
     S.NS = (int)d.inten.size();
@@ -334,7 +358,7 @@ std::string prepare(const SceneDesc& d, Scene& S) {
     std::vector<uint64_t> keys; std::vector<CellRec> recs;
     if (d.h > 0.f) {
         S.h = d.h;
-        sample_cells(M, S.h, S.lo, keys, recs);
+        sample_cells(S.mesh, S.h, S.lo, keys, recs);
     } else {
         if (d.target_points <= 0) return "either h or target_points must be set";
         double atot = 0;
@@ -346,7 +370,7 @@ std::string prepare(const SceneDesc& d, Scene& S) {
         }
         S.h = (float)std::sqrt(atot / d.target_points);
         for (int it = 0; it < 5; ++it) {
-            sample_cells(M, S.h, S.lo, keys, recs);
+            sample_cells(S.mesh, S.h, S.lo, keys, recs);
             double ratio = (double)keys.size() / d.target_points;
             if (std::fabs(ratio - 1.0) < 0.08 || it == 4) break;
             S.h *= (float)std::min(2.0, std::max(0.5, std::sqrt(ratio)));
@@ -365,6 +389,7 @@ std::string prepare(const SceneDesc& d, Scene& S) {
         const CellRec& c = recs[i];
         for (int x = 0; x < 3; ++x) { S.Pst.push_back(c.p[x]); S.Nst.push_back(c.n[x]); }
         S.A.push_back((float)(c.wsum > 0 ? c.asum / c.wsum : 0.5));
+        S.mat.push_back(c.mat);
         int cx, cy, cz, b;
         decode_key(keys[i], cx, cy, cz, b);
         auto z = zone.try_emplace(cell_key(cx >> 2, cy >> 2, cz >> 2, b), (int)zone.size());
@@ -390,7 +415,9 @@ std::string prepare(const SceneDesc& d, Scene& S) {
             S.dims.push_back(nu); S.dims.push_back(nv); S.off.push_back(npt);
             int tu = (nu + 3) / 4, tv = (nv + 3) / 4;
             for (int iv = 0; iv < nv; ++iv)
-                for (int iu = 0; iu < nu; ++iu) { S.A.push_back(d.dyn_albedo); S.region.push_back(S.NR + (iv / 4) * tu + iu / 4); }
+                for (int iu = 0; iu < nu; ++iu) {
+                    S.A.push_back(d.dyn_albedo); S.mat.push_back(dyn_mat); S.region.push_back(S.NR + (iv / 4) * tu + iu / 4);
+                }
             S.NR += tu * tv;
             npt += nu * nv;
         }
@@ -570,6 +597,12 @@ std::string load_scene_file(const char* path, Scene& S, FileRefs& R) {
     in.rd(S.L, 3 * (size_t)NL); in.rd(S.A, Np); in.rd(S.region, Np);
     S.ltype.assign(NL, 0);
     if (!in.ok) return "truncated light / albedo / region data";
+    S.matalb = S.A;   // materials: the distinct entry albedos, sorted
+    std::sort(S.matalb.begin(), S.matalb.end());
+    S.matalb.erase(std::unique(S.matalb.begin(), S.matalb.end()), S.matalb.end());
+    S.nmat = (int)S.matalb.size();
+    S.mat.resize(Np);
+    for (int i = 0; i < Np; ++i) S.mat[i] = (int)(std::lower_bound(S.matalb.begin(), S.matalb.end(), S.A[i]) - S.matalb.begin());
     R = FileRefs(); R.has = has_ref != 0;
     for (int st = 0; st < NS; ++st) {
         std::vector<float> inten, P, N, Q, T; std::vector<int> dims, off, TQ;
@@ -617,10 +650,10 @@ std::string load_scene_file(const char* path, Scene& S, FileRefs& R) {
 }
 
 // ------------------------------------------------------------------------------------ generators
-static void face_grid(Mesh& M, const float F[12], float cell, float albedo) {
+static void face_grid(Mesh& M, const float F[12], float cell, float albedo, int material = -1) {
     float lu = std::sqrt(F[3] * F[3] + F[4] * F[4] + F[5] * F[5]), lv = std::sqrt(F[6] * F[6] + F[7] * F[7] + F[8] * F[8]);
     int nu = std::max(1, (int)std::lround(lu / cell)), nv = std::max(1, (int)std::lround(lv / cell));
-    M.add_quad_grid(F, F + 3, F + 6, nu, nv, albedo, false);
+    M.add_quad_grid(F, F + 3, F + 6, nu, nv, albedo, false, material);
 }
 static int face_tris(const float F[12], float cell) {
     float lu = std::sqrt(F[3] * F[3] + F[4] * F[4] + F[5] * F[5]), lv = std::sqrt(F[6] * F[6] + F[7] * F[7] + F[8] * F[8]);
@@ -701,10 +734,18 @@ SceneDesc gen_street(uint64_t seed, int nboxes, int target_tris, int nlamps, boo
     int cols = (int)std::ceil(std::sqrt(nboxes * 1.25)), rows = (nboxes + cols - 1) / cols;
     std::vector<std::array<float, 12>> faces;
     std::vector<float> falb;
+    std::vector<int> fmat;
+    // 4096 materials with distinct albedos (stratified in [0.2, 0.8)); every box draws one
+    const int nmat = 4096;
+    {
+        HostRng rm(seed ^ 0x6d6174657269616cull);
+        d.matalb.resize(nmat);
+        for (int m = 0; m < nmat; ++m) d.matalb[m] = 0.2f + 0.6f * (((float)m + rm.u()) / (float)nmat);
+    }
     float gx = cols * pitch, gy = rows * pitch;
     {   // ground
         std::array<float, 12> g = {-pitch, -pitch, 0, gx + 2 * pitch, 0, 0, 0, gy + 2 * pitch, 0, 0, 0, 1};
-        faces.push_back(g); falb.push_back(0.35f);
+        faces.push_back(g); falb.push_back(d.matalb[1024]); fmat.push_back(1024);
     }
     for (int b = 0; b < nboxes; ++b) {
         int ci = b % cols, ri = b / cols;
@@ -716,13 +757,17 @@ SceneDesc gen_street(uint64_t seed, int nboxes, int target_tris, int nlamps, boo
             c[a] = base + e[a] + room * r.u();
         }
         c[2] = e[2];
-        float alb = 0.2f + 0.6f * r.u();
+        const int m = std::min((int)(r.u() * (float)nmat), nmat - 1);
         float F[6][12];
         box_faces(c, e, F);
-        for (int q = 0; q < 5; ++q) { std::array<float, 12> a; memcpy(a.data(), F[q], 48); faces.push_back(a); falb.push_back(alb); }
+        for (int q = 0; q < 5; ++q) {
+            std::array<float, 12> a; memcpy(a.data(), F[q], 48); faces.push_back(a); falb.push_back(d.matalb[m]); fmat.push_back(m);
+        }
     }
     float cell = fit_cell(faces, target_tris);
-    for (size_t k = 0; k < faces.size(); ++k) face_grid(d.mesh, faces[k].data(), cell, falb[k]);
+    for (size_t k = 0; k < faces.size(); ++k) face_grid(d.mesh, faces[k].data(), cell, falb[k], fmat[k]);
+    d.dyn_mat = 3413;
+    d.dyn_albedo = d.matalb[d.dyn_mat];
     // lamps at street crossings (x = i*pitch, y = j*pitch), lamp 0 at the crossing nearest the middle
     int cx0 = cols / 2, cy0 = rows / 2;
     std::vector<std::pair<int, int>> used{{cx0, cy0}};

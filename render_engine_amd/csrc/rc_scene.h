@@ -24,13 +24,14 @@ int normal_bucket(float nx, float ny, float nz);
 void decode_key(uint64_t k, int& cx, int& cy, int& cz, int& b);
 uint64_t spread3(uint64_t x);
 
-struct Mesh {                       // input mesh: triangles, per-triangle albedo and double-sided flag
+struct Mesh {                       // input mesh: triangles, per-triangle albedo, material and double-sided flag
     std::vector<float> V;           // 9 per triangle
     std::vector<float> alb;
+    std::vector<int> mat;           // material index per triangle (-1: derived from the albedo)
     std::vector<uint8_t> ds;
     int nt() const { return (int)alb.size(); }
-    void add_tri(const float* a, const float* b, const float* c, float albedo, bool dsided);
-    void add_quad_grid(const float o[3], const float u[3], const float v[3], int nu, int nv, float albedo, bool dsided);
+    void add_tri(const float* a, const float* b, const float* c, float albedo, bool dsided, int material = -1);
+    void add_quad_grid(const float o[3], const float u[3], const float v[3], int nu, int nv, float albedo, bool dsided, int material = -1);
 };
 
 struct DynBox { float c[3]; float e[3]; bool present; };   // axis-aligned box object per state
@@ -48,6 +49,8 @@ struct SceneDesc {
     int vx = 8, vy = 4, vz = 4;     // transit grid (cuda/build.sh:12)
     rc_camera cam{};
     float dyn_albedo = 0.7f;
+    std::vector<float> matalb;      // material albedo table (empty: the distinct albedos, sorted)
+    int dyn_mat = -1;               // material of the dynamic objects' points (-1: by dyn_albedo)
     int fine = 0;                   // fine voxel grid fine^3 for per-light volumes (0: none)
 };
 
@@ -67,6 +70,11 @@ struct Scene {
     std::vector<uint64_t> HK; std::vector<int> HV;
     std::vector<float> Pst, Nst, A;
     std::vector<int> region;
+    // materials: per-entry index into the albedo table (the material of the cell's
+    // representative sample; dynamic points: the objects' material)
+    int nmat = 0;
+    std::vector<float> matalb;
+    std::vector<int> mat;
     // lights
     int nl = 0;
     std::vector<float> L; std::vector<int> ltype;
@@ -100,7 +108,7 @@ void build_bvh(const Mesh& M, std::vector<rc_bvh_node>& nodes, std::vector<int>&
 // their triangle ranges).
 void build_wide(const std::vector<rc_bvh_node>& bn, std::vector<rc_wnode>& wn, int& depth);
 
-struct CellRec { float p[3]; float d2; float n[3]; double asum, wsum; };
+struct CellRec { float p[3]; float d2; float n[3]; double asum, wsum; int mat; };
 // Cache-cell sampling on the triangles (cuda/prep_mesh.cuh:129-157): returns cells sorted by
 // (morton*8 + bucket, key) with their representative point, normal and area-weighted albedo.
 void sample_cells(const Mesh& M, float h, const float lo[3], std::vector<uint64_t>& keys, std::vector<CellRec>& recs);

@@ -137,3 +137,19 @@ typedef struct rc_qwork {
     int32_t* idx; int32_t* sidx;   // per-segment compacted id lists: extension rays, shadow rays
     int32_t* seg;                  // [2 * segments]: extension-list counts, then shadow-list counts
 } rc_qwork;
+
+// Device buffers of the cached-radiance adjoint (Np entries, nl lights, B bounces).
+typedef struct rc_adjbuf {
+    float* G;                      // [Np][nl] unit-intensity direct light
+    uint64_t* SG;                  // [Np] support of G
+    int32_t* mat;                  // [Np] material of each entry
+    float* A; float* E;            // [Np] albedo, emission-weighted direct light of the last forward
+    float* L;                      // [B][Np] L^(1..B) of the last forward
+    uint64_t* S[2];                // [Np] support ping-pong of the forward bounces
+    uint64_t* SR;                  // [Np] support of R
+    float* dA; float* dE;          // [Np] adjoints of A and E
+    float* lam[2];                 // [Np] adjoint of L^(b) ping-pong
+    float* part;                   // [blocks][nl] partial sums of dI
+    int32_t* perm; int32_t* moff;  // entries sorted by material, offsets [nmat + 1]
+    int32_t nmat, pad0;
+} rc_adjbuf;
