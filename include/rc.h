@@ -51,6 +51,10 @@ typedef struct rc_method {
     int memory, ramp, wake_verify, snaps, snap_verify;
     int flush;           /* event to a state not in a snapshot slot clears the whole cache */
     int targeted;        /* dependency-targeted invalidation at the event */
+    int grad_masks;      /* > 0: targeted invalidation reads, instead of the recorded light masks, the
+                            gradient-support masks of the cached-radiance adjoint of both states
+                            (grad_masks rays per entry and bounce, 3 bounces, seed 0x5eed0000 + state;
+                            computed at rc_cache_begin) */
 } rc_method;
 
 #ifndef RC_CAMERA_DEFINED

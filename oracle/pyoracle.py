@@ -45,7 +45,7 @@ class OrcMethod(C.Structure):
         ("oracle", C.c_int), ("track_every", C.c_int), ("rescale", C.c_int), ("resp_mask", C.c_uint64),
         ("nmax_resp", C.c_float), ("ddgi", C.c_int), ("ddgi_h", C.c_float), ("ddgi_thr", C.c_float),
         ("ddgi_clamp", C.c_int), ("memory", C.c_int), ("ramp", C.c_int), ("wake_verify", C.c_int), ("snaps", C.c_int),
-        ("snap_verify", C.c_int), ("flush", C.c_int), ("targeted", C.c_int),
+        ("snap_verify", C.c_int), ("flush", C.c_int), ("targeted", C.c_int), ("grad_masks", C.c_int),
     ]
 
 
@@ -89,6 +89,7 @@ def _load():
     lib.orc_runner_create.restype = _P
     lib.orc_runner_create.argtypes = [_P, C.c_int, C.POINTER(OrcMethod), C.c_int, _P, _P, _P, _P, _P, _P, _P, _P, _P]
     lib.orc_runner_destroy.argtypes = [_P]
+    lib.orc_runner_set_grad_masks.argtypes = [_P, _P]
     lib.orc_runner_begin.argtypes = [_P, C.c_int, C.c_uint32]
     lib.orc_runner_frame.restype = C.c_double
     lib.orc_runner_frame.argtypes = [_P, C.c_int, C.c_int, C.POINTER(C.c_int)]
@@ -196,6 +197,11 @@ class OracleRunner:
         if getattr(self, "h", None):
             _destroy(self.h)
             self.h = None
+
+    def set_grad_masks(self, masks):
+        """per-state gradient-support masks [NS][Np] for the grad_masks methods"""
+        self.gmask = np.ascontiguousarray(masks, np.uint64)
+        lib.orc_runner_set_grad_masks(self.h, ptr(self.gmask))
 
     def begin(self, st, seed):
         lib.orc_runner_begin(self.h, st, seed)

@@ -747,9 +747,13 @@ void orc_k_oracle_method(const orc_cache* c, int Np, const float* rold, const fl
 }
 /* resolve-time targeted invalidation (rtxgi_bench.patch:1028-1050) */
 void orc_k_invalidate(const orc_cache* c, int Np, int nl, uint64_t inv_lights, const int32_t* V, int nv, int nw) {
+    orc_k_invalidate_masks(c, Np, nl, inv_lights, V, nv, nw, NULL, NULL);
+}
+void orc_k_invalidate_masks(const orc_cache* c, int Np, int nl, uint64_t inv_lights, const int32_t* V, int nv, int nw,
+                            const uint64_t* ga, const uint64_t* gb) {
     for (int i = 0; i < Np; ++i) {
-        uint64_t ldp = c->lcur[i] | c->lprev[i];
-        int known = ldp != 0ull;
+        uint64_t ldp = ga ? (ga[i] | gb[i]) : (c->lcur[i] | c->lprev[i]);
+        int known = ga != NULL || ldp != 0ull;
         for (int w = 0; w < nw && !known; ++w) known = (c->cur[(size_t)w * Np + i] | c->prev[(size_t)w * Np + i]) != 0u;
         int t = !known || (ldp & inv_lights) != 0ull;
         for (int j = 0; j < nv && !t; ++j) {
@@ -967,7 +971,10 @@ struct orc_runner {
     int slot_state[ORC_KSNAP];
     uint64_t dormant; float ilearn[128];
     float nmax;
+    const uint64_t* gmask;
 };
+
+void orc_runner_set_grad_masks(orc_runner* r, const uint64_t* masks) { r->gmask = masks; }
 
 orc_runner* orc_runner_create(const orc_scene* states, int NS, const orc_method* m, int trav, const int32_t* occ_flat,
                               const int64_t* occ_off, const int32_t* em_flat, const int64_t* em_off, const float* const* Pw,
@@ -1107,7 +1114,9 @@ static void event_block(orc_runner* r, int e, int si, int* invalidated) {
         }
         if (inv || nv) {
             c->npend[1] = 0;
-            orc_k_invalidate(c, Np, nl, inv, r->occ + r->occ_off[pe], nv, s->nw);
+            const int gm = M->grad_masks > 0 && r->gmask;
+            orc_k_invalidate_masks(c, Np, nl, inv, r->occ + r->occ_off[pe], nv, s->nw, gm ? r->gmask + (size_t)e * Np : NULL,
+                                   gm ? r->gmask + (size_t)si * Np : NULL);
             *invalidated = c->npend[1];
             c->npend[1] = 0;
         }

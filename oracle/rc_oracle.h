@@ -123,6 +123,9 @@ void orc_k_touched(const orc_cache* c, int Np, int nl, const int32_t* V, int nv,
 void orc_k_wake(const orc_cache* c, int Np, int nl, int l, int verify, uint64_t dormant);
 void orc_k_oracle_method(const orc_cache* c, int Np, const float* rold, const float* rnew);
 void orc_k_invalidate(const orc_cache* c, int Np, int nl, uint64_t inv_lights, const int32_t* V, int nv, int nw);
+/* the same with the entries' light dependencies taken from masks ga[i] | gb[i] (no unknown entries) */
+void orc_k_invalidate_masks(const orc_cache* c, int Np, int nl, uint64_t inv_lights, const int32_t* V, int nv, int nw,
+                            const uint64_t* ga, const uint64_t* gb);
 void orc_k_flush(const orc_cache* c, int Np, int nl);
 void orc_k_error(const orc_cache* c, int Np, int nl, const float* ref, const uint8_t* valid, uint64_t dormant);
 void orc_k_ref(const orc_scene* s, int trav, const float* Lin, float* Lout, float* acc, float* closec, uint32_t seed, int K,
@@ -171,7 +174,7 @@ typedef struct orc_runner orc_runner;
 typedef struct orc_method {
     int mode; float nmax, budget; int deps, verify, oracle, track_every, rescale;
     uint64_t resp_mask; float nmax_resp; int ddgi; float ddgi_h, ddgi_thr; int ddgi_clamp;
-    int memory, ramp, wake_verify, snaps, snap_verify, flush, targeted;
+    int memory, ramp, wake_verify, snaps, snap_verify, flush, targeted, grad_masks;
 } orc_method;
 /* states[NS]: one orc_scene per state (static fields shared); occ / em event lists like the
  * library's (occ[a*NS+b], em[(a*NS+b)*nl+l]) given as offsets into one flat int array. */
@@ -180,6 +183,9 @@ orc_runner* orc_runner_create(const orc_scene* states, int NS, const orc_method*
                               const float* const* ref, const uint8_t* const* valid, const int32_t* occf_flat,
                               const int64_t* occf_off);
 void orc_runner_destroy(orc_runner* r);
+/* per-state gradient-support masks (masks[st * Np + i]) read by targeted invalidation when the
+ * method sets grad_masks; the array stays owned by the caller */
+void orc_runner_set_grad_masks(orc_runner* r, const uint64_t* masks);
 void orc_runner_begin(orc_runner* r, int state0, uint32_t seed);
 /* one frame; returns the relative RMS error (or -1 without a reference) */
 double orc_runner_frame(orc_runner* r, int state, int want_err, int* invalidated);

@@ -386,6 +386,7 @@ struct rc_cache {
     int slot_state[RC_KSNAP];
     rc_params p{};
     int render_mode = RC_RENDER_FUSED;
+    std::vector<uint64_t*> gmask;   // per state: gradient-support masks (grad_masks methods)
     rc_qwork qw{};                // staged camera-pass workspace, sized for qw_paths paths
     long long qw_paths = 0;
     std::vector<void*> qw_owned;
@@ -431,6 +432,8 @@ extern "C" rc_status rc_cache_create(rc_scene* s, const rc_method* m, rc_cache**
 
 extern "C" void rc_cache_destroy(rc_cache* c) { delete c; }
 
+static rc_status cache_grad_masks(rc_cache* c);
+
 extern "C" rc_status rc_cache_begin(rc_cache* c, int state0, uint32_t seed) {
     if (!c) return fail(RC_ERR_ARG, "NULL cache");
     rc_scene* S = c->S;
@@ -454,6 +457,10 @@ extern "C" rc_status rc_cache_begin(rc_cache* c, int state0, uint32_t seed) {
     HIPCK(hipMemsetAsync(d.sumpr, 0, 16, st));
     HIPCK(rc_launch_sumpr(&d, (int)Np, st));   // sum of 1/(n+1) at n = nmax
     HIPCK(hipStreamSynchronize(st));
+    if (c->M.targeted && c->M.grad_masks > 0 && c->gmask.empty()) {
+        rc_status r = cache_grad_masks(c);
+        if (r != RC_OK) return r;
+    }
     c->begun = true;
     c->r_valid = false;
     c->prev_state = state0;
@@ -557,7 +564,8 @@ static rc_status run_event(rc_cache* c, int e, int si, int& invalidated) {
         }
         if (inv || nv) {
             HIPCK(hipMemsetAsync(d.npend + 1, 0, 4, st));
-            HIPCK(rc_launch_invalidate(&d, Np, nl, inv, S->dOcc[pe], nv, H.nw, st));
+            const bool gm = !c->gmask.empty();
+            HIPCK(rc_launch_invalidate(&d, Np, nl, inv, S->dOcc[pe], nv, H.nw, gm ? c->gmask[e] : nullptr, gm ? c->gmask[si] : nullptr, st));
             int h2 = 0;
             HIPCK(hipMemcpyAsync(&h2, d.npend + 1, 4, hipMemcpyDeviceToHost, st));
             HIPCK(hipStreamSynchronize(st));
@@ -918,5 +926,35 @@ extern "C" rc_status rc_adjoint_get(const rc_adjoint* a, int which, void* host_d
     if (bytes != b) return fail(RC_ERR_ARG, "rc_adjoint_get: %zu bytes given, %zu expected", bytes, b);
     HIPCK(hipDeviceSynchronize());
     HIPCK(hipMemcpy(host_dst, p, b, hipMemcpyDeviceToHost));
+    return RC_OK;
+}
+
+// Gradient-support masks of every state for the grad_masks methods: one adjoint forward per
+// state (unit emission, the material albedos; the support does not depend on their values
+// while they are positive), grad_masks rays per entry and bounce, 3 bounces.
+static rc_status cache_grad_masks(rc_cache* c) {
+    rc_scene* S = c->S;
+    const rc::Scene& H = S->H;
+    const size_t Np = H.Np;
+    float *dI = nullptr, *dalb = nullptr;
+    HIPCK(dalloc(&dI, (size_t)H.nl, c->owned));
+    HIPCK(dalloc(&dalb, (size_t)H.nmat, c->owned));
+    std::vector<float> ones(H.nl, 1.f);
+    HIPCK(hipMemcpyAsync(dI, ones.data(), ones.size() * 4, hipMemcpyHostToDevice, S->stream));
+    HIPCK(hipMemcpyAsync(dalb, H.matalb.data(), H.matalb.size() * 4, hipMemcpyHostToDevice, S->stream));
+    HIPCK(hipStreamSynchronize(S->stream));
+    for (int st = 0; st < H.NS; ++st) {
+        rc_adjoint* a = nullptr;
+        rc_status r = rc_adjoint_create(S, st, c->M.grad_masks, 3, 0x5eed0000u + (uint32_t)st, &a);
+        if (r != RC_OK) return r;
+        uint64_t* g = nullptr;
+        hipError_t e = dalloc(&g, Np, c->owned);
+        if (e == hipSuccess) e = rc_launch_adj_forward(S->trav, &S->ds, &S->dst[st], &a->w, dI, dalb, a->K, a->B, a->seed, nullptr, S->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(g, a->w.SR, Np * 8, hipMemcpyDeviceToDevice, S->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(S->stream);
+        rc_adjoint_destroy(a);
+        if (e != hipSuccess) return fail(RC_ERR_HIP, "gradient masks: %s", hipGetErrorString(e));
+        c->gmask.push_back(g);
+    }
     return RC_OK;
 }

@@ -19,7 +19,8 @@ hipError_t rc_launch_rescale(const rc_dcache* c, int Np, int nl, int l, float ra
 hipError_t rc_launch_touched(const rc_dcache* c, int Np, int nl, const int* V, int nv, int kind, uint64_t dormant, hipStream_t stream);
 hipError_t rc_launch_wake(const rc_dcache* c, int Np, int nl, int l, int verify, uint64_t dormant, hipStream_t stream);
 hipError_t rc_launch_oracle_method(const rc_dcache* c, int Np, const float* rold, const float* rnew, hipStream_t stream);
-hipError_t rc_launch_invalidate(const rc_dcache* c, int Np, int nl, uint64_t inv_lights, const int* V, int nv, int nw, hipStream_t stream);
+hipError_t rc_launch_invalidate(const rc_dcache* c, int Np, int nl, uint64_t inv_lights, const int* V, int nv, int nw,
+                                const uint64_t* ga, const uint64_t* gb, hipStream_t stream);
 hipError_t rc_launch_flush(const rc_dcache* c, int Np, int nl, hipStream_t stream);
 hipError_t rc_launch_error(const rc_dcache* c, int Np, int nl, const float* ref, const uint8_t* valid, uint64_t dormant, hipStream_t stream);
 hipError_t rc_launch_ref(int trav, const rc_dscene* s, const rc_dstate* st, const float* Lin, float* Lout, float* acc, float* closec,

@@ -361,13 +361,15 @@ __global__ void __launch_bounds__(RC_TB) k_oracle_method(rc_dcache c, int Np, co
 // recorded dependencies intersect the event (changed lights, or event voxels in the transit
 // banks) restarts from zero; an entry with no recorded dependency is reset too (unknown ->
 // conservative). Returns the number of invalidated entries in c.npend[1].
+// ga / gb (may be NULL): per-entry gradient-support masks of the two states, used instead of
+// the recorded light masks (an entry with no support does not depend on any light)
 __global__ void __launch_bounds__(RC_TB) k_invalidate(rc_dcache c, int Np, int nl, uint64_t inv_lights,
-                                                     const int* V, int nv, int nw) {
+                                                     const int* V, int nv, int nw, const uint64_t* ga, const uint64_t* gb) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     int hit = 0;
     if (i < Np) {
-        uint64_t ld = c.lcur[i] | c.lprev[i];
-        bool known = ld != 0ull;
+        uint64_t ld = ga ? (ga[i] | gb[i]) : (c.lcur[i] | c.lprev[i]);
+        bool known = ga != nullptr || ld != 0ull;
         for (int w = 0; w < nw && !known; ++w) known = (c.cur[(int64_t)w * Np + i] | c.prev[(int64_t)w * Np + i]) != 0u;
         bool t = !known || (ld & inv_lights) != 0ull;
         for (int j = 0; j < nv && !t; ++j) {
@@ -915,8 +917,9 @@ hipError_t rc_launch_oracle_method(const rc_dcache* c, int Np, const float* rold
     hipLaunchKernelGGL(k_oracle_method, dim3(rc_grid(Np)), dim3(RC_TB), 0, stream, *c, Np, rold, rnew);
     return hipGetLastError();
 }
-hipError_t rc_launch_invalidate(const rc_dcache* c, int Np, int nl, uint64_t inv_lights, const int* V, int nv, int nw, hipStream_t stream) {
-    hipLaunchKernelGGL(k_invalidate, dim3(rc_grid(Np)), dim3(RC_TB), 0, stream, *c, Np, nl, inv_lights, V, nv, nw);
+hipError_t rc_launch_invalidate(const rc_dcache* c, int Np, int nl, uint64_t inv_lights, const int* V, int nv, int nw,
+                                const uint64_t* ga, const uint64_t* gb, hipStream_t stream) {
+    hipLaunchKernelGGL(k_invalidate, dim3(rc_grid(Np)), dim3(RC_TB), 0, stream, *c, Np, nl, inv_lights, V, nv, nw, ga, gb);
     return hipGetLastError();
 }
 hipError_t rc_launch_flush(const rc_dcache* c, int Np, int nl, hipStream_t stream) {

@@ -7,10 +7,10 @@ from ._ffi import Method
 
 def make(mode=0, nmax=16.0, budget=1.0, deps=False, verify=False, oracle=False, track_every=1, rescale=False, resp_mask=0,
          nmax_resp=8.0, ddgi=False, ddgi_h=0.97, ddgi_thr=0.25, ddgi_clamp=True, memory=False, ramp=False,
-         wake_verify=False, snaps=False, snap_verify=False, flush=False, targeted=False):
+         wake_verify=False, snaps=False, snap_verify=False, flush=False, targeted=False, grad_masks=0):
     return Method(mode, nmax, budget, int(deps), int(verify), int(oracle), track_every, int(rescale), resp_mask, nmax_resp,
                   int(ddgi), ddgi_h, ddgi_thr, int(ddgi_clamp), int(memory), int(ramp), int(wake_verify), int(snaps),
-                  int(snap_verify), int(flush), int(targeted))
+                  int(snap_verify), int(flush), int(targeted), int(grad_masks))
 
 
 # ---- the four methods of the north-star interface
@@ -27,6 +27,13 @@ def flush_snapshots(nmax=32.0, budget=1.0):
 def targeted(nmax=32.0, budget=1.0, track_every=1):
     """entries whose recorded light / voxel dependencies meet the event restart (rtxgi_bench.patch:1028-1050)"""
     return make(mode=0, nmax=nmax, budget=budget, deps=True, targeted=True, track_every=track_every)
+
+
+def targeted_grad(nmax=32.0, budget=1.0, rays=4):
+    """targeted invalidation whose per-entry light dependencies are the nonzero support of the
+    cached-radiance gradient d R_i / d I_l (adjoint forward per state, `rays` rays per entry and
+    bounce, 3 bounces) instead of masks recorded by the entry pass"""
+    return make(mode=0, nmax=nmax, budget=budget, deps=True, targeted=True, grad_masks=rays)
 
 
 # ---- the prototype's timing list (cuda/poc_gpu.cu:1348-1358)

@@ -135,11 +135,21 @@ def test_hash_insert_lookup(torch_gpu, street):
 RUN_METHODS = [
     ("plain", lambda: rea.methods.plain()),
     ("targeted", lambda: rea.methods.targeted()),
+    ("targeted_grad", lambda: rea.methods.targeted_grad(rays=3)),
     ("flush_snapshots", lambda: rea.methods.flush_snapshots()),
 ] + [(n, (lambda m=m: m)) for n, m in rea.methods.timing_list()] + [
     (n, (lambda m=m: m)) for n, m in rea.methods.scenario_list(dyn=0b11) if "Oracle" not in n]
 
 SEQ = {"room": [0] * 22 + [1] * 22 + [0] * 22, "street": [0] * 21 + [3] * 12 + [2] * 12 + [0] * 12 + [1] * 6}
+
+
+def _oracle_grad_masks(scene, osc, rays):
+    """the gradient-support masks a grad_masks cache computes at begin: adjoint forward of every
+    state with unit emission and the material albedos, 3 bounces, seed 0x5eed0000 + state"""
+    i = scene.info
+    alb = scene.array("MATALB")
+    return np.stack([po.OracleAdjoint(osc, st, rays, 3, 0x5EED0000 + st).forward32(np.ones(i.nl, np.float32), alb)[1]
+                     for st in range(i.NS)])
 
 
 @pytest.mark.parametrize("name,mk", RUN_METHODS, ids=[n for n, _ in RUN_METHODS])
@@ -149,6 +159,8 @@ def test_runner_bit_exact(torch_gpu, which, name, mk, request):
     m = mk()
     cache = rea.Cache(scene, m)
     orun = po.OracleRunner(osc, m)
+    if m.grad_masks:
+        orun.set_grad_masks(_oracle_grad_masks(scene, osc, m.grad_masks))
     seq = SEQ[which]
     cache.begin(seq[0], 77)
     orun.begin(seq[0], 77)
