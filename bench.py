@@ -29,6 +29,10 @@ CONFIGS = {
     # street at 1920x1080, 4 spp, 3 bounces, ~1M points, 2^23 grid, 64 lamps
     3: dict(kind="street", seed=1, nboxes=2000, tris=1_000_000, nlamps=64, sun=True, points=1_000_000, table_log2=23, W=1920,
             H=1080, spp=4, bounces=3),
+    # differentiation: the 960x540 street at 2 spp, 63 lamps + sun = 64 emission scalars, 4,096 material albedos;
+    # adjoint of all cached radiance, 3 bounces, 4 rays per entry and bounce
+    4: dict(kind="street", seed=1, nboxes=2000, tris=200_000, nlamps=63, sun=True, points=530_000, table_log2=22, W=960, H=540,
+            spp=2, bounces=3, adjoint=dict(K=4, B=3, seed=17)),
 }
 
 
@@ -122,7 +126,7 @@ def cpu_baseline(osc, scene, seconds, log):
                        f"same scene, state 0", fps=frames / t_tot)
 
 
-def quality(rea, scene, cam, log, pre=20, post=60, ref_spp_rounds=16):
+def quality(rea, scene, cam, log, pre=20, post=60, ref_spp_rounds=16, occluder=False):
     """Light-off events against converged references (cuda/prep_mesh.cuh:454-520 solve, then the
     same query at 64 spp): per method, the region-luminance bias sum(est)/sum(ref) - 1 over the
     pixels where the switched-off light carried > 50 % of the converged radiance (9x9 blur at
@@ -146,6 +150,8 @@ def quality(rea, scene, cam, log, pre=20, post=60, ref_spp_rounds=16):
             acc += img.cpu().numpy()
         refs[st] = (acc / ref_spp_rounds).reshape(cam.H, cam.W)
     events = {"lamp 0 off": (0, 2), "night (sun off)": (0, 1)} if scene.info.NS >= 3 else {"light 1 off": (0, 1)}
+    if occluder and scene.info.NS >= 4:
+        events["occluder inserted"] = (0, 3)
     out = {}
     for name, (a, b) in events.items():
         region = sc.light_off_region(refs[a], refs[b])
@@ -154,7 +160,7 @@ def quality(rea, scene, cam, log, pre=20, post=60, ref_spp_rounds=16):
             out[name] = ev
             continue
         for mname, m in (("plain", rea.methods.plain()), ("flush_snapshots", rea.methods.flush_snapshots()),
-                         ("targeted", rea.methods.targeted())):
+                         ("targeted", rea.methods.targeted()), ("targeted_grad", rea.methods.targeted_grad())):
             c = rea.Cache(scene, m)
             c.begin(a, 777)
             biases, errs, inval = [], [], 0
@@ -177,6 +183,95 @@ def quality(rea, scene, cam, log, pre=20, post=60, ref_spp_rounds=16):
         out[name] = ev
         log(f"{name}: {json.dumps(ev)}")
     return out
+
+
+def bench_adjoint(args, rea, scene, cfg, log):
+    """Config 4: one step = forward + backward of the cached-radiance adjoint (B bounces, K rays
+    per entry and bounce; the backward re-traces them) for J = sum of all cached radiance, giving
+    the gradient w.r.t. every light's emission and every material's albedo. Reports rays/s, the
+    central-difference check (h = 1e-3) of the largest components, the gradient-support masks,
+    and the light-off / occluder events with targeted invalidation from recorded masks vs from
+    the gradient support."""
+    import numpy as np
+    import torch
+
+    import render_engine_amd.adjoint as ad
+
+    info = scene.info
+    a = cfg["adjoint"]
+    t0 = time.time()
+    adj = ad.Adjoint(scene, state=0, K=a["K"], bounces=a["B"], seed=a["seed"])
+    t_unit = time.time() - t0
+    I = torch.tensor(scene.array("INTEN", 0), dtype=torch.float32, device="cuda")
+    alb = torch.tensor(scene.array("MATALB"), dtype=torch.float32, device="cuda")
+    g = torch.ones(info.Np, dtype=torch.float32, device="cuda")
+    for _ in range(args.warmup):
+        adj.forward(I, alb)
+        adj.backward(g)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    fwd, bwd = [], []
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    for _ in range(args.steps):
+        ev[0].record()
+        R = adj.forward(I, alb)
+        ev[1].record()
+        gI, galb = adj.backward(g)
+        ev[2].record()
+        ev[2].synchronize()
+        fwd.append(ev[0].elapsed_time(ev[1]))
+        bwd.append(ev[1].elapsed_time(ev[2]))
+    elapsed = time.perf_counter() - t_start
+    rays_step = 2 * info.Np * a["K"] * a["B"]   # forward rays + the backward's re-traced rays (every entry's adjoint is nonzero)
+    gI_h, galb_h = gI.cpu().numpy(), galb.cpu().numpy()
+    h = 1e-3
+
+    def J(Iv, Av):
+        return float(adj.forward(Iv, Av).double().sum())
+
+    fd = {}
+    for l in np.argsort(-np.abs(gI_h))[:2]:
+        e = torch.zeros_like(I); e[l] = h
+        d = (J(I + e, alb) - J(I - e, alb)) / (2 * h)
+        fd[f"emission[{l}]"] = {"adjoint": float(gI_h[l]), "central_difference": d, "rel_err": float(abs(d - gI_h[l]) / abs(gI_h[l]))}
+    for m in np.argsort(-np.abs(galb_h))[:2]:
+        e = torch.zeros_like(alb); e[m] = h
+        d = (J(I, alb + e) - J(I, alb - e)) / (2 * h)
+        fd[f"albedo[{m}]"] = {"adjoint": float(galb_h[m]), "central_difference": d, "rel_err": float(abs(d - galb_h[m]) / abs(galb_h[m]))}
+    adj.forward(I, alb)
+    sr = adj.support()
+    bits = np.unpackbits(sr.view(np.uint8)).reshape(-1, 64).sum(1)
+    support = {"mean_lights_per_entry": round(float(bits.mean()), 2),
+               "entries_depending_on_lamp0": int(((sr & np.uint64(1)) != 0).sum()),
+               "entries_depending_on_sun": int(((sr >> np.uint64(min(info.nl - 1, 63))) & np.uint64(1)).sum())}
+    log(f"adjoint: fwd {np.mean(fwd):.3f} ms, bwd {np.mean(bwd):.3f} ms, FD {json.dumps(fd)}")
+    qual = None
+    if not args.no_quality:
+        try:
+            qual = quality(rea, scene, scene.camera(), log, occluder=True)
+        except Exception as e:
+            log(f"quality phase failed: {e!r}")
+    ms = float(np.mean(fwd) + np.mean(bwd))
+    out = {
+        "metric": "Mrays/s + cache Mpts/s; frames to <5% error after light-off, 1 GPU",
+        "value": round(rays_step / (ms * 1e3), 2),
+        "unit": "Mrays/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(elapsed * 1e3 / args.steps, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (procedural scene, seeded)",
+        "config": {"workload": f"adjoint of all cached radiance: street {info.nst} tris, {info.Np} entries, {info.nl} emission "
+                               f"scalars, {info.nmat} material albedos, {a['B']} bounces x {a['K']} rays per entry",
+                   "config_index": args.config},
+        "kernels": {"forward_ms": float(np.mean(fwd)), "backward_ms": float(np.mean(bwd)), "unit_direct_light_s": round(t_unit, 3),
+                    "rays_per_step": rays_step},
+        "gradient_check": fd,
+        "gradient_support": support,
+        "roofline": None,
+        "cpu_baseline": None,
+        "light_off": qual,
+    }
+    print(json.dumps(out))
 
 
 def main():
@@ -217,6 +312,9 @@ def main():
     t0 = time.time()
     scene.upload(trav=args.trav)
     log(f"upload + per-state build {time.time() - t0:.1f} s")
+    if "adjoint" in cfg:
+        bench_adjoint(args, rea, scene, cfg, log)
+        return
     m = rea.methods.plain()
     cache = rea.Cache(scene, m)
     cache.set_render_mode(rea.RENDER_STAGED if args.render == "staged" else rea.RENDER_FUSED)
@@ -272,13 +370,20 @@ def main():
     }
     dom = max(kernels, key=lambda k: kernels[k]["ms"])
     log(f"kernels: {json.dumps(kernels)}")
-    osc = oracle_scene(scene)
-    R = cache.get("R")
-    qb, qwork = query_bytes(osc, R, cam, seed + args.warmup + args.steps - 1, args.warmup + args.steps, log)
-    kernels["k_query"].update({"alg_bytes": qb, "GB_s": qb / t_q / 1e6, "work": qwork})
+    # CPU leg (the C oracle on the host cores): the baseline timing, and the algorithmic bytes of
+    # the query pass counted on the same paths (nodes / triangles / probes / resolved reads)
+    cpu, qb = None, None
+    if not args.no_cpu:
+        try:
+            osc = oracle_scene(scene)
+            qb, qwork = query_bytes(osc, cache.get("R"), cam, seed + args.warmup + args.steps - 1, args.warmup + args.steps, log)
+            kernels["k_query"].update({"alg_bytes": qb, "GB_s": qb / t_q / 1e6, "work": qwork})
+            cpu = cpu_baseline(osc, scene, args.cpu_seconds, log)
+        except Exception as e:  # the baseline is reported, not required
+            log(f"cpu leg failed: {e!r}")
     if dom == "k_update":
         ach = upd_bytes / t_up / 1e6
-    elif dom == "k_query":
+    elif dom == "k_query" and qb:
         ach = qb / t_q / 1e6
     else:
         ach = None
@@ -292,12 +397,6 @@ def main():
             "traffic": traffic, "traffic_source": "profiles/pmc_street960x540.json" if traffic else None, "kernel": dom,
             "basis": ("k_query: 32 B/binary node + 36 B/triangle + 8 B/hash probe + 4 B/resolved read per path, oracle-counted"
                       if dom == "k_query" else "k_update: 12 B + 12 B x lights per entry")}
-    cpu = None
-    if not args.no_cpu:
-        try:
-            cpu = cpu_baseline(osc, scene, args.cpu_seconds, log)
-        except Exception as e:  # the baseline is reported, not required
-            log(f"cpu baseline failed: {e!r}")
     qual = None
     if not args.no_quality:
         try:
