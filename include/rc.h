@@ -195,6 +195,41 @@ enum { RC_ADJ_G = 0, RC_ADJ_SG, RC_ADJ_SR, RC_ADJ_LB, RC_ADJ_DA, RC_ADJ_DE, RC_A
 size_t rc_adjoint_array_bytes(const rc_adjoint* a, int which);
 rc_status rc_adjoint_get(const rc_adjoint* a, int which, void* host_dst, size_t bytes);
 
+/* ---- hash-encoded MLP radiance cache (extension) ----
+ * Input: 16-level multiresolution hash encoding of the position (2 features per level,
+ * 2^log2_table entries per level, resolutions base_res..max_res in geometric steps), normal and
+ * view direction; 5 hidden layers of 64 ReLU units; scalar output. BF16 MFMA with FP32
+ * accumulation; Adam on FP32 master weights and table. No counterpart in the reference: its
+ * learned-cache extension (callers: online training from the cache's scene, the camera pass). */
+typedef struct rc_mlp rc_mlp;
+typedef struct rc_mlp_config {
+    int log2_table;        /* hash-table entries per level (19) */
+    int base_res, max_res; /* grid resolution of the coarsest / finest level (16, 2048) */
+    int max_batch;         /* largest batch of one call (65536) */
+    float lr, beta1, beta2, eps;   /* Adam (1e-2, 0.9, 0.99, 1e-15) */
+    uint32_t seed;         /* initialisation of weights and table */
+    float lo[3], ext[3];   /* position bounds */
+} rc_mlp_config;
+rc_status rc_mlp_create(const rc_mlp_config* cfg, rc_mlp** out);
+void rc_mlp_destroy(rc_mlp* m);
+/* d_y[n] from float4 positions / normals / view directions (xyz used; device arrays) */
+rc_status rc_mlp_infer(rc_mlp* m, const float* d_P4, const float* d_N4, const float* d_D4, int n, float* d_y, void* stream);
+/* gradients of sum_i dy_i y_i into the weight gradient (overwritten) and the table gradient (accumulated
+ * until the next rc_mlp_step / rc_mlp_zero_grad) */
+rc_status rc_mlp_backward(rc_mlp* m, const float* d_P4, const float* d_N4, const float* d_D4, const float* d_dy, int n, void* stream);
+/* one training step on targets d_t: loss mean (y - t)^2 / (y^2 + 0.01), gradients, Adam; loss_out (host, optional;
+ * when set the call waits for the step) */
+rc_status rc_mlp_train(rc_mlp* m, const float* d_P4, const float* d_N4, const float* d_D4, const float* d_t, int n, float* loss_out,
+                       void* stream);
+/* Adam step on the current gradients (then cleared) */
+rc_status rc_mlp_step(rc_mlp* m, void* stream);
+rc_status rc_mlp_zero_grad(rc_mlp* m, void* stream);
+/* parameters / gradients (host copies): W master weights [5][64][64] + [64][16] f32, TABLE [16][2^log2_table][2] f32 */
+enum { RC_MLP_W = 0, RC_MLP_TABLE, RC_MLP_GW, RC_MLP_GTABLE, RC_MLP_COUNT };
+size_t rc_mlp_array_bytes(const rc_mlp* m, int which);
+rc_status rc_mlp_get(const rc_mlp* m, int which, void* host_dst, size_t bytes);
+rc_status rc_mlp_set(rc_mlp* m, int which, const void* host_src, size_t bytes);
+
 #ifdef __cplusplus
 }
 #endif

@@ -94,6 +94,16 @@ _SIGS = {
     "rc_adjoint_backward": (C.c_int, [_P, _P, _P, _P, _P]),
     "rc_adjoint_array_bytes": (C.c_size_t, [_P, C.c_int]),
     "rc_adjoint_get": (C.c_int, [_P, C.c_int, _P, C.c_size_t]),
+    "rc_mlp_create": (C.c_int, [_P, C.POINTER(_P)]),
+    "rc_mlp_destroy": (None, [_P]),
+    "rc_mlp_infer": (C.c_int, [_P, _P, _P, _P, C.c_int, _P, _P]),
+    "rc_mlp_backward": (C.c_int, [_P, _P, _P, _P, _P, C.c_int, _P]),
+    "rc_mlp_train": (C.c_int, [_P, _P, _P, _P, _P, C.c_int, C.POINTER(C.c_float), _P]),
+    "rc_mlp_step": (C.c_int, [_P, _P]),
+    "rc_mlp_zero_grad": (C.c_int, [_P, _P]),
+    "rc_mlp_array_bytes": (C.c_size_t, [_P, C.c_int]),
+    "rc_mlp_get": (C.c_int, [_P, C.c_int, _P, C.c_size_t]),
+    "rc_mlp_set": (C.c_int, [_P, C.c_int, _P, C.c_size_t]),
     "rc_trace": (C.c_int, [_P, C.c_int, _P, _P, C.c_int, _P, _P, C.c_int, _P]),
     "rc_hash_insert": (C.c_int, [_P, _P, C.c_uint32, _P, _P, C.c_int, _P, _P]),
     "rc_hash_query": (C.c_int, [_P, _P, C.c_uint32, _P, C.c_int, _P, _P]),

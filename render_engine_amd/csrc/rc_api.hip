@@ -958,3 +958,168 @@ static rc_status cache_grad_masks(rc_cache* c) {
     }
     return RC_OK;
 }
+
+// ------------------------------------------------------------------------------------ MLP cache
+struct rc_mlp {
+    rc_mlp_config cfg{};
+    rc_mlpdev d{};
+    rc_mlpbuf b{};
+    int step = 0;
+    std::vector<void*> owned;
+    ~rc_mlp() { for (void* p : owned) (void)hipFree(p); }
+};
+
+extern "C" rc_status rc_mlp_create(const rc_mlp_config* c, rc_mlp** out) {
+    if (!c || !out) return fail(RC_ERR_ARG, "NULL argument");
+    if (c->log2_table < 10 || c->log2_table > 24 || c->base_res < 2 || c->max_res < c->base_res || c->max_res > (1 << 20) ||
+        c->max_batch < 1 || c->max_batch > (1 << 24) || !(c->lr > 0.f) || !(c->ext[0] > 0.f) || !(c->ext[1] > 0.f) || !(c->ext[2] > 0.f))
+        return fail(RC_ERR_ARG, "rc_mlp_create: bad configuration");
+    rc_mlp* m = new rc_mlp();
+    m->cfg = *c;
+    rc_mlpdev& d = m->d;
+    d.log2T = c->log2_table;
+    const size_t T = (size_t)1 << c->log2_table, nt = (size_t)RC_MLP_LEVELS * 2 * T;
+    const double gs = std::exp((std::log((double)c->max_res) - std::log((double)c->base_res)) / (RC_MLP_LEVELS - 1));
+    d.dense = 0;
+    for (int l = 0; l < RC_MLP_LEVELS; ++l) {
+        d.res[l] = (int)std::floor(c->base_res * std::pow(gs, l) + 1e-9);
+        const double r1 = d.res[l] + 1.0;
+        if (r1 * r1 * r1 <= (double)T) d.dense |= 1u << l;
+    }
+    for (int a = 0; a < 3; ++a) { d.lo[a] = c->lo[a]; d.iext[a] = 1.f / c->ext[a]; }
+    rc_mlpbuf& b = m->b;
+    b.cap = rc_mlp_pad(c->max_batch);
+    const size_t cap = b.cap, nsplit = (cap + 1023) / 1024;
+#define MK(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) { delete m; return fail(RC_ERR_HIP, "%s: %s", #x, hipGetErrorString(e_)); } } while (0)
+    MK(dalloc(&d.table, nt, m->owned)); MK(dalloc(&d.gtable, nt, m->owned));
+    MK(dalloc(&d.Wm, (size_t)RC_MLP_NW, m->owned)); MK(dalloc(&d.gW, (size_t)RC_MLP_NW, m->owned));
+    uint16_t *wt = nullptr, *wk = nullptr;
+    MK(dalloc(&wt, (size_t)RC_MLP_NW, m->owned)); MK(dalloc(&wk, (size_t)RC_MLP_NW, m->owned));
+    d.Wt = wt; d.Wk = wk;
+    uint16_t *act = nullptr, *gz = nullptr;
+    MK(dalloc(&act, (size_t)(RC_MLP_HIDDEN + 1) * 64 * cap, m->owned)); MK(dalloc(&gz, (size_t)RC_MLP_HIDDEN * 64 * cap, m->owned));
+    b.actT = act; b.gzT = gz;
+    MK(dalloc(&b.y, cap, m->owned)); MK(dalloc(&b.dy, cap, m->owned)); MK(dalloc(&b.gX, cap * 2 * RC_MLP_LEVELS, m->owned));
+    MK(dalloc(&b.part, (size_t)(RC_MLP_HIDDEN + 1) * nsplit * 64 * 64, m->owned));
+    MK(dalloc(&b.mW, (size_t)RC_MLP_NW, m->owned)); MK(dalloc(&b.vW, (size_t)RC_MLP_NW, m->owned));
+    MK(dalloc(&b.mT, nt, m->owned)); MK(dalloc(&b.vT, nt, m->owned)); MK(dalloc(&b.loss, 1, m->owned));
+    // initialisation: weights uniform +-sqrt(6 / 64) (output layer x 0.1), table uniform +-1e-4
+    std::vector<float> W(RC_MLP_NW), tb(nt);
+    uint64_t r = (uint64_t)c->seed * 0x9E3779B97F4A7C15ull + 1;
+    auto u = [&]() { return (float)(rc::mix64(r += 0x9E3779B97F4A7C15ull) >> 40) * (1.f / 16777216.f); };
+    const float bw = std::sqrt(6.f / 64.f);
+    for (int e = 0; e < RC_MLP_NW; ++e) {
+        const bool out_layer = e >= RC_MLP_HIDDEN * 64 * 64;
+        const bool used = !out_layer || ((e - RC_MLP_HIDDEN * 64 * 64) % 16) == 0;
+        W[e] = used ? (2.f * u() - 1.f) * bw * (out_layer ? 0.1f : 1.f) : 0.f;
+    }
+    for (size_t e = 0; e < nt; ++e) tb[e] = (2.f * u() - 1.f) * 1e-4f;
+    MK(hipMemcpy(d.Wm, W.data(), W.size() * 4, hipMemcpyHostToDevice));
+    MK(hipMemcpy(d.table, tb.data(), tb.size() * 4, hipMemcpyHostToDevice));
+    for (void* p : {(void*)d.gtable, (void*)d.gW, (void*)b.mW, (void*)b.vW, (void*)b.mT, (void*)b.vT})
+        MK(hipMemset(p, 0, (p == d.gW || p == b.mW || p == b.vW) ? (size_t)RC_MLP_NW * 4 : nt * 4));
+    MK(rc_launch_mlp_pack(&d, nullptr));
+    MK(hipDeviceSynchronize());
+#undef MK
+    *out = m;
+    return RC_OK;
+}
+
+extern "C" void rc_mlp_destroy(rc_mlp* m) { delete m; }
+
+static rc_status mlp_check(rc_mlp* m, const float* P, const float* N, const float* D, int n) {
+    if (!m) return fail(RC_ERR_ARG, "NULL mlp");
+    if (n < 0 || n > m->cfg.max_batch) return fail(RC_ERR_ARG, "batch of %d outside 0..%d", n, m->cfg.max_batch);
+    if (n > 0 && (!P || !N || !D)) return fail(RC_ERR_ARG, "NULL input array");
+    return RC_OK;
+}
+
+extern "C" rc_status rc_mlp_infer(rc_mlp* m, const float* d_P4, const float* d_N4, const float* d_D4, int n, float* d_y, void* stream) {
+    rc_status r = mlp_check(m, d_P4, d_N4, d_D4, n);
+    if (r != RC_OK) return r;
+    if (n > 0 && !d_y) return fail(RC_ERR_ARG, "NULL output");
+    HIPCK(rc_launch_mlp_infer(&m->d, (const float4*)d_P4, (const float4*)d_N4, (const float4*)d_D4, n, d_y, (hipStream_t)stream));
+    return RC_OK;
+}
+
+extern "C" rc_status rc_mlp_backward(rc_mlp* m, const float* d_P4, const float* d_N4, const float* d_D4, const float* d_dy, int n,
+                                     void* stream) {
+    rc_status r = mlp_check(m, d_P4, d_N4, d_D4, n);
+    if (r != RC_OK) return r;
+    if (n > 0 && !d_dy) return fail(RC_ERR_ARG, "NULL dy");
+    hipStream_t st = (hipStream_t)stream;
+    // the kernels read dy over the padded batch: stage it with zero padding
+    HIPCK(rc_launch_mlp_zero(m->b.dy, m->b.cap, st));
+    if (n > 0) HIPCK(hipMemcpyAsync(m->b.dy, d_dy, (size_t)n * 4, hipMemcpyDeviceToDevice, st));
+    HIPCK(rc_launch_mlp_grad(&m->d, &m->b, (const float4*)d_P4, (const float4*)d_N4, (const float4*)d_D4, nullptr, m->b.dy, n, nullptr, st));
+    return RC_OK;
+}
+
+extern "C" rc_status rc_mlp_step(rc_mlp* m, void* stream) {
+    if (!m) return fail(RC_ERR_ARG, "NULL mlp");
+    ++m->step;
+    HIPCK(rc_launch_mlp_adam(&m->d, &m->b, m->cfg.lr, m->cfg.beta1, m->cfg.beta2, m->cfg.eps, m->step, (hipStream_t)stream));
+    return RC_OK;
+}
+
+extern "C" rc_status rc_mlp_zero_grad(rc_mlp* m, void* stream) {
+    if (!m) return fail(RC_ERR_ARG, "NULL mlp");
+    const long long nt = (long long)RC_MLP_LEVELS * ((long long)2 << m->d.log2T);
+    HIPCK(rc_launch_mlp_zero(m->d.gtable, nt, (hipStream_t)stream));
+    HIPCK(rc_launch_mlp_zero(m->d.gW, RC_MLP_NW, (hipStream_t)stream));
+    return RC_OK;
+}
+
+extern "C" rc_status rc_mlp_train(rc_mlp* m, const float* d_P4, const float* d_N4, const float* d_D4, const float* d_t, int n,
+                                  float* loss_out, void* stream) {
+    rc_status r = mlp_check(m, d_P4, d_N4, d_D4, n);
+    if (r != RC_OK) return r;
+    if (n < 1 || !d_t) return fail(RC_ERR_ARG, "rc_mlp_train needs targets and n >= 1");
+    hipStream_t st = (hipStream_t)stream;
+    HIPCK(rc_launch_mlp_zero(m->b.loss, 1, st));
+    HIPCK(rc_launch_mlp_grad(&m->d, &m->b, (const float4*)d_P4, (const float4*)d_N4, (const float4*)d_D4, d_t, nullptr, n, m->b.loss, st));
+    ++m->step;
+    HIPCK(rc_launch_mlp_adam(&m->d, &m->b, m->cfg.lr, m->cfg.beta1, m->cfg.beta2, m->cfg.eps, m->step, st));
+    if (loss_out) {
+        HIPCK(hipMemcpyAsync(loss_out, m->b.loss, 4, hipMemcpyDeviceToHost, st));
+        HIPCK(hipStreamSynchronize(st));
+    }
+    return RC_OK;
+}
+
+static bool mlp_array(const rc_mlp* m, int which, void** p, size_t* bytes) {
+    const size_t nt = (size_t)RC_MLP_LEVELS * 2 * ((size_t)1 << m->d.log2T);
+    switch (which) {
+        case RC_MLP_W: *p = m->d.Wm; *bytes = (size_t)RC_MLP_NW * 4; return true;
+        case RC_MLP_TABLE: *p = m->d.table; *bytes = nt * 4; return true;
+        case RC_MLP_GW: *p = m->d.gW; *bytes = (size_t)RC_MLP_NW * 4; return true;
+        case RC_MLP_GTABLE: *p = m->d.gtable; *bytes = nt * 4; return true;
+        default: return false;
+    }
+}
+
+extern "C" size_t rc_mlp_array_bytes(const rc_mlp* m, int which) {
+    void* p; size_t b = 0;
+    if (!m || !mlp_array(m, which, &p, &b)) return 0;
+    return b;
+}
+
+extern "C" rc_status rc_mlp_get(const rc_mlp* m, int which, void* host_dst, size_t bytes) {
+    void* p; size_t b = 0;
+    if (!m || !host_dst || !mlp_array(m, which, &p, &b)) return fail(RC_ERR_ARG, "rc_mlp_get: bad arguments");
+    if (bytes != b) return fail(RC_ERR_ARG, "rc_mlp_get: %zu bytes given, %zu expected", bytes, b);
+    HIPCK(hipDeviceSynchronize());
+    HIPCK(hipMemcpy(host_dst, p, b, hipMemcpyDeviceToHost));
+    return RC_OK;
+}
+
+extern "C" rc_status rc_mlp_set(rc_mlp* m, int which, const void* host_src, size_t bytes) {
+    void* p; size_t b = 0;
+    if (!m || !host_src || !mlp_array(m, which, &p, &b)) return fail(RC_ERR_ARG, "rc_mlp_set: bad arguments");
+    if (bytes != b) return fail(RC_ERR_ARG, "rc_mlp_set: %zu bytes given, %zu expected", bytes, b);
+    HIPCK(hipDeviceSynchronize());
+    HIPCK(hipMemcpy(p, host_src, b, hipMemcpyHostToDevice));
+    if (which == RC_MLP_W) HIPCK(rc_launch_mlp_pack(&m->d, nullptr));
+    HIPCK(hipDeviceSynchronize());
+    return RC_OK;
+}

@@ -45,3 +45,10 @@ hipError_t rc_launch_adj_forward(int trav, const rc_dscene* s, const rc_dstate* 
                                  const float* alb, int K, int B, uint32_t seed, float* R, hipStream_t stream);
 hipError_t rc_launch_adj_backward(int trav, const rc_dscene* s, const rc_dstate* st, const rc_adjbuf* w, const float* gR, int K,
                                   int B, uint32_t seed, float* dI, float* dalb, hipStream_t stream);
+int rc_mlp_pad(int n);
+hipError_t rc_launch_mlp_infer(const rc_mlpdev* m, const float4* P, const float4* N, const float4* D, int n, float* y, hipStream_t st);
+hipError_t rc_launch_mlp_grad(const rc_mlpdev* m, const rc_mlpbuf* b, const float4* P, const float4* N, const float4* D, const float* t,
+                              const float* dy_in, int n, float* loss, hipStream_t st);
+hipError_t rc_launch_mlp_adam(const rc_mlpdev* m, const rc_mlpbuf* b, float lr, float b1, float b2, float eps, int step, hipStream_t st);
+hipError_t rc_launch_mlp_pack(const rc_mlpdev* m, hipStream_t st);
+hipError_t rc_launch_mlp_zero(float* x, long long n, hipStream_t st);

@@ -1,0 +1,359 @@
+// Hash-encoded MLP radiance cache (extension) for gfx950.
+//
+// Input encoding: 16-level multiresolution hash grid over the position normalised by the scene
+// bounds (2 features per level, 2^log2T entries per level; levels whose (res+1)^3 grid fits the
+// table are indexed densely, finer ones by the xor of the corner coordinates times 1,
+// 2654435761, 805459861), trilinear interpolation; then the normal and the view direction
+// mapped to [0, 1], a constant 1, zero padding: 64 inputs. Network: 5 hidden layers of 64 ReLU
+// units, scalar output, no biases. Weights and activations are BF16 MFMA operands
+// (v_mfma_f32_16x16x32_bf16, FP32 accumulation); Adam keeps FP32 master weights and table.
+//
+// MFMA lane layout (16x16x32): lane l holds A[l % 16][8 (l / 16) + j] and B[8 (l / 16) + j][l % 16]
+// (j = 0..7), and D[4 (l / 16) + i][l % 16] (i = 0..3). Every product below is D = P Q^T with P and
+// Q stored K-contiguous, so both operands are 16-byte loads.
+#include "rc_device.h"
+#include "rc_kernels.h"
+#include "rc_trace.h"
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+#define MW 64                   // layer width
+#define MLD 72                  // LDS row stride in bf16 (144-byte rows spread 16-byte reads over the banks)
+#define MROWS 64                // samples per wave
+#define MWAVES 4                // waves per block
+
+__device__ __forceinline__ __bf16 rc_bf(float x) { return (__bf16)x; }
+__device__ __forceinline__ float rc_fb(__bf16 x) { return (float)x; }
+
+__device__ __forceinline__ f32x4 rc_mfma(bf16x8 a, bf16x8 b, f32x4 c) { return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0); }
+// 16 rows x 32 K of a K-contiguous bf16 matrix (row stride ld) as this lane's operand
+__device__ __forceinline__ bf16x8 rc_opnd(const __bf16* M, int ld, int r0, int k0) {
+    const int l = threadIdx.x & 63;
+    return *(const bf16x8*)(M + (int64_t)(r0 + (l & 15)) * ld + k0 + 8 * (l >> 4));
+}
+
+// ------------------------------------------------------------------------------ encoding
+__device__ __forceinline__ uint32_t rc_mlp_index(const rc_mlpdev& m, int lv, uint32_t x, uint32_t y, uint32_t z) {
+    const uint32_t r1 = (uint32_t)m.res[lv] + 1u;
+    if ((m.dense >> lv) & 1u) return x + r1 * (y + r1 * z);
+    return (x ^ (y * 2654435761u) ^ (z * 805459861u)) & ((1u << m.log2T) - 1u);
+}
+
+// corner grid of level lv for the normalised position x: base corner and fractions
+__device__ __forceinline__ void rc_mlp_cell(const rc_mlpdev& m, int lv, f3 x, uint32_t& ix, uint32_t& iy, uint32_t& iz, f3& fr) {
+    const float r = (float)m.res[lv];
+    const float sx = x.x * r, sy = x.y * r, sz = x.z * r;
+    const float fx = fminf(floorf(sx), r - 1.f), fy = fminf(floorf(sy), r - 1.f), fz = fminf(floorf(sz), r - 1.f);
+    ix = (uint32_t)fx; iy = (uint32_t)fy; iz = (uint32_t)fz;
+    fr = mk3(sx - fx, sy - fy, sz - fz);
+}
+
+__device__ __forceinline__ f3 rc_mlp_norm(const rc_mlpdev& m, float4 p) {
+    return mk3(fminf(fmaxf((p.x - m.lo[0]) * m.iext[0], 0.f), 1.f), fminf(fmaxf((p.y - m.lo[1]) * m.iext[1], 0.f), 1.f),
+               fminf(fmaxf((p.z - m.lo[2]) * m.iext[2], 0.f), 1.f));
+}
+
+// the 64 inputs of one sample into row[0..63]
+__device__ void rc_mlp_encode(const rc_mlpdev& m, float4 p, float4 n, float4 v, __bf16* row) {
+    const f3 x = rc_mlp_norm(m, p);
+    const size_t T = (size_t)1 << m.log2T;
+    for (int lv = 0; lv < RC_MLP_LEVELS; ++lv) {
+        uint32_t ix, iy, iz; f3 fr;
+        rc_mlp_cell(m, lv, x, ix, iy, iz, fr);
+        float f0 = 0.f, f1 = 0.f;
+        for (int c = 0; c < 8; ++c) {
+            const uint32_t cx = ix + (c & 1), cy = iy + ((c >> 1) & 1), cz = iz + (c >> 2);
+            const float w = ((c & 1) ? fr.x : 1.f - fr.x) * (((c >> 1) & 1) ? fr.y : 1.f - fr.y) * ((c >> 2) ? fr.z : 1.f - fr.z);
+            const float2 t = *(const float2*)(m.table + 2 * ((size_t)lv * T + rc_mlp_index(m, lv, cx, cy, cz)));
+            f0 += w * t.x;
+            f1 += w * t.y;
+        }
+        row[2 * lv] = rc_bf(f0);
+        row[2 * lv + 1] = rc_bf(f1);
+    }
+    row[32] = rc_bf(0.5f * (n.x + 1.f)); row[33] = rc_bf(0.5f * (n.y + 1.f)); row[34] = rc_bf(0.5f * (n.z + 1.f));
+    row[35] = rc_bf(0.5f * (v.x + 1.f)); row[36] = rc_bf(0.5f * (v.y + 1.f)); row[37] = rc_bf(0.5f * (v.z + 1.f));
+    row[38] = rc_bf(1.f);
+    for (int k = 39; k < MW; ++k) row[k] = rc_bf(0.f);
+}
+
+// ------------------------------------------------------------------------------ forward
+// One wave = 64 samples; activations in the wave's LDS tile, updated in place layer by layer
+// (a row tile is read for all its products before it is overwritten). TRAIN also stores every
+// layer's activations transposed, actT[layer][feature][sample] (row stride ldT), for the backward.
+template <bool TRAIN>
+__global__ void __launch_bounds__(MROWS * MWAVES) k_mlp_fwd(rc_mlpdev m, const float4* P, const float4* N, const float4* D, int n,
+                                                            float* y, __bf16* actT, int ldT) {
+    __shared__ __align__(16) __bf16 s_act[MWAVES][MROWS * MLD];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, g = lane >> 4, c16 = lane & 15;
+    const int base = (blockIdx.x * MWAVES + wv) * MROWS;
+    __bf16* act = s_act[wv];
+    {
+        const int s = base + lane;
+        __bf16* row = act + lane * MLD;
+        if (s < n) rc_mlp_encode(m, P[s], N[s], D[s], row);
+        else for (int k = 0; k < MW; ++k) row[k] = rc_bf(0.f);
+        if (TRAIN) for (int k = 0; k < MW; ++k) actT[(int64_t)k * ldT + base + lane] = row[k];
+    }
+    __syncthreads();
+    const __bf16* Wt = (const __bf16*)m.Wt;
+    for (int L = 0; L < RC_MLP_HIDDEN; ++L) {
+        const __bf16* W = Wt + (size_t)L * MW * MW;
+        for (int rt = 0; rt < MROWS / 16; ++rt) {
+            f32x4 acc[4];
+            for (int ct = 0; ct < 4; ++ct) acc[ct] = f32x4{0.f, 0.f, 0.f, 0.f};
+            for (int ks = 0; ks < 2; ++ks) {
+                const bf16x8 a = rc_opnd(act, MLD, rt * 16, ks * 32);
+                for (int ct = 0; ct < 4; ++ct) acc[ct] = rc_mfma(a, rc_opnd(W, MW, ct * 16, ks * 32), acc[ct]);
+            }
+            for (int ct = 0; ct < 4; ++ct) {
+                __bf16 h[4];
+                for (int i = 0; i < 4; ++i) {
+                    h[i] = rc_bf(fmaxf(acc[ct][i], 0.f));
+                    act[(rt * 16 + 4 * g + i) * MLD + ct * 16 + c16] = h[i];
+                }
+                if (TRAIN) {
+                    __bf16* dst = actT + ((int64_t)(L + 1) * MW + ct * 16 + c16) * ldT + base + rt * 16 + 4 * g;
+                    for (int i = 0; i < 4; ++i) dst[i] = h[i];
+                }
+            }
+        }
+        __syncthreads();
+    }
+    const __bf16* W5 = Wt + (size_t)RC_MLP_HIDDEN * MW * MW;   // [16][64], output column 0
+    for (int rt = 0; rt < MROWS / 16; ++rt) {
+        f32x4 acc = f32x4{0.f, 0.f, 0.f, 0.f};
+        for (int ks = 0; ks < 2; ++ks) acc = rc_mfma(rc_opnd(act, MLD, rt * 16, ks * 32), rc_opnd(W5, MW, 0, ks * 32), acc);
+        if (c16 == 0)
+            for (int i = 0; i < 4; ++i) {
+                const int s = base + rt * 16 + 4 * g + i;
+                if (s < n) y[s] = acc[i];
+            }
+    }
+}
+
+// relative L2 loss (y - t)^2 / (y^2 + 0.01) averaged over n (the denominator is held constant in
+// the gradient); dy over the padded batch (zero past n); loss sum accumulated in *loss
+__global__ void __launch_bounds__(RC_TB) k_mlp_loss(const float* y, const float* t, int n, int npad, float* dy, float* loss) {
+    __shared__ float s_w[RC_TB / 64];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    float l = 0.f;
+    if (i < npad) {
+        float d = 0.f;
+        if (i < n) {
+            const float yi = y[i], den = yi * yi + 0.01f, e = yi - t[i];
+            d = 2.f * e / den / (float)n;
+            l = e * e / den / (float)n;
+        }
+        dy[i] = d;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) l += __shfl_down(l, o, 64);
+    if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = l;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float t2 = 0.f;
+        for (int k = 0; k < RC_TB / 64; ++k) t2 += s_w[k];
+        unsafeAtomicAdd(loss, t2);
+    }
+}
+
+// ------------------------------------------------------------------------------ backward
+// Gradients w.r.t. the pre-activations of every layer, gzT[L][feature][sample] for z_{L+1}
+// (L = 0..4), and w.r.t. the 32 hash features of the input (gX[sample][32], FP32).
+//   g_z5 = dy W5[:, 0] (.) [a5 > 0];  g_a_L = g_z_{L+1} W_L^T;  g_z_L = g_a_L (.) [a_L > 0] (L >= 1);  gX = g_a_0
+__global__ void __launch_bounds__(MROWS * MWAVES) k_mlp_bwd(rc_mlpdev m, const float* dy, const __bf16* actT, int ldT, __bf16* gzT,
+                                                            float* gX) {
+    __shared__ __align__(16) __bf16 s_g[MWAVES][MROWS * MLD];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, g = lane >> 4, c16 = lane & 15;
+    const int base = (blockIdx.x * MWAVES + wv) * MROWS;
+    __bf16* gs = s_g[wv];
+    const __bf16* Wk = (const __bf16*)m.Wk;
+    const __bf16* W5 = Wk + (size_t)RC_MLP_HIDDEN * MW * MW;   // [64][16]
+    {
+        const int s = base + lane;
+        const float d = dy[s];
+        for (int k = 0; k < MW; ++k) {
+            const float a5 = rc_fb(actT[((int64_t)RC_MLP_HIDDEN * MW + k) * ldT + s]);
+            const __bf16 v = rc_bf(a5 > 0.f ? d * rc_fb(W5[k * 16]) : 0.f);
+            gs[lane * MLD + k] = v;
+            gzT[((int64_t)(RC_MLP_HIDDEN - 1) * MW + k) * ldT + s] = v;
+        }
+    }
+    __syncthreads();
+    for (int L = RC_MLP_HIDDEN - 1; L >= 0; --L) {
+        const __bf16* W = Wk + (size_t)L * MW * MW;   // W_L[k][j]
+        for (int rt = 0; rt < MROWS / 16; ++rt) {
+            f32x4 acc[4];
+            for (int ct = 0; ct < 4; ++ct) acc[ct] = f32x4{0.f, 0.f, 0.f, 0.f};
+            for (int ks = 0; ks < 2; ++ks) {
+                const bf16x8 a = rc_opnd(gs, MLD, rt * 16, ks * 32);
+                for (int ct = 0; ct < 4; ++ct) acc[ct] = rc_mfma(a, rc_opnd(W, MW, ct * 16, ks * 32), acc[ct]);
+            }
+            const int s0 = base + rt * 16 + 4 * g;
+            for (int ct = 0; ct < 4; ++ct) {
+                const int k = ct * 16 + c16;
+                if (L > 0) {
+                    const __bf16* ap = actT + ((int64_t)L * MW + k) * ldT + s0;
+                    __bf16* dst = gzT + ((int64_t)(L - 1) * MW + k) * ldT + s0;
+                    for (int i = 0; i < 4; ++i) {
+                        const __bf16 v = rc_bf(rc_fb(ap[i]) > 0.f ? acc[ct][i] : 0.f);
+                        gs[(rt * 16 + 4 * g + i) * MLD + k] = v;
+                        dst[i] = v;
+                    }
+                } else if (k < 2 * RC_MLP_LEVELS) {
+                    for (int i = 0; i < 4; ++i) gX[(int64_t)(s0 + i) * (2 * RC_MLP_LEVELS) + k] = acc[ct][i];
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Weight gradients, split over the batch: dW_L[k][j] = sum_b a_L[b][k] g_z_{L+1}[b][j] for L = 0..4
+// (P = actT[L], Q = gzT[L], both sample-contiguous), dW_5[k][0] = sum_b a_5[b][k] dy[b].
+// Block (split, L): wave w computes rows k = 16w..16w+15; partial sums part[L][split][k][j].
+__global__ void __launch_bounds__(MROWS * MWAVES) k_mlp_dw(const __bf16* actT, const __bf16* gzT, const float* dy, int ldT, int chunk,
+                                                           float* part) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, g = lane >> 4, c16 = lane & 15;
+    const int L = blockIdx.y, split = blockIdx.x, nsplit = gridDim.x;
+    const int b0 = split * chunk, b1 = min(ldT, b0 + chunk);
+    float* out = part + ((size_t)L * nsplit + split) * MW * MW;
+    if (L < RC_MLP_HIDDEN) {
+        const __bf16* Pm = actT + (size_t)L * MW * ldT;
+        const __bf16* Qm = gzT + (size_t)L * MW * ldT;
+        f32x4 acc[4];
+        for (int ct = 0; ct < 4; ++ct) acc[ct] = f32x4{0.f, 0.f, 0.f, 0.f};
+        for (int bs = b0; bs < b1; bs += 32) {
+            const bf16x8 a = rc_opnd(Pm, ldT, wv * 16, bs);
+            for (int ct = 0; ct < 4; ++ct) acc[ct] = rc_mfma(a, rc_opnd(Qm, ldT, ct * 16, bs), acc[ct]);
+        }
+        for (int ct = 0; ct < 4; ++ct)
+            for (int i = 0; i < 4; ++i) out[(wv * 16 + 4 * g + i) * MW + ct * 16 + c16] = acc[ct][i];
+    } else {
+        const int k = threadIdx.x;
+        if (k < MW) {
+            const __bf16* a5 = actT + ((size_t)RC_MLP_HIDDEN * MW + k) * ldT;
+            float s = 0.f;
+            for (int b = b0; b < b1; ++b) s += rc_fb(a5[b]) * dy[b];
+            for (int j = 0; j < MW; ++j) out[k * MW + j] = j == 0 ? s : 0.f;
+        }
+    }
+}
+
+// gW (master layout: layers 0..4 [64][64], layer 5 [64][16]) = sum over splits, in split order
+__global__ void __launch_bounds__(RC_TB) k_mlp_dw_sum(const float* part, int nsplit, float* gW) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= RC_MLP_NW) return;
+    int L, k, j;
+    if (e < RC_MLP_HIDDEN * MW * MW) { L = e / (MW * MW); k = (e / MW) % MW; j = e % MW; }
+    else { L = RC_MLP_HIDDEN; k = (e - RC_MLP_HIDDEN * MW * MW) / 16; j = (e - RC_MLP_HIDDEN * MW * MW) % 16; }
+    float s = 0.f;
+    for (int sp = 0; sp < nsplit; ++sp) s += part[((size_t)L * nsplit + sp) * MW * MW + k * MW + j];
+    gW[e] = s;
+}
+
+// hash-table gradient: each sample's feature gradients spread over its 8 corners per level
+__global__ void __launch_bounds__(RC_TB) k_mlp_enc_bwd(rc_mlpdev m, const float4* P, int n, const float* gX) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n * RC_MLP_LEVELS) return;
+    const int s = t / RC_MLP_LEVELS, lv = t % RC_MLP_LEVELS;
+    const float g0 = gX[(int64_t)s * (2 * RC_MLP_LEVELS) + 2 * lv], g1 = gX[(int64_t)s * (2 * RC_MLP_LEVELS) + 2 * lv + 1];
+    if (g0 == 0.f && g1 == 0.f) return;
+    const f3 x = rc_mlp_norm(m, P[s]);
+    uint32_t ix, iy, iz; f3 fr;
+    rc_mlp_cell(m, lv, x, ix, iy, iz, fr);
+    const size_t T = (size_t)1 << m.log2T;
+    for (int c = 0; c < 8; ++c) {
+        const uint32_t cx = ix + (c & 1), cy = iy + ((c >> 1) & 1), cz = iz + (c >> 2);
+        const float w = ((c & 1) ? fr.x : 1.f - fr.x) * (((c >> 1) & 1) ? fr.y : 1.f - fr.y) * ((c >> 2) ? fr.z : 1.f - fr.z);
+        float* gt = m.gtable + 2 * ((size_t)lv * T + rc_mlp_index(m, lv, cx, cy, cz));
+        unsafeAtomicAdd(gt, w * g0);
+        unsafeAtomicAdd(gt + 1, w * g1);
+    }
+}
+
+// Adam (torch.optim.Adam's formula): m, v moments, bias corrections c1 = 1 - b1^t, c2 = 1 - b2^t;
+// the gradient is cleared for the next step
+__global__ void __launch_bounds__(RC_TB) k_mlp_adam(float* p, float* gr, float* mo, float* vo, long long n, float lr, float b1, float b2,
+                                                   float eps, float c1, float c2) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float g = gr[i];
+    const float mm = b1 * mo[i] + (1.f - b1) * g;
+    const float vv = b2 * vo[i] + (1.f - b2) * g * g;
+    mo[i] = mm; vo[i] = vv;
+    p[i] -= (lr / c1) * mm / (sqrtf(vv) / sqrtf(c2) + eps);
+    gr[i] = 0.f;
+}
+
+// BF16 operand copies of the master weights: Wt[L][j][k] (forward), Wk[L][k][j] (backward)
+__global__ void __launch_bounds__(RC_TB) k_mlp_pack(const float* Wm, __bf16* Wt, __bf16* Wk) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= RC_MLP_NW) return;
+    const __bf16 v = rc_bf(Wm[e]);
+    Wk[e] = v;
+    if (e < RC_MLP_HIDDEN * MW * MW) {
+        const int L = e / (MW * MW), k = (e / MW) % MW, j = e % MW;
+        Wt[(size_t)L * MW * MW + j * MW + k] = v;
+    } else {
+        const int r = e - RC_MLP_HIDDEN * MW * MW, k = r / 16, j = r % 16;
+        Wt[(size_t)RC_MLP_HIDDEN * MW * MW + j * MW + k] = v;
+    }
+}
+
+__global__ void __launch_bounds__(RC_TB) k_mlp_zero(float* x, long long n) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = 0.f;
+}
+
+// ------------------------------------------------------------------------------ launchers
+static inline int rc_mlp_blocks(int n) { return (n + MROWS * MWAVES - 1) / (MROWS * MWAVES); }
+int rc_mlp_pad(int n) { return rc_mlp_blocks(n) * MROWS * MWAVES; }
+
+hipError_t rc_launch_mlp_infer(const rc_mlpdev* m, const float4* P, const float4* N, const float4* D, int n, float* y, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_mlp_fwd<false>, dim3(rc_mlp_blocks(n)), dim3(MROWS * MWAVES), 0, st, *m, P, N, D, n, y, (__bf16*)nullptr, 0);
+    return hipGetLastError();
+}
+
+// forward (train) + backward: gradients of sum_i dy_i y_i added to m->gW (overwritten) and m->gtable (accumulated)
+hipError_t rc_launch_mlp_grad(const rc_mlpdev* m, const rc_mlpbuf* b, const float4* P, const float4* N, const float4* D, const float* t,
+                              const float* dy_in, int n, float* loss, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    const int npad = rc_mlp_pad(n), nb = rc_mlp_blocks(n);
+    __bf16* actT = (__bf16*)b->actT;
+    __bf16* gzT = (__bf16*)b->gzT;
+    hipLaunchKernelGGL(k_mlp_fwd<true>, dim3(nb), dim3(MROWS * MWAVES), 0, st, *m, P, N, D, n, b->y, actT, npad);
+    if (t) hipLaunchKernelGGL(k_mlp_loss, dim3(rc_grid(npad)), dim3(RC_TB), 0, st, (const float*)b->y, t, n, npad, b->dy, loss);
+    const float* dy = t ? (const float*)b->dy : dy_in;
+    hipLaunchKernelGGL(k_mlp_bwd, dim3(nb), dim3(MROWS * MWAVES), 0, st, *m, dy, (const __bf16*)actT, npad, gzT, b->gX);
+    const int chunk = 1024, nsplit = (npad + chunk - 1) / chunk;
+    hipLaunchKernelGGL(k_mlp_dw, dim3(nsplit, RC_MLP_HIDDEN + 1), dim3(MROWS * MWAVES), 0, st, (const __bf16*)actT, (const __bf16*)gzT, dy,
+                       npad, chunk, b->part);
+    hipLaunchKernelGGL(k_mlp_dw_sum, dim3(rc_grid(RC_MLP_NW)), dim3(RC_TB), 0, st, (const float*)b->part, nsplit, m->gW);
+    hipLaunchKernelGGL(k_mlp_enc_bwd, dim3(rc_grid((long long)n * RC_MLP_LEVELS)), dim3(RC_TB), 0, st, *m, P, n, (const float*)b->gX);
+    return hipGetLastError();
+}
+
+hipError_t rc_launch_mlp_adam(const rc_mlpdev* m, const rc_mlpbuf* b, float lr, float b1, float b2, float eps, int step, hipStream_t st) {
+    const float c1 = 1.f - powf(b1, (float)step), c2 = 1.f - powf(b2, (float)step);
+    const long long nt = (long long)RC_MLP_LEVELS * ((long long)2 << m->log2T);
+    hipLaunchKernelGGL(k_mlp_adam, dim3(rc_grid(RC_MLP_NW)), dim3(RC_TB), 0, st, m->Wm, m->gW, b->mW, b->vW, (long long)RC_MLP_NW, lr, b1, b2,
+                       eps, c1, c2);
+    hipLaunchKernelGGL(k_mlp_adam, dim3(rc_grid(nt)), dim3(RC_TB), 0, st, m->table, m->gtable, b->mT, b->vT, nt, lr, b1, b2, eps, c1, c2);
+    hipLaunchKernelGGL(k_mlp_pack, dim3(rc_grid(RC_MLP_NW)), dim3(RC_TB), 0, st, (const float*)m->Wm, (__bf16*)m->Wt, (__bf16*)m->Wk);
+    return hipGetLastError();
+}
+
+hipError_t rc_launch_mlp_pack(const rc_mlpdev* m, hipStream_t st) {
+    hipLaunchKernelGGL(k_mlp_pack, dim3(rc_grid(RC_MLP_NW)), dim3(RC_TB), 0, st, (const float*)m->Wm, (__bf16*)m->Wt, (__bf16*)m->Wk);
+    return hipGetLastError();
+}
+
+hipError_t rc_launch_mlp_zero(float* x, long long n, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_mlp_zero, dim3(rc_grid(n)), dim3(RC_TB), 0, st, x, n);
+    return hipGetLastError();
+}

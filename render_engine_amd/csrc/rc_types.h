@@ -153,3 +153,23 @@ typedef struct rc_adjbuf {
     int32_t* perm; int32_t* moff;  // entries sorted by material, offsets [nmat + 1]
     int32_t nmat, pad0;
 } rc_adjbuf;
+
+// Hash-encoded MLP cache: 16 levels x 2 features, 5 hidden layers of 64, scalar output.
+#define RC_MLP_LEVELS 16
+#define RC_MLP_HIDDEN 5
+#define RC_MLP_NW (RC_MLP_HIDDEN * 64 * 64 + 64 * 16)   // master weights: [5][64][64] + [64][16]
+typedef struct rc_mlpdev {
+    float* table; float* gtable;   // [16][T][2] hash-grid features and their gradient
+    float* Wm; float* gW;          // FP32 master weights and their gradient
+    void* Wt; void* Wk;            // BF16 operand copies: Wt[L][out][in], Wk[L][in][out]
+    int32_t log2T; uint32_t dense; // table size 2^log2T per level; bit lv: level lv indexed densely
+    int32_t res[RC_MLP_LEVELS];    // grid resolution per level
+    float lo[3], iext[3];          // position bounds (lo, 1 / extent)
+} rc_mlpdev;
+typedef struct rc_mlpbuf {
+    void* actT; void* gzT;         // BF16 [6][64][cap] activations, [5][64][cap] pre-activation gradients
+    float* y; float* dy; float* gX; float* part;   // outputs, output gradients, input-feature gradients [cap][32], dW partials
+    float* mW; float* vW; float* mT; float* vT;    // Adam moments (weights, table)
+    float* loss;                   // [1]
+    int32_t cap, pad0;             // padded batch capacity
+} rc_mlpbuf;
