@@ -226,6 +226,14 @@ rc_status rc_mlp_step(rc_mlp* m, void* stream);
 rc_status rc_mlp_zero_grad(rc_mlp* m, void* stream);
 /* parameters / gradients (host copies): W master weights [5][64][64] + [64][16] f32, TABLE [16][2^log2_table][2] f32 */
 enum { RC_MLP_W = 0, RC_MLP_TABLE, RC_MLP_GW, RC_MLP_GTABLE, RC_MLP_COUNT };
+/* One online training step from the cache's scene and current state: n (<= max_batch) random cache
+ * entries, one cosine ray each, self-training targets A_i (E_i + max(y(hit), 0)) with E_i the state's
+ * direct light; judge_out = relative RMS of the MLP's predictions at those entries (before the step)
+ * against the cache's resolved radiance (the plain grid cache as error judge). Waits for the step. */
+rc_status rc_mlp_train_scene(rc_mlp* m, rc_cache* c, int n, uint32_t seed, float* loss_out, float* judge_out);
+/* camera pass over the cache's scene and state whose paths end on the MLP at bounce cam->cache_bounce
+ * (staged schedule; one batched inference over the query points); ms_out as rc_cache_render */
+rc_status rc_mlp_render(rc_mlp* m, rc_cache* c, const rc_camera* cam, uint32_t seed, float* d_img, float* ms_out);
 size_t rc_mlp_array_bytes(const rc_mlp* m, int which);
 rc_status rc_mlp_get(const rc_mlp* m, int which, void* host_dst, size_t bytes);
 rc_status rc_mlp_set(rc_mlp* m, int which, const void* host_src, size_t bytes);

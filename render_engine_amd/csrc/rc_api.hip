@@ -680,6 +680,7 @@ static hipError_t qwork_reserve(rc_cache* c, long long npath) {
     QA(w.rs, 4 * n); QA(w.thr, 4 * n); QA(w.rad, 4 * n);
     const size_t nseg = (n + 1023) / 1024;
     QA(w.idx, 4 * nseg * 1024); QA(w.sidx, 4 * nseg * 1024); QA(w.seg, 8 * nseg);
+    QA(w.mq0, 16 * n); QA(w.mq1, 16 * n); QA(w.mq2, 16 * n); QA(w.mpred, 4 * n);
 #undef QA
     c->qw_paths = npath;
     return hipSuccess;
@@ -713,7 +714,7 @@ extern "C" rc_status rc_cache_render(rc_cache* c, const rc_camera* cam, int no_c
     if (ms_out) HIPCK(hipEventRecord(c->ev[0], st));
     if (staged)
         HIPCK(rc_launch_query_staged(&S->ds, &S->dst[si], no_cache ? nullptr : c->c.R, cam, seed, (uint32_t)c->f, c->dormant, d_img, ctr,
-                                     &c->qw, st));
+                                     &c->qw, nullptr, st));
     else
         HIPCK(rc_launch_query(S->trav, &S->ds, &S->dst[si], no_cache ? nullptr : c->c.R, cam, seed, (uint32_t)c->f, c->dormant, d_img, ctr,
                               st));
@@ -965,6 +966,11 @@ struct rc_mlp {
     rc_mlpdev d{};
     rc_mlpbuf b{};
     int step = 0;
+    // online-training workspace (max_batch samples)
+    float4 *sP = nullptr, *sN = nullptr, *sD = nullptr, *hP = nullptr, *hN = nullptr, *hD = nullptr;
+    float2* AE = nullptr;
+    float *hy = nullptr, *tg = nullptr, *jacc = nullptr;
+    int* ent = nullptr;
     std::vector<void*> owned;
     ~rc_mlp() { for (void* p : owned) (void)hipFree(p); }
 };
@@ -1003,6 +1009,13 @@ extern "C" rc_status rc_mlp_create(const rc_mlp_config* c, rc_mlp** out) {
     MK(dalloc(&b.part, (size_t)(RC_MLP_HIDDEN + 1) * nsplit * 64 * 64, m->owned));
     MK(dalloc(&b.mW, (size_t)RC_MLP_NW, m->owned)); MK(dalloc(&b.vW, (size_t)RC_MLP_NW, m->owned));
     MK(dalloc(&b.mT, nt, m->owned)); MK(dalloc(&b.vT, nt, m->owned)); MK(dalloc(&b.loss, 1, m->owned));
+    {
+        const size_t nb = (size_t)c->max_batch;
+        MK(dalloc(&m->sP, nb, m->owned)); MK(dalloc(&m->sN, nb, m->owned)); MK(dalloc(&m->sD, nb, m->owned));
+        MK(dalloc(&m->hP, nb, m->owned)); MK(dalloc(&m->hN, nb, m->owned)); MK(dalloc(&m->hD, nb, m->owned));
+        MK(dalloc(&m->AE, nb, m->owned)); MK(dalloc(&m->hy, nb, m->owned)); MK(dalloc(&m->tg, nb, m->owned));
+        MK(dalloc(&m->ent, nb, m->owned)); MK(dalloc(&m->jacc, 2, m->owned));
+    }
     // initialisation: weights uniform +-sqrt(6 / 64) (output layer x 0.1), table uniform +-1e-4
     std::vector<float> W(RC_MLP_NW), tb(nt);
     uint64_t r = (uint64_t)c->seed * 0x9E3779B97F4A7C15ull + 1;
@@ -1121,5 +1134,54 @@ extern "C" rc_status rc_mlp_set(rc_mlp* m, int which, const void* host_src, size
     HIPCK(hipMemcpy(p, host_src, b, hipMemcpyHostToDevice));
     if (which == RC_MLP_W) HIPCK(rc_launch_mlp_pack(&m->d, nullptr));
     HIPCK(hipDeviceSynchronize());
+    return RC_OK;
+}
+
+extern "C" rc_status rc_mlp_train_scene(rc_mlp* m, rc_cache* c, int n, uint32_t seed, float* loss_out, float* judge_out) {
+    if (!m || !c) return fail(RC_ERR_ARG, "NULL argument");
+    if (!c->begun) return fail(RC_ERR_STATE, "rc_cache_begin not called");
+    if (n < 1 || n > m->cfg.max_batch) return fail(RC_ERR_ARG, "batch of %d outside 1..%d", n, m->cfg.max_batch);
+    rc_scene* S = c->S;
+    hipStream_t st = S->stream;
+    const int si = c->prev_state;
+    if (!c->r_valid) {   // the judge reads the cache's resolved radiance
+        HIPCK(rc_launch_resolve(&S->ds, &S->dst[si], c->c.Lc, c->c.R, c->dormant, st));
+        c->r_valid = true;
+    }
+    HIPCK(rc_launch_mlp_samples(S->trav, &S->ds, &S->dst[si], c->dormant, seed, n, m->sP, m->sN, m->sD, m->hP, m->hN, m->hD, m->AE, m->ent, st));
+    HIPCK(rc_launch_mlp_infer(&m->d, m->hP, m->hN, m->hD, n, m->hy, st));
+    HIPCK(rc_launch_mlp_targets(n, m->hP, m->hy, m->AE, m->tg, st));
+    HIPCK(rc_launch_mlp_zero(m->b.loss, 1, st));
+    HIPCK(rc_launch_mlp_zero(m->jacc, 2, st));
+    HIPCK(rc_launch_mlp_grad(&m->d, &m->b, m->sP, m->sN, m->sD, m->tg, nullptr, n, m->b.loss, st));
+    HIPCK(rc_launch_mlp_judge(n, m->b.y, c->c.R, m->ent, m->jacc, st));
+    ++m->step;
+    HIPCK(rc_launch_mlp_adam(&m->d, &m->b, m->cfg.lr, m->cfg.beta1, m->cfg.beta2, m->cfg.eps, m->step, st));
+    float h[3];
+    HIPCK(hipMemcpyAsync(h, m->b.loss, 4, hipMemcpyDeviceToHost, st));
+    HIPCK(hipMemcpyAsync(h + 1, m->jacc, 8, hipMemcpyDeviceToHost, st));
+    HIPCK(hipStreamSynchronize(st));
+    if (loss_out) *loss_out = h[0];
+    if (judge_out) *judge_out = h[2] > 0.f ? std::sqrt(h[1] / h[2]) : -1.f;
+    return RC_OK;
+}
+
+extern "C" rc_status rc_mlp_render(rc_mlp* m, rc_cache* c, const rc_camera* cam, uint32_t seed, float* d_img, float* ms_out) {
+    if (!m || !c || !cam || !d_img) return fail(RC_ERR_ARG, "NULL argument");
+    if (!c->begun) return fail(RC_ERR_STATE, "rc_cache_begin not called");
+    if (cam->W < 1 || cam->H < 1 || cam->spp < 1 || cam->bounces < 0 || cam->cache_bounce < 0 || (long long)cam->W * cam->H * cam->spp > (1ll << 30))
+        return fail(RC_ERR_ARG, "camera: bad size / spp / bounces");
+    rc_scene* S = c->S;
+    if (S->trav != 2) return fail(RC_ERR_STATE, "the learned-cache camera pass runs on the 4-wide BVH (trav 2)");
+    hipStream_t st = S->stream;
+    const int si = c->prev_state;
+    HIPCK(qwork_reserve(c, (long long)cam->W * cam->H * cam->spp));
+    if (ms_out) HIPCK(hipEventRecord(c->ev[0], st));
+    HIPCK(rc_launch_query_staged(&S->ds, &S->dst[si], nullptr, cam, seed, (uint32_t)c->f, c->dormant, d_img, nullptr, &c->qw, &m->d, st));
+    if (ms_out) {
+        HIPCK(hipEventRecord(c->ev[1], st));
+        HIPCK(hipEventSynchronize(c->ev[1]));
+        *ms_out = ms_between(c->ev[0], c->ev[1]);
+    }
     return RC_OK;
 }

@@ -39,7 +39,7 @@ hipError_t rc_launch_vol_lights(int nl, const uint32_t* Va, const uint32_t* Vb, 
                                 unsigned long long* out, hipStream_t stream);
 hipError_t rc_launch_query_staged(const rc_dscene* s, const rc_dstate* st, const float* R, const rc_camera* cam, uint32_t seed,
                                   uint32_t frame, uint64_t dormant, float* img, unsigned long long* counters, const rc_qwork* w,
-                                  hipStream_t stream);
+                                  const rc_mlpdev* mlp, hipStream_t stream);
 hipError_t rc_launch_adj_unit(int trav, const rc_dscene* s, const rc_dstate* st, float* G, uint64_t* SG, hipStream_t stream);
 hipError_t rc_launch_adj_forward(int trav, const rc_dscene* s, const rc_dstate* st, const rc_adjbuf* w, const float* I,
                                  const float* alb, int K, int B, uint32_t seed, float* R, hipStream_t stream);
@@ -52,3 +52,7 @@ hipError_t rc_launch_mlp_grad(const rc_mlpdev* m, const rc_mlpbuf* b, const floa
 hipError_t rc_launch_mlp_adam(const rc_mlpdev* m, const rc_mlpbuf* b, float lr, float b1, float b2, float eps, int step, hipStream_t st);
 hipError_t rc_launch_mlp_pack(const rc_mlpdev* m, hipStream_t st);
 hipError_t rc_launch_mlp_zero(float* x, long long n, hipStream_t st);
+hipError_t rc_launch_mlp_samples(int trav, const rc_dscene* s, const rc_dstate* st, uint64_t dormant, uint32_t seed, int n, float4* sP,
+                                 float4* sN, float4* sD, float4* hP, float4* hN, float4* hD, float2* AE, int* ent, hipStream_t stream);
+hipError_t rc_launch_mlp_targets(int n, const float4* hP, const float* hy, const float2* AE, float* t, hipStream_t stream);
+hipError_t rc_launch_mlp_judge(int n, const float* y, const float* R, const int* ent, float* acc, hipStream_t stream);

@@ -66,7 +66,6 @@ __device__ __forceinline__ float rc_est_of(const float* Lc, int64_t i, int nl, u
     for (int l = 0; l < nl; ++l) if (!(l < 64 && ((dormant >> l) & 1ull))) e += Lc[(int64_t)nl * i + l];
     return e;
 }
-__device__ __forceinline__ bool rc_is_dormant(uint64_t dormant, int l) { return l < 64 && ((dormant >> l) & 1ull); }
 
 // k rays per entry, cache read at the hit (multi-bounce through the cache), per-light sample
 // means in c.m, dependency masks (cuda/poc_gpu.cu:433-530). The per-light accumulator is the
@@ -449,22 +448,6 @@ __global__ void __launch_bounds__(RC_TB) k_ref(rc_dscene s, rc_dstate st, const 
 }
 
 // ------------------------------------------------------------------------------ camera query
-// Facing normal and entry of a hit (rc_resolve_hit plus the normal the path continues with).
-__device__ __forceinline__ bool rc_resolve_hit_n(const rc_dscene& s, const rc_dstate& st, f3 H, f3 D, int q, int& pi, f3& n) {
-    if (q >= 0) {
-        pi = rc_point_index(H, q, st);
-        n = ld3(st.Q + 12 * q, 3);
-        return dot3(n, D) > 0.f;
-    }
-    const float* n4 = s.STN + 4 * (int64_t)(-2 - q);
-    n = mk3(n4[0], n4[1], n4[2]);
-    if (dot3(n, D) > 0.f) {
-        if (!(__float_as_int(n4[3]) & 1)) { pi = -1; return true; }
-        n = n * -1.f;
-    }
-    pi = rc_hash_point(s, H, n);
-    return pi < 0;
-}
 
 // One-light next-event estimate at H (normal n): light chosen uniformly, weight nl.
 template <int TRAV>
@@ -612,6 +595,7 @@ __global__ void __launch_bounds__(RC_TB) k_q_gen(rc_camera cam, uint32_t seed, u
     int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p == 0 && stat) atomicAdd(stat, (unsigned long long)npath);
     if (p >= npath) return;
+    ((float4*)w.mq0)[p] = make_float4(0.f, 0.f, 0.f, 0.f);   // no learned-cache query yet
     int pix = p / cam.spp;
     int px = pix % cam.W, py = pix / cam.W;
     uint32_t rs = rc_seed(seed, frame, (uint32_t)p);
@@ -661,7 +645,7 @@ __global__ void __launch_bounds__(RC_TB) k_q_trace(rc_dscene s, rc_dstate st, rc
 // next-event candidate (shadow ray + its term, rc_nee up to its shadow ray) and the bounce ray.
 // The segment's new extension and shadow lists are written in place, in id order.
 __global__ void __launch_bounds__(RC_QB) k_q_shade(rc_dscene s, rc_dstate st, const float* R, rc_camera cam, uint64_t dormant, int b,
-                                                  int npath, int nblk, rc_qwork w) {
+                                                  int npath, int nblk, rc_qwork w, int mlp) {
     __shared__ int s_w[RC_QB / 64];
     const int seg = blockIdx.x, k = threadIdx.x;
     const int cnt = b == 0 ? min(RC_QB, npath - seg * RC_QB) : w.seg[seg];
@@ -678,7 +662,11 @@ __global__ void __launch_bounds__(RC_QB) k_q_shade(rc_dscene s, rc_dstate st, co
             int e;
             if (!rc_resolve_hit_n(s, st, H, d, q, e, nrm)) {
                 float thr = w.thr[p];
-                if (R && b == cam.cache_bounce) {
+                if (mlp && b == cam.cache_bounce) {   // learned cache: query point, view direction back along the ray
+                    ((float4*)w.mq0)[p] = make_float4(H.x, H.y, H.z, 1.f);
+                    ((float4*)w.mq1)[p] = make_float4(nrm.x, nrm.y, nrm.z, 0.f);
+                    ((float4*)w.mq2)[p] = make_float4(-d.x, -d.y, -d.z, 0.f);
+                } else if (R && b == cam.cache_bounce) {
                     w.rad[p] = w.rad[p] + thr * R[e];
                 } else {
                     uint32_t rs = w.rs[p];
@@ -724,6 +712,13 @@ __global__ void __launch_bounds__(RC_QB) k_q_shade(rc_dscene s, rc_dstate st, co
     if (emit_sh) w.sidx[seg * RC_QB + rs_] = p;
     if (emit_b) w.idx[seg * RC_QB + rb_] = p;
     if (k == 0) { w.seg[nblk + seg] = ns; w.seg[seg] = nb2; }
+}
+
+// the learned cache's prediction ends the paths that queried it (their last addition)
+__global__ void __launch_bounds__(RC_TB) k_q_mlp_add(rc_qwork w, int npath) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= npath || ((const float4*)w.mq0)[p].w == 0.f) return;
+    w.rad[p] = w.rad[p] + w.thr[p] * fmaxf(w.mpred[p], 0.f);
 }
 
 // rays in the lists (ray counters): sum of the segment counts
@@ -989,21 +984,26 @@ hipError_t rc_launch_vol_lights(int nl, const uint32_t* Va, const uint32_t* Vb, 
 
 hipError_t rc_launch_query_staged(const rc_dscene* s, const rc_dstate* st, const float* R, const rc_camera* cam, uint32_t seed,
                                   uint32_t frame, uint64_t dormant, float* img, unsigned long long* counters, const rc_qwork* w,
-                                  hipStream_t stream) {
+                                  const rc_mlpdev* mlp, hipStream_t stream) {
     const int npath = cam->W * cam->H * cam->spp;
     const size_t shm = rc_shm(*st, *s, 2);
     const int nblk = (npath + RC_QB - 1) / RC_QB;
     const dim3 tgrid(nblk * (RC_QB / RC_TB));
     hipLaunchKernelGGL(k_q_gen, dim3(rc_grid(npath)), dim3(RC_TB), 0, stream, *cam, seed, frame, npath, *w, counters);
-    const int last = R ? min(cam->cache_bounce, cam->bounces) : cam->bounces;
+    const int last = (R || mlp) ? min(cam->cache_bounce, cam->bounces) : cam->bounces;
     for (int b = 0; b <= last; ++b) {
         if (b > 0 && counters) hipLaunchKernelGGL(k_q_count, dim3(1), dim3(RC_TB), 0, stream, (const int*)w->seg, nblk, counters);
         hipLaunchKernelGGL(k_q_trace<false>, tgrid, dim3(RC_TB), shm, stream, *s, *st, *w, b == 0 ? (const int*)nullptr : (const int*)w->idx,
                            (const int*)w->seg, npath);
-        hipLaunchKernelGGL(k_q_shade, dim3(nblk), dim3(RC_QB), 0, stream, *s, *st, R, *cam, dormant, b, npath, nblk, *w);
-        if (R && b == cam->cache_bounce) break;   // cache read: no shadow or bounce rays follow
+        hipLaunchKernelGGL(k_q_shade, dim3(nblk), dim3(RC_QB), 0, stream, *s, *st, R, *cam, dormant, b, npath, nblk, *w, mlp ? 1 : 0);
+        if ((R || mlp) && b == cam->cache_bounce) break;   // cache read: no shadow or bounce rays follow
         if (counters) hipLaunchKernelGGL(k_q_count, dim3(1), dim3(RC_TB), 0, stream, (const int*)w->seg + nblk, nblk, counters + 1);
         hipLaunchKernelGGL(k_q_trace<true>, tgrid, dim3(RC_TB), shm, stream, *s, *st, *w, (const int*)w->sidx, (const int*)w->seg + nblk, npath);
+    }
+    if (mlp) {
+        hipError_t e = rc_launch_mlp_infer(mlp, (const float4*)w->mq0, (const float4*)w->mq1, (const float4*)w->mq2, npath, w->mpred, stream);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(k_q_mlp_add, dim3(rc_grid(npath)), dim3(RC_TB), 0, stream, *w, npath);
     }
     hipLaunchKernelGGL(k_q_pixels, dim3(rc_grid((long long)cam->W * cam->H)), dim3(RC_TB), 0, stream, *cam, *w, img);
     return hipGetLastError();

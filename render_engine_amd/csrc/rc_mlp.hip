@@ -308,6 +308,76 @@ __global__ void __launch_bounds__(RC_TB) k_mlp_zero(float* x, long long n) {
     if (i < n) x[i] = 0.f;
 }
 
+// ------------------------------------------------------------------------------ online training from the scene
+// Sample s: cache entry i = hash(seed, s) mod Np of the state, one cosine ray from it. Training
+// input (P_i, N_i, ray direction); the hit's query (point, facing normal, direction back along the
+// ray; w = 1 when the hit is a valid entry); A_i and the direct light E_i = sum of the active
+// lights' Edir (the cache's resolved radiance is R_i = A_i (E_i + indirect)).
+template <int TRAV>
+__global__ void __launch_bounds__(RC_TB) k_mlp_samples(rc_dscene s, rc_dstate st, uint64_t dormant, uint32_t seed, int n, float4* sP,
+                                                      float4* sN, float4* sD, float4* hP, float4* hN, float4* hD, float2* AE, int* ent) {
+    extern __shared__ float sh[];
+    int* shq = (int*)(sh + 9 * st.nt);
+    uint32_t* lstk = rc_lstack(sh, st.nt);
+    rc_load_dyn(st, sh, shq);
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    uint32_t rs = rc_hash32(seed ^ rc_hash32((uint32_t)k + 1u));
+    const int i = (int)(rc_hash32(rs ^ 0x5bd1e995u) % (uint32_t)s.Np);
+    const f3 P = ld3(st.P, i), N = ld3(st.N, i), O = P + N * 1e-3f;
+    const float u1 = rc_rnd(rs), u2 = rc_rnd(rs);
+    const f3 D = rc_cosine_dir(N, u1, u2);
+    float t; int q;
+    const bool hit = rc_trace_full<TRAV>(O, D, sh, shq, st.nt, s, t, q, lstk);
+    float valid = 0.f;
+    f3 H = O, nh = N;
+    if (hit) {
+        H = O + D * t;
+        int e;
+        valid = rc_resolve_hit_n(s, st, H, D, q, e, nh) ? 0.f : 1.f;
+    }
+    sP[k] = make_float4(P.x, P.y, P.z, 0.f);
+    sN[k] = make_float4(N.x, N.y, N.z, 0.f);
+    sD[k] = make_float4(D.x, D.y, D.z, 0.f);
+    hP[k] = make_float4(H.x, H.y, H.z, valid);
+    hN[k] = make_float4(nh.x, nh.y, nh.z, 0.f);
+    hD[k] = make_float4(-D.x, -D.y, -D.z, 0.f);
+    const float* ed = st.Edir + (int64_t)s.nl * i;
+    float E = 0.f;
+    for (int l = 0; l < s.nl; ++l) E += rc_is_dormant(dormant, l) ? 0.f : ed[l];
+    AE[k] = make_float2(s.Alb[i], E);
+    ent[k] = i;
+}
+
+// self-training targets t = A_i (E_i + max(y(hit), 0)) (0 for the indirect part on a miss)
+__global__ void __launch_bounds__(RC_TB) k_mlp_targets(int n, const float4* hP, const float* hy, const float2* AE, float* t) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const float2 ae = AE[k];
+    t[k] = ae.x * (ae.y + (hP[k].w != 0.f ? fmaxf(hy[k], 0.f) : 0.f));
+}
+
+// judge: sums of (y - R_i)^2 and R_i^2 over the samples (the grid cache's resolved radiance)
+__global__ void __launch_bounds__(RC_TB) k_mlp_judge(int n, const float* y, const float* R, const int* ent, float* acc) {
+    __shared__ float s_w[2][RC_TB / 64];
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    float d2 = 0.f, r2 = 0.f;
+    if (k < n) {
+        const float r = R[ent[k]], e = y[k] - r;
+        d2 = e * e; r2 = r * r;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) { d2 += __shfl_down(d2, o, 64); r2 += __shfl_down(r2, o, 64); }
+    if ((threadIdx.x & 63) == 0) { s_w[0][threadIdx.x >> 6] = d2; s_w[1][threadIdx.x >> 6] = r2; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float a = 0.f, b = 0.f;
+        for (int w = 0; w < RC_TB / 64; ++w) { a += s_w[0][w]; b += s_w[1][w]; }
+        unsafeAtomicAdd(acc, a);
+        unsafeAtomicAdd(acc + 1, b);
+    }
+}
+
 // ------------------------------------------------------------------------------ launchers
 static inline int rc_mlp_blocks(int n) { return (n + MROWS * MWAVES - 1) / (MROWS * MWAVES); }
 int rc_mlp_pad(int n) { return rc_mlp_blocks(n) * MROWS * MWAVES; }
@@ -355,5 +425,28 @@ hipError_t rc_launch_mlp_pack(const rc_mlpdev* m, hipStream_t st) {
 hipError_t rc_launch_mlp_zero(float* x, long long n, hipStream_t st) {
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_mlp_zero, dim3(rc_grid(n)), dim3(RC_TB), 0, st, x, n);
+    return hipGetLastError();
+}
+
+hipError_t rc_launch_mlp_samples(int trav, const rc_dscene* s, const rc_dstate* st, uint64_t dormant, uint32_t seed, int n, float4* sP,
+                                 float4* sN, float4* sD, float4* hP, float4* hN, float4* hD, float2* AE, int* ent, hipStream_t stream) {
+    if (n <= 0) return hipSuccess;
+    const dim3 g(rc_grid(n)), b(RC_TB);
+    const size_t shm = rc_shm(*st, *s, trav);
+    if (trav == 0) hipLaunchKernelGGL(k_mlp_samples<0>, g, b, shm, stream, *s, *st, dormant, seed, n, sP, sN, sD, hP, hN, hD, AE, ent);
+    else if (trav == 1) hipLaunchKernelGGL(k_mlp_samples<1>, g, b, shm, stream, *s, *st, dormant, seed, n, sP, sN, sD, hP, hN, hD, AE, ent);
+    else hipLaunchKernelGGL(k_mlp_samples<2>, g, b, shm, stream, *s, *st, dormant, seed, n, sP, sN, sD, hP, hN, hD, AE, ent);
+    return hipGetLastError();
+}
+
+hipError_t rc_launch_mlp_targets(int n, const float4* hP, const float* hy, const float2* AE, float* t, hipStream_t stream) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_mlp_targets, dim3(rc_grid(n)), dim3(RC_TB), 0, stream, n, hP, hy, AE, t);
+    return hipGetLastError();
+}
+
+hipError_t rc_launch_mlp_judge(int n, const float* y, const float* R, const int* ent, float* acc, hipStream_t stream) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_mlp_judge, dim3(rc_grid(n)), dim3(RC_TB), 0, stream, n, y, R, ent, acc);
     return hipGetLastError();
 }

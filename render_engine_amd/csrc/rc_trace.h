@@ -47,6 +47,25 @@ __device__ __forceinline__ void rc_load_dyn(const rc_dstate& st, float* sh, int*
 }
 __device__ __forceinline__ uint32_t* rc_lstack(float* sh, int nt) { return (uint32_t*)(sh + ((10 * nt + 3) & ~3)); }
 
+__device__ __forceinline__ bool rc_is_dormant(uint64_t dormant, int l) { return l < 64 && ((dormant >> l) & 1ull); }
+
+// Facing normal and entry of a hit (rc_resolve_hit plus the normal the path continues with).
+__device__ __forceinline__ bool rc_resolve_hit_n(const rc_dscene& s, const rc_dstate& st, f3 H, f3 D, int q, int& pi, f3& n) {
+    if (q >= 0) {
+        pi = rc_point_index(H, q, st);
+        n = ld3(st.Q + 12 * q, 3);
+        return dot3(n, D) > 0.f;
+    }
+    const float* n4 = s.STN + 4 * (int64_t)(-2 - q);
+    n = mk3(n4[0], n4[1], n4[2]);
+    if (dot3(n, D) > 0.f) {
+        if (!(__float_as_int(n4[3]) & 1)) { pi = -1; return true; }
+        n = n * -1.f;
+    }
+    pi = rc_hash_point(s, H, n);
+    return pi < 0;
+}
+
 // ------------------------------------------------------------------------------ host-side launch sizes
 static inline int rc_grid(long long n) { return (int)((n + RC_TB - 1) / RC_TB); }
 // dynamic LDS of a tracing kernel: dynamic triangles + (wide traversal) the lane stacks

@@ -136,6 +136,8 @@ typedef struct rc_qwork {
     uint32_t* rs; float* thr; float* rad;   // RNG state, throughput, radiance
     int32_t* idx; int32_t* sidx;   // per-segment compacted id lists: extension rays, shadow rays
     int32_t* seg;                  // [2 * segments]: extension-list counts, then shadow-list counts
+    void* mq0; void* mq1; void* mq2;   // float4 learned-cache query of each path: (point, queried), normal, view direction
+    float* mpred;                  // the learned cache's prediction per path
 } rc_qwork;
 
 // Device buffers of the cached-radiance adjoint (Np entries, nl lights, B bounces).

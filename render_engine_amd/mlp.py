@@ -10,7 +10,7 @@ import ctypes as C
 import numpy as np
 import torch
 
-from ._ffi import check, lib
+from ._ffi import Camera, check, lib  # noqa: F401
 
 MLP_W, MLP_TABLE, MLP_GW, MLP_GTABLE = range(4)
 LEVELS, HIDDEN, WIDTH = 16, 5, 64
@@ -87,6 +87,20 @@ class MLPCache:
 
     def zero_grad(self):
         check(lib.rc_mlp_zero_grad(self._h, _stream()))
+
+    def train_scene(self, cache, n=1 << 16, seed=1):
+        """one online training step from the cache's scene and state; returns (loss, judge) where
+        judge is the relative RMS of the predictions against the grid cache's resolved radiance"""
+        loss, judge = C.c_float(0.0), C.c_float(0.0)
+        check(lib.rc_mlp_train_scene(self._h, cache._h, n, seed, C.byref(loss), C.byref(judge)))
+        return loss.value, judge.value
+
+    def render(self, cache, img, cam=None, seed=7, timed=False):
+        """camera pass of the cache's scene ending on the MLP at cam.cache_bounce; ms when timed"""
+        cam = cam or cache.scene.camera()
+        ms = C.c_float(0.0)
+        check(lib.rc_mlp_render(self._h, cache._h, C.byref(cam), seed, C.c_void_p(img.data_ptr()), C.byref(ms) if timed else None))
+        return ms.value if timed else None
 
     def get(self, which):
         nb = lib.rc_mlp_array_bytes(self._h, which)
