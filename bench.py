@@ -33,6 +33,10 @@ CONFIGS = {
     # adjoint of all cached radiance, 3 bounces, 4 rays per entry and bounce
     4: dict(kind="street", seed=1, nboxes=2000, tris=200_000, nlamps=63, sun=True, points=530_000, table_log2=22, W=960, H=540,
             spp=2, bounces=3, adjoint=dict(K=4, B=3, seed=17)),
+    # learned cache: config 2's street; hash-encoded MLP (16 levels x 2 features, 2^19 table, 5 x 64 ReLU), BF16 MFMA,
+    # Adam lr 1e-2, 2^16 self-training samples per frame, queried at 1920x1080
+    5: dict(kind="street", seed=1, nboxes=2000, tris=200_000, nlamps=32, sun=True, points=530_000, table_log2=22, W=960, H=540,
+            spp=4, bounces=3, mlp=dict(log2_table=19, base_res=16, max_res=2048, batch=1 << 16, lr=1e-2, W=1920, H=1080)),
 }
 
 
@@ -126,15 +130,11 @@ def cpu_baseline(osc, scene, seconds, log):
                        f"same scene, state 0", fps=frames / t_tot)
 
 
-def quality(rea, scene, cam, log, pre=20, post=60, ref_spp_rounds=16, occluder=False):
-    """Light-off events against converged references (cuda/prep_mesh.cuh:454-520 solve, then the
-    same query at 64 spp): per method, the region-luminance bias sum(est)/sum(ref) - 1 over the
-    pixels where the switched-off light carried > 50 % of the converged radiance (9x9 blur at
-    960 px, windows/sondes.py:30-47), per frame after the event; frames until |bias| < 5 %."""
+def reference_images(rea, scene, cam, log, ref_spp_rounds=16):
+    """converged per-state references (two-chain Jacobi solve), then per state the camera image at
+    ref_spp_rounds x spp through the converged cache"""
     import numpy as np
     import torch
-
-    from render_engine_amd import scenarios as sc
 
     t0 = time.time()
     noise, iters = scene.reference(K=32, warm=8, block=32, max_iter=256, target=0.03)
@@ -149,6 +149,21 @@ def quality(rea, scene, cam, log, pre=20, post=60, ref_spp_rounds=16, occluder=F
             c.render(img, cam, seed=100000 + k)
             acc += img.cpu().numpy()
         refs[st] = (acc / ref_spp_rounds).reshape(cam.H, cam.W)
+    return refs
+
+
+def quality(rea, scene, cam, log, pre=20, post=60, ref_spp_rounds=16, occluder=False):
+    """Light-off events against converged references (cuda/prep_mesh.cuh:454-520 solve, then the
+    same query at 64 spp): per method, the region-luminance bias sum(est)/sum(ref) - 1 over the
+    pixels where the switched-off light carried > 50 % of the converged radiance (9x9 blur at
+    960 px, windows/sondes.py:30-47), per frame after the event; frames until |bias| < 5 %."""
+    import numpy as np
+    import torch
+
+    from render_engine_amd import scenarios as sc
+
+    refs = reference_images(rea, scene, cam, log, ref_spp_rounds)
+    img = torch.zeros(cam.W * cam.H, dtype=torch.float32, device="cuda")
     events = {"lamp 0 off": (0, 2), "night (sun off)": (0, 1)} if scene.info.NS >= 3 else {"light 1 off": (0, 1)}
     if occluder and scene.info.NS >= 4:
         events["occluder inserted"] = (0, 3)
@@ -274,6 +289,133 @@ def bench_adjoint(args, rea, scene, cfg, log):
     print(json.dumps(out))
 
 
+def bench_mlp(args, rea, scene, cfg, log):
+    """Config 5: one step = one frame of the plain grid cache + one online training step of the
+    hash-encoded MLP cache on 2^16 self-training samples + the camera pass at 1920x1080 ending on
+    the MLP (batched BF16 MFMA inference at the cache bounce). Reports frame throughput, inference
+    TFLOP/s against the BF16 MFMA peak, the error of the MLP judged against the grid cache, and the
+    light-off events with the grid cache as judge and fallback: a grid cache with targeted
+    invalidation runs beside the MLP, and the frame's image comes from it while the MLP's error
+    judged against it is above 25 %."""
+    import numpy as np
+    import torch
+
+    import render_engine_amd.mlp as M
+    from render_engine_amd import scenarios as sc
+
+    info = scene.info
+    mc = cfg["mlp"]
+    mlp = M.MLPCache(M.config(list(info.lo), list(info.ext), log2_table=mc["log2_table"], base_res=mc["base_res"],
+                              max_res=mc["max_res"], max_batch=mc["batch"], lr=mc["lr"]))
+    cache = rea.Cache(scene, rea.methods.plain())
+    cache.set_render_mode(rea.RENDER_STAGED)
+    qcam = scene.camera()
+    qcam.W, qcam.H, qcam.spp, qcam.cache_bounce = mc["W"], mc["H"], 1, 1
+    qimg = torch.zeros(qcam.W * qcam.H, dtype=torch.float32, device="cuda")
+    B = mc["batch"]
+    # warm-up: the grid cache and the MLP train together from a cold MLP
+    cache.begin(0, 11)
+    judges = []
+    for f in range(args.warmup + 200):
+        cache.frame(0)
+        _, j = mlp.train_scene(cache, n=B, seed=7 + f)
+        judges.append(j)
+    log(f"judge after warm-up: {judges[-1]:.3f} (first {judges[0]:.3f})")
+    # timed frames
+    tr, rd, fr = [], [], []
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    for f in range(args.steps):
+        t0 = time.perf_counter()
+        cache.frame(0)
+        cache.sync()
+        t1 = time.perf_counter()
+        _, j = mlp.train_scene(cache, n=B, seed=1000 + f)
+        t2 = time.perf_counter()
+        ms = mlp.render(cache, qimg, qcam, seed=2000 + f, timed=True)
+        fr.append((t1 - t0) * 1e3); tr.append((t2 - t1) * 1e3); rd.append(ms)
+        judges.append(j)
+    elapsed = time.perf_counter() - t_start
+    # inference throughput of the MFMA network on one full batch (positions of random cache entries)
+    P = torch.from_numpy(np.concatenate([scene.array("P", 0).reshape(-1, 3)[:B], np.zeros((B, 1), np.float32)], 1)).cuda()
+    Nn = torch.from_numpy(np.concatenate([scene.array("N", 0).reshape(-1, 3)[:B], np.zeros((B, 1), np.float32)], 1)).cuda()
+    y = torch.empty(B, dtype=torch.float32, device="cuda")
+    for _ in range(3):
+        mlp.infer(P, Nn, Nn, y)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    reps = 20
+    e0.record()
+    for _ in range(reps):
+        mlp.infer(P, Nn, Nn, y)
+    e1.record()
+    e1.synchronize()
+    ms_inf = e0.elapsed_time(e1) / reps
+    flop_per_query = 2 * (M.HIDDEN * 64 * 64 + 64 * 16)
+    tflops = B * flop_per_query / (ms_inf * 1e-3) / 1e12
+    log(f"frame {np.mean(fr):.3f} ms, train {np.mean(tr):.3f} ms, render {np.mean(rd):.3f} ms, infer {ms_inf:.4f} ms "
+        f"({tflops:.1f} TFLOP/s), judge {np.mean(judges[-args.steps:]):.3f}")
+    qual = None
+    if not args.no_quality:
+        try:
+            cam = scene.camera()
+            refs = reference_images(rea, scene, cam, log)
+            img = torch.zeros(cam.W * cam.H, dtype=torch.float32, device="cuda")
+            qual = {}
+            for name, (a, b) in {"lamp 0 off": (0, 2), "night (sun off)": (0, 1)}.items():
+                region = sc.light_off_region(refs[a], refs[b])
+                c = rea.Cache(scene, rea.methods.targeted())
+                c.set_render_mode(rea.RENDER_STAGED)
+                c.begin(a, 777)
+                ml = M.MLPCache(M.config(list(info.lo), list(info.ext), log2_table=mc["log2_table"], base_res=mc["base_res"],
+                                         max_res=mc["max_res"], max_batch=mc["batch"], lr=mc["lr"]))
+                bm, bg, bo, jj, fb = [], [], [], [], 0
+                for f in range(200 + 60):
+                    st = a if f < 200 else b
+                    c.frame(st)
+                    _, j = ml.train_scene(c, n=B, seed=5000 + f)
+                    if f >= 200:
+                        ml.render(c, img, cam, seed=9000 + f)
+                        em = img.cpu().numpy().reshape(cam.H, cam.W)
+                        c.render(img, cam, seed=9000 + f)
+                        eg = img.cpu().numpy().reshape(cam.H, cam.W)
+                        bm.append(sc.region_bias(em, refs[b], region))
+                        bg.append(sc.region_bias(eg, refs[b], region))
+                        use_grid = j > 0.25
+                        fb += int(use_grid)
+                        bo.append(bg[-1] if use_grid else bm[-1])
+                        jj.append(round(j, 4))
+                qual[name] = {"region_pixels": int(region.sum()),
+                              "mlp": {"frames_to_5pct": sc.frames_to_below(bm, 0.05), "bias_first_10": [round(x, 4) for x in bm[:10]]},
+                              "grid_targeted": {"frames_to_5pct": sc.frames_to_below(bg, 0.05)},
+                              "mlp_with_grid_fallback": {"frames_to_5pct": sc.frames_to_below(bo, 0.05), "fallback_frames": fb},
+                              "judge_first_10": jj[:10]}
+                log(f"{name}: {json.dumps(qual[name])}")
+        except Exception as e:
+            log(f"quality phase failed: {e!r}")
+    step_ms = float(np.mean(fr) + np.mean(tr) + np.mean(rd))
+    out = {
+        "metric": "Mrays/s + cache Mpts/s; frames to <5% error after light-off, 1 GPU",
+        "value": round(qcam.W * qcam.H / (np.mean(rd) * 1e3), 2),
+        "unit": "Mqueries/s (1920x1080 camera pass ending on the learned cache)",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(elapsed * 1e3 / args.steps, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "bf16 (fp32 accumulate)",
+        "data": "synthetic (procedural scene, seeded)",
+        "config": {"workload": f"hash-encoded MLP cache on the street ({info.Np} entries): 16 levels x 2 features, 2^{mc['log2_table']} "
+                               f"table, 5 x 64 ReLU, {B} training samples per frame, queried at {qcam.W}x{qcam.H}",
+                   "config_index": args.config},
+        "kernels": {"grid_frame_ms": float(np.mean(fr)), "mlp_train_ms": float(np.mean(tr)), "mlp_render_1080p_ms": float(np.mean(rd)),
+                    "mlp_infer_65536_ms": ms_inf, "step_ms": step_ms},
+        "roofline": {"bound": "mfma", "achieved": tflops, "peak": 2500.0, "unit": "TFLOP/s", "frac": tflops / 2500.0, "traffic": None,
+                     "kernel": "k_mlp_fwd (inference, 65536 queries)",
+                     "basis": f"{flop_per_query} FLOP per query (5 x 64x64 + 64x16 multiply-adds)"},
+        "judge_relrms": {"after_warmup": round(float(judges[args.warmup + 199]), 4), "timed_mean": round(float(np.mean(judges[-args.steps:])), 4)},
+        "cpu_baseline": None,
+        "light_off": qual,
+    }
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -314,6 +456,9 @@ def main():
     log(f"upload + per-state build {time.time() - t0:.1f} s")
     if "adjoint" in cfg:
         bench_adjoint(args, rea, scene, cfg, log)
+        return
+    if "mlp" in cfg:
+        bench_mlp(args, rea, scene, cfg, log)
         return
     m = rea.methods.plain()
     cache = rea.Cache(scene, m)

@@ -254,11 +254,13 @@ __global__ void __launch_bounds__(RC_TB) k_mlp_dw_sum(const float* part, int nsp
     gW[e] = s;
 }
 
-// hash-table gradient: each sample's feature gradients spread over its 8 corners per level
+// hash-table gradient: each sample's feature gradients spread over its 8 corners per level;
+// threads run level-major (consecutive threads: consecutive samples of one level), so the
+// atomics of a wave stay within one level's table
 __global__ void __launch_bounds__(RC_TB) k_mlp_enc_bwd(rc_mlpdev m, const float4* P, int n, const float* gX) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n * RC_MLP_LEVELS) return;
-    const int s = t / RC_MLP_LEVELS, lv = t % RC_MLP_LEVELS;
+    const int lv = t / n, s = t % n;
     const float g0 = gX[(int64_t)s * (2 * RC_MLP_LEVELS) + 2 * lv], g1 = gX[(int64_t)s * (2 * RC_MLP_LEVELS) + 2 * lv + 1];
     if (g0 == 0.f && g1 == 0.f) return;
     const f3 x = rc_mlp_norm(m, P[s]);
