@@ -51,6 +51,9 @@ typedef struct rc_method {
     int memory, ramp, wake_verify, snaps, snap_verify;
     int flush;           /* event to a state not in a snapshot slot clears the whole cache */
     int targeted;        /* dependency-targeted invalidation at the event */
+    int compact;         /* with snaps: snapshots as 24-byte records of the entries with samples (rc_snap_rec):
+                            restore flushes, then writes each record's radiance into channel 0 with its
+                            history (rtxgi_bench.patch:1289-1345); full per-channel snapshots otherwise */
     int grad_masks;      /* > 0: targeted invalidation reads, instead of the recorded light masks, the
                             gradient-support masks of the cached-radiance adjoint of both states
                             (grad_masks rays per entry and bounce, 3 bounces, seed 0x5eed0000 + state;
@@ -148,6 +151,15 @@ rc_status rc_cache_render(rc_cache* c, const rc_camera* cam, int no_cache, uint3
  * staged workspace (~100 B per path) is kept by the cache. */
 enum { RC_RENDER_FUSED = 0, RC_RENDER_STAGED = 1 };
 rc_status rc_cache_set_render_mode(rc_cache* c, int mode);
+/* Compact snapshot record (24 bytes; the SDK bench's SnapRec, rtxgi_bench.patch:1289-1345): entry index,
+ * history count, entry key (cell key of the hash grid; 1 << 63 | index for points of dynamic quads),
+ * indirect radiance summed over the light channels, frames since reset. */
+typedef struct rc_snap_rec { uint32_t index; float n; uint64_t key; float radiance; float age; } rc_snap_rec;
+/* records of the cache's entries with samples, in entry order, into host_records (capacity records);
+ * count_out = records written (the number of entries with samples when capacity suffices) */
+rc_status rc_cache_snapshot_save(rc_cache* c, rc_snap_rec* host_records, size_t capacity, size_t* count_out);
+/* flush, then restore the records whose key matches its entry (radiance into channel 0, history, age) */
+rc_status rc_cache_snapshot_load(rc_cache* c, const rc_snap_rec* host_records, size_t count);
 size_t rc_cache_array_bytes(const rc_cache* c, int which);
 rc_status rc_cache_get(const rc_cache* c, int which, void* host_dst, size_t bytes);
 rc_status rc_cache_set(rc_cache* c, int which, const void* host_src, size_t bytes);

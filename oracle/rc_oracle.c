@@ -972,6 +972,10 @@ struct orc_runner {
     uint64_t dormant; float ilearn[128];
     float nmax;
     const uint64_t* gmask;
+    /* compact snapshots: records (index, n, key, radiance, age) of the entries with samples */
+    struct orc_rec { uint32_t index; float n; uint64_t key; float radiance; float age; } * csnap;
+    int slot_count[ORC_KSNAP];
+    uint64_t* ekeys;
 };
 
 void orc_runner_set_grad_masks(orc_runner* r, const uint64_t* masks) { r->gmask = masks; }
@@ -998,7 +1002,13 @@ orc_runner* orc_runner_create(const orc_scene* states, int NS, const orc_method*
     c->stale = (uint64_t*)calloc(Np, 8); c->age = (float*)calloc(Np, 4);
     c->sumpr = (long long*)calloc(2, 8); c->npend = (int32_t*)calloc(2, 4); c->err = (double*)calloc(3, 8);
     r->lnew = (uint64_t*)calloc(Np, 8);
-    if (m->snaps) r->snap = (float*)calloc((size_t)ORC_KSNAP * nl * Np, 4);
+    if (m->snaps && !m->compact) r->snap = (float*)calloc((size_t)ORC_KSNAP * nl * Np, 4);
+    if (m->snaps && m->compact) r->csnap = (struct orc_rec*)calloc((size_t)ORC_KSNAP * Np, sizeof(struct orc_rec));
+    r->ekeys = (uint64_t*)malloc(sizeof(uint64_t) * Np);   /* entry keys from the hash grid (quad points: 1 << 63 | index) */
+    for (size_t i = 0; i < Np; ++i) r->ekeys[i] = (1ull << 63) | (uint64_t)i;
+    if (s->HK && s->hmask < 0x80000000u)   /* scenes without a hash table (box scenes) have hmask = size - 1 = ~0 */
+        for (size_t k = 0; k <= (size_t)s->hmask; ++k)
+            if (s->HK[k] != 0ull && s->HV[k] >= 0 && s->HV[k] < (int)Np) r->ekeys[s->HV[k]] = s->HK[k];
     return r;
 }
 void orc_runner_destroy(orc_runner* r) {
@@ -1006,7 +1016,7 @@ void orc_runner_destroy(orc_runner* r) {
     orc_cache* c = &r->c;
     free(c->Lc); free(c->Lc2); free(c->m); free(c->R); free(c->n); free(c->c0); free(c->kk); free(c->pending); free(c->cur); free(c->prev);
     free(c->lcur); free(c->lprev); free(c->rstat); free(c->dstat); free(c->stale); free(c->age); free(c->sumpr); free(c->npend);
-    free(c->err); free(r->lnew); free(r->snap); free(r);
+    free(c->err); free(r->lnew); free(r->snap); free(r->csnap); free(r->ekeys); free(r);
 }
 const orc_cache* orc_runner_cache(const orc_runner* r) { return &r->c; }
 
@@ -1046,7 +1056,20 @@ static void event_block(orc_runner* r, int e, int si, int* invalidated) {
         for (int a = 0; a < r->nslots; ++a) if (r->slot_state[a] == e) k = a;
         if (k < 0) { if (r->nslots < ORC_KSNAP) k = r->nslots++; else { k = r->slot_next; r->slot_next = (r->slot_next + 1) % ORC_KSNAP; } }
         r->slot_state[k] = e;
-        memcpy(r->snap + (size_t)k * Np * nl, c->Lc, (size_t)Np * nl * 4);
+        if (M->compact) {   /* records of the entries with samples, entry order (rtxgi_bench.patch:1312-1334) */
+            struct orc_rec* out = r->csnap + (size_t)k * Np;
+            int cnt = 0;
+            for (int i = 0; i < Np; ++i) {
+                if (!(c->n[i] > 0.f)) continue;
+                float L = 0.f;
+                for (int l = 0; l < nl; ++l) L += c->Lc[(size_t)nl * i + l];
+                out[cnt].index = (uint32_t)i; out[cnt].n = c->n[i]; out[cnt].key = r->ekeys[i]; out[cnt].radiance = L; out[cnt].age = c->age[i];
+                ++cnt;
+            }
+            r->slot_count[k] = cnt;
+        } else {
+            memcpy(r->snap + (size_t)k * Np * nl, c->Lc, (size_t)Np * nl * 4);
+        }
     }
     if (M->oracle && r->ref) orc_k_oracle_method(c, Np, r->ref[e], r->ref[si]);
     if (M->rescale && !M->deps) {
@@ -1122,7 +1145,23 @@ static void event_block(orc_runner* r, int e, int si, int* invalidated) {
         }
     }
     if (M->flush && hit < 0) orc_k_flush(c, Np, nl);
-    if (M->snaps && hit >= 0) {
+    if (M->snaps && M->compact && hit >= 0) {   /* flush, then the records (rtxgi_bench.patch:1337-1345) */
+        orc_k_flush(c, Np, nl);
+        c->npend[0] = c->npend[1] = 0;
+        if (M->snap_verify) memset(c->rstat, 0, 4 * (size_t)NR * 8);
+        const struct orc_rec* rec = r->csnap + (size_t)hit * Np;
+        for (int j = 0; j < r->slot_count[hit]; ++j) {
+            const uint32_t i = rec[j].index;
+            if (i >= (uint32_t)Np || r->ekeys[i] != rec[j].key) continue;
+            float* row = c->Lc + (size_t)nl * i;
+            row[0] = rec[j].radiance;
+            for (int l = 1; l < nl; ++l) row[l] = 0.f;
+            c->n[i] = rec[j].n; c->age[i] = rec[j].age; c->pending[i] = 0;
+            if (M->snap_verify) { c->pending[i] = 1; c->c0[i] = est_of(c->Lc, i, nl, r->dormant); c->npend[1] += 1; }
+        }
+        if (M->snap_verify) { c->npend[0] = c->npend[1]; r->pend_left = ORC_VERIF; }
+        else r->pend_left = 0;
+    } else if (M->snaps && hit >= 0) {
         memcpy(c->Lc, r->snap + (size_t)hit * Np * nl, (size_t)Np * nl * 4);
         if (M->snap_verify) {
             c->npend[0] = c->npend[1] = 0;

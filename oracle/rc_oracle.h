@@ -174,7 +174,7 @@ typedef struct orc_runner orc_runner;
 typedef struct orc_method {
     int mode; float nmax, budget; int deps, verify, oracle, track_every, rescale;
     uint64_t resp_mask; float nmax_resp; int ddgi; float ddgi_h, ddgi_thr; int ddgi_clamp;
-    int memory, ramp, wake_verify, snaps, snap_verify, flush, targeted, grad_masks;
+    int memory, ramp, wake_verify, snaps, snap_verify, flush, targeted, compact, grad_masks;
 } orc_method;
 /* states[NS]: one orc_scene per state (static fields shared); occ / em event lists like the
  * library's (occ[a*NS+b], em[(a*NS+b)*nl+l]) given as offsets into one flat int array. */

@@ -17,6 +17,8 @@ __all__ = ["Scene", "Cache", "Method", "Camera", "RcError", "methods", "scenario
 
 TRAV_REF, TRAV_BINARY, TRAV_WIDE = 0, 1, 2
 RENDER_FUSED, RENDER_STAGED = 0, 1
+# compact snapshot record (rc_snap_rec, 24 bytes)
+SNAP_REC = np.dtype([("index", np.uint32), ("n", np.float32), ("key", np.uint64), ("radiance", np.float32), ("age", np.float32)])
 
 _DT = {"ST": np.float32, "STN": np.float32, "HK": np.uint64, "HV": np.int32, "P": np.float32, "N": np.float32,
        "ALB": np.float32, "REGION": np.int32, "L": np.float32, "LTYPE": np.int32, "INTEN": np.float32, "Q": np.float32,
@@ -184,6 +186,20 @@ class Cache:
         if timed:
             out.append(ms.value)
         return out[0] if len(out) == 1 else (tuple(out) if out else None)
+
+    def snapshot_save(self):
+        """compact snapshot of the entries with samples: structured array of 24-byte records
+        (index, n, key, radiance, age) in entry order"""
+        cap = self.scene.info.Np
+        out = np.zeros(cap, SNAP_REC)
+        n = C.c_size_t(0)
+        check(lib.rc_cache_snapshot_save(self._h, out.ctypes.data_as(C.c_void_p), cap, C.byref(n)))
+        return out[: n.value]
+
+    def snapshot_load(self, records):
+        """flush, then restore the records whose key matches its entry"""
+        r = np.ascontiguousarray(records, SNAP_REC)
+        check(lib.rc_cache_snapshot_load(self._h, r.ctypes.data_as(C.c_void_p), len(r)))
 
     def set_render_mode(self, mode):
         """RENDER_FUSED (default): one thread per path; RENDER_STAGED (4-wide BVH): stage kernels over

@@ -21,7 +21,7 @@ class Method(C.Structure):
         ("oracle", C.c_int), ("track_every", C.c_int), ("rescale", C.c_int), ("resp_mask", C.c_uint64),
         ("nmax_resp", C.c_float), ("ddgi", C.c_int), ("ddgi_h", C.c_float), ("ddgi_thr", C.c_float),
         ("ddgi_clamp", C.c_int), ("memory", C.c_int), ("ramp", C.c_int), ("wake_verify", C.c_int),
-        ("snaps", C.c_int), ("snap_verify", C.c_int), ("flush", C.c_int), ("targeted", C.c_int), ("grad_masks", C.c_int),
+        ("snaps", C.c_int), ("snap_verify", C.c_int), ("flush", C.c_int), ("targeted", C.c_int), ("compact", C.c_int), ("grad_masks", C.c_int),
     ]
 
 
@@ -88,6 +88,8 @@ _SIGS = {
     "rc_cache_set": (C.c_int, [_P, C.c_int, _P, C.c_size_t]),
     "rc_cache_sync": (C.c_int, [_P]),
     "rc_cache_set_render_mode": (C.c_int, [_P, C.c_int]),
+    "rc_cache_snapshot_save": (C.c_int, [_P, _P, C.c_size_t, C.POINTER(C.c_size_t)]),
+    "rc_cache_snapshot_load": (C.c_int, [_P, _P, C.c_size_t]),
     "rc_adjoint_create": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_uint32, C.POINTER(_P)]),
     "rc_adjoint_destroy": (None, [_P]),
     "rc_adjoint_forward": (C.c_int, [_P, _P, _P, _P, _P]),

@@ -66,6 +66,7 @@ struct rc_scene {
     std::vector<rc_dstate> dst;
     std::vector<int*> dOcc, dEm, dOccF;
     unsigned long long* d_lights = nullptr;   // event light set (k_vol_lights)
+    uint64_t* d_ekeys = nullptr;              // per entry: its hash-grid key (1 << 63 | index for quad points)
     std::vector<void*> owned;
     // converged references (host + device)
     bool has_ref = false;
@@ -272,6 +273,13 @@ extern "C" rc_status rc_scene_upload(rc_scene* s, int trav) {
     for (auto& v : H.occ) { int* p; std::vector<int> w = v.empty() ? std::vector<int>{0} : v; HIPCK(dput(&p, w, s->owned)); s->dOcc.push_back(p); }
     for (auto& v : H.em) { int* p; std::vector<int> w = v.empty() ? std::vector<int>{0} : v; HIPCK(dput(&p, w, s->owned)); s->dEm.push_back(p); }
     for (auto& v : H.occ_fine) { int* p; std::vector<int> w = v.empty() ? std::vector<int>{0} : v; HIPCK(dput(&p, w, s->owned)); s->dOccF.push_back(p); }
+    {   // entry keys for the compact snapshot records
+        std::vector<uint64_t> ek(H.Np);
+        for (int i = 0; i < H.Np; ++i) ek[i] = (1ull << 63) | (uint64_t)i;
+        for (size_t k = 0; k < H.HK.size(); ++k)
+            if (H.HK[k] != 0ull && H.HV[k] >= 0 && H.HV[k] < H.Np) ek[H.HV[k]] = H.HK[k];
+        HIPCK(dput(&s->d_ekeys, ek, s->owned));
+    }
     for (int k = 0; k < H.NS; ++k) HIPCK(rc_launch_build(trav, &s->ds, &s->dst[k], s->stream));
     HIPCK(hipStreamSynchronize(s->stream));
     if (s->has_ref) {   // references read from a scene file
@@ -384,6 +392,9 @@ struct rc_cache {
     std::vector<float> ilearn;
     int nslots = 0, slot_next = 0, last_event_f = 0;
     int slot_state[RC_KSNAP];
+    int slot_count[RC_KSNAP];     // records of each compact snapshot slot
+    int* d_blk = nullptr;         // per-block counts / offsets of the snapshot packing
+    std::vector<int> h_blk;
     rc_params p{};
     int render_mode = RC_RENDER_FUSED;
     std::vector<uint64_t*> gmask;   // per state: gradient-support masks (grad_masks methods)
@@ -421,7 +432,9 @@ extern "C" rc_status rc_cache_create(rc_scene* s, const rc_method* m, rc_cache**
     CK2(dalloc(&d.lcur, Np, c->owned)); CK2(dalloc(&d.lprev, Np, c->owned)); CK2(dalloc(&c->lnew, Np, c->owned));
     CK2(dalloc(&d.rstat, 4 * (size_t)H.NR, c->owned)); CK2(dalloc(&d.dstat, 3 * (size_t)H.NR, c->owned));
     CK2(dalloc(&d.stale, Np, c->owned)); CK2(dalloc(&d.age, Np, c->owned));
-    if (m->snaps) CK2(dalloc(&d.snap, (size_t)RC_KSNAP * nl * Np, c->owned));
+    if (m->snaps && !m->compact) CK2(dalloc(&d.snap, (size_t)RC_KSNAP * nl * Np, c->owned));
+    if (m->snaps && m->compact) { rc_snaprec* cs = nullptr; CK2(dalloc(&cs, (size_t)RC_KSNAP * Np, c->owned)); d.csnap = cs; }
+    CK2(dalloc(&c->d_blk, (size_t)(Np + 255) / 256, c->owned));
     CK2(dalloc(&d.sumpr, 2, c->owned)); CK2(dalloc(&d.npend, 2, c->owned)); CK2(dalloc(&d.err, 3, c->owned));
     CK2(dalloc(&c->d_counters, 2, c->owned));
     for (auto& e : c->ev) CK2(hipEventCreate(&e));
@@ -433,6 +446,25 @@ extern "C" rc_status rc_cache_create(rc_scene* s, const rc_method* m, rc_cache**
 extern "C" void rc_cache_destroy(rc_cache* c) { delete c; }
 
 static rc_status cache_grad_masks(rc_cache* c);
+
+// pack the entries with samples into compact records at dst (entry order); *count = records
+static hipError_t snap_pack(rc_cache* c, rc_snaprec* dst, int* count) {
+    rc_scene* S = c->S;
+    const int Np = S->H.Np, nb = (Np + 255) / 256;
+    hipStream_t st = S->stream;
+    hipError_t e = rc_launch_snap_count(&c->c, Np, c->d_blk, st);
+    c->h_blk.resize(nb);
+    if (e == hipSuccess) e = hipMemcpyAsync(c->h_blk.data(), c->d_blk, (size_t)nb * 4, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return e;
+    int run = 0;
+    for (int b = 0; b < nb; ++b) { const int v = c->h_blk[b]; c->h_blk[b] = run; run += v; }
+    e = hipMemcpyAsync(c->d_blk, c->h_blk.data(), (size_t)nb * 4, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = rc_launch_snap_pack(&c->c, Np, S->H.nl, S->d_ekeys, c->d_blk, dst, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);   // the host copy of the offsets must outlive the kernel
+    *count = run;
+    return e;
+}
 
 extern "C" rc_status rc_cache_begin(rc_cache* c, int state0, uint32_t seed) {
     if (!c) return fail(RC_ERR_ARG, "NULL cache");
@@ -470,7 +502,7 @@ extern "C" rc_status rc_cache_begin(rc_cache* c, int state0, uint32_t seed) {
     c->dormant = 0;
     c->ilearn.assign(nl, 0.f);
     c->nslots = 0; c->slot_next = 0; c->last_event_f = 0;
-    for (int k = 0; k < RC_KSNAP; ++k) c->slot_state[k] = -1;
+    for (int k = 0; k < RC_KSNAP; ++k) { c->slot_state[k] = -1; c->slot_count[k] = 0; }
     rc_params& p = c->p;
     memset(&p, 0, sizeof p);
     p.Np = (int)Np; p.NR = H.NR; p.mode = c->M.mode; p.budget = c->M.budget; p.nmax = c->nmax; p.seed = seed;
@@ -497,7 +529,13 @@ static rc_status run_event(rc_cache* c, int e, int si, int& invalidated) {
         for (int a = 0; a < c->nslots; ++a) if (c->slot_state[a] == e) k = a;
         if (k < 0) { if (c->nslots < RC_KSNAP) k = c->nslots++; else { k = c->slot_next; c->slot_next = (c->slot_next + 1) % RC_KSNAP; } }
         c->slot_state[k] = e;
-        HIPCK(hipMemcpyAsync(d.snap + (size_t)k * Np * nl, d.Lc, (size_t)Np * nl * 4, hipMemcpyDeviceToDevice, st));
+        if (M.compact) {
+            int cnt = 0;
+            HIPCK(snap_pack(c, (rc_snaprec*)d.csnap + (size_t)k * Np, &cnt));
+            c->slot_count[k] = cnt;
+        } else {
+            HIPCK(hipMemcpyAsync(d.snap + (size_t)k * Np * nl, d.Lc, (size_t)Np * nl * 4, hipMemcpyDeviceToDevice, st));
+        }
     }
     if (M.oracle && S->has_ref) HIPCK(rc_launch_oracle_method(&d, Np, S->dst[e].ref, S->dst[si].ref, st));
     if (M.rescale && !M.deps) {
@@ -574,7 +612,19 @@ static rc_status run_event(rc_cache* c, int e, int si, int& invalidated) {
         }
     }
     if (M.flush && hit < 0) HIPCK(rc_launch_flush(&d, Np, nl, st));   // new state: clear (rtxgi_bench.patch:363-366)
-    if (M.snaps && hit >= 0) {   // revisited state: restore its snapshot
+    if (M.snaps && M.compact && hit >= 0) {   // revisited state: flush, then its records (rtxgi_bench.patch:1337-1345)
+        HIPCK(rc_launch_flush(&d, Np, nl, st));
+        HIPCK(hipMemsetAsync(d.npend, 0, 8, st));
+        if (M.snap_verify) HIPCK(hipMemsetAsync(d.rstat, 0, 4 * (size_t)H.NR * 8, st));
+        HIPCK(rc_launch_snap_restore(&d, Np, nl, S->d_ekeys, (const rc_snaprec*)d.csnap + (size_t)hit * Np, c->slot_count[hit],
+                                     M.snap_verify ? 1 : 0, c->dormant, st));
+        if (M.snap_verify) {
+            HIPCK(hipMemcpyAsync(d.npend, d.npend + 1, 4, hipMemcpyDeviceToDevice, st));
+            c->pend_left = RC_VERIF_FRAMES;
+        } else {
+            c->pend_left = 0;
+        }
+    } else if (M.snaps && hit >= 0) {   // revisited state: restore its snapshot
         HIPCK(hipMemcpyAsync(d.Lc, d.snap + (size_t)hit * Np * nl, (size_t)Np * nl * 4, hipMemcpyDeviceToDevice, st));
         if (M.snap_verify) {
             HIPCK(hipMemsetAsync(d.npend, 0, 8, st));
@@ -1183,5 +1233,44 @@ extern "C" rc_status rc_mlp_render(rc_mlp* m, rc_cache* c, const rc_camera* cam,
         HIPCK(hipEventSynchronize(c->ev[1]));
         *ms_out = ms_between(c->ev[0], c->ev[1]);
     }
+    return RC_OK;
+}
+
+// ------------------------------------------------------------------------------------ compact snapshot wire format
+static_assert(sizeof(rc_snap_rec) == 24 && sizeof(rc_snaprec) == 24, "snapshot records are 24 bytes");
+
+extern "C" rc_status rc_cache_snapshot_save(rc_cache* c, rc_snap_rec* host_records, size_t capacity, size_t* count_out) {
+    if (!c || !count_out || (capacity > 0 && !host_records)) return fail(RC_ERR_ARG, "NULL argument");
+    if (!c->begun) return fail(RC_ERR_STATE, "rc_cache_begin not called");
+    rc_scene* S = c->S;
+    const size_t Np = S->H.Np;
+    rc_snaprec* tmp = nullptr;
+    HIPCK(hipMalloc((void**)&tmp, Np * sizeof(rc_snaprec)));
+    int cnt = 0;
+    hipError_t e = snap_pack(c, tmp, &cnt);
+    const size_t n = std::min((size_t)cnt, capacity);
+    if (e == hipSuccess && n) e = hipMemcpy(host_records, tmp, n * sizeof(rc_snaprec), hipMemcpyDeviceToHost);
+    (void)hipFree(tmp);
+    if (e != hipSuccess) return fail(RC_ERR_HIP, "snapshot save: %s", hipGetErrorString(e));
+    *count_out = n;
+    return RC_OK;
+}
+
+extern "C" rc_status rc_cache_snapshot_load(rc_cache* c, const rc_snap_rec* host_records, size_t count) {
+    if (!c || (count > 0 && !host_records)) return fail(RC_ERR_ARG, "NULL argument");
+    if (!c->begun) return fail(RC_ERR_STATE, "rc_cache_begin not called");
+    rc_scene* S = c->S;
+    const int Np = S->H.Np, nl = S->H.nl;
+    if (count > (size_t)Np) return fail(RC_ERR_ARG, "more records (%zu) than entries (%d)", count, Np);
+    hipStream_t st = S->stream;
+    rc_snaprec* tmp = nullptr;
+    HIPCK(hipMalloc((void**)&tmp, std::max<size_t>(count, 1) * sizeof(rc_snaprec)));
+    hipError_t e = count ? hipMemcpyAsync(tmp, host_records, count * sizeof(rc_snaprec), hipMemcpyHostToDevice, st) : hipSuccess;
+    if (e == hipSuccess) e = rc_launch_flush(&c->c, Np, nl, st);
+    if (e == hipSuccess) e = rc_launch_snap_restore(&c->c, Np, nl, S->d_ekeys, tmp, (int)count, 0, c->dormant, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    (void)hipFree(tmp);
+    if (e != hipSuccess) return fail(RC_ERR_HIP, "snapshot load: %s", hipGetErrorString(e));
+    c->r_valid = false;
     return RC_OK;
 }

@@ -56,3 +56,8 @@ hipError_t rc_launch_mlp_samples(int trav, const rc_dscene* s, const rc_dstate* 
                                  float4* sN, float4* sD, float4* hP, float4* hN, float4* hD, float2* AE, int* ent, hipStream_t stream);
 hipError_t rc_launch_mlp_targets(int n, const float4* hP, const float* hy, const float2* AE, float* t, hipStream_t stream);
 hipError_t rc_launch_mlp_judge(int n, const float* y, const float* R, const int* ent, float* acc, hipStream_t stream);
+hipError_t rc_launch_snap_count(const rc_dcache* c, int Np, int* blockcnt, hipStream_t stream);
+hipError_t rc_launch_snap_pack(const rc_dcache* c, int Np, int nl, const uint64_t* ekeys, const int* blockoff, rc_snaprec* out,
+                               hipStream_t stream);
+hipError_t rc_launch_snap_restore(const rc_dcache* c, int Np, int nl, const uint64_t* ekeys, const rc_snaprec* rec, int count, int verify,
+                                  uint64_t dormant, hipStream_t stream);

@@ -388,6 +388,54 @@ __global__ void __launch_bounds__(RC_TB) k_invalidate(rc_dcache c, int Np, int n
 }
 
 // whole-cache flush (event to a new state, rtxgi_bench.patch:363-366)
+// Compact snapshots. Entries with samples (n > 0) become 24-byte records in entry order: per
+// block count, host prefix, then each block writes its entries at its offset (wave ballots).
+__global__ void __launch_bounds__(RC_TB) k_snap_count(rc_dcache c, int Np, int* blockcnt) {
+    __shared__ int s_w[RC_TB / 64];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const unsigned long long m = __ballot(i < Np && c.n[i] > 0.f);
+    if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = __popcll(m);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int t = 0;
+        for (int k = 0; k < RC_TB / 64; ++k) t += s_w[k];
+        blockcnt[blockIdx.x] = t;
+    }
+}
+__global__ void __launch_bounds__(RC_TB) k_snap_pack(rc_dcache c, int Np, int nl, const uint64_t* ekeys, const int* blockoff, rc_snaprec* out) {
+    __shared__ int s_w[RC_TB / 64];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const bool v = i < Np && c.n[i] > 0.f;
+    const unsigned long long m = __ballot(v);
+    if (lane == 0) s_w[wv] = __popcll(m);
+    __syncthreads();
+    if (!v) return;
+    int before = 0;
+    for (int k = 0; k < wv; ++k) before += s_w[k];
+    const int slot = blockoff[blockIdx.x] + before + __popcll(m & ((1ull << lane) - 1ull));
+    const float* row = c.Lc + (int64_t)nl * i;
+    float L = 0.f;
+    for (int l = 0; l < nl; ++l) L += row[l];
+    rc_snaprec r;
+    r.index = (uint32_t)i; r.n = c.n[i]; r.key = ekeys[i]; r.radiance = L; r.age = c.age[i];
+    out[slot] = r;
+}
+// restore after a flush: record j's entry (if its key matches) gets its radiance in channel 0,
+// its history and age; verify opens the verification window on it
+__global__ void __launch_bounds__(RC_TB) k_snap_restore(rc_dcache c, int Np, int nl, const uint64_t* ekeys, const rc_snaprec* rec, int count,
+                                                       int verify, uint64_t dormant) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= count) return;
+    const rc_snaprec r = rec[j];
+    const int i = (int)r.index;
+    if (r.index >= (uint32_t)Np || ekeys[i] != r.key) return;
+    float* row = c.Lc + (int64_t)nl * i;
+    row[0] = r.radiance;
+    for (int l = 1; l < nl; ++l) row[l] = 0.f;
+    c.n[i] = r.n; c.age[i] = r.age; c.pending[i] = 0;
+    if (verify) { c.pending[i] = 1; c.c0[i] = rc_est_of(c.Lc, i, nl, dormant); atomicAdd(&c.npend[1], 1); }
+}
+
 __global__ void __launch_bounds__(RC_TB) k_flush(rc_dcache c, int Np, int nl) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= Np) return;
@@ -1006,5 +1054,21 @@ hipError_t rc_launch_query_staged(const rc_dscene* s, const rc_dstate* st, const
         hipLaunchKernelGGL(k_q_mlp_add, dim3(rc_grid(npath)), dim3(RC_TB), 0, stream, *w, npath);
     }
     hipLaunchKernelGGL(k_q_pixels, dim3(rc_grid((long long)cam->W * cam->H)), dim3(RC_TB), 0, stream, *cam, *w, img);
+    return hipGetLastError();
+}
+
+hipError_t rc_launch_snap_count(const rc_dcache* c, int Np, int* blockcnt, hipStream_t stream) {
+    hipLaunchKernelGGL(k_snap_count, dim3(rc_grid(Np)), dim3(RC_TB), 0, stream, *c, Np, blockcnt);
+    return hipGetLastError();
+}
+hipError_t rc_launch_snap_pack(const rc_dcache* c, int Np, int nl, const uint64_t* ekeys, const int* blockoff, rc_snaprec* out,
+                               hipStream_t stream) {
+    hipLaunchKernelGGL(k_snap_pack, dim3(rc_grid(Np)), dim3(RC_TB), 0, stream, *c, Np, nl, ekeys, blockoff, out);
+    return hipGetLastError();
+}
+hipError_t rc_launch_snap_restore(const rc_dcache* c, int Np, int nl, const uint64_t* ekeys, const rc_snaprec* rec, int count, int verify,
+                                  uint64_t dormant, hipStream_t stream) {
+    if (count <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_snap_restore, dim3(rc_grid(count)), dim3(RC_TB), 0, stream, *c, Np, nl, ekeys, rec, count, verify, dormant);
     return hipGetLastError();
 }

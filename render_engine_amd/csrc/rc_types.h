@@ -93,6 +93,7 @@ typedef struct rc_dcache {
     long long* dstat;            // [3 NR] fixed point (RC_FX_STAT)
     uint64_t* stale;             // [Np] bit l = dormant channel l is out of date here (l < 64)
     float* snap;                 // [KSNAP NL Np]
+    void* csnap;                 // [KSNAP][Np] compact snapshot records (24 B)
     float* age;                  // [Np]
     long long* sumpr;            // [2] fixed point (RC_FX_PR): current, next
     int32_t* npend;              // [2]
@@ -175,3 +176,6 @@ typedef struct rc_mlpbuf {
     float* loss;                   // [1]
     int32_t cap, pad0;             // padded batch capacity
 } rc_mlpbuf;
+
+// compact snapshot record, 24 bytes (the C ABI's rc_snap_rec)
+typedef struct rc_snaprec { uint32_t index; float n; uint64_t key; float radiance; float age; } rc_snaprec;

@@ -137,6 +137,7 @@ RUN_METHODS = [
     ("targeted", lambda: rea.methods.targeted()),
     ("targeted_grad", lambda: rea.methods.targeted_grad(rays=3)),
     ("flush_snapshots", lambda: rea.methods.flush_snapshots()),
+    ("flush_snapshots_compact", lambda: rea.methods.flush_snapshots_compact()),
 ] + [(n, (lambda m=m: m)) for n, m in rea.methods.timing_list()] + [
     (n, (lambda m=m: m)) for n, m in rea.methods.scenario_list(dyn=0b11) if "Oracle" not in n]
 
@@ -292,3 +293,41 @@ def test_query_any_spp(torch_gpu, room):
         exp = np.zeros(cam.W * cam.H, np.float32)
         po.lib.orc_k_query(osc.state(0), 1, po.ptr(R), C.byref(cam), 8, 0, 0, po.ptr(exp), None, 0, cam.W * cam.H)
         assert np.array_equal(img.cpu().numpy().view(np.uint32), exp.view(np.uint32)), f"spp {spp}"
+
+
+def test_snapshot_records_round_trip(torch_gpu, street):
+    """24-byte records of the entries with samples, in entry order, radiance = the channel sum in
+    light order, keys = the hash-grid keys; loading them into another cache restores channel 0,
+    history and age after a flush, and skips records whose key does not match."""
+    scene, osc = street
+    i = scene.info
+    cache = rea.Cache(scene, rea.methods.plain())
+    cache.begin(0, 4)
+    for _ in range(5):
+        cache.frame(0)
+    n, lc, age = cache.get("N"), cache.get("LC").reshape(i.Np, i.nl), cache.get("AGE")
+    rec = cache.snapshot_save()
+    assert rea.SNAP_REC.itemsize == 24
+    valid = np.nonzero(n > 0)[0]
+    assert np.array_equal(rec["index"], valid)
+    s = np.zeros(i.Np, np.float32)
+    for l in range(i.nl):
+        s = (s + lc[:, l]).astype(np.float32)
+    assert np.array_equal(rec["radiance"].view(np.uint32), s[valid].view(np.uint32))
+    assert np.array_equal(rec["n"], n[valid]) and np.array_equal(rec["age"], age[valid])
+    ek = (np.uint64(1) << np.uint64(63)) | np.arange(i.Np, dtype=np.uint64)
+    HK, HV = scene.array("HK"), scene.array("HV")
+    occ = HK != 0
+    ek[HV[occ]] = HK[occ]
+    assert np.array_equal(rec["key"], ek[valid])
+    other = rea.Cache(scene, rea.methods.plain())
+    other.begin(1, 9)
+    other.frame(1)
+    bad = rec.copy()
+    bad["key"][:10] ^= np.uint64(1)          # ten records that no longer match their entry
+    other.snapshot_load(bad)
+    n2, lc2, age2 = other.get("N"), other.get("LC").reshape(i.Np, i.nl), other.get("AGE")
+    ok = valid[10:]
+    assert np.array_equal(lc2[ok, 0].view(np.uint32), s[ok].view(np.uint32))
+    assert np.all(lc2[ok, 1:] == 0) and np.array_equal(n2[ok], n[ok]) and np.array_equal(age2[ok], age[ok])
+    assert np.all(n2[valid[:10]] == 0) and np.all(lc2[valid[:10]] == 0)
