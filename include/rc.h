@@ -113,6 +113,12 @@ rc_status rc_scene_street(uint64_t seed, int nboxes, int target_tris, int nlamps
 /* Prototype scene file, format 2 (box scenes) or 3 (mesh scenes), uncompressed (cuda/poc_gpu.cu:751-801).
  * Converged references stored in the file become the scene's references. */
 rc_status rc_scene_load(const char* path, rc_scene** out);
+/* Bench scene around an MSH1 mesh (the prep mode's scene, cuda/prep_mesh.cuh:303-453): 3 point lamps and a cube
+ * at open points, 5 states, cells toward target_points; cube_m (may be NULL) = the cube's side. */
+rc_status rc_scene_mesh(const char* msh_path, int target_points, rc_scene** out, float* cube_m);
+/* Format-3 scene file readable by rc_scene_load (cuda/prep_mesh.cuh:242-297), with the converged references
+ * when the scene has them; point lights only. */
+rc_status rc_scene_save(const rc_scene* s, const char* path);
 void rc_scene_destroy(rc_scene* s);
 rc_status rc_scene_info_get(const rc_scene* s, rc_scene_info* out);
 /* byte size / host copy of a scene array (state: state index, or state pair a*NS+b for OCC,

@@ -95,6 +95,17 @@ class Scene:
                 os.unlink(tmp)
         return cls(h)
 
+    @classmethod
+    def mesh(cls, path, target_points=300000):
+        """bench scene around an MSH1 mesh (3 lamps, a cube, 5 states); returns (scene, cube side in m)"""
+        h, b = C.c_void_p(), C.c_float(0.0)
+        check(lib.rc_scene_mesh(path.encode(), target_points, C.byref(h), C.byref(b)))
+        return cls(h), b.value
+
+    def save(self, path):
+        """format-3 scene file (with the converged references when computed); point lights only"""
+        check(lib.rc_scene_save(self._h, path.encode()))
+
     def __del__(self, _destroy=lib.rc_scene_destroy):
         h = getattr(self, "_h", None)
         if h is not None and h.value:

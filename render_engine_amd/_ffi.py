@@ -68,6 +68,8 @@ _SIGS = {
     "rc_scene_street": (C.c_int, [C.c_uint64, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                   C.c_int, C.c_int, C.POINTER(_P)]),
     "rc_scene_load": (C.c_int, [C.c_char_p, C.POINTER(_P)]),
+    "rc_scene_mesh": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(_P), C.POINTER(C.c_float)]),
+    "rc_scene_save": (C.c_int, [_P, C.c_char_p]),
     "rc_scene_destroy": (None, [_P]),
     "rc_scene_info_get": (C.c_int, [_P, C.POINTER(SceneInfo)]),
     "rc_scene_array_bytes": (C.c_size_t, [_P, C.c_int, C.c_int]),

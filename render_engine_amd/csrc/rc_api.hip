@@ -112,6 +112,24 @@ extern "C" rc_status rc_scene_load(const char* path, rc_scene** out) {
     return RC_OK;
 }
 
+extern "C" rc_status rc_scene_mesh(const char* msh_path, int target_points, rc_scene** out, float* cube_m) {
+    if (!msh_path || !out) return fail(RC_ERR_ARG, "NULL argument");
+    if (target_points < 16) return fail(RC_ERR_ARG, "target_points must be >= 16");
+    rc::Mesh M;
+    std::string e = rc::read_msh(msh_path, M);
+    if (!e.empty()) return fail(RC_ERR_SCENE, "mesh %s: %s", msh_path, e.c_str());
+    return make_scene(rc::desc_from_mesh(M, target_points, cube_m), out);
+}
+
+extern "C" rc_status rc_scene_save(const rc_scene* s, const char* path) {
+    if (!s || !path) return fail(RC_ERR_ARG, "NULL argument");
+    rc::FileRefs R;
+    if (s->has_ref) { R.has = true; R.ref = s->ref; R.Pw = s->Pw; R.valid = s->valid; }
+    std::string e = rc::save_scene_file(s->H, R, path);
+    if (!e.empty()) return fail(RC_ERR_SCENE, "scene file %s: %s", path, e.c_str());
+    return RC_OK;
+}
+
 extern "C" void rc_scene_destroy(rc_scene* s) { delete s; }
 
 extern "C" rc_status rc_scene_info_get(const rc_scene* s, rc_scene_info* o) {

@@ -112,14 +112,54 @@ rc_status run(rc_cache* c, const std::vector<int>& seq, uint32_t seed, bool want
     return rc_cache_sync(c);
 }
 
+// prep: mesh (or the procedural room) -> bench scene with converged references, written as a format-3 file;
+// one "prep" record (cuda/prep_mesh.cuh:303-531, record :522-528)
+int prep_mode(int argc, char** argv) {
+    if (argc < 4) { fprintf(stderr, "usage: rc_bench prep <mesh.msh|room> <out.bin> [points]\n"); return 1; }
+    const double t0 = wall_ms();
+    const int target = argc > 4 ? atoi(argv[4]) : 300000;
+    rc_scene* S = nullptr;
+    float cube = 0.f;
+    if (!strcmp(argv[2], "room")) { if (rc_scene_box_room(1, 2000, &S) != RC_OK) return die("scene"); }
+    else if (rc_scene_mesh(argv[2], target, &S, &cube) != RC_OK) return die("mesh");
+    if (rc_scene_upload(S, 2) != RC_OK) return die("upload");
+    rc_scene_info info;
+    rc_scene_info_get(S, &info);
+    std::vector<float> noise(info.NS);
+    std::vector<int> iters(info.NS);
+    if (rc_scene_reference(S, 32, 16, 64, 512, 0.03f, noise.data(), iters.data()) != RC_OK) return die("reference");
+    double vfrac = 0;
+    for (int s = 0; s < info.NS; ++s) {
+        std::vector<uint8_t> v(info.Np);
+        if (rc_scene_array(S, RC_A_VALID, s, v.data(), v.size()) != RC_OK) return die("valid");
+        for (uint8_t x : v) vfrac += x;
+    }
+    vfrac /= (double)info.Np * info.NS;
+    if (rc_scene_save(S, argv[3]) != RC_OK) return die("save");
+    std::vector<float> L(3 * (size_t)info.nl);
+    rc_scene_array(S, RC_A_L, 0, L.data(), L.size() * 4);
+    printf("{\"type\":\"prep\",\"triangles\":%d,\"np\":%d,\"cellule_m\":%.4f,\"zones\":%d,\"diag_m\":%.2f,\"cube_m\":%.2f,\"bvh_noeuds\":%d,"
+           "\"points_valides\":%.4f,\"bruit_ref\":[", info.nst, info.Np, info.h, info.NR, info.diag, cube, info.nnodes, vfrac);
+    for (int k = 0; k < info.NS; ++k) printf("%s%.5f", k ? "," : "", noise[k]);
+    printf("],\"iterations_ref\":[");
+    for (int k = 0; k < info.NS; ++k) printf("%s%d", k ? "," : "", iters[k]);
+    printf("],\"lampes\":[");
+    for (int l = 0; l < info.nl; ++l) printf("%s[%.3f,%.3f,%.3f]", l ? "," : "", L[3 * l], L[3 * l + 1], L[3 * l + 2]);
+    printf("],\"secondes\":%.1f}\n", (wall_ms() - t0) / 1e3);
+    rc_scene_destroy(S);
+    return 0;
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
     if (argc < 3) {
-        fprintf(stderr, "usage: rc_bench {timing|scenf|res} <scene.bin|room|street> [nseeds scenarios.txt [budget]]\n");
+        fprintf(stderr, "usage: rc_bench {timing|scenf|res} <scene.bin|room|street> [nseeds scenarios.txt [budget]]\n"
+                        "       rc_bench prep <mesh.msh|room> <out.bin> [points]\n");
         return 1;
     }
     std::string mode = argv[1];
+    if (mode == "prep") return prep_mode(argc, argv);
     rc_scene* S = nullptr;
     if (open_scene(argv[2], &S) != RC_OK) return die("scene");
     if (rc_scene_upload(S, 2) != RC_OK) return die("upload");

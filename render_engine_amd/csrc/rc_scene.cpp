@@ -805,3 +805,151 @@ SceneDesc gen_street(uint64_t seed, int nboxes, int target_tris, int nlamps, boo
 }
 
 }  // namespace rc
+
+// ------------------------------------------------------------------------------------ scene files (writer, MSH1)
+namespace rc {
+namespace {
+struct Writer {
+    FILE* f = nullptr;
+    bool ok = true;
+    template <class T> void wr(const std::vector<T>& v) { if (ok && !v.empty() && fwrite(v.data(), sizeof(T), v.size(), f) != v.size()) ok = false; }
+    template <class T> void one(const T& x) { if (ok && fwrite(&x, sizeof(T), 1, f) != 1) ok = false; }
+    ~Writer() { if (f) fclose(f); }
+};
+}  // namespace
+
+std::string save_scene_file(const Scene& S, const FileRefs& R, const char* path) {
+    for (int l = 0; l < S.nl; ++l) if (S.ltype[l] != 0) return "the scene file holds point lights only";
+    for (int st = 1; st < S.NS; ++st) if (S.nt[st] != S.nt[0]) return "dynamic triangle counts differ between states";
+    if (S.BN.empty() || S.HK.empty()) return "scene has no BVH / hash table";
+    Writer w;
+    w.f = fopen(path, "wb");
+    if (!w.f) return std::string("cannot create ") + path;
+    const int Nt = S.NS ? S.nt[0] : 0;
+    w.wr(std::vector<int>{S.Np, S.Nq, Nt, S.NS, S.nl, S.NR, R.has ? 1 : 0, 3});
+    w.wr(std::vector<float>{S.lo[0], S.lo[1], S.lo[2]});
+    w.wr(std::vector<float>{S.ext[0], S.ext[1], S.ext[2]});
+    w.one(S.h);
+    w.wr(std::vector<int>{S.nst, (int)S.BN.size(), (int)S.HK.size(), S.npst});
+    w.wr(S.ST); w.wr(S.STN); w.wr(S.BN); w.wr(S.HK); w.wr(S.HV);
+    w.wr(S.L); w.wr(S.A); w.wr(S.region);
+    for (int st = 0; st < S.NS; ++st) {
+        w.wr(S.inten[st]); w.wr(S.P[st]); w.wr(S.N[st]); w.wr(S.Q[st]); w.wr(S.dims); w.wr(S.off); w.wr(S.T[st]); w.wr(S.TQ[st]);
+        if (R.has) { w.wr(R.ref[st]); w.wr(R.valid[st]); w.wr(R.Pw[st]); }
+    }
+    for (int e = 0; e < S.NS * S.NS; ++e) {
+        w.one((int)S.occ[e].size()); w.wr(S.occ[e]);
+        for (int l = 0; l < S.nl; ++l) { const auto& m = S.em[(size_t)e * S.nl + l]; w.one((int)m.size()); w.wr(m); }
+    }
+    return w.ok ? "" : std::string("write error on ") + path;
+}
+
+std::string read_msh(const char* path, Mesh& M) {
+    Reader in;
+    in.f = fopen(path, "rb");
+    if (!in.f) return std::string("cannot open ") + path;
+    std::vector<char> magic;
+    in.rd(magic, 4);
+    const int nt = in.one<int>();
+    if (!in.ok || memcmp(magic.data(), "MSH1", 4) != 0) return "not an MSH1 mesh";
+    if (nt <= 0 || nt > (1 << 27)) return "bad triangle count";
+    std::vector<float> V, alb; std::vector<uint8_t> ds;
+    in.rd(V, 9 * (size_t)nt); in.rd(alb, nt); in.rd(ds, nt);
+    if (!in.ok) return "truncated mesh";
+    M = Mesh();
+    for (int t = 0; t < nt; ++t) M.add_tri(&V[9 * (size_t)t], &V[9 * (size_t)t + 3], &V[9 * (size_t)t + 6], alb[t], ds[t] != 0);
+    return "";
+}
+
+SceneDesc desc_from_mesh(const Mesh& M, int target_points, float* cube_m) {
+    SceneDesc d;
+    d.mesh = M;
+    float lo[3] = {1e30f, 1e30f, 1e30f}, hi[3] = {-1e30f, -1e30f, -1e30f};
+    for (size_t k = 0; k < M.V.size(); ++k) { lo[k % 3] = std::min(lo[k % 3], M.V[k]); hi[k % 3] = std::max(hi[k % 3], M.V[k]); }
+    float ext[3], diag = 0.f;
+    for (int a = 0; a < 3; ++a) { ext[a] = std::max(hi[a] - lo[a], 1e-3f); diag += ext[a] * ext[a]; }
+    diag = std::sqrt(diag);
+    // surface samples (triangle centroids, at most 200k) in a uniform grid for nearest-distance queries
+    const int G = 48;
+    std::vector<std::vector<int>> grid((size_t)G * G * G);
+    std::vector<float> pts;
+    const int nt = M.nt(), stride = std::max(1, nt / 200000);
+    auto cell = [&](const float* p, int a) { return std::min(G - 1, std::max(0, (int)((p[a] - lo[a]) / ext[a] * G))); };
+    for (int t = 0; t < nt; t += stride) {
+        float c[3];
+        for (int a = 0; a < 3; ++a) c[a] = (M.V[9 * (size_t)t + a] + M.V[9 * (size_t)t + 3 + a] + M.V[9 * (size_t)t + 6 + a]) / 3.f;
+        grid[((size_t)cell(c, 2) * G + cell(c, 1)) * G + cell(c, 0)].push_back((int)(pts.size() / 3));
+        pts.insert(pts.end(), c, c + 3);
+    }
+    auto openness = [&](const float* p) {   // distance to the nearest sample, searched ring by ring
+        int c0 = cell(p, 0), c1 = cell(p, 1), c2 = cell(p, 2);
+        float best = 1e30f;
+        for (int r = 0; r < G; ++r) {
+            for (int z = std::max(0, c2 - r); z <= std::min(G - 1, c2 + r); ++z)
+                for (int y = std::max(0, c1 - r); y <= std::min(G - 1, c1 + r); ++y)
+                    for (int x = std::max(0, c0 - r); x <= std::min(G - 1, c0 + r); ++x) {
+                        if (std::max(std::abs(x - c0), std::max(std::abs(y - c1), std::abs(z - c2))) != r) continue;
+                        for (int i : grid[((size_t)z * G + y) * G + x]) {
+                            float dx = pts[3 * i] - p[0], dy = pts[3 * i + 1] - p[1], dz = pts[3 * i + 2] - p[2];
+                            best = std::min(best, std::sqrt(dx * dx + dy * dy + dz * dz));
+                        }
+                    }
+            float reach = r * std::min(ext[0], std::min(ext[1], ext[2])) / G;
+            if (best < reach) break;
+        }
+        return best;
+    };
+    struct Cand { float p[3]; float open; };
+    std::vector<Cand> cands;
+    const float hz[3] = {0.15f, 0.35f, 0.6f};
+    for (int k = 0; k < 3; ++k)
+        for (int j = 0; j < 12; ++j)
+            for (int i = 0; i < 12; ++i) {
+                Cand c;
+                c.p[0] = lo[0] + ext[0] * (i + 0.5f) / 12.f; c.p[1] = lo[1] + ext[1] * (j + 0.5f) / 12.f; c.p[2] = lo[2] + ext[2] * hz[k];
+                c.open = openness(c.p);
+                cands.push_back(c);
+            }
+    std::stable_sort(cands.begin(), cands.end(), [](const Cand& a, const Cand& b) { return a.open > b.open; });
+    const size_t top = std::max<size_t>(3, cands.size() * 3 / 10);   // the most open 30 %
+    auto dist = [](const float* a, const float* b) { float s = 0; for (int x = 0; x < 3; ++x) s += (a[x] - b[x]) * (a[x] - b[x]); return std::sqrt(s); };
+    std::vector<int> lamps{0};
+    while (lamps.size() < 3) {   // farthest from the lamps placed, among the most open
+        int bi = -1; float bd = -1.f;
+        for (size_t c = 0; c < top; ++c) {
+            float dmin = 1e30f;
+            for (int l : lamps) dmin = std::min(dmin, dist(cands[c].p, cands[l].p));
+            if (dmin > bd) { bd = dmin; bi = (int)c; }
+        }
+        lamps.push_back(bi);
+    }
+    const float b = std::min(0.1f * diag, std::max(0.02f * diag, cands[top - 1].open));   // cube side
+    int box[2] = {-1, -1};
+    for (int k = 0; k < 2; ++k) {   // open points away from the lamps, the first near lamp 0, the second near lamp 1
+        float bd = 1e30f;
+        for (size_t c = 0; c < top; ++c) {
+            bool free = (int)c != box[0];
+            for (int l : lamps) if (dist(cands[c].p, cands[l].p) < 1.2f * b) free = false;
+            if (!free) continue;
+            float dl = dist(cands[c].p, cands[lamps[k]].p);
+            if (dl < bd) { bd = dl; box[k] = (int)c; }
+        }
+        if (box[k] < 0) box[k] = lamps[k];
+    }
+    d.nl = 3;
+    for (int l : lamps) { for (int x = 0; x < 3; ++x) d.L.push_back(cands[l].p[x]); d.ltype.push_back(0); }
+    const float I[5][3] = {{1, 1, 0}, {0, 1, 0}, {1, 1, 0}, {1, 1, 1}, {0, 1, 0}};
+    const int BP[5] = {0, 0, 1, 0, 1};
+    for (int s = 0; s < 5; ++s) {
+        d.inten.push_back(std::vector<float>(I[s], I[s] + 3));
+        const float* c = cands[box[BP[s]]].p;
+        d.objects.push_back({DynBox{{c[0], c[1], c[2]}, {0.5f * b, 0.5f * b, 0.5f * b}, true}});
+    }
+    d.h = 0.f;
+    d.target_points = target_points;
+    d.vx = 8; d.vy = 4; d.vz = 4;
+    d.fine = 64;
+    if (cube_m) *cube_m = b;
+    return d;
+}
+}  // namespace rc

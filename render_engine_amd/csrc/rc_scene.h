@@ -124,6 +124,15 @@ std::string prepare(const SceneDesc& d, Scene& S);
 // references (ref, valid, per-light Pw) are returned when the file holds them.
 struct FileRefs { bool has = false; std::vector<std::vector<float>> ref, Pw; std::vector<std::vector<uint8_t>> valid; };
 std::string load_scene_file(const char* path, Scene& S, FileRefs& R);
+// Format-3 scene file (the layout load_scene_file reads; cuda/prep_mesh.cuh:242-297): point lights only.
+std::string save_scene_file(const Scene& S, const FileRefs& R, const char* path);
+// MSH1 mesh file: "MSH1", int nt, float V[9 nt], float albedo[nt], uint8 double-sided[nt] (cuda/prep_mesh.cuh:20-30).
+std::string read_msh(const char* path, Mesh& M);
+// Bench scene around a mesh (the prep mode's scene, cuda/prep_mesh.cuh:303-453): 3 point lamps and a cube of
+// side cube_m at open points, 5 states (lamp intensities {1,1,0} {0,1,0} {1,1,0} {1,1,1} {0,1,0}, cube positions
+// 0 0 1 0 1), cache cells adjusted toward target_points. Open points: candidates on a grid at 15-60 % of the
+// height (z), scored by the distance to the nearest surface sample.
+SceneDesc desc_from_mesh(const Mesh& M, int target_points, float* cube_m);
 
 // Procedural generators.
 SceneDesc gen_box_room(uint64_t seed, int target_tris);

@@ -49,3 +49,21 @@ def test_scenf_matches_python_runner(torch_gpu):
     assert scen[0]["name"] == name
     errs = rea.Cache(s, m).run([0] * 12 + [1] * 10, 3000)
     assert np.allclose(scen[0]["curve"], np.round(errs, 5), atol=2e-5)
+
+
+def test_prep_record_and_file(torch_gpu, tmp_path):
+    """rc_bench prep: the prep record (cuda/prep_mesh.cuh:522-528 fields) and a format-3 file with the
+    converged references that rc_scene_load reads back"""
+    out = str(tmp_path / "room_prep.bin")
+    recs = _run(["prep", "room", out])
+    r = recs[-1]
+    assert r["type"] == "prep"
+    for k in ("triangles", "np", "cellule_m", "zones", "diag_m", "cube_m", "bvh_noeuds", "points_valides", "bruit_ref",
+              "iterations_ref", "lampes", "secondes"):
+        assert k in r, k
+    s = rea.Scene.load(out)
+    i = s.info
+    assert i.has_ref == 1 and i.Np == r["np"] and len(r["bruit_ref"]) == i.NS and len(r["lampes"]) == i.nl
+    valid = np.mean([s.array("VALID", k).mean() for k in range(i.NS)])
+    assert abs(valid - r["points_valides"]) < 1e-3
+    assert all(0 < x < 0.1 for x in r["bruit_ref"])

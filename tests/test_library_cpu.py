@@ -103,3 +103,61 @@ def test_street_events_and_quads():
     P3 = s.array("P", 3).reshape(-1, 3)[i.npst:]
     ob = s.array("OBJBOX", 3)
     assert np.all(P3 >= ob[:3] - 1e-4) and np.all(P3 <= ob[3:] + 1e-4)
+
+
+def _same(a, b):
+    return a.shape == b.shape and np.array_equal(a.view(np.uint8), b.view(np.uint8))
+
+
+def _check_round_trip(s, path):
+    s.save(path)
+    t = rea.Scene.load(path)
+    i, j = s.info, t.info
+    for f in ("Np", "npst", "nst", "NS", "nl", "NR", "Nq", "hsize"):
+        assert getattr(i, f) == getattr(j, f), f
+    for name in ("ST", "STN", "BN", "HK", "HV", "L", "ALB", "REGION", "DIMS", "OFF"):
+        assert _same(s.array(name), t.array(name)), name
+    for st in range(i.NS):
+        for name in ("P", "N", "Q", "T", "TQ", "INTEN"):
+            assert _same(s.array(name, st), t.array(name, st)), (name, st)
+    for e in range(i.NS * i.NS):
+        assert _same(s.array("OCC", e), t.array("OCC", e))
+        for l in range(i.nl):
+            assert _same(s.array("EM", e * i.nl + l), t.array("EM", e * i.nl + l))
+    return t
+
+
+def test_scene_file_round_trip(tmp_path):
+    """the format-3 writer and the reader agree on every array (cuda/prep_mesh.cuh:242-297 layout)"""
+    _check_round_trip(rea.Scene.box_room(seed=1, target_tris=600), str(tmp_path / "room.bin"))
+    street = rea.Scene.street(seed=3, nboxes=20, target_tris=2000, nlamps=2, sun=True, target_points=3000, table_log2=0,
+                              W=32, H=16, spp=1, bounces=1)
+    with pytest.raises(rea.RcError, match="point lights only"):
+        street.save(str(tmp_path / "street.bin"))
+
+
+def test_mesh_scene(tmp_path):
+    """an MSH1 mesh becomes the prep mode's bench scene: 3 point lamps and a cube inside the bounds,
+    5 states with the prep mode's intensities and cube positions, quads of the cube per state"""
+    room = rea.Scene.box_room(seed=2, target_tris=800)
+    V, alb, ds = room.array("MESH_V"), room.array("MESH_ALB"), room.array("MESH_DS")
+    nt = len(alb)
+    path = str(tmp_path / "room.msh")
+    with open(path, "wb") as f:
+        f.write(b"MSH1")
+        f.write(np.int32(nt).tobytes())
+        f.write(V.astype(np.float32).tobytes()); f.write(alb.astype(np.float32).tobytes()); f.write(ds.astype(np.uint8).tobytes())
+    s, cube = rea.Scene.mesh(path, target_points=20000)
+    i = s.info
+    assert i.nl == 3 and i.NS == 5 and i.Nq == 6 and cube > 0
+    I = np.array([s.array("INTEN", k) for k in range(5)])
+    assert np.array_equal(I, [[1, 1, 0], [0, 1, 0], [1, 1, 0], [1, 1, 1], [0, 1, 0]])
+    lo, hi = V.reshape(-1, 3).min(0), V.reshape(-1, 3).max(0)
+    L = s.array("L").reshape(3, 3)
+    assert np.all(L >= lo) and np.all(L <= hi)
+    q0, q2 = s.array("Q", 0).reshape(6, 12), s.array("Q", 2).reshape(6, 12)
+    assert not np.array_equal(q0, q2) and np.array_equal(q0, s.array("Q", 1).reshape(6, 12))
+    assert len(s.array("OCC", 0 * 5 + 2)) > 0 and len(s.array("OCC", 0 * 5 + 1)) == 0
+    _check_round_trip(s, str(tmp_path / "mesh.bin"))
+    with pytest.raises(rea.RcError):
+        rea.Scene.mesh(str(tmp_path / "room.bin"))
