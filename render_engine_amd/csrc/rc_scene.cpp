@@ -79,7 +79,10 @@ void build_bvh(const Mesh& M, std::vector<rc_bvh_node>& nodes, std::vector<int>&
     nodes.push_back(rc_bvh_node{});
     struct Item { int node, start, count, depth; };
     std::vector<Item> work{{0, 0, n, 0}};
-    const int NB = 16;
+    // 32 centroid bins on each of the three axes; the cheapest split over all of them wins.
+    // Leaves hold at most LEAF triangles: the wide traversal's triangle tests cost more than its
+    // box tests, so single-triangle leaves (the SAH splits every pair) trace fastest.
+    const int NB = 32, LEAF = 2;
     depth_out = 0;
     while (!work.empty()) {
         Item it = work.back(); work.pop_back();
@@ -97,17 +100,19 @@ void build_bvh(const Mesh& M, std::vector<rc_bvh_node>& nodes, std::vector<int>&
         for (int a = 0; a < 3; ++a) { nd.lo[a] = nlo[a]; nd.hi[a] = nhi[a]; }
         int ax = 0;
         for (int a = 1; a < 3; ++a) if (chi[a] - clo[a] > chi[ax] - clo[ax]) ax = a;
-        const float ext = chi[ax] - clo[ax];
         auto make_leaf = [&]() { nodes[it.node].lf = it.start; nodes[it.node].cnt = it.count; };
-        if ((it.count <= 4 || it.depth >= 56 || ext < 1e-7f) && (it.count <= 64 || it.depth >= 56)) { make_leaf(); continue; }
-        int mid = -1;
-        if (ext >= 1e-7f) {
+        if ((it.count <= LEAF || it.depth >= 56 || chi[ax] - clo[ax] < 1e-7f) && (it.count <= 64 || it.depth >= 56)) { make_leaf(); continue; }
+        auto bin_of = [&](int t, int a) {
+            return std::min(NB - 1, (int)((cen[3 * (size_t)t + a] - clo[a]) / (chi[a] - clo[a]) * NB));
+        };
+        float best = 1e30f; int bs = -1, bax = -1;
+        for (int a0 = 0; a0 < 3; ++a0) {
+            if (chi[a0] - clo[a0] < 1e-7f) continue;
             int bc[NB] = {0};
             float blo[NB][3], bhi[NB][3];
             for (int b = 0; b < NB; ++b) for (int a = 0; a < 3; ++a) { blo[b][a] = 1e30f; bhi[b][a] = -1e30f; }
-            auto bin_of = [&](int t) { return std::min(NB - 1, (int)((cen[3 * (size_t)t + ax] - clo[ax]) / ext * NB)); };
             for (int k = it.start; k < it.start + it.count; ++k) {
-                int t = order[k], b = bin_of(t);
+                int t = order[k], b = bin_of(t, a0);
                 bc[b]++;
                 for (int a = 0; a < 3; ++a) { blo[b][a] = std::min(blo[b][a], tlo[3 * (size_t)t + a]); bhi[b][a] = std::max(bhi[b][a], thi[3 * (size_t)t + a]); }
             }
@@ -125,19 +130,19 @@ void build_bvh(const Mesh& M, std::vector<rc_bvh_node>& nodes, std::vector<int>&
                 }
                 sc[b] = (b < NB - 1 ? sc[b + 1] : 0) + bc[b];
             }
-            float best = 1e30f; int bs = -1;
             for (int s = 1; s < NB; ++s) {
                 int nl = pc[s - 1], nr = sc[s];
                 if (!nl || !nr) continue;
                 float c = nl * area_of(plo[s - 1], phi[s - 1]) + nr * area_of(slo[s], shi[s]);
-                if (c < best) { best = c; bs = s; }
+                if (c < best) { best = c; bs = s; bax = a0; }
             }
-            float leaf_cost = it.count * area_of(nlo, nhi);
-            if (bs > 0 && !(it.count <= 4 && best >= leaf_cost)) {
-                auto pm = std::partition(order.begin() + it.start, order.begin() + it.start + it.count, [&](int t) { return bin_of(t) < bs; });
-                mid = (int)(pm - order.begin());
-            } else if (it.count <= 4) { make_leaf(); continue; }
         }
+        int mid = -1;
+        const float leaf_cost = it.count * area_of(nlo, nhi);
+        if (bs > 0 && !(it.count <= LEAF && best >= leaf_cost)) {
+            auto pm = std::partition(order.begin() + it.start, order.begin() + it.start + it.count, [&](int t) { return bin_of(t, bax) < bs; });
+            mid = (int)(pm - order.begin());
+        } else if (it.count <= LEAF) { make_leaf(); continue; }
         if (mid <= it.start || mid >= it.start + it.count) {
             mid = it.start + it.count / 2;
             std::nth_element(order.begin() + it.start, order.begin() + mid, order.begin() + it.start + it.count,

@@ -101,8 +101,10 @@ struct Scene {
 // disappears between states a and b (both placements), for every ordered pair.
 void fine_events(Scene& S, int g);
 
-// Binned-SAH binary BVH over the mesh (16 bins, leaves of <= 4 triangles or when SAH says so,
-// median split fallback, depth <= 56; cuda/prep_mesh.cuh:42-114).
+// Binned-SAH binary BVH over the mesh: 32 bins on each of the three axes (the reference bins
+// 16 along the longest one, cuda/prep_mesh.cuh:42-114), leaves of <= 2 triangles or when SAH
+// says so, median split fallback, depth <= 56. Hits do not depend on the tree's shape: the
+// closest-hit rule orders ties by triangle index.
 void build_bvh(const Mesh& M, std::vector<rc_bvh_node>& nodes, std::vector<int>& order, int& depth);
 // Collapse of the binary BVH into 4-wide nodes (children: up to 4 descendants, leaves keep
 // their triangle ranges).
