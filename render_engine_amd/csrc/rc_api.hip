@@ -453,7 +453,7 @@ extern "C" rc_status rc_cache_create(rc_scene* s, const rc_method* m, rc_cache**
     if (m->snaps && !m->compact) CK2(dalloc(&d.snap, (size_t)RC_KSNAP * nl * Np, c->owned));
     if (m->snaps && m->compact) { rc_snaprec* cs = nullptr; CK2(dalloc(&cs, (size_t)RC_KSNAP * Np, c->owned)); d.csnap = cs; }
     CK2(dalloc(&c->d_blk, (size_t)(Np + 255) / 256, c->owned));
-    CK2(dalloc(&d.sumpr, 2, c->owned)); CK2(dalloc(&d.npend, 2, c->owned)); CK2(dalloc(&d.err, 3, c->owned));
+    CK2(dalloc(&d.sumpr, 2, c->owned)); CK2(dalloc(&d.prpart, (size_t)(Np + 63) / 64, c->owned)); CK2(dalloc(&d.npend, 2, c->owned)); CK2(dalloc(&d.err, 3, c->owned));
     CK2(dalloc(&c->d_counters, 2, c->owned));
     for (auto& e : c->ev) CK2(hipEventCreate(&e));
 #undef CK2
@@ -658,7 +658,6 @@ static rc_status run_event(rc_cache* c, int e, int si, int& invalidated) {
     }
     c->last_event_f = f;
     if (M.mode == 1) {   // ray allocation must see this frame's invalidation
-        HIPCK(hipMemsetAsync(d.sumpr, 0, 8, st));
         HIPCK(rc_launch_sumpr(&d, Np, st));
     }
     return RC_OK;
@@ -707,9 +706,7 @@ extern "C" rc_status rc_cache_frame(rc_cache* c, int si, int want_err, int timed
         c->pend_left--;
     }
     if (timed) HIPCK(hipEventRecord(c->ev[3], st));
-    HIPCK(hipMemsetAsync(d.sumpr + 1, 0, 8, st));
-    HIPCK(rc_launch_update(&S->ds, &S->dst[si], &d, &p, track ? c->lnew : nullptr, st));
-    HIPCK(hipMemcpyAsync(d.sumpr, d.sumpr + 1, 8, hipMemcpyDeviceToDevice, st));
+    HIPCK(rc_launch_update(&S->ds, &S->dst[si], &d, &p, track ? c->lnew : nullptr, st));   // also sets sumpr[0] = sumpr[1]
     std::swap(d.Lc, d.Lc2);
     c->r_valid = true;
     if (timed) {

@@ -201,57 +201,78 @@ __device__ __forceinline__ void rc_trace_bvh_closest(f3 o, f3 d, const rc_dscene
     }
 }
 
+// Compare-exchange of (distance, node) pairs by selects (no branch: five VALU operations).
 __device__ __forceinline__ void rc_cswap(float& ta, int& ia, float& tb, int& ib) {
-    if (tb < ta) { float t = ta; ta = tb; tb = t; int i = ia; ia = ib; ib = i; }
+    const bool c = tb < ta;
+    const float lo = c ? tb : ta, hi = c ? ta : tb;
+    const int il = c ? ib : ia, ih = c ? ia : ib;
+    ta = lo; tb = hi; ia = il; ib = ih;
 }
 
 // Stack entries of the wide traversal are 32-bit: node index << 8 | an 8-bit code of the
 // node's entry distance t. The code keeps t's exponent and top 3 mantissa bits (t >= 2^-15),
 // and decodes to a lower bound of t (0 for code 0), so culling a popped node whose bound
 // exceeds the current hit never drops a node the exact distance would have kept.
+// Only called with t >= 0 (+0 or -0 included): the field extract drops the sign bit, so both
+// zeros give code 0; then one subtract and one clamp.
 __device__ __forceinline__ uint32_t rc_tcode(float t) {
-    if (!(t > 0.f)) return 0u;   // +0, -0: lower bound 0
-    int c = (int)(__float_as_uint(t) >> 20) - (112 << 3) + 1;
-    return c < 1 ? 0u : (c > 255 ? 255u : (uint32_t)c);
+    const int c = (int)__builtin_amdgcn_ubfe(__float_as_uint(t), 20, 11) - (112 << 3) + 1;
+    return (uint32_t)min(max(c, 0), 255);
 }
 __device__ __forceinline__ float rc_tlb(uint32_t code) {
     return code == 0u ? 0.f : __uint_as_float(((code - 1u) + (112u << 3)) << 20);
 }
 
 // Per-ray constants of the wide traversal: origin, direction, reciprocal direction and the
-// rows of a node that hold each axis's near planes (lo when the direction is positive, hi
-// when negative; node rows: 0-2 lo x/y/z, 3-5 hi x/y/z, 6 children, 7 leaf counts).
-struct rc_wray { f3 o, d, inv; int nx, ny, nz; };
+// byte offsets, inside a 128-byte node, of the rows that hold each axis's near and far planes
+// (near = lo when the direction is positive, hi when negative; node rows of 16 B: 0-2 lo
+// x/y/z, 3-5 hi x/y/z, 6 children, 7 leaf counts).
+struct rc_wray { f3 o, d, inv; uint32_t nx, ny, nz, fx, fy, fz; };
 __device__ __forceinline__ rc_wray rc_wray_make(f3 o, f3 d) {
     rc_wray r;
     r.o = o; r.d = d; r.inv = rc_inv_dir(d);
-    r.nx = r.inv.x < 0.f ? 3 : 0; r.ny = r.inv.y < 0.f ? 4 : 1; r.nz = r.inv.z < 0.f ? 5 : 2;
+    r.nx = r.inv.x < 0.f ? 48u : 0u; r.ny = r.inv.y < 0.f ? 64u : 16u; r.nz = r.inv.z < 0.f ? 80u : 32u;
+    r.fx = 48u - r.nx; r.fy = 80u - r.ny; r.fz = 112u - r.nz;
     return r;
+}
+
+// One 16-byte row of a wide node: the scene's node array (uniform) plus a 32-bit byte offset,
+// so the load takes the scalar-base + 32-bit vector-offset form (no 64-bit address math).
+__device__ __forceinline__ float4 rc_wrow(const rc_dscene& s, uint32_t off) {
+    return *(const float4*)((const char*)s.WN + off);
+}
+__device__ __forceinline__ int4 rc_wrow_i(const rc_dscene& s, uint32_t off) {
+    return *(const int4*)((const char*)s.WN + off);
 }
 
 // Entry distances of a node's four child boxes against [0, tmax] (1e30: missed or empty
 // slot), branch-free. With lo <= hi, (lo - o) * inv and (hi - o) * inv are ordered by the sign
 // of inv, so the near-plane value is exactly the per-axis minimum of rc_box_enter and the
-// result equals rc_box_enter's bit for bit.
-__device__ __forceinline__ void rc_child_enter(const rc_wray& r, const float4* nb, int4 ch, float tmax, float& t0, float& t1, float& t2,
-                                               float& t3) {
-    const float4 nx = nb[r.nx], ny = nb[r.ny], nz = nb[r.nz];
-    const float4 fx = nb[3 - r.nx], fy = nb[5 - r.ny], fz = nb[7 - r.nz];
-#define RC_SLAB(tc, c, id)                                                                                      \
+// result equals rc_box_enter's bit for bit. Empty slots hold lo = +inf, hi = -inf: their
+// near value is +inf and their far value -inf whatever the direction, so they always miss.
+__device__ __forceinline__ void rc_child_enter(const rc_wray& r, const rc_dscene& s, uint32_t node, float tmax, float& t0, float& t1,
+                                               float& t2, float& t3) {
+    const float4 nx = rc_wrow(s, node + r.nx), ny = rc_wrow(s, node + r.ny), nz = rc_wrow(s, node + r.nz);
+    const float4 fx = rc_wrow(s, node + r.fx), fy = rc_wrow(s, node + r.fy), fz = rc_wrow(s, node + r.fz);
+#define RC_SLAB(tc, c)                                                                                          \
     {                                                                                                           \
         const float tn = fmaxf(fmaxf((nx.c - r.o.x) * r.inv.x, (ny.c - r.o.y) * r.inv.y), fmaxf((nz.c - r.o.z) * r.inv.z, 0.f)); \
         const float tf = fminf(fminf((fx.c - r.o.x) * r.inv.x, (fy.c - r.o.y) * r.inv.y), fminf((fz.c - r.o.z) * r.inv.z, tmax)); \
-        tc = (id != -1 && tn <= tf) ? tn : 1e30f;                                                               \
+        tc = tn <= tf ? tn : 1e30f;                                                                             \
     }
-    RC_SLAB(t0, x, ch.x) RC_SLAB(t1, y, ch.y) RC_SLAB(t2, z, ch.z) RC_SLAB(t3, w, ch.w)
+    RC_SLAB(t0, x) RC_SLAB(t1, y) RC_SLAB(t2, z) RC_SLAB(t3, w)
 #undef RC_SLAB
 }
 
 // Leaf children of a node as up to four triangle ranges [a, b), walked by one flat loop so a
-// wave iterates as often as its busiest lane (not once per child slot). Empty ranges: a = b.
-#define RC_LEAF_RANGE(tc, ic, cnt, a, b)          \
-    if (tc < 1e30f && ic <= -2) {                 \
-        a = -2 - ic; b = a + (cnt); tc = 1e30f;   \
+// wave iterates as often as its busiest lane (not once per child slot). A leaf child stores
+// its first triangle and count (> 0); inner and empty slots have count 0, so their range is
+// empty. Leaf children leave the sort (distance set to 1e30).
+#define RC_LEAF_RANGE(tc, ic, cnt, a, b)             \
+    {                                                \
+        const bool leaf_ = tc < 1e30f && (cnt) > 0;  \
+        a = ic; b = ic + (leaf_ ? (cnt) : 0);        \
+        tc = leaf_ ? 1e30f : tc;                     \
     }
 #define RC_LEAF_WALK(BODY)                                                              \
     for (;;) {                                                                          \
@@ -270,10 +291,10 @@ __device__ __forceinline__ void rc_child_enter(const rc_wray& r, const float4* n
 // far-to-near, the nearest visited next. The stack lives in LDS, one column per lane
 // (stk[level * 256]), 32-bit entries node << 8 | code(t). Returns true when finished.
 __device__ __forceinline__ bool rc_closest_node(const rc_wray& r, const rc_dscene& s, float& tmin, int& q, int& cur, int& sp, uint32_t* stk) {
-    const float4* nb = (const float4*)(s.WN + cur);
-    const int4 ch = ((const int4*)nb)[6], cc = ((const int4*)nb)[7];
+    const uint32_t node = (uint32_t)cur << 7;
+    const int4 ch = rc_wrow_i(s, node + 96u), cc = rc_wrow_i(s, node + 112u);
     float t0, t1, t2, t3;
-    rc_child_enter(r, nb, ch, tmin, t0, t1, t2, t3);
+    rc_child_enter(r, s, node, tmin, t0, t1, t2, t3);
     int i0 = ch.x, i1 = ch.y, i2 = ch.z, i3 = ch.w;
     int a0 = 0, b0 = 0, a1 = 0, b1 = 0, a2 = 0, b2 = 0, a3 = 0, b3 = 0;
     RC_LEAF_RANGE(t0, i0, cc.x, a0, b0) RC_LEAF_RANGE(t1, i1, cc.y, a1, b1)
@@ -283,15 +304,13 @@ __device__ __forceinline__ bool rc_closest_node(const rc_wray& r, const rc_dscen
     rc_cswap(t0, i0, t1, i1); rc_cswap(t2, i2, t3, i3);
     rc_cswap(t0, i0, t2, i2); rc_cswap(t1, i1, t3, i3);
     rc_cswap(t1, i1, t2, i2);
-    int next = -1; float next_t = 0.f;
-#define RC_PUSH_OR_NEXT(tc, ic)                                                           \
-    if (tc < 1e30f && tc <= tmin) {                                                       \
-        if (next >= 0) { stk[sp * 256] = ((uint32_t)next << 8) | rc_tcode(next_t); ++sp; } \
-        next = ic; next_t = tc;                                                           \
-    }
-    RC_PUSH_OR_NEXT(t3, i3) RC_PUSH_OR_NEXT(t2, i2) RC_PUSH_OR_NEXT(t1, i1) RC_PUSH_OR_NEXT(t0, i0)
-#undef RC_PUSH_OR_NEXT
-    if (next >= 0) { cur = next; return false; }
+    // the children still in front of the current hit are a prefix of the sorted list: push
+    // all but the nearest far-to-near, visit the nearest next
+#define RC_PUSH(tc, ic) \
+    if (tc < 1e30f && tc <= tmin) { stk[sp * 256] = ((uint32_t)ic << 8) | rc_tcode(tc); ++sp; }
+    RC_PUSH(t3, i3) RC_PUSH(t2, i2) RC_PUSH(t1, i1)
+#undef RC_PUSH
+    if (t0 < 1e30f && t0 <= tmin) { cur = i0; return false; }
     while (sp > 0) {
         --sp;
         const uint32_t e = stk[sp * 256];
@@ -316,10 +335,10 @@ __device__ __forceinline__ void rc_trace_wide_closest(f3 o, f3 d, const rc_dscen
 // shading point, the likely ones, are found early; the answer does not depend on the order.
 // Returns true when finished (blocked set on a blocker).
 __device__ __forceinline__ bool rc_any_node(const rc_wray& r, const rc_dscene& s, float tlim, int& cur, int& sp, uint32_t* stk, bool& blocked) {
-    const float4* nb = (const float4*)(s.WN + cur);
-    const int4 ch = ((const int4*)nb)[6], cc = ((const int4*)nb)[7];
+    const uint32_t node = (uint32_t)cur << 7;
+    const int4 ch = rc_wrow_i(s, node + 96u), cc = rc_wrow_i(s, node + 112u);
     float t0, t1, t2, t3;
-    rc_child_enter(r, nb, ch, tlim, t0, t1, t2, t3);
+    rc_child_enter(r, s, node, tlim, t0, t1, t2, t3);
     int i0 = ch.x, i1 = ch.y, i2 = ch.z, i3 = ch.w;
     int a0 = 0, b0 = 0, a1 = 0, b1 = 0, a2 = 0, b2 = 0, a3 = 0, b3 = 0;
     RC_LEAF_RANGE(t0, i0, cc.x, a0, b0) RC_LEAF_RANGE(t1, i1, cc.y, a1, b1)
@@ -328,15 +347,12 @@ __device__ __forceinline__ bool rc_any_node(const rc_wray& r, const rc_dscene& s
     rc_cswap(t0, i0, t1, i1); rc_cswap(t2, i2, t3, i3);
     rc_cswap(t0, i0, t2, i2); rc_cswap(t1, i1, t3, i3);
     rc_cswap(t1, i1, t2, i2);
-    int next = -1;
-#define RC_ANYPUSH(tc, ic)                                                                \
-    if (tc < 1e30f) {                                                                     \
-        if (next >= 0) { stk[sp * 256] = (uint32_t)next << 8; ++sp; }                     \
-        next = ic;                                                                        \
-    }
-    RC_ANYPUSH(t3, i3) RC_ANYPUSH(t2, i2) RC_ANYPUSH(t1, i1) RC_ANYPUSH(t0, i0)
+    // entered children are a prefix of the sorted list: push all but the nearest far-to-near
+#define RC_ANYPUSH(tc, ic) \
+    if (tc < 1e30f) { stk[sp * 256] = (uint32_t)ic << 8; ++sp; }
+    RC_ANYPUSH(t3, i3) RC_ANYPUSH(t2, i2) RC_ANYPUSH(t1, i1)
 #undef RC_ANYPUSH
-    if (next >= 0) { cur = next; return false; }
+    if (t0 < 1e30f) { cur = i0; return false; }
     if (sp == 0) return true;
     --sp;
     cur = (int)(stk[sp * 256] >> 8);

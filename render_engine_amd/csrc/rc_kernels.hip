@@ -246,7 +246,7 @@ __global__ void __launch_bounds__(RC_TB) k_update(rc_dscene s, rc_dstate st, rc_
             if (lnew) { uint64_t x = lnew[i]; if (x) c.lcur[i] |= x; }
         }
         pr = rc_wave_sum_i64(pr);
-        if (t == 0) atomicAdd((unsigned long long*)&c.sumpr[1], (unsigned long long)pr);
+        if (t == 0) c.prpart[blockIdx.x] = pr;   // summed by k_prsum (no same-address atomics)
     }
     for (int e = t; e < nl * RC_UT; e += RC_TB) {
         int l = e / RC_UT, j = e % RC_UT;
@@ -292,11 +292,38 @@ __global__ void __launch_bounds__(RC_TB) k_resolve(rc_dscene s, rc_dstate st, co
     R[i] = val;
 }
 
+// Fixed-point sum of 1/(n+1) over all entries, in two passes: per-block partials, then one
+// block adds them (k_prsum). Integer addition is exact, so the sum does not depend on the
+// order; it replaces one same-address device atomic per wave, which this multi-die part
+// serialises (~12 ns each: 100 us for 8k of them).
 __global__ void __launch_bounds__(RC_TB) k_sumpr(rc_dcache c, int Np) {
+    __shared__ long long s_w[RC_TB / 64];
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     long long pr = i < Np ? rc_fx(1.f / (c.n[i] + 1.f), RC_FX_PR) : 0;
     pr = rc_wave_sum_i64(pr);
-    if ((threadIdx.x & 63) == 0) atomicAdd((unsigned long long*)&c.sumpr[0], (unsigned long long)pr);
+    if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = pr;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        long long v = 0;
+        for (int k = 0; k < RC_TB / 64; ++k) v += s_w[k];
+        c.prpart[blockIdx.x] = v;
+    }
+}
+
+#define RC_PRB 1024
+__global__ void __launch_bounds__(RC_PRB) k_prsum(const long long* __restrict__ part, int n, long long* out0, long long* out1) {
+    __shared__ long long s_w[RC_PRB / 64];
+    long long v = 0;
+    for (int j = threadIdx.x; j < n; j += RC_PRB) v += part[j];
+    v = rc_wave_sum_i64(v);
+    if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        long long t = 0;
+        for (int k = 0; k < RC_PRB / 64; ++k) t += s_w[k];
+        *out0 = t;
+        if (out1) *out1 = t;
+    }
 }
 
 // restored snapshot: full confidence, optionally verified (cuda/poc_gpu.cu:612-617)
@@ -929,6 +956,7 @@ hipError_t rc_launch_update(const rc_dscene* s, const rc_dstate* st, const rc_dc
     dim3 g((p->Np + RC_UT - 1) / RC_UT), b(RC_TB);
     size_t shm = (size_t)2 * p->nl * RC_UT * sizeof(float);
     hipLaunchKernelGGL(k_update, g, b, shm, stream, *s, *st, *c, *p, lnew);
+    hipLaunchKernelGGL(k_prsum, dim3(1), dim3(RC_PRB), 0, stream, (const long long*)c->prpart, (int)g.x, c->sumpr + 1, c->sumpr);
     return hipGetLastError();
 }
 hipError_t rc_launch_resolve(const rc_dscene* s, const rc_dstate* st, const float* Lc, float* R, uint64_t dormant, hipStream_t stream) {
@@ -937,7 +965,10 @@ hipError_t rc_launch_resolve(const rc_dscene* s, const rc_dstate* st, const floa
     return hipGetLastError();
 }
 hipError_t rc_launch_sumpr(const rc_dcache* c, int Np, hipStream_t stream) {
-    hipLaunchKernelGGL(k_sumpr, dim3(rc_grid(Np)), dim3(RC_TB), 0, stream, *c, Np);
+    if (Np <= 0) return hipSuccess;
+    const int nb = rc_grid(Np);
+    hipLaunchKernelGGL(k_sumpr, dim3(nb), dim3(RC_TB), 0, stream, *c, Np);
+    hipLaunchKernelGGL(k_prsum, dim3(1), dim3(RC_PRB), 0, stream, (const long long*)c->prpart, nb, c->sumpr, (long long*)nullptr);
     return hipGetLastError();
 }
 hipError_t rc_launch_restored(const rc_dcache* c, int Np, int nl, float nmax, uint64_t dormant, int verify, hipStream_t stream) {

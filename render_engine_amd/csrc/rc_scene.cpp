@@ -2,6 +2,7 @@
 #include "rc_scene.h"
 
 #include <algorithm>
+#include <limits>
 #include <array>
 #include <cmath>
 #include <cstdio>
@@ -185,16 +186,16 @@ void build_wide(const std::vector<rc_bvh_node>& bn, std::vector<rc_wnode>& wn, i
         }
         rc_wnode w;
         for (int c = 0; c < 4; ++c) {
-            w.lox[c] = w.loy[c] = w.loz[c] = 1e30f;
-            w.hix[c] = w.hiy[c] = w.hiz[c] = -1e30f;
-            w.child[c] = -1; w.ccount[c] = 0;
+            w.lox[c] = w.loy[c] = w.loz[c] = std::numeric_limits<float>::infinity();   // empty slot: never entered
+            w.hix[c] = w.hiy[c] = w.hiz[c] = -std::numeric_limits<float>::infinity();
+            w.child[c] = 0; w.ccount[c] = 0;
         }
         std::vector<Job> sub;
         for (size_t k = 0; k < kids.size(); ++k) {
             const rc_bvh_node& c = bn[kids[k]];
             w.lox[k] = c.lo[0]; w.loy[k] = c.lo[1]; w.loz[k] = c.lo[2];
             w.hix[k] = c.hi[0]; w.hiy[k] = c.hi[1]; w.hiz[k] = c.hi[2];
-            if (c.cnt > 0) { w.child[k] = -2 - c.lf; w.ccount[k] = c.cnt; }
+            if (c.cnt > 0) { w.child[k] = c.lf; w.ccount[k] = c.cnt; }
             else {
                 int slot = (int)wn.size();
                 wn.push_back(rc_wnode{});

@@ -25,9 +25,9 @@ typedef struct rc_bvh_node {
 } rc_bvh_node;
 
 // 4-wide BVH node (128 B = two 64-byte lines): child boxes as fp32 SoA so the four slab
-// tests of a node read whole float4s. child[c] >= 0: inner node; child[c] <= -2: leaf holding
-// triangles first = -2 - child[c], count = ccount[c] (same triangle order and indices as the
-// binary BVH); child[c] == -1: empty slot.
+// tests of a node read whole float4s. ccount[c] > 0: leaf holding triangles child[c] ..
+// child[c] + ccount[c] - 1 (same triangle order and indices as the binary BVH); ccount[c] == 0:
+// inner node child[c], or an empty slot (child 0, box lo = +inf, hi = -inf: never entered).
 typedef struct rc_wnode {
     float lox[4], loy[4], loz[4], hix[4];   // 64 B
     float hiy[4], hiz[4];                   // 32 B
@@ -96,6 +96,7 @@ typedef struct rc_dcache {
     void* csnap;                 // [KSNAP][Np] compact snapshot records (24 B)
     float* age;                  // [Np]
     long long* sumpr;            // [2] fixed point (RC_FX_PR): current, next
+    long long* prpart;           // [ceil(Np / 64)] per-block partial sums of the sumpr terms
     int32_t* npend;              // [2]
     double* err;                 // [3]
 } rc_dcache;
