@@ -199,13 +199,16 @@ __global__ void __launch_bounds__(RC_TB) k_decide(rc_dscene s, rc_dcache c, rc_p
 }
 
 #define RC_UT 64   // entries per k_update block
+#define RC_UE ((RC_UT * RC_MAX_LIGHTS + RC_TB - 1) / RC_TB)   // channel elements per thread at most
 
 // EMA update of the per-light channels + history count (cuda/poc_gpu.cu:562-602), with the
 // light-dependency merge of this frame's samples, the next frame's sum of 1/(n+1), and the
 // resolved radiance R = A * sum_l (Edir_l + Lc_l) of the updated entry (the value the query
-// pass reads, one float instead of 2 NL). A block owns RC_UT entries: per-entry factors are
-// computed by its first wave into LDS; the per-light blend then runs over the tile's AoS rows
-// in flat order (coalesced Lc / Edir / Lc2), reading the SoA sample means from an LDS copy.
+// pass reads, one float instead of 2 NL). A block owns RC_UT entries, whose AoS channel rows
+// form one contiguous span: every thread first issues the loads of its elements of that span
+// (Lc, Edir) and of the SoA sample means, so the block waits for memory once; the per-entry
+// factors are computed by the first wave meanwhile, then the blend runs over the span in flat
+// order (coalesced Lc2 stores).
 __global__ void __launch_bounds__(RC_TB) k_update(rc_dscene s, rc_dstate st, rc_dcache c, rc_params p, const uint64_t* lnew) {
     __shared__ float s_al[RC_UT], s_alr[RC_UT], s_A[RC_UT];
     extern __shared__ float sm[];                 // [nl][RC_UT] sample means, then [RC_UT][nl] terms
@@ -213,8 +216,24 @@ __global__ void __launch_bounds__(RC_TB) k_update(rc_dscene s, rc_dstate st, rc_
     const int base = blockIdx.x * RC_UT;
     const int T = min(RC_UT, Np - base);
     const int t = threadIdx.x;
+    const int nel = T * nl;
     float* s_m = sm;
     float* s_v = sm + nl * RC_UT;
+    const int64_t g0 = (int64_t)base * nl;
+    const float* __restrict__ Lc = c.Lc + g0;
+    const float* __restrict__ Ed = st.Edir + g0;
+    float* __restrict__ Lc2 = c.Lc2 + g0;
+    float ra[RC_UE], re[RC_UE], rm[RC_UE];
+#pragma unroll
+    for (int u = 0; u < RC_UE; ++u) {
+        const int e = t + u * RC_TB;
+        if (e < nel) { ra[u] = Lc[e]; re[u] = Ed[e]; }
+    }
+#pragma unroll
+    for (int u = 0; u < RC_UE; ++u) {
+        const int e = t + u * RC_TB;   // s_m element: light e / RC_UT, entry e % RC_UT
+        if (e < nl * RC_UT) { const int j = e % RC_UT; rm[u] = j < T ? c.m[(int64_t)(e / RC_UT) * Np + base + j] : 0.f; }
+    }
     if (t < RC_UT) {
         long long pr = 0;
         if (t < T) {
@@ -248,32 +267,40 @@ __global__ void __launch_bounds__(RC_TB) k_update(rc_dscene s, rc_dstate st, rc_
         pr = rc_wave_sum_i64(pr);
         if (t == 0) c.prpart[blockIdx.x] = pr;   // summed by k_prsum (no same-address atomics)
     }
-    for (int e = t; e < nl * RC_UT; e += RC_TB) {
-        int l = e / RC_UT, j = e % RC_UT;
-        s_m[e] = j < T ? c.m[(int64_t)l * Np + base + j] : 0.f;
+#pragma unroll
+    for (int u = 0; u < RC_UE; ++u) {
+        const int e = t + u * RC_TB;
+        if (e < nl * RC_UT) s_m[e] = rm[u];
     }
     __syncthreads();
-    const int64_t g0 = (int64_t)base * nl;
-    for (int e = t; e < T * nl; e += RC_TB) {
-        int j = e / nl, l = e - j * nl;
-        float a = c.Lc[g0 + e];
-        float mm = s_m[l * RC_UT + j];
-        bool dm = rc_is_dormant(p.dormant, l);
-        float o;
-        if (p.ddgi) o = dm ? a : a + s_al[j] * (mm - a);
-        else {
-            float al = (l < 64 && ((p.resp_mask >> l) & 1ull)) ? s_alr[j] : s_al[j];
-            if (dm) al = 0.f;
-            o = a + al * (mm - a);
+    // flat element e = j * nl + l of the span; (j, l) advanced by RC_TB per step
+    const int dq = RC_TB / nl, dr = RC_TB - dq * nl;
+    int j = t / nl, l = t - (t / nl) * nl;
+#pragma unroll
+    for (int u = 0; u < RC_UE; ++u) {
+        const int e = t + u * RC_TB;
+        if (e < nel) {
+            const float a = ra[u];
+            const float mm = s_m[l * RC_UT + j];
+            const bool dm = rc_is_dormant(p.dormant, l);
+            float o;
+            if (p.ddgi) o = dm ? a : a + s_al[j] * (mm - a);
+            else {
+                float al = (l < 64 && ((p.resp_mask >> l) & 1ull)) ? s_alr[j] : s_al[j];
+                if (dm) al = 0.f;
+                o = a + al * (mm - a);
+            }
+            Lc2[e] = o;
+            const float lv = dm ? 0.f : o;
+            s_v[e] = s_A[j] * (re[u] + lv);
         }
-        c.Lc2[g0 + e] = o;
-        float lv = dm ? 0.f : o;
-        s_v[e] = s_A[j] * (st.Edir[g0 + e] + lv);
+        j += dq; l += dr;
+        if (l >= nl) { l -= nl; ++j; }
     }
     __syncthreads();
     if (t < T) {
         float val = 0.f;
-        for (int l = 0; l < nl; ++l) val += s_v[t * nl + l];
+        for (int k = 0; k < nl; ++k) val += s_v[t * nl + k];
         c.R[base + t] = val;
     }
 }
