@@ -245,35 +245,32 @@ __device__ __forceinline__ int4 rc_wrow_i(const rc_dscene& s, uint32_t off) {
     return *(const int4*)((const char*)s.WN + off);
 }
 
-// Entry distances of a node's four child boxes against [0, tmax] (1e30: missed or empty
-// slot), branch-free. With lo <= hi, (lo - o) * inv and (hi - o) * inv are ordered by the sign
-// of inv, so the near-plane value is exactly the per-axis minimum of rc_box_enter and the
-// result equals rc_box_enter's bit for bit. Empty slots hold lo = +inf, hi = -inf: their
-// near value is +inf and their far value -inf whatever the direction, so they always miss.
-__device__ __forceinline__ void rc_child_enter(const rc_wray& r, const rc_dscene& s, uint32_t node, float tmax, float& t0, float& t1,
-                                               float& t2, float& t3) {
+// Entry distances of a node's four child boxes against [0, tmax] (1e30: missed, empty slot or
+// leaf), branch-free, and the triangle ranges [a, b) of the entered leaf children (a = b
+// otherwise). With lo <= hi, (lo - o) * inv and (hi - o) * inv are ordered by the sign of inv,
+// so the near-plane value is exactly the per-axis minimum of rc_box_enter and the result
+// equals rc_box_enter's bit for bit. Empty slots hold lo = +inf, hi = -inf: their near value
+// is +inf and their far value -inf whatever the direction, so they always miss. A leaf child
+// stores its first triangle and end (end > first); inner and empty slots store end = child.
+__device__ __forceinline__ void rc_child_enter(const rc_wray& r, const rc_dscene& s, uint32_t node, float tmax, int4 ch, int4 ce,
+                                               float& t0, float& t1, float& t2, float& t3, int& a0, int& b0, int& a1, int& b1,
+                                               int& a2, int& b2, int& a3, int& b3) {
     const float4 nx = rc_wrow(s, node + r.nx), ny = rc_wrow(s, node + r.ny), nz = rc_wrow(s, node + r.nz);
     const float4 fx = rc_wrow(s, node + r.fx), fy = rc_wrow(s, node + r.fy), fz = rc_wrow(s, node + r.fz);
-#define RC_SLAB(tc, c)                                                                                          \
+#define RC_SLAB(tc, c, a, b)                                                                                    \
     {                                                                                                           \
         const float tn = fmaxf(fmaxf((nx.c - r.o.x) * r.inv.x, (ny.c - r.o.y) * r.inv.y), fmaxf((nz.c - r.o.z) * r.inv.z, 0.f)); \
         const float tf = fminf(fminf((fx.c - r.o.x) * r.inv.x, (fy.c - r.o.y) * r.inv.y), fminf((fz.c - r.o.z) * r.inv.z, tmax)); \
-        tc = tn <= tf ? tn : 1e30f;                                                                             \
+        const bool in_ = tn <= tf, leaf_ = ce.c > ch.c;                                                          \
+        tc = in_ && !leaf_ ? tn : 1e30f;                                                                        \
+        a = ch.c; b = in_ && leaf_ ? ce.c : ch.c;                                                               \
     }
-    RC_SLAB(t0, x) RC_SLAB(t1, y) RC_SLAB(t2, z) RC_SLAB(t3, w)
+    RC_SLAB(t0, x, a0, b0) RC_SLAB(t1, y, a1, b1) RC_SLAB(t2, z, a2, b2) RC_SLAB(t3, w, a3, b3)
 #undef RC_SLAB
 }
 
 // Leaf children of a node as up to four triangle ranges [a, b), walked by one flat loop so a
-// wave iterates as often as its busiest lane (not once per child slot). A leaf child stores
-// its first triangle and count (> 0); inner and empty slots have count 0, so their range is
-// empty. Leaf children leave the sort (distance set to 1e30).
-#define RC_LEAF_RANGE(tc, ic, cnt, a, b)             \
-    {                                                \
-        const bool leaf_ = tc < 1e30f && (cnt) > 0;  \
-        a = ic; b = ic + (leaf_ ? (cnt) : 0);        \
-        tc = leaf_ ? 1e30f : tc;                     \
-    }
+// wave iterates as often as its busiest lane (not once per child slot). Empty ranges: a = b.
 #define RC_LEAF_WALK(BODY)                                                              \
     for (;;) {                                                                          \
         if (a0 >= b0) {                                                                 \
@@ -286,46 +283,78 @@ __device__ __forceinline__ void rc_child_enter(const rc_wray& r, const rc_dscene
         BODY                                                                            \
     }
 
-// One node of the 4-wide closest-hit traversal (order-independent rule): leaf children are
-// intersected at once, inner children ordered by a 4-input sorting network and pushed
-// far-to-near, the nearest visited next. The stack lives in LDS, one column per lane
-// (stk[level * 256]), 32-bit entries node << 8 | code(t). Returns true when finished.
-__device__ __forceinline__ bool rc_closest_node(const rc_wray& r, const rc_dscene& s, float& tmin, int& q, int& cur, int& sp, uint32_t* stk) {
-    const uint32_t node = (uint32_t)cur << 7;
-    const int4 ch = rc_wrow_i(s, node + 96u), cc = rc_wrow_i(s, node + 112u);
-    float t0, t1, t2, t3;
-    rc_child_enter(r, s, node, tmin, t0, t1, t2, t3);
-    int i0 = ch.x, i1 = ch.y, i2 = ch.z, i3 = ch.w;
-    int a0 = 0, b0 = 0, a1 = 0, b1 = 0, a2 = 0, b2 = 0, a3 = 0, b3 = 0;
-    RC_LEAF_RANGE(t0, i0, cc.x, a0, b0) RC_LEAF_RANGE(t1, i1, cc.y, a1, b1)
-    RC_LEAF_RANGE(t2, i2, cc.z, a2, b2) RC_LEAF_RANGE(t3, i3, cc.w, a3, b3)
-    RC_LEAF_WALK(if (rc_closer(t, j, tmin, q)) { tmin = t; q = -2 - j; })
-    // sorting network on (t, node): ascending entry distance, misses (1e30) last
-    rc_cswap(t0, i0, t1, i1); rc_cswap(t2, i2, t3, i3);
-    rc_cswap(t0, i0, t2, i2); rc_cswap(t1, i1, t3, i3);
-    rc_cswap(t1, i1, t2, i2);
-    // the children still in front of the current hit are a prefix of the sorted list: push
-    // all but the nearest far-to-near, visit the nearest next
-#define RC_PUSH(tc, ic) \
-    if (tc < 1e30f && tc <= tmin) { stk[sp * 256] = ((uint32_t)ic << 8) | rc_tcode(tc); ++sp; }
-    RC_PUSH(t3, i3) RC_PUSH(t2, i2) RC_PUSH(t1, i1)
-#undef RC_PUSH
-    if (t0 < 1e30f && t0 <= tmin) { cur = i0; return false; }
-    while (sp > 0) {
-        --sp;
-        const uint32_t e = stk[sp * 256];
-        if (rc_tlb(e & 255u) <= tmin) { cur = (int)(e >> 8); return false; }
-    }
-    return true;
-}
-
-// 4-wide BVH closest hit of the static geometry (rc_closest_node until the stack is empty);
-// the stack depth is bounded by the scene's wstack levels (3 per wide level).
+// 4-wide BVH closest hit of the static geometry, order-independent rule, with postponed
+// leaves. A node step tests the node's four child boxes, queues the triangle ranges of the
+// entered leaf children (one LDS column per lane, RC_WQ levels after the stack; entries
+// first << 7 | count) and goes on with the nearest entered inner child, the others pushed
+// far-to-near (LDS stack, 32-bit entries node << 8 | code(t)), or with a pop. The wave walks
+// the queued triangles of all its lanes together once every lane still traversing has some
+// queued (or one queue could overflow), so the triangle loop runs with most lanes busy
+// instead of after each node step for the few lanes that entered a leaf. Culling uses the
+// current tmin, which may lag the queued hits: more node steps at worst, never a missed hit,
+// and the hit itself does not depend on the order. A leaf longer than 127 triangles (only at
+// the builder's depth limit) is walked at once. The stack depth is bounded by the scene's
+// wstack levels (3 per wide level).
 __device__ __forceinline__ void rc_trace_wide_closest(f3 o, f3 d, const rc_dscene& s, float& tmin, int& q, uint32_t* lstk) {
     uint32_t* stk = lstk + (threadIdx.x & 255);
+    uint32_t* que = stk + (size_t)s.wstack * 256;
     const rc_wray r = rc_wray_make(o, d);
-    int sp = 0, cur = 0;
-    while (!rc_closest_node(r, s, tmin, q, cur, sp, stk)) {}
+    int sp = 0, cur = 0, nq = 0;   // cur < 0: this lane's traversal is over
+    for (;;) {
+        const bool trav = cur >= 0, pend = nq > 0;
+        if (!__any(trav || pend)) break;
+        if (__any(pend) && (__all(pend || !trav) || __any(nq > RC_WQ - 4))) {
+            int k = 0, a = 0, b = 0;
+            for (;;) {
+                if (a >= b) {
+                    if (k >= nq) break;
+                    const uint32_t e = que[k * 256];
+                    ++k;
+                    a = (int)(e >> 7); b = a + (int)(e & 127u);
+                }
+                const int j = a++;
+                const float t = rc_tri_static4(r.o, r.d, (const float4*)s.ST4 + 3 * (int64_t)j);
+                if (rc_closer(t, j, tmin, q)) { tmin = t; q = -2 - j; }
+            }
+            nq = 0;
+            continue;
+        }
+        if (!trav) continue;
+        const uint32_t node = (uint32_t)cur << 7;
+        const int4 ch = rc_wrow_i(s, node + 96u), ce = rc_wrow_i(s, node + 112u);
+        float t0, t1, t2, t3;
+        int a0, b0, a1, b1, a2, b2, a3, b3;
+        rc_child_enter(r, s, node, tmin, ch, ce, t0, t1, t2, t3, a0, b0, a1, b1, a2, b2, a3, b3);
+        int i0 = ch.x, i1 = ch.y, i2 = ch.z, i3 = ch.w;
+#define RC_QUEUE(a, b)                                                                          \
+        if (b > a) {                                                                            \
+            if (b - a <= 127) { que[nq * 256] = ((uint32_t)a << 7) | (uint32_t)(b - a); ++nq; } \
+            else                                                                                \
+                for (int j = a; j < b; ++j) {                                                   \
+                    const float t = rc_tri_static4(r.o, r.d, (const float4*)s.ST4 + 3 * (int64_t)j); \
+                    if (rc_closer(t, j, tmin, q)) { tmin = t; q = -2 - j; }                     \
+                }                                                                               \
+        }
+        RC_QUEUE(a0, b0) RC_QUEUE(a1, b1) RC_QUEUE(a2, b2) RC_QUEUE(a3, b3)
+#undef RC_QUEUE
+        // sorting network on (t, node): ascending entry distance, misses (1e30) last
+        rc_cswap(t0, i0, t1, i1); rc_cswap(t2, i2, t3, i3);
+        rc_cswap(t0, i0, t2, i2); rc_cswap(t1, i1, t3, i3);
+        rc_cswap(t1, i1, t2, i2);
+        // the children still in front of the current hit are a prefix of the sorted list: push
+        // all but the nearest far-to-near, visit the nearest next
+#define RC_PUSH(tc, ic) \
+        if (tc < 1e30f && tc <= tmin) { stk[sp * 256] = ((uint32_t)ic << 8) | rc_tcode(tc); ++sp; }
+        RC_PUSH(t3, i3) RC_PUSH(t2, i2) RC_PUSH(t1, i1)
+#undef RC_PUSH
+        if (t0 < 1e30f && t0 <= tmin) { cur = i0; continue; }
+        cur = -1;
+        while (sp > 0) {
+            --sp;
+            const uint32_t e = stk[sp * 256];
+            if (rc_tlb(e & 255u) <= tmin) { cur = (int)(e >> 8); break; }
+        }
+    }
 }
 
 // One node of the any-hit test of the static geometry for a shadow segment (a hit at
@@ -336,13 +365,11 @@ __device__ __forceinline__ void rc_trace_wide_closest(f3 o, f3 d, const rc_dscen
 // Returns true when finished (blocked set on a blocker).
 __device__ __forceinline__ bool rc_any_node(const rc_wray& r, const rc_dscene& s, float tlim, int& cur, int& sp, uint32_t* stk, bool& blocked) {
     const uint32_t node = (uint32_t)cur << 7;
-    const int4 ch = rc_wrow_i(s, node + 96u), cc = rc_wrow_i(s, node + 112u);
+    const int4 ch = rc_wrow_i(s, node + 96u), ce = rc_wrow_i(s, node + 112u);
     float t0, t1, t2, t3;
-    rc_child_enter(r, s, node, tlim, t0, t1, t2, t3);
+    int a0, b0, a1, b1, a2, b2, a3, b3;
+    rc_child_enter(r, s, node, tlim, ch, ce, t0, t1, t2, t3, a0, b0, a1, b1, a2, b2, a3, b3);
     int i0 = ch.x, i1 = ch.y, i2 = ch.z, i3 = ch.w;
-    int a0 = 0, b0 = 0, a1 = 0, b1 = 0, a2 = 0, b2 = 0, a3 = 0, b3 = 0;
-    RC_LEAF_RANGE(t0, i0, cc.x, a0, b0) RC_LEAF_RANGE(t1, i1, cc.y, a1, b1)
-    RC_LEAF_RANGE(t2, i2, cc.z, a2, b2) RC_LEAF_RANGE(t3, i3, cc.w, a3, b3)
     RC_LEAF_WALK(if (t > 1e-4f && t < tlim) { blocked = true; return true; })
     rc_cswap(t0, i0, t1, i1); rc_cswap(t2, i2, t3, i3);
     rc_cswap(t0, i0, t2, i2); rc_cswap(t1, i1, t3, i3);

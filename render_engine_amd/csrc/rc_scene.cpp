@@ -188,18 +188,18 @@ void build_wide(const std::vector<rc_bvh_node>& bn, std::vector<rc_wnode>& wn, i
         for (int c = 0; c < 4; ++c) {
             w.lox[c] = w.loy[c] = w.loz[c] = std::numeric_limits<float>::infinity();   // empty slot: never entered
             w.hix[c] = w.hiy[c] = w.hiz[c] = -std::numeric_limits<float>::infinity();
-            w.child[c] = 0; w.ccount[c] = 0;
+            w.child[c] = 0; w.cend[c] = 0;
         }
         std::vector<Job> sub;
         for (size_t k = 0; k < kids.size(); ++k) {
             const rc_bvh_node& c = bn[kids[k]];
             w.lox[k] = c.lo[0]; w.loy[k] = c.lo[1]; w.loz[k] = c.lo[2];
             w.hix[k] = c.hi[0]; w.hiy[k] = c.hi[1]; w.hiz[k] = c.hi[2];
-            if (c.cnt > 0) { w.child[k] = c.lf; w.ccount[k] = c.cnt; }
+            if (c.cnt > 0) { w.child[k] = c.lf; w.cend[k] = c.lf + c.cnt; }
             else {
                 int slot = (int)wn.size();
                 wn.push_back(rc_wnode{});
-                w.child[k] = slot;
+                w.child[k] = slot; w.cend[k] = slot;
                 sub.push_back({kids[k], slot, jb.depth + 1});
             }
         }
@@ -345,6 +345,7 @@ std::string prepare(const SceneDesc& d, Scene& S) {
     build_wide(S.BN, S.WN, S.wide_depth);
     if (3 * S.wide_depth + 1 > RC_WSTACK) return "wide BVH too deep for the LDS traversal stack";
     if (S.WN.size() >= (1u << 24)) return "wide BVH has more than 2^24 nodes";
+    if (M.nt() >= (1 << 25)) return "more than 2^25 static triangles (the traversal's leaf queue holds 25-bit indices)";
     S.nst = M.nt();
     S.ST.resize(9 * (size_t)S.nst); S.STN.resize(4 * (size_t)S.nst); S.Talb.resize(S.nst);
     for (int j = 0; j < S.nst; ++j) {
@@ -595,6 +596,8 @@ std::string load_scene_file(const char* path, Scene& S, FileRefs& R) {
         if (!in.ok) return "truncated mesh data";
         build_wide(S.BN, S.WN, S.wide_depth);
         if (3 * S.wide_depth + 1 > RC_WSTACK) return "wide BVH too deep for the LDS traversal stack";
+        if (S.WN.size() >= (1u << 24)) return "wide BVH has more than 2^24 nodes";
+        if (S.nst >= (1 << 25)) return "more than 2^25 static triangles (the traversal's leaf queue holds 25-bit indices)";
         S.order.resize(S.nst); std::iota(S.order.begin(), S.order.end(), 0);
         S.Talb.assign(S.nst, 0.5f);
     }

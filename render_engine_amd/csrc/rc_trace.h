@@ -39,7 +39,7 @@ __device__ __forceinline__ bool rc_trace_full(f3 o, f3 d, const float* T, const 
 }
 
 // Dynamic LDS of the tracing kernels: nt triangles (9 floats) + their quad ids, then the
-// traversal stack (wstack levels x 256 lanes) at a 16-byte aligned offset.
+// traversal stack (wstack levels x 256 lanes) and leaf queue (RC_WQ levels) at a 16-byte aligned offset.
 __device__ __forceinline__ void rc_load_dyn(const rc_dstate& st, float* sh, int* shq) {
     for (int j = threadIdx.x; j < 9 * st.nt; j += blockDim.x) sh[j] = st.T[j];
     for (int j = threadIdx.x; j < st.nt; j += blockDim.x) shq[j] = st.TQ[j];
@@ -71,5 +71,5 @@ static inline int rc_grid(long long n) { return (int)((n + RC_TB - 1) / RC_TB); 
 // dynamic LDS of a tracing kernel: dynamic triangles + (wide traversal) the lane stacks
 static inline size_t rc_shm(const rc_dstate& st, const rc_dscene& s, int trav) {
     size_t tri = (((size_t)st.nt * 10 + 3) & ~(size_t)3) * sizeof(float);
-    return tri + (trav == 2 ? (size_t)s.wstack * 256 * sizeof(uint32_t) : 0);
+    return tri + (trav == 2 ? (size_t)(s.wstack + RC_WQ) * 256 * sizeof(uint32_t) : 0);
 }

@@ -25,16 +25,17 @@ typedef struct rc_bvh_node {
 } rc_bvh_node;
 
 // 4-wide BVH node (128 B = two 64-byte lines): child boxes as fp32 SoA so the four slab
-// tests of a node read whole float4s. ccount[c] > 0: leaf holding triangles child[c] ..
-// child[c] + ccount[c] - 1 (same triangle order and indices as the binary BVH); ccount[c] == 0:
-// inner node child[c], or an empty slot (child 0, box lo = +inf, hi = -inf: never entered).
+// tests of a node read whole float4s. cend[c] > child[c]: leaf holding triangles child[c] ..
+// cend[c] - 1 (same triangle order and indices as the binary BVH); cend[c] == child[c]: inner
+// node child[c], or an empty slot (child 0, box lo = +inf, hi = -inf: never entered).
 typedef struct rc_wnode {
     float lox[4], loy[4], loz[4], hix[4];   // 64 B
     float hiy[4], hiz[4];                   // 32 B
     int32_t child[4];                       // 16 B
-    int32_t ccount[4];                      // 16 B
+    int32_t cend[4];                        // 16 B
 } rc_wnode;
 #define RC_WSTACK 48          // most LDS stack levels per lane (48 KB per 256-lane block)
+#define RC_WQ 8               // queued leaf ranges per lane of the wide closest-hit traversal (LDS levels)
 
 // Per-state device view of the scene (cuda/poc_gpu.cu:222-237).
 typedef struct rc_dstate {

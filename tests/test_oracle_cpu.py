@@ -189,19 +189,19 @@ def test_bvh_structure(street):
     # empty slots carry the never-entered box (lo = +inf, hi = -inf)
     wn = scene.array("WN").reshape(-1, 32)
     wf = wn.view(np.float32)
-    ch, cc = wn[:, 24:28].view(np.int32), wn[:, 28:32].view(np.int32)
+    ch, ce = wn[:, 24:28].view(np.int32), wn[:, 28:32].view(np.int32)
     wlo, whi = wf[:, 0:12].reshape(-1, 3, 4), wf[:, 12:24].reshape(-1, 3, 4)
     wseen = np.zeros(nst, np.int32)
     reached = np.zeros(len(wn), np.int32)
     for k in range(len(wn)):
         for c in range(4):
             empty = np.all(np.isposinf(wlo[k, :, c])) and np.all(np.isneginf(whi[k, :, c]))
-            if cc[k, c] > 0:
-                wseen[ch[k, c]:ch[k, c] + cc[k, c]] += 1
-                v0 = ST[ch[k, c]:ch[k, c] + cc[k, c], 0:3]
+            if ce[k, c] > ch[k, c]:
+                wseen[ch[k, c]:ce[k, c]] += 1
+                v0 = ST[ch[k, c]:ce[k, c], 0:3]
                 assert np.all(v0 >= wlo[k, :, c] - 1e-5) and np.all(v0 <= whi[k, :, c] + 1e-5)
             elif not empty:
-                assert 0 < ch[k, c] < len(wn)
+                assert 0 < ch[k, c] < len(wn) and ce[k, c] == ch[k, c]
                 reached[ch[k, c]] += 1
     assert np.all(wseen == 1)
     assert reached[0] == 0 and np.all(reached[1:] == 1)
