@@ -269,20 +269,6 @@ __device__ __forceinline__ void rc_child_enter(const rc_wray& r, const rc_dscene
 #undef RC_SLAB
 }
 
-// Leaf children of a node as up to four triangle ranges [a, b), walked by one flat loop so a
-// wave iterates as often as its busiest lane (not once per child slot). Empty ranges: a = b.
-#define RC_LEAF_WALK(BODY)                                                              \
-    for (;;) {                                                                          \
-        if (a0 >= b0) {                                                                 \
-            if (a1 >= b1 && a2 >= b2 && a3 >= b3) break;                                \
-            a0 = a1; b0 = b1; a1 = a2; b1 = b2; a2 = a3; b2 = b3; a3 = 0; b3 = 0;       \
-            continue;                                                                   \
-        }                                                                               \
-        const int j = a0++;                                                             \
-        const float t = rc_tri_static4(r.o, r.d, (const float4*)s.ST4 + 3 * (int64_t)j); \
-        BODY                                                                            \
-    }
-
 // 4-wide BVH closest hit of the static geometry, order-independent rule, with postponed
 // leaves. A node step tests the node's four child boxes, queues the triangle ranges of the
 // entered leaf children (one LDS column per lane, RC_WQ levels after the stack; entries
@@ -357,41 +343,70 @@ __device__ __forceinline__ void rc_trace_wide_closest(f3 o, f3 d, const rc_dscen
     }
 }
 
-// One node of the any-hit test of the static geometry for a shadow segment (a hit at
-// 1e-4 < t < tlim blocks it; equivalent to the closest-hit test "closest t >= tlim" of the
-// direct-light estimate, but stops at the first blocker). Boxes are culled against tlim; leaf
-// children are tested at once and inner children visited near-first, so blockers next to the
-// shading point, the likely ones, are found early; the answer does not depend on the order.
-// Returns true when finished (blocked set on a blocker).
-__device__ __forceinline__ bool rc_any_node(const rc_wray& r, const rc_dscene& s, float tlim, int& cur, int& sp, uint32_t* stk, bool& blocked) {
-    const uint32_t node = (uint32_t)cur << 7;
-    const int4 ch = rc_wrow_i(s, node + 96u), ce = rc_wrow_i(s, node + 112u);
-    float t0, t1, t2, t3;
-    int a0, b0, a1, b1, a2, b2, a3, b3;
-    rc_child_enter(r, s, node, tlim, ch, ce, t0, t1, t2, t3, a0, b0, a1, b1, a2, b2, a3, b3);
-    int i0 = ch.x, i1 = ch.y, i2 = ch.z, i3 = ch.w;
-    RC_LEAF_WALK(if (t > 1e-4f && t < tlim) { blocked = true; return true; })
-    rc_cswap(t0, i0, t1, i1); rc_cswap(t2, i2, t3, i3);
-    rc_cswap(t0, i0, t2, i2); rc_cswap(t1, i1, t3, i3);
-    rc_cswap(t1, i1, t2, i2);
-    // entered children are a prefix of the sorted list: push all but the nearest far-to-near
-#define RC_ANYPUSH(tc, ic) \
-    if (tc < 1e30f) { stk[sp * 256] = (uint32_t)ic << 8; ++sp; }
-    RC_ANYPUSH(t3, i3) RC_ANYPUSH(t2, i2) RC_ANYPUSH(t1, i1)
-#undef RC_ANYPUSH
-    if (t0 < 1e30f) { cur = i0; return false; }
-    if (sp == 0) return true;
-    --sp;
-    cur = (int)(stk[sp * 256] >> 8);
-    return false;
-}
-
+// Any-hit test of the static geometry for a shadow segment: a hit at 1e-4 < t < tlim blocks
+// it (equivalent to the closest-hit test "closest t >= tlim" of the direct-light estimate, but
+// stops at the first blocker). Boxes are culled against tlim; inner children are visited
+// near-first, so blockers next to the shading point, the likely ones, are found early. Leaves
+// are postponed as in the closest-hit walk (queued per lane, walked by the wave together); a
+// lane stops at its first blocker. The answer does not depend on the order.
 __device__ __forceinline__ bool rc_occluded_wide(f3 o, f3 d, const rc_dscene& s, float tlim, uint32_t* lstk) {
     uint32_t* stk = lstk + (threadIdx.x & 255);
+    uint32_t* que = stk + (size_t)s.wstack * 256;
     const rc_wray r = rc_wray_make(o, d);
-    int sp = 0, cur = 0;
+    int sp = 0, cur = 0, nq = 0;   // cur < 0: this lane's traversal is over
     bool blocked = false;
-    while (!rc_any_node(r, s, tlim, cur, sp, stk, blocked)) {}
+    for (;;) {
+        const bool trav = cur >= 0, pend = nq > 0;
+        if (!__any(trav || pend)) break;
+        if (__any(pend) && (__all(pend || !trav) || __any(nq > RC_WQ - 4))) {
+            int k = 0, a = 0, b = 0;
+            for (;;) {
+                if (a >= b) {
+                    if (k >= nq) break;
+                    const uint32_t e = que[k * 256];
+                    ++k;
+                    a = (int)(e >> 7); b = a + (int)(e & 127u);
+                }
+                const int j = a++;
+                const float t = rc_tri_static4(r.o, r.d, (const float4*)s.ST4 + 3 * (int64_t)j);
+                if (t > 1e-4f && t < tlim) { blocked = true; break; }
+            }
+            nq = 0;
+            if (blocked) cur = -1;
+            continue;
+        }
+        if (!trav) continue;
+        const uint32_t node = (uint32_t)cur << 7;
+        const int4 ch = rc_wrow_i(s, node + 96u), ce = rc_wrow_i(s, node + 112u);
+        float t0, t1, t2, t3;
+        int a0, b0, a1, b1, a2, b2, a3, b3;
+        rc_child_enter(r, s, node, tlim, ch, ce, t0, t1, t2, t3, a0, b0, a1, b1, a2, b2, a3, b3);
+        int i0 = ch.x, i1 = ch.y, i2 = ch.z, i3 = ch.w;
+#define RC_QUEUE(a, b)                                                                          \
+        if (b > a) {                                                                            \
+            if (b - a <= 127) { que[nq * 256] = ((uint32_t)a << 7) | (uint32_t)(b - a); ++nq; } \
+            else                                                                                \
+                for (int j = a; j < b && !blocked; ++j) {                                       \
+                    const float t = rc_tri_static4(r.o, r.d, (const float4*)s.ST4 + 3 * (int64_t)j); \
+                    if (t > 1e-4f && t < tlim) blocked = true;                                  \
+                }                                                                               \
+        }
+        RC_QUEUE(a0, b0) RC_QUEUE(a1, b1) RC_QUEUE(a2, b2) RC_QUEUE(a3, b3)
+#undef RC_QUEUE
+        if (blocked) { cur = -1; nq = 0; continue; }
+        rc_cswap(t0, i0, t1, i1); rc_cswap(t2, i2, t3, i3);
+        rc_cswap(t0, i0, t2, i2); rc_cswap(t1, i1, t3, i3);
+        rc_cswap(t1, i1, t2, i2);
+        // entered children are a prefix of the sorted list: push all but the nearest far-to-near
+#define RC_ANYPUSH(tc, ic) \
+        if (tc < 1e30f) { stk[sp * 256] = (uint32_t)ic << 8; ++sp; }
+        RC_ANYPUSH(t3, i3) RC_ANYPUSH(t2, i2) RC_ANYPUSH(t1, i1)
+#undef RC_ANYPUSH
+        if (t0 < 1e30f) { cur = i0; continue; }
+        if (sp == 0) { cur = -1; continue; }
+        --sp;
+        cur = (int)(stk[sp * 256] >> 8);
+    }
     return blocked;
 }
 
