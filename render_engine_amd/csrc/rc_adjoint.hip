@@ -12,7 +12,8 @@
 //
 // Reverse mode for J = sum_i g_i R_i keeps the per-bounce radiance L^(b) (one float per entry
 // and bounce) and re-traces each bounce's rays from their seeds instead of storing paths; the
-// adjoint of a ray's contribution is scattered to its hit entry with hardware float atomics.
+// weight of a ray's contribution is scattered to its hit entry with one hardware float atomic,
+// and the per-entry factors are applied afterwards by a streaming pass.
 // The final reductions (per light over entries, per material over its entries) run in a fixed
 // order. The 64-bit support masks (bit min(l, 63) when dR_i/dI_l != 0) propagate with the
 // forward: all terms are non-negative, so the nonzero pattern is the union over the rays.
@@ -66,16 +67,29 @@ __global__ void __launch_bounds__(RC_TB) k_adj_zero(int n, float* x) {
     if (i < n) x[i] = 0.f;
 }
 
-// A_i = alb[mat_i], E_i = sum_l I_l G_il (lights in order)
+// A_i = alb[mat_i], E_i = sum_l I_l G_il (lights in order). A block owns RC_AS entries, whose
+// G rows form one contiguous span: copied to LDS with coalesced loads (rows padded to nl + 1
+// floats, so the per-entry walks along a row hit distinct banks), then summed per entry.
+#define RC_AS 64
 __global__ void __launch_bounds__(RC_TB) k_adj_setup(int Np, int nl, const int* mat, const float* alb, const float* I, const float* G,
                                                     float* A, float* E) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= Np) return;
-    A[i] = alb[mat[i]];
-    const float* g = G + (int64_t)nl * i;
-    float e = 0.f;
-    for (int l = 0; l < nl; ++l) e += I[l] * g[l];
-    E[i] = e;
+    __shared__ float s_g[RC_AS * (RC_MAX_LIGHTS + 1)];
+    const int base = blockIdx.x * RC_AS;
+    const int T = min(RC_AS, Np - base), stride = nl + 1;
+    const float* g = G + (int64_t)base * nl;
+    for (int e = threadIdx.x; e < T * nl; e += blockDim.x) {
+        const int j = e / nl;
+        s_g[j * stride + (e - j * nl)] = g[e];
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < T) {
+        const int i = base + threadIdx.x;
+        A[i] = alb[mat[i]];
+        const float* row = s_g + threadIdx.x * stride;
+        float e = 0.f;
+        for (int l = 0; l < nl; ++l) e += I[l] * row[l];
+        E[i] = e;
+    }
 }
 
 // the ray rule of the adjoint bounces: hit entry h of ray k of entry i, or -1
@@ -138,11 +152,12 @@ __global__ void __launch_bounds__(RC_TB) k_adj_bwd_init(int Np, const float* g, 
     lam[i] = gi * a;
 }
 
-// one reverse bounce: re-trace the rays of bounce b; each hit h receives w = lam_i / K times
-// (E_h + L^(b)_h) on dA, A_h on dE and (b > 0) A_h on the adjoint of L^(b)
+// one reverse bounce: re-trace the rays of bounce b; each hit h collects the weight
+// w = lam_i / K of the ray's entry i into S_h (one atomic per hit). Every term the hit feeds
+// is a per-h factor times that weight, so k_adj_bwd_fin then adds (E_h + L^(b)_h) S_h to dA,
+// A_h S_h to dE and sets the adjoint of L^(b) to A_h S_h.
 template <int TRAV>
-__global__ void __launch_bounds__(RC_TB) k_adj_bwd(rc_dscene s, rc_dstate st, const float* A, const float* E, const float* Lp,
-                                                  const float* lam_in, float* lam_out, float* dA, float* dE, uint32_t seed, int K) {
+__global__ void __launch_bounds__(RC_TB) k_adj_bwd(rc_dscene s, rc_dstate st, const float* lam_in, float* S, uint32_t seed, int K) {
     extern __shared__ float sh[];
     int* shq = (int*)(sh + 9 * st.nt);
     uint32_t* lstk = rc_lstack(sh, st.nt);
@@ -155,12 +170,22 @@ __global__ void __launch_bounds__(RC_TB) k_adj_bwd(rc_dscene s, rc_dstate st, co
     const f3 P = ld3(st.P, i), N = ld3(st.N, i), O = P + N * 1e-3f;
     for (int k = 0; k < K; ++k) {
         const int h = rc_adj_ray<TRAV>(s, st, sh, shq, lstk, i, O, N, rs);
-        if (h < 0) continue;
-        const float Ah = A[h];
-        unsafeAtomicAdd(dA + h, w * (E[h] + (Lp ? Lp[h] : 0.f)));
-        unsafeAtomicAdd(dE + h, w * Ah);
-        if (lam_out) unsafeAtomicAdd(lam_out + h, w * Ah);
+        if (h >= 0) unsafeAtomicAdd(S + h, w);
     }
+}
+
+// per entry h after a reverse bounce: dA_h += (E_h + L^(b)_h) S_h, dE_h += A_h S_h, and the
+// adjoint of L^(b) (lam_out, may alias S) = A_h S_h
+__global__ void __launch_bounds__(RC_TB) k_adj_bwd_fin(int Np, const float* A, const float* E, const float* Lp, const float* S,
+                                                      float* lam_out, float* dA, float* dE) {
+    const int h = blockIdx.x * blockDim.x + threadIdx.x;
+    if (h >= Np) return;
+    const float sw = S[h];
+    if (sw == 0.f) { if (lam_out) lam_out[h] = 0.f; return; }
+    const float a = A[h];
+    dA[h] += (E[h] + (Lp ? Lp[h] : 0.f)) * sw;
+    dE[h] += a * sw;
+    if (lam_out) lam_out[h] = a * sw;
 }
 
 // block sum in a fixed order (wave shuffles, then the waves in order)
@@ -176,15 +201,26 @@ __device__ __forceinline__ float rc_block_sum(float v, float* s_w) {
     return t;
 }
 
-// per-block partial sums of dE_i G_il for every light: part[block][l]
+// per-block partial sums of dE_i G_il for every light: part[block][l]. Lanes are lights (rows
+// of G read whole, 64 lights at a time); each wave walks every fourth entry of the block's
+// RC_TB, then the four waves' sums are added in wave order.
 __global__ void __launch_bounds__(RC_TB) k_adj_dI_part(int Np, int nl, const float* dE, const float* G, float* part) {
-    __shared__ float s_w[RC_TB / 64];
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const float de = i < Np ? dE[i] : 0.f;
-    const float* g = G + (int64_t)nl * (i < Np ? i : 0);
-    for (int l = 0; l < nl; ++l) {
-        const float t = rc_block_sum(i < Np ? de * g[l] : 0.f, s_w);
-        if (threadIdx.x == 0) part[(int64_t)blockIdx.x * nl + l] = t;
+    __shared__ float s_p[RC_TB / 64][RC_MAX_LIGHTS];
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int base = blockIdx.x * RC_TB;
+    for (int l = lane; l < nl; l += 64) {
+        float acc = 0.f;
+        for (int j = wv; j < RC_TB; j += RC_TB / 64) {
+            const int i = base + j;
+            if (i < Np) acc += dE[i] * G[(int64_t)i * nl + l];
+        }
+        s_p[wv][l] = acc;
+    }
+    __syncthreads();
+    for (int l = threadIdx.x; l < nl; l += RC_TB) {
+        float t = 0.f;
+        for (int k = 0; k < RC_TB / 64; ++k) t += s_p[k][l];
+        part[(int64_t)blockIdx.x * nl + l] = t;
     }
 }
 
@@ -199,12 +235,33 @@ __global__ void __launch_bounds__(RC_TB) k_adj_dI_sum(int nblk, int nl, const fl
 }
 
 // dalb_m = sum of dA over the entries of material m (perm: entries sorted by material, moff:
-// offsets); one wave per material
-__global__ void __launch_bounds__(RC_TB) k_adj_dalb(int nmat, const float* dA, const int* perm, const int* moff, float* dalb) {
+// offsets, pmat: material of each sorted position), balanced whatever the material sizes: each
+// block scans one RC_TB tile of the sorted positions (segmented by material, fixed order) and
+// leaves each segment's sum at the segment's last position; then one wave per material adds
+// its tiles' segment sums in tile order.
+__global__ void __launch_bounds__(RC_TB) k_adj_dalb_tile(int Np, const float* dA, const int* perm, const int* pmat, float* tail) {
+    __shared__ float s_v[RC_TB];
+    __shared__ int s_k[RC_TB];
+    const int t = threadIdx.x, p = blockIdx.x * RC_TB + t;
+    const bool live = p < Np;
+    s_v[t] = live ? dA[perm[p]] : 0.f;
+    s_k[t] = live ? pmat[p] : -1;
+    __syncthreads();
+    for (int d = 1; d < RC_TB; d <<= 1) {   // inclusive segmented scan (Hillis-Steele)
+        const float add = (t >= d && s_k[t - d] == s_k[t]) ? s_v[t - d] : 0.f;
+        __syncthreads();
+        s_v[t] += add;
+        __syncthreads();
+    }
+    if (live && (t == RC_TB - 1 || p == Np - 1 || s_k[t + 1] != s_k[t])) tail[p] = s_v[t];
+}
+__global__ void __launch_bounds__(RC_TB) k_adj_dalb(int nmat, const int* moff, const float* tail, float* dalb) {
     const int m = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, lane = threadIdx.x & 63;
     if (m >= nmat) return;
+    const int p0 = moff[m], p1 = moff[m + 1];
     float v = 0.f;
-    for (int j = moff[m] + lane; j < moff[m + 1]; j += 64) v += dA[perm[j]];
+    if (p1 > p0)
+        for (int tl = p0 / RC_TB + lane; tl <= (p1 - 1) / RC_TB; tl += 64) v += tail[min((tl + 1) * RC_TB, p1) - 1];
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
     if (lane == 0) dalb[m] = v;
@@ -230,7 +287,7 @@ hipError_t rc_launch_adj_forward(int trav, const rc_dscene* s, const rc_dstate* 
     const int Np = s->Np;
     const dim3 g(rc_grid(Np)), b(RC_TB);
     const size_t shm = rc_shm(*st, *s, trav);
-    hipLaunchKernelGGL(k_adj_setup, g, b, 0, stream, Np, s->nl, w->mat, alb, I, w->G, w->A, w->E);
+    hipLaunchKernelGGL(k_adj_setup, dim3((Np + RC_AS - 1) / RC_AS), b, 0, stream, Np, s->nl, w->mat, alb, I, w->G, w->A, w->E);
     for (int k = 0; k < B; ++k) {
         const uint32_t sb = rc_hash32(seed + (uint32_t)k * 0x9E3779B9u);
         RC_ADJ_T(k_adj_fwd, *s, *st, w->A, w->E, w->SG, k > 0 ? w->L + (size_t)(k - 1) * Np : (const float*)nullptr,
@@ -250,16 +307,20 @@ hipError_t rc_launch_adj_backward(int trav, const rc_dscene* s, const rc_dstate*
                        w->dA, w->dE, w->lam[B & 1]);
     for (int k = B - 1; k >= 0; --k) {
         const uint32_t sb = rc_hash32(seed + (uint32_t)k * 0x9E3779B9u);
-        float* lam_out = k > 0 ? w->lam[k & 1] : nullptr;
-        // the adjoint of L^(k) starts at zero: a fill kernel (no memset node in a captured graph)
-        if (lam_out) hipLaunchKernelGGL(k_adj_zero, g, b, 0, stream, Np, lam_out);
-        RC_ADJ_T(k_adj_bwd, *s, *st, w->A, w->E, k > 0 ? w->L + (size_t)(k - 1) * Np : (const float*)nullptr, w->lam[(k + 1) & 1],
-                 lam_out, w->dA, w->dE, sb, K);
+        // the hit weights S collect in the buffer that then holds the adjoint of L^(k) (at k = 0,
+        // where that adjoint is not needed, in the same buffer as scratch); it starts at zero
+        // with a fill kernel (no memset node in a captured graph)
+        float* S = w->lam[k & 1];
+        hipLaunchKernelGGL(k_adj_zero, g, b, 0, stream, Np, S);
+        RC_ADJ_T(k_adj_bwd, *s, *st, w->lam[(k + 1) & 1], S, sb, K);
+        hipLaunchKernelGGL(k_adj_bwd_fin, g, b, 0, stream, Np, w->A, w->E, k > 0 ? w->L + (size_t)(k - 1) * Np : (const float*)nullptr,
+                           (const float*)S, k > 0 ? S : (float*)nullptr, w->dA, w->dE);
     }
     const int nblk = rc_grid(Np);
     hipLaunchKernelGGL(k_adj_dI_part, dim3(nblk), b, 0, stream, Np, nl, w->dE, w->G, w->part);
     hipLaunchKernelGGL(k_adj_dI_sum, dim3(nl), b, 0, stream, nblk, nl, w->part, dI);
-    hipLaunchKernelGGL(k_adj_dalb, dim3(rc_grid((long long)w->nmat * 64)), b, 0, stream, w->nmat, w->dA, w->perm, w->moff, dalb);
+    hipLaunchKernelGGL(k_adj_dalb_tile, dim3(nblk), b, 0, stream, Np, w->dA, w->perm, w->pmat, w->tail);
+    hipLaunchKernelGGL(k_adj_dalb, dim3(rc_grid((long long)w->nmat * 64)), b, 0, stream, w->nmat, w->moff, w->tail, dalb);
     return hipGetLastError();
 }
 #undef RC_ADJ_T

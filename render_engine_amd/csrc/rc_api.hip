@@ -928,6 +928,7 @@ extern "C" rc_status rc_adjoint_create(rc_scene* s, int state, int K, int bounce
     AK(dalloc(&w.lam[0], Np, a->owned)); AK(dalloc(&w.lam[1], Np, a->owned));
     AK(dalloc(&w.part, nblk * nl, a->owned));
     AK(dalloc(&w.perm, Np, a->owned)); AK(dalloc(&w.moff, (size_t)H.nmat + 1, a->owned));
+    AK(dalloc(&w.pmat, Np, a->owned)); AK(dalloc(&w.tail, Np, a->owned));
     w.nmat = H.nmat;
     // entries grouped by material (counting sort, entry order kept inside a material)
     std::vector<int> moff(H.nmat + 1, 0), perm(Np);
@@ -940,6 +941,9 @@ extern "C" rc_status rc_adjoint_create(rc_scene* s, int state, int K, int bounce
     AK(hipMemcpyAsync(w.mat, H.mat.data(), Np * 4, hipMemcpyHostToDevice, s->stream));
     AK(hipMemcpyAsync(w.perm, perm.data(), Np * 4, hipMemcpyHostToDevice, s->stream));
     AK(hipMemcpyAsync(w.moff, moff.data(), moff.size() * 4, hipMemcpyHostToDevice, s->stream));
+    std::vector<int> pmat(Np);
+    for (int m = 0; m < H.nmat; ++m) for (int j = moff[m]; j < moff[m + 1]; ++j) pmat[j] = m;
+    AK(hipMemcpyAsync(w.pmat, pmat.data(), Np * 4, hipMemcpyHostToDevice, s->stream));
     AK(rc_launch_adj_unit(s->trav, &s->ds, &s->dst[state], w.G, w.SG, s->stream));
     AK(hipStreamSynchronize(s->stream));
 #undef AK

@@ -156,6 +156,8 @@ typedef struct rc_adjbuf {
     float* lam[2];                 // [Np] adjoint of L^(b) ping-pong
     float* part;                   // [blocks][nl] partial sums of dI
     int32_t* perm; int32_t* moff;  // entries sorted by material, offsets [nmat + 1]
+    int32_t* pmat;                 // [Np] material of each sorted position
+    float* tail;                   // [Np] per-tile segment sums of dA (at each segment's last position)
     int32_t nmat, pad0;
 } rc_adjbuf;
 
