@@ -256,23 +256,44 @@ __global__ void __launch_bounds__(RC_TB) k_mlp_dw_sum(const float* part, int nsp
 
 // hash-table gradient: each sample's feature gradients spread over its 8 corners per level;
 // threads run level-major (consecutive threads: consecutive samples of one level), so the
-// atomics of a wave stay within one level's table
+// atomics of a wave stay within one level's table. Training samples come in entry order,
+// which is spatially sorted, so at the coarse levels neighbouring lanes share corners: lanes
+// holding the same corner address in a run of consecutive lanes add their values first (a
+// segmented scan by shuffles) and only the run's last lane issues the two atomics.
 __global__ void __launch_bounds__(RC_TB) k_mlp_enc_bwd(rc_mlpdev m, const float4* P, int n, const float* gX) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n * RC_MLP_LEVELS) return;
-    const int lv = t / n, s = t % n;
-    const float g0 = gX[(int64_t)s * (2 * RC_MLP_LEVELS) + 2 * lv], g1 = gX[(int64_t)s * (2 * RC_MLP_LEVELS) + 2 * lv + 1];
-    if (g0 == 0.f && g1 == 0.f) return;
-    const f3 x = rc_mlp_norm(m, P[s]);
-    uint32_t ix, iy, iz; f3 fr;
-    rc_mlp_cell(m, lv, x, ix, iy, iz, fr);
+    const int lane = threadIdx.x & 63;
+    const bool live = t < n * RC_MLP_LEVELS;
+    const int lv = live ? t / n : 0, s = live ? t % n : 0;
+    float g0 = 0.f, g1 = 0.f;
+    if (live) { g0 = gX[(int64_t)s * (2 * RC_MLP_LEVELS) + 2 * lv]; g1 = gX[(int64_t)s * (2 * RC_MLP_LEVELS) + 2 * lv + 1]; }
+    const bool act = live && (g0 != 0.f || g1 != 0.f);
+    if (!__any(act)) return;   // wave-uniform exit
+    uint32_t ix = 0, iy = 0, iz = 0;
+    f3 fr = mk3(0.f, 0.f, 0.f);
+    if (act) rc_mlp_cell(m, lv, rc_mlp_norm(m, P[s]), ix, iy, iz, fr);
     const size_t T = (size_t)1 << m.log2T;
+    const unsigned long long below = (1ull << lane) - 1ull;
     for (int c = 0; c < 8; ++c) {
         const uint32_t cx = ix + (c & 1), cy = iy + ((c >> 1) & 1), cz = iz + (c >> 2);
         const float w = ((c & 1) ? fr.x : 1.f - fr.x) * (((c >> 1) & 1) ? fr.y : 1.f - fr.y) * ((c >> 2) ? fr.z : 1.f - fr.z);
-        float* gt = m.gtable + 2 * ((size_t)lv * T + rc_mlp_index(m, lv, cx, cy, cz));
-        unsafeAtomicAdd(gt, w * g0);
-        unsafeAtomicAdd(gt + 1, w * g1);
+        const long long idx = act ? (long long)((size_t)lv * T + rc_mlp_index(m, lv, cx, cy, cz)) : -1 - lane;   // inactive: own run
+        float v0 = act ? w * g0 : 0.f, v1 = act ? w * g1 : 0.f;
+        const long long prev = __shfl_up(idx, 1, 64);
+        const unsigned long long heads = __ballot(lane == 0 || prev != idx);
+        const int seg = __popcll(heads & (below | (1ull << lane)));   // run number of this lane
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const float o0 = __shfl_up(v0, d, 64), o1 = __shfl_up(v1, d, 64);
+            const int os = __shfl_up(seg, d, 64);
+            if (lane >= d && os == seg) { v0 += o0; v1 += o1; }
+        }
+        const bool tail = lane == 63 || ((heads >> (lane + 1)) & 1ull);
+        if (act && tail) {
+            float* gt = m.gtable + 2 * idx;
+            unsafeAtomicAdd(gt, v0);
+            unsafeAtomicAdd(gt + 1, v1);
+        }
     }
 }
 
@@ -325,7 +346,11 @@ __global__ void __launch_bounds__(RC_TB) k_mlp_samples(rc_dscene s, rc_dstate st
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     uint32_t rs = rc_hash32(seed ^ rc_hash32((uint32_t)k + 1u));
-    const int i = (int)(rc_hash32(rs ^ 0x5bd1e995u) % (uint32_t)s.Np);
+    // stratified over the entries in entry order (one uniform pick in each of n equal strata):
+    // every entry equally likely, and consecutive samples spatially close (entries are stored
+    // in Morton order), which the table-gradient scatter exploits
+    const float u0 = rc_rnd(rs);
+    const int i = min(s.Np - 1, (int)(((double)k + (double)u0) * (double)s.Np / (double)n));
     const f3 P = ld3(st.P, i), N = ld3(st.N, i), O = P + N * 1e-3f;
     const float u1 = rc_rnd(rs), u2 = rc_rnd(rs);
     const f3 D = rc_cosine_dir(N, u1, u2);
