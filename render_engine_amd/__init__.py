@@ -37,6 +37,12 @@ def device_count():
     return int(lib.rc_device_count())
 
 
+def prep_device(on):
+    """Sample, reduce and Morton-order the cache cells of the scenes created from now on on the
+    GPU (True; the same cells bit for bit) or on the host (False, the default)."""
+    check(lib.rc_scene_prep_device(1 if on else 0))
+
+
 def _ptr(t):
     """device pointer of a torch tensor (or None -> NULL)"""
     if t is None:

@@ -77,6 +77,7 @@ _SIGS = {
     "rc_scene_upload": (C.c_int, [_P, C.c_int]),
     "rc_scene_set_trav": (C.c_int, [_P, C.c_int]),
     "rc_scene_set_fine_grid": (C.c_int, [_P, C.c_int]),
+    "rc_scene_prep_device": (C.c_int, [C.c_int]),
     "rc_scene_reference": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_float, C.POINTER(C.c_float),
                                      C.POINTER(C.c_int)]),
     "rc_cache_create": (C.c_int, [_P, C.POINTER(Method), C.POINTER(_P)]),

@@ -234,6 +234,12 @@ static rc_status need_device() {
     return RC_OK;
 }
 
+extern "C" rc_status rc_scene_prep_device(int on) {
+    if (on != 0 && on != 1) return fail(RC_ERR_ARG, "rc_scene_prep_device: on must be 0 or 1");
+    rc::g_prep_device = on != 0;
+    return RC_OK;
+}
+
 extern "C" rc_status rc_scene_set_fine_grid(rc_scene* s, int g) {
     if (!s || g < 0 || g > 1024) return fail(RC_ERR_ARG, "fine grid must be 0..1024");
     if (s->uploaded) return fail(RC_ERR_STATE, "fine grid must be set before upload");

@@ -12,6 +12,8 @@
 
 namespace rc {
 
+bool g_prep_device = false;
+
 uint64_t mix64(uint64_t x) {
     x ^= x >> 30; x *= 0xbf58476d1ce4e5b9ull; x ^= x >> 27; x *= 0x94d049bb133111ebull; x ^= x >> 31; return x;
 }
@@ -363,9 +365,15 @@ std::string prepare(const SceneDesc& d, Scene& S) {
 
     // 2. cache cells (h fixed, or adjusted toward the target count: cuda/prep_mesh.cuh:338-348)
     std::vector<uint64_t> keys; std::vector<CellRec> recs;
+    auto cells = [&]() -> std::string {
+        if (g_prep_device) return sample_cells_device(S.mesh, S.h, S.lo, keys, recs);
+        sample_cells(S.mesh, S.h, S.lo, keys, recs);
+        return "";
+    };
     if (d.h > 0.f) {
         S.h = d.h;
-        sample_cells(S.mesh, S.h, S.lo, keys, recs);
+        std::string e = cells();
+        if (!e.empty()) return e;
     } else {
         if (d.target_points <= 0) return "either h or target_points must be set";
         double atot = 0;
@@ -377,7 +385,8 @@ std::string prepare(const SceneDesc& d, Scene& S) {
         }
         S.h = (float)std::sqrt(atot / d.target_points);
         for (int it = 0; it < 5; ++it) {
-            sample_cells(S.mesh, S.h, S.lo, keys, recs);
+            std::string e = cells();
+            if (!e.empty()) return e;
             double ratio = (double)keys.size() / d.target_points;
             if (std::fabs(ratio - 1.0) < 0.08 || it == 4) break;
             S.h *= (float)std::min(2.0, std::max(0.5, std::sqrt(ratio)));

@@ -114,6 +114,10 @@ struct CellRec { float p[3]; float d2; float n[3]; double asum, wsum; int mat; }
 // Cache-cell sampling on the triangles (cuda/prep_mesh.cuh:129-157): returns cells sorted by
 // (morton*8 + bucket, key) with their representative point, normal and area-weighted albedo.
 void sample_cells(const Mesh& M, float h, const float lo[3], std::vector<uint64_t>& keys, std::vector<CellRec>& recs);
+// The same cells computed on the GPU (rc_prep.hip), bit for bit; "" on success, else the error.
+std::string sample_cells_device(const Mesh& M, float h, const float lo[3], std::vector<uint64_t>& keys, std::vector<CellRec>& recs);
+// Where prepare() samples the cache cells: false (default) host, true GPU (set by rc_scene_prep_device).
+extern bool g_prep_device;
 // Table of size 2^log2 (linear probing in the sorted-key order, cuda/prep_mesh.cuh:356-366).
 void build_hash(const std::vector<uint64_t>& keys, int log2, std::vector<uint64_t>& HK, std::vector<int>& HV);
 

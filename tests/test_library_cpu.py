@@ -161,3 +161,9 @@ def test_mesh_scene(tmp_path):
     _check_round_trip(s, str(tmp_path / "mesh.bin"))
     with pytest.raises(rea.RcError):
         rea.Scene.mesh(str(tmp_path / "room.bin"))
+
+
+def test_prep_device_flag():
+    """the cell-preparation switch takes 0 or 1 only (the device path itself is a GPU test)"""
+    assert rea.lib.rc_scene_prep_device(2) != 0
+    assert rea.lib.rc_scene_prep_device(0) == 0
