@@ -104,9 +104,9 @@ int rc_version(void);
 int rc_device_count(void);
 
 /* ---- scenes (cuda/poc_gpu.cu:751-801 load, cuda/prep_mesh.cuh:303-531 prep) ---- */
-/* Where scene preparation samples, reduces and Morton-orders the cache cells (cuda/prep_mesh.cuh:129-157)
- * for the scenes created after the call: on = 1 the GPU (same cells bit for bit), on = 0 the host
- * (default). Process-wide. */
+/* Where scene preparation builds the binned-SAH BVH (cuda/prep_mesh.cuh:42-114) and samples, reduces and
+ * Morton-orders the cache cells (:129-157) for the scenes created after the call: on = 1 the GPU (the
+ * same tree and cells bit for bit), on = 0 the host (default). Process-wide. */
 rc_status rc_scene_prep_device(int on);
 /* Procedural box room: 6 walls + 2 cubes (~target_tris), 2 point lights, states {both on, light 1 off}. */
 rc_status rc_scene_box_room(uint64_t seed, int target_tris, rc_scene** out);

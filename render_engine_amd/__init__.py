@@ -38,8 +38,9 @@ def device_count():
 
 
 def prep_device(on):
-    """Sample, reduce and Morton-order the cache cells of the scenes created from now on on the
-    GPU (True; the same cells bit for bit) or on the host (False, the default)."""
+    """Build the BVH and sample, reduce and Morton-order the cache cells of the scenes created
+    from now on on the GPU (True; the same tree and cells bit for bit) or on the host (False,
+    the default)."""
     check(lib.rc_scene_prep_device(1 if on else 0))
 
 

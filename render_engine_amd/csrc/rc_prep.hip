@@ -8,6 +8,9 @@
 // and area sums in the host's addition order), and the cells are ordered by Morton code.
 #include <hipcub/hipcub.hpp>
 
+#include <numeric>
+
+#include "rc_bvh.h"
 #include "rc_device.h"
 #include "rc_scene.h"
 
@@ -137,6 +140,291 @@ struct DevBuf {   // device allocations of one call, freed on every return path
     ~DevBuf() { for (void* q : p) (void)hipFree(q); }
 };
 
+
+// ------------------------------------------------------------------------------ BVH build
+// The host builder's breadth-first binned SAH (rc_bvh.h), level by level: items of a level
+// with at most 64 triangles are decided by one wave each; larger ones are cut into chunks of
+// RC_BCH triangles whose bounds and bins blocks reduce (exact min / max, integer counts), then
+// one wave per item combines its chunks and runs the same sweep. A stable partition (scan of
+// the per-triangle side flags) and a scan over the split items (children in item order) give
+// the host's triangle order and node numbering.
+#define RC_BCH 2048
+struct BItem { int node, start, count, depth; };
+struct BDec { int choice, bax, bs, nleft; float clo, chi, pad0, pad1; };
+
+__device__ __forceinline__ uint32_t pr_ord(float f) { const uint32_t b = __float_as_uint(f); return (b & 0x80000000u) ? ~b : (b | 0x80000000u); }
+__device__ __forceinline__ float pr_unord(uint32_t u) { return __uint_as_float((u & 0x80000000u) ? (u & 0x7fffffffu) : ~u); }
+__device__ __forceinline__ float pr_wmin(float v) { for (int o = 32; o > 0; o >>= 1) v = rc_fmin(v, __shfl_xor(v, o, 64)); return v; }
+__device__ __forceinline__ float pr_wmax(float v) { for (int o = 32; o > 0; o >>= 1) v = rc_fmax(v, __shfl_xor(v, o, 64)); return v; }
+
+__global__ void __launch_bounds__(256) k_bvh_tri(const float* V, int n, float4* tlo, float4* thi, float4* cen) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const float* v = V + 9 * (size_t)t;
+    float mn[3], mx[3], c[3];
+    for (int a = 0; a < 3; ++a) {
+        const float x0 = v[a] + 0.f, x1 = v[3 + a] + 0.f, x2 = v[6 + a] + 0.f;
+        mn[a] = rc_fmin(rc_fmin(x0, x1), x2); mx[a] = rc_fmax(rc_fmax(x0, x1), x2);
+        c[a] = 0.5f * (mn[a] + mx[a]) + 0.f;
+    }
+    tlo[t] = make_float4(mn[0], mn[1], mn[2], 0.f); thi[t] = make_float4(mx[0], mx[1], mx[2], 0.f); cen[t] = make_float4(c[0], c[1], c[2], 0.f);
+}
+
+// LDS bins of one axis as ordered integers (exact atomic min / max), then read back as floats
+struct PrBinsU { int c[RC_BVH_NB]; uint32_t lo[RC_BVH_NB][3], hi[RC_BVH_NB][3]; };
+__device__ __forceinline__ void pr_bins_clear(PrBinsU& B, int lane, int nl) {
+    for (int b = lane; b < RC_BVH_NB; b += nl) { B.c[b] = 0; for (int a = 0; a < 3; ++a) { B.lo[b][a] = pr_ord(1e30f); B.hi[b][a] = pr_ord(-1e30f); } }
+}
+__device__ __forceinline__ void pr_bins_add(PrBinsU& B, int b, float4 lo, float4 hi) {
+    atomicAdd(&B.c[b], 1);
+    atomicMin(&B.lo[b][0], pr_ord(lo.x)); atomicMin(&B.lo[b][1], pr_ord(lo.y)); atomicMin(&B.lo[b][2], pr_ord(lo.z));
+    atomicMax(&B.hi[b][0], pr_ord(hi.x)); atomicMax(&B.hi[b][1], pr_ord(hi.y)); atomicMax(&B.hi[b][2], pr_ord(hi.z));
+}
+__device__ __forceinline__ void pr_bins_float(const PrBinsU& U, rc_bins& B, int lane, int nl) {
+    for (int b = lane; b < RC_BVH_NB; b += nl) {
+        B.c[b] = U.c[b];
+        for (int a = 0; a < 3; ++a) { B.lo[b][a] = pr_unord(U.lo[b][a]); B.hi[b][a] = pr_unord(U.hi[b][a]); }
+    }
+}
+
+// shared tail of both paths (one wave, lane 0 decides): sweeps, choice, node bounds and decision
+struct PrSweepLds { rc_bins B; float slo[RC_BVH_NB][3], shi[RC_BVH_NB][3]; int sc[RC_BVH_NB]; };
+
+// items of at most 64 triangles: one wave (block of 64) per item, one triangle per lane
+__global__ void __launch_bounds__(64) k_bvh_small(const BItem* items, int m, const int* ord, const float4* tlo, const float4* thi,
+                                                 const float4* cen, rc_bvh_node* BN, BDec* dec) {
+    __shared__ PrBinsU U;
+    __shared__ PrSweepLds W;
+    const int k = blockIdx.x, lane = threadIdx.x;
+    if (k >= m) return;
+    const BItem it = items[k];
+    if (it.count > 64) return;   // chunked path
+    const bool live = lane < it.count;
+    const int t = live ? ord[it.start + lane] : 0;
+    const float4 l4 = live ? tlo[t] : make_float4(1e30f, 1e30f, 1e30f, 0.f), h4 = live ? thi[t] : make_float4(-1e30f, -1e30f, -1e30f, 0.f);
+    const float4 c4 = live ? cen[t] : make_float4(0.f, 0.f, 0.f, 0.f);
+    float nlo[3] = {pr_wmin(l4.x), pr_wmin(l4.y), pr_wmin(l4.z)}, nhi[3] = {pr_wmax(h4.x), pr_wmax(h4.y), pr_wmax(h4.z)};
+    float clo[3] = {pr_wmin(live ? c4.x : 1e30f), pr_wmin(live ? c4.y : 1e30f), pr_wmin(live ? c4.z : 1e30f)};
+    float chi[3] = {pr_wmax(live ? c4.x : -1e30f), pr_wmax(live ? c4.y : -1e30f), pr_wmax(live ? c4.z : -1e30f)};
+    int ax = 0;
+    for (int a = 1; a < 3; ++a) if (chi[a] - clo[a] > chi[ax] - clo[ax]) ax = a;
+    int choice = 0, bs = -1, bax = -1;
+    if (!rc_bvh_leaf_now(it.count, it.depth, chi[ax] - clo[ax])) {
+        float best = 1e30f;
+        for (int a0 = 0; a0 < 3; ++a0) {
+            if (chi[a0] - clo[a0] < 1e-7f) continue;
+            pr_bins_clear(U, lane, 64);
+            __syncthreads();
+            if (live) pr_bins_add(U, rc_bvh_bin(a0 == 0 ? c4.x : a0 == 1 ? c4.y : c4.z, clo[a0], chi[a0]), l4, h4);
+            __syncthreads();
+            pr_bins_float(U, W.B, lane, 64);
+            __syncthreads();
+            if (lane == 0) rc_bins_sweep(W.B, a0, best, bs, bax, W.slo, W.shi, W.sc);
+            __syncthreads();
+        }
+        if (lane == 0) choice = rc_bvh_choice(it.count, best, bs, (float)it.count * rc_bvh_area(nlo, nhi));
+    }
+    if (lane == 0) {
+        rc_bvh_node& nd = BN[it.node];
+        for (int a = 0; a < 3; ++a) { nd.lo[a] = nlo[a]; nd.hi[a] = nhi[a]; }
+        BDec d; d.choice = choice; d.bax = bax; d.bs = bs; d.nleft = it.count / 2;
+        d.clo = bax >= 0 ? clo[bax] : 0.f; d.chi = bax >= 0 ? chi[bax] : 0.f; d.pad0 = d.pad1 = 0.f;
+        dec[k] = d;
+    }
+}
+
+// chunks of the larger items: chunk count per item, then the chunk table
+__global__ void __launch_bounds__(256) k_bvh_nchunk(const BItem* items, int m, int* nch) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < m) nch[k] = items[k].count > 64 ? (items[k].count + RC_BCH - 1) / RC_BCH : 0;
+    if (k == m) nch[k] = 0;
+}
+__global__ void __launch_bounds__(256) k_bvh_chunks(const BItem* items, int m, const int* choff, int2* chunk) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= m) return;
+    for (int c = choff[k]; c < choff[k + 1]; ++c) chunk[c] = make_int2(k, c - choff[k]);
+}
+
+// per chunk: triangle box and centroid box (6 + 6 ordered integers)
+__global__ void __launch_bounds__(256) k_bvh_cbounds(const BItem* items, const int2* chunk, const int* ord, const float4* tlo, const float4* thi,
+                                                    const float4* cen, uint32_t* part) {
+    __shared__ uint32_t s[12];
+    const int2 ch = chunk[blockIdx.x];
+    const BItem it = items[ch.x];
+    const int j0 = it.start + ch.y * RC_BCH, j1 = min(it.start + it.count, j0 + RC_BCH);
+    if (threadIdx.x < 12) s[threadIdx.x] = threadIdx.x % 6 < 3 ? pr_ord(1e30f) : pr_ord(-1e30f);
+    __syncthreads();
+    float v[12] = {1e30f, 1e30f, 1e30f, -1e30f, -1e30f, -1e30f, 1e30f, 1e30f, 1e30f, -1e30f, -1e30f, -1e30f};
+    for (int j = j0 + threadIdx.x; j < j1; j += blockDim.x) {
+        const int t = ord[j];
+        const float4 l = tlo[t], h = thi[t], c = cen[t];
+        v[0] = rc_fmin(v[0], l.x); v[1] = rc_fmin(v[1], l.y); v[2] = rc_fmin(v[2], l.z);
+        v[3] = rc_fmax(v[3], h.x); v[4] = rc_fmax(v[4], h.y); v[5] = rc_fmax(v[5], h.z);
+        v[6] = rc_fmin(v[6], c.x); v[7] = rc_fmin(v[7], c.y); v[8] = rc_fmin(v[8], c.z);
+        v[9] = rc_fmax(v[9], c.x); v[10] = rc_fmax(v[10], c.y); v[11] = rc_fmax(v[11], c.z);
+    }
+    for (int q = 0; q < 12; ++q) { if (q % 6 < 3) atomicMin(&s[q], pr_ord(v[q])); else atomicMax(&s[q], pr_ord(v[q])); }
+    __syncthreads();
+    if (threadIdx.x < 12) part[12 * (size_t)blockIdx.x + threadIdx.x] = s[threadIdx.x];
+}
+
+// per large item (one wave): combine the chunk boxes; leaf test; sets the item's centroid ranges
+__global__ void __launch_bounds__(64) k_bvh_lbounds(const BItem* items, int m, const int* choff, const uint32_t* part, rc_bvh_node* BN,
+                                                   float* crange, int* needbins, BDec* dec) {
+    const int k = blockIdx.x, lane = threadIdx.x;
+    if (k >= m) return;
+    const BItem it = items[k];
+    if (it.count <= 64) return;
+    if (lane != 0) return;
+    uint32_t u[12];
+    for (int q = 0; q < 12; ++q) u[q] = q % 6 < 3 ? pr_ord(1e30f) : pr_ord(-1e30f);
+    for (int c = choff[k]; c < choff[k + 1]; ++c)
+        for (int q = 0; q < 12; ++q) { const uint32_t x = part[12 * (size_t)c + q]; u[q] = q % 6 < 3 ? min(u[q], x) : max(u[q], x); }
+    float nlo[3], nhi[3], clo[3], chi[3];
+    for (int a = 0; a < 3; ++a) { nlo[a] = pr_unord(u[a]); nhi[a] = pr_unord(u[3 + a]); clo[a] = pr_unord(u[6 + a]); chi[a] = pr_unord(u[9 + a]); }
+    rc_bvh_node& nd = BN[it.node];
+    for (int a = 0; a < 3; ++a) { nd.lo[a] = nlo[a]; nd.hi[a] = nhi[a]; crange[6 * (size_t)k + a] = clo[a]; crange[6 * (size_t)k + 3 + a] = chi[a]; }
+    int ax = 0;
+    for (int a = 1; a < 3; ++a) if (chi[a] - clo[a] > chi[ax] - clo[ax]) ax = a;
+    const bool leaf = rc_bvh_leaf_now(it.count, it.depth, chi[ax] - clo[ax]);
+    needbins[k] = leaf ? 0 : 1;
+    if (leaf) { BDec d; d.choice = 0; d.bax = -1; d.bs = -1; d.nleft = it.count / 2; d.clo = d.chi = d.pad0 = d.pad1 = 0.f; dec[k] = d; }
+}
+
+// per chunk of an item that needs bins: the three axes' bins in LDS, then to global
+__global__ void __launch_bounds__(256) k_bvh_cbins(const BItem* items, const int2* chunk, const int* needbins, const float* crange, const int* ord,
+                                                  const float4* tlo, const float4* thi, const float4* cen, PrBinsU* part) {
+    __shared__ PrBinsU U[3];
+    const int2 ch = chunk[blockIdx.x];
+    if (!needbins[ch.x]) return;
+    const BItem it = items[ch.x];
+    const float* cr = crange + 6 * (size_t)ch.x;
+    for (int a0 = 0; a0 < 3; ++a0) pr_bins_clear(U[a0], threadIdx.x, blockDim.x);
+    __syncthreads();
+    const int j0 = it.start + ch.y * RC_BCH, j1 = min(it.start + it.count, j0 + RC_BCH);
+    for (int j = j0 + threadIdx.x; j < j1; j += blockDim.x) {
+        const int t = ord[j];
+        const float4 l = tlo[t], h = thi[t], c = cen[t];
+        const float cc[3] = {c.x, c.y, c.z};
+        for (int a0 = 0; a0 < 3; ++a0)
+            if (cr[3 + a0] - cr[a0] >= 1e-7f) pr_bins_add(U[a0], rc_bvh_bin(cc[a0], cr[a0], cr[3 + a0]), l, h);
+    }
+    __syncthreads();
+    PrBinsU* out = part + 3 * (size_t)blockIdx.x;
+    for (int e = threadIdx.x; e < 3 * RC_BVH_NB; e += blockDim.x) {
+        const int a0 = e / RC_BVH_NB, b = e % RC_BVH_NB;
+        out[a0].c[b] = U[a0].c[b];
+        for (int a = 0; a < 3; ++a) { out[a0].lo[b][a] = U[a0].lo[b][a]; out[a0].hi[b][a] = U[a0].hi[b][a]; }
+    }
+}
+
+// per large item needing bins (one wave): combine the chunks' bins, sweep, choose
+__global__ void __launch_bounds__(64) k_bvh_ldecide(const BItem* items, int m, const int* choff, const int* needbins, const float* crange,
+                                                   const PrBinsU* part, const rc_bvh_node* BN, BDec* dec) {
+    __shared__ PrBinsU U;
+    __shared__ PrSweepLds W;
+    const int k = blockIdx.x, lane = threadIdx.x;
+    if (k >= m) return;
+    const BItem it = items[k];
+    if (it.count <= 64 || !needbins[k]) return;
+    const float* cr = crange + 6 * (size_t)k;
+    float best = 1e30f;
+    int bs = -1, bax = -1;
+    for (int a0 = 0; a0 < 3; ++a0) {
+        if (cr[3 + a0] - cr[a0] < 1e-7f) continue;
+        if (lane < RC_BVH_NB) {
+            int c = 0;
+            uint32_t lo[3] = {pr_ord(1e30f), pr_ord(1e30f), pr_ord(1e30f)}, hi[3] = {pr_ord(-1e30f), pr_ord(-1e30f), pr_ord(-1e30f)};
+            for (int q = choff[k]; q < choff[k + 1]; ++q) {
+                const PrBinsU& P = part[3 * (size_t)q + a0];
+                c += P.c[lane];
+                for (int a = 0; a < 3; ++a) { lo[a] = min(lo[a], P.lo[lane][a]); hi[a] = max(hi[a], P.hi[lane][a]); }
+            }
+            U.c[lane] = c;
+            for (int a = 0; a < 3; ++a) { U.lo[lane][a] = lo[a]; U.hi[lane][a] = hi[a]; }
+        }
+        __syncthreads();
+        pr_bins_float(U, W.B, lane, 64);
+        __syncthreads();
+        if (lane == 0) rc_bins_sweep(W.B, a0, best, bs, bax, W.slo, W.shi, W.sc);
+        __syncthreads();
+    }
+    if (lane == 0) {
+        const rc_bvh_node nd = BN[it.node];
+        BDec d; d.choice = rc_bvh_choice(it.count, best, bs, (float)it.count * rc_bvh_area(nd.lo, nd.hi));
+        d.bax = bax; d.bs = bs; d.nleft = it.count / 2;
+        d.clo = bax >= 0 ? cr[bax] : 0.f; d.chi = bax >= 0 ? cr[3 + bax] : 0.f; d.pad0 = d.pad1 = 0.f;
+        dec[k] = d;
+    }
+}
+
+// element -> item of the level: a head mark at each item's start, then a max-scan
+__global__ void __launch_bounds__(256) k_bvh_marks(const BItem* items, int m, int* mark) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < m && items[k].count > 0) mark[items[k].start] = k + 1;
+}
+// side flag of every element of a split item (1: left); 0 elsewhere
+__global__ void __launch_bounds__(256) k_bvh_flags(int n, const int* eitem, const BItem* items, const BDec* dec, const int* ord,
+                                                  const float4* cen, int* flag) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j > n) return;
+    int f = 0;
+    if (j < n) {
+        const int k = eitem[j] - 1;
+        if (k >= 0) {
+            const BItem it = items[k];
+            if (j < it.start + it.count) {
+                const BDec d = dec[k];
+                if (d.choice == 1) {
+                    const float4 c = cen[ord[j]];
+                    f = rc_bvh_bin(d.bax == 0 ? c.x : d.bax == 1 ? c.y : c.z, d.clo, d.chi) < d.bs ? 1 : 0;
+                } else if (d.choice == 2) f = j - it.start < d.nleft ? 1 : 0;
+            }
+        }
+    }
+    flag[j] = f;
+}
+// stable partition inside each split item; other elements stay
+__global__ void __launch_bounds__(256) k_bvh_scatter(int n, const int* eitem, const BItem* items, const BDec* dec, const int* flag,
+                                                    const int* scan, const int* ord, int* ord2) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    int pos = j;
+    const int k = eitem[j] - 1;
+    if (k >= 0) {
+        const BItem it = items[k];
+        if (j < it.start + it.count && dec[k].choice != 0) {
+            const int pre = scan[j] - scan[it.start], nleft = scan[it.start + it.count] - scan[it.start];
+            pos = flag[j] ? it.start + pre : it.start + nleft + (j - it.start - pre);
+        }
+    }
+    ord2[pos] = ord[j];
+}
+__global__ void __launch_bounds__(256) k_bvh_splitflag(const BDec* dec, int m, int* sf) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < m) sf[k] = dec[k].choice != 0 ? 1 : 0;
+    if (k == m) sf[k] = 0;
+}
+// children of the split items (in item order) and the nodes' child / leaf fields
+__global__ void __launch_bounds__(256) k_bvh_children(const BItem* items, int m, const BDec* dec, const int* sfscan, const int* scan, int nodes0,
+                                                     rc_bvh_node* BN, BItem* next) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= m) return;
+    const BItem it = items[k];
+    const BDec d = dec[k];
+    rc_bvh_node& nd = BN[it.node];
+    if (d.choice == 0) { nd.lf = it.start; nd.cnt = it.count; return; }
+    const int r = sfscan[k], left = nodes0 + 2 * r;
+    const int nleft = scan[it.start + it.count] - scan[it.start];
+    nd.lf = left; nd.cnt = 0;
+    BItem a, b;
+    a.node = left; a.start = it.start; a.count = nleft; a.depth = it.depth + 1;
+    b.node = left + 1; b.start = it.start + nleft; b.count = it.count - nleft; b.depth = it.depth + 1;
+    next[2 * r] = a; next[2 * r + 1] = b;
+}
+
+struct PrMax { __device__ __forceinline__ int operator()(int a, int b) const { return a > b ? a : b; } };
+
 }  // namespace
 
 namespace rc {
@@ -217,6 +505,90 @@ std::string sample_cells_device(const Mesh& M, float h, const float lo[3], std::
         r.asum = hw[c].x; r.wsum = hw[c].y; r.mat = hm[c];
         keys_out[j] = hk[c];
     }
+    return "";
+}
+
+
+std::string build_bvh_device(const Mesh& M, std::vector<rc_bvh_node>& nodes_out, std::vector<int>& order_out, int& depth_out) {
+    const int n = M.nt();
+    hipStream_t st = nullptr;
+    DevBuf B;
+    float* dV; float4 *tlo, *thi, *cen; int *ord, *ord2, *mark, *eitem, *flag, *scan, *nch, *choff, *sf, *sfscan, *needbins;
+    BItem *items, *next; BDec* dec; rc_bvh_node* BN; float* crange;
+    const size_t cap = (size_t)n + 2;
+    PK(B.get(&dV, 9 * (size_t)std::max(n, 1))); PK(B.get(&tlo, cap)); PK(B.get(&thi, cap)); PK(B.get(&cen, cap));
+    PK(B.get(&ord, cap)); PK(B.get(&ord2, cap)); PK(B.get(&mark, cap)); PK(B.get(&eitem, cap)); PK(B.get(&flag, cap)); PK(B.get(&scan, cap));
+    PK(B.get(&nch, cap)); PK(B.get(&choff, cap)); PK(B.get(&sf, cap)); PK(B.get(&sfscan, cap)); PK(B.get(&needbins, cap));
+    PK(B.get(&items, cap)); PK(B.get(&next, cap)); PK(B.get(&dec, cap)); PK(B.get(&BN, 2 * cap)); PK(B.get(&crange, 6 * cap));
+    if (n > 0) PK(hipMemcpyAsync(dV, M.V.data(), 36 * (size_t)n, hipMemcpyHostToDevice, st));
+    std::vector<int> iota(std::max(n, 1));
+    std::iota(iota.begin(), iota.end(), 0);
+    PK(hipMemcpyAsync(ord, iota.data(), 4 * (size_t)n, hipMemcpyHostToDevice, st));
+    const dim3 b256(256), gn((n + 256) / 256);
+    if (n > 0) hipLaunchKernelGGL(k_bvh_tri, gn, b256, 0, st, dV, n, tlo, thi, cen);
+    BItem root{0, 0, n, 0};
+    PK(hipMemcpyAsync(items, &root, sizeof root, hipMemcpyHostToDevice, st));
+    // temp storage for the scans (sized for the largest)
+    size_t tb1 = 0, tb2 = 0;
+    PK(hipcub::DeviceScan::ExclusiveSum(nullptr, tb1, flag, scan, n + 1, st));
+    PK(hipcub::DeviceScan::InclusiveScan(nullptr, tb2, mark, eitem, PrMax(), n, st));
+    void* tmp; PK(B.get((char**)&tmp, std::max(tb1, tb2) + 256));
+    const size_t tbs = std::max(tb1, tb2) + 256;
+    int m = 1, nodes = 1, depth = 0;
+    uint32_t* cpart = nullptr; PrBinsU* bpart = nullptr; int2* chunk = nullptr; size_t ccap = 0;
+    while (m > 0) {
+        depth = std::max(depth, 0);
+        const dim3 gm((m + 256) / 256);
+        // decisions: small items by one wave each, large items by chunks
+        hipLaunchKernelGGL(k_bvh_small, dim3(m), dim3(64), 0, st, items, m, ord, tlo, thi, cen, BN, dec);
+        hipLaunchKernelGGL(k_bvh_nchunk, gm, b256, 0, st, items, m, nch);
+        size_t tb = tbs;
+        PK(hipcub::DeviceScan::ExclusiveSum(tmp, tb, nch, choff, m + 1, st));
+        int nck = 0;
+        PK(hipMemcpyAsync(&nck, choff + m, 4, hipMemcpyDeviceToHost, st));
+        PK(hipStreamSynchronize(st));
+        if (nck > 0) {
+            if ((size_t)nck > ccap) {
+                ccap = (size_t)nck * 2;
+                PK(B.get(&cpart, 12 * ccap)); PK(B.get(&bpart, 3 * ccap)); PK(B.get(&chunk, ccap));
+            }
+            hipLaunchKernelGGL(k_bvh_chunks, gm, b256, 0, st, items, m, choff, chunk);
+            hipLaunchKernelGGL(k_bvh_cbounds, dim3(nck), b256, 0, st, items, chunk, ord, tlo, thi, cen, cpart);
+            hipLaunchKernelGGL(k_bvh_lbounds, dim3(m), dim3(64), 0, st, items, m, choff, cpart, BN, crange, needbins, dec);
+            hipLaunchKernelGGL(k_bvh_cbins, dim3(nck), b256, 0, st, items, chunk, needbins, crange, ord, tlo, thi, cen, bpart);
+            hipLaunchKernelGGL(k_bvh_ldecide, dim3(m), dim3(64), 0, st, items, m, choff, needbins, crange, bpart, BN, dec);
+        }
+        // partition and children
+        PK(hipMemsetAsync(mark, 0, 4 * (size_t)std::max(n, 1), st));
+        hipLaunchKernelGGL(k_bvh_marks, gm, b256, 0, st, items, m, mark);
+        if (n > 0) {
+            tb = tbs;
+            PK(hipcub::DeviceScan::InclusiveScan(tmp, tb, mark, eitem, PrMax(), n, st));
+        }
+        hipLaunchKernelGGL(k_bvh_flags, gn, b256, 0, st, n, eitem, items, dec, ord, cen, flag);
+        tb = tbs;
+        PK(hipcub::DeviceScan::ExclusiveSum(tmp, tb, flag, scan, n + 1, st));
+        if (n > 0) hipLaunchKernelGGL(k_bvh_scatter, gn, b256, 0, st, n, eitem, items, dec, flag, scan, ord, ord2);
+        hipLaunchKernelGGL(k_bvh_splitflag, gm, b256, 0, st, dec, m, sf);
+        tb = tbs;
+        PK(hipcub::DeviceScan::ExclusiveSum(tmp, tb, sf, sfscan, m + 1, st));
+        hipLaunchKernelGGL(k_bvh_children, gm, b256, 0, st, items, m, dec, sfscan, scan, nodes, BN, next);
+        PK(hipGetLastError());
+        int nsplit = 0;
+        PK(hipMemcpyAsync(&nsplit, sfscan + m, 4, hipMemcpyDeviceToHost, st));
+        PK(hipStreamSynchronize(st));
+        if (n > 0) std::swap(ord, ord2);
+        std::swap(items, next);
+        nodes += 2 * nsplit;
+        if (nsplit > 0) ++depth;
+        m = 2 * nsplit;
+    }
+    nodes_out.resize(nodes);
+    order_out.resize(n);
+    PK(hipMemcpyAsync(nodes_out.data(), BN, sizeof(rc_bvh_node) * (size_t)nodes, hipMemcpyDeviceToHost, st));
+    if (n > 0) PK(hipMemcpyAsync(order_out.data(), ord, 4 * (size_t)n, hipMemcpyDeviceToHost, st));
+    PK(hipStreamSynchronize(st));
+    depth_out = depth;
     return "";
 }
 

@@ -1,5 +1,6 @@
 // Host-side scene preparation (see rc_scene.h).
 #include "rc_scene.h"
+#include "rc_bvh.h"
 
 #include <algorithm>
 #include <limits>
@@ -59,11 +60,9 @@ void Mesh::add_quad_grid(const float o[3], const float u[3], const float v[3], i
 }
 
 // ------------------------------------------------------------------------------------ BVH
-static float area_of(const float* lo, const float* hi) {
-    float d0 = hi[0] - lo[0], d1 = hi[1] - lo[1], d2 = hi[2] - lo[2];
-    return d0 < 0 ? 0.f : 2.f * (d0 * d1 + d1 * d2 + d2 * d0);
-}
-
+// Level by level (breadth first): the nodes of a level are split in order and each split
+// appends its two children, so the node numbering is the device builder's; partitions are
+// stable. Triangle boxes use +0 for zero coordinates, so every min / max is order-free.
 void build_bvh(const Mesh& M, std::vector<rc_bvh_node>& nodes, std::vector<int>& order, int& depth_out) {
     const int n = M.nt();
     order.resize(n);
@@ -72,91 +71,72 @@ void build_bvh(const Mesh& M, std::vector<rc_bvh_node>& nodes, std::vector<int>&
     for (int t = 0; t < n; ++t)
         for (int a = 0; a < 3; ++a) {
             const float* v = &M.V[9 * (size_t)t];
-            float x0 = v[a], x1 = v[3 + a], x2 = v[6 + a];
-            float mn = std::min(x0, std::min(x1, x2)), mx = std::max(x0, std::max(x1, x2));
+            const float x0 = v[a] + 0.f, x1 = v[3 + a] + 0.f, x2 = v[6 + a] + 0.f;
+            const float mn = rc_fmin(rc_fmin(x0, x1), x2), mx = rc_fmax(rc_fmax(x0, x1), x2);
             tlo[3 * (size_t)t + a] = mn; thi[3 * (size_t)t + a] = mx;
-            cen[3 * (size_t)t + a] = 0.5f * (mn + mx);
+            cen[3 * (size_t)t + a] = 0.5f * (mn + mx) + 0.f;
         }
     nodes.clear();
     nodes.reserve(2 * (size_t)n + 1);
     nodes.push_back(rc_bvh_node{});
     struct Item { int node, start, count, depth; };
-    std::vector<Item> work{{0, 0, n, 0}};
-    // 32 centroid bins on each of the three axes; the cheapest split over all of them wins.
-    // Leaves hold at most LEAF triangles: the wide traversal's triangle tests cost more than its
-    // box tests, so single-triangle leaves (the SAH splits every pair) trace fastest.
-    const int NB = 32, LEAF = 2;
+    std::vector<Item> cur{{0, 0, n, 0}}, next;
+    std::vector<int> tmp;
+    rc_bins B[3];
+    float slo[RC_BVH_NB][3], shi[RC_BVH_NB][3];
+    int sc[RC_BVH_NB];
     depth_out = 0;
-    while (!work.empty()) {
-        Item it = work.back(); work.pop_back();
-        depth_out = std::max(depth_out, it.depth);
-        float nlo[3] = {1e30f, 1e30f, 1e30f}, nhi[3] = {-1e30f, -1e30f, -1e30f};
-        float clo[3] = {1e30f, 1e30f, 1e30f}, chi[3] = {-1e30f, -1e30f, -1e30f};
-        for (int k = it.start; k < it.start + it.count; ++k) {
-            int t = order[k];
-            for (int a = 0; a < 3; ++a) {
-                nlo[a] = std::min(nlo[a], tlo[3 * (size_t)t + a]); nhi[a] = std::max(nhi[a], thi[3 * (size_t)t + a]);
-                clo[a] = std::min(clo[a], cen[3 * (size_t)t + a]); chi[a] = std::max(chi[a], cen[3 * (size_t)t + a]);
-            }
-        }
-        rc_bvh_node& nd = nodes[it.node];
-        for (int a = 0; a < 3; ++a) { nd.lo[a] = nlo[a]; nd.hi[a] = nhi[a]; }
-        int ax = 0;
-        for (int a = 1; a < 3; ++a) if (chi[a] - clo[a] > chi[ax] - clo[ax]) ax = a;
-        auto make_leaf = [&]() { nodes[it.node].lf = it.start; nodes[it.node].cnt = it.count; };
-        if ((it.count <= LEAF || it.depth >= 56 || chi[ax] - clo[ax] < 1e-7f) && (it.count <= 64 || it.depth >= 56)) { make_leaf(); continue; }
-        auto bin_of = [&](int t, int a) {
-            return std::min(NB - 1, (int)((cen[3 * (size_t)t + a] - clo[a]) / (chi[a] - clo[a]) * NB));
-        };
-        float best = 1e30f; int bs = -1, bax = -1;
-        for (int a0 = 0; a0 < 3; ++a0) {
-            if (chi[a0] - clo[a0] < 1e-7f) continue;
-            int bc[NB] = {0};
-            float blo[NB][3], bhi[NB][3];
-            for (int b = 0; b < NB; ++b) for (int a = 0; a < 3; ++a) { blo[b][a] = 1e30f; bhi[b][a] = -1e30f; }
+    while (!cur.empty()) {
+        next.clear();
+        for (const Item& it : cur) {
+            depth_out = std::max(depth_out, it.depth);
+            float nlo[3] = {1e30f, 1e30f, 1e30f}, nhi[3] = {-1e30f, -1e30f, -1e30f};
+            float clo[3] = {1e30f, 1e30f, 1e30f}, chi[3] = {-1e30f, -1e30f, -1e30f};
             for (int k = it.start; k < it.start + it.count; ++k) {
-                int t = order[k], b = bin_of(t, a0);
-                bc[b]++;
-                for (int a = 0; a < 3; ++a) { blo[b][a] = std::min(blo[b][a], tlo[3 * (size_t)t + a]); bhi[b][a] = std::max(bhi[b][a], thi[3 * (size_t)t + a]); }
-            }
-            // prefix / suffix sweeps over the bins
-            float plo[NB][3], phi[NB][3], slo[NB][3], shi[NB][3];
-            int pc[NB], sc[NB];
-            for (int b = 0; b < NB; ++b)
+                const int t = order[k];
                 for (int a = 0; a < 3; ++a) {
-                    plo[b][a] = std::min(b ? plo[b - 1][a] : 1e30f, blo[b][a]); phi[b][a] = std::max(b ? phi[b - 1][a] : -1e30f, bhi[b][a]);
+                    nlo[a] = rc_fmin(nlo[a], tlo[3 * (size_t)t + a]); nhi[a] = rc_fmax(nhi[a], thi[3 * (size_t)t + a]);
+                    clo[a] = rc_fmin(clo[a], cen[3 * (size_t)t + a]); chi[a] = rc_fmax(chi[a], cen[3 * (size_t)t + a]);
                 }
-            for (int b = 0; b < NB; ++b) pc[b] = (b ? pc[b - 1] : 0) + bc[b];
-            for (int b = NB - 1; b >= 0; --b) {
-                for (int a = 0; a < 3; ++a) {
-                    slo[b][a] = std::min(b < NB - 1 ? slo[b + 1][a] : 1e30f, blo[b][a]); shi[b][a] = std::max(b < NB - 1 ? shi[b + 1][a] : -1e30f, bhi[b][a]);
+            }
+            rc_bvh_node& nd = nodes[it.node];
+            for (int a = 0; a < 3; ++a) { nd.lo[a] = nlo[a]; nd.hi[a] = nhi[a]; }
+            int ax = 0;
+            for (int a = 1; a < 3; ++a) if (chi[a] - clo[a] > chi[ax] - clo[ax]) ax = a;
+            int choice = 0, bs = -1, bax = -1;
+            if (!rc_bvh_leaf_now(it.count, it.depth, chi[ax] - clo[ax])) {
+                float best = 1e30f;
+                for (int a0 = 0; a0 < 3; ++a0) {
+                    if (chi[a0] - clo[a0] < 1e-7f) continue;
+                    rc_bins_clear(B[a0]);
+                    for (int k = it.start; k < it.start + it.count; ++k) {
+                        const int t = order[k], b = rc_bvh_bin(cen[3 * (size_t)t + a0], clo[a0], chi[a0]);
+                        B[a0].c[b]++;
+                        for (int a = 0; a < 3; ++a) {
+                            B[a0].lo[b][a] = rc_fmin(B[a0].lo[b][a], tlo[3 * (size_t)t + a]);
+                            B[a0].hi[b][a] = rc_fmax(B[a0].hi[b][a], thi[3 * (size_t)t + a]);
+                        }
+                    }
+                    rc_bins_sweep(B[a0], a0, best, bs, bax, slo, shi, sc);
                 }
-                sc[b] = (b < NB - 1 ? sc[b + 1] : 0) + bc[b];
+                choice = rc_bvh_choice(it.count, best, bs, (float)it.count * rc_bvh_area(nlo, nhi));
             }
-            for (int s = 1; s < NB; ++s) {
-                int nl = pc[s - 1], nr = sc[s];
-                if (!nl || !nr) continue;
-                float c = nl * area_of(plo[s - 1], phi[s - 1]) + nr * area_of(slo[s], shi[s]);
-                if (c < best) { best = c; bs = s; bax = a0; }
+            if (choice == 0) { nd.lf = it.start; nd.cnt = it.count; continue; }
+            int mid = it.start + it.count / 2;
+            if (choice == 1) {
+                auto pm = std::stable_partition(order.begin() + it.start, order.begin() + it.start + it.count, [&](int t) {
+                    return rc_bvh_bin(cen[3 * (size_t)t + bax], clo[bax], chi[bax]) < bs;
+                });
+                mid = (int)(pm - order.begin());
             }
+            const int left = (int)nodes.size();
+            nodes.push_back(rc_bvh_node{});
+            nodes.push_back(rc_bvh_node{});
+            nodes[it.node].lf = left; nodes[it.node].cnt = 0;
+            next.push_back({left, it.start, mid - it.start, it.depth + 1});
+            next.push_back({left + 1, mid, it.start + it.count - mid, it.depth + 1});
         }
-        int mid = -1;
-        const float leaf_cost = it.count * area_of(nlo, nhi);
-        if (bs > 0 && !(it.count <= LEAF && best >= leaf_cost)) {
-            auto pm = std::partition(order.begin() + it.start, order.begin() + it.start + it.count, [&](int t) { return bin_of(t, bax) < bs; });
-            mid = (int)(pm - order.begin());
-        } else if (it.count <= LEAF) { make_leaf(); continue; }
-        if (mid <= it.start || mid >= it.start + it.count) {
-            mid = it.start + it.count / 2;
-            std::nth_element(order.begin() + it.start, order.begin() + mid, order.begin() + it.start + it.count,
-                             [&](int a, int b) { return cen[3 * (size_t)a + ax] < cen[3 * (size_t)b + ax]; });
-        }
-        int left = (int)nodes.size();
-        nodes.push_back(rc_bvh_node{});
-        nodes.push_back(rc_bvh_node{});
-        nodes[it.node].lf = left; nodes[it.node].cnt = 0;
-        work.push_back({left, it.start, mid - it.start, it.depth + 1});
-        work.push_back({left + 1, mid, it.start + it.count - mid, it.depth + 1});
+        std::swap(cur, next);
     }
 }
 
@@ -178,7 +158,7 @@ void build_wide(const std::vector<rc_bvh_node>& bn, std::vector<rc_wnode>& wn, i
             for (size_t k = 0; k < kids.size(); ++k) {
                 const rc_bvh_node& c = bn[kids[k]];
                 if (c.cnt > 0) continue;
-                float a = area_of(c.lo, c.hi);
+                float a = rc_bvh_area(c.lo, c.hi);
                 if (a > ba) { ba = a; best = (int)k; }
             }
             if (best < 0) break;
@@ -343,7 +323,12 @@ std::string prepare(const SceneDesc& d, Scene& S) {
     S.sun_len = 2.f * S.diag;
 
     // 1. BVH + triangles in leaf order (v0, e1, e2) with normal + double-sided flag
-    build_bvh(M, S.BN, S.order, S.bvh_depth);
+    if (g_prep_device) {
+        std::string e = build_bvh_device(M, S.BN, S.order, S.bvh_depth);
+        if (!e.empty()) return e;
+    } else {
+        build_bvh(M, S.BN, S.order, S.bvh_depth);
+    }
     build_wide(S.BN, S.WN, S.wide_depth);
     if (3 * S.wide_depth + 1 > RC_WSTACK) return "wide BVH too deep for the LDS traversal stack";
     if (S.WN.size() >= (1u << 24)) return "wide BVH has more than 2^24 nodes";

@@ -101,11 +101,14 @@ struct Scene {
 // disappears between states a and b (both placements), for every ordered pair.
 void fine_events(Scene& S, int g);
 
-// Binned-SAH binary BVH over the mesh: 32 bins on each of the three axes (the reference bins
-// 16 along the longest one, cuda/prep_mesh.cuh:42-114), leaves of <= 2 triangles or when SAH
-// says so, median split fallback, depth <= 56. Hits do not depend on the tree's shape: the
+// Binned-SAH binary BVH over the mesh (split rule in rc_bvh.h: 32 bins on each of the three
+// axes, leaves of <= 2 triangles or when SAH says so, a middle split for equal centroids,
+// depth <= 56; the reference bins 16 along the longest axis, cuda/prep_mesh.cuh:42-114), built
+// breadth first with stable partitions. Hits do not depend on the tree's shape: the
 // closest-hit rule orders ties by triangle index.
 void build_bvh(const Mesh& M, std::vector<rc_bvh_node>& nodes, std::vector<int>& order, int& depth);
+// The same tree built on the GPU (rc_prep.hip), bit for bit; "" on success, else the error.
+std::string build_bvh_device(const Mesh& M, std::vector<rc_bvh_node>& nodes, std::vector<int>& order, int& depth);
 // Collapse of the binary BVH into 4-wide nodes (children: up to 4 descendants, leaves keep
 // their triangle ranges).
 void build_wide(const std::vector<rc_bvh_node>& bn, std::vector<rc_wnode>& wn, int& depth);
@@ -116,7 +119,8 @@ struct CellRec { float p[3]; float d2; float n[3]; double asum, wsum; int mat; }
 void sample_cells(const Mesh& M, float h, const float lo[3], std::vector<uint64_t>& keys, std::vector<CellRec>& recs);
 // The same cells computed on the GPU (rc_prep.hip), bit for bit; "" on success, else the error.
 std::string sample_cells_device(const Mesh& M, float h, const float lo[3], std::vector<uint64_t>& keys, std::vector<CellRec>& recs);
-// Where prepare() samples the cache cells: false (default) host, true GPU (set by rc_scene_prep_device).
+// Where prepare() builds the BVH and samples the cache cells: false (default) host, true GPU
+// (set by rc_scene_prep_device).
 extern bool g_prep_device;
 // Table of size 2^log2 (linear probing in the sorted-key order, cuda/prep_mesh.cuh:356-366).
 void build_hash(const std::vector<uint64_t>& keys, int log2, std::vector<uint64_t>& HK, std::vector<int>& HV);

@@ -1,6 +1,7 @@
-"""Scene preparation on the GPU (cuda/prep_mesh.cuh:129-157): the cache cells sampled, reduced and
-Morton-ordered by the device path equal the host path's bit for bit, so every scene array built
-from them does too (entry points, normals, albedos, materials, zones, hash table)."""
+"""Scene preparation on the GPU: the binned-SAH BVH (cuda/prep_mesh.cuh:42-114) and the cache cells
+sampled, reduced and Morton-ordered (:129-157) by the device path equal the host path's bit for
+bit, so every scene array built from them does too (nodes, triangle order, wide nodes, entry
+points, normals, albedos, materials, zones, hash table)."""
 import time
 
 import numpy as np
@@ -10,8 +11,8 @@ import render_engine_amd as rea
 
 pytestmark = pytest.mark.gpu
 
-INFO = ("Np", "npst", "nbox", "NR", "hsize", "nmat", "nst", "nnodes", "nwnodes")
-ARRAYS = ("ALB", "REGION", "HK", "HV", "MAT", "MATALB", "ST", "BN")
+INFO = ("Np", "npst", "nbox", "NR", "hsize", "nmat", "nst", "nnodes", "nwnodes", "bvh_depth", "wide_depth")
+ARRAYS = ("ALB", "REGION", "HK", "HV", "MAT", "MATALB", "ST", "STN", "BN", "WN", "ORDER")
 
 
 def _both(make):
@@ -58,7 +59,7 @@ def test_prep_device_street_full_size(torch_gpu):
     """the 960x540 street (199,978 triangles, ~530k cells, cell size adjusted toward the target)"""
     a, b, th, td = _both(lambda: rea.Scene.street(1, 2000, 200_000, 32, True, 530_000, 22, 960, 540, 4, 3))
     _same(a, b)
-    print(f"prepare: host cells {th:.2f} s, device cells {td:.2f} s (whole scene preparation)")
+    print(f"scene preparation: host {th:.2f} s, device BVH + cells {td:.2f} s")
 
 
 def test_prep_device_double_sided_mesh(torch_gpu, tmp_path):
@@ -74,3 +75,10 @@ def test_prep_device_double_sided_mesh(torch_gpu, tmp_path):
         f.write(V.astype(np.float32).tobytes()); f.write(alb.astype(np.float32).tobytes()); f.write(ds.tobytes())
     a, b, _, _ = _both(lambda: rea.Scene.mesh(path, target_points=12000)[0])
     _same(a, b)
+
+
+def test_prep_device_street_1m(torch_gpu):
+    """the 1920x1080 street (~1M triangles, ~1M cells): chunked node bounds and bins at the top levels"""
+    a, b, th, td = _both(lambda: rea.Scene.street(1, 2000, 1_000_000, 64, True, 1_000_000, 23, 1920, 1080, 4, 3))
+    _same(a, b)
+    print(f"scene preparation (1M triangles): host {th:.2f} s, device BVH + cells {td:.2f} s")
