@@ -209,6 +209,9 @@ __global__ void __launch_bounds__(RC_TB) k_decide(rc_dscene s, rc_dcache c, rc_p
 // (Lc, Edir) and of the SoA sample means, so the block waits for memory once; the per-entry
 // factors are computed by the first wave meanwhile, then the blend runs over the span in flat
 // order (coalesced Lc2 stores).
+// UE: channel elements per thread (ceil(RC_UT * nl / RC_TB) at most), instantiated for a few
+// light counts so that small ones keep few registers (more waves in flight)
+template <int UE>
 __global__ void __launch_bounds__(RC_TB) k_update(rc_dscene s, rc_dstate st, rc_dcache c, rc_params p, const uint64_t* lnew) {
     __shared__ float s_al[RC_UT], s_alr[RC_UT], s_A[RC_UT];
     extern __shared__ float sm[];                 // [nl][RC_UT] sample means, then [RC_UT][nl] terms
@@ -223,14 +226,14 @@ __global__ void __launch_bounds__(RC_TB) k_update(rc_dscene s, rc_dstate st, rc_
     const float* __restrict__ Lc = c.Lc + g0;
     const float* __restrict__ Ed = st.Edir + g0;
     float* __restrict__ Lc2 = c.Lc2 + g0;
-    float ra[RC_UE], re[RC_UE], rm[RC_UE];
+    float ra[UE], re[UE], rm[UE];
 #pragma unroll
-    for (int u = 0; u < RC_UE; ++u) {
+    for (int u = 0; u < UE; ++u) {
         const int e = t + u * RC_TB;
         if (e < nel) { ra[u] = Lc[e]; re[u] = Ed[e]; }
     }
 #pragma unroll
-    for (int u = 0; u < RC_UE; ++u) {
+    for (int u = 0; u < UE; ++u) {
         const int e = t + u * RC_TB;   // s_m element: light e / RC_UT, entry e % RC_UT
         if (e < nl * RC_UT) { const int j = e % RC_UT; rm[u] = j < T ? c.m[(int64_t)(e / RC_UT) * Np + base + j] : 0.f; }
     }
@@ -268,7 +271,7 @@ __global__ void __launch_bounds__(RC_TB) k_update(rc_dscene s, rc_dstate st, rc_
         if (t == 0) c.prpart[blockIdx.x] = pr;   // summed by k_prsum (no same-address atomics)
     }
 #pragma unroll
-    for (int u = 0; u < RC_UE; ++u) {
+    for (int u = 0; u < UE; ++u) {
         const int e = t + u * RC_TB;
         if (e < nl * RC_UT) s_m[e] = rm[u];
     }
@@ -277,7 +280,7 @@ __global__ void __launch_bounds__(RC_TB) k_update(rc_dscene s, rc_dstate st, rc_
     const int dq = RC_TB / nl, dr = RC_TB - dq * nl;
     int j = t / nl, l = t - (t / nl) * nl;
 #pragma unroll
-    for (int u = 0; u < RC_UE; ++u) {
+    for (int u = 0; u < UE; ++u) {
         const int e = t + u * RC_TB;
         if (e < nel) {
             const float a = ra[u];
@@ -982,7 +985,10 @@ hipError_t rc_launch_update(const rc_dscene* s, const rc_dstate* st, const rc_dc
     if (p->Np <= 0) return hipSuccess;
     dim3 g((p->Np + RC_UT - 1) / RC_UT), b(RC_TB);
     size_t shm = (size_t)2 * p->nl * RC_UT * sizeof(float);
-    hipLaunchKernelGGL(k_update, g, b, shm, stream, *s, *st, *c, *p, lnew);
+    const int ue = (RC_UT * p->nl + RC_TB - 1) / RC_TB;
+    if (ue <= 4) hipLaunchKernelGGL(k_update<4>, g, b, shm, stream, *s, *st, *c, *p, lnew);
+    else if (ue <= 9) hipLaunchKernelGGL(k_update<9>, g, b, shm, stream, *s, *st, *c, *p, lnew);
+    else hipLaunchKernelGGL(k_update<RC_UE>, g, b, shm, stream, *s, *st, *c, *p, lnew);
     hipLaunchKernelGGL(k_prsum, dim3(1), dim3(RC_PRB), 0, stream, (const long long*)c->prpart, (int)g.x, c->sumpr + 1, c->sumpr);
     return hipGetLastError();
 }
