@@ -152,7 +152,9 @@ rc_status rc_cache_begin(rc_cache* c, int state0, uint32_t seed);
 rc_status rc_cache_frame(rc_cache* c, int state, int want_err, int timed, rc_frame_stats* out);
 /* camera query pass over the current cache into d_img[W*H] (no_cache: plain path tracing).
  * rays_out[2] = extension rays, shadow rays (may be NULL); ms_out = GPU time of the pass
- * (may be NULL; when set the call waits for the pass). */
+ * (may be NULL; when set the call waits for the pass). The pass runs on the cache's own blocking
+ * stream, after the work queued so far; the next rc_cache_frame's entry pass overlaps it.
+ * rc_cache_sync, or a synchronisation of the device or of the legacy default stream, waits for it. */
 rc_status rc_cache_render(rc_cache* c, const rc_camera* cam, int no_cache, uint32_t seed, float* d_img, long long* rays_out,
                           float* ms_out);
 /* Schedule of the camera pass (same image bit for bit): RC_RENDER_FUSED (default) runs each path

@@ -421,6 +421,13 @@ struct rc_cache {
     std::vector<int> h_blk;
     rc_params p{};
     int render_mode = RC_RENDER_FUSED;
+    // The camera pass runs on its own stream (rs) after the frame's update (ev_upd): the next
+    // frame's entry pass, which touches nothing the camera pass reads, can then overlap it on
+    // the scene stream. Whatever writes what the camera pass reads (R, the staged workspace,
+    // event handling, a new run) first makes the scene stream wait for the last pass (ev_q).
+    hipStream_t rs = nullptr;
+    hipEvent_t ev_upd = nullptr, ev_q = nullptr;
+    bool q_pending = false;
     std::vector<uint64_t*> gmask;   // per state: gradient-support masks (grad_masks methods)
     rc_qwork qw{};                // staged camera-pass workspace, sized for qw_paths paths
     long long qw_paths = 0;
@@ -430,11 +437,22 @@ struct rc_cache {
         qw_owned.clear(); qw = rc_qwork{}; qw_paths = 0;
     }
     ~rc_cache() {
+        if (rs) { (void)hipStreamSynchronize(rs); (void)hipStreamDestroy(rs); }
+        if (S) (void)hipStreamSynchronize(S->stream);
         for (auto& e : ev) if (e) (void)hipEventDestroy(e);
+        if (ev_upd) (void)hipEventDestroy(ev_upd);
+        if (ev_q) (void)hipEventDestroy(ev_q);
         for (void* q : owned) (void)hipFree(q);
         free_qwork();
     }
 };
+
+// The scene stream waits for the last camera pass (before anything writes what it reads).
+static hipError_t wait_query(rc_cache* c) {
+    if (!c->q_pending) return hipSuccess;
+    c->q_pending = false;
+    return hipStreamWaitEvent(c->S->stream, c->ev_q, 0);
+}
 
 extern "C" rc_status rc_cache_create(rc_scene* s, const rc_method* m, rc_cache** out) {
     if (!s || !m || !out) return fail(RC_ERR_ARG, "NULL argument");
@@ -462,6 +480,8 @@ extern "C" rc_status rc_cache_create(rc_scene* s, const rc_method* m, rc_cache**
     CK2(dalloc(&d.sumpr, 2, c->owned)); CK2(dalloc(&d.prpart, (size_t)(Np + 63) / 64, c->owned)); CK2(dalloc(&d.npend, 2, c->owned)); CK2(dalloc(&d.err, 3, c->owned));
     CK2(dalloc(&c->d_counters, 2, c->owned));
     for (auto& e : c->ev) CK2(hipEventCreate(&e));
+    CK2(hipStreamCreate(&c->rs));
+    CK2(hipEventCreateWithFlags(&c->ev_upd, hipEventDisableTiming)); CK2(hipEventCreateWithFlags(&c->ev_q, hipEventDisableTiming));
 #undef CK2
     *out = c;
     return RC_OK;
@@ -498,6 +518,7 @@ extern "C" rc_status rc_cache_begin(rc_cache* c, int state0, uint32_t seed) {
     hipStream_t st = S->stream;
     const size_t Np = H.Np, nl = H.nl, nw = H.nw;
     rc_dcache& d = c->c;
+    HIPCK(wait_query(c));
     if (S->has_ref) HIPCK(hipMemcpyAsync(d.Lc, S->dst[state0].Pw, nl * Np * 4, hipMemcpyDeviceToDevice, st));
     else HIPCK(hipMemsetAsync(d.Lc, 0, nl * Np * 4, st));
     c->nmax = c->M.nmax;
@@ -685,6 +706,7 @@ extern "C" rc_status rc_cache_frame(rc_cache* c, int si, int want_err, int timed
     fs.err = -1.0;
     p.frame = (uint32_t)c->f;
     if (c->f > 0 && c->prev_state != si) {
+        HIPCK(wait_query(c));   // event handling rewrites what the last camera pass reads
         if (timed) HIPCK(hipEventRecord(c->ev[4], st));
         rc_status r = run_event(c, c->prev_state, si, fs.invalidated);
         if (r != RC_OK) return r;
@@ -711,6 +733,7 @@ extern "C" rc_status rc_cache_frame(rc_cache* c, int si, int want_err, int timed
         HIPCK(hipMemcpyAsync(d.npend, d.npend + 1, 4, hipMemcpyDeviceToDevice, st));
         c->pend_left--;
     }
+    HIPCK(wait_query(c));   // the update rewrites R, which the last camera pass reads
     if (timed) HIPCK(hipEventRecord(c->ev[3], st));
     HIPCK(rc_launch_update(&S->ds, &S->dst[si], &d, &p, track ? c->lnew : nullptr, st));   // also sets sumpr[0] = sumpr[1]
     std::swap(d.Lc, d.Lc2);
@@ -740,6 +763,9 @@ extern "C" rc_status rc_cache_frame(rc_cache* c, int si, int want_err, int timed
 // Staged camera-pass workspace for npath paths (grown on demand, kept between calls).
 static hipError_t qwork_reserve(rc_cache* c, long long npath) {
     if (npath <= c->qw_paths) return hipSuccess;
+    hipError_t es = hipStreamSynchronize(c->rs);   // a staged pass may still use the old workspace
+    if (es == hipSuccess) es = hipStreamSynchronize(c->S->stream);
+    if (es != hipSuccess) return es;
     c->free_qwork();
     rc_qwork& w = c->qw;
     const size_t n = (size_t)npath;
@@ -774,30 +800,37 @@ extern "C" rc_status rc_cache_render(rc_cache* c, const rc_camera* cam, int no_c
     rc_scene* S = c->S;
     hipStream_t st = S->stream;
     int si = c->prev_state;
-    unsigned long long* ctr = nullptr;
-    if (rays_out) { HIPCK(hipMemsetAsync(c->d_counters, 0, 16, st)); ctr = c->d_counters; }
     if (!no_cache && !c->r_valid) {   // no frame yet: resolve the warm-start channels
+        HIPCK(wait_query(c));
         HIPCK(rc_launch_resolve(&S->ds, &S->dst[si], c->c.Lc, c->c.R, c->dormant, st));
         c->r_valid = true;
     }
     const bool staged = c->render_mode == RC_RENDER_STAGED && S->trav == 2;
     if (staged) HIPCK(qwork_reserve(c, (long long)cam->W * cam->H * cam->spp));
-    if (ms_out) HIPCK(hipEventRecord(c->ev[0], st));
+    // the pass runs on the cache's render stream, after everything queued on the scene stream
+    hipStream_t qs = c->rs;
+    HIPCK(hipEventRecord(c->ev_upd, st));
+    HIPCK(hipStreamWaitEvent(qs, c->ev_upd, 0));
+    unsigned long long* ctr = nullptr;
+    if (rays_out) { HIPCK(hipMemsetAsync(c->d_counters, 0, 16, qs)); ctr = c->d_counters; }
+    if (ms_out) HIPCK(hipEventRecord(c->ev[0], qs));
     if (staged)
         HIPCK(rc_launch_query_staged(&S->ds, &S->dst[si], no_cache ? nullptr : c->c.R, cam, seed, (uint32_t)c->f, c->dormant, d_img, ctr,
-                                     &c->qw, nullptr, st));
+                                     &c->qw, nullptr, qs));
     else
         HIPCK(rc_launch_query(S->trav, &S->ds, &S->dst[si], no_cache ? nullptr : c->c.R, cam, seed, (uint32_t)c->f, c->dormant, d_img, ctr,
-                              st));
+                              qs));
+    HIPCK(hipEventRecord(c->ev_q, qs));
+    c->q_pending = true;
     if (ms_out) {
-        HIPCK(hipEventRecord(c->ev[1], st));
+        HIPCK(hipEventRecord(c->ev[1], qs));
         HIPCK(hipEventSynchronize(c->ev[1]));
         *ms_out = ms_between(c->ev[0], c->ev[1]);
     }
     if (rays_out) {
         unsigned long long h2[2];
-        HIPCK(hipMemcpyAsync(h2, c->d_counters, 16, hipMemcpyDeviceToHost, st));
-        HIPCK(hipStreamSynchronize(st));
+        HIPCK(hipMemcpyAsync(h2, c->d_counters, 16, hipMemcpyDeviceToHost, qs));
+        HIPCK(hipStreamSynchronize(qs));
         rays_out[0] = (long long)h2[0]; rays_out[1] = (long long)h2[1];
     }
     return RC_OK;
@@ -839,6 +872,7 @@ extern "C" rc_status rc_cache_get(const rc_cache* c, int which, void* host_dst, 
     void* p; size_t b = 0;
     if (!c || !cache_array(c, which, &p, &b)) return fail(RC_ERR_ARG, "unknown cache array %d", which);
     if (bytes != b) return fail(RC_ERR_ARG, "cache array %d: %zu bytes requested, %zu available", which, bytes, b);
+    HIPCK(hipStreamSynchronize(c->rs));
     HIPCK(hipStreamSynchronize(c->S->stream));
     HIPCK(hipMemcpy(host_dst, p, b, hipMemcpyDeviceToHost));
     return RC_OK;
@@ -847,12 +881,14 @@ extern "C" rc_status rc_cache_set(rc_cache* c, int which, const void* host_src, 
     void* p; size_t b = 0;
     if (!c || !cache_array(c, which, &p, &b)) return fail(RC_ERR_ARG, "unknown cache array %d", which);
     if (bytes != b) return fail(RC_ERR_ARG, "cache array %d: %zu bytes given, %zu expected", which, bytes, b);
+    HIPCK(hipStreamSynchronize(c->rs));
     HIPCK(hipStreamSynchronize(c->S->stream));
     HIPCK(hipMemcpy(p, host_src, b, hipMemcpyHostToDevice));
     return RC_OK;
 }
 extern "C" rc_status rc_cache_sync(rc_cache* c) {
     if (!c) return fail(RC_ERR_ARG, "NULL cache");
+    HIPCK(hipStreamSynchronize(c->rs));
     HIPCK(hipStreamSynchronize(c->S->stream));
     return RC_OK;
 }
@@ -1219,6 +1255,7 @@ extern "C" rc_status rc_mlp_train_scene(rc_mlp* m, rc_cache* c, int n, uint32_t 
     rc_scene* S = c->S;
     hipStream_t st = S->stream;
     const int si = c->prev_state;
+    HIPCK(wait_query(c));
     if (!c->r_valid) {   // the judge reads the cache's resolved radiance
         HIPCK(rc_launch_resolve(&S->ds, &S->dst[si], c->c.Lc, c->c.R, c->dormant, st));
         c->r_valid = true;
@@ -1250,6 +1287,7 @@ extern "C" rc_status rc_mlp_render(rc_mlp* m, rc_cache* c, const rc_camera* cam,
     if (S->trav != 2) return fail(RC_ERR_STATE, "the learned-cache camera pass runs on the 4-wide BVH (trav 2)");
     hipStream_t st = S->stream;
     const int si = c->prev_state;
+    HIPCK(wait_query(c));   // the grid cache's camera pass may still use the staged workspace
     HIPCK(qwork_reserve(c, (long long)cam->W * cam->H * cam->spp));
     if (ms_out) HIPCK(hipEventRecord(c->ev[0], st));
     HIPCK(rc_launch_query_staged(&S->ds, &S->dst[si], nullptr, cam, seed, (uint32_t)c->f, c->dormant, d_img, nullptr, &c->qw, &m->d, st));
@@ -1269,6 +1307,7 @@ extern "C" rc_status rc_cache_snapshot_save(rc_cache* c, rc_snap_rec* host_recor
     if (!c->begun) return fail(RC_ERR_STATE, "rc_cache_begin not called");
     rc_scene* S = c->S;
     const size_t Np = S->H.Np;
+    HIPCK(wait_query(c));
     rc_snaprec* tmp = nullptr;
     HIPCK(hipMalloc((void**)&tmp, Np * sizeof(rc_snaprec)));
     int cnt = 0;
@@ -1288,6 +1327,7 @@ extern "C" rc_status rc_cache_snapshot_load(rc_cache* c, const rc_snap_rec* host
     const int Np = S->H.Np, nl = S->H.nl;
     if (count > (size_t)Np) return fail(RC_ERR_ARG, "more records (%zu) than entries (%d)", count, Np);
     hipStream_t st = S->stream;
+    HIPCK(wait_query(c));
     rc_snaprec* tmp = nullptr;
     HIPCK(hipMalloc((void**)&tmp, std::max<size_t>(count, 1) * sizeof(rc_snaprec)));
     hipError_t e = count ? hipMemcpyAsync(tmp, host_records, count * sizeof(rc_snaprec), hipMemcpyHostToDevice, st) : hipSuccess;
