@@ -308,7 +308,7 @@ def bench_mlp(args, rea, scene, cfg, log):
     mlp = M.MLPCache(M.config(list(info.lo), list(info.ext), log2_table=mc["log2_table"], base_res=mc["base_res"],
                               max_res=mc["max_res"], max_batch=mc["batch"], lr=mc["lr"]))
     cache = rea.Cache(scene, rea.methods.plain())
-    cache.set_render_mode(rea.RENDER_STAGED)
+    cache.set_render_mode(rea.RENDER_STAGED if args.render == "staged" else rea.RENDER_FUSED)
     qcam = scene.camera()
     qcam.W, qcam.H, qcam.spp, qcam.cache_bounce = mc["W"], mc["H"], 1, 1
     qimg = torch.zeros(qcam.W * qcam.H, dtype=torch.float32, device="cuda")
@@ -364,7 +364,7 @@ def bench_mlp(args, rea, scene, cfg, log):
             for name, (a, b) in {"lamp 0 off": (0, 2), "night (sun off)": (0, 1)}.items():
                 region = sc.light_off_region(refs[a], refs[b])
                 c = rea.Cache(scene, rea.methods.targeted())
-                c.set_render_mode(rea.RENDER_STAGED)
+                c.set_render_mode(rea.RENDER_STAGED if args.render == "staged" else rea.RENDER_FUSED)
                 c.begin(a, 777)
                 ml = M.MLPCache(M.config(list(info.lo), list(info.ext), log2_table=mc["log2_table"], base_res=mc["base_res"],
                                          max_res=mc["max_res"], max_batch=mc["batch"], lr=mc["lr"]))

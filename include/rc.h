@@ -255,8 +255,9 @@ enum { RC_MLP_W = 0, RC_MLP_TABLE, RC_MLP_GW, RC_MLP_GTABLE, RC_MLP_COUNT };
  * direct light; judge_out = relative RMS of the MLP's predictions at those entries (before the step)
  * against the cache's resolved radiance (the plain grid cache as error judge). Waits for the step. */
 rc_status rc_mlp_train_scene(rc_mlp* m, rc_cache* c, int n, uint32_t seed, float* loss_out, float* judge_out);
-/* camera pass over the cache's scene and state whose paths end on the MLP at bounce cam->cache_bounce
- * (staged schedule; one batched inference over the query points); ms_out as rc_cache_render */
+/* camera pass over the cache's scene and state whose paths end on the MLP at bounce cam->cache_bounce:
+ * with the cache's render mode RC_RENDER_FUSED one thread per path records its query point, then one
+ * batched inference; RC_RENDER_STAGED runs the staged schedule (the same image); ms_out as rc_cache_render */
 rc_status rc_mlp_render(rc_mlp* m, rc_cache* c, const rc_camera* cam, uint32_t seed, float* d_img, float* ms_out);
 size_t rc_mlp_array_bytes(const rc_mlp* m, int which);
 rc_status rc_mlp_get(const rc_mlp* m, int which, void* host_dst, size_t bytes);

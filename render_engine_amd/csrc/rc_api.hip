@@ -1290,7 +1290,10 @@ extern "C" rc_status rc_mlp_render(rc_mlp* m, rc_cache* c, const rc_camera* cam,
     HIPCK(wait_query(c));   // the grid cache's camera pass may still use the staged workspace
     HIPCK(qwork_reserve(c, (long long)cam->W * cam->H * cam->spp));
     if (ms_out) HIPCK(hipEventRecord(c->ev[0], st));
-    HIPCK(rc_launch_query_staged(&S->ds, &S->dst[si], nullptr, cam, seed, (uint32_t)c->f, c->dormant, d_img, nullptr, &c->qw, &m->d, st));
+    if (c->render_mode == RC_RENDER_STAGED)
+        HIPCK(rc_launch_query_staged(&S->ds, &S->dst[si], nullptr, cam, seed, (uint32_t)c->f, c->dormant, d_img, nullptr, &c->qw, &m->d, st));
+    else
+        HIPCK(rc_launch_query_mlp(&S->ds, &S->dst[si], cam, seed, (uint32_t)c->f, c->dormant, d_img, &c->qw, &m->d, st));
     if (ms_out) {
         HIPCK(hipEventRecord(c->ev[1], st));
         HIPCK(hipEventSynchronize(c->ev[1]));

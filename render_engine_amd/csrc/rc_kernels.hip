@@ -665,6 +665,58 @@ __global__ void __launch_bounds__(RC_TB) k_query(rc_dscene s, rc_dstate st, cons
     }
 }
 
+// ------------------------------------------------------------------------------ learned-cache query
+// The fused camera pass for the learned cache: one thread per path (any spp) runs rc_path's
+// arithmetic and RNG draws, and at the cache bounce records the query (hit point, normal,
+// direction back along the ray) and the path's throughput in the staged pass's workspace
+// instead of reading R; the radiance so far goes to w.rad. One batched inference, the
+// prediction added to the querying paths and the per-pixel sums (the staged pass's last
+// kernels) finish the image, bit for bit the staged learned-cache pass's.
+template <int TRAV>
+__global__ void __launch_bounds__(RC_TB) k_query_mlp(rc_dscene s, rc_dstate st, rc_camera cam, uint32_t seed, uint32_t frame,
+                                                    uint64_t dormant, rc_qwork w, int npath) {
+    extern __shared__ float sh[];
+    int* shq = (int*)(sh + 9 * st.nt);
+    uint32_t* lstk = rc_lstack(sh, st.nt);
+    rc_load_dyn(st, sh, shq);
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= npath) return;
+    const int pix = p / cam.spp, px = pix % cam.W, py = pix / cam.W;
+    uint32_t rs = rc_seed(seed, frame, (uint32_t)p);
+    float ju = rc_rnd(rs), jv = rc_rnd(rs);
+    float sx = (2.f * ((float)px + ju) / (float)cam.W - 1.f) * cam.tanx;
+    float sy = (1.f - 2.f * ((float)py + jv) / (float)cam.H) * cam.tany;
+    f3 d = mk3(cam.fwd[0], cam.fwd[1], cam.fwd[2]) + mk3(cam.right[0], cam.right[1], cam.right[2]) * sx +
+           mk3(cam.up[0], cam.up[1], cam.up[2]) * sy;
+    d = d * (1.f / sqrtf(dot3(d, d)));
+    f3 o = mk3(cam.pos[0], cam.pos[1], cam.pos[2]);
+    float thr = 1.f, rad = 0.f;
+    float4 q0 = make_float4(0.f, 0.f, 0.f, 0.f), q1 = q0, q2 = q0;
+    int sh_rays = 0;
+    for (int b = 0; ; ++b) {
+        float t; int q;
+        const bool hit = rc_trace_full<TRAV>(o, d, sh, shq, st.nt, s, t, q, lstk);
+        if (!hit) break;
+        f3 H = o + d * t, n;
+        int e;
+        if (rc_resolve_hit_n(s, st, H, d, q, e, n)) break;
+        if (b == cam.cache_bounce) {
+            q0 = make_float4(H.x, H.y, H.z, 1.f); q1 = make_float4(n.x, n.y, n.z, 0.f); q2 = make_float4(-d.x, -d.y, -d.z, 0.f);
+            break;
+        }
+        const float A = s.Alb[e];
+        rad += thr * (A * rc_nee<TRAV>(s, st, sh, shq, H, n, rs, dormant, sh_rays, lstk));
+        if (b >= cam.bounces) break;
+        thr = thr * A;
+        const float u1 = rc_rnd(rs), u2 = rc_rnd(rs);
+        d = rc_cosine_dir(n, u1, u2);
+        o = H + n * 1e-3f;
+    }
+    w.rad[p] = rad; w.thr[p] = thr;
+    ((float4*)w.mq0)[p] = q0;
+    if (q0.w != 0.f) { ((float4*)w.mq1)[p] = q1; ((float4*)w.mq2)[p] = q2; }
+}
+
 // ------------------------------------------------------------------------------ wavefront query
 // The camera pass as stages over compacted ray lists: generate primary rays, trace a batch,
 // shade hits (cache read, next-event candidate, bounce), trace the shadow batch (any hit),
@@ -1094,6 +1146,18 @@ hipError_t rc_launch_vol_lights(int nl, const uint32_t* Va, const uint32_t* Vb, 
     return hipGetLastError();
 }
 
+hipError_t rc_launch_query_mlp(const rc_dscene* s, const rc_dstate* st, const rc_camera* cam, uint32_t seed, uint32_t frame, uint64_t dormant,
+                               float* img, const rc_qwork* w, const rc_mlpdev* mlp, hipStream_t stream) {
+    const int npath = cam->W * cam->H * cam->spp;
+    if (npath <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_query_mlp<2>, dim3(rc_grid(npath)), dim3(RC_TB), rc_shm(*st, *s, 2), stream, *s, *st, *cam, seed, frame, dormant, *w,
+                       npath);
+    hipError_t e = rc_launch_mlp_infer(mlp, (const float4*)w->mq0, (const float4*)w->mq1, (const float4*)w->mq2, npath, w->mpred, stream);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_q_mlp_add, dim3(rc_grid(npath)), dim3(RC_TB), 0, stream, *w, npath);
+    hipLaunchKernelGGL(k_q_pixels, dim3(rc_grid((long long)cam->W * cam->H)), dim3(RC_TB), 0, stream, *cam, *w, img);
+    return hipGetLastError();
+}
 hipError_t rc_launch_query_staged(const rc_dscene* s, const rc_dstate* st, const float* R, const rc_camera* cam, uint32_t seed,
                                   uint32_t frame, uint64_t dormant, float* img, unsigned long long* counters, const rc_qwork* w,
                                   const rc_mlpdev* mlp, hipStream_t stream) {

@@ -96,7 +96,8 @@ class MLPCache:
         return loss.value, judge.value
 
     def render(self, cache, img, cam=None, seed=7, timed=False):
-        """camera pass of the cache's scene ending on the MLP at cam.cache_bounce; ms when timed"""
+        """camera pass of the cache's scene ending on the MLP at cam.cache_bounce (the cache's render
+        mode: fused paths recording their queries, or the staged pass; the same image); ms when timed"""
         cam = cam or cache.scene.camera()
         ms = C.c_float(0.0)
         check(lib.rc_mlp_render(self._h, cache._h, C.byref(cam), seed, C.c_void_p(img.data_ptr()), C.byref(ms) if timed else None))

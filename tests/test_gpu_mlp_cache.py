@@ -31,7 +31,6 @@ def test_mlp_render_constant_matches_grid(torch_gpu, street):
     W[M.HIDDEN * 4096 + 0 * 16 + 0] = c0
     mlp.set(M.MLP_W, W)
     cache = rea.Cache(street, rea.methods.plain())
-    cache.set_render_mode(rea.RENDER_STAGED)
     cache.begin(0, 3)
     cache.frame(0)
     cache.set("R", np.full(street.info.Np, c0, np.float32))
@@ -41,12 +40,14 @@ def test_mlp_render_constant_matches_grid(torch_gpu, street):
             cache.frame(st)
             cache.set("R", np.full(street.info.Np, c0, np.float32))
         cam.spp = spp
-        a = torch.zeros(cam.W * cam.H, dtype=torch.float32, device="cuda")
-        b = torch.zeros_like(a)
-        cache.render(a, cam, seed=21)
-        mlp.render(cache, b, cam, seed=21)
-        ga, gb = a.cpu().numpy(), b.cpu().numpy()
-        assert np.array_equal(ga.view(np.uint32), gb.view(np.uint32)), f"state {st}: {np.sum(ga != gb)} pixels differ"
+        for mode in (rea.RENDER_STAGED, rea.RENDER_FUSED):   # learned-cache pass: staged, then fused
+            cache.set_render_mode(mode)
+            a = torch.zeros(cam.W * cam.H, dtype=torch.float32, device="cuda")
+            b = torch.zeros_like(a)
+            cache.render(a, cam, seed=21)
+            mlp.render(cache, b, cam, seed=21)
+            ga, gb = a.cpu().numpy(), b.cpu().numpy()
+            assert np.array_equal(ga.view(np.uint32), gb.view(np.uint32)), f"state {st} mode {mode}: {np.sum(ga != gb)} pixels differ"
 
 
 def test_mlp_online_training_against_the_grid(torch_gpu, street):
