@@ -178,11 +178,13 @@ def quality(rea, scene, cam, log, pre=20, post=60, ref_spp_rounds=16, occluder=F
                          ("targeted", rea.methods.targeted()), ("targeted_grad", rea.methods.targeted_grad())):
             c = rea.Cache(scene, m)
             c.begin(a, 777)
-            biases, errs, inval = [], [], 0
+            biases, errs, inval, ev_ms = [], [], 0, None
             for f in range(pre + post):
                 st = a if f < pre else b
-                fs = c.frame(st, want_err=True)
+                fs = c.frame(st, want_err=True, timed=(f == pre))
                 inval += fs.invalidated
+                if f == pre:
+                    ev_ms = fs.ms_event   # GPU time of the event block (invalidation / flush / restore)
                 c.render(img, cam, seed=5000 + f)
                 if f >= pre:
                     est = img.cpu().numpy().reshape(cam.H, cam.W)
@@ -194,10 +196,78 @@ def quality(rea, scene, cam, log, pre=20, post=60, ref_spp_rounds=16, occluder=F
                 "bias_mean_30": round(float(np.mean(np.abs(biases[:30]))), 4),
                 "entry_relrms_first_5": [round(x, 4) for x in errs[:5]],
                 "entries_invalidated": inval,
+                "event_ms": round(float(ev_ms), 4),
             }
         out[name] = ev
         log(f"{name}: {json.dumps(ev)}")
     return out
+
+
+def adjoint_cpu(scene, adj, a, seconds, rays_step, log):
+    """CPU leg of the adjoint workload. (1) Algorithmic bytes of one k_adj_fwd launch, counted by
+    the oracle on the same rays (every bounce, 1/8 of the entries in 8 bands, scaled): per entry
+    36 B, per ray 32 B per binary-BVH node visited + 36 B per triangle tested + 8 B per hash
+    probe, per hit 16 B (28 B when the previous bounce's L and S are read); the mean over the
+    bounces. (2) The oracle's step on the host cores: every bounce's rays traced once (the hits
+    kept for the reverse pass), the FP32 forward and the FP64 reverse pass, with the unit direct
+    light copied from the GPU (computed once per handle, outside the step on both sides)."""
+    import numpy as np
+
+    from render_engine_amd import adjoint as ad
+
+    osc = oracle_scene(scene)   # puts the oracle on the path
+    import pyoracle as po
+    info = scene.info
+    Np, nl, K, B, seed = info.Np, info.nl, a["K"], a["B"], a["seed"]
+    st = osc.state(0)
+    band = max(1, Np // 64)
+    per_launch, work = [], []
+    for b in range(B):
+        po.lib.orc_count_enable(1)
+        sampled, nhit = 0, 0
+        for j in range(8):
+            i0 = j * Np // 8
+            i1 = min(Np, i0 + band)
+            h = np.empty((i1 - i0) * K, np.int32)
+            po.lib.orc_adj_hits_rows(st, 1, K, b, seed, i0, i1, po.ptr(h))
+            nhit += int((h >= 0).sum())
+            sampled += i1 - i0
+        cnt = np.zeros(4, np.int64)
+        po.lib.orc_count_get(po.ptr(cnt))
+        po.lib.orc_count_enable(0)
+        sc = Np / sampled
+        per_launch.append(36.0 * Np + (32 * cnt[0] + 36 * cnt[1] + 8 * cnt[2]) * sc + nhit * sc * (16 if b == 0 else 28))
+        work.append({"nodes": round(cnt[0] * sc), "tris": round(cnt[1] * sc), "probes": round(cnt[2] * sc), "hits": round(nhit * sc)})
+    fb = float(np.mean(per_launch))
+    log(f"adjoint forward work per bounce (oracle count, 1/{Np / sampled:.0f} of entries): {json.dumps(work)}")
+    # the oracle's step
+    mat = np.ascontiguousarray(scene.array("MAT"), np.int32)
+    G = adj.get(ad.ADJ_G)
+    SG = adj.get(ad.ADJ_SG)
+    G64 = G.astype(np.float64)
+    I32 = np.ascontiguousarray(scene.array("INTEN", 0), np.float32)
+    alb32 = np.ascontiguousarray(scene.array("MATALB"), np.float32)
+    I64, alb64 = I32.astype(np.float64), alb32.astype(np.float64)
+    g64 = np.ones(Np)
+    hits = np.zeros(B * Np * K, np.int32)
+    R, SR = np.zeros(Np, np.float32), np.zeros(Np, np.uint64)
+    R64, dI, dalb = np.zeros(Np), np.zeros(nl), np.zeros(info.nmat)
+    steps, t_tot = 0, 0.0
+    while steps < 1 or (t_tot < seconds and steps < 20):
+        t0 = time.perf_counter()
+        po.lib.orc_adj_hits(st, 1, K, B, seed, po.ptr(hits))
+        po.lib.orc_adj_forward(Np, nl, K, B, po.ptr(hits), po.ptr(mat), po.ptr(alb32), po.ptr(I32), po.ptr(G), po.ptr(SG),
+                               po.ptr(R), po.ptr(SR), None)
+        po.lib.orc_adj_grad64(Np, nl, info.nmat, K, B, po.ptr(hits), po.ptr(mat), po.ptr(alb64), po.ptr(I64), po.ptr(G64),
+                              po.ptr(g64), po.ptr(R64), po.ptr(dI), po.ptr(dalb))
+        t_tot += time.perf_counter() - t0
+        steps += 1
+    log(f"cpu baseline (adjoint): {steps} steps in {t_tot:.1f} s on {po.lib.orc_num_threads()} threads")
+    cpu = dict(value=rays_step * steps / t_tot / 1e6, unit="Mrays/s", cores=int(po.lib.orc_num_threads()), kind="port",
+               sample=f"{steps} full forward + reverse steps of the same workload (the oracle traces each of the "
+                      f"{Np * K * B} rays once and keeps the hits; value = the step's ray count over the step time)",
+               steps_per_s=steps / t_tot)
+    return fb, work, cpu
 
 
 def bench_adjoint(args, rea, scene, cfg, log):
@@ -260,6 +330,30 @@ def bench_adjoint(args, rea, scene, cfg, log):
                "entries_depending_on_lamp0": int(((sr & np.uint64(1)) != 0).sum()),
                "entries_depending_on_sun": int(((sr >> np.uint64(min(info.nl - 1, 63))) & np.uint64(1)).sum())}
     log(f"adjoint: fwd {np.mean(fwd):.3f} ms, bwd {np.mean(bwd):.3f} ms, FD {json.dumps(fd)}")
+    # per-launch time of the traced bounces (k_adj_fwd, k_adj_bwd): HIP events on the stream they run on
+    adj.set_timed(True)
+    tf, tb = [], []
+    for _ in range(max(3, args.steps // 4)):
+        adj.forward(I, alb)
+        adj.backward(g)
+        f_ms, b_ms = adj.trace_ms()
+        tf.append(f_ms / a["B"]); tb.append(b_ms / a["B"])
+    adj.set_timed(False)
+    k_fwd, k_bwd = float(np.mean(tf)), float(np.mean(tb))
+    cpu, fb = None, None
+    if not args.no_cpu:
+        try:
+            fb, fwork, cpu = adjoint_cpu(scene, adj, a, args.cpu_seconds, rays_step, log)
+        except Exception as e:  # the baseline is reported, not required
+            log(f"cpu leg failed: {e!r}")
+    roof = None
+    if fb:
+        ach = fb / k_fwd / 1e6
+        roof = {"bound": "hbm", "achieved": ach, "peak": 8000.0, "unit": "GB/s", "frac": ach / 8000.0, "traffic": None,
+                "kernel": "k_adj_fwd",
+                "basis": "per entry 36 B (P, N, L and S out) + per ray 32 B/binary node + 36 B/triangle + 8 B/hash probe + "
+                         "per hit 16 B (A, E, SG; 28 B with the previous bounce's L, S), oracle-counted, mean over the bounces",
+                "work": fwork}
     qual = None
     if not args.no_quality:
         try:
@@ -279,11 +373,14 @@ def bench_adjoint(args, rea, scene, cfg, log):
                                f"scalars, {info.nmat} material albedos, {a['B']} bounces x {a['K']} rays per entry",
                    "config_index": args.config},
         "kernels": {"forward_ms": float(np.mean(fwd)), "backward_ms": float(np.mean(bwd)), "unit_direct_light_s": round(t_unit, 3),
-                    "rays_per_step": rays_step},
+                    "rays_per_step": rays_step,
+                    "k_adj_fwd": {"ms": k_fwd, "rays": info.Np * a["K"], "mrays_s": info.Np * a["K"] / k_fwd / 1e3,
+                                  "alg_bytes": fb, "GB_s": (fb / k_fwd / 1e6) if fb else None},
+                    "k_adj_bwd": {"ms": k_bwd, "rays": info.Np * a["K"], "mrays_s": info.Np * a["K"] / k_bwd / 1e3}},
         "gradient_check": fd,
         "gradient_support": support,
-        "roofline": None,
-        "cpu_baseline": None,
+        "roofline": roof,
+        "cpu_baseline": cpu,
         "light_off": qual,
     }
     print(json.dumps(out))

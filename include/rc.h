@@ -218,6 +218,12 @@ rc_status rc_adjoint_backward(rc_adjoint* a, const float* d_gR, float* d_gI, flo
 enum { RC_ADJ_G = 0, RC_ADJ_SG, RC_ADJ_SR, RC_ADJ_LB, RC_ADJ_DA, RC_ADJ_DE, RC_ADJ_COUNT };
 size_t rc_adjoint_array_bytes(const rc_adjoint* a, int which);
 rc_status rc_adjoint_get(const rc_adjoint* a, int which, void* host_dst, size_t bytes);
+/* kernel timing: timed != 0 brackets every traced bounce of the following forward / backward calls
+ * with HIP events on their stream (each call then waits for its last bounce); ms_out[0] = GPU time
+ * of the last forward's traced bounces (summed over the B launches), ms_out[1] the same for the last
+ * backward, 0 before a timed call. No counterpart in the reference (caller: the benchmark). */
+rc_status rc_adjoint_set_timed(rc_adjoint* a, int timed);
+rc_status rc_adjoint_trace_ms(const rc_adjoint* a, float ms_out[2]);
 
 /* ---- hash-encoded MLP radiance cache (extension) ----
  * Input: 16-level multiresolution hash encoding of the position (2 features per level,

@@ -98,6 +98,7 @@ def _load():
     lib.orc_adj_seed.restype = C.c_uint32; lib.orc_adj_seed.argtypes = [C.c_uint32, C.c_int]
     lib.orc_adj_unit.argtypes = [C.POINTER(OrcScene), C.c_int, _P, _P, _P]
     lib.orc_adj_hits.argtypes = [C.POINTER(OrcScene), C.c_int, C.c_int, C.c_int, C.c_uint32, _P]
+    lib.orc_adj_hits_rows.argtypes = [C.POINTER(OrcScene), C.c_int, C.c_int, C.c_int, C.c_uint32, C.c_int, C.c_int, _P]
     lib.orc_adj_forward.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, _P, _P, _P, _P, _P, _P, _P, _P, _P]
     lib.orc_adj_grad64.restype = C.c_double
     lib.orc_adj_grad64.argtypes = [C.c_int] * 5 + [_P] * 9

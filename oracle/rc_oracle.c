@@ -1269,24 +1269,26 @@ void orc_adj_unit(const orc_scene* s, int trav, float* G, uint64_t* SG, double* 
     }
 }
 
-void orc_adj_hits(const orc_scene* s, int trav, int K, int B, uint32_t seed, int32_t* hits) {
-    for (int b = 0; b < B; ++b) {
-        const uint32_t sb = orc_adj_seed(seed, b);
+void orc_adj_hits_rows(const orc_scene* s, int trav, int K, int b, uint32_t seed, int i0, int i1, int32_t* hits) {
+    const uint32_t sb = orc_adj_seed(seed, b);
 #pragma omp parallel for schedule(dynamic, 64)
-        for (int i = 0; i < s->Np; ++i) {
-            uint32_t rs = orc_hash32(sb ^ orc_hash32((uint32_t)i + 1u));
-            v3 P = ld(s->P, i), N = ld(s->N, i), O = add(P, mul(N, 1e-3f));
-            for (int k = 0; k < K; ++k) {
-                float u1 = orc_rnd(&rs), u2 = orc_rnd(&rs);
-                v3 D = cosine_dir(N, u1, u2);
-                float t; int q;
-                int hit = trace_full(s, trav, O, D, &t, &q);
-                v3 H = hit ? add(O, mul(D, t)) : O;
-                int pi;
-                hits[((size_t)b * s->Np + i) * K + k] = resolve_hit(s, hit, H, D, q, i, &pi) ? -1 : pi;
-            }
+    for (int i = i0; i < i1; ++i) {
+        uint32_t rs = orc_hash32(sb ^ orc_hash32((uint32_t)i + 1u));
+        v3 P = ld(s->P, i), N = ld(s->N, i), O = add(P, mul(N, 1e-3f));
+        for (int k = 0; k < K; ++k) {
+            float u1 = orc_rnd(&rs), u2 = orc_rnd(&rs);
+            v3 D = cosine_dir(N, u1, u2);
+            float t; int q;
+            int hit = trace_full(s, trav, O, D, &t, &q);
+            v3 H = hit ? add(O, mul(D, t)) : O;
+            int pi;
+            hits[(size_t)(i - i0) * K + k] = resolve_hit(s, hit, H, D, q, i, &pi) ? -1 : pi;
         }
     }
+}
+
+void orc_adj_hits(const orc_scene* s, int trav, int K, int B, uint32_t seed, int32_t* hits) {
+    for (int b = 0; b < B; ++b) orc_adj_hits_rows(s, trav, K, b, seed, 0, s->Np, hits + (size_t)b * s->Np * K);
 }
 
 void orc_adj_forward(int Np, int nl, int K, int B, const int32_t* hits, const int32_t* mat, const float* alb, const float* I,

@@ -162,6 +162,8 @@ uint32_t orc_adj_seed(uint32_t seed, int b);
 void orc_adj_unit(const orc_scene* s, int trav, float* G, uint64_t* SG, double* G64);
 /* hit entry of every ray, hits[(b * Np + i) * K + k], -1 when the ray does not count */
 void orc_adj_hits(const orc_scene* s, int trav, int K, int B, uint32_t seed, int32_t* hits);
+/* the rays of bounce b of entries i0..i1-1 only: hits[(i - i0) * K + k] */
+void orc_adj_hits_rows(const orc_scene* s, int trav, int K, int b, uint32_t seed, int i0, int i1, int32_t* hits);
 /* FP32 forward in the kernels' operation order: R, support SR, L^(B) (LB may be NULL) */
 void orc_adj_forward(int Np, int nl, int K, int B, const int32_t* hits, const int32_t* mat, const float* alb, const float* I,
                      const float* G, const uint64_t* SG, float* R, uint64_t* SR, float* LB);

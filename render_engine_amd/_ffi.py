@@ -99,6 +99,8 @@ _SIGS = {
     "rc_adjoint_backward": (C.c_int, [_P, _P, _P, _P, _P]),
     "rc_adjoint_array_bytes": (C.c_size_t, [_P, C.c_int]),
     "rc_adjoint_get": (C.c_int, [_P, C.c_int, _P, C.c_size_t]),
+    "rc_adjoint_set_timed": (C.c_int, [_P, C.c_int]),
+    "rc_adjoint_trace_ms": (C.c_int, [_P, C.POINTER(C.c_float)]),
     "rc_mlp_create": (C.c_int, [_P, C.POINTER(_P)]),
     "rc_mlp_destroy": (None, [_P]),
     "rc_mlp_infer": (C.c_int, [_P, _P, _P, _P, C.c_int, _P, _P]),

@@ -71,6 +71,17 @@ class Adjoint:
         check(lib.rc_adjoint_get(self._h, which, out.ctypes.data_as(C.c_void_p), nb))
         return out
 
+    def set_timed(self, on=True):
+        """bracket every traced bounce of the following forward / backward calls with HIP events
+        (each call then waits for its last bounce)"""
+        check(lib.rc_adjoint_set_timed(self._h, 1 if on else 0))
+
+    def trace_ms(self):
+        """(forward, backward): GPU time of the traced bounces of the last timed calls, summed over the B launches"""
+        ms = (C.c_float * 2)()
+        check(lib.rc_adjoint_trace_ms(self._h, ms))
+        return float(ms[0]), float(ms[1])
+
     def support(self):
         """per-entry 64-bit masks of the lights l with dR_i/dI_l != 0 at the last forward"""
         return self.get(ADJ_SR)

@@ -283,15 +283,17 @@ hipError_t rc_launch_adj_unit(int trav, const rc_dscene* s, const rc_dstate* st,
 }
 
 hipError_t rc_launch_adj_forward(int trav, const rc_dscene* s, const rc_dstate* st, const rc_adjbuf* w, const float* I,
-                                 const float* alb, int K, int B, uint32_t seed, float* R, hipStream_t stream) {
+                                 const float* alb, int K, int B, uint32_t seed, float* R, hipStream_t stream, hipEvent_t* ev) {
     const int Np = s->Np;
     const dim3 g(rc_grid(Np)), b(RC_TB);
     const size_t shm = rc_shm(*st, *s, trav);
     hipLaunchKernelGGL(k_adj_setup, dim3((Np + RC_AS - 1) / RC_AS), b, 0, stream, Np, s->nl, w->mat, alb, I, w->G, w->A, w->E);
     for (int k = 0; k < B; ++k) {
         const uint32_t sb = rc_hash32(seed + (uint32_t)k * 0x9E3779B9u);
+        if (ev) { const hipError_t e = hipEventRecord(ev[2 * k], stream); if (e != hipSuccess) return e; }
         RC_ADJ_T(k_adj_fwd, *s, *st, w->A, w->E, w->SG, k > 0 ? w->L + (size_t)(k - 1) * Np : (const float*)nullptr,
                  k > 0 ? w->S[(k - 1) & 1] : (const uint64_t*)nullptr, w->L + (size_t)k * Np, w->S[k & 1], sb, K);
+        if (ev) { const hipError_t e = hipEventRecord(ev[2 * k + 1], stream); if (e != hipSuccess) return e; }
     }
     hipLaunchKernelGGL(k_adj_out, g, b, 0, stream, Np, w->A, w->E, B > 0 ? w->L + (size_t)(B - 1) * Np : (const float*)nullptr, w->SG,
                        B > 0 ? w->S[(B - 1) & 1] : (const uint64_t*)nullptr, R, w->SR);
@@ -299,7 +301,7 @@ hipError_t rc_launch_adj_forward(int trav, const rc_dscene* s, const rc_dstate* 
 }
 
 hipError_t rc_launch_adj_backward(int trav, const rc_dscene* s, const rc_dstate* st, const rc_adjbuf* w, const float* gR, int K,
-                                  int B, uint32_t seed, float* dI, float* dalb, hipStream_t stream) {
+                                  int B, uint32_t seed, float* dI, float* dalb, hipStream_t stream, hipEvent_t* ev) {
     const int Np = s->Np, nl = s->nl;
     const dim3 g(rc_grid(Np)), b(RC_TB);
     const size_t shm = rc_shm(*st, *s, trav);
@@ -312,7 +314,9 @@ hipError_t rc_launch_adj_backward(int trav, const rc_dscene* s, const rc_dstate*
         // with a fill kernel (no memset node in a captured graph)
         float* S = w->lam[k & 1];
         hipLaunchKernelGGL(k_adj_zero, g, b, 0, stream, Np, S);
+        if (ev) { const hipError_t e = hipEventRecord(ev[2 * k], stream); if (e != hipSuccess) return e; }
         RC_ADJ_T(k_adj_bwd, *s, *st, w->lam[(k + 1) & 1], S, sb, K);
+        if (ev) { const hipError_t e = hipEventRecord(ev[2 * k + 1], stream); if (e != hipSuccess) return e; }
         hipLaunchKernelGGL(k_adj_bwd_fin, g, b, 0, stream, Np, w->A, w->E, k > 0 ? w->L + (size_t)(k - 1) * Np : (const float*)nullptr,
                            (const float*)S, k > 0 ? S : (float*)nullptr, w->dA, w->dE);
     }

@@ -947,8 +947,29 @@ struct rc_adjoint {
     rc_adjbuf w{};
     bool forward_done = false, backward_done = false;
     std::vector<void*> owned;
-    ~rc_adjoint() { for (void* p : owned) (void)hipFree(p); }
+    bool timed = false;
+    hipEvent_t tev[2 * 16] = {};   // around each traced bounce (B <= 16)
+    float trace_ms[2] = {0.f, 0.f};
+    ~rc_adjoint() {
+        for (hipEvent_t e : tev) if (e) (void)hipEventDestroy(e);
+        for (void* p : owned) (void)hipFree(p);
+    }
 };
+
+// summed GPU time of the B traced bounces bracketed by a->tev
+static rc_status adj_trace_time(rc_adjoint* a, float* out) {
+    *out = 0.f;
+    if (a->B < 1) return RC_OK;
+    for (int k = 0; k < a->B; ++k) {
+        float ms = 0.f;
+        // wait for both events: a completed later event does not make an earlier one report ready
+        HIPCK(hipEventSynchronize(a->tev[2 * k]));
+        HIPCK(hipEventSynchronize(a->tev[2 * k + 1]));
+        HIPCK(hipEventElapsedTime(&ms, a->tev[2 * k], a->tev[2 * k + 1]));
+        *out += ms;
+    }
+    return RC_OK;
+}
 
 extern "C" rc_status rc_adjoint_create(rc_scene* s, int state, int K, int bounces, uint32_t seed, rc_adjoint** out) {
     if (!s || !out) return fail(RC_ERR_ARG, "NULL argument");
@@ -998,17 +1019,35 @@ extern "C" void rc_adjoint_destroy(rc_adjoint* a) { delete a; }
 extern "C" rc_status rc_adjoint_forward(rc_adjoint* a, const float* d_I, const float* d_alb, float* d_R, void* stream) {
     if (!a || !d_I || !d_alb) return fail(RC_ERR_ARG, "NULL argument");
     rc_scene* s = a->S;
-    HIPCK(rc_launch_adj_forward(s->trav, &s->ds, &s->dst[a->state], &a->w, d_I, d_alb, a->K, a->B, a->seed, d_R, (hipStream_t)stream));
+    HIPCK(rc_launch_adj_forward(s->trav, &s->ds, &s->dst[a->state], &a->w, d_I, d_alb, a->K, a->B, a->seed, d_R, (hipStream_t)stream,
+                                a->timed ? a->tev : nullptr));
     a->forward_done = true;
-    return RC_OK;
+    return a->timed ? adj_trace_time(a, &a->trace_ms[0]) : RC_OK;
 }
 
 extern "C" rc_status rc_adjoint_backward(rc_adjoint* a, const float* d_gR, float* d_gI, float* d_galb, void* stream) {
     if (!a || !d_gR || !d_gI || !d_galb) return fail(RC_ERR_ARG, "NULL argument");
     if (!a->forward_done) return fail(RC_ERR_STATE, "rc_adjoint_backward before rc_adjoint_forward");
     rc_scene* s = a->S;
-    HIPCK(rc_launch_adj_backward(s->trav, &s->ds, &s->dst[a->state], &a->w, d_gR, a->K, a->B, a->seed, d_gI, d_galb, (hipStream_t)stream));
+    HIPCK(rc_launch_adj_backward(s->trav, &s->ds, &s->dst[a->state], &a->w, d_gR, a->K, a->B, a->seed, d_gI, d_galb, (hipStream_t)stream,
+                                 a->timed ? a->tev : nullptr));
     a->backward_done = true;
+    return a->timed ? adj_trace_time(a, &a->trace_ms[1]) : RC_OK;
+}
+
+extern "C" rc_status rc_adjoint_set_timed(rc_adjoint* a, int timed) {
+    if (!a) return fail(RC_ERR_ARG, "NULL argument");
+    if (timed && !a->tev[0])
+        for (int k = 0; k < 2 * a->B; ++k) HIPCK(hipEventCreate(&a->tev[k]));
+    a->timed = timed != 0;
+    a->trace_ms[0] = a->trace_ms[1] = 0.f;
+    return RC_OK;
+}
+
+extern "C" rc_status rc_adjoint_trace_ms(const rc_adjoint* a, float ms_out[2]) {
+    if (!a || !ms_out) return fail(RC_ERR_ARG, "NULL argument");
+    ms_out[0] = a->trace_ms[0];
+    ms_out[1] = a->trace_ms[1];
     return RC_OK;
 }
 

@@ -44,9 +44,10 @@ hipError_t rc_launch_query_staged(const rc_dscene* s, const rc_dstate* st, const
                                   const rc_mlpdev* mlp, hipStream_t stream);
 hipError_t rc_launch_adj_unit(int trav, const rc_dscene* s, const rc_dstate* st, float* G, uint64_t* SG, hipStream_t stream);
 hipError_t rc_launch_adj_forward(int trav, const rc_dscene* s, const rc_dstate* st, const rc_adjbuf* w, const float* I,
-                                 const float* alb, int K, int B, uint32_t seed, float* R, hipStream_t stream);
+                                 const float* alb, int K, int B, uint32_t seed, float* R, hipStream_t stream,
+                                 hipEvent_t* ev = nullptr);   // ev: 2 B events around the traced bounces (or NULL)
 hipError_t rc_launch_adj_backward(int trav, const rc_dscene* s, const rc_dstate* st, const rc_adjbuf* w, const float* gR, int K,
-                                  int B, uint32_t seed, float* dI, float* dalb, hipStream_t stream);
+                                  int B, uint32_t seed, float* dI, float* dalb, hipStream_t stream, hipEvent_t* ev = nullptr);
 int rc_mlp_pad(int n);
 hipError_t rc_launch_mlp_infer(const rc_mlpdev* m, const float4* P, const float4* N, const float4* D, int n, float* y, hipStream_t st);
 hipError_t rc_launch_mlp_grad(const rc_mlpdev* m, const rc_mlpbuf* b, const float4* P, const float4* N, const float4* D, const float* t,

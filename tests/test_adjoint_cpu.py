@@ -84,3 +84,16 @@ def test_fp32_forward_close_to_fp64(room_adj):
     R64 = adj.grad64(I.astype(np.float32).astype(np.float64), alb.astype(np.float32).astype(np.float64), g)[1]
     rel = np.sqrt(np.mean((R32 - R64) ** 2)) / np.sqrt(np.mean(R64 ** 2))
     assert rel < 1e-5, rel
+
+
+def test_hits_rows_are_slices_of_the_full_trace(room_adj):
+    """the row-range tracer (used to count the rays' work on a sample of entries) gives the same
+    hits as the full trace of every bounce"""
+    s, osc, adj = room_adj
+    Np, K = s.info.Np, adj.K
+    for b in range(adj.B):
+        for i0, i1 in ((0, Np), (Np // 3, Np // 3 + 17), (Np - 5, Np)):
+            h = np.empty((i1 - i0) * K, np.int32)
+            po.lib.orc_adj_hits_rows(osc.state(0), 1, K, b, 5, i0, i1, po.ptr(h))
+            full = adj.hits[b * Np * K:(b + 1) * Np * K]
+            assert np.array_equal(h, full[i0 * K:i1 * K]), (b, i0, i1)

@@ -96,3 +96,23 @@ def test_autograd_function(torch_gpu, scenes):
     _, _, dI64, dalb64 = oadj.grad64(I.astype(np.float64), alb.astype(np.float64), g.astype(np.float64))
     assert _rel(It.grad.cpu().numpy().astype(np.float64), dI64) < 1e-4
     assert _rel(At.grad.cpu().numpy().astype(np.float64), dalb64) < 1e-4
+
+
+def test_timed_calls_give_the_same_results(torch_gpu, scenes):
+    """timed forward / backward (events around every traced bounce) return the same values as
+    untimed ones, and report positive bounce times"""
+    torch = torch_gpu
+    s, _ = scenes["street"]
+    adj = ad.Adjoint(s, state=0, K=2, bounces=3, seed=4)
+    I, alb, g = (torch.tensor(x, device="cuda") for x in _params(s, 0, 9))
+    R0 = adj.forward(I, alb).cpu().numpy()
+    gI0, ga0 = (x.cpu().numpy() for x in adj.backward(g))
+    assert adj.trace_ms() == (0.0, 0.0)
+    adj.set_timed(True)
+    R1 = adj.forward(I, alb).cpu().numpy()
+    gI1, ga1 = (x.cpu().numpy() for x in adj.backward(g))
+    f_ms, b_ms = adj.trace_ms()
+    adj.set_timed(False)
+    assert np.array_equal(R0.view(np.uint32), R1.view(np.uint32))
+    assert _rel(gI1, gI0) < 1e-5 and _rel(ga1, ga0) < 1e-5   # float atomics: order may differ
+    assert f_ms > 0 and b_ms > 0
