@@ -522,6 +522,8 @@ def main():
     ap.add_argument("--trav", type=int, default=2)
     ap.add_argument("--render", choices=["staged", "fused"], default="fused",
                     help="camera-pass schedule (same image): staged ray lists or one thread per path")
+    ap.add_argument("--stack-levels", type=int, default=0,
+                    help="LDS levels of the camera pass's traversal stacks (0: the library's choice)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-quality", action="store_true")
@@ -562,6 +564,8 @@ def main():
     m = rea.methods.plain()
     cache = rea.Cache(scene, m)
     cache.set_render_mode(rea.RENDER_STAGED if args.render == "staged" else rea.RENDER_FUSED)
+    if args.stack_levels:
+        cache.set_stack_levels(args.stack_levels)
     cam = scene.camera()
     img = torch.zeros(cam.W * cam.H, dtype=torch.float32, device="cuda")
     seed = 4242
@@ -664,7 +668,8 @@ def main():
                                 f"table 2^{int(np.log2(info.hsize))}, {cam.W}x{cam.H} x {cam.spp} spp, plain cache")
                    if cfg["kind"] == "street" else f"box room {info.nst} tris, {info.Np} entries, {cam.W}x{cam.H}",
                    "config_index": args.config, "traversal": ["binary-ref", "binary", "wide4"][args.trav],
-                   "render": args.render if args.trav == 2 else "fused"},
+                   "render": args.render if args.trav == 2 else "fused",
+                   "stack_levels_lds": args.stack_levels or "auto"},
         "cache_mpts_s": round(info.Np * args.steps / elapsed / 1e6, 2),
         "frames_per_s": round(args.steps / elapsed, 2),
         "scene_prep_s": round(prep_s, 3),

@@ -163,6 +163,17 @@ rc_status rc_cache_render(rc_cache* c, const rc_camera* cam, int no_cache, uint3
  * staged workspace (~100 B per path) is kept by the cache. */
 enum { RC_RENDER_FUSED = 0, RC_RENDER_STAGED = 1 };
 rc_status rc_cache_set_render_mode(rc_cache* c, int mode);
+/* Traversal stacks of the fused camera pass. By default each lane keeps all its stack levels in LDS,
+ * or only the first 16 with the deeper ones in a global buffer the cache keeps (4 B per path and
+ * level beyond 16) where that lets more waves reside; after this call always the first `levels`
+ * levels in LDS (1..64). The image does not depend on it. No counterpart in the reference (its
+ * traversal stack is per-thread local memory, cuda/poc_gpu.cu:297-338). */
+rc_status rc_cache_set_stack_levels(rc_cache* c, int levels);
+/* The same for the scene's other tracing kernels (entry pass, direct light, reference solve,
+ * learned-cache samples and camera pass) and for adjoint handles created afterwards; results do
+ * not depend on it. No counterpart in the reference (per-thread local-memory stacks,
+ * cuda/poc_gpu.cu:297-338). */
+rc_status rc_scene_set_stack_levels(rc_scene* s, int levels);
 /* Compact snapshot record (24 bytes; the SDK bench's SnapRec, rtxgi_bench.patch:1289-1345): entry index,
  * history count, entry key (cell key of the hash grid; 1 << 63 | index for points of dynamic quads),
  * indirect radiance summed over the light channels, frames since reset. */

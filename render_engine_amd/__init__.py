@@ -149,6 +149,13 @@ class Scene:
     def set_trav(self, trav):
         check(lib.rc_scene_set_trav(self._h, trav))
 
+    def set_stack_levels(self, levels):
+        """keep exactly `levels` levels of the traversal stacks of the scene's tracing kernels (and of
+        adjoint handles created afterwards) in LDS, the deeper ones in global memory (default: all
+        in LDS, or 16 where that lets more waves reside); results do not depend on it"""
+        check(lib.rc_scene_set_stack_levels(self._h, int(levels)))
+        return self
+
     def reference(self, K=32, warm=16, block=64, max_iter=2048, target=0.03):
         ns = self.info.NS
         noise = (C.c_float * ns)()
@@ -223,6 +230,12 @@ class Cache:
         """RENDER_FUSED (default): one thread per path; RENDER_STAGED (4-wide BVH): stage kernels over
         per-segment compacted ray lists. Both give the same image bit for bit."""
         check(lib.rc_cache_set_render_mode(self._h, int(mode)))
+
+    def set_stack_levels(self, levels):
+        """keep exactly `levels` levels of the fused camera pass's traversal stacks in LDS, the
+        deeper ones in a global buffer (default: all in LDS, or 16 where that lets more waves
+        reside); the image does not depend on it"""
+        check(lib.rc_cache_set_stack_levels(self._h, int(levels)))
 
     def get(self, name):
         which = CA[name]

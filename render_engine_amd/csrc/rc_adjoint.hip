@@ -24,11 +24,11 @@
 // G[i][l] and its support SG[i]: the shadow-ray visibility of k_build (closest hit beyond
 // r - 1e-3, equivalently no hit in (1e-4, r - 1e-3)), at unit intensity; rays are skipped
 // where the cosine is not positive (G = 0 there).
-template <int TRAV>
+template <int TRAV, bool SPLIT>
 __global__ void __launch_bounds__(RC_TB) k_adj_unit(rc_dscene s, rc_dstate st, float* G, uint64_t* SG) {
     extern __shared__ float sh[];
     int* shq = (int*)(sh + 9 * st.nt);
-    uint32_t* lstk = rc_lstack(sh, st.nt);
+    const auto lstk = rc_lstk_sel<SPLIT>::make(sh, st, s);
     rc_load_dyn(st, sh, shq);
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= s.Np) return;
@@ -93,8 +93,8 @@ __global__ void __launch_bounds__(RC_TB) k_adj_setup(int Np, int nl, const int* 
 }
 
 // the ray rule of the adjoint bounces: hit entry h of ray k of entry i, or -1
-template <int TRAV>
-__device__ __forceinline__ int rc_adj_ray(const rc_dscene& s, const rc_dstate& st, const float* sh, const int* shq, uint32_t* lstk,
+template <int TRAV, class LS>
+__device__ __forceinline__ int rc_adj_ray(const rc_dscene& s, const rc_dstate& st, const float* sh, const int* shq, const LS& lstk,
                                           int i, f3 O, f3 N, uint32_t& rs) {
     const float u1 = rc_rnd(rs), u2 = rc_rnd(rs);
     const f3 D = rc_cosine_dir(N, u1, u2);
@@ -107,12 +107,12 @@ __device__ __forceinline__ int rc_adj_ray(const rc_dscene& s, const rc_dstate& s
 
 // one forward bounce: Lout_i = (1/K) sum A_h (E_h + Lin_h), Sout_i = union of SG_h | Sin_h over
 // hits with A_h > 0 (Lin, Sin NULL: zero)
-template <int TRAV>
+template <int TRAV, bool SPLIT>
 __global__ void __launch_bounds__(RC_TB) k_adj_fwd(rc_dscene s, rc_dstate st, const float* A, const float* E, const uint64_t* SG,
                                                   const float* Lin, const uint64_t* Sin, float* Lout, uint64_t* Sout, uint32_t seed, int K) {
     extern __shared__ float sh[];
     int* shq = (int*)(sh + 9 * st.nt);
-    uint32_t* lstk = rc_lstack(sh, st.nt);
+    const auto lstk = rc_lstk_sel<SPLIT>::make(sh, st, s);
     rc_load_dyn(st, sh, shq);
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= s.Np) return;
@@ -156,11 +156,11 @@ __global__ void __launch_bounds__(RC_TB) k_adj_bwd_init(int Np, const float* g, 
 // w = lam_i / K of the ray's entry i into S_h (one atomic per hit). Every term the hit feeds
 // is a per-h factor times that weight, so k_adj_bwd_fin then adds (E_h + L^(b)_h) S_h to dA,
 // A_h S_h to dE and sets the adjoint of L^(b) to A_h S_h.
-template <int TRAV>
+template <int TRAV, bool SPLIT>
 __global__ void __launch_bounds__(RC_TB) k_adj_bwd(rc_dscene s, rc_dstate st, const float* lam_in, float* S, uint32_t seed, int K) {
     extern __shared__ float sh[];
     int* shq = (int*)(sh + 9 * st.nt);
-    uint32_t* lstk = rc_lstack(sh, st.nt);
+    const auto lstk = rc_lstk_sel<SPLIT>::make(sh, st, s);
     rc_load_dyn(st, sh, shq);
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= s.Np) return;
@@ -270,14 +270,14 @@ __global__ void __launch_bounds__(RC_TB) k_adj_dalb(int nmat, const int* moff, c
 // ------------------------------------------------------------------------------ launchers
 #define RC_ADJ_T(KER, ...)                                                                         \
     do {                                                                                            \
-        if (trav == 0) hipLaunchKernelGGL(KER<0>, g, b, shm, stream, __VA_ARGS__);                 \
-        else if (trav == 1) hipLaunchKernelGGL(KER<1>, g, b, shm, stream, __VA_ARGS__);            \
-        else hipLaunchKernelGGL(KER<2>, g, b, shm, stream, __VA_ARGS__);                           \
+        if (trav == 0) hipLaunchKernelGGL((KER<0, false>), g, b, shm, stream, __VA_ARGS__);        \
+        else if (trav == 1) hipLaunchKernelGGL((KER<1, false>), g, b, shm, stream, __VA_ARGS__);   \
+        else RC_LAUNCH_WIDE((KER<2, true>), (KER<2, false>), false, g, st, s, stream, __VA_ARGS__);       \
     } while (0)
 
 hipError_t rc_launch_adj_unit(int trav, const rc_dscene* s, const rc_dstate* st, float* G, uint64_t* SG, hipStream_t stream) {
     const dim3 g(rc_grid(s->Np)), b(RC_TB);
-    const size_t shm = rc_shm(*st, *s, trav);
+    const size_t shm = rc_shm(*st, *s, trav, false);
     RC_ADJ_T(k_adj_unit, *s, *st, G, SG);
     return hipGetLastError();
 }
@@ -286,7 +286,7 @@ hipError_t rc_launch_adj_forward(int trav, const rc_dscene* s, const rc_dstate* 
                                  const float* alb, int K, int B, uint32_t seed, float* R, hipStream_t stream, hipEvent_t* ev) {
     const int Np = s->Np;
     const dim3 g(rc_grid(Np)), b(RC_TB);
-    const size_t shm = rc_shm(*st, *s, trav);
+    const size_t shm = rc_shm(*st, *s, trav, false);
     hipLaunchKernelGGL(k_adj_setup, dim3((Np + RC_AS - 1) / RC_AS), b, 0, stream, Np, s->nl, w->mat, alb, I, w->G, w->A, w->E);
     for (int k = 0; k < B; ++k) {
         const uint32_t sb = rc_hash32(seed + (uint32_t)k * 0x9E3779B9u);
@@ -304,7 +304,7 @@ hipError_t rc_launch_adj_backward(int trav, const rc_dscene* s, const rc_dstate*
                                   int B, uint32_t seed, float* dI, float* dalb, hipStream_t stream, hipEvent_t* ev) {
     const int Np = s->Np, nl = s->nl;
     const dim3 g(rc_grid(Np)), b(RC_TB);
-    const size_t shm = rc_shm(*st, *s, trav);
+    const size_t shm = rc_shm(*st, *s, trav, false);
     hipLaunchKernelGGL(k_adj_bwd_init, g, b, 0, stream, Np, gR, w->A, w->E, B > 0 ? w->L + (size_t)(B - 1) * Np : (const float*)nullptr,
                        w->dA, w->dE, w->lam[B & 1]);
     for (int k = B - 1; k >= 0; --k) {

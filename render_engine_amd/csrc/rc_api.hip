@@ -68,15 +68,53 @@ struct rc_scene {
     unsigned long long* d_lights = nullptr;   // event light set (k_vol_lights)
     uint64_t* d_ekeys = nullptr;              // per entry: its hash-grid key (1 << 63 | index for quad points)
     std::vector<void*> owned;
+    // global levels of the lane stacks of the wide-traversal kernels on the scene stream (levels
+    // beyond RC_LCAP, one column per thread of a launch; grown on demand, the stream orders its users)
+    uint32_t* ovf = nullptr;
+    size_t ovf_words = 0;
+    int lcap = RC_LCAP;           // LDS levels of those split lane stacks (and of adjoint handles created later)
+    bool lforce = false;          // split always (set by rc_scene_set_stack_levels) or where it raises occupancy
     // converged references (host + device)
     bool has_ref = false;
     std::vector<std::vector<float>> ref, Pw;
     std::vector<std::vector<uint8_t>> valid;
     ~rc_scene() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        if (ovf) (void)hipFree(ovf);
         for (void* p : owned) (void)hipFree(p);
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
+
+// ------------------------------------------------------------------------------------ lane stacks
+// State `base` for a wide-traversal launch of nthreads threads whose lane stacks may keep cap levels
+// in LDS and the deeper ones in *buf (grown to what the launch needs; `sync`, the only stream that
+// uses *buf, is drained before the old buffer is freed): always (force) or where that lets more
+// blocks reside than all levels in LDS. cap 0 or a binary-BVH scene: all in LDS.
+static hipError_t split_dst(const rc_dstate& base, const rc_dscene& ds, int trav, int cap, bool force, uint32_t** buf, size_t* words,
+                            long long nthreads, hipStream_t sync, rc_dstate* out) {
+    rc_dstate d = base;
+    d.ocap = trav == 2 ? cap : 0;
+    d.oforce = force ? 1 : 0;
+    const size_t need = rc_ovf_words(&d, &ds, nthreads);
+    if (need > *words) {
+        hipError_t e = hipStreamSynchronize(sync);
+        if (e != hipSuccess) return e;
+        if (*buf) { (void)hipFree(*buf); *buf = nullptr; *words = 0; }
+        e = hipMalloc(buf, need * sizeof(uint32_t));
+        if (e != hipSuccess) { *buf = nullptr; return e; }
+        *words = need;
+    }
+    d.ovf = *buf;
+    d.ovf_words = *words;
+    *out = d;
+    return hipSuccess;
+}
+static long long grid_threads(long long n) { return rc_grid_threads(n); }
+// state si for a launch of nthreads threads on the scene stream
+static hipError_t scene_dst(rc_scene* s, int si, long long nthreads, rc_dstate* out) {
+    return split_dst(s->dst[si], s->ds, s->trav, s->lcap, s->lforce, &s->ovf, &s->ovf_words, nthreads, s->stream, out);
+}
 
 // ------------------------------------------------------------------------------------ scenes
 static rc_status make_scene(const rc::SceneDesc& d, rc_scene** out) {
@@ -304,7 +342,11 @@ extern "C" rc_status rc_scene_upload(rc_scene* s, int trav) {
             if (H.HK[k] != 0ull && H.HV[k] >= 0 && H.HV[k] < H.Np) ek[H.HV[k]] = H.HK[k];
         HIPCK(dput(&s->d_ekeys, ek, s->owned));
     }
-    for (int k = 0; k < H.NS; ++k) HIPCK(rc_launch_build(trav, &s->ds, &s->dst[k], s->stream));
+    for (int k = 0; k < H.NS; ++k) {
+        rc_dstate dk;
+        HIPCK(scene_dst(s, k, grid_threads(H.Np), &dk));
+        HIPCK(rc_launch_build(trav, &s->ds, &dk, s->stream));
+    }
     HIPCK(hipStreamSynchronize(s->stream));
     if (s->has_ref) {   // references read from a scene file
         for (int si = 0; si < H.NS; ++si) {
@@ -347,7 +389,9 @@ extern "C" rc_status rc_scene_reference(rc_scene* s, int K, int warm, int block,
                 for (int k = 0; k < n_it; ++k) {
                     int it = it_done + k;
                     uint32_t seed = (uint32_t)rc::mix64(0x51ull + 7919ull * si + 104729ull * c + 31ull * it);
-                    TCK(rc_launch_ref(s->trav, &s->ds, &s->dst[si], a, b, acc[c], closec, seed, K, it >= warm, c == 0, s->stream));
+                    rc_dstate dk;
+                    TCK(scene_dst(s, si, grid_threads(Np), &dk));
+                    TCK(rc_launch_ref(s->trav, &s->ds, &dk, a, b, acc[c], closec, seed, K, it >= warm, c == 0, s->stream));
                     std::swap(a, b);
                 }
                 if (a != chain[c]) TCK(hipMemcpyAsync(chain[c], a, nl * Np * 4, hipMemcpyDeviceToDevice, s->stream));
@@ -429,6 +473,10 @@ struct rc_cache {
     hipEvent_t ev_upd = nullptr, ev_q = nullptr;
     bool q_pending = false;
     std::vector<uint64_t*> gmask;   // per state: gradient-support masks (grad_masks methods)
+    uint32_t* qovf = nullptr;     // lane-stack overflow of the fused camera pass (qovf_words words)
+    size_t qovf_words = 0;
+    int lcap = RC_LCAP;           // LDS levels of the camera pass's split lane stacks
+    bool lforce = false;          // split always (set by rc_cache_set_stack_levels) or where it raises occupancy
     rc_qwork qw{};                // staged camera-pass workspace, sized for qw_paths paths
     long long qw_paths = 0;
     std::vector<void*> qw_owned;
@@ -444,6 +492,7 @@ struct rc_cache {
         if (ev_q) (void)hipEventDestroy(ev_q);
         for (void* q : owned) (void)hipFree(q);
         free_qwork();
+        if (qovf) (void)hipFree(qovf);
     }
 };
 
@@ -725,7 +774,9 @@ extern "C" rc_status rc_cache_frame(rc_cache* c, int si, int want_err, int timed
     }
     if (timed) HIPCK(hipEventRecord(c->ev[1], st));
     bool track = M.deps && (c->f % M.track_every == 0);
-    HIPCK(rc_launch_trace2(S->trav, track ? 1 : 0, &S->ds, &S->dst[si], &d, &p, c->lnew, st));
+    rc_dstate dk;
+    HIPCK(scene_dst(S, si, grid_threads(p.Np), &dk));
+    HIPCK(rc_launch_trace2(S->trav, track ? 1 : 0, &S->ds, &dk, &d, &p, c->lnew, st));
     if (timed) HIPCK(hipEventRecord(c->ev[2], st));
     if (c->pend_left > 0) {
         HIPCK(hipMemsetAsync(d.npend + 1, 0, 4, st));
@@ -783,6 +834,22 @@ static hipError_t qwork_reserve(rc_cache* c, long long npath) {
     return hipSuccess;
 }
 
+extern "C" rc_status rc_scene_set_stack_levels(rc_scene* s, int levels) {
+    if (!s) return fail(RC_ERR_ARG, "NULL argument");
+    if (levels < 1 || levels > 64) return fail(RC_ERR_ARG, "stack levels must be in [1, 64]");
+    s->lcap = levels;
+    s->lforce = true;
+    return RC_OK;
+}
+
+extern "C" rc_status rc_cache_set_stack_levels(rc_cache* c, int levels) {
+    if (!c) return fail(RC_ERR_ARG, "NULL argument");
+    if (levels < 1 || levels > 64) return fail(RC_ERR_ARG, "stack levels must be in [1, 64]");
+    c->lcap = levels;
+    c->lforce = true;
+    return RC_OK;
+}
+
 extern "C" rc_status rc_cache_set_render_mode(rc_cache* c, int mode) {
     if (!c) return fail(RC_ERR_ARG, "NULL argument");
     if (mode != RC_RENDER_FUSED && mode != RC_RENDER_STAGED) return fail(RC_ERR_ARG, "render mode must be 0 (fused) or 1 (staged)");
@@ -807,6 +874,9 @@ extern "C" rc_status rc_cache_render(rc_cache* c, const rc_camera* cam, int no_c
     }
     const bool staged = c->render_mode == RC_RENDER_STAGED && S->trav == 2;
     if (staged) HIPCK(qwork_reserve(c, (long long)cam->W * cam->H * cam->spp));
+    rc_dstate dq = S->dst[si];   // the fused pass: lane stacks split at c->lcap levels, the rest in c->qovf
+    if (!staged)
+        HIPCK(split_dst(S->dst[si], S->ds, S->trav, c->lcap, c->lforce, &c->qovf, &c->qovf_words, rc_query_threads(cam), c->rs, &dq));
     // the pass runs on the cache's render stream, after everything queued on the scene stream
     hipStream_t qs = c->rs;
     HIPCK(hipEventRecord(c->ev_upd, st));
@@ -818,8 +888,7 @@ extern "C" rc_status rc_cache_render(rc_cache* c, const rc_camera* cam, int no_c
         HIPCK(rc_launch_query_staged(&S->ds, &S->dst[si], no_cache ? nullptr : c->c.R, cam, seed, (uint32_t)c->f, c->dormant, d_img, ctr,
                                      &c->qw, nullptr, qs));
     else
-        HIPCK(rc_launch_query(S->trav, &S->ds, &S->dst[si], no_cache ? nullptr : c->c.R, cam, seed, (uint32_t)c->f, c->dormant, d_img, ctr,
-                              qs));
+        HIPCK(rc_launch_query(S->trav, &S->ds, &dq, no_cache ? nullptr : c->c.R, cam, seed, (uint32_t)c->f, c->dormant, d_img, ctr, qs));
     HIPCK(hipEventRecord(c->ev_q, qs));
     c->q_pending = true;
     if (ms_out) {
@@ -950,8 +1019,12 @@ struct rc_adjoint {
     bool timed = false;
     hipEvent_t tev[2 * 16] = {};   // around each traced bounce (B <= 16)
     float trace_ms[2] = {0.f, 0.f};
+    rc_dstate dst{};               // the state with this handle's lane-stack split (global levels in ovf)
+    uint32_t* ovf = nullptr;
+    size_t ovf_words = 0;
     ~rc_adjoint() {
         for (hipEvent_t e : tev) if (e) (void)hipEventDestroy(e);
+        if (ovf) (void)hipFree(ovf);
         for (void* p : owned) (void)hipFree(p);
     }
 };
@@ -1007,7 +1080,9 @@ extern "C" rc_status rc_adjoint_create(rc_scene* s, int state, int K, int bounce
     std::vector<int> pmat(Np);
     for (int m = 0; m < H.nmat; ++m) for (int j = moff[m]; j < moff[m + 1]; ++j) pmat[j] = m;
     AK(hipMemcpyAsync(w.pmat, pmat.data(), Np * 4, hipMemcpyHostToDevice, s->stream));
-    AK(rc_launch_adj_unit(s->trav, &s->ds, &s->dst[state], w.G, w.SG, s->stream));
+    // the handle's kernels run on the caller's stream: a global stack buffer of their own
+    AK(split_dst(s->dst[state], s->ds, s->trav, s->lcap, s->lforce, &a->ovf, &a->ovf_words, grid_threads(H.Np), s->stream, &a->dst));
+    AK(rc_launch_adj_unit(s->trav, &s->ds, &a->dst, w.G, w.SG, s->stream));
     AK(hipStreamSynchronize(s->stream));
 #undef AK
     *out = a;
@@ -1019,7 +1094,7 @@ extern "C" void rc_adjoint_destroy(rc_adjoint* a) { delete a; }
 extern "C" rc_status rc_adjoint_forward(rc_adjoint* a, const float* d_I, const float* d_alb, float* d_R, void* stream) {
     if (!a || !d_I || !d_alb) return fail(RC_ERR_ARG, "NULL argument");
     rc_scene* s = a->S;
-    HIPCK(rc_launch_adj_forward(s->trav, &s->ds, &s->dst[a->state], &a->w, d_I, d_alb, a->K, a->B, a->seed, d_R, (hipStream_t)stream,
+    HIPCK(rc_launch_adj_forward(s->trav, &s->ds, &a->dst, &a->w, d_I, d_alb, a->K, a->B, a->seed, d_R, (hipStream_t)stream,
                                 a->timed ? a->tev : nullptr));
     a->forward_done = true;
     return a->timed ? adj_trace_time(a, &a->trace_ms[0]) : RC_OK;
@@ -1029,7 +1104,7 @@ extern "C" rc_status rc_adjoint_backward(rc_adjoint* a, const float* d_gR, float
     if (!a || !d_gR || !d_gI || !d_galb) return fail(RC_ERR_ARG, "NULL argument");
     if (!a->forward_done) return fail(RC_ERR_STATE, "rc_adjoint_backward before rc_adjoint_forward");
     rc_scene* s = a->S;
-    HIPCK(rc_launch_adj_backward(s->trav, &s->ds, &s->dst[a->state], &a->w, d_gR, a->K, a->B, a->seed, d_gI, d_galb, (hipStream_t)stream,
+    HIPCK(rc_launch_adj_backward(s->trav, &s->ds, &a->dst, &a->w, d_gR, a->K, a->B, a->seed, d_gI, d_galb, (hipStream_t)stream,
                                  a->timed ? a->tev : nullptr));
     a->backward_done = true;
     return a->timed ? adj_trace_time(a, &a->trace_ms[1]) : RC_OK;
@@ -1100,7 +1175,7 @@ static rc_status cache_grad_masks(rc_cache* c) {
         if (r != RC_OK) return r;
         uint64_t* g = nullptr;
         hipError_t e = dalloc(&g, Np, c->owned);
-        if (e == hipSuccess) e = rc_launch_adj_forward(S->trav, &S->ds, &S->dst[st], &a->w, dI, dalb, a->K, a->B, a->seed, nullptr, S->stream);
+        if (e == hipSuccess) e = rc_launch_adj_forward(S->trav, &S->ds, &a->dst, &a->w, dI, dalb, a->K, a->B, a->seed, nullptr, S->stream);
         if (e == hipSuccess) e = hipMemcpyAsync(g, a->w.SR, Np * 8, hipMemcpyDeviceToDevice, S->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(S->stream);
         rc_adjoint_destroy(a);
@@ -1299,7 +1374,9 @@ extern "C" rc_status rc_mlp_train_scene(rc_mlp* m, rc_cache* c, int n, uint32_t 
         HIPCK(rc_launch_resolve(&S->ds, &S->dst[si], c->c.Lc, c->c.R, c->dormant, st));
         c->r_valid = true;
     }
-    HIPCK(rc_launch_mlp_samples(S->trav, &S->ds, &S->dst[si], c->dormant, seed, n, m->sP, m->sN, m->sD, m->hP, m->hN, m->hD, m->AE, m->ent, st));
+    rc_dstate dk;
+    HIPCK(scene_dst(S, si, grid_threads(n), &dk));
+    HIPCK(rc_launch_mlp_samples(S->trav, &S->ds, &dk, c->dormant, seed, n, m->sP, m->sN, m->sD, m->hP, m->hN, m->hD, m->AE, m->ent, st));
     HIPCK(rc_launch_mlp_infer(&m->d, m->hP, m->hN, m->hD, n, m->hy, st));
     HIPCK(rc_launch_mlp_targets(n, m->hP, m->hy, m->AE, m->tg, st));
     HIPCK(rc_launch_mlp_zero(m->b.loss, 1, st));
@@ -1328,11 +1405,13 @@ extern "C" rc_status rc_mlp_render(rc_mlp* m, rc_cache* c, const rc_camera* cam,
     const int si = c->prev_state;
     HIPCK(wait_query(c));   // the grid cache's camera pass may still use the staged workspace
     HIPCK(qwork_reserve(c, (long long)cam->W * cam->H * cam->spp));
+    rc_dstate dk = S->dst[si];
+    if (c->render_mode != RC_RENDER_STAGED) HIPCK(scene_dst(S, si, grid_threads((long long)cam->W * cam->H * cam->spp), &dk));
     if (ms_out) HIPCK(hipEventRecord(c->ev[0], st));
     if (c->render_mode == RC_RENDER_STAGED)
         HIPCK(rc_launch_query_staged(&S->ds, &S->dst[si], nullptr, cam, seed, (uint32_t)c->f, c->dormant, d_img, nullptr, &c->qw, &m->d, st));
     else
-        HIPCK(rc_launch_query_mlp(&S->ds, &S->dst[si], cam, seed, (uint32_t)c->f, c->dormant, d_img, &c->qw, &m->d, st));
+        HIPCK(rc_launch_query_mlp(&S->ds, &dk, cam, seed, (uint32_t)c->f, c->dormant, d_img, &c->qw, &m->d, st));
     if (ms_out) {
         HIPCK(hipEventRecord(c->ev[1], st));
         HIPCK(hipEventSynchronize(c->ev[1]));

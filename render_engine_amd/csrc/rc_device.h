@@ -269,6 +269,41 @@ __device__ __forceinline__ void rc_child_enter(const rc_wray& r, const rc_dscene
 #undef RC_SLAB
 }
 
+// Lane stack and leaf queue of the wide traversal: LDS levels one column per lane (256 entries
+// apart), the RC_WQ queue levels after the first cap levels. rc_lstk_l keeps every stack level in
+// LDS (cap = wstack); rc_lstk_s keeps the first cap levels in LDS and the deeper ones in global
+// memory (uniform base < 4 GiB; level cap + k of thread g at word k * ostride + g, g = the
+// thread's index in the grid), which only a ray nested deeper than cap touches. The stack never
+// holds more than wstack entries (at most 3 pushes per wide level).
+struct rc_lstk_l {
+    uint32_t* col;        // this lane's LDS column
+    int cap;
+    __device__ __forceinline__ void push(int sp, uint32_t e) const { col[sp * 256] = e; }
+    __device__ __forceinline__ uint32_t pop(int sp) const { return col[sp * 256]; }
+    __device__ __forceinline__ uint32_t* queue() const { return col + (size_t)cap * 256; }
+};
+struct rc_lstk_s {
+    uint32_t* col;
+    uint32_t* ovf;
+    uint32_t ostride;
+    int cap;
+    // byte offset of global level sp (32-bit: a scalar base plus one vector offset per access)
+    __device__ __forceinline__ uint32_t off(int sp) const {
+        return ((uint32_t)(sp - cap) * ostride + blockIdx.x * blockDim.x + threadIdx.x) * 4u;
+    }
+    __device__ __forceinline__ void push(int sp, uint32_t e) const {
+        if (sp < cap) col[sp * 256] = e;
+        else *(uint32_t*)((char*)ovf + off(sp)) = e;
+    }
+    __device__ __forceinline__ uint32_t pop(int sp) const {
+        uint32_t e;
+        if (sp < cap) e = col[sp * 256];
+        else e = *(const uint32_t*)((const char*)ovf + off(sp));
+        return e;
+    }
+    __device__ __forceinline__ uint32_t* queue() const { return col + (size_t)cap * 256; }
+};
+
 // 4-wide BVH closest hit of the static geometry, order-independent rule, with postponed
 // leaves. A node step tests the node's four child boxes, queues the triangle ranges of the
 // entered leaf children (one LDS column per lane, RC_WQ levels after the stack; entries
@@ -280,10 +315,10 @@ __device__ __forceinline__ void rc_child_enter(const rc_wray& r, const rc_dscene
 // current tmin, which may lag the queued hits: more node steps at worst, never a missed hit,
 // and the hit itself does not depend on the order. A leaf longer than 127 triangles (only at
 // the builder's depth limit) is walked at once. The stack depth is bounded by the scene's
-// wstack levels (3 per wide level).
-__device__ __forceinline__ void rc_trace_wide_closest(f3 o, f3 d, const rc_dscene& s, float& tmin, int& q, uint32_t* lstk) {
-    uint32_t* stk = lstk + (threadIdx.x & 255);
-    uint32_t* que = stk + (size_t)s.wstack * 256;
+// wstack levels (3 per wide level); the levels beyond lstk.cap live in global memory.
+template <class LS>
+__device__ __forceinline__ void rc_trace_wide_closest(f3 o, f3 d, const rc_dscene& s, float& tmin, int& q, const LS& lstk) {
+    uint32_t* que = lstk.queue();
     const rc_wray r = rc_wray_make(o, d);
     int sp = 0, cur = 0, nq = 0;   // cur < 0: this lane's traversal is over
     for (;;) {
@@ -330,14 +365,14 @@ __device__ __forceinline__ void rc_trace_wide_closest(f3 o, f3 d, const rc_dscen
         // the children still in front of the current hit are a prefix of the sorted list: push
         // all but the nearest far-to-near, visit the nearest next
 #define RC_PUSH(tc, ic) \
-        if (tc < 1e30f && tc <= tmin) { stk[sp * 256] = ((uint32_t)ic << 8) | rc_tcode(tc); ++sp; }
+        if (tc < 1e30f && tc <= tmin) { lstk.push(sp, ((uint32_t)ic << 8) | rc_tcode(tc)); ++sp; }
         RC_PUSH(t3, i3) RC_PUSH(t2, i2) RC_PUSH(t1, i1)
 #undef RC_PUSH
         if (t0 < 1e30f && t0 <= tmin) { cur = i0; continue; }
         cur = -1;
         while (sp > 0) {
             --sp;
-            const uint32_t e = stk[sp * 256];
+            const uint32_t e = lstk.pop(sp);
             if (rc_tlb(e & 255u) <= tmin) { cur = (int)(e >> 8); break; }
         }
     }
@@ -349,9 +384,9 @@ __device__ __forceinline__ void rc_trace_wide_closest(f3 o, f3 d, const rc_dscen
 // near-first, so blockers next to the shading point, the likely ones, are found early. Leaves
 // are postponed as in the closest-hit walk (queued per lane, walked by the wave together); a
 // lane stops at its first blocker. The answer does not depend on the order.
-__device__ __forceinline__ bool rc_occluded_wide(f3 o, f3 d, const rc_dscene& s, float tlim, uint32_t* lstk) {
-    uint32_t* stk = lstk + (threadIdx.x & 255);
-    uint32_t* que = stk + (size_t)s.wstack * 256;
+template <class LS>
+__device__ __forceinline__ bool rc_occluded_wide(f3 o, f3 d, const rc_dscene& s, float tlim, const LS& lstk) {
+    uint32_t* que = lstk.queue();
     const rc_wray r = rc_wray_make(o, d);
     int sp = 0, cur = 0, nq = 0;   // cur < 0: this lane's traversal is over
     bool blocked = false;
@@ -399,13 +434,13 @@ __device__ __forceinline__ bool rc_occluded_wide(f3 o, f3 d, const rc_dscene& s,
         rc_cswap(t1, i1, t2, i2);
         // entered children are a prefix of the sorted list: push all but the nearest far-to-near
 #define RC_ANYPUSH(tc, ic) \
-        if (tc < 1e30f) { stk[sp * 256] = (uint32_t)ic << 8; ++sp; }
+        if (tc < 1e30f) { lstk.push(sp, (uint32_t)ic << 8); ++sp; }
         RC_ANYPUSH(t3, i3) RC_ANYPUSH(t2, i2) RC_ANYPUSH(t1, i1)
 #undef RC_ANYPUSH
         if (t0 < 1e30f) { cur = i0; continue; }
         if (sp == 0) { cur = -1; continue; }
         --sp;
-        cur = (int)(stk[sp * 256] >> 8);
+        cur = (int)(lstk.pop(sp) >> 8);
     }
     return blocked;
 }

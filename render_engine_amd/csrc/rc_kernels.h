@@ -25,8 +25,13 @@ hipError_t rc_launch_flush(const rc_dcache* c, int Np, int nl, hipStream_t strea
 hipError_t rc_launch_error(const rc_dcache* c, int Np, int nl, const float* ref, const uint8_t* valid, uint64_t dormant, hipStream_t stream);
 hipError_t rc_launch_ref(int trav, const rc_dscene* s, const rc_dstate* st, const float* Lin, float* Lout, float* acc, float* closec,
                          uint32_t seed, int K, int accumulate, int count_close, hipStream_t stream);
+// camera pass (fused); rc_query_threads: threads of its grid (the global stack levels need one column each)
 hipError_t rc_launch_query(int trav, const rc_dscene* s, const rc_dstate* st, const float* R, const rc_camera* cam, uint32_t seed,
                            uint32_t frame, uint64_t dormant, float* img, unsigned long long* counters, hipStream_t stream);
+long long rc_query_threads(const rc_camera* cam);
+// global lane-stack words a launch of nthreads threads needs with st's split; threads of an n-item grid
+size_t rc_ovf_words(const rc_dstate* st, const rc_dscene* s, long long nthreads);
+long long rc_grid_threads(long long n);
 hipError_t rc_launch_hash_insert(unsigned long long* HK, int* HV, uint32_t hmask, const unsigned long long* keys, const int* vals,
                                  int n, int* fail, hipStream_t stream);
 hipError_t rc_launch_hash_query(const unsigned long long* HK, const int* HV, uint32_t hmask, const unsigned long long* keys, int n,

@@ -9,12 +9,12 @@
 #include "rc_trace.h"
 
 // ------------------------------------------------------------------------------ rays (kernel ABI)
-template <int TRAV>
+template <int TRAV, bool SPLIT>
 __global__ void __launch_bounds__(RC_TB) k_trace_rays(rc_dscene s, rc_dstate st, const float* __restrict__ O,
                                                      const float* __restrict__ D, int n, float* tout, int* qout) {
     extern __shared__ float sh[];
     int* shq = (int*)(sh + 9 * st.nt);
-    uint32_t* lstk = rc_lstack(sh, st.nt);
+    const auto lstk = rc_lstk_sel<SPLIT>::make(sh, st, s);
     rc_load_dyn(st, sh, shq);
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -27,11 +27,11 @@ __global__ void __launch_bounds__(RC_TB) k_trace_rays(rc_dscene s, rc_dstate st,
 // Direct light per entry and light, static transit mask of the shadow segments and the
 // entry's light-visibility mask (cuda/poc_gpu.cu:369-391). Directional lights (the sun) use
 // no distance falloff and a shadow segment of length s.sun_len.
-template <int TRAV>
+template <int TRAV, bool SPLIT>
 __global__ void __launch_bounds__(RC_TB) k_build(rc_dscene s, rc_dstate st) {
     extern __shared__ float sh[];
     int* shq = (int*)(sh + 9 * st.nt);
-    uint32_t* lstk = rc_lstack(sh, st.nt);
+    const auto lstk = rc_lstk_sel<SPLIT>::make(sh, st, s);
     rc_load_dyn(st, sh, shq);
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= s.Np) return;
@@ -72,11 +72,11 @@ __device__ __forceinline__ float rc_est_of(const float* Lc, int64_t i, int nl, u
 // thread's own row of c.m (rays outer, lights inner) so any light count runs in one kernel.
 // Light-dependency masks (deps on): own visible lights | hit's visible lights | hit's recorded
 // light deps (cache-terminated path inherits them), staged in lnew and merged by k_update.
-template <bool DEPS, int TRAV>
+template <bool DEPS, int TRAV, bool SPLIT>
 __global__ void __launch_bounds__(RC_TB) k_trace2(rc_dscene s, rc_dstate st, rc_dcache c, rc_params p, uint64_t* lnew) {
     extern __shared__ float sh[];
     int* shq = (int*)(sh + 9 * st.nt);
-    uint32_t* lstk = rc_lstack(sh, st.nt);
+    const auto lstk = rc_lstk_sel<SPLIT>::make(sh, st, s);
     rc_load_dyn(st, sh, shq);
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= p.Np) return;
@@ -515,12 +515,12 @@ __global__ void __launch_bounds__(RC_TB) k_error(rc_dcache c, int Np, int nl, co
 
 // One fresh-ray Jacobi iteration Lout = mean_K A (Edir + Lin) of the converged reference
 // (cuda/prep_mesh.cuh:182-213).
-template <int TRAV>
+template <int TRAV, bool SPLIT>
 __global__ void __launch_bounds__(RC_TB) k_ref(rc_dscene s, rc_dstate st, const float* Lin, float* Lout, float* acc,
                                               float* closec, uint32_t seed, int K, int accumulate, int count_close) {
     extern __shared__ float sh[];
     int* shq = (int*)(sh + 9 * st.nt);
-    uint32_t* lstk = rc_lstack(sh, st.nt);
+    const auto lstk = rc_lstk_sel<SPLIT>::make(sh, st, s);
     rc_load_dyn(st, sh, shq);
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= s.Np) return;
@@ -555,9 +555,9 @@ __global__ void __launch_bounds__(RC_TB) k_ref(rc_dscene s, rc_dstate st, const 
 // ------------------------------------------------------------------------------ camera query
 
 // One-light next-event estimate at H (normal n): light chosen uniformly, weight nl.
-template <int TRAV>
+template <int TRAV, class LS>
 __device__ __forceinline__ float rc_nee(const rc_dscene& s, const rc_dstate& st, const float* sh, const int* shq,
-                                        f3 H, f3 n, uint32_t& rs, uint64_t dormant, int& shadow_rays, uint32_t* lstk) {
+                                        f3 H, f3 n, uint32_t& rs, uint64_t dormant, int& shadow_rays, const LS& lstk) {
     const int nl = s.nl;
     float u = rc_rnd(rs);
     int l = min((int)(u * (float)nl), nl - 1);
@@ -587,10 +587,10 @@ __device__ __forceinline__ float rc_nee(const rc_dscene& s, const rc_dstate& st,
 // bounce `cache_bounce` the path ends on the resolved radiance R of the hit entry (the value a
 // cache-terminated ray reads, A (Edir + Lc) over the active lights, cuda/poc_gpu.cu:470-478);
 // R == nullptr: no cache, the path ends after `bounces` bounces.
-template <int TRAV>
+template <int TRAV, class LS>
 __device__ __forceinline__ float rc_path(const rc_dscene& s, const rc_dstate& st, const float* sh, const int* shq, const float* R,
                                          const rc_camera& cam, int px, int py, uint32_t& rs, uint64_t dormant, int& ext_rays,
-                                         int& sh_rays, uint32_t* lstk) {
+                                         int& sh_rays, const LS& lstk) {
     float ju = rc_rnd(rs), jv = rc_rnd(rs);
     float sx = (2.f * ((float)px + ju) / (float)cam.W - 1.f) * cam.tanx;
     float sy = (1.f - 2.f * ((float)py + jv) / (float)cam.H) * cam.tany;
@@ -622,12 +622,12 @@ __device__ __forceinline__ float rc_path(const rc_dscene& s, const rc_dstate& st
 // Camera pass, one thread per path (spp divides 64: the spp paths of a pixel sit in adjacent
 // lanes and are summed in sample order by the pixel's first lane), or one thread per pixel
 // looping over its samples (any spp). Both sum ((r0 + r1) + r2) + ... and divide by spp.
-template <int TRAV, bool PER_PATH>
+template <int TRAV, bool PER_PATH, bool SPLIT>
 __global__ void __launch_bounds__(RC_TB) k_query(rc_dscene s, rc_dstate st, const float* R, rc_camera cam, uint32_t seed,
                                                 uint32_t frame, uint64_t dormant, float* img, unsigned long long* counters) {
     extern __shared__ float sh[];
     int* shq = (int*)(sh + 9 * st.nt);
-    uint32_t* lstk = rc_lstack(sh, st.nt);
+    const auto lstk = rc_lstk_sel<SPLIT>::make(sh, st, s);
     rc_load_dyn(st, sh, shq);
     const int tid = blockIdx.x * blockDim.x + threadIdx.x;
     int ext_rays = 0, sh_rays = 0;
@@ -672,12 +672,12 @@ __global__ void __launch_bounds__(RC_TB) k_query(rc_dscene s, rc_dstate st, cons
 // instead of reading R; the radiance so far goes to w.rad. One batched inference, the
 // prediction added to the querying paths and the per-pixel sums (the staged pass's last
 // kernels) finish the image, bit for bit the staged learned-cache pass's.
-template <int TRAV>
+template <int TRAV, bool SPLIT>
 __global__ void __launch_bounds__(RC_TB) k_query_mlp(rc_dscene s, rc_dstate st, rc_camera cam, uint32_t seed, uint32_t frame,
                                                     uint64_t dormant, rc_qwork w, int npath) {
     extern __shared__ float sh[];
     int* shq = (int*)(sh + 9 * st.nt);
-    uint32_t* lstk = rc_lstack(sh, st.nt);
+    const auto lstk = rc_lstk_sel<SPLIT>::make(sh, st, s);
     rc_load_dyn(st, sh, shq);
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= npath) return;
@@ -780,7 +780,7 @@ __global__ void __launch_bounds__(RC_TB) k_q_trace(rc_dscene s, rc_dstate st, rc
     if ((blockIdx.x % (RC_QB / RC_TB)) * RC_TB >= cnt) return;   // whole block past the segment's count
     extern __shared__ float sh[];
     int* shq = (int*)(sh + 9 * st.nt);
-    uint32_t* lstk = rc_lstack(sh, st.nt);
+    const auto lstk = rc_lstk_lds(sh, st.nt, s);
     rc_load_dyn(st, sh, shq);
     if (local >= cnt) return;
     const int p = ids ? ids[seg * RC_QB + local] : seg * RC_QB + local;
@@ -999,20 +999,20 @@ hipError_t rc_launch_trace_rays(int trav, const rc_dscene* s, const rc_dstate* s
                                 float* t, int* q, hipStream_t stream) {
     if (n <= 0) return hipSuccess;
     dim3 g(rc_grid(n)), b(RC_TB);
-    size_t shm = rc_shm(*st, *s, trav);
-    if (trav == 0) hipLaunchKernelGGL(k_trace_rays<0>, g, b, shm, stream, *s, *st, O, D, n, t, q);
-    else if (trav == 1) hipLaunchKernelGGL(k_trace_rays<1>, g, b, shm, stream, *s, *st, O, D, n, t, q);
-    else hipLaunchKernelGGL(k_trace_rays<2>, g, b, shm, stream, *s, *st, O, D, n, t, q);
+    size_t shm = rc_shm(*st, *s, trav, false);
+    if (trav == 0) hipLaunchKernelGGL((k_trace_rays<0, false>), g, b, shm, stream, *s, *st, O, D, n, t, q);
+    else if (trav == 1) hipLaunchKernelGGL((k_trace_rays<1, false>), g, b, shm, stream, *s, *st, O, D, n, t, q);
+    else RC_LAUNCH_WIDE((k_trace_rays<2, true>), (k_trace_rays<2, false>), false, g, st, s, stream, *s, *st, O, D, n, t, q);
     return hipGetLastError();
 }
 
 hipError_t rc_launch_build(int trav, const rc_dscene* s, const rc_dstate* st, hipStream_t stream) {
     if (s->Np <= 0) return hipSuccess;
     dim3 g(rc_grid(s->Np)), b(RC_TB);
-    size_t shm = rc_shm(*st, *s, trav);
-    if (trav == 0) hipLaunchKernelGGL(k_build<0>, g, b, shm, stream, *s, *st);
-    else if (trav == 1) hipLaunchKernelGGL(k_build<1>, g, b, shm, stream, *s, *st);
-    else hipLaunchKernelGGL(k_build<2>, g, b, shm, stream, *s, *st);
+    size_t shm = rc_shm(*st, *s, trav, false);
+    if (trav == 0) hipLaunchKernelGGL((k_build<0, false>), g, b, shm, stream, *s, *st);
+    else if (trav == 1) hipLaunchKernelGGL((k_build<1, false>), g, b, shm, stream, *s, *st);
+    else RC_LAUNCH_WIDE((k_build<2, true>), (k_build<2, false>), false, g, st, s, stream, *s, *st);
     return hipGetLastError();
 }
 
@@ -1020,10 +1020,15 @@ hipError_t rc_launch_trace2(int trav, int deps, const rc_dscene* s, const rc_dst
                             uint64_t* lnew, hipStream_t stream) {
     if (p->Np <= 0) return hipSuccess;
     dim3 g(rc_grid(p->Np)), b(RC_TB);
-    size_t shm = rc_shm(*st, *s, trav);
-#define RC_T2(D_, T_) hipLaunchKernelGGL((k_trace2<D_, T_>), g, b, shm, stream, *s, *st, *c, *p, lnew)
-    if (deps) { if (trav == 0) RC_T2(true, 0); else if (trav == 1) RC_T2(true, 1); else RC_T2(true, 2); }
-    else { if (trav == 0) RC_T2(false, 0); else if (trav == 1) RC_T2(false, 1); else RC_T2(false, 2); }
+    size_t shm = rc_shm(*st, *s, trav, false);
+#define RC_T2(D_, T_) hipLaunchKernelGGL((k_trace2<D_, T_, false>), g, b, shm, stream, *s, *st, *c, *p, lnew)
+    if (deps) {
+        if (trav == 0) RC_T2(true, 0); else if (trav == 1) RC_T2(true, 1);
+        else RC_LAUNCH_WIDE((k_trace2<true, 2, true>), (k_trace2<true, 2, false>), false, g, st, s, stream, *s, *st, *c, *p, lnew);
+    } else {
+        if (trav == 0) RC_T2(false, 0); else if (trav == 1) RC_T2(false, 1);
+        else RC_LAUNCH_WIDE((k_trace2<false, 2, true>), (k_trace2<false, 2, false>), false, g, st, s, stream, *s, *st, *c, *p, lnew);
+    }
 #undef RC_T2
     return hipGetLastError();
 }
@@ -1092,21 +1097,30 @@ hipError_t rc_launch_error(const rc_dcache* c, int Np, int nl, const float* ref,
 hipError_t rc_launch_ref(int trav, const rc_dscene* s, const rc_dstate* st, const float* Lin, float* Lout, float* acc, float* closec,
                          uint32_t seed, int K, int accumulate, int count_close, hipStream_t stream) {
     dim3 g(rc_grid(s->Np)), b(RC_TB);
-    size_t shm = rc_shm(*st, *s, trav);
-    if (trav == 0) hipLaunchKernelGGL(k_ref<0>, g, b, shm, stream, *s, *st, Lin, Lout, acc, closec, seed, K, accumulate, count_close);
-    else if (trav == 1) hipLaunchKernelGGL(k_ref<1>, g, b, shm, stream, *s, *st, Lin, Lout, acc, closec, seed, K, accumulate, count_close);
-    else hipLaunchKernelGGL(k_ref<2>, g, b, shm, stream, *s, *st, Lin, Lout, acc, closec, seed, K, accumulate, count_close);
+    size_t shm = rc_shm(*st, *s, trav, false);
+    if (trav == 0) hipLaunchKernelGGL((k_ref<0, false>), g, b, shm, stream, *s, *st, Lin, Lout, acc, closec, seed, K, accumulate, count_close);
+    else if (trav == 1) hipLaunchKernelGGL((k_ref<1, false>), g, b, shm, stream, *s, *st, Lin, Lout, acc, closec, seed, K, accumulate, count_close);
+    else RC_LAUNCH_WIDE((k_ref<2, true>), (k_ref<2, false>), false, g, st, s, stream, *s, *st, Lin, Lout, acc, closec, seed, K, accumulate, count_close);
     return hipGetLastError();
+}
+size_t rc_ovf_words(const rc_dstate* st, const rc_dscene* s, long long nthreads) { return rc_ovf_need(*st, *s, nthreads); }
+long long rc_grid_threads(long long n) { return (long long)rc_grid(n) * RC_TB; }
+long long rc_query_threads(const rc_camera* cam) {
+    const bool per_path = cam->spp <= 64 && 64 % cam->spp == 0;
+    const long long n = (long long)cam->W * cam->H * (per_path ? cam->spp : 1);
+    return (long long)rc_grid(n) * RC_TB;
 }
 hipError_t rc_launch_query(int trav, const rc_dscene* s, const rc_dstate* st, const float* R, const rc_camera* cam, uint32_t seed,
                            uint32_t frame, uint64_t dormant, float* img, unsigned long long* counters, hipStream_t stream) {
     const bool per_path = cam->spp <= 64 && 64 % cam->spp == 0;
-    long long n = (long long)cam->W * cam->H * (per_path ? cam->spp : 1);
-    dim3 g(rc_grid(n)), b(RC_TB);
-    size_t shm = rc_shm(*st, *s, trav);
-#define RC_Q(T_, P_) hipLaunchKernelGGL((k_query<T_, P_>), g, b, shm, stream, *s, *st, R, *cam, seed, frame, dormant, img, counters)
-    if (per_path) { if (trav == 0) RC_Q(0, true); else if (trav == 1) RC_Q(1, true); else RC_Q(2, true); }
-    else { if (trav == 0) RC_Q(0, false); else if (trav == 1) RC_Q(1, false); else RC_Q(2, false); }
+    dim3 g((unsigned)(rc_query_threads(cam) / RC_TB)), b(RC_TB);
+    size_t shm = rc_shm(*st, *s, trav, false);
+#define RC_Q(T_, P_) hipLaunchKernelGGL((k_query<T_, P_, false>), g, b, shm, stream, *s, *st, R, *cam, seed, frame, dormant, img, counters)
+#define RC_QW(P_) RC_LAUNCH_WIDE((k_query<2, P_, true>), (k_query<2, P_, false>), true, g, st, s, stream, *s, *st, R, *cam, seed, frame, dormant, \
+                                 img, counters)
+    if (per_path) { if (trav == 0) RC_Q(0, true); else if (trav == 1) RC_Q(1, true); else RC_QW(true); }
+    else { if (trav == 0) RC_Q(0, false); else if (trav == 1) RC_Q(1, false); else RC_QW(false); }
+#undef RC_QW
 #undef RC_Q
     return hipGetLastError();
 }
@@ -1150,8 +1164,8 @@ hipError_t rc_launch_query_mlp(const rc_dscene* s, const rc_dstate* st, const rc
                                float* img, const rc_qwork* w, const rc_mlpdev* mlp, hipStream_t stream) {
     const int npath = cam->W * cam->H * cam->spp;
     if (npath <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_query_mlp<2>, dim3(rc_grid(npath)), dim3(RC_TB), rc_shm(*st, *s, 2), stream, *s, *st, *cam, seed, frame, dormant, *w,
-                       npath);
+    const dim3 g(rc_grid(npath));
+    RC_LAUNCH_WIDE((k_query_mlp<2, true>), (k_query_mlp<2, false>), false, g, st, s, stream, *s, *st, *cam, seed, frame, dormant, *w, npath);
     hipError_t e = rc_launch_mlp_infer(mlp, (const float4*)w->mq0, (const float4*)w->mq1, (const float4*)w->mq2, npath, w->mpred, stream);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_q_mlp_add, dim3(rc_grid(npath)), dim3(RC_TB), 0, stream, *w, npath);
@@ -1162,7 +1176,7 @@ hipError_t rc_launch_query_staged(const rc_dscene* s, const rc_dstate* st, const
                                   uint32_t frame, uint64_t dormant, float* img, unsigned long long* counters, const rc_qwork* w,
                                   const rc_mlpdev* mlp, hipStream_t stream) {
     const int npath = cam->W * cam->H * cam->spp;
-    const size_t shm = rc_shm(*st, *s, 2);
+    const size_t shm = rc_shm(*st, *s, 2, false);   // the stage kernels keep every stack level in LDS
     const int nblk = (npath + RC_QB - 1) / RC_QB;
     const dim3 tgrid(nblk * (RC_QB / RC_TB));
     hipLaunchKernelGGL(k_q_gen, dim3(rc_grid(npath)), dim3(RC_TB), 0, stream, *cam, seed, frame, npath, *w, counters);

@@ -336,12 +336,12 @@ __global__ void __launch_bounds__(RC_TB) k_mlp_zero(float* x, long long n) {
 // input (P_i, N_i, ray direction); the hit's query (point, facing normal, direction back along the
 // ray; w = 1 when the hit is a valid entry); A_i and the direct light E_i = sum of the active
 // lights' Edir (the cache's resolved radiance is R_i = A_i (E_i + indirect)).
-template <int TRAV>
+template <int TRAV, bool SPLIT>
 __global__ void __launch_bounds__(RC_TB) k_mlp_samples(rc_dscene s, rc_dstate st, uint64_t dormant, uint32_t seed, int n, float4* sP,
                                                       float4* sN, float4* sD, float4* hP, float4* hN, float4* hD, float2* AE, int* ent) {
     extern __shared__ float sh[];
     int* shq = (int*)(sh + 9 * st.nt);
-    uint32_t* lstk = rc_lstack(sh, st.nt);
+    const auto lstk = rc_lstk_sel<SPLIT>::make(sh, st, s);
     rc_load_dyn(st, sh, shq);
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
@@ -459,10 +459,11 @@ hipError_t rc_launch_mlp_samples(int trav, const rc_dscene* s, const rc_dstate* 
                                  float4* sN, float4* sD, float4* hP, float4* hN, float4* hD, float2* AE, int* ent, hipStream_t stream) {
     if (n <= 0) return hipSuccess;
     const dim3 g(rc_grid(n)), b(RC_TB);
-    const size_t shm = rc_shm(*st, *s, trav);
-    if (trav == 0) hipLaunchKernelGGL(k_mlp_samples<0>, g, b, shm, stream, *s, *st, dormant, seed, n, sP, sN, sD, hP, hN, hD, AE, ent);
-    else if (trav == 1) hipLaunchKernelGGL(k_mlp_samples<1>, g, b, shm, stream, *s, *st, dormant, seed, n, sP, sN, sD, hP, hN, hD, AE, ent);
-    else hipLaunchKernelGGL(k_mlp_samples<2>, g, b, shm, stream, *s, *st, dormant, seed, n, sP, sN, sD, hP, hN, hD, AE, ent);
+    const size_t shm = rc_shm(*st, *s, trav, false);
+    if (trav == 0) hipLaunchKernelGGL((k_mlp_samples<0, false>), g, b, shm, stream, *s, *st, dormant, seed, n, sP, sN, sD, hP, hN, hD, AE, ent);
+    else if (trav == 1) hipLaunchKernelGGL((k_mlp_samples<1, false>), g, b, shm, stream, *s, *st, dormant, seed, n, sP, sN, sD, hP, hN, hD, AE, ent);
+    else RC_LAUNCH_WIDE((k_mlp_samples<2, true>), (k_mlp_samples<2, false>), false, g, st, s, stream, *s, *st, dormant, seed, n, sP, sN, sD, hP, hN, hD,
+                        AE, ent);
     return hipGetLastError();
 }
 

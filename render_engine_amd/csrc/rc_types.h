@@ -36,6 +36,7 @@ typedef struct rc_wnode {
 } rc_wnode;
 #define RC_WSTACK 48          // most LDS stack levels per lane (48 KB per 256-lane block)
 #define RC_WQ 8               // queued leaf ranges per lane of the wide closest-hit traversal (LDS levels)
+#define RC_LCAP 16            // LDS levels of the camera pass's lane stack (deeper levels: global overflow)
 
 // Per-state device view of the scene (cuda/poc_gpu.cu:222-237).
 typedef struct rc_dstate {
@@ -54,7 +55,11 @@ typedef struct rc_dstate {
     const float* Pw;         // [NL Np] converged per-light reference (optional)
     const float* inten;      // [NL] light intensities of the state
     uint32_t* VOL;           // [NL][fg^3 / 32] per-light fine voxel volumes of the lit shadow segments
+    uint32_t* ovf;           // [ovf_words] global levels of the wide traversal's lane stacks (levels >= ocap),
+    unsigned long long ovf_words;   //   one column per thread of the launch; set per launch by its owner
     int32_t nt;              // dynamic triangles in this state
+    int32_t ocap;            // LDS levels of the split lane stacks (0: all wstack levels in LDS, ovf unused)
+    int32_t oforce;          // 1: wide-traversal launches take the split stacks; 0: only where that raises occupancy
     int32_t pad0;
 } rc_dstate;
 

@@ -201,6 +201,30 @@ def test_query_bit_exact(torch_gpu, street):
         assert tuple(rays) == (er[0], er[1])
 
 
+def test_query_stack_levels(torch_gpu, street):
+    """The camera pass's traversal stacks keep their first levels in LDS and the deeper ones in a
+    global buffer: the image and ray counts do not depend on the split (1 and 2 levels put nearly
+    every push of every ray in the global buffer; 64 keeps every level in LDS)"""
+    torch = torch_gpu
+    scene, _ = street
+    cache = rea.Cache(scene, rea.methods.plain())
+    cache.begin(0, 9)
+    for _ in range(2):
+        cache.frame(0)
+    cam = scene.camera()
+    img = torch.zeros(cam.W * cam.H, dtype=torch.float32, device="cuda")
+    out = {}
+    for levels in (64, 1, 2, 5, 16):
+        cache.set_stack_levels(levels)
+        rays = cache.render(img, cam, seed=6, count_rays=True)
+        out[levels] = (img.cpu().numpy().view(np.uint32).copy(), tuple(rays))
+    for levels, (im, rays) in out.items():
+        assert np.array_equal(im, out[64][0]), f"levels={levels}: {np.sum(im != out[64][0])} pixels differ"
+        assert rays == out[64][1], levels
+    with pytest.raises(rea.RcError):
+        cache.set_stack_levels(0)
+
+
 @pytest.mark.parametrize("which", ["room", "street"])
 def test_query_staged_equals_fused(torch_gpu, which, request):
     """The staged camera pass (per-segment compacted ray lists, one trace kernel per stage)

@@ -46,6 +46,9 @@ def test_argument_errors():
         s.array("EDIR", 0)   # device-produced array before upload
     with pytest.raises(rea.RcError):
         s.array("P", 7)      # state out of range
+    for levels in (0, 65):
+        with pytest.raises(rea.RcError, match="stack levels"):
+            s.set_stack_levels(levels)
 
 
 @pytest.mark.parametrize("kind", ["room", "street"])
