@@ -27,7 +27,8 @@ extern "C" {
 typedef struct orc_node { float lo[3]; int32_t lf; float hi[3]; int32_t cnt; } orc_node;
 typedef struct orc_wnode {
     float lox[4], loy[4], loz[4], hix[4], hiy[4], hiz[4];
-    int32_t child[4], cend[4];   /* leaf: triangles child .. cend - 1; inner / empty: cend == child */
+    int32_t child[4], cend[4];   /* leaf: child = 1 << 31 | first << 7 | (count <= 127 ? count : 0), cend = first + count;
+                                    inner: child = node, cend 0; empty: both 0 */
 } orc_wnode;
 
 /* static scene + one state (inputs); Edir/SV/LM are the state's build outputs */

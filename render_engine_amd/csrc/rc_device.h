@@ -224,15 +224,15 @@ __device__ __forceinline__ float rc_tlb(uint32_t code) {
 }
 
 // Per-ray constants of the wide traversal: origin, direction, reciprocal direction and the
-// byte offsets, inside a 128-byte node, of the rows that hold each axis's near and far planes
-// (near = lo when the direction is positive, hi when negative; node rows of 16 B: 0-2 lo
-// x/y/z, 3-5 hi x/y/z, 6 children, 7 leaf counts).
-struct rc_wray { f3 o, d, inv; uint32_t nx, ny, nz, fx, fy, fz; };
+// byte offsets, inside a 128-byte node, of the rows that hold each axis's near planes (near = lo
+// when the direction is positive, hi when negative; node rows of 16 B: 0-2 lo x/y/z, 3-5 hi
+// x/y/z, 6 children and leaf entries, 7 leaf ends). The far row of axis a is at (48 + 32 a) - near offset:
+// node - near + constant, the constant going into the load's immediate offset (3 fewer VGPRs).
+struct rc_wray { f3 o, d, inv; uint32_t nx, ny, nz; };
 __device__ __forceinline__ rc_wray rc_wray_make(f3 o, f3 d) {
     rc_wray r;
     r.o = o; r.d = d; r.inv = rc_inv_dir(d);
     r.nx = r.inv.x < 0.f ? 48u : 0u; r.ny = r.inv.y < 0.f ? 64u : 16u; r.nz = r.inv.z < 0.f ? 80u : 32u;
-    r.fx = 48u - r.nx; r.fy = 80u - r.ny; r.fz = 112u - r.nz;
     return r;
 }
 
@@ -246,26 +246,24 @@ __device__ __forceinline__ int4 rc_wrow_i(const rc_dscene& s, uint32_t off) {
 }
 
 // Entry distances of a node's four child boxes against [0, tmax] (1e30: missed, empty slot or
-// leaf), branch-free, and the triangle ranges [a, b) of the entered leaf children (a = b
-// otherwise). With lo <= hi, (lo - o) * inv and (hi - o) * inv are ordered by the sign of inv,
-// so the near-plane value is exactly the per-axis minimum of rc_box_enter and the result
-// equals rc_box_enter's bit for bit. Empty slots hold lo = +inf, hi = -inf: their near value
-// is +inf and their far value -inf whatever the direction, so they always miss. A leaf child
-// stores its first triangle and end (end > first); inner and empty slots store end = child.
-__device__ __forceinline__ void rc_child_enter(const rc_wray& r, const rc_dscene& s, uint32_t node, float tmax, int4 ch, int4 ce,
-                                               float& t0, float& t1, float& t2, float& t3, int& a0, int& b0, int& a1, int& b1,
-                                               int& a2, int& b2, int& a3, int& b3) {
+// leaf), branch-free, and the leaf entries e of the entered leaf children (1 << 31 | first << 7 |
+// count; 0 otherwise). With lo <= hi, (lo - o) * inv and (hi - o) * inv are ordered by the sign
+// of inv, so the near-plane value is exactly the per-axis minimum of rc_box_enter and the result
+// equals rc_box_enter's bit for bit. Empty slots hold lo = +inf, hi = -inf: their near value is
+// +inf and their far value -inf whatever the direction, so they always miss.
+__device__ __forceinline__ void rc_child_enter(const rc_wray& r, const rc_dscene& s, uint32_t node, float tmax, int4 ch,
+                                               float& t0, float& t1, float& t2, float& t3, int& e0, int& e1, int& e2, int& e3) {
     const float4 nx = rc_wrow(s, node + r.nx), ny = rc_wrow(s, node + r.ny), nz = rc_wrow(s, node + r.nz);
-    const float4 fx = rc_wrow(s, node + r.fx), fy = rc_wrow(s, node + r.fy), fz = rc_wrow(s, node + r.fz);
-#define RC_SLAB(tc, c, a, b)                                                                                    \
+    const float4 fx = rc_wrow(s, (node - r.nx) + 48u), fy = rc_wrow(s, (node - r.ny) + 80u), fz = rc_wrow(s, (node - r.nz) + 112u);
+#define RC_SLAB(tc, c, e)                                                                                       \
     {                                                                                                           \
         const float tn = fmaxf(fmaxf((nx.c - r.o.x) * r.inv.x, (ny.c - r.o.y) * r.inv.y), fmaxf((nz.c - r.o.z) * r.inv.z, 0.f)); \
         const float tf = fminf(fminf((fx.c - r.o.x) * r.inv.x, (fy.c - r.o.y) * r.inv.y), fminf((fz.c - r.o.z) * r.inv.z, tmax)); \
-        const bool in_ = tn <= tf, leaf_ = ce.c > ch.c;                                                          \
+        const bool in_ = tn <= tf, leaf_ = ch.c < 0;                                                            \
         tc = in_ && !leaf_ ? tn : 1e30f;                                                                        \
-        a = ch.c; b = in_ && leaf_ ? ce.c : ch.c;                                                               \
+        e = in_ && leaf_ ? ch.c : 0;                                                                            \
     }
-    RC_SLAB(t0, x, a0, b0) RC_SLAB(t1, y, a1, b1) RC_SLAB(t2, z, a2, b2) RC_SLAB(t3, w, a3, b3)
+    RC_SLAB(t0, x, e0) RC_SLAB(t1, y, e1) RC_SLAB(t2, z, e2) RC_SLAB(t3, w, e3)
 #undef RC_SLAB
 }
 
@@ -342,21 +340,23 @@ __device__ __forceinline__ void rc_trace_wide_closest(f3 o, f3 d, const rc_dscen
         }
         if (!trav) continue;
         const uint32_t node = (uint32_t)cur << 7;
-        const int4 ch = rc_wrow_i(s, node + 96u), ce = rc_wrow_i(s, node + 112u);
+        const int4 ch = rc_wrow_i(s, node + 96u);
         float t0, t1, t2, t3;
-        int a0, b0, a1, b1, a2, b2, a3, b3;
-        rc_child_enter(r, s, node, tmin, ch, ce, t0, t1, t2, t3, a0, b0, a1, b1, a2, b2, a3, b3);
+        int e0, e1, e2, e3;
+        rc_child_enter(r, s, node, tmin, ch, t0, t1, t2, t3, e0, e1, e2, e3);
         int i0 = ch.x, i1 = ch.y, i2 = ch.z, i3 = ch.w;
-#define RC_QUEUE(a, b)                                                                          \
-        if (b > a) {                                                                            \
-            if (b - a <= 127) { que[nq * 256] = ((uint32_t)a << 7) | (uint32_t)(b - a); ++nq; } \
-            else                                                                                \
-                for (int j = a; j < b; ++j) {                                                   \
+#define RC_QUEUE(e, cw)                                                                         \
+        if (e != 0) {                                                                           \
+            if (e & 127) { que[nq * 256] = (uint32_t)e & 0x7fffffffu; ++nq; }                   \
+            else {   /* more than 127 triangles: their end in row 7 */                          \
+                const int a_ = (int)(((uint32_t)e >> 7) & 0xffffffu), b_ = rc_wrow_i(s, node + 112u).cw; \
+                for (int j = a_; j < b_; ++j) {                                                 \
                     const float t = rc_tri_static4(r.o, r.d, (const float4*)s.ST4 + 3 * (int64_t)j); \
                     if (rc_closer(t, j, tmin, q)) { tmin = t; q = -2 - j; }                     \
                 }                                                                               \
+            }                                                                                   \
         }
-        RC_QUEUE(a0, b0) RC_QUEUE(a1, b1) RC_QUEUE(a2, b2) RC_QUEUE(a3, b3)
+        RC_QUEUE(e0, x) RC_QUEUE(e1, y) RC_QUEUE(e2, z) RC_QUEUE(e3, w)
 #undef RC_QUEUE
         // sorting network on (t, node): ascending entry distance, misses (1e30) last
         rc_cswap(t0, i0, t1, i1); rc_cswap(t2, i2, t3, i3);
@@ -412,21 +412,23 @@ __device__ __forceinline__ bool rc_occluded_wide(f3 o, f3 d, const rc_dscene& s,
         }
         if (!trav) continue;
         const uint32_t node = (uint32_t)cur << 7;
-        const int4 ch = rc_wrow_i(s, node + 96u), ce = rc_wrow_i(s, node + 112u);
+        const int4 ch = rc_wrow_i(s, node + 96u);
         float t0, t1, t2, t3;
-        int a0, b0, a1, b1, a2, b2, a3, b3;
-        rc_child_enter(r, s, node, tlim, ch, ce, t0, t1, t2, t3, a0, b0, a1, b1, a2, b2, a3, b3);
+        int e0, e1, e2, e3;
+        rc_child_enter(r, s, node, tlim, ch, t0, t1, t2, t3, e0, e1, e2, e3);
         int i0 = ch.x, i1 = ch.y, i2 = ch.z, i3 = ch.w;
-#define RC_QUEUE(a, b)                                                                          \
-        if (b > a) {                                                                            \
-            if (b - a <= 127) { que[nq * 256] = ((uint32_t)a << 7) | (uint32_t)(b - a); ++nq; } \
-            else                                                                                \
-                for (int j = a; j < b && !blocked; ++j) {                                       \
+#define RC_QUEUE(e, cw)                                                                         \
+        if (e != 0) {                                                                           \
+            if (e & 127) { que[nq * 256] = (uint32_t)e & 0x7fffffffu; ++nq; }                   \
+            else {   /* more than 127 triangles: their end in row 7 */                          \
+                const int a_ = (int)(((uint32_t)e >> 7) & 0xffffffu), b_ = rc_wrow_i(s, node + 112u).cw; \
+                for (int j = a_; j < b_ && !blocked; ++j) {                                     \
                     const float t = rc_tri_static4(r.o, r.d, (const float4*)s.ST4 + 3 * (int64_t)j); \
                     if (t > 1e-4f && t < tlim) blocked = true;                                  \
                 }                                                                               \
+            }                                                                                   \
         }
-        RC_QUEUE(a0, b0) RC_QUEUE(a1, b1) RC_QUEUE(a2, b2) RC_QUEUE(a3, b3)
+        RC_QUEUE(e0, x) RC_QUEUE(e1, y) RC_QUEUE(e2, z) RC_QUEUE(e3, w)
 #undef RC_QUEUE
         if (blocked) { cur = -1; nq = 0; continue; }
         rc_cswap(t0, i0, t1, i1); rc_cswap(t2, i2, t3, i3);

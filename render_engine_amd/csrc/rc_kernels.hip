@@ -570,17 +570,19 @@ __device__ __forceinline__ float rc_nee(const rc_dscene& s, const rc_dstate& st,
     if (!(c > 0.f)) return 0.f;
     f3 O = H + n * 1e-3f;
     ++shadow_rays;
+    // the light's contribution if unblocked, computed before the shadow ray (fewer values live across it)
+    float val;
+    if (s.ltype[l] == 1) val = (float)nl * (I * c);
+    else { const float rr = fmaxf(r, 0.05f); val = (float)nl * (I * c / (rr * rr)); }
+    const float tlim = r - 1e-3f;
     if (TRAV == 2) {   // visibility only: any hit closer than r - 1e-3 blocks the light
-        const float tlim = r - 1e-3f;
         if (rc_occluded_lds(O, D, sh, st.nt, tlim) || (s.nst > 0 && rc_occluded_wide(O, D, s, tlim, lstk))) return 0.f;
     } else {
         float t; int q;
         rc_trace_full<TRAV>(O, D, sh, shq, st.nt, s, t, q, lstk);
-        if (!(t >= r - 1e-3f)) return 0.f;
+        if (!(t >= tlim)) return 0.f;
     }
-    if (s.ltype[l] == 1) return (float)nl * (I * c);
-    float rr = fmaxf(r, 0.05f);
-    return (float)nl * (I * c / (rr * rr));
+    return val;
 }
 
 // One camera path: primary hit x0, one-light NEE at every explicit vertex, cosine bounces; at

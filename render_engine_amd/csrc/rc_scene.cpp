@@ -177,11 +177,13 @@ void build_wide(const std::vector<rc_bvh_node>& bn, std::vector<rc_wnode>& wn, i
             const rc_bvh_node& c = bn[kids[k]];
             w.lox[k] = c.lo[0]; w.loy[k] = c.lo[1]; w.loz[k] = c.lo[2];
             w.hix[k] = c.hi[0]; w.hiy[k] = c.hi[1]; w.hiz[k] = c.hi[2];
-            if (c.cnt > 0) { w.child[k] = c.lf; w.cend[k] = c.lf + c.cnt; }
-            else {
+            if (c.cnt > 0) {   // leaf: the traversal's queue entry with the top bit set; a count above 127 reads as 0
+                w.child[k] = (int32_t)(0x80000000u | (uint32_t)c.lf << 7 | (uint32_t)(c.cnt <= 127 ? c.cnt : 0));
+                w.cend[k] = c.lf + c.cnt;
+            } else {
                 int slot = (int)wn.size();
                 wn.push_back(rc_wnode{});
-                w.child[k] = slot; w.cend[k] = slot;
+                w.child[k] = slot; w.cend[k] = 0;
                 sub.push_back({kids[k], slot, jb.depth + 1});
             }
         }
@@ -332,7 +334,7 @@ std::string prepare(const SceneDesc& d, Scene& S) {
     build_wide(S.BN, S.WN, S.wide_depth);
     if (3 * S.wide_depth + 1 > RC_WSTACK) return "wide BVH too deep for the LDS traversal stack";
     if (S.WN.size() >= (1u << 24)) return "wide BVH has more than 2^24 nodes";
-    if (M.nt() >= (1 << 25)) return "more than 2^25 static triangles (the traversal's leaf queue holds 25-bit indices)";
+    if (M.nt() >= (1 << 24)) return "more than 2^24 static triangles (the wide nodes' leaf entries hold 24-bit indices)";
     S.nst = M.nt();
     S.ST.resize(9 * (size_t)S.nst); S.STN.resize(4 * (size_t)S.nst); S.Talb.resize(S.nst);
     for (int j = 0; j < S.nst; ++j) {
@@ -591,7 +593,7 @@ std::string load_scene_file(const char* path, Scene& S, FileRefs& R) {
         build_wide(S.BN, S.WN, S.wide_depth);
         if (3 * S.wide_depth + 1 > RC_WSTACK) return "wide BVH too deep for the LDS traversal stack";
         if (S.WN.size() >= (1u << 24)) return "wide BVH has more than 2^24 nodes";
-        if (S.nst >= (1 << 25)) return "more than 2^25 static triangles (the traversal's leaf queue holds 25-bit indices)";
+        if (S.nst >= (1 << 24)) return "more than 2^24 static triangles (the wide nodes' leaf entries hold 24-bit indices)";
         S.order.resize(S.nst); std::iota(S.order.begin(), S.order.end(), 0);
         S.Talb.assign(S.nst, 0.5f);
     }

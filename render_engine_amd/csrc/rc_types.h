@@ -25,9 +25,11 @@ typedef struct rc_bvh_node {
 } rc_bvh_node;
 
 // 4-wide BVH node (128 B = two 64-byte lines): child boxes as fp32 SoA so the four slab
-// tests of a node read whole float4s. cend[c] > child[c]: leaf holding triangles child[c] ..
-// cend[c] - 1 (same triangle order and indices as the binary BVH); cend[c] == child[c]: inner
-// node child[c], or an empty slot (child 0, box lo = +inf, hi = -inf: never entered).
+// tests of a node read whole float4s. child[c] < 0: leaf, child[c] = 1 << 31 | first << 7 |
+// count (triangles first .. first + count - 1, same order and indices as the binary BVH; count
+// 0 when above 127, the end then in cend[c]; cend[c] = first + count for every leaf); child[c] >
+// 0: inner node child[c] (cend 0); child[c] == 0: empty slot (box lo = +inf, hi = -inf: never
+// entered). The traversal reads rows 0-6 only (row 7 for leaves of more than 127 triangles).
 typedef struct rc_wnode {
     float lox[4], loy[4], loz[4], hix[4];   // 64 B
     float hiy[4], hiz[4];                   // 32 B

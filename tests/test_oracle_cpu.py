@@ -185,24 +185,30 @@ def test_bvh_structure(street):
             for c in (lf[k], lf[k] + 1):
                 assert np.all(lo[c] >= lo[k]) and np.all(hi[c] <= hi[k])
     assert np.all(seen == 1)
-    # 4-wide nodes: every triangle in exactly one leaf range, inner children reached once,
-    # empty slots carry the never-entered box (lo = +inf, hi = -inf)
+    # 4-wide nodes: every triangle in exactly one leaf range (leaf child = 1 << 31 | first << 7 |
+    # count, count 0 above 127, end in the last row), inner children reached once, empty slots
+    # (child 0) carry the never-entered box (lo = +inf, hi = -inf)
     wn = scene.array("WN").reshape(-1, 32)
     wf = wn.view(np.float32)
-    ch, ce = wn[:, 24:28].view(np.int32), wn[:, 28:32].view(np.int32)
+    ch, ce = wn[:, 24:28].view(np.uint32), wn[:, 28:32].view(np.int32)
     wlo, whi = wf[:, 0:12].reshape(-1, 3, 4), wf[:, 12:24].reshape(-1, 3, 4)
     wseen = np.zeros(nst, np.int32)
     reached = np.zeros(len(wn), np.int32)
     for k in range(len(wn)):
         for c in range(4):
             empty = np.all(np.isposinf(wlo[k, :, c])) and np.all(np.isneginf(whi[k, :, c]))
-            if ce[k, c] > ch[k, c]:
-                wseen[ch[k, c]:ce[k, c]] += 1
-                v0 = ST[ch[k, c]:ce[k, c], 0:3]
+            e = int(ch[k, c])
+            if e >> 31:
+                first, count = (e >> 7) & 0xFFFFFF, e & 127
+                assert ce[k, c] > first and (count == ce[k, c] - first if count else ce[k, c] - first > 127)
+                wseen[first:ce[k, c]] += 1
+                v0 = ST[first:ce[k, c], 0:3]
                 assert np.all(v0 >= wlo[k, :, c] - 1e-5) and np.all(v0 <= whi[k, :, c] + 1e-5)
-            elif not empty:
-                assert 0 < ch[k, c] < len(wn) and ce[k, c] == ch[k, c]
-                reached[ch[k, c]] += 1
+            elif e == 0:
+                assert empty and ce[k, c] == 0
+            else:
+                assert not empty and e < len(wn) and ce[k, c] == 0
+                reached[e] += 1
     assert np.all(wseen == 1)
     assert reached[0] == 0 and np.all(reached[1:] == 1)
 
