@@ -554,12 +554,20 @@ __global__ void __launch_bounds__(RC_TB) k_ref(rc_dscene s, rc_dstate st, const 
 
 // ------------------------------------------------------------------------------ camera query
 
-// One-light next-event estimate at H (normal n): light chosen uniformly, weight nl.
+// One-light next-event estimate at H (normal n, shadow-ray origin O = H + n 1e-3): light chosen
+// uniformly, weight nl. When the path goes on (more), the next bounce's cosine direction is
+// drawn right after the light into dn: the order of the path's draws (light, then bounce) is
+// unchanged, and H and n need not stay live across the shadow ray.
 template <int TRAV, class LS>
 __device__ __forceinline__ float rc_nee(const rc_dscene& s, const rc_dstate& st, const float* sh, const int* shq,
-                                        f3 H, f3 n, uint32_t& rs, uint64_t dormant, int& shadow_rays, const LS& lstk) {
+                                        f3 H, f3 n, f3 O, uint32_t& rs, uint64_t dormant, int& shadow_rays, const LS& lstk,
+                                        bool more, f3& dn) {
     const int nl = s.nl;
     float u = rc_rnd(rs);
+    if (more) {
+        const float u1 = rc_rnd(rs), u2 = rc_rnd(rs);
+        dn = rc_cosine_dir(n, u1, u2);
+    }
     int l = min((int)(u * (float)nl), nl - 1);
     float I = st.inten[l];
     if (!(I > 0.f) || rc_is_dormant(dormant, l)) return 0.f;
@@ -568,7 +576,6 @@ __device__ __forceinline__ float rc_nee(const rc_dscene& s, const rc_dstate& st,
     else { f3 dv = ld3(s.L, l) - H; r = sqrtf(dot3(dv, dv)); D = dv * (1.f / r); }
     float c = dot3(n, D);
     if (!(c > 0.f)) return 0.f;
-    f3 O = H + n * 1e-3f;
     ++shadow_rays;
     // the light's contribution if unblocked, computed before the shadow ray (fewer values live across it)
     float val;
@@ -611,12 +618,14 @@ __device__ __forceinline__ float rc_path(const rc_dscene& s, const rc_dstate& st
         if (rc_resolve_hit_n(s, st, H, d, q, e, n)) break;
         if (R && b == cam.cache_bounce) { rad += thr * R[e]; break; }
         float A = s.Alb[e];
-        rad += thr * (A * rc_nee<TRAV>(s, st, sh, shq, H, n, rs, dormant, sh_rays, lstk));
-        if (b >= cam.bounces) break;
+        const bool more = b < cam.bounces;
+        const f3 on = H + n * 1e-3f;   // the shadow ray's and the next bounce's origin
+        f3 dn = d;
+        rad += thr * (A * rc_nee<TRAV>(s, st, sh, shq, H, n, on, rs, dormant, sh_rays, lstk, more, dn));
+        if (!more) break;
         thr = thr * A;
-        float u1 = rc_rnd(rs), u2 = rc_rnd(rs);
-        d = rc_cosine_dir(n, u1, u2);
-        o = H + n * 1e-3f;
+        d = dn;
+        o = on;
     }
     return rad;
 }
@@ -707,12 +716,14 @@ __global__ void __launch_bounds__(RC_TB) k_query_mlp(rc_dscene s, rc_dstate st, 
             break;
         }
         const float A = s.Alb[e];
-        rad += thr * (A * rc_nee<TRAV>(s, st, sh, shq, H, n, rs, dormant, sh_rays, lstk));
+        const f3 on = H + n * 1e-3f;
+        f3 dn = d;
+        rad += thr * (A * rc_nee<TRAV>(s, st, sh, shq, H, n, on, rs, dormant, sh_rays, lstk, false, dn));
         if (b >= cam.bounces) break;
         thr = thr * A;
         const float u1 = rc_rnd(rs), u2 = rc_rnd(rs);
         d = rc_cosine_dir(n, u1, u2);
-        o = H + n * 1e-3f;
+        o = on;
     }
     w.rad[p] = rad; w.thr[p] = thr;
     ((float4*)w.mq0)[p] = q0;
