@@ -267,6 +267,9 @@ __device__ __forceinline__ void rc_child_enter(const rc_wray& r, const rc_dscene
 #undef RC_SLAB
 }
 
+// four floats at a 4-byte aligned address, read by one 16-byte load
+typedef float rc_f4u __attribute__((ext_vector_type(4), aligned(4)));
+
 // Lane stack and leaf queue of the wide traversal: LDS levels one column per lane (256 entries
 // apart), the RC_WQ queue levels after the first cap levels. rc_lstk_l keeps every stack level in
 // LDS (cap = wstack); rc_lstk_s keeps the first cap levels in LDS and the deeper ones in global

@@ -116,12 +116,21 @@ __global__ void __launch_bounds__(RC_TB) k_trace2(rc_dscene s, rc_dstate st, rc_
             float A = s.Alb[pi];
             const float* ed = st.Edir + (int64_t)nl * pi;
             const float* lc = c.Lc + (int64_t)nl * pi;
-            for (int l = 0; l < nl; ++l) {
-                float lv = rc_is_dormant(p.dormant, l) ? 0.f : lc[l];
-                float v = A * (ed[l] + lv);
-                if (r == 0) mcol[l * ms] = v; else mcol[l * ms] += v;
-                val += v;
+            // the hit entry's rows read 16 B at a time (rows are only 4 B aligned: nl floats each)
+#define RC_ACC1(l_, e_, c_)                                                  \
+            {                                                                \
+                const float lv = rc_is_dormant(p.dormant, l_) ? 0.f : (c_);  \
+                const float v = A * ((e_) + lv);                             \
+                if (r == 0) mcol[(l_) * ms] = v; else mcol[(l_) * ms] += v;  \
+                val += v;                                                    \
             }
+            int l = 0;
+            for (; l + 4 <= nl; l += 4) {
+                const rc_f4u e4 = *(const rc_f4u*)(ed + l), c4 = *(const rc_f4u*)(lc + l);
+                RC_ACC1(l, e4.x, c4.x) RC_ACC1(l + 1, e4.y, c4.y) RC_ACC1(l + 2, e4.z, c4.z) RC_ACC1(l + 3, e4.w, c4.w)
+            }
+            for (; l < nl; ++l) RC_ACC1(l, ed[l], lc[l])
+#undef RC_ACC1
         } else if (r == 0) {
             for (int l = 0; l < nl; ++l) mcol[l * ms] = 0.f;
         }
@@ -542,7 +551,13 @@ __global__ void __launch_bounds__(RC_TB) k_ref(rc_dscene s, rc_dstate st, const 
         float A = s.Alb[pi];
         const float* ed = st.Edir + (int64_t)nl * pi;
         const float* li = Lin + (int64_t)nl * pi;
-        for (int l = 0; l < nl; ++l) out[l] += A * (ed[l] + li[l]);
+        int l = 0;
+        for (; l + 4 <= nl; l += 4) {   // 16-byte reads of the 4-byte aligned rows
+            const rc_f4u e4 = *(const rc_f4u*)(ed + l), i4 = *(const rc_f4u*)(li + l);
+            out[l] += A * (e4.x + i4.x); out[l + 1] += A * (e4.y + i4.y);
+            out[l + 2] += A * (e4.z + i4.z); out[l + 3] += A * (e4.w + i4.w);
+        }
+        for (; l < nl; ++l) out[l] += A * (ed[l] + li[l]);
     }
     for (int l = 0; l < nl; ++l) {
         float v = out[l] / (float)K;
