@@ -58,10 +58,12 @@ __device__ __forceinline__ f3 rc_mlp_norm(const rc_mlpdev& m, float4 p) {
 __device__ void rc_mlp_encode(const rc_mlpdev& m, float4 p, float4 n, float4 v, __bf16* row) {
     const f3 x = rc_mlp_norm(m, p);
     const size_t T = (size_t)1 << m.log2T;
+#pragma unroll 4   // 32 corner gathers in flight per lane (each level's sum keeps its own order)
     for (int lv = 0; lv < RC_MLP_LEVELS; ++lv) {
         uint32_t ix, iy, iz; f3 fr;
         rc_mlp_cell(m, lv, x, ix, iy, iz, fr);
         float f0 = 0.f, f1 = 0.f;
+#pragma unroll
         for (int c = 0; c < 8; ++c) {
             const uint32_t cx = ix + (c & 1), cy = iy + ((c >> 1) & 1), cz = iz + (c >> 2);
             const float w = ((c & 1) ? fr.x : 1.f - fr.x) * (((c >> 1) & 1) ? fr.y : 1.f - fr.y) * ((c >> 2) ? fr.z : 1.f - fr.z);
