@@ -645,9 +645,24 @@ __device__ __forceinline__ float rc_path(const rc_dscene& s, const rc_dstate& st
     return rad;
 }
 
+// Pixel tile of one wave in the per-path camera pass: the 64 / spp pixels of a wave (spp divides
+// 64, so a power of two) form a tw x th tile (4 x 4 at 4 spp, 8 x 8 at 1 spp), tiles in row-major
+// order over the image, so a wave's primary rays and the points its bounces leave from are
+// compact on screen. Paths keep their row-major ids (pixel * spp + sample) for seeds and output.
+struct rc_qtile { int tw, th, tx, ty; };
+__host__ __device__ inline rc_qtile rc_query_tile(int W, int H, int spp) {
+    const int n = 64 / spp;
+    int tw = 1;
+    while (tw * tw < n) tw <<= 1;
+    rc_qtile t;
+    t.tw = tw; t.th = n / tw; t.tx = (W + tw - 1) / tw; t.ty = (H + t.th - 1) / t.th;
+    return t;
+}
+
 // Camera pass, one thread per path (spp divides 64: the spp paths of a pixel sit in adjacent
-// lanes and are summed in sample order by the pixel's first lane), or one thread per pixel
-// looping over its samples (any spp). Both sum ((r0 + r1) + r2) + ... and divide by spp.
+// lanes and are summed in sample order by the pixel's first lane; waves take pixel tiles), or
+// one thread per pixel looping over its samples (any spp). Both sum ((r0 + r1) + r2) + ... and
+// divide by spp.
 template <int TRAV, bool PER_PATH, bool SPLIT>
 __global__ void __launch_bounds__(RC_TB) k_query(rc_dscene s, rc_dstate st, const float* R, rc_camera cam, uint32_t seed,
                                                 uint32_t frame, uint64_t dormant, float* img, unsigned long long* counters) {
@@ -660,20 +675,22 @@ __global__ void __launch_bounds__(RC_TB) k_query(rc_dscene s, rc_dstate st, cons
     const int npix = cam.W * cam.H;
     if (PER_PATH) {
         const int spp = cam.spp;
-        const bool live = tid < npix * spp;
+        const int lane = threadIdx.x & 63, w = tid >> 6, lp = lane / spp, first = lane - lane % spp;
+        const rc_qtile qt = rc_query_tile(cam.W, cam.H, spp);
+        const int px = (w % qt.tx) * qt.tw + lp % qt.tw, py = (w / qt.tx) * qt.th + lp / qt.tw;
+        const bool live = px < cam.W && py < cam.H;
+        const int pix = py * cam.W + px;
         float rad = 0.f;
         if (live) {
-            int pix = tid / spp;
-            uint32_t rs = rc_seed(seed, frame, (uint32_t)tid);
-            rad = rc_path<TRAV>(s, st, sh, shq, R, cam, pix % cam.W, pix / cam.W, rs, dormant, ext_rays, sh_rays, lstk);
+            uint32_t rs = rc_seed(seed, frame, (uint32_t)(pix * spp + lane - first));
+            rad = rc_path<TRAV>(s, st, sh, shq, R, cam, px, py, rs, dormant, ext_rays, sh_rays, lstk);
         }
-        const int lane = threadIdx.x & 63, first = lane - lane % spp;
         float sum = rad;
         for (int k = 1; k < spp; ++k) {
             float rk = __shfl(rad, first + k, 64);
             if (lane == first) sum += rk;
         }
-        if (live && lane == first) img[tid / spp] = sum / (float)spp;
+        if (live && lane == first) img[pix] = sum / (float)spp;
     } else if (tid < npix) {
         float sum = 0.f;
         for (int sp = 0; sp < cam.spp; ++sp) {
@@ -1135,8 +1152,11 @@ size_t rc_ovf_words(const rc_dstate* st, const rc_dscene* s, long long nthreads)
 long long rc_grid_threads(long long n) { return (long long)rc_grid(n) * RC_TB; }
 long long rc_query_threads(const rc_camera* cam) {
     const bool per_path = cam->spp <= 64 && 64 % cam->spp == 0;
-    const long long n = (long long)cam->W * cam->H * (per_path ? cam->spp : 1);
-    return (long long)rc_grid(n) * RC_TB;
+    if (per_path) {   // one wave per pixel tile
+        const rc_qtile t = rc_query_tile(cam->W, cam->H, cam->spp);
+        return (long long)rc_grid((long long)t.tx * t.ty * 64) * RC_TB;
+    }
+    return (long long)rc_grid((long long)cam->W * cam->H) * RC_TB;
 }
 hipError_t rc_launch_query(int trav, const rc_dscene* s, const rc_dstate* st, const float* R, const rc_camera* cam, uint32_t seed,
                            uint32_t frame, uint64_t dormant, float* img, unsigned long long* counters, hipStream_t stream) {
