@@ -1406,7 +1406,7 @@ extern "C" rc_status rc_mlp_render(rc_mlp* m, rc_cache* c, const rc_camera* cam,
     HIPCK(wait_query(c));   // the grid cache's camera pass may still use the staged workspace
     HIPCK(qwork_reserve(c, (long long)cam->W * cam->H * cam->spp));
     rc_dstate dk = S->dst[si];
-    if (c->render_mode != RC_RENDER_STAGED) HIPCK(scene_dst(S, si, grid_threads((long long)cam->W * cam->H * cam->spp), &dk));
+    if (c->render_mode != RC_RENDER_STAGED) HIPCK(scene_dst(S, si, rc_query_mlp_threads(cam), &dk));
     if (ms_out) HIPCK(hipEventRecord(c->ev[0], st));
     if (c->render_mode == RC_RENDER_STAGED)
         HIPCK(rc_launch_query_staged(&S->ds, &S->dst[si], nullptr, cam, seed, (uint32_t)c->f, c->dormant, d_img, nullptr, &c->qw, &m->d, st));

@@ -29,6 +29,7 @@ hipError_t rc_launch_ref(int trav, const rc_dscene* s, const rc_dstate* st, cons
 hipError_t rc_launch_query(int trav, const rc_dscene* s, const rc_dstate* st, const float* R, const rc_camera* cam, uint32_t seed,
                            uint32_t frame, uint64_t dormant, float* img, unsigned long long* counters, hipStream_t stream);
 long long rc_query_threads(const rc_camera* cam);
+long long rc_query_mlp_threads(const rc_camera* cam);   // threads of the fused learned-cache pass's grid
 // global lane-stack words a launch of nthreads threads needs with st's split; threads of an n-item grid
 size_t rc_ovf_words(const rc_dstate* st, const rc_dscene* s, long long nthreads);
 long long rc_grid_threads(long long n);

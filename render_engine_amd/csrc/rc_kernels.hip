@@ -722,9 +722,21 @@ __global__ void __launch_bounds__(RC_TB) k_query_mlp(rc_dscene s, rc_dstate st, 
     int* shq = (int*)(sh + 9 * st.nt);
     const auto lstk = rc_lstk_sel<SPLIT>::make(sh, st, s);
     rc_load_dyn(st, sh, shq);
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= npath) return;
-    const int pix = p / cam.spp, px = pix % cam.W, py = pix / cam.W;
+    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+    int p, px, py;
+    if (64 % cam.spp == 0) {   // waves take pixel tiles, as in k_query; p stays the row-major path id
+        const int lane = threadIdx.x & 63, lp = lane / cam.spp;
+        const rc_qtile qt = rc_query_tile(cam.W, cam.H, cam.spp);
+        px = ((tid >> 6) % qt.tx) * qt.tw + lp % qt.tw;
+        py = ((tid >> 6) / qt.tx) * qt.th + lp / qt.tw;
+        if (px >= cam.W || py >= cam.H) return;
+        p = (py * cam.W + px) * cam.spp + lane % cam.spp;
+    } else {
+        p = tid;
+        if (p >= npath) return;
+        const int pix = p / cam.spp;
+        px = pix % cam.W; py = pix / cam.W;
+    }
     uint32_t rs = rc_seed(seed, frame, (uint32_t)p);
     float ju = rc_rnd(rs), jv = rc_rnd(rs);
     float sx = (2.f * ((float)px + ju) / (float)cam.W - 1.f) * cam.tanx;
@@ -1150,6 +1162,13 @@ hipError_t rc_launch_ref(int trav, const rc_dscene* s, const rc_dstate* st, cons
 }
 size_t rc_ovf_words(const rc_dstate* st, const rc_dscene* s, long long nthreads) { return rc_ovf_need(*st, *s, nthreads); }
 long long rc_grid_threads(long long n) { return (long long)rc_grid(n) * RC_TB; }
+long long rc_query_mlp_threads(const rc_camera* cam) {
+    if (64 % cam->spp == 0) {
+        const rc_qtile t = rc_query_tile(cam->W, cam->H, cam->spp);
+        return (long long)rc_grid((long long)t.tx * t.ty * 64) * RC_TB;
+    }
+    return (long long)rc_grid((long long)cam->W * cam->H * cam->spp) * RC_TB;
+}
 long long rc_query_threads(const rc_camera* cam) {
     const bool per_path = cam->spp <= 64 && 64 % cam->spp == 0;
     if (per_path) {   // one wave per pixel tile
@@ -1212,7 +1231,7 @@ hipError_t rc_launch_query_mlp(const rc_dscene* s, const rc_dstate* st, const rc
                                float* img, const rc_qwork* w, const rc_mlpdev* mlp, hipStream_t stream) {
     const int npath = cam->W * cam->H * cam->spp;
     if (npath <= 0) return hipSuccess;
-    const dim3 g(rc_grid(npath));
+    const dim3 g((unsigned)(rc_query_mlp_threads(cam) / RC_TB));
     RC_LAUNCH_WIDE((k_query_mlp<2, true>), (k_query_mlp<2, false>), false, g, st, s, stream, *s, *st, *cam, seed, frame, dormant, *w, npath);
     hipError_t e = rc_launch_mlp_infer(mlp, (const float4*)w->mq0, (const float4*)w->mq1, (const float4*)w->mq2, npath, w->mpred, stream);
     if (e != hipSuccess) return e;
