@@ -655,8 +655,15 @@ __host__ __device__ inline rc_qtile rc_query_tile(int W, int H, int spp) {
     int tw = 1;
     while (tw * tw < n) tw <<= 1;
     rc_qtile t;
-    t.tw = tw; t.th = n / tw; t.tx = (W + tw - 1) / tw; t.ty = (H + t.th - 1) / t.th;
+    t.tw = tw; t.th = n / tw;
+    t.tx = (W + 2 * tw - 1) / (2 * tw) * 2; t.ty = (H + 2 * t.th - 1) / (2 * t.th) * 2;   // even: blocks of 2 x 2 tiles
     return t;
+}
+// tile of wave w: the 4 waves of a 256-thread block take a 2 x 2 group of tiles
+__host__ __device__ inline void rc_query_tile_of(const rc_qtile& t, int w, int& tx, int& ty) {
+    const int b = w >> 2, q = w & 3, bx = t.tx >> 1;
+    tx = (b % bx) * 2 + (q & 1);
+    ty = (b / bx) * 2 + (q >> 1);
 }
 
 // Camera pass, one thread per path (spp divides 64: the spp paths of a pixel sit in adjacent
@@ -677,7 +684,9 @@ __global__ void __launch_bounds__(RC_TB) k_query(rc_dscene s, rc_dstate st, cons
         const int spp = cam.spp;
         const int lane = threadIdx.x & 63, w = tid >> 6, lp = lane / spp, first = lane - lane % spp;
         const rc_qtile qt = rc_query_tile(cam.W, cam.H, spp);
-        const int px = (w % qt.tx) * qt.tw + lp % qt.tw, py = (w / qt.tx) * qt.th + lp / qt.tw;
+        int tx, ty;
+        rc_query_tile_of(qt, w, tx, ty);
+        const int px = tx * qt.tw + lp % qt.tw, py = ty * qt.th + lp / qt.tw;
         const bool live = px < cam.W && py < cam.H;
         const int pix = py * cam.W + px;
         float rad = 0.f;
@@ -727,8 +736,10 @@ __global__ void __launch_bounds__(RC_TB) k_query_mlp(rc_dscene s, rc_dstate st, 
     if (64 % cam.spp == 0) {   // waves take pixel tiles, as in k_query; p stays the row-major path id
         const int lane = threadIdx.x & 63, lp = lane / cam.spp;
         const rc_qtile qt = rc_query_tile(cam.W, cam.H, cam.spp);
-        px = ((tid >> 6) % qt.tx) * qt.tw + lp % qt.tw;
-        py = ((tid >> 6) / qt.tx) * qt.th + lp / qt.tw;
+        int tx, ty;
+        rc_query_tile_of(qt, tid >> 6, tx, ty);
+        px = tx * qt.tw + lp % qt.tw;
+        py = ty * qt.th + lp / qt.tw;
         if (px >= cam.W || py >= cam.H) return;
         p = (py * cam.W + px) * cam.spp + lane % cam.spp;
     } else {
