@@ -472,6 +472,12 @@ struct rc_cache {
     hipStream_t rs = nullptr;
     hipEvent_t ev_upd = nullptr, ev_q = nullptr;
     bool q_pending = false;
+    // R is double-buffered: a frame's update writes the buffer the camera pass two frames back
+    // read (waiting for that pass's ev_r only), so it need not wait for the last camera pass.
+    float* Rb[2] = {nullptr, nullptr};
+    int rb = 0;                   // the buffer d.R points to
+    hipEvent_t ev_r[2] = {nullptr, nullptr};
+    bool r_pend[2] = {false, false};
     std::vector<uint64_t*> gmask;   // per state: gradient-support masks (grad_masks methods)
     uint32_t* qovf = nullptr;     // lane-stack overflow of the fused camera pass (qovf_words words)
     size_t qovf_words = 0;
@@ -490,6 +496,7 @@ struct rc_cache {
         for (auto& e : ev) if (e) (void)hipEventDestroy(e);
         if (ev_upd) (void)hipEventDestroy(ev_upd);
         if (ev_q) (void)hipEventDestroy(ev_q);
+        for (hipEvent_t e : ev_r) if (e) (void)hipEventDestroy(e);
         for (void* q : owned) (void)hipFree(q);
         free_qwork();
         if (qovf) (void)hipFree(qovf);
@@ -517,7 +524,8 @@ extern "C" rc_status rc_cache_create(rc_scene* s, const rc_method* m, rc_cache**
 #define CK2(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) { delete c; return fail(RC_ERR_HIP, "%s: %s", #x, hipGetErrorString(e_)); } } while (0)
     CK2(dalloc(&d.Lc, nl * Np, c->owned)); CK2(dalloc(&d.Lc2, nl * Np, c->owned)); CK2(dalloc(&d.m, nl * Np, c->owned));
     CK2(dalloc(&d.n, Np, c->owned)); CK2(dalloc(&d.c0, Np, c->owned)); CK2(dalloc(&d.kk, Np, c->owned));
-    CK2(dalloc(&d.R, Np, c->owned));
+    CK2(dalloc(&c->Rb[0], Np, c->owned)); CK2(dalloc(&c->Rb[1], Np, c->owned));
+    d.R = c->Rb[0];
     CK2(dalloc(&d.pending, Np, c->owned));
     CK2(dalloc(&d.cur, nw * Np, c->owned)); CK2(dalloc(&d.prev, nw * Np, c->owned));
     CK2(dalloc(&d.lcur, Np, c->owned)); CK2(dalloc(&d.lprev, Np, c->owned)); CK2(dalloc(&c->lnew, Np, c->owned));
@@ -531,6 +539,7 @@ extern "C" rc_status rc_cache_create(rc_scene* s, const rc_method* m, rc_cache**
     for (auto& e : c->ev) CK2(hipEventCreate(&e));
     CK2(hipStreamCreate(&c->rs));
     CK2(hipEventCreateWithFlags(&c->ev_upd, hipEventDisableTiming)); CK2(hipEventCreateWithFlags(&c->ev_q, hipEventDisableTiming));
+    for (auto& e : c->ev_r) CK2(hipEventCreateWithFlags(&e, hipEventDisableTiming));
 #undef CK2
     *out = c;
     return RC_OK;
@@ -784,7 +793,12 @@ extern "C" rc_status rc_cache_frame(rc_cache* c, int si, int want_err, int timed
         HIPCK(hipMemcpyAsync(d.npend, d.npend + 1, 4, hipMemcpyDeviceToDevice, st));
         c->pend_left--;
     }
-    HIPCK(wait_query(c));   // the update rewrites R, which the last camera pass reads
+    {   // the update writes the other R buffer: wait only for the camera pass that last read it
+        const int k = c->rb ^ 1;
+        if (c->r_pend[k]) { HIPCK(hipStreamWaitEvent(st, c->ev_r[k], 0)); c->r_pend[k] = false; }
+        c->rb = k;
+        d.R = c->Rb[k];
+    }
     if (timed) HIPCK(hipEventRecord(c->ev[3], st));
     HIPCK(rc_launch_update(&S->ds, &S->dst[si], &d, &p, track ? c->lnew : nullptr, st));   // also sets sumpr[0] = sumpr[1]
     std::swap(d.Lc, d.Lc2);
@@ -891,6 +905,8 @@ extern "C" rc_status rc_cache_render(rc_cache* c, const rc_camera* cam, int no_c
         HIPCK(rc_launch_query(S->trav, &S->ds, &dq, no_cache ? nullptr : c->c.R, cam, seed, (uint32_t)c->f, c->dormant, d_img, ctr, qs));
     HIPCK(hipEventRecord(c->ev_q, qs));
     c->q_pending = true;
+    HIPCK(hipEventRecord(c->ev_r[c->rb], qs));   // the pass read R buffer rb
+    c->r_pend[c->rb] = true;
     if (ms_out) {
         HIPCK(hipEventRecord(c->ev[1], qs));
         HIPCK(hipEventSynchronize(c->ev[1]));
