@@ -355,3 +355,27 @@ def test_snapshot_records_round_trip(torch_gpu, street):
     assert np.array_equal(lc2[ok, 0].view(np.uint32), s[ok].view(np.uint32))
     assert np.all(lc2[ok, 1:] == 0) and np.array_equal(n2[ok], n[ok]) and np.array_equal(age2[ok], age[ok])
     assert np.all(n2[valid[:10]] == 0) and np.all(lc2[valid[:10]] == 0)
+
+
+def test_query_tiles_odd_sizes(torch_gpu, room):
+    """the per-path camera pass takes square pixel tiles per wave and 2 x 2 tiles per block; image
+    sizes that no tile divides leave part of the edge tiles idle, and every spp that divides 64
+    (tile shapes 8 x 8 down to 1 x 1 pixels) gives the oracle's image"""
+    torch = torch_gpu
+    scene, osc = room
+    m = rea.methods.plain()
+    cache = rea.Cache(scene, m)
+    orun = po.OracleRunner(osc, m)
+    cache.begin(0, 3)
+    orun.begin(0, 3)
+    R = np.zeros(osc.Np, np.float32)
+    po.lib.orc_k_resolve(osc.state(0), po.ptr(orun.get("LC")), po.ptr(R), 0)
+    cam = scene.camera()
+    for W, H, spp in ((67, 45, 4), (33, 7, 1), (19, 23, 2), (9, 5, 16), (13, 3, 64)):
+        cam.W, cam.H, cam.spp, cam.bounces = W, H, spp, 1
+        img = torch.full((W * H,), -1.0, dtype=torch.float32, device="cuda")
+        cache.render(img, cam, seed=11)   # before any frame: resolved from the warm-start channels
+        exp = np.zeros(W * H, np.float32)
+        po.lib.orc_k_query(osc.state(0), 1, po.ptr(R), C.byref(cam), 11, 0, 0, po.ptr(exp), None, 0, W * H)
+        got = img.cpu().numpy()
+        assert np.array_equal(got.view(np.uint32), exp.view(np.uint32)), f"{W}x{H} spp {spp}: {np.sum(got != exp)} pixels differ"
