@@ -37,7 +37,7 @@ typedef struct rc_wnode {
     int32_t cend[4];                        // 16 B
 } rc_wnode;
 #define RC_WSTACK 48          // most LDS stack levels per lane (48 KB per 256-lane block)
-#define RC_WQ 8               // queued leaf ranges per lane of the wide closest-hit traversal (LDS levels)
+#define RC_WQ 7               // queued leaf ranges per lane of the wide closest-hit traversal (LDS levels)
 #define RC_LCAP 16            // LDS levels of the camera pass's lane stack (deeper levels: global overflow)
 
 // Per-state device view of the scene (cuda/poc_gpu.cu:222-237).
