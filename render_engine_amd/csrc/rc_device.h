@@ -373,10 +373,13 @@ __device__ __forceinline__ void rc_trace_wide_closest(f3 o, f3 d, const rc_dscen
 #undef RC_PUSH
         if (t0 < 1e30f && t0 <= tmin) { cur = i0; continue; }
         cur = -1;
+        // pop culling on the codes: rc_tlb(c) <= tmin exactly when c <= rc_tcode(tmin) (tmin >= 0; the
+        // code is monotone and rc_tlb(c) is the least value with code c), so no decode per entry
+        const uint32_t ct = rc_tcode(tmin);
         while (sp > 0) {
             --sp;
             const uint32_t e = lstk.pop(sp);
-            if (rc_tlb(e & 255u) <= tmin) { cur = (int)(e >> 8); break; }
+            if ((e & 255u) <= ct) { cur = (int)(e >> 8); break; }
         }
     }
 }

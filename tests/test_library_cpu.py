@@ -170,3 +170,28 @@ def test_prep_device_flag():
     """the cell-preparation switch takes 0 or 1 only (the device path itself is a GPU test)"""
     assert rea.lib.rc_scene_prep_device(2) != 0
     assert rea.lib.rc_scene_prep_device(0) == 0
+
+
+def test_stack_code_culling_equivalence():
+    """The wide traversal stores an 8-bit code of each pushed child's entry distance (exponent and
+    top 3 mantissa bits, t >= 2^-15; 0 below) and culls a popped child when its code exceeds the
+    code of the current hit distance. That equals culling on the code's lower bound (the least
+    distance with that code) against the distance itself, for every code and every distance:
+    checked here over all 256 codes and distances at and around every code boundary."""
+    def tcode(t):
+        b = np.asarray(t, np.float32).view(np.uint32).astype(np.int64)
+        return np.clip(((b >> 20) & 0x7FF) - 895, 0, 255)
+
+    def tlb(c):
+        c = np.asarray(c, np.int64)
+        v = (((c - 1) + 896) << 20).astype(np.uint32).view(np.float32)
+        return np.where(c == 0, np.float32(0), v)
+
+    codes = np.arange(256)
+    bounds = tlb(codes[1:]).astype(np.float32)
+    t = np.concatenate([bounds, np.nextafter(bounds, np.float32(0)), np.nextafter(bounds, np.float32(np.inf)),
+                        np.float32([0.0, 1e-4, 1e-30, 3.4e38, np.inf]),
+                        np.random.default_rng(1).random(4000).astype(np.float32) * np.float32(1e4)]).astype(np.float32)
+    assert np.all(tlb(tcode(t)) <= t)                      # a distance's code bound never exceeds it
+    for c in codes:
+        assert np.array_equal(tlb(c) <= t, c <= tcode(t)), c
