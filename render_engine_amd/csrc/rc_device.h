@@ -297,9 +297,13 @@ struct rc_lstk_s {
         else *(uint32_t*)((char*)ovf + off(sp)) = e;
     }
     __device__ __forceinline__ uint32_t pop(int sp) const {
+        // the global read names the global address space: otherwise the compiler merges the two
+        // reads into one generic (flat) load through a select of addresses, slower than the LDS
+        // read taken almost always
+        typedef const __attribute__((address_space(1))) uint32_t gu32;
         uint32_t e;
         if (sp < cap) e = col[sp * 256];
-        else e = *(const uint32_t*)((const char*)ovf + off(sp));
+        else e = *(gu32*)((const __attribute__((address_space(1))) char*)ovf + off(sp));
         return e;
     }
     __device__ __forceinline__ uint32_t* queue() const { return col + (size_t)cap * 256; }
