@@ -105,6 +105,74 @@ __device__ __forceinline__ int rc_adj_ray(const rc_dscene& s, const rc_dstate& s
     return rc_resolve_hit(s, st, hit, H, D, q, i, pi) ? -1 : pi;
 }
 
+// The reverse bounce traces its rays in direction order within each block: the entry threads
+// draw up to RC_ADJ_KS rays each (the RNG state before each ray is kept, so the ray is drawn again
+// exactly by whichever thread traces it), a counting sort in LDS groups them by octant and
+// dominant axis, and every thread then traces rays from the sorted list. The block's entries are
+// neighbours (Morton order), so a wave's rays then share an origin region and a direction class
+// and walk much the same nodes; entries of zero weight draw no rays, so the live rays are also
+// compacted. The sums into S are float atomics either way: their order was never fixed. (The
+// forward bounce keeps each entry's rays in the entry's thread: the same sort measured slower
+// there, by the per-entry state it has to carry across the block's barriers.)
+#define RC_ADJ_KS 4
+#define RC_ADJ_NB 24
+struct rc_adj_rays {
+    uint32_t v[RC_TB * RC_ADJ_KS];   // RNG state before the ray
+    uint16_t ord[RC_TB * RC_ADJ_KS]; // slots (thread * RC_ADJ_KS + ray) in direction order
+    int cnt[RC_ADJ_NB];
+    uint32_t rs[RC_TB];              // each thread's entry's RNG state between passes
+};
+__device__ __forceinline__ int rc_dir_class(f3 D) {
+    const float ax = fabsf(D.x), ay = fabsf(D.y), az = fabsf(D.z);
+    const int axis = ax >= ay && ax >= az ? 0 : (ay >= az ? 1 : 2);
+    return ((D.x < 0.f) | ((D.y < 0.f) << 1) | ((D.z < 0.f) << 2)) * 3 + axis;
+}
+// one pass over the next nk rays (nk <= RC_ADJ_KS) of the block's live entries; R.rs[t] is the
+// RNG state of thread t's entry, advanced past the pass's rays; each hit h of a ray of entry i
+// adds wgt[i] / K to S[h]. The state carried between passes lives in LDS, so nothing but the
+// tracing loop's own state is held in registers across the traversal.
+template <int TRAV, class LS>
+__device__ __forceinline__ void rc_adj_pass(const rc_dscene& s, const rc_dstate& st, const float* sh, const int* shq, const LS& lstk,
+                                            rc_adj_rays& R, bool live, int nk, const float* wgt, int K, float* S) {
+    const int t = threadIdx.x;
+    if (t < RC_ADJ_NB) R.cnt[t] = 0;
+    __syncthreads();
+    const f3 N = live ? ld3(st.N, blockIdx.x * RC_TB + t) : f3{0.f, 0.f, 1.f};
+    int key[RC_ADJ_KS];
+    uint32_t rs = R.rs[t];
+#pragma unroll
+    for (int j = 0; j < RC_ADJ_KS; ++j) {
+        key[j] = -1;
+        if (live && j < nk) {
+            R.v[t * RC_ADJ_KS + j] = rs;
+            const float u1 = rc_rnd(rs), u2 = rc_rnd(rs);
+            key[j] = rc_dir_class(rc_cosine_dir(N, u1, u2));
+            atomicAdd(&R.cnt[key[j]], 1);
+        }
+    }
+    R.rs[t] = rs;
+    __syncthreads();
+    if (t == 0) {
+        int run = 0;
+        for (int b = 0; b < RC_ADJ_NB; ++b) { const int c = R.cnt[b]; R.cnt[b] = run; run += c; }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < RC_ADJ_KS; ++j)
+        if (key[j] >= 0) R.ord[atomicAdd(&R.cnt[key[j]], 1)] = (uint16_t)(t * RC_ADJ_KS + j);
+    __syncthreads();
+    const int n = R.cnt[RC_ADJ_NB - 1];   // every bucket's start moved to its end: the last end is the count
+    for (int p = t; p < n; p += RC_TB) {
+        const int r = R.ord[p];
+        const int i = blockIdx.x * RC_TB + r / RC_ADJ_KS;
+        uint32_t q = R.v[r];
+        const f3 P = ld3(st.P, i), Ne = ld3(st.N, i), O = P + Ne * 1e-3f;
+        const int h = rc_adj_ray<TRAV>(s, st, sh, shq, lstk, i, O, Ne, q);
+        if (h >= 0) unsafeAtomicAdd(S + h, wgt[i] / (float)K);
+    }
+    __syncthreads();
+}
+
 // one forward bounce: Lout_i = (1/K) sum A_h (E_h + Lin_h), Sout_i = union of SG_h | Sin_h over
 // hits with A_h > 0 (Lin, Sin NULL: zero)
 template <int TRAV, bool SPLIT>
@@ -157,21 +225,18 @@ __global__ void __launch_bounds__(RC_TB) k_adj_bwd_init(int Np, const float* g, 
 // is a per-h factor times that weight, so k_adj_bwd_fin then adds (E_h + L^(b)_h) S_h to dA,
 // A_h S_h to dE and sets the adjoint of L^(b) to A_h S_h.
 template <int TRAV, bool SPLIT>
-__global__ void __launch_bounds__(RC_TB) k_adj_bwd(rc_dscene s, rc_dstate st, const float* lam_in, float* S, uint32_t seed, int K) {
+__global__ void __launch_bounds__(RC_TB) __attribute__((amdgpu_waves_per_eu(5))) k_adj_bwd(rc_dscene s, rc_dstate st, const float* lam_in, float* S, uint32_t seed, int K) {
     extern __shared__ float sh[];
+    __shared__ rc_adj_rays R;
     int* shq = (int*)(sh + 9 * st.nt);
     const auto lstk = rc_lstk_sel<SPLIT>::make(sh, st, s);
     rc_load_dyn(st, sh, shq);
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= s.Np) return;
-    const float w = lam_in[i] / (float)K;
-    if (w == 0.f) return;   // no contribution: the rays need not be traced
-    uint32_t rs = rc_hash32(seed ^ rc_hash32((uint32_t)i + 1u));
-    const f3 P = ld3(st.P, i), N = ld3(st.N, i), O = P + N * 1e-3f;
-    for (int k = 0; k < K; ++k) {
-        const int h = rc_adj_ray<TRAV>(s, st, sh, shq, lstk, i, O, N, rs);
-        if (h >= 0) unsafeAtomicAdd(S + h, w);
-    }
+    // an entry of zero weight contributes nothing: its rays are not traced
+    const bool live = i < s.Np && lam_in[i] / (float)K != 0.f;
+    R.rs[threadIdx.x] = rc_hash32(seed ^ rc_hash32((uint32_t)i + 1u));
+    for (int k0 = 0; k0 < K; k0 += RC_ADJ_KS)
+        rc_adj_pass<TRAV>(s, st, sh, shq, lstk, R, live, min(RC_ADJ_KS, K - k0), lam_in, K, S);
 }
 
 // per entry h after a reverse bounce: dA_h += (E_h + L^(b)_h) S_h, dE_h += A_h S_h, and the
