@@ -55,6 +55,28 @@ def test_adjoint_forward_bit_exact_and_gradient(torch_gpu, scenes, which, st):
     assert _rel(galb.cpu().numpy().astype(np.float64), dalb64) < 1e-4
 
 
+@pytest.mark.parametrize("which,K", [("room", 6), ("street", 9)])
+def test_adjoint_backward_several_ray_passes_and_zero_weights(torch_gpu, scenes, which, K):
+    """The reverse bounce traces a block's rays in passes of up to 4 rays per entry, sorted by
+    direction, and draws none for entries of zero weight: K = 6 and 9 take several passes (the
+    last one partial), and every third entry's cotangent is zero."""
+    torch = torch_gpu
+    s, osc = scenes[which]
+    B, seed, st = 2, 5, 0
+    adj = ad.Adjoint(s, state=st, K=K, bounces=B, seed=seed)
+    oadj = po.OracleAdjoint(osc, st, K, B, seed)
+    print(f"adjoint passes case {which} K {K}", flush=True)
+    I, alb, g = _params(s, st, 13)
+    g[::3] = 0.0
+    R = adj.forward(torch.from_numpy(I).cuda(), torch.from_numpy(alb).cuda())
+    R32, SR, LB = oadj.forward32(I, alb)
+    assert np.array_equal(R.cpu().numpy().view(np.uint32), R32.view(np.uint32))
+    gI, galb = adj.backward(torch.from_numpy(g).cuda())
+    J, R64, dI64, dalb64 = oadj.grad64(I.astype(np.float64), alb.astype(np.float64), g.astype(np.float64))
+    assert _rel(gI.cpu().numpy().astype(np.float64), dI64) < 1e-4
+    assert _rel(galb.cpu().numpy().astype(np.float64), dalb64) < 1e-4
+
+
 def test_adjoint_central_differences(torch_gpu, scenes):
     """dJ/dtheta of the GPU adjoint against (J(theta + h) - J(theta - h)) / 2h of the GPU FP32
     forward, h = 1e-3, for the largest emission and albedo components. FP32 rounding of J over
