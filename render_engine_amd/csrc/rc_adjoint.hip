@@ -174,9 +174,10 @@ __device__ __forceinline__ void rc_adj_pass(const rc_dscene& s, const rc_dstate&
 }
 
 // one forward bounce: Lout_i = (1/K) sum A_h (E_h + Lin_h), Sout_i = union of SG_h | Sin_h over
-// hits with A_h > 0 (Lin, Sin NULL: zero)
+// hits with A_h > 0 (Lin, Sin NULL: zero). Five waves per SIMD: the split-stack build fits 96
+// VGPRs (one spilled) and its smaller LDS lets five blocks reside, 3 % faster than four waves.
 template <int TRAV, bool SPLIT>
-__global__ void __launch_bounds__(RC_TB) k_adj_fwd(rc_dscene s, rc_dstate st, const float* A, const float* E, const uint64_t* SG,
+__global__ void __launch_bounds__(RC_TB) __attribute__((amdgpu_waves_per_eu(5))) k_adj_fwd(rc_dscene s, rc_dstate st, const float* A, const float* E, const uint64_t* SG,
                                                   const float* Lin, const uint64_t* Sin, float* Lout, uint64_t* Sout, uint32_t seed, int K) {
     extern __shared__ float sh[];
     int* shq = (int*)(sh + 9 * st.nt);
