@@ -76,6 +76,7 @@ def _load():
     lib.orc_dda_segments.argtypes = [C.POINTER(OrcScene), _P, _P, C.c_int, _P]
     lib.orc_dda_volume.argtypes = [C.POINTER(OrcScene), C.c_int, C.c_int, C.c_int, _P, _P, _P, C.c_int, _P, C.c_longlong]
     lib.orc_k_build.argtypes = [C.POINTER(OrcScene), C.c_int, _P, _P, _P]
+    lib.orc_k_build_rows.argtypes = [C.POINTER(OrcScene), C.c_int, C.c_int, C.c_int, _P, _P, _P, _P]
     lib.orc_k_trace2.argtypes = [C.POINTER(OrcScene), C.c_int, C.c_int, C.POINTER(OrcCache), C.POINTER(OrcParams), _P, _P, C.c_int]
     lib.orc_k_update.argtypes = [C.POINTER(OrcScene), C.POINTER(OrcCache), C.POINTER(OrcParams), _P]
     lib.orc_k_ref.argtypes = [C.POINTER(OrcScene), C.c_int, _P, _P, _P, _P, C.c_uint32, C.c_int, C.c_int, C.c_int]

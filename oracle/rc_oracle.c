@@ -474,33 +474,39 @@ static void rast_seg(const orc_scene* s, v3 a, v3 b, uint32_t* m) {   /* 20 samp
 }
 
 /* ------------------------------------------------------------------ state build (:369-391) */
+static void build_entry(const orc_scene* s, int trav, int i, float* Edir, uint32_t* SV, uint64_t* LM, uint32_t* VOL) {
+    v3 P = ld(s->P, i), N = ld(s->N, i), O = add(P, mul(N, 1e-3f));
+    uint32_t m[ORC_MAX_NW] = {0, 0, 0, 0};
+    uint64_t lm = 0;
+    for (int l = 0; l < s->nl; ++l) {
+        float r; v3 D;
+        if (s->ltype[l] == 1) { D = ld(s->L, l); r = s->sun_len; }
+        else { v3 dv = sub(ld(s->L, l), P); r = sqrtf(dot(dv, dv)); D = mul(dv, 1.f / r); }
+        float c = dot(N, D), t; int q;
+        trace_full(s, trav, O, D, &t, &q);
+        int vis = t >= r - 1e-3f;
+        float e;
+        if (s->ltype[l] == 1) e = s->inten[l] * fmaxf(c, 0.f) * (vis ? 1.f : 0.f);
+        else { float rr = fmaxf(r, 0.05f); e = s->inten[l] * fmaxf(c, 0.f) / (rr * rr) * (vis ? 1.f : 0.f); }
+        Edir[(size_t)s->nl * i + l] = e;
+        v3 end = add(O, mul(D, fminf(t, r)));
+        if (vis && c > 0.f) {
+            lm |= 1ull << (l < 63 ? l : 63);
+            if (s->fg > 0 && VOL) dda_walk(s, s->fg, s->fg, s->fg, O, end, set_mask_bit_atomic, VOL + (long long)l * fine_words(s->fg));
+        }
+        rast_seg(s, O, end, m);
+    }
+    for (int w = 0; w < s->nw; ++w) SV[(size_t)s->nw * i + w] = m[w];
+    LM[i] = lm;
+}
 void orc_k_build(const orc_scene* s, int trav, float* Edir, uint32_t* SV, uint64_t* LM) {
 #pragma omp parallel for schedule(dynamic, 64)
-    for (int i = 0; i < s->Np; ++i) {
-        v3 P = ld(s->P, i), N = ld(s->N, i), O = add(P, mul(N, 1e-3f));
-        uint32_t m[ORC_MAX_NW] = {0, 0, 0, 0};
-        uint64_t lm = 0;
-        for (int l = 0; l < s->nl; ++l) {
-            float r; v3 D;
-            if (s->ltype[l] == 1) { D = ld(s->L, l); r = s->sun_len; }
-            else { v3 dv = sub(ld(s->L, l), P); r = sqrtf(dot(dv, dv)); D = mul(dv, 1.f / r); }
-            float c = dot(N, D), t; int q;
-            trace_full(s, trav, O, D, &t, &q);
-            int vis = t >= r - 1e-3f;
-            float e;
-            if (s->ltype[l] == 1) e = s->inten[l] * fmaxf(c, 0.f) * (vis ? 1.f : 0.f);
-            else { float rr = fmaxf(r, 0.05f); e = s->inten[l] * fmaxf(c, 0.f) / (rr * rr) * (vis ? 1.f : 0.f); }
-            Edir[(size_t)s->nl * i + l] = e;
-            v3 end = add(O, mul(D, fminf(t, r)));
-            if (vis && c > 0.f) {
-                lm |= 1ull << (l < 63 ? l : 63);
-                if (s->fg > 0 && s->VOL) dda_walk(s, s->fg, s->fg, s->fg, O, end, set_mask_bit_atomic, s->VOL + (long long)l * fine_words(s->fg));
-            }
-            rast_seg(s, O, end, m);
-        }
-        for (int w = 0; w < s->nw; ++w) SV[(size_t)s->nw * i + w] = m[w];
-        LM[i] = lm;
-    }
+    for (int i = 0; i < s->Np; ++i) build_entry(s, trav, i, Edir, SV, LM, s->VOL);
+}
+void orc_k_build_rows(const orc_scene* s, int trav, int i0, int i1, float* Edir, uint32_t* SV, uint64_t* LM, uint32_t* VOL) {
+    if (i1 > s->Np) i1 = s->Np;
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int i = i0; i < i1; ++i) build_entry(s, trav, i, Edir, SV, LM, VOL);
 }
 
 static inline int dormant_l(uint64_t dm, int l) { return l < 64 && ((dm >> l) & 1ull); }

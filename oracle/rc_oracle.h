@@ -114,6 +114,9 @@ void orc_dda_volume(const orc_scene* s, int gx, int gy, int gz, const float* A, 
 /* ---- frame kernels (one call = the whole grid) ---- */
 /* also marks each lit shadow segment into s->VOL[l] when s->fg > 0 */
 void orc_k_build(const orc_scene* s, int trav, float* Edir, uint32_t* SV, uint64_t* LM);
+/* the same for entries [i0, i1) only, into full-size Edir / SV / LM; fine-volume bits of those entries'
+ * lit segments are ORed into VOL (may be NULL) */
+void orc_k_build_rows(const orc_scene* s, int trav, int i0, int i1, float* Edir, uint32_t* SV, uint64_t* LM, uint32_t* VOL);
 void orc_k_trace2(const orc_scene* s, int trav, int deps, const orc_cache* c, const orc_params* p, uint64_t* lnew, int32_t* hits, int kmax);
 void orc_k_decide(const orc_scene* s, const orc_cache* c, const orc_params* p);
 void orc_k_update(const orc_scene* s, const orc_cache* c, const orc_params* p, const uint64_t* lnew);

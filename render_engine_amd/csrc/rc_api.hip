@@ -535,6 +535,7 @@ extern "C" rc_status rc_cache_create(rc_scene* s, const rc_method* m, rc_cache**
     if (m->snaps && m->compact) { rc_snaprec* cs = nullptr; CK2(dalloc(&cs, (size_t)RC_KSNAP * Np, c->owned)); d.csnap = cs; }
     CK2(dalloc(&c->d_blk, (size_t)(Np + 255) / 256, c->owned));
     CK2(dalloc(&d.sumpr, 2, c->owned)); CK2(dalloc(&d.prpart, (size_t)(Np + 63) / 64, c->owned)); CK2(dalloc(&d.npend, 2, c->owned)); CK2(dalloc(&d.err, 3, c->owned));
+    CK2(dalloc(&d.errpart, 3 * (size_t)rc_grid_threads(Np) / 256, c->owned));   // 256-thread blocks of the error pass
     CK2(dalloc(&c->d_counters, 2, c->owned));
     for (auto& e : c->ev) CK2(hipEventCreate(&e));
     CK2(hipStreamCreate(&c->rs));
@@ -812,7 +813,6 @@ extern "C" rc_status rc_cache_frame(rc_cache* c, int si, int want_err, int timed
         fs.ms_update = ms_between(c->ev[3], c->ev[6]);
     }
     if (want_err && S->has_ref) {
-        HIPCK(hipMemsetAsync(d.err, 0, 24, st));
         HIPCK(rc_launch_error(&d, Np, nl, S->dst[si].ref, S->dst[si].valid, c->dormant, st));
         double h3[3];
         HIPCK(hipMemcpyAsync(h3, d.err, 24, hipMemcpyDeviceToHost, st));

@@ -510,16 +510,57 @@ __global__ void __launch_bounds__(RC_TB) k_flush(rc_dcache c, int Np, int nl) {
     for (int l = 0; l < nl; ++l) row[l] = 0.f;
 }
 
-// relative RMS of the estimate against the converged reference (cuda/poc_gpu.cu:683-696)
-__global__ void __launch_bounds__(RC_TB) k_error(rc_dcache c, int Np, int nl, const float* ref, const uint8_t* valid, uint64_t dormant) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+// Relative RMS of the estimate against the converged reference (cuda/poc_gpu.cu:683-696), in two
+// passes: each block writes its FP64 partial sums (squared error, reference, count) to part[3][blocks],
+// then one block adds them in block order into err[0..2]. No same-address atomics (one per wave
+// serialised ~12 ns each on this multi-die part), and the same sums in every run. An entry's light
+// row is read 16 bytes at a time.
+__global__ void __launch_bounds__(RC_TB) k_error(rc_dcache c, int Np, int nl, const float* __restrict__ ref,
+                                                const uint8_t* __restrict__ valid, uint64_t dormant, double* part) {
+    __shared__ double s_w[3][RC_TB / 64];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
     double d2 = 0, rs = 0, cnt = 0;
     if (i < Np && valid[i]) {
-        float e = rc_est_of(c.Lc, i, nl, dormant) - ref[i];
-        d2 = (double)e * e; rs = ref[i]; cnt = 1;
+        const float* row = c.Lc + (int64_t)nl * i;
+        float e = 0.f;
+        int l = 0;
+        if (!dormant) {
+            for (; l + 4 <= nl; l += 4) {
+                const rc_f4u v = *(const rc_f4u*)(row + l);
+                e += v.x; e += v.y; e += v.z; e += v.w;
+            }
+        }
+        for (; l < nl; ++l) if (!rc_is_dormant(dormant, l)) e += row[l];
+        const float r = ref[i];
+        const float d = e - r;
+        d2 = (double)d * d; rs = r; cnt = 1;
     }
     d2 = rc_wave_sum_f64(d2); rs = rc_wave_sum_f64(rs); cnt = rc_wave_sum_f64(cnt);
-    if ((threadIdx.x & 63) == 0) { atomicAdd(&c.err[0], d2); atomicAdd(&c.err[1], rs); atomicAdd(&c.err[2], cnt); }
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) { s_w[0][w] = d2; s_w[1][w] = rs; s_w[2][w] = cnt; }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        double v = 0;
+        for (int k = 0; k < RC_TB / 64; ++k) v += s_w[threadIdx.x][k];
+        part[(int64_t)threadIdx.x * gridDim.x + blockIdx.x] = v;
+    }
+}
+#define RC_ERB 1024
+__global__ void __launch_bounds__(RC_ERB) k_error_sum(const double* __restrict__ part, int nblk, double* err) {
+    __shared__ double s_w[RC_ERB / 64];
+    for (int q = 0; q < 3; ++q) {
+        double v = 0;
+        for (int j = threadIdx.x; j < nblk; j += RC_ERB) v += part[(int64_t)q * nblk + j];
+        v = rc_wave_sum_f64(v);
+        if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = v;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double t = 0;
+            for (int k = 0; k < RC_ERB / 64; ++k) t += s_w[k];
+            err[q] = t;
+        }
+        __syncthreads();
+    }
 }
 
 // One fresh-ray Jacobi iteration Lout = mean_K A (Edir + Lin) of the converged reference
@@ -1160,7 +1201,9 @@ hipError_t rc_launch_flush(const rc_dcache* c, int Np, int nl, hipStream_t strea
     return hipGetLastError();
 }
 hipError_t rc_launch_error(const rc_dcache* c, int Np, int nl, const float* ref, const uint8_t* valid, uint64_t dormant, hipStream_t stream) {
-    hipLaunchKernelGGL(k_error, dim3(rc_grid(Np)), dim3(RC_TB), 0, stream, *c, Np, nl, ref, valid, dormant);
+    const int nblk = rc_grid(Np);
+    hipLaunchKernelGGL(k_error, dim3(nblk), dim3(RC_TB), 0, stream, *c, Np, nl, ref, valid, dormant, c->errpart);
+    hipLaunchKernelGGL(k_error_sum, dim3(1), dim3(RC_ERB), 0, stream, (const double*)c->errpart, nblk, c->err);
     return hipGetLastError();
 }
 hipError_t rc_launch_ref(int trav, const rc_dscene* s, const rc_dstate* st, const float* Lin, float* Lout, float* acc, float* closec,

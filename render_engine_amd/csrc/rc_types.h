@@ -106,7 +106,8 @@ typedef struct rc_dcache {
     long long* sumpr;            // [2] fixed point (RC_FX_PR): current, next
     long long* prpart;           // [ceil(Np / 64)] per-block partial sums of the sumpr terms
     int32_t* npend;              // [2]
-    double* err;                 // [3]
+    double* err;                 // [3] squared error, reference, valid count of the error pass
+    double* errpart;             // [3][ceil(Np / 256)] per-block partials of the error pass
 } rc_dcache;
 
 // Per-frame parameters (cuda/poc_gpu.cu:411-422).
