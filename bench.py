@@ -1,15 +1,20 @@
 """Benchmark of the radiance-cache frame on one MI355X.
 
-Workload (default): procedural street of 2,000 boxes (~200k triangles), 32 point lamps + sun,
-~530k cache points in a 2^22-entry hash grid, camera 960x540 at 4 spp with up to 3 bounces.
-One step = one frame of the plain-cache method: entry pass (k_trace2: one cosine ray per cache
-entry, cache read at the hit) + resolve (k_update EMA) + camera query pass (k_query: primary ray,
-one-light NEE shadow ray, one bounce ray ending on the cached radiance).
+Workload (default): procedural street of 2,000 boxes (~1M triangles), 64 point lamps + sun, ~1M cache
+points in a 2^23-entry hash grid, 256^3 light volumes, camera 1920x1080 at 4 spp. One step = one
+frame of the plain-cache method: entry pass (k_trace2: one cosine ray per cache entry, cache read at
+the hit) + resolve (k_update EMA) + camera query pass (k_query: primary ray, one-light NEE shadow ray,
+one bounce ray ending on the cached radiance; without a cache a path takes up to 3 bounces).
 
-Prints one JSON line: value = Mrays/s over all rays traced in the timed frames (entry rays +
-query extension rays + query shadow rays), with inputs resident in HBM. Extra fields: cache
-Mpts/s, frames/s, per-kernel times, roofline of the dominant kernel and the CPU baseline
-(the C oracle on the host cores, bounded sample).
+Prints one JSON line: value = Mrays/s over all rays traced in the timed frames (entry rays + query
+extension rays + query shadow rays), with inputs resident in HBM. Extra fields: cache Mpts/s,
+frames/s, per-kernel times, the roofline of the dominant kernel (VALU issue for the camera pass, from
+rocprofv3 SQ counters of this workload, with its HBM traffic beside it), the CPU baseline (the C
+oracle on the host cores, bounded sample), the relative RMS of the FP32 results against their FP64
+twins, and the event timeline (light-off, night, occluder: each held 90 frames, then reverted).
+
+Other modes: --config 4 (adjoint), --config 5 (learned cache), --events (the event kernels' workload,
+for profiling), --pmc-summary (rocprofv3 counter CSVs -> the JSON summary read by the roofline).
 """
 import argparse
 import json
@@ -152,11 +157,22 @@ def reference_images(rea, scene, cam, log, ref_spp_rounds=16):
     return refs
 
 
-def quality(rea, scene, cam, log, pre=20, post=60, ref_spp_rounds=16, occluder=False):
-    """Light-off events against converged references (cuda/prep_mesh.cuh:454-520 solve, then the
-    same query at 64 spp): per method, the region-luminance bias sum(est)/sum(ref) - 1 over the
-    pixels where the switched-off light carried > 50 % of the converged radiance (9x9 blur at
-    960 px, windows/sondes.py:30-47), per frame after the event; frames until |bias| < 5 %."""
+def event_list(scene):
+    """the street's events (config 2: sun to 0, lamp 0 off, 12 x 10 m occluder inserted), each from the
+    base state; the box room's light-1-off event. (name, base state, event state, region kind)"""
+    if scene.info.NS >= 4:
+        return [("night (sun off)", 0, 1, "light"), ("lamp 0 off", 0, 2, "light"), ("occluder inserted", 0, 3, "panel")]
+    return [("light 1 off", 0, 1, "light")]
+
+
+def quality(rea, scene, cam, log, pre=20, hold=90, after=30, ref_spp_rounds=16, methods=None):
+    """Event timeline per method against converged references (cuda/prep_mesh.cuh:454-520 solve, then
+    the same camera pass at 16 x spp through the converged cache): `pre` frames in the base state, the
+    event state held `hold` frames, then reverted for `after` frames (snapshot restore and relighting).
+    Per frame the region-luminance bias sum(est) / sum(ref) - 1 over the affected region: for a light
+    event the pixels where the light carried > 50 % of the converged radiance, for a geometry event the
+    pixels whose converged radiance changes by > 25 % (9 x 9 blur at 960 px, windows/sondes.py:30-47);
+    frames until |bias| < 5 % after the event and after the revert."""
     import numpy as np
     import torch
 
@@ -164,43 +180,200 @@ def quality(rea, scene, cam, log, pre=20, post=60, ref_spp_rounds=16, occluder=F
 
     refs = reference_images(rea, scene, cam, log, ref_spp_rounds)
     img = torch.zeros(cam.W * cam.H, dtype=torch.float32, device="cuda")
-    events = {"lamp 0 off": (0, 2), "night (sun off)": (0, 1)} if scene.info.NS >= 3 else {"light 1 off": (0, 1)}
-    if occluder and scene.info.NS >= 4:
-        events["occluder inserted"] = (0, 3)
+    if methods is None:
+        methods = (("plain", rea.methods.plain()), ("flush_snapshots", rea.methods.flush_snapshots()),
+                   ("targeted", rea.methods.targeted()), ("targeted_grad", rea.methods.targeted_grad()))
     out = {}
-    for name, (a, b) in events.items():
-        region = sc.light_off_region(refs[a], refs[b])
-        ev = {"region_pixels": int(region.sum()), "methods": {}}
+    for name, a, b, kind in event_list(scene):
+        region = sc.light_off_region(refs[a], refs[b]) if kind == "light" else sc.panel_region(refs[a], refs[b])
+        ev = {"region": "light > 50 % of the converged radiance" if kind == "light" else "converged radiance changes > 25 %",
+              "region_pixels": int(region.sum()), "timeline": f"{pre} frames base, {hold} event, {after} reverted", "methods": {}}
         if region.sum() == 0:
             out[name] = ev
             continue
-        for mname, m in (("plain", rea.methods.plain()), ("flush_snapshots", rea.methods.flush_snapshots()),
-                         ("targeted", rea.methods.targeted()), ("targeted_grad", rea.methods.targeted_grad())):
+        reg = torch.from_numpy(region.ravel()).cuda()
+        ref_sum = {st: float(refs[st][region].sum()) for st in (a, b)}
+        for mname, m in methods:
             c = rea.Cache(scene, m)
             c.begin(a, 777)
-            biases, errs, inval, ev_ms = [], [], 0, None
-            for f in range(pre + post):
-                st = a if f < pre else b
-                fs = c.frame(st, want_err=True, timed=(f == pre))
-                inval += fs.invalidated
-                if f == pre:
-                    ev_ms = fs.ms_event   # GPU time of the event block (invalidation / flush / restore)
-                c.render(img, cam, seed=5000 + f)
+            seq = [a] * pre + [b] * hold + [a] * after
+            biases, errs, inval, ev_ms = [], [], {}, {}
+            for f, st in enumerate(seq):
+                tr = "event" if f == pre else ("revert" if f == pre + hold else None)
+                fs = c.frame(st, want_err=pre <= f < pre + 5, timed=tr is not None)
+                if tr:
+                    ev_ms[tr] = round(float(fs.ms_event), 4)   # GPU time of the event block
+                    inval[tr] = int(fs.invalidated)
+                if pre <= f < pre + 5:
+                    errs.append(round(float(fs.err), 4))
                 if f >= pre:
-                    est = img.cpu().numpy().reshape(cam.H, cam.W)
-                    biases.append(sc.region_bias(est, refs[b], region))
-                    errs.append(fs.err)
+                    c.render(img, cam, seed=5000 + f)
+                    biases.append(float(img[reg].double().sum()) / ref_sum[st] - 1.0)
+            post, rev = biases[:hold], biases[hold:]
             ev["methods"][mname] = {
-                "frames_to_5pct": sc.frames_to_below(biases, 0.05),
-                "bias_first_10": [round(x, 4) for x in biases[:10]],
-                "bias_mean_30": round(float(np.mean(np.abs(biases[:30]))), 4),
-                "entry_relrms_first_5": [round(x, 4) for x in errs[:5]],
+                "frames_to_5pct": sc.frames_to_below(post, 0.05),
+                "bias_first_10": [round(x, 4) for x in post[:10]],
+                "bias_mean_30": round(float(np.mean(np.abs(post[:30]))), 4),
+                "bias_last_10": round(float(np.mean(np.abs(post[-10:]))), 4),
+                "revert_frames_to_5pct": sc.frames_to_below(rev, 0.05),
+                "revert_bias_first_10": [round(x, 4) for x in rev[:10]],
+                "entry_relrms_first_5": errs,
                 "entries_invalidated": inval,
-                "event_ms": round(float(ev_ms), 4),
+                "event_block_ms": ev_ms,
             }
         out[name] = ev
         log(f"{name}: {json.dumps(ev)}")
     return out
+
+
+def fp64_results(rea, osc, scene, cam, log, frames=3, seed=1234):
+    """Relative RMS of the FP32 results against their FP64 twins (the oracle replaying the same rays and
+    hits with the radiometry in double), after `frames` plain-cache frames from the same start: cache
+    channels (every entry and light), resolved radiance (every entry), camera image (8 row bands of 8
+    rows). The FP32 cache arrays are first checked equal to the oracle's bit for bit."""
+    import numpy as np
+    import torch
+    import pyoracle as po
+
+    m = rea.methods.plain()
+    cache = rea.Cache(scene, m)
+    cache.begin(0, seed)
+    orun = po.OracleRunner(osc, m)
+    orun.begin(0, seed)
+    twin = po.FP64Twin(osc, orun, 0)
+    for _ in range(frames):
+        cache.frame(0)
+        twin.frame()
+    lc = cache.get("LC")
+    same = bool(np.array_equal(lc.view(np.uint32), orun.get("LC").view(np.uint32)))
+    W, H = cam.W, cam.H
+    img = torch.zeros(W * H, dtype=torch.float32, device="cuda")
+    cache.render(img, cam, seed=seed + 1)
+    got = img.cpu().numpy()
+    a, b = [], []
+    for k in range(8):
+        r0 = k * H // 8
+        a.append(got[r0 * W:(r0 + 8) * W])
+        b.append(twin.image64(cam, seed + 1, frames, r0 * W, (r0 + 8) * W)[r0 * W:(r0 + 8) * W])
+    out = {"cache_channels": po.rel_rms(lc, twin.L64), "resolved_radiance": po.rel_rms(cache.get("R"), twin.R64()),
+           "camera_image": po.rel_rms(np.concatenate(a), np.concatenate(b)), "tolerance": 1e-4,
+           "fp32_bit_exact_with_oracle": same,
+           "sample": f"{frames} plain-cache frames; channels and resolved radiance of all {scene.info.Np} entries; image rows "
+                     f"in 8 bands of 8"}
+    log(f"fp64 twins: {json.dumps(out)}")
+    return out
+
+
+def pmc_profile(tag):
+    """the committed rocprofv3 counter summary of this workload (profiles/pmc_<tag>.json), or None"""
+    path = os.path.join(ROOT, "profiles", f"pmc_{tag}.json")
+    return json.load(open(path)) if os.path.exists(path) else None
+
+
+def kernel_counters(prof, prefix):
+    if not prof:
+        return None
+    for kname, v in prof["kernels"].items():
+        if kname.split("<")[0] == prefix:
+            return v
+    return None
+
+
+VALU_PEAK = 256 * 2.4   # G wave-instructions/s: 256 CUs x one VALU wave-instruction per CU per cycle x 2.4 GHz
+
+
+def bench_events(args, rea, scene, log):
+    """The event kernels' workload (for rocprofv3): each event method runs 21 frames in the base state
+    (a snapshot slot), then every event and its revert with 2 frames each, the event blocks timed with
+    HIP events; then the DDA bit-volume marking of 2^20 random segments into a fine volume."""
+    import numpy as np
+    import torch
+
+    info = scene.info
+    ms = rea.methods
+    tl = dict(ms.timing_list())
+    methods = (("flush_snapshots", ms.flush_snapshots()), ("flush_snapshots_compact", ms.flush_snapshots_compact()),
+               ("targeted", ms.targeted()), ("targeted_grad", ms.targeted_grad()), ("verify (v3 complet)", tl["v3 complet"]))
+    seq = [0] * 21
+    for _, a, b, _k in event_list(scene):
+        seq += [b] * 2 + [a] * 2
+    res = {}
+    for name, m in methods:
+        c = rea.Cache(scene, m)
+        c.begin(0, 9)
+        times = []
+        for f, st in enumerate(seq):
+            ev = f > 0 and seq[f - 1] != st
+            fs = c.frame(st, timed=ev)
+            if ev:
+                times.append({"from": seq[f - 1], "to": st, "ms": round(float(fs.ms_event), 4), "invalidated": int(fs.invalidated)})
+        c.sync()
+        res[name] = times
+        log(f"events {name}: {json.dumps(times)}")
+    g = info.fine if info.fine > 0 else 128
+    n = 1 << 20
+    rng = np.random.default_rng(3)
+    lo, ext = np.array(list(info.lo)), np.array(list(info.ext))
+    A = torch.from_numpy((lo + ext * rng.random((n, 3))).astype(np.float32)).cuda()
+    B = torch.from_numpy((lo + ext * rng.random((n, 3))).astype(np.float32)).cuda()
+    wpv = (g ** 3 + 31) // 32
+    vol = torch.zeros(wpv, dtype=torch.int32, device="cuda")
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    dda = []
+    for _ in range(3):
+        vol.zero_()
+        e0.record()
+        rea.dda_volume(scene, (g, g, g), A, B, None, vol, wpv)
+        e1.record()
+        e1.synchronize()
+        dda.append(e0.elapsed_time(e1))
+    out = {"metric": "event block GPU ms per method and transition", "workload": f"street {info.nst} tris, {info.Np} entries, "
+           f"{info.nl} lights", "events": res, "dda_volume": {"segments": n, "grid": g, "ms": round(float(np.median(dda)), 4)}}
+    print(json.dumps(out))
+
+
+def summarize_pmc(dirs, tag, workload):
+    """rocprofv3 --pmc CSV outputs -> profiles/pmc_<tag>.json: per kernel the mean per dispatch of every
+    counter, FETCH_SIZE / WRITE_SIZE scaled by what the same counter reads for a 256 MiB device copy
+    (the directories named cal_FETCH_SIZE / cal_WRITE_SIZE), and the HBM bytes per dispatch"""
+    import csv
+    import glob
+
+    import numpy as np
+
+    per = {}
+    cal = {}
+    for d in dirs:
+        for f in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
+            rows = list(csv.DictReader(open(f)))
+            for r in rows:
+                k = r.get("Kernel_Name", "")
+                ctr, val = r.get("Counter_Name"), float(r.get("Counter_Value", 0) or 0)
+                key = (os.path.basename(d.rstrip("/")), k, ctr, r.get("Dispatch_Id"))
+                per.setdefault(key, 0.0)
+                per[key] += val   # counters of one dispatch may come in several rows (per dimension)
+    kernels = {}
+    for (d, k, ctr, disp), v in per.items():
+        name = k.split("(")[0].replace("void ", "").strip()
+        if d.startswith("cal_"):
+            cal.setdefault(ctr, []).append(v)
+            continue
+        kernels.setdefault(name, {}).setdefault(ctr, []).append(v)
+    copy_kib = 256 * 1024
+    factor = {c: copy_kib / max(v) for c, v in cal.items()}   # the copy kernel: the largest dispatch of the calibration run
+    out = {"workload": workload, "copy_bytes": copy_kib * 1024.0, "fetch_factor": factor.get("FETCH_SIZE"),
+           "write_factor": factor.get("WRITE_SIZE"), "kernels": {}}
+    for name, ctrs in kernels.items():
+        e = {c: float(np.mean(v)) for c, v in ctrs.items()}
+        e["dispatches"] = max(len(v) for v in ctrs.values())
+        if "FETCH_SIZE" in e and "WRITE_SIZE" in e and factor.get("FETCH_SIZE") and factor.get("WRITE_SIZE"):
+            e["fetch_bytes"] = e["FETCH_SIZE"] * 1024 * factor["FETCH_SIZE"]
+            e["write_bytes"] = e["WRITE_SIZE"] * 1024 * factor["WRITE_SIZE"]
+            e["traffic_bytes"] = e["fetch_bytes"] + e["write_bytes"]
+        out["kernels"][name] = e
+    path = os.path.join(ROOT, "profiles", f"pmc_{tag}.json")
+    json.dump(out, open(path, "w"), indent=1, sort_keys=True)
+    print(path)
 
 
 def adjoint_cpu(scene, adj, a, seconds, rays_step, log):
@@ -357,7 +530,8 @@ def bench_adjoint(args, rea, scene, cfg, log):
     qual = None
     if not args.no_quality:
         try:
-            qual = quality(rea, scene, scene.camera(), log, occluder=True)
+            qual = quality(rea, scene, scene.camera(), log, methods=(("plain", rea.methods.plain()), ("targeted", rea.methods.targeted()),
+                                                                    ("targeted_grad", rea.methods.targeted_grad())))
         except Exception as e:
             log(f"quality phase failed: {e!r}")
     ms = float(np.mean(fwd) + np.mean(bwd))
@@ -518,7 +692,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=30)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--config", type=int, default=2, choices=sorted(CONFIGS))
+    ap.add_argument("--config", type=int, default=3, choices=sorted(CONFIGS))
     ap.add_argument("--trav", type=int, default=2)
     ap.add_argument("--render", choices=["staged", "fused"], default="fused",
                     help="camera-pass schedule (same image): staged ray lists or one thread per path")
@@ -527,12 +701,21 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-quality", action="store_true")
+    ap.add_argument("--events", action="store_true", help="the event kernels' workload (for profiling) instead of frames")
+    ap.add_argument("--pmc-summary", nargs="+", metavar="DIR",
+                    help="summarise rocprofv3 --pmc CSV directories of this config's frame workload into profiles/pmc_<tag>.json")
     ap.add_argument("--quiet", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only: --gpus {args.gpus} is not supported (the path shards by screen tiles and "
               f"hash-key ranges, see the design notes; this build measures one GPU's share)", file=sys.stderr)
         sys.exit(2)
+    cfg = CONFIGS[args.config]
+    tag = f"{cfg['kind']}{cfg['W']}x{cfg['H']}" if cfg["kind"] == "street" else "room"
+    if args.pmc_summary:
+        summarize_pmc(args.pmc_summary, tag + ("_events" if args.events else ""), f"bench.py --config {args.config}"
+                      + (" --events" if args.events else ""))
+        return
 
     def log(msg):
         if not args.quiet:
@@ -544,7 +727,6 @@ def main():
     import render_engine_amd as rea
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
-    cfg = CONFIGS[args.config]
     rea.prep_device(True)   # BVH build and cache cells on the GPU (the host path's arrays, bit for bit)
     t0 = time.time()
     scene = make_scene(rea, cfg)
@@ -555,6 +737,9 @@ def main():
     t0 = time.time()
     scene.upload(trav=args.trav)
     log(f"upload + per-state build {time.time() - t0:.1f} s")
+    if args.events:
+        bench_events(args, rea, scene, log)
+        return
     if "adjoint" in cfg:
         bench_adjoint(args, rea, scene, cfg, log)
         return
@@ -598,7 +783,7 @@ def main():
     ms_per_step = elapsed * 1e3 / args.steps
     total_rays = n_entry + n_ext + n_sh
     mrays = total_rays / elapsed / 1e6
-    # 3) per-kernel GPU time with HIP events on the library stream (same frames again)
+    # 3) per-kernel GPU time with HIP events on the stream each kernel runs on (same frames again)
     cache.begin(0, seed)
     for f in range(args.warmup):
         one_frame(f)
@@ -613,44 +798,46 @@ def main():
     upd_bytes = update_bytes_per_entry(info.nl) * info.Np
     kernels = {
         "k_trace2": {"ms": t_tr, "rays": n_entry / args.steps, "mrays_s": n_entry / args.steps / t_tr / 1e3},
-        "k_update": {"ms": t_up, "entries": info.Np, "bytes": upd_bytes, "GB_s": upd_bytes / t_up / 1e6},
+        "k_update": {"ms": t_up, "entries": info.Np, "alg_bytes": upd_bytes, "GB_s": upd_bytes / t_up / 1e6,
+                     "hbm_frac": upd_bytes / t_up / 1e6 / 8000.0},
         "k_query": {"ms": t_q, "rays": (n_ext + n_sh) / args.steps, "mrays_s": (n_ext + n_sh) / args.steps / t_q / 1e3},
     }
-    dom = max(kernels, key=lambda k: kernels[k]["ms"])
     log(f"kernels: {json.dumps(kernels)}")
-    # CPU leg (the C oracle on the host cores): the baseline timing, and the algorithmic bytes of
-    # the query pass counted on the same paths (nodes / triangles / probes / resolved reads)
-    cpu, qb = None, None
+    # CPU leg (the C oracle on the host cores): the algorithmic bytes of the query pass counted on the
+    # same paths (nodes / triangles / probes / resolved reads), the FP64 twins, the baseline timing
+    cpu, qb, fp64 = None, None, None
     if not args.no_cpu:
         try:
             osc = oracle_scene(scene)
             qb, qwork = query_bytes(osc, cache.get("R"), cam, seed + args.warmup + args.steps - 1, args.warmup + args.steps, log)
-            kernels["k_query"].update({"alg_bytes": qb, "GB_s": qb / t_q / 1e6, "work": qwork})
+            kernels["k_query"].update({"touched_bytes": qb, "touched_GB_s": qb / t_q / 1e6, "work": qwork})
+            fp64 = fp64_results(rea, osc, scene, cam, log)
             cpu = cpu_baseline(osc, scene, args.cpu_seconds, log)
         except Exception as e:  # the baseline is reported, not required
             log(f"cpu leg failed: {e!r}")
-    if dom == "k_update":
-        ach = upd_bytes / t_up / 1e6
-    elif dom == "k_query" and qb:
-        ach = qb / t_q / 1e6
-    else:
-        ach = None
-    traffic = None
-    pmc = os.path.join(ROOT, "profiles", "pmc_street960x540.json")
-    if args.config == 2 and os.path.exists(pmc):   # HBM bytes per launch measured by rocprofv3 PMC on this workload
-        for kname, v in json.load(open(pmc))["kernels"].items():
-            if kname.startswith(dom):
-                traffic = v["traffic_bytes"]
-    roof = {"bound": "hbm", "achieved": ach, "peak": 8000.0, "unit": "GB/s", "frac": (ach / 8000.0) if ach else None,
-            "traffic": traffic, "traffic_source": "profiles/pmc_street960x540.json" if traffic else None, "kernel": dom,
-            "basis": ("k_query: 32 B/binary node + 36 B/triangle + 8 B/hash probe + 4 B/resolved read per path, oracle-counted"
-                      if dom == "k_query" else "k_update: 12 B + 12 B x lights per entry")}
+    # roofline of the camera pass: VALU issue (rocprofv3 SQ_INSTS_VALU of this workload per launch over the
+    # launch time measured above), its counter HBM traffic and the oracle-counted touched bytes beside it
+    prof = pmc_profile(tag)
+    kq = kernel_counters(prof, "k_query")
+    valu = kq.get("SQ_INSTS_VALU") if kq else None
+    traffic = kq.get("traffic_bytes") if kq else None
+    ach = valu / (t_q * 1e-3) / 1e9 if valu else None
+    roof = {"bound": "valu", "achieved": ach, "peak": VALU_PEAK, "unit": "G wave-instructions/s",
+            "frac": ach / VALU_PEAK if ach else None, "traffic": traffic, "kernel": "k_query",
+            "valu_per_launch": valu, "hbm_GB_s": traffic / t_q / 1e6 if traffic else None,
+            "hbm_frac": traffic / t_q / 1e6 / 8000.0 if traffic else None,
+            "touched_bytes": qb, "touched_frac_of_hbm_peak": qb / t_q / 1e6 / 8000.0 if qb else None,
+            "counters": f"profiles/pmc_{tag}.json" if prof else None,
+            "basis": "SQ_INSTS_VALU per launch (rocprofv3, this workload) / in-run launch time; peak = 256 CUs x 1 VALU "
+                     "wave-instruction per cycle x 2.4 GHz. traffic = FETCH_SIZE + WRITE_SIZE per launch (copy-calibrated); "
+                     "touched bytes = 32 B/binary node + 36 B/triangle + 8 B/hash probe + 4 B/resolved read, oracle-counted"}
     qual = None
     if not args.no_quality:
         try:
             qual = quality(rea, scene, cam, log)
         except Exception as e:
             log(f"quality phase failed: {e!r}")
+    cb = cam.cache_bounce
     out = {
         "metric": "Mrays/s + cache Mpts/s; frames to <5% error after light-off, 1 GPU",
         "value": round(mrays, 2),
@@ -665,7 +852,9 @@ def main():
         "dtype": "f32",
         "data": "synthetic (procedural scene, seeded)",
         "config": {"workload": (f"street {cfg.get('nboxes')} boxes / {info.nst} tris, {info.nl} lights, {info.Np} cache entries, "
-                                f"table 2^{int(np.log2(info.hsize))}, {cam.W}x{cam.H} x {cam.spp} spp, plain cache")
+                                f"table 2^{int(np.log2(info.hsize))}, {info.fine}^3 light volumes, {cam.W}x{cam.H} x {cam.spp} spp, "
+                                f"plain cache; camera paths end on the cache at bounce {cb} (up to {cam.bounces} bounces "
+                                f"without a cache)")
                    if cfg["kind"] == "street" else f"box room {info.nst} tris, {info.Np} entries, {cam.W}x{cam.H}",
                    "config_index": args.config, "traversal": ["binary-ref", "binary", "wide4"][args.trav],
                    "render": args.render if args.trav == 2 else "fused",
@@ -677,6 +866,7 @@ def main():
         "kernels": kernels,
         "roofline": roof,
         "cpu_baseline": cpu,
+        "fp64_relrms": fp64,
         "light_off": qual,
     }
     if cpu:

@@ -58,6 +58,9 @@ typedef struct rc_method {
                             gradient-support masks of the cached-radiance adjoint of both states
                             (grad_masks rays per entry and bounce, 3 bounces, seed 0x5eed0000 + state;
                             computed at rc_cache_begin) */
+    int eventboost;      /* > 0: for this many frames after an event that restores no snapshot, the history cap
+                            ramps as min(nmax, 2 + k/2) frames, k = frames since the event (the SDK bench's
+                            eventboost, rtxgi_bench.patch:355-356, :409-421; windows/run.sh:100-101) */
 } rc_method;
 
 #ifndef RC_CAMERA_DEFINED

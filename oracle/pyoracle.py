@@ -46,6 +46,7 @@ class OrcMethod(C.Structure):
         ("nmax_resp", C.c_float), ("ddgi", C.c_int), ("ddgi_h", C.c_float), ("ddgi_thr", C.c_float),
         ("ddgi_clamp", C.c_int), ("memory", C.c_int), ("ramp", C.c_int), ("wake_verify", C.c_int), ("snaps", C.c_int),
         ("snap_verify", C.c_int), ("flush", C.c_int), ("targeted", C.c_int), ("compact", C.c_int), ("grad_masks", C.c_int),
+        ("eventboost", C.c_int),
     ]
 
 
@@ -95,6 +96,9 @@ def _load():
     lib.orc_runner_frame.restype = C.c_double
     lib.orc_runner_frame.argtypes = [_P, C.c_int, C.c_int, C.POINTER(C.c_int)]
     lib.orc_runner_cache.restype = C.POINTER(OrcCache); lib.orc_runner_cache.argtypes = [_P]
+    lib.orc_runner_params.restype = C.POINTER(OrcParams); lib.orc_runner_params.argtypes = [_P]
+    lib.orc_k_resolve_f64.argtypes = [C.POINTER(OrcScene), _P, _P, C.c_uint64, _P]
+    lib.orc_k_query_f64.argtypes = [C.POINTER(OrcScene), C.c_int, _P, _P, C.c_uint32, C.c_uint32, C.c_uint64, _P, C.c_int, C.c_int]
     lib.orc_num_threads.restype = C.c_int
     lib.orc_adj_seed.restype = C.c_uint32; lib.orc_adj_seed.argtypes = [C.c_uint32, C.c_int]
     lib.orc_adj_unit.argtypes = [C.POINTER(OrcScene), C.c_int, _P, _P, _P]
@@ -227,6 +231,63 @@ class OracleRunner:
         addr = getattr(c, field)
         buf = (C.c_char * (n * np.dtype(dt).itemsize)).from_address(addr)
         return np.frombuffer(buf, dt).copy()
+
+
+class FP64Twin:
+    """FP64 twin of an OracleRunner's cache in one state (no events): each frame's entry-pass rays and
+    hits are recorded from the FP32 pass (orc_k_trace2 on a copy of the runner's cache), the sample
+    means and the EMA update are replayed in double with the FP32 run's ray counts and history, and
+    the direct light is the FP32 build's visibility with the radiometry in double. Create it right
+    after the runner's begin and call frame() instead of the runner's frame()."""
+
+    def __init__(self, osc, orun, st, kmax=4):
+        self.osc, self.orun, self.st, self.kmax, self.f = osc, orun, st, kmax, 0
+        i = osc.info
+        self.E64 = np.zeros(i.Np * i.nl)
+        lib.orc_k_build_f64(osc.state(st), 1, ptr(self.E64))
+        self.L64 = orun.get("LC").astype(np.float64)
+
+    def frame(self):
+        i, orun = self.osc.info, self.orun
+        s = self.osc.state(self.st)
+        n_before = orun.get("N")
+        c = lib.orc_runner_cache(orun.h).contents
+        p = OrcParams()
+        C.memmove(C.byref(p), lib.orc_runner_params(orun.h), C.sizeof(OrcParams))
+        p.frame, p.pend_left = self.f, 0
+        hits = np.full(i.Np * self.kmax, -2, np.int32)
+        mcopy = np.zeros(i.Np * i.nl, np.float32)
+        kk = np.zeros(i.Np, np.int32)
+        tmp = OrcCache()
+        for fld, _ in OrcCache._fields_:
+            setattr(tmp, fld, getattr(c, fld))
+        tmp.m, tmp.kk = mcopy.ctypes.data, kk.ctypes.data
+        lib.orc_k_trace2(s, 1, 0, C.byref(tmp), C.byref(p), None, ptr(hits), self.kmax)
+        assert kk.max(initial=0) <= self.kmax, "more rays per entry than recorded"
+        m64 = np.zeros(i.Np * i.nl)
+        lib.orc_k_trace2_f64(s, ptr(self.E64), ptr(self.L64), ptr(kk), ptr(hits), self.kmax, p.dormant, ptr(m64))
+        out = np.zeros_like(self.L64)
+        lib.orc_k_update_f64(s, ptr(n_before), ptr(kk), C.byref(p), ptr(self.L64), ptr(m64), ptr(out))
+        self.L64 = out
+        orun.frame(self.st)
+        self.f += 1
+
+    def R64(self, dormant=0):
+        R = np.zeros(self.osc.info.Np)
+        lib.orc_k_resolve_f64(self.osc.state(self.st), ptr(self.E64), ptr(self.L64), dormant, ptr(R))
+        return R
+
+    def image64(self, cam, seed, frame, pix0, pix1, no_cache=False):
+        img = np.zeros(cam.W * cam.H)
+        R = None if no_cache else self.R64()
+        lib.orc_k_query_f64(self.osc.state(self.st), 1, ptr(R), C.byref(cam), seed, frame, 0, ptr(img), pix0, pix1)
+        return img
+
+
+def rel_rms(a, b):
+    """relative RMS of a against the reference b: sqrt(mean((a - b)^2)) / mean(|b|)"""
+    a, b = np.asarray(a, np.float64), np.asarray(b, np.float64)
+    return float(np.sqrt(np.mean((a - b) ** 2)) / max(np.mean(np.abs(b)), 1e-300))
 
 
 class OracleAdjoint:

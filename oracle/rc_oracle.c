@@ -965,6 +965,89 @@ void orc_k_update_f64(const orc_scene* s, const float* n_before, const int32_t* 
     }
 }
 
+/* resolved radiance in double: R64_i = A_i sum_l (Edir64_il + Lc64_il) over the active lights */
+void orc_k_resolve_f64(const orc_scene* s, const double* Edir64, const double* Lc64, uint64_t dormant, double* R64) {
+    const int nl = s->nl;
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < s->Np; ++i) {
+        double v = 0.0;
+        for (int l = 0; l < nl; ++l) v += Edir64[(size_t)nl * i + l] + (dormant_l(dormant, l) ? 0.0 : Lc64[(size_t)nl * i + l]);
+        R64[i] = (double)s->Alb[i] * v;
+    }
+}
+
+/* FP64 twin of orc_k_query: each path takes the FP32 pass's rays, hits and light choices (the same
+ * draws and traversals), and its radiometry (next-event terms, throughput, the cached value R64 at the
+ * cache bounce, the pixel mean) is evaluated in double from the FP32 geometry. */
+void orc_k_query_f64(const orc_scene* s, int trav, const double* R64, const orc_camera* cam, uint32_t seed, uint32_t frame,
+                     uint64_t dormant, double* img64, int pix0, int pix1) {
+    if (pix1 > cam->W * cam->H) pix1 = cam->W * cam->H;
+    const int nl = s->nl;
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int pix = pix0; pix < pix1; ++pix) {
+        int px = pix % cam->W, py = pix / cam->W;
+        v3 pos = V3(cam->pos[0], cam->pos[1], cam->pos[2]), fw = V3(cam->fwd[0], cam->fwd[1], cam->fwd[2]);
+        v3 rt = V3(cam->right[0], cam->right[1], cam->right[2]), up = V3(cam->up[0], cam->up[1], cam->up[2]);
+        double sum = 0.0;
+        for (int sp = 0; sp < cam->spp; ++sp) {
+            uint32_t rs = orc_seed(seed, frame, (uint32_t)(pix * cam->spp + sp));
+            float ju = orc_rnd(&rs), jv = orc_rnd(&rs);
+            float sx = (2.f * ((float)px + ju) / (float)cam->W - 1.f) * cam->tanx;
+            float sy = (1.f - 2.f * ((float)py + jv) / (float)cam->H) * cam->tany;
+            v3 d = add(add(fw, mul(rt, sx)), mul(up, sy));
+            d = mul(d, 1.f / sqrtf(dot(d, d)));
+            v3 o = pos;
+            double thr = 1.0, rad = 0.0;
+            for (int b = 0;; ++b) {
+                float t; int q;
+                int hit = trace_full(s, trav, o, d, &t, &q);
+                if (!hit) break;
+                v3 H = add(o, mul(d, t)), n;
+                int e;
+                if (resolve_hit_n(s, H, d, q, &e, &n)) break;
+                if (R64 && b == cam->cache_bounce) { rad += thr * R64[e]; break; }
+                const double A = s->Alb[e];
+                /* next-event term: the FP32 light choice and visibility, the value in double */
+                float u = orc_rnd(&rs);
+                int l = imin((int)(u * (float)nl), nl - 1);
+                float I = s->inten[l];
+                if (I > 0.f && !dormant_l(dormant, l)) {
+                    float r; v3 D;
+                    if (s->ltype[l] == 1) { D = ld(s->L, l); r = s->sun_len; }
+                    else { v3 dv = sub(ld(s->L, l), H); r = sqrtf(dot(dv, dv)); D = mul(dv, 1.f / r); }
+                    float c = dot(n, D);
+                    if (c > 0.f) {
+                        v3 O = add(H, mul(n, 1e-3f));
+                        float ts; int qs;
+                        trace_full(s, trav, O, D, &ts, &qs);
+                        if (ts >= r - 1e-3f) {
+                            double val;
+                            if (s->ltype[l] == 1) {
+                                double cd = (double)n.x * s->L[3 * l] + (double)n.y * s->L[3 * l + 1] + (double)n.z * s->L[3 * l + 2];
+                                val = (double)nl * I * cd;
+                            } else {
+                                double dx = (double)s->L[3 * l] - H.x, dy = (double)s->L[3 * l + 1] - H.y, dz = (double)s->L[3 * l + 2] - H.z;
+                                double rd = sqrt(dx * dx + dy * dy + dz * dz);
+                                double cd = ((double)n.x * dx + (double)n.y * dy + (double)n.z * dz) / rd;
+                                double rr = rd > 0.05 ? rd : 0.05;
+                                val = (double)nl * I * cd / (rr * rr);
+                            }
+                            rad += thr * (A * val);
+                        }
+                    }
+                }
+                if (b >= cam->bounces) break;
+                thr = thr * A;
+                float u1 = orc_rnd(&rs), u2 = orc_rnd(&rs);
+                d = cosine_dir(n, u1, u2);
+                o = add(H, mul(n, 1e-3f));
+            }
+            sum += rad;
+        }
+        img64[pix] = sum / cam->spp;
+    }
+}
+
 /* ------------------------------------------------------------------ runner (cuda/poc_gpu.cu:826-1016) */
 struct orc_runner {
     const orc_scene* st; int NS; orc_method M; int trav;
@@ -977,6 +1060,7 @@ struct orc_runner {
     int slot_state[ORC_KSNAP];
     uint64_t dormant; float ilearn[128];
     float nmax;
+    int boost;        /* frames left of the post-event history-cap ramp (rtxgi_bench.patch:355-356, :409-421) */
     const uint64_t* gmask;
     /* compact snapshots: records (index, n, key, radiance, age) of the entries with samples */
     struct orc_rec { uint32_t index; float n; uint64_t key; float radiance; float age; } * csnap;
@@ -1025,6 +1109,7 @@ void orc_runner_destroy(orc_runner* r) {
     free(c->err); free(r->lnew); free(r->snap); free(r->csnap); free(r->ekeys); free(r);
 }
 const orc_cache* orc_runner_cache(const orc_runner* r) { return &r->c; }
+const orc_params* orc_runner_params(const orc_runner* r) { return &r->p; }
 
 void orc_runner_begin(orc_runner* r, int state0, uint32_t seed) {
     const orc_scene* s = &r->st[0];
@@ -1038,7 +1123,7 @@ void orc_runner_begin(orc_runner* r, int state0, uint32_t seed) {
     memset(c->lcur, 0, Np * 8); memset(c->lprev, 0, Np * 8); memset(r->lnew, 0, Np * 8); memset(c->stale, 0, Np * 8);
     c->npend[0] = c->npend[1] = 0; c->sumpr[0] = c->sumpr[1] = 0;
     orc_k_sumpr(c, (int)Np);
-    r->prev_state = state0; r->f = 0; r->pend_left = 0; r->dormant = 0;
+    r->prev_state = state0; r->f = 0; r->pend_left = 0; r->dormant = 0; r->boost = 0;
     memset(r->ilearn, 0, sizeof r->ilearn);
     r->nslots = 0; r->slot_next = 0; r->last_event_f = 0;
     for (int k = 0; k < ORC_KSNAP; ++k) r->slot_state[k] = -1;
@@ -1182,6 +1267,7 @@ static void event_block(orc_runner* r, int e, int si, int* invalidated) {
         }
     }
     r->last_event_f = r->f;
+    if (M->eventboost > 0 && !(M->snaps && hit >= 0)) r->boost = M->eventboost;
     if (M->mode == 1) { c->sumpr[0] = 0; orc_k_sumpr(c, Np); }
 }
 
@@ -1197,6 +1283,13 @@ double orc_runner_frame(orc_runner* r, int si, int want_err, int* invalidated) {
     if (invalidated) *invalidated = inv;
     p->pend_left = r->pend_left;
     p->dormant = r->dormant;
+    if (r->boost > 0) {   /* SHaRC accumulation frames min(default, 2 + k/2), k frames after the event */
+        const int k = M->eventboost - r->boost;
+        p->nmax = fminf(r->nmax, (2.f + (float)(k / 2)) * p->rays);
+        r->boost--;
+    } else {
+        p->nmax = r->nmax;
+    }
     if (M->ddgi) memset(c->dstat, 0, 3 * (size_t)s->NR * 8);
     if (M->deps && r->f % 32 == 0) {
         uint32_t* t = c->cur; c->cur = c->prev; c->prev = t;

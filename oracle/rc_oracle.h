@@ -151,6 +151,12 @@ void orc_k_build_f64(const orc_scene* s, int trav, double* Edir64);
  * -1 = ray did not count), the FP32 run's kk and a double cache Lc64 / Edir64 */
 void orc_k_trace2_f64(const orc_scene* s, const double* Edir64, const double* Lc64, const int32_t* kk, const int32_t* hits,
                       int kmax, uint64_t dormant, double* m64);
+/* resolved radiance in double, A_i sum_l (Edir64 + Lc64) over the active lights */
+void orc_k_resolve_f64(const orc_scene* s, const double* Edir64, const double* Lc64, uint64_t dormant, double* R64);
+/* the camera pass's FP64 twin: the FP32 pass's rays, hits and light choices, radiometry in double
+ * (R64 = NULL: no cache) */
+void orc_k_query_f64(const orc_scene* s, int trav, const double* R64, const orc_camera* cam, uint32_t seed, uint32_t frame,
+                     uint64_t dormant, double* img64, int pix0, int pix1);
 /* EMA update in double with the FP32 run's history counts (n before the update) */
 void orc_k_update_f64(const orc_scene* s, const float* n_before, const int32_t* kk, const orc_params* p, const double* Lc64,
                       const double* m64, double* Lc64_out);
@@ -181,6 +187,7 @@ typedef struct orc_method {
     int mode; float nmax, budget; int deps, verify, oracle, track_every, rescale;
     uint64_t resp_mask; float nmax_resp; int ddgi; float ddgi_h, ddgi_thr; int ddgi_clamp;
     int memory, ramp, wake_verify, snaps, snap_verify, flush, targeted, compact, grad_masks;
+    int eventboost;   /* frames of the history-cap ramp min(nmax, 2 + k/2) after an event that restores nothing */
 } orc_method;
 /* states[NS]: one orc_scene per state (static fields shared); occ / em event lists like the
  * library's (occ[a*NS+b], em[(a*NS+b)*nl+l]) given as offsets into one flat int array. */
@@ -196,6 +203,9 @@ void orc_runner_begin(orc_runner* r, int state0, uint32_t seed);
 /* one frame; returns the relative RMS error (or -1 without a reference) */
 double orc_runner_frame(orc_runner* r, int state, int want_err, int* invalidated);
 const orc_cache* orc_runner_cache(const orc_runner* r);
+/* the runner's frame parameters (those of the last frame; frame index, history cap and pending
+ * count are set again at the start of each frame) */
+const orc_params* orc_runner_params(const orc_runner* r);
 
 int orc_num_threads(void);
 

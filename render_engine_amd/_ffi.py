@@ -22,6 +22,7 @@ class Method(C.Structure):
         ("nmax_resp", C.c_float), ("ddgi", C.c_int), ("ddgi_h", C.c_float), ("ddgi_thr", C.c_float),
         ("ddgi_clamp", C.c_int), ("memory", C.c_int), ("ramp", C.c_int), ("wake_verify", C.c_int),
         ("snaps", C.c_int), ("snap_verify", C.c_int), ("flush", C.c_int), ("targeted", C.c_int), ("compact", C.c_int), ("grad_masks", C.c_int),
+        ("eventboost", C.c_int),
     ]
 
 

@@ -455,14 +455,17 @@ struct rc_cache {
     int f = 0;
     uint32_t seed = 0;
     float nmax = 0.f;
+    int boost = 0;                // frames left of the post-event history-cap ramp
     int pend_left = 0;
     uint64_t dormant = 0;
     std::vector<float> ilearn;
     int nslots = 0, slot_next = 0, last_event_f = 0;
     int slot_state[RC_KSNAP];
-    int slot_count[RC_KSNAP];     // records of each compact snapshot slot
+    int* d_slotcnt = nullptr;     // [RC_KSNAP + 1] records of each compact snapshot slot (device), then a spare count
     int* d_blk = nullptr;         // per-block counts / offsets of the snapshot packing
-    std::vector<int> h_blk;
+    int* h_inv = nullptr;         // pinned: entries reset by the last targeted invalidation
+    hipEvent_t ev_inv = nullptr;  // recorded after its copy to h_inv
+    bool inv_pending = false;
     rc_params p{};
     int render_mode = RC_RENDER_FUSED;
     // The camera pass runs on its own stream (rs) after the frame's update (ev_upd): the next
@@ -496,6 +499,8 @@ struct rc_cache {
         for (auto& e : ev) if (e) (void)hipEventDestroy(e);
         if (ev_upd) (void)hipEventDestroy(ev_upd);
         if (ev_q) (void)hipEventDestroy(ev_q);
+        if (ev_inv) (void)hipEventDestroy(ev_inv);
+        if (h_inv) (void)hipHostFree(h_inv);
         for (hipEvent_t e : ev_r) if (e) (void)hipEventDestroy(e);
         for (void* q : owned) (void)hipFree(q);
         free_qwork();
@@ -534,8 +539,12 @@ extern "C" rc_status rc_cache_create(rc_scene* s, const rc_method* m, rc_cache**
     if (m->snaps && !m->compact) CK2(dalloc(&d.snap, (size_t)RC_KSNAP * nl * Np, c->owned));
     if (m->snaps && m->compact) { rc_snaprec* cs = nullptr; CK2(dalloc(&cs, (size_t)RC_KSNAP * Np, c->owned)); d.csnap = cs; }
     CK2(dalloc(&c->d_blk, (size_t)(Np + 255) / 256, c->owned));
+    CK2(dalloc(&c->d_slotcnt, (size_t)RC_KSNAP + 1, c->owned));
+    CK2(hipHostMalloc((void**)&c->h_inv, sizeof(int), hipHostMallocDefault));
+    CK2(hipEventCreateWithFlags(&c->ev_inv, hipEventDisableTiming));
     CK2(dalloc(&d.sumpr, 2, c->owned)); CK2(dalloc(&d.prpart, (size_t)(Np + 63) / 64, c->owned)); CK2(dalloc(&d.npend, 2, c->owned)); CK2(dalloc(&d.err, 3, c->owned));
     CK2(dalloc(&d.errpart, 3 * (size_t)rc_grid_threads(Np) / 256, c->owned));   // 256-thread blocks of the error pass
+    CK2(dalloc(&d.blkcnt, (size_t)rc_grid_threads(Np) / 256 + 1, c->owned));
     CK2(dalloc(&c->d_counters, 2, c->owned));
     for (auto& e : c->ev) CK2(hipEventCreate(&e));
     CK2(hipStreamCreate(&c->rs));
@@ -550,22 +559,15 @@ extern "C" void rc_cache_destroy(rc_cache* c) { delete c; }
 
 static rc_status cache_grad_masks(rc_cache* c);
 
-// pack the entries with samples into compact records at dst (entry order); *count = records
-static hipError_t snap_pack(rc_cache* c, rc_snaprec* dst, int* count) {
+// pack the entries with samples into compact records at dst (entry order), their number into *d_count
+// (device): per-block counts, a device scan of them, then the ballot-ranked writes (no host round trip)
+static hipError_t snap_pack(rc_cache* c, rc_snaprec* dst, int* d_count) {
     rc_scene* S = c->S;
     const int Np = S->H.Np, nb = (Np + 255) / 256;
     hipStream_t st = S->stream;
     hipError_t e = rc_launch_snap_count(&c->c, Np, c->d_blk, st);
-    c->h_blk.resize(nb);
-    if (e == hipSuccess) e = hipMemcpyAsync(c->h_blk.data(), c->d_blk, (size_t)nb * 4, hipMemcpyDeviceToHost, st);
-    if (e == hipSuccess) e = hipStreamSynchronize(st);
-    if (e != hipSuccess) return e;
-    int run = 0;
-    for (int b = 0; b < nb; ++b) { const int v = c->h_blk[b]; c->h_blk[b] = run; run += v; }
-    e = hipMemcpyAsync(c->d_blk, c->h_blk.data(), (size_t)nb * 4, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = rc_launch_scan_blocks(c->d_blk, nb, d_count, st);
     if (e == hipSuccess) e = rc_launch_snap_pack(&c->c, Np, S->H.nl, S->d_ekeys, c->d_blk, dst, st);
-    if (e == hipSuccess) e = hipStreamSynchronize(st);   // the host copy of the offsets must outlive the kernel
-    *count = run;
     return e;
 }
 
@@ -603,10 +605,12 @@ extern "C" rc_status rc_cache_begin(rc_cache* c, int state0, uint32_t seed) {
     c->f = 0;
     c->seed = seed;
     c->pend_left = 0;
+    c->boost = 0;
     c->dormant = 0;
     c->ilearn.assign(nl, 0.f);
     c->nslots = 0; c->slot_next = 0; c->last_event_f = 0;
-    for (int k = 0; k < RC_KSNAP; ++k) { c->slot_state[k] = -1; c->slot_count[k] = 0; }
+    for (int k = 0; k < RC_KSNAP; ++k) c->slot_state[k] = -1;
+    HIPCK(hipMemsetAsync(c->d_slotcnt, 0, (RC_KSNAP + 1) * sizeof(int), st));
     rc_params& p = c->p;
     memset(&p, 0, sizeof p);
     p.Np = (int)Np; p.NR = H.NR; p.mode = c->M.mode; p.budget = c->M.budget; p.nmax = c->nmax; p.seed = seed;
@@ -619,7 +623,17 @@ extern "C" rc_status rc_cache_begin(rc_cache* c, int state0, uint32_t seed) {
 static float ms_between(hipEvent_t a, hipEvent_t b) { float ms = 0.f; (void)hipEventElapsedTime(&ms, a, b); return ms; }
 
 // event block of Runner::run for the transition e -> si (cuda/poc_gpu.cu:856-943)
-static rc_status run_event(rc_cache* c, int e, int si, int& invalidated) {
+// event voxels of the transit grid (a voxel list) as the per-word mask the event kernels test
+static rc_vmask vox_mask(const std::vector<int>& V) {
+    rc_vmask m;
+    memset(&m, 0, sizeof m);
+    for (int v : V) if (v >= 0 && (v >> 5) < RC_MAX_NW) m.w[v >> 5] |= 1u << (v & 31);
+    return m;
+}
+
+// The event block queues its work on the scene stream without waiting for it: the number of entries
+// the targeted invalidation reset is copied to pinned host memory and read at the end of the frame.
+static rc_status run_event(rc_cache* c, int e, int si) {
     rc_scene* S = c->S;
     const rc::Scene& H = S->H;
     const rc_method& M = c->M;
@@ -634,9 +648,7 @@ static rc_status run_event(rc_cache* c, int e, int si, int& invalidated) {
         if (k < 0) { if (c->nslots < RC_KSNAP) k = c->nslots++; else { k = c->slot_next; c->slot_next = (c->slot_next + 1) % RC_KSNAP; } }
         c->slot_state[k] = e;
         if (M.compact) {
-            int cnt = 0;
-            HIPCK(snap_pack(c, (rc_snaprec*)d.csnap + (size_t)k * Np, &cnt));
-            c->slot_count[k] = cnt;
+            HIPCK(snap_pack(c, (rc_snaprec*)d.csnap + (size_t)k * Np, c->d_slotcnt + k));
         } else {
             HIPCK(hipMemcpyAsync(d.snap + (size_t)k * Np * nl, d.Lc, (size_t)Np * nl * 4, hipMemcpyDeviceToDevice, st));
         }
@@ -675,16 +687,16 @@ static rc_status run_event(rc_cache* c, int e, int si, int& invalidated) {
         for (int l = 0; l < nl; ++l) {   // new light without memory: touched entries restart
             int ix = pe * nl + l;
             if (!H.em[ix].empty() && !(l < 64 && ((woke >> l) & 1ull)))
-                HIPCK(rc_launch_touched(&d, Np, nl, S->dEm[ix], (int)H.em[ix].size(), 0, c->dormant, st));
+                HIPCK(rc_launch_touched(&d, Np, nl, vox_mask(H.em[ix]), H.nw, 0, c->dormant, st));
         }
         if (!H.occ[pe].empty()) {
             if (M.verify) {
                 HIPCK(hipMemsetAsync(d.npend, 0, 8, st));
                 HIPCK(hipMemsetAsync(d.rstat, 0, 4 * (size_t)H.NR * 8, st));
-                HIPCK(rc_launch_touched(&d, Np, nl, S->dOcc[pe], (int)H.occ[pe].size(), 1, c->dormant, st));
+                HIPCK(rc_launch_touched(&d, Np, nl, vox_mask(H.occ[pe]), H.nw, 1, c->dormant, st));
                 HIPCK(hipMemcpyAsync(d.npend, d.npend + 1, 4, hipMemcpyDeviceToDevice, st));
                 c->pend_left = RC_VERIF_FRAMES;
-            } else if (c->dormant) HIPCK(rc_launch_touched(&d, Np, nl, S->dOcc[pe], (int)H.occ[pe].size(), 2, c->dormant, st));
+            } else if (c->dormant) HIPCK(rc_launch_touched(&d, Np, nl, vox_mask(H.occ[pe]), H.nw, 2, c->dormant, st));
         }
     }
     int hit = -1;
@@ -693,26 +705,23 @@ static rc_status run_event(rc_cache* c, int e, int si, int& invalidated) {
     if (M.targeted) {   // resolve-time dependency invalidation (rtxgi_bench.patch:368-386, :1028-1050)
         uint64_t inv = 0;
         for (int l = 0; l < nl; ++l) if (H.inten[e][l] != H.inten[si][l]) inv |= 1ull << (l < 63 ? l : 63);
-        int nv = (int)H.occ[pe].size();
+        const int nv = (int)H.occ[pe].size();
+        const unsigned long long* extra = nullptr;
         if (H.fine > 0 && !H.occ_fine[pe].empty()) {   // lights whose lit segments cross the object (before or after)
             if (!S->d_lights) { HIPCK(hipMalloc((void**)&S->d_lights, 8)); S->owned.push_back(S->d_lights); }
             HIPCK(hipMemsetAsync(S->d_lights, 0, 8, st));
             HIPCK(rc_launch_vol_lights(nl, S->dst[e].VOL, S->dst[si].VOL, (long long)fine_words(H.fine), S->dOccF[pe],
                                        (int)H.occ_fine[pe].size(), S->d_lights, st));
-            unsigned long long lights = 0;
-            HIPCK(hipMemcpyAsync(&lights, S->d_lights, 8, hipMemcpyDeviceToHost, st));
-            HIPCK(hipStreamSynchronize(st));
-            inv |= lights;
+            extra = S->d_lights;   // read by the invalidation kernel: no host round trip
         }
-        if (inv || nv) {
-            HIPCK(hipMemsetAsync(d.npend + 1, 0, 4, st));
+        if (inv || nv || extra) {
             const bool gm = !c->gmask.empty();
-            HIPCK(rc_launch_invalidate(&d, Np, nl, inv, S->dOcc[pe], nv, H.nw, gm ? c->gmask[e] : nullptr, gm ? c->gmask[si] : nullptr, st));
-            int h2 = 0;
-            HIPCK(hipMemcpyAsync(&h2, d.npend + 1, 4, hipMemcpyDeviceToHost, st));
-            HIPCK(hipStreamSynchronize(st));
-            HIPCK(hipMemsetAsync(d.npend + 1, 0, 4, st));
-            invalidated = h2;
+            int* cnt = d.blkcnt + (Np + 255) / 256;
+            HIPCK(rc_launch_invalidate(&d, Np, nl, inv, extra, vox_mask(H.occ[pe]), H.nw, gm ? c->gmask[e] : nullptr,
+                                       gm ? c->gmask[si] : nullptr, cnt, st));
+            HIPCK(hipMemcpyAsync(c->h_inv, cnt, sizeof(int), hipMemcpyDeviceToHost, st));
+            HIPCK(hipEventRecord(c->ev_inv, st));
+            c->inv_pending = true;
         }
     }
     if (M.flush && hit < 0) HIPCK(rc_launch_flush(&d, Np, nl, st));   // new state: clear (rtxgi_bench.patch:363-366)
@@ -720,7 +729,7 @@ static rc_status run_event(rc_cache* c, int e, int si, int& invalidated) {
         HIPCK(rc_launch_flush(&d, Np, nl, st));
         HIPCK(hipMemsetAsync(d.npend, 0, 8, st));
         if (M.snap_verify) HIPCK(hipMemsetAsync(d.rstat, 0, 4 * (size_t)H.NR * 8, st));
-        HIPCK(rc_launch_snap_restore(&d, Np, nl, S->d_ekeys, (const rc_snaprec*)d.csnap + (size_t)hit * Np, c->slot_count[hit],
+        HIPCK(rc_launch_snap_restore(&d, Np, nl, S->d_ekeys, (const rc_snaprec*)d.csnap + (size_t)hit * Np, c->d_slotcnt + hit, Np,
                                      M.snap_verify ? 1 : 0, c->dormant, st));
         if (M.snap_verify) {
             HIPCK(hipMemcpyAsync(d.npend, d.npend + 1, 4, hipMemcpyDeviceToDevice, st));
@@ -743,6 +752,7 @@ static rc_status run_event(rc_cache* c, int e, int si, int& invalidated) {
         }
     }
     c->last_event_f = f;
+    if (M.eventboost > 0 && !(M.snaps && hit >= 0)) c->boost = M.eventboost;   // rtxgi_bench.patch:355-356
     if (M.mode == 1) {   // ray allocation must see this frame's invalidation
         HIPCK(rc_launch_sumpr(&d, Np, st));
     }
@@ -767,13 +777,20 @@ extern "C" rc_status rc_cache_frame(rc_cache* c, int si, int want_err, int timed
     if (c->f > 0 && c->prev_state != si) {
         HIPCK(wait_query(c));   // event handling rewrites what the last camera pass reads
         if (timed) HIPCK(hipEventRecord(c->ev[4], st));
-        rc_status r = run_event(c, c->prev_state, si, fs.invalidated);
+        rc_status r = run_event(c, c->prev_state, si);
         if (r != RC_OK) return r;
         fs.event = 1;
         if (timed) { HIPCK(hipEventRecord(c->ev[5], st)); HIPCK(hipEventSynchronize(c->ev[5])); fs.ms_event = ms_between(c->ev[4], c->ev[5]); }
     }
     p.pend_left = c->pend_left;
     p.dormant = c->dormant;
+    if (c->boost > 0) {   // post-event history cap: min(nmax, 2 + k/2) frames, k frames after the event (:409-421)
+        const int k = M.eventboost - c->boost;
+        p.nmax = fminf(c->nmax, (2.f + (float)(k / 2)) * p.rays);
+        c->boost--;
+    } else {
+        p.nmax = c->nmax;
+    }
     if (M.ddgi) HIPCK(hipMemsetAsync(d.dstat, 0, 3 * (size_t)H.NR * 8, st));
     if (timed) HIPCK(hipEventRecord(c->ev[0], st));
     if (M.deps && c->f % 32 == 0) {   // dependency banks: swap every 32 frames, clear the new current bank
@@ -818,6 +835,11 @@ extern "C" rc_status rc_cache_frame(rc_cache* c, int si, int want_err, int timed
         HIPCK(hipMemcpyAsync(h3, d.err, 24, hipMemcpyDeviceToHost, st));
         HIPCK(hipStreamSynchronize(st));
         fs.err = h3[2] > 0 ? std::sqrt(h3[0] / h3[2]) / (h3[1] / h3[2]) : 0.0;
+    }
+    if (c->inv_pending) {   // the event's reset count: its copy was queued before this frame's trace
+        HIPCK(hipEventSynchronize(c->ev_inv));
+        fs.invalidated = *c->h_inv;
+        c->inv_pending = false;
     }
     c->prev_state = si;
     c->f++;
@@ -1448,7 +1470,9 @@ extern "C" rc_status rc_cache_snapshot_save(rc_cache* c, rc_snap_rec* host_recor
     rc_snaprec* tmp = nullptr;
     HIPCK(hipMalloc((void**)&tmp, Np * sizeof(rc_snaprec)));
     int cnt = 0;
-    hipError_t e = snap_pack(c, tmp, &cnt);
+    hipError_t e = snap_pack(c, tmp, c->d_slotcnt + RC_KSNAP);
+    if (e == hipSuccess) e = hipMemcpyAsync(&cnt, c->d_slotcnt + RC_KSNAP, sizeof(int), hipMemcpyDeviceToHost, S->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(S->stream);
     const size_t n = std::min((size_t)cnt, capacity);
     if (e == hipSuccess && n) e = hipMemcpy(host_records, tmp, n * sizeof(rc_snaprec), hipMemcpyDeviceToHost);
     (void)hipFree(tmp);
@@ -1468,8 +1492,10 @@ extern "C" rc_status rc_cache_snapshot_load(rc_cache* c, const rc_snap_rec* host
     rc_snaprec* tmp = nullptr;
     HIPCK(hipMalloc((void**)&tmp, std::max<size_t>(count, 1) * sizeof(rc_snaprec)));
     hipError_t e = count ? hipMemcpyAsync(tmp, host_records, count * sizeof(rc_snaprec), hipMemcpyHostToDevice, st) : hipSuccess;
+    const int n = (int)count;
+    if (e == hipSuccess) e = hipMemcpyAsync(c->d_slotcnt + RC_KSNAP, &n, sizeof(int), hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = rc_launch_flush(&c->c, Np, nl, st);
-    if (e == hipSuccess) e = rc_launch_snap_restore(&c->c, Np, nl, S->d_ekeys, tmp, (int)count, 0, c->dormant, st);
+    if (e == hipSuccess) e = rc_launch_snap_restore(&c->c, Np, nl, S->d_ekeys, tmp, c->d_slotcnt + RC_KSNAP, n, 0, c->dormant, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     (void)hipFree(tmp);
     if (e != hipSuccess) return fail(RC_ERR_HIP, "snapshot load: %s", hipGetErrorString(e));

@@ -16,11 +16,13 @@ hipError_t rc_launch_resolve(const rc_dscene* s, const rc_dstate* st, const floa
 hipError_t rc_launch_sumpr(const rc_dcache* c, int Np, hipStream_t stream);
 hipError_t rc_launch_restored(const rc_dcache* c, int Np, int nl, float nmax, uint64_t dormant, int verify, hipStream_t stream);
 hipError_t rc_launch_rescale(const rc_dcache* c, int Np, int nl, int l, float ratio, hipStream_t stream);
-hipError_t rc_launch_touched(const rc_dcache* c, int Np, int nl, const int* V, int nv, int kind, uint64_t dormant, hipStream_t stream);
+// kind 1 adds the entries it puts in verification to c->npend[1]
+hipError_t rc_launch_touched(const rc_dcache* c, int Np, int nl, rc_vmask vm, int nw, int kind, uint64_t dormant, hipStream_t stream);
 hipError_t rc_launch_wake(const rc_dcache* c, int Np, int nl, int l, int verify, uint64_t dormant, hipStream_t stream);
 hipError_t rc_launch_oracle_method(const rc_dcache* c, int Np, const float* rold, const float* rnew, hipStream_t stream);
-hipError_t rc_launch_invalidate(const rc_dcache* c, int Np, int nl, uint64_t inv_lights, const int* V, int nv, int nw,
-                                const uint64_t* ga, const uint64_t* gb, hipStream_t stream);
+// inv_extra (device, may be NULL): more changed-light bits, read by the kernel; count (device): entries reset
+hipError_t rc_launch_invalidate(const rc_dcache* c, int Np, int nl, uint64_t inv_lights, const unsigned long long* inv_extra, rc_vmask vm,
+                                int nw, const uint64_t* ga, const uint64_t* gb, int* count, hipStream_t stream);
 hipError_t rc_launch_flush(const rc_dcache* c, int Np, int nl, hipStream_t stream);
 hipError_t rc_launch_error(const rc_dcache* c, int Np, int nl, const float* ref, const uint8_t* valid, uint64_t dormant, hipStream_t stream);
 hipError_t rc_launch_ref(int trav, const rc_dscene* s, const rc_dstate* st, const float* Lin, float* Lout, float* acc, float* closec,
@@ -68,5 +70,8 @@ hipError_t rc_launch_mlp_judge(int n, const float* y, const float* R, const int*
 hipError_t rc_launch_snap_count(const rc_dcache* c, int Np, int* blockcnt, hipStream_t stream);
 hipError_t rc_launch_snap_pack(const rc_dcache* c, int Np, int nl, const uint64_t* ekeys, const int* blockoff, rc_snaprec* out,
                                hipStream_t stream);
-hipError_t rc_launch_snap_restore(const rc_dcache* c, int Np, int nl, const uint64_t* ekeys, const rc_snaprec* rec, int count, int verify,
-                                  uint64_t dormant, hipStream_t stream);
+// count (device): records at rec; max_count: most records the grid covers
+hipError_t rc_launch_snap_restore(const rc_dcache* c, int Np, int nl, const uint64_t* ekeys, const rc_snaprec* rec, const int* count,
+                                  int max_count, int verify, uint64_t dormant, hipStream_t stream);
+// exclusive prefix of n ints in place (one block), total into *total (device)
+hipError_t rc_launch_scan_blocks(int* cnt, int n, int* total, hipStream_t stream);

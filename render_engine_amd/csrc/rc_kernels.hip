@@ -180,31 +180,35 @@ __global__ void __launch_bounds__(RC_TB) k_trace2(rc_dscene s, rc_dstate st, rc_
     }
 }
 
-// verification decision per pending entry (cuda/poc_gpu.cu:532-555)
+// verification decision per pending entry (cuda/poc_gpu.cu:532-555); entries still pending are counted
+// per wave (one atomic per wave instead of one per entry)
 __global__ void __launch_bounds__(RC_TB) k_decide(rc_dscene s, rc_dcache c, rc_params p) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= p.Np) return;
-    if (!c.pending[i]) return;
-    const long long* r4 = c.rstat + 4 * s.region[i];
-    const float isc = 1.0f / RC_FX_STAT;
-    float cnt = (float)r4[3] * isc;
-    bool done = false;
-    if (cnt >= 8.f) {
-        float mean = ((float)r4[0] * isc) / cnt;
-        float se = sqrtf(fmaxf(((float)r4[1] * isc) / cnt - mean * mean, 0.f) / cnt);
-        float base = ((float)r4[2] * isc) / cnt;
-        float rel = mean / base;
-        if (fabsf(mean) > 2.f * se && fabsf(rel) > 0.03f) {
-            float tgt = fminf(fmaxf(0.5f / fabsf(rel), 1.f) * p.rays, p.nmax);
-            c.n[i] = fminf(c.n[i], tgt);
-            c.pending[i] = 0;
-            done = true;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    bool still = false;
+    if (i < p.Np && c.pending[i]) {
+        const long long* r4 = c.rstat + 4 * s.region[i];
+        const float isc = 1.0f / RC_FX_STAT;
+        float cnt = (float)r4[3] * isc;
+        bool done = false;
+        if (cnt >= 8.f) {
+            float mean = ((float)r4[0] * isc) / cnt;
+            float se = sqrtf(fmaxf(((float)r4[1] * isc) / cnt - mean * mean, 0.f) / cnt);
+            float base = ((float)r4[2] * isc) / cnt;
+            float rel = mean / base;
+            if (fabsf(mean) > 2.f * se && fabsf(rel) > 0.03f) {
+                float tgt = fminf(fmaxf(0.5f / fabsf(rel), 1.f) * p.rays, p.nmax);
+                c.n[i] = fminf(c.n[i], tgt);
+                c.pending[i] = 0;
+                done = true;
+            }
+        }
+        if (!done) {
+            if (p.pend_left <= 1) c.pending[i] = 0;
+            else still = true;
         }
     }
-    if (!done) {
-        if (p.pend_left <= 1) c.pending[i] = 0;
-        else atomicAdd(&c.npend[1], 1);
-    }
+    const unsigned long long b = __ballot(still);
+    if ((threadIdx.x & 63) == 0 && b) atomicAdd(&c.npend[1], (int)__popcll(b));
 }
 
 #define RC_UT 64   // entries per k_update block
@@ -365,12 +369,16 @@ __global__ void __launch_bounds__(RC_PRB) k_prsum(const long long* __restrict__ 
     }
 }
 
-// restored snapshot: full confidence, optionally verified (cuda/poc_gpu.cu:612-617)
+// restored snapshot: full confidence, optionally verified (cuda/poc_gpu.cu:612-617); with verify every
+// entry becomes pending, so the pending count is Np
 __global__ void __launch_bounds__(RC_TB) k_restored(rc_dcache c, int Np, int nl, float nmax, uint64_t dormant, int verify) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= Np) return;
     c.n[i] = nmax; c.age[i] = 1e4f; c.pending[i] = 0;
-    if (verify) { c.pending[i] = 1; c.c0[i] = rc_est_of(c.Lc, i, nl, dormant); atomicAdd(&c.npend[1], 1); }
+    if (verify) {
+        c.pending[i] = 1; c.c0[i] = rc_est_of(c.Lc, i, nl, dormant);
+        if (i == 0) c.npend[1] = Np;
+    }
 }
 
 // light-intensity change a -> b on channel l: linear rescale (cuda/poc_gpu.cu:637-640)
@@ -379,39 +387,72 @@ __global__ void __launch_bounds__(RC_TB) k_rescale(rc_dcache c, int Np, int nl, 
     if (i < Np) c.Lc[(int64_t)nl * i + l] *= ratio;
 }
 
-// Event invalidation from the transit-grid banks (cuda/poc_gpu.cu:642-660).
-// kind 0: emitter appeared -> reset history; 1: occluder -> verification; 2: stale marking only.
-__global__ void __launch_bounds__(RC_TB) k_touched(rc_dcache c, int Np, int nl, const int* V, int nv, int kind, uint64_t dormant) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= Np) return;
-    bool t = false;
-    for (int j = 0; j < nv && !t; ++j) {
-        int v = V[j];
-        int64_t o = (int64_t)(v >> 5) * Np + i;
-        t = ((c.cur[o] | c.prev[o]) >> (v & 31)) & 1u;
+// Per-block counts of the event kernels (blkcnt[block]) added in block order by one block into
+// *out (= the previous *out when add, else overwritten): no same-address atomics per entry.
+__device__ __forceinline__ void rc_block_count(bool flag, int* blkcnt) {
+    __shared__ int s_c[RC_TB / 64];
+    const unsigned long long b = __ballot(flag);
+    if ((threadIdx.x & 63) == 0) s_c[threadIdx.x >> 6] = __popcll(b);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int t = 0;
+        for (int k = 0; k < RC_TB / 64; ++k) t += s_c[k];
+        blkcnt[blockIdx.x] = t;
     }
-    if (!t) return;
-    if (kind == 0) { c.n[i] = 0.f; c.age[i] = 0.f; }
-    else if (kind == 1) {
-        c.pending[i] = 1;
-        c.c0[i] = rc_est_of(c.Lc, i, nl, dormant);
-        atomicAdd(&c.npend[1], 1);
+}
+#define RC_CSB 1024
+__global__ void __launch_bounds__(RC_CSB) k_count_sum(const int* __restrict__ blkcnt, int nblk, int* out, int add) {
+    __shared__ long long s_w[RC_CSB / 64];
+    long long v = 0;
+    for (int j = threadIdx.x; j < nblk; j += RC_CSB) v += blkcnt[j];
+    v = rc_wave_sum_i64(v);
+    if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        long long t = 0;
+        for (int k = 0; k < RC_CSB / 64; ++k) t += s_w[k];
+        *out = (int)(add ? *out + t : t);
     }
-    if (kind >= 1 && dormant) c.stale[i] |= dormant;
 }
 
-// light wakes from dormancy (cuda/poc_gpu.cu:663-674)
+// Event invalidation from the transit-grid banks (cuda/poc_gpu.cu:642-660): an entry is touched when
+// an event voxel's bit is set in cur | prev. The voxel list is given as a per-word mask vm (bits of the
+// nw transit words), so each entry reads its nw words of both banks once (SoA, coalesced).
+// kind 0: emitter appeared -> reset history; 1: occluder -> verification (count of pending entries into
+// blkcnt); 2: stale marking only.
+__global__ void __launch_bounds__(RC_TB) k_touched(rc_dcache c, int Np, int nl, rc_vmask vm, int nw, int kind, uint64_t dormant,
+                                                  int* blkcnt) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    bool t = false;
+    if (i < Np) {
+        uint32_t hv = 0;
+#pragma unroll
+        for (int w = 0; w < RC_MAX_NW; ++w)
+            if (w < nw && vm.w[w]) hv |= (c.cur[(int64_t)w * Np + i] | c.prev[(int64_t)w * Np + i]) & vm.w[w];
+        t = hv != 0;
+        if (t) {
+            if (kind == 0) { c.n[i] = 0.f; c.age[i] = 0.f; }
+            else if (kind == 1) { c.pending[i] = 1; c.c0[i] = rc_est_of(c.Lc, i, nl, dormant); }
+            if (kind >= 1 && dormant) c.stale[i] |= dormant;
+        }
+    }
+    if (kind == 1) rc_block_count(t, blkcnt);
+}
+
+// light wakes from dormancy (cuda/poc_gpu.cu:663-674); verified entries counted per wave
 __global__ void __launch_bounds__(RC_TB) k_wake(rc_dcache c, int Np, int nl, int l, int verify, uint64_t dormant) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= Np) return;
-    if ((c.stale[i] >> l) & 1ull) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    bool pend = false;
+    if (i < Np && ((c.stale[i] >> l) & 1ull)) {
         if (verify) {
             c.pending[i] = 1;
             c.c0[i] = rc_est_of(c.Lc, i, nl, dormant);
-            atomicAdd(&c.npend[1], 1);
+            pend = true;
         } else { c.Lc[(int64_t)nl * i + l] = 0.f; c.n[i] = 0.f; c.age[i] = 0.f; }
         c.stale[i] &= ~(1ull << l);
     }
+    const unsigned long long b = __ballot(pend);
+    if ((threadIdx.x & 63) == 0 && b) atomicAdd(&c.npend[1], (int)__popcll(b));
 }
 
 // informed upper bound: shrink history by the true reference change (cuda/poc_gpu.cu:676-681)
@@ -423,34 +464,51 @@ __global__ void __launch_bounds__(RC_TB) k_oracle_method(rc_dcache c, int Np, co
 }
 
 // Dependency-targeted invalidation (resolve-time, rtxgi_bench.patch:1028-1050): an entry whose
-// recorded dependencies intersect the event (changed lights, or event voxels in the transit
-// banks) restarts from zero; an entry with no recorded dependency is reset too (unknown ->
-// conservative). Returns the number of invalidated entries in c.npend[1].
-// ga / gb (may be NULL): per-entry gradient-support masks of the two states, used instead of
-// the recorded light masks (an entry with no support does not depend on any light)
+// recorded dependencies intersect the event (changed lights inv_lights | *inv_extra, or event voxels
+// vm in the transit banks) restarts from zero; an entry with no recorded dependency is reset too
+// (unknown -> conservative). ga / gb (may be NULL): per-entry gradient-support masks of the two
+// states, used instead of the recorded light masks (an entry with no support depends on no light).
+// A block owns RC_TB consecutive entries: one thread per entry decides (light masks, then the transit
+// words only when needed), then the block zeroes the light rows of its reset entries over the tile's
+// contiguous span of Lc in flat order (coalesced stores). Reset entries are counted per block.
 __global__ void __launch_bounds__(RC_TB) k_invalidate(rc_dcache c, int Np, int nl, uint64_t inv_lights,
-                                                     const int* V, int nv, int nw, const uint64_t* ga, const uint64_t* gb) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    int hit = 0;
-    if (i < Np) {
-        uint64_t ld = ga ? (ga[i] | gb[i]) : (c.lcur[i] | c.lprev[i]);
-        bool known = ga != nullptr || ld != 0ull;
-        for (int w = 0; w < nw && !known; ++w) known = (c.cur[(int64_t)w * Np + i] | c.prev[(int64_t)w * Np + i]) != 0u;
-        bool t = !known || (ld & inv_lights) != 0ull;
-        for (int j = 0; j < nv && !t; ++j) {
-            int v = V[j];
-            int64_t o = (int64_t)(v >> 5) * Np + i;
-            t = ((c.cur[o] | c.prev[o]) >> (v & 31)) & 1u;
+                                                     const unsigned long long* inv_extra, rc_vmask vm, int nw, const uint64_t* ga,
+                                                     const uint64_t* gb, int* blkcnt) {
+    __shared__ uint8_t s_hit[RC_TB];
+    const int t = threadIdx.x, base = blockIdx.x * RC_TB, i = base + t;
+    const uint64_t inv = inv_lights | (inv_extra ? (uint64_t)*inv_extra : 0ull);
+    uint32_t vany = 0;
+#pragma unroll
+    for (int w = 0; w < RC_MAX_NW; ++w) vany |= w < nw ? vm.w[w] : 0u;
+    bool hit = false;
+    if (i < Np && (inv != 0ull || vany != 0u)) {   // an event with no changed light and no voxel resets nothing
+        const uint64_t ld = ga ? (ga[i] | gb[i]) : (c.lcur[i] | c.lprev[i]);
+        const bool known_l = ga != nullptr || ld != 0ull;
+        hit = (ld & inv) != 0ull;
+        if (!hit && (!known_l || vany)) {
+            uint32_t any = 0, hv = 0;
+#pragma unroll
+            for (int w = 0; w < RC_MAX_NW; ++w) {
+                if (w >= nw) break;
+                const uint32_t x = c.cur[(int64_t)w * Np + i] | c.prev[(int64_t)w * Np + i];
+                any |= x; hv |= x & vm.w[w];
+            }
+            hit = (!known_l && any == 0u) || hv != 0u;
         }
-        if (t) {
-            c.n[i] = 0.f; c.age[i] = 0.f;
-            float* row = c.Lc + (int64_t)nl * i;
-            for (int l = 0; l < nl; ++l) row[l] = 0.f;
-            hit = 1;
-        }
+        if (hit) { c.n[i] = 0.f; c.age[i] = 0.f; }
     }
-    long long h = rc_wave_sum_i64((long long)hit);
-    if ((threadIdx.x & 63) == 0 && h) atomicAdd(&c.npend[1], (int)h);
+    s_hit[t] = hit ? 1 : 0;
+    rc_block_count(hit, blkcnt);      // its barrier also publishes s_hit
+    if (!__syncthreads_or(hit)) return;
+    const int T = min(RC_TB, Np - base);
+    float* __restrict__ L = c.Lc + (int64_t)base * nl;
+    const int nel = T * nl, dq = RC_TB / nl, dr = RC_TB - dq * nl;
+    int j = t / nl, l = t - (t / nl) * nl;
+    for (int e = t; e < nel; e += RC_TB) {
+        if (s_hit[j]) L[e] = 0.f;
+        j += dq; l += dr;
+        if (l >= nl) { l -= nl; ++j; }
+    }
 }
 
 // whole-cache flush (event to a new state, rtxgi_bench.patch:363-366)
@@ -486,28 +544,64 @@ __global__ void __launch_bounds__(RC_TB) k_snap_pack(rc_dcache c, int Np, int nl
     r.index = (uint32_t)i; r.n = c.n[i]; r.key = ekeys[i]; r.radiance = L; r.age = c.age[i];
     out[slot] = r;
 }
-// restore after a flush: record j's entry (if its key matches) gets its radiance in channel 0,
-// its history and age; verify opens the verification window on it
-__global__ void __launch_bounds__(RC_TB) k_snap_restore(rc_dcache c, int Np, int nl, const uint64_t* ekeys, const rc_snaprec* rec, int count,
-                                                       int verify, uint64_t dormant) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= count) return;
-    const rc_snaprec r = rec[j];
-    const int i = (int)r.index;
-    if (r.index >= (uint32_t)Np || ekeys[i] != r.key) return;
-    float* row = c.Lc + (int64_t)nl * i;
-    row[0] = r.radiance;
-    for (int l = 1; l < nl; ++l) row[l] = 0.f;
-    c.n[i] = r.n; c.age[i] = r.age; c.pending[i] = 0;
-    if (verify) { c.pending[i] = 1; c.c0[i] = rc_est_of(c.Lc, i, nl, dormant); atomicAdd(&c.npend[1], 1); }
+// Exclusive prefix of n per-block counts in place, their total into *total (one block, any n): the
+// record offsets of the snapshot packing stay on the device.
+#define RC_SCB 1024
+__global__ void __launch_bounds__(RC_SCB) k_scan_blocks(int* cnt, int n, int* total) {
+    __shared__ int s_w[RC_SCB / 64];
+    __shared__ int s_carry;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (threadIdx.x == 0) s_carry = 0;
+    __syncthreads();
+    for (int base = 0; base < n; base += RC_SCB) {
+        const int j = base + threadIdx.x;
+        const int v = j < n ? cnt[j] : 0;
+        int x = v;   // inclusive scan over the wave
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const int y = __shfl_up(x, o, 64);
+            if (lane >= o) x += y;
+        }
+        if (lane == 63) s_w[wv] = x;
+        __syncthreads();
+        int before = s_carry;
+        for (int k = 0; k < wv; ++k) before += s_w[k];
+        if (j < n) cnt[j] = before + x - v;
+        __syncthreads();
+        if (threadIdx.x == RC_SCB - 1) s_carry = before + x;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *total = s_carry;
 }
 
+// restore after a flush (rows already zero): record j < *countp, if its key matches its entry, gives the
+// entry its radiance in channel 0, its history and age; verify opens the verification window on it
+// (counted per wave). The record count is read on the device (the grid covers the most records).
+__global__ void __launch_bounds__(RC_TB) k_snap_restore(rc_dcache c, int Np, int nl, const uint64_t* ekeys, const rc_snaprec* rec,
+                                                       const int* countp, int verify, uint64_t dormant) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    bool pend = false;
+    if (j < *countp) {
+        const rc_snaprec r = rec[j];
+        const int i = (int)r.index;
+        if (r.index < (uint32_t)Np && ekeys[i] == r.key) {
+            c.Lc[(int64_t)nl * i] = r.radiance;
+            c.n[i] = r.n; c.age[i] = r.age; c.pending[i] = 0;
+            if (verify) { c.pending[i] = 1; c.c0[i] = rc_est_of(c.Lc, i, nl, dormant); pend = true; }
+        }
+    }
+    const unsigned long long b = __ballot(pend);
+    if ((threadIdx.x & 63) == 0 && b) atomicAdd(&c.npend[1], (int)__popcll(b));
+}
+
+// Whole-cache flush (event to a new state, rtxgi_bench.patch:363-366): the light rows as 16-byte stores
+// over the whole Lc buffer (hipMalloc'd, so 16-byte aligned), then history and age.
 __global__ void __launch_bounds__(RC_TB) k_flush(rc_dcache c, int Np, int nl) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= Np) return;
-    c.n[i] = 0.f; c.age[i] = 0.f;
-    float* row = c.Lc + (int64_t)nl * i;
-    for (int l = 0; l < nl; ++l) row[l] = 0.f;
+    const int64_t nf = (int64_t)Np * nl, n4 = nf >> 2;
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n4) ((float4*)c.Lc)[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+    else if (k < n4 + Np) { const int64_t i = k - n4; c.n[i] = 0.f; c.age[i] = 0.f; }
+    if (k < (nf & 3)) c.Lc[(n4 << 2) + k] = 0.f;
 }
 
 // Relative RMS of the estimate against the converged reference (cuda/poc_gpu.cu:683-696), in two
@@ -1179,8 +1273,10 @@ hipError_t rc_launch_rescale(const rc_dcache* c, int Np, int nl, int l, float ra
     hipLaunchKernelGGL(k_rescale, dim3(rc_grid(Np)), dim3(RC_TB), 0, stream, *c, Np, nl, l, ratio);
     return hipGetLastError();
 }
-hipError_t rc_launch_touched(const rc_dcache* c, int Np, int nl, const int* V, int nv, int kind, uint64_t dormant, hipStream_t stream) {
-    hipLaunchKernelGGL(k_touched, dim3(rc_grid(Np)), dim3(RC_TB), 0, stream, *c, Np, nl, V, nv, kind, dormant);
+hipError_t rc_launch_touched(const rc_dcache* c, int Np, int nl, rc_vmask vm, int nw, int kind, uint64_t dormant, hipStream_t stream) {
+    const int nblk = rc_grid(Np);
+    hipLaunchKernelGGL(k_touched, dim3(nblk), dim3(RC_TB), 0, stream, *c, Np, nl, vm, nw, kind, dormant, c->blkcnt);
+    if (kind == 1) hipLaunchKernelGGL(k_count_sum, dim3(1), dim3(RC_CSB), 0, stream, (const int*)c->blkcnt, nblk, c->npend + 1, 1);
     return hipGetLastError();
 }
 hipError_t rc_launch_wake(const rc_dcache* c, int Np, int nl, int l, int verify, uint64_t dormant, hipStream_t stream) {
@@ -1191,13 +1287,16 @@ hipError_t rc_launch_oracle_method(const rc_dcache* c, int Np, const float* rold
     hipLaunchKernelGGL(k_oracle_method, dim3(rc_grid(Np)), dim3(RC_TB), 0, stream, *c, Np, rold, rnew);
     return hipGetLastError();
 }
-hipError_t rc_launch_invalidate(const rc_dcache* c, int Np, int nl, uint64_t inv_lights, const int* V, int nv, int nw,
-                                const uint64_t* ga, const uint64_t* gb, hipStream_t stream) {
-    hipLaunchKernelGGL(k_invalidate, dim3(rc_grid(Np)), dim3(RC_TB), 0, stream, *c, Np, nl, inv_lights, V, nv, nw, ga, gb);
+hipError_t rc_launch_invalidate(const rc_dcache* c, int Np, int nl, uint64_t inv_lights, const unsigned long long* inv_extra, rc_vmask vm,
+                                int nw, const uint64_t* ga, const uint64_t* gb, int* count, hipStream_t stream) {
+    const int nblk = rc_grid(Np);
+    hipLaunchKernelGGL(k_invalidate, dim3(nblk), dim3(RC_TB), 0, stream, *c, Np, nl, inv_lights, inv_extra, vm, nw, ga, gb, c->blkcnt);
+    hipLaunchKernelGGL(k_count_sum, dim3(1), dim3(RC_CSB), 0, stream, (const int*)c->blkcnt, nblk, count, 0);
     return hipGetLastError();
 }
 hipError_t rc_launch_flush(const rc_dcache* c, int Np, int nl, hipStream_t stream) {
-    hipLaunchKernelGGL(k_flush, dim3(rc_grid(Np)), dim3(RC_TB), 0, stream, *c, Np, nl);
+    const long long n = (((long long)Np * nl) >> 2) + Np;
+    hipLaunchKernelGGL(k_flush, dim3(rc_grid(n)), dim3(RC_TB), 0, stream, *c, Np, nl);
     return hipGetLastError();
 }
 hipError_t rc_launch_error(const rc_dcache* c, int Np, int nl, const float* ref, const uint8_t* valid, uint64_t dormant, hipStream_t stream) {
@@ -1330,9 +1429,13 @@ hipError_t rc_launch_snap_pack(const rc_dcache* c, int Np, int nl, const uint64_
     hipLaunchKernelGGL(k_snap_pack, dim3(rc_grid(Np)), dim3(RC_TB), 0, stream, *c, Np, nl, ekeys, blockoff, out);
     return hipGetLastError();
 }
-hipError_t rc_launch_snap_restore(const rc_dcache* c, int Np, int nl, const uint64_t* ekeys, const rc_snaprec* rec, int count, int verify,
-                                  uint64_t dormant, hipStream_t stream) {
-    if (count <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_snap_restore, dim3(rc_grid(count)), dim3(RC_TB), 0, stream, *c, Np, nl, ekeys, rec, count, verify, dormant);
+hipError_t rc_launch_snap_restore(const rc_dcache* c, int Np, int nl, const uint64_t* ekeys, const rc_snaprec* rec, const int* count,
+                                  int max_count, int verify, uint64_t dormant, hipStream_t stream) {
+    if (max_count <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_snap_restore, dim3(rc_grid(max_count)), dim3(RC_TB), 0, stream, *c, Np, nl, ekeys, rec, count, verify, dormant);
+    return hipGetLastError();
+}
+hipError_t rc_launch_scan_blocks(int* cnt, int n, int* total, hipStream_t stream) {
+    hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(RC_SCB), 0, stream, cnt, n, total);
     return hipGetLastError();
 }

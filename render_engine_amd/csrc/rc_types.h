@@ -108,7 +108,12 @@ typedef struct rc_dcache {
     int32_t* npend;              // [2]
     double* err;                 // [3] squared error, reference, valid count of the error pass
     double* errpart;             // [3][ceil(Np / 256)] per-block partials of the error pass
+    int32_t* blkcnt;             // [ceil(Np / 256) + 1] per-block counts of the event kernels, then their total
 } rc_dcache;
+
+// Event voxels of the transit grid as a per-word bit mask (bit v & 31 of word v >> 5), built on the host
+// from the event's voxel list (cuda/poc_gpu.cu:899-911 lists; the kernels test cur | prev against it).
+typedef struct rc_vmask { uint32_t w[RC_MAX_NW]; } rc_vmask;
 
 // Per-frame parameters (cuda/poc_gpu.cu:411-422).
 typedef struct rc_params {

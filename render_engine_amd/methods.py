@@ -7,10 +7,11 @@ from ._ffi import Method
 
 def make(mode=0, nmax=16.0, budget=1.0, deps=False, verify=False, oracle=False, track_every=1, rescale=False, resp_mask=0,
          nmax_resp=8.0, ddgi=False, ddgi_h=0.97, ddgi_thr=0.25, ddgi_clamp=True, memory=False, ramp=False,
-         wake_verify=False, snaps=False, snap_verify=False, flush=False, targeted=False, compact=False, grad_masks=0):
+         wake_verify=False, snaps=False, snap_verify=False, flush=False, targeted=False, compact=False, grad_masks=0,
+         eventboost=0):
     return Method(mode, nmax, budget, int(deps), int(verify), int(oracle), track_every, int(rescale), resp_mask, nmax_resp,
                   int(ddgi), ddgi_h, ddgi_thr, int(ddgi_clamp), int(memory), int(ramp), int(wake_verify), int(snaps),
-                  int(snap_verify), int(flush), int(targeted), int(compact), int(grad_masks))
+                  int(snap_verify), int(flush), int(targeted), int(compact), int(grad_masks), int(eventboost))
 
 
 # ---- the four methods of the north-star interface
@@ -19,15 +20,16 @@ def plain(nmax=32.0, budget=1.0):
     return make(mode=0, nmax=nmax, budget=budget)
 
 
-def flush_snapshots(nmax=32.0, budget=1.0):
-    """whole-cache flush on a new state, snapshot restore on a revisited one (rtxgi_bench.patch:323-366)"""
-    return make(mode=0, nmax=nmax, budget=budget, flush=True, snaps=True)
+def flush_snapshots(nmax=32.0, budget=1.0, eventboost=30):
+    """whole-cache flush on a new state, snapshot restore on a revisited one (rtxgi_bench.patch:323-366), and the
+    post-event history-cap ramp over `eventboost` frames (windows/run.sh:100: snapshots 1, eventboost 30, eventclear 1)"""
+    return make(mode=0, nmax=nmax, budget=budget, flush=True, snaps=True, eventboost=eventboost)
 
 
-def flush_snapshots_compact(nmax=32.0, budget=1.0):
+def flush_snapshots_compact(nmax=32.0, budget=1.0, eventboost=30):
     """flush on a new state, restore from 24-byte records of the entries with samples on a revisited
-    one (rtxgi_bench.patch:1289-1345, compactsnap 1)"""
-    return make(mode=0, nmax=nmax, budget=budget, flush=True, snaps=True, compact=True)
+    one (rtxgi_bench.patch:1289-1345, compactsnap 1), with the post-event ramp (windows/run.sh:101)"""
+    return make(mode=0, nmax=nmax, budget=budget, flush=True, snaps=True, compact=True, eventboost=eventboost)
 
 
 def targeted(nmax=32.0, budget=1.0, track_every=1):

@@ -7,6 +7,9 @@
   than half of the converged radiance, after a k x k box blur with k = 9 at 960 px width
   scaled with the width (windows/sondes.py:30-47). Region bias = sum(est)/sum(ref) - 1
   over the region (rtxgi_bench.patch:1258-1272).
+* Affected region of a geometry event (an object inserted or moved): pixels whose blurred converged
+  radiance changes by more than 25 % relative to the larger of the two states (the moving panel's
+  probe, windows/sondes.py:41,44).
 """
 import numpy as np
 
@@ -52,6 +55,14 @@ def light_off_region(ref_on, ref_off, thr=0.5):
     b_on, b_off = box_blur(ref_on, k), box_blur(ref_off, k)
     part = np.clip(b_on - b_off, 0, None) / np.maximum(b_on, 1e-6)
     return part > thr
+
+
+def panel_region(ref_a, ref_b, thr=0.25):
+    """pixels where |a - b| / max(a, b) > thr after the blur (the moving panel's probe)"""
+    H, W = ref_a.shape
+    k = blur_kernel(W)
+    a, b = box_blur(ref_a, k), box_blur(ref_b, k)
+    return np.abs(a - b) / np.maximum(np.maximum(a, b), 1e-6) > thr
 
 
 def region_bias(est, ref, region):

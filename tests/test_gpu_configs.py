@@ -174,6 +174,40 @@ def test_query_full_shape_bands(torch_gpu, name):
     _say(f"{name}: camera pass bands bit-exact ({W}x{H} x {cam.spp} spp)")
 
 
+@pytest.mark.parametrize("name", list(SHAPES))
+def test_radiance_vs_fp64_full_shape(torch_gpu, name):
+    """FP32 cache channels, resolved radiance and the camera image (row bands) after three frames of the
+    plain cache within 1e-4 relative RMS of the FP64 twin replaying the same rays and hits"""
+    torch = torch_gpu
+    s, osc, flags = _shape(torch_gpu, name)
+    _fill(s, osc, flags)
+    m = rea.methods.plain()
+    cache = rea.Cache(s, m)
+    orun = po.OracleRunner(osc, m)
+    cache.begin(0, 44)
+    orun.begin(0, 44)
+    twin = po.FP64Twin(osc, orun, 0)
+    for _ in range(3):
+        cache.frame(0)
+        twin.frame()
+    assert _eq(cache.get("LC"), orun.get("LC"))[0]
+    rl = po.rel_rms(cache.get("LC"), twin.L64)
+    rr = po.rel_rms(cache.get("R"), twin.R64())
+    cam = s.camera()
+    W, H = cam.W, cam.H
+    img = torch.zeros(W * H, dtype=torch.float32, device="cuda")
+    cache.render(img, cam, seed=45)
+    got = img.cpu().numpy()
+    a, b = [], []
+    for k in range(4):
+        r0 = k * H // 4
+        b.append(twin.image64(cam, 45, 3, r0 * W, (r0 + 8) * W)[r0 * W:(r0 + 8) * W])
+        a.append(got[r0 * W:(r0 + 8) * W])
+    ri = po.rel_rms(np.concatenate(a), np.concatenate(b))
+    _say(f"{name} fp64 twin: LC {rl:.2e}, R {rr:.2e}, image {ri:.2e}")
+    assert rl < 1e-4 and rr < 1e-4 and ri < 1e-4, (rl, rr, ri)
+
+
 # ---------------------------------------------------------------- adjoint at 64 emission scalars
 def test_adjoint_64_lights_4096_materials(torch_gpu):
     """63 lamps + sun = 64 emission scalars (the sun on bit 63 alone), 4,096 materials, 3 bounces x 4 rays:

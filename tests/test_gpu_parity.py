@@ -254,48 +254,31 @@ def test_query_staged_equals_fused(torch_gpu, which, request):
     cache.set_render_mode(rea.RENDER_FUSED)
 
 
-def test_radiance_vs_fp64(torch_gpu, room):
-    """FP32 cache radiance after several frames within 1e-4 relative RMS of the FP64 twin
-    driven by the same rays and hits."""
-    scene, osc = room
-    i = scene.info
+@pytest.mark.parametrize("which", ["room", "street"])
+def test_radiance_vs_fp64(torch_gpu, which, request):
+    """FP32 cache channels, resolved radiance and camera image after several frames within 1e-4
+    relative RMS of the FP64 twin driven by the same rays and hits (the FP32 state bit-equal to the
+    oracle's first)."""
+    torch = torch_gpu
+    scene, osc = request.getfixturevalue(which)
     m = rea.methods.plain()
     cache = rea.Cache(scene, m)
     cache.begin(0, 21)
-    s = osc.states[0]
-    E64 = np.zeros(i.Np * i.nl)
-    po.lib.orc_k_build_f64(C.byref(s), 1, po.ptr(E64))
-    # FP32 oracle state that the GPU must equal, plus the FP64 twin
     orun = po.OracleRunner(osc, m)
     orun.begin(0, 21)
-    L64 = orun.get("LC").astype(np.float64)
-    kmax = 4
+    twin = po.FP64Twin(osc, orun, 0)
     for f in range(8):
-        n_before = orun.get("N")
-        # record this frame's hits with a standalone oracle trace2 on a copy of the runner state
-        hits = np.full(i.Np * kmax, -2, np.int32)
-        c = po.lib.orc_runner_cache(orun.h).contents
-        p = po.OrcParams()
-        p.Np, p.NR, p.mode, p.budget, p.nmax, p.seed, p.frame = i.Np, i.NR, 0, 1.0, m.nmax, 21, f
-        p.nmax_resp, p.rays, p.nl = m.nmax_resp, 1.0, i.nl
-        mcopy = np.zeros(i.Np * i.nl, np.float32)   # SoA sample means of this frame
-        kk = np.zeros(i.Np, np.int32)
-        tmp = po.OrcCache()
-        for fld, _ in po.OrcCache._fields_:
-            setattr(tmp, fld, getattr(c, fld))
-        tmp.m = mcopy.ctypes.data
-        tmp.kk = kk.ctypes.data
-        po.lib.orc_k_trace2(C.byref(s), 1, 0, C.byref(tmp), C.byref(p), None, po.ptr(hits), kmax)
-        m64 = np.zeros(i.Np * i.nl)
-        po.lib.orc_k_trace2_f64(C.byref(s), po.ptr(E64), po.ptr(L64), po.ptr(kk), po.ptr(hits), kmax, 0, po.ptr(m64))
-        out64 = np.zeros_like(L64)
-        po.lib.orc_k_update_f64(C.byref(s), po.ptr(n_before), po.ptr(kk), C.byref(p), po.ptr(L64), po.ptr(m64), po.ptr(out64))
-        L64 = out64
         cache.frame(0)
-        orun.frame(0)
-    g = cache.get("LC").astype(np.float64)
-    rel = np.sqrt(np.mean((g - L64) ** 2)) / np.mean(np.abs(L64))
-    assert rel < 1e-4, f"relative RMS vs FP64 = {rel:.3e}"
+        twin.frame()
+    assert np.array_equal(cache.get("LC").view(np.uint32), orun.get("LC").view(np.uint32))
+    rl = po.rel_rms(cache.get("LC"), twin.L64)
+    rr = po.rel_rms(cache.get("R"), twin.R64())
+    cam = scene.camera()
+    img = torch.zeros(cam.W * cam.H, dtype=torch.float32, device="cuda")
+    cache.render(img, cam, seed=31)
+    ri = po.rel_rms(img.cpu().numpy(), twin.image64(cam, 31, 8, 0, cam.W * cam.H))
+    print(f"fp64 twin {which}: LC {rl:.2e}, R {rr:.2e}, image {ri:.2e}", flush=True)
+    assert rl < 1e-4 and rr < 1e-4 and ri < 1e-4, (rl, rr, ri)
 
 
 def test_query_any_spp(torch_gpu, room):

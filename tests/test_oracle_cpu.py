@@ -336,6 +336,14 @@ def test_scenario_parsing_and_region_metric():
     reg = sc.light_off_region(on, off)
     assert reg[:, :18].all() and not reg[:, 32:].any()
     assert sc.frames_to_below([0.3, 0.1, 0.04, 0.02]) == 2
+    # geometry event: |a - b| / max(a, b) > 0.25 after the blur, both signs (darkened and brightened)
+    a = np.ones((40, 60)); b = np.ones((40, 60)); b[:, :20] = 0.5; b[:, 40:] = 2.0
+    pan = sc.panel_region(a, b)
+    assert pan[:, :15].all() and pan[:, 45:].all() and not pan[:, 25:35].any()
+    k = sc.blur_kernel(60)
+    ba, bb = sc.box_blur(a, k), sc.box_blur(b, k)
+    naive = np.array([[abs(x - y) / max(x, y, 1e-6) > 0.25 for x, y in zip(ra, rb)] for ra, rb in zip(ba, bb)])
+    assert np.array_equal(pan, naive)
 
 
 def test_resolved_radiance(room):
