@@ -439,6 +439,9 @@ extern "C" rc_status rc_scene_reference(rc_scene* s, int K, int warm, int block,
 }
 
 // ------------------------------------------------------------------------------------ cache
+// row pitch (entries) of the SoA sample means: a multiple of 64, so every update tile's rows are aligned
+static size_t mpitch_of(size_t Np) { return (Np + 63) & ~(size_t)63; }
+
 struct rc_cache {
     rc_scene* S = nullptr;
     rc_method M{};
@@ -527,7 +530,7 @@ extern "C" rc_status rc_cache_create(rc_scene* s, const rc_method* m, rc_cache**
     const size_t Np = H.Np, nl = H.nl, nw = H.nw;
     rc_dcache& d = c->c;
 #define CK2(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) { delete c; return fail(RC_ERR_HIP, "%s: %s", #x, hipGetErrorString(e_)); } } while (0)
-    CK2(dalloc(&d.Lc, nl * Np, c->owned)); CK2(dalloc(&d.Lc2, nl * Np, c->owned)); CK2(dalloc(&d.m, nl * Np, c->owned));
+    CK2(dalloc(&d.Lc, nl * Np, c->owned)); CK2(dalloc(&d.Lc2, nl * Np, c->owned)); CK2(dalloc(&d.m, nl * mpitch_of(Np), c->owned));
     CK2(dalloc(&d.n, Np, c->owned)); CK2(dalloc(&d.c0, Np, c->owned)); CK2(dalloc(&d.kk, Np, c->owned));
     CK2(dalloc(&c->Rb[0], Np, c->owned)); CK2(dalloc(&c->Rb[1], Np, c->owned));
     d.R = c->Rb[0];
@@ -545,6 +548,7 @@ extern "C" rc_status rc_cache_create(rc_scene* s, const rc_method* m, rc_cache**
     CK2(dalloc(&d.sumpr, 2, c->owned)); CK2(dalloc(&d.prpart, (size_t)(Np + 63) / 64, c->owned)); CK2(dalloc(&d.npend, 2, c->owned)); CK2(dalloc(&d.err, 3, c->owned));
     CK2(dalloc(&d.errpart, 3 * (size_t)rc_grid_threads(Np) / 256, c->owned));   // 256-thread blocks of the error pass
     CK2(dalloc(&d.blkcnt, (size_t)rc_grid_threads(Np) / 256 + 1, c->owned));
+    CK2(dalloc(&d.rsum, Np, c->owned));
     CK2(dalloc(&c->d_counters, 2, c->owned));
     for (auto& e : c->ev) CK2(hipEventCreate(&e));
     CK2(hipStreamCreate(&c->rs));
@@ -616,7 +620,7 @@ extern "C" rc_status rc_cache_begin(rc_cache* c, int state0, uint32_t seed) {
     p.Np = (int)Np; p.NR = H.NR; p.mode = c->M.mode; p.budget = c->M.budget; p.nmax = c->nmax; p.seed = seed;
     p.deps = c->M.deps; p.resp_mask = c->M.resp_mask; p.nmax_resp = c->M.nmax_resp;
     p.ddgi = c->M.ddgi; p.ddgi_h = c->M.ddgi_h; p.ddgi_thr = c->M.ddgi_thr; p.ddgi_clamp = c->M.ddgi_clamp;
-    p.ramp = c->M.ramp; p.rays = 1.f; p.nl = (int)nl;
+    p.ramp = c->M.ramp; p.rays = 1.f; p.nl = (int)nl; p.mpitch = (int)mpitch_of(Np);
     return RC_OK;
 }
 
@@ -981,7 +985,9 @@ extern "C" rc_status rc_cache_get(const rc_cache* c, int which, void* host_dst, 
     if (bytes != b) return fail(RC_ERR_ARG, "cache array %d: %zu bytes requested, %zu available", which, bytes, b);
     HIPCK(hipStreamSynchronize(c->rs));
     HIPCK(hipStreamSynchronize(c->S->stream));
-    HIPCK(hipMemcpy(host_dst, p, b, hipMemcpyDeviceToHost));
+    const size_t Np = c->S->H.Np;
+    if (which == RC_C_M) HIPCK(hipMemcpy2D(host_dst, Np * 4, p, mpitch_of(Np) * 4, Np * 4, c->S->H.nl, hipMemcpyDeviceToHost));
+    else HIPCK(hipMemcpy(host_dst, p, b, hipMemcpyDeviceToHost));
     return RC_OK;
 }
 extern "C" rc_status rc_cache_set(rc_cache* c, int which, const void* host_src, size_t bytes) {
@@ -990,7 +996,9 @@ extern "C" rc_status rc_cache_set(rc_cache* c, int which, const void* host_src, 
     if (bytes != b) return fail(RC_ERR_ARG, "cache array %d: %zu bytes given, %zu expected", which, bytes, b);
     HIPCK(hipStreamSynchronize(c->rs));
     HIPCK(hipStreamSynchronize(c->S->stream));
-    HIPCK(hipMemcpy(p, host_src, b, hipMemcpyHostToDevice));
+    const size_t Np = c->S->H.Np;
+    if (which == RC_C_M) HIPCK(hipMemcpy2D(p, mpitch_of(Np) * 4, host_src, Np * 4, Np * 4, c->S->H.nl, hipMemcpyHostToDevice));
+    else HIPCK(hipMemcpy(p, host_src, b, hipMemcpyHostToDevice));
     return RC_OK;
 }
 extern "C" rc_status rc_cache_sync(rc_cache* c) {

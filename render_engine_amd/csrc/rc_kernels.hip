@@ -67,6 +67,32 @@ __device__ __forceinline__ float rc_est_of(const float* Lc, int64_t i, int nl, u
     return e;
 }
 
+// Per-entry sum of the light channels over the active lights, in light order (rc_est_of's sum, bit for
+// bit) for every entry: a block of RC_TB threads stages the contiguous Lc span of 64 entries in LDS with
+// coalesced loads (rows padded to an odd pitch: conflict-free row reads), then one thread per entry
+// adds its row. Replaces one lane per entry walking its own row, whose loads touch 64 rows at once.
+#define RC_RSE 64
+__global__ void __launch_bounds__(RC_TB) k_rowsum(const float* __restrict__ Lc, int Np, int nl, uint64_t dormant, float* __restrict__ out) {
+    __shared__ float s_r[RC_RSE * (RC_MAX_LIGHTS + 1)];
+    const int t = threadIdx.x, base = blockIdx.x * RC_RSE;
+    const int T = min(RC_RSE, Np - base), pitch = nl | 1, nel = T * nl;
+    const float* __restrict__ src = Lc + (int64_t)base * nl;
+    const int dq = RC_TB / nl, dr = RC_TB - dq * nl;
+    int j = t / nl, l = t - (t / nl) * nl;
+    for (int e = t; e < nel; e += RC_TB) {
+        s_r[j * pitch + l] = src[e];
+        j += dq; l += dr;
+        if (l >= nl) { l -= nl; ++j; }
+    }
+    __syncthreads();
+    if (t < T) {
+        float e = 0.f;
+        const float* row = s_r + t * pitch;
+        for (int k = 0; k < nl; ++k) if (!rc_is_dormant(dormant, k)) e += row[k];
+        out[base + t] = e;
+    }
+}
+
 // k rays per entry, cache read at the hit (multi-bounce through the cache), per-light sample
 // means in c.m, dependency masks (cuda/poc_gpu.cu:433-530). The per-light accumulator is the
 // thread's own row of c.m (rays outer, lights inner) so any light count runs in one kernel.
@@ -101,8 +127,8 @@ __global__ void __launch_bounds__(RC_TB) k_trace2(rc_dscene s, rc_dstate st, rc_
     float c0 = pend ? c.c0[i] : 0.f;
     int rg = pend ? s.region[i] : 0;
     float sd = 0.f, sq = 0.f;
-    float* mcol = c.m + i;   // SoA: light l of entry i at mcol[l * Np], coalesced across lanes
-    const int64_t ms = p.Np;
+    float* mcol = c.m + i;   // SoA: light l of entry i at mcol[l * mpitch], coalesced across lanes
+    const int64_t ms = p.mpitch;
     for (int r = 0; r < k; ++r) {
         float u1 = rc_rnd(rs), u2 = rc_rnd(rs);
         f3 D = rc_cosine_dir(N, u1, u2);
@@ -180,9 +206,37 @@ __global__ void __launch_bounds__(RC_TB) k_trace2(rc_dscene s, rc_dstate st, rc_
     }
 }
 
+// Per-block counts of the event kernels (blkcnt[block]) added in block order by one block into
+// *out (= the previous *out when add, else overwritten): no same-address atomics per entry.
+__device__ __forceinline__ void rc_block_count(bool flag, int* blkcnt) {
+    __shared__ int s_c[RC_TB / 64];
+    const unsigned long long b = __ballot(flag);
+    if ((threadIdx.x & 63) == 0) s_c[threadIdx.x >> 6] = __popcll(b);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int t = 0;
+        for (int k = 0; k < RC_TB / 64; ++k) t += s_c[k];
+        blkcnt[blockIdx.x] = t;
+    }
+}
+#define RC_CSB 1024
+__global__ void __launch_bounds__(RC_CSB) k_count_sum(const int* __restrict__ blkcnt, int nblk, int* out, int add) {
+    __shared__ long long s_w[RC_CSB / 64];
+    long long v = 0;
+    for (int j = threadIdx.x; j < nblk; j += RC_CSB) v += blkcnt[j];
+    v = rc_wave_sum_i64(v);
+    if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        long long t = 0;
+        for (int k = 0; k < RC_CSB / 64; ++k) t += s_w[k];
+        *out = (int)(add ? *out + t : t);
+    }
+}
+
 // verification decision per pending entry (cuda/poc_gpu.cu:532-555); entries still pending are counted
-// per wave (one atomic per wave instead of one per entry)
-__global__ void __launch_bounds__(RC_TB) k_decide(rc_dscene s, rc_dcache c, rc_params p) {
+// per block (blkcnt), then added into npend[1] by k_count_sum
+__global__ void __launch_bounds__(RC_TB) k_decide(rc_dscene s, rc_dcache c, rc_params p, int* blkcnt) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     bool still = false;
     if (i < p.Np && c.pending[i]) {
@@ -207,8 +261,7 @@ __global__ void __launch_bounds__(RC_TB) k_decide(rc_dscene s, rc_dcache c, rc_p
             else still = true;
         }
     }
-    const unsigned long long b = __ballot(still);
-    if ((threadIdx.x & 63) == 0 && b) atomicAdd(&c.npend[1], (int)__popcll(b));
+    rc_block_count(still, blkcnt);
 }
 
 #define RC_UT 64   // entries per k_update block
@@ -219,22 +272,24 @@ __global__ void __launch_bounds__(RC_TB) k_decide(rc_dscene s, rc_dcache c, rc_p
 // resolved radiance R = A * sum_l (Edir_l + Lc_l) of the updated entry (the value the query
 // pass reads, one float instead of 2 NL). A block owns RC_UT entries, whose AoS channel rows
 // form one contiguous span: every thread first issues the loads of its elements of that span
-// (Lc, Edir) and of the SoA sample means, so the block waits for memory once; the per-entry
+// (Lc, Edir) and of the tile's rows of the SoA sample means (rows of pitch p.mpitch, a multiple of
+// RC_UT, so every row's tile is 256-byte aligned), so the block waits for memory once; the per-entry
 // factors are computed by the first wave meanwhile, then the blend runs over the span in flat
-// order (coalesced Lc2 stores).
+// order (coalesced Lc2 stores). The means are staged in LDS at pitch RC_UT + 1: the flat order's
+// lanes read consecutive lights of one entry, which then fall in distinct banks.
 // UE: channel elements per thread (ceil(RC_UT * nl / RC_TB) at most), instantiated for a few
 // light counts so that small ones keep few registers (more waves in flight)
 template <int UE>
 __global__ void __launch_bounds__(RC_TB) k_update(rc_dscene s, rc_dstate st, rc_dcache c, rc_params p, const uint64_t* lnew) {
     __shared__ float s_al[RC_UT], s_alr[RC_UT], s_A[RC_UT];
-    extern __shared__ float sm[];                 // [nl][RC_UT] sample means, then [RC_UT][nl] terms
+    extern __shared__ float sm[];                 // [nl][RC_UT + 1] sample means, then [RC_UT][nl] terms
     const int nl = p.nl, Np = p.Np;
     const int base = blockIdx.x * RC_UT;
     const int T = min(RC_UT, Np - base);
     const int t = threadIdx.x;
     const int nel = T * nl;
     float* s_m = sm;
-    float* s_v = sm + nl * RC_UT;
+    float* s_v = sm + nl * (RC_UT + 1);
     const int64_t g0 = (int64_t)base * nl;
     const float* __restrict__ Lc = c.Lc + g0;
     const float* __restrict__ Ed = st.Edir + g0;
@@ -248,7 +303,7 @@ __global__ void __launch_bounds__(RC_TB) k_update(rc_dscene s, rc_dstate st, rc_
 #pragma unroll
     for (int u = 0; u < UE; ++u) {
         const int e = t + u * RC_TB;   // s_m element: light e / RC_UT, entry e % RC_UT
-        if (e < nl * RC_UT) { const int j = e % RC_UT; rm[u] = j < T ? c.m[(int64_t)(e / RC_UT) * Np + base + j] : 0.f; }
+        if (e < nl * RC_UT) rm[u] = c.m[(int64_t)(e / RC_UT) * p.mpitch + base + e % RC_UT];   // past Np: the padded row
     }
     if (t < RC_UT) {
         long long pr = 0;
@@ -286,7 +341,7 @@ __global__ void __launch_bounds__(RC_TB) k_update(rc_dscene s, rc_dstate st, rc_
 #pragma unroll
     for (int u = 0; u < UE; ++u) {
         const int e = t + u * RC_TB;
-        if (e < nl * RC_UT) s_m[e] = rm[u];
+        if (e < nl * RC_UT) s_m[e + e / RC_UT] = rm[u];
     }
     __syncthreads();
     // flat element e = j * nl + l of the span; (j, l) advanced by RC_TB per step
@@ -297,7 +352,7 @@ __global__ void __launch_bounds__(RC_TB) k_update(rc_dscene s, rc_dstate st, rc_
         const int e = t + u * RC_TB;
         if (e < nel) {
             const float a = ra[u];
-            const float mm = s_m[l * RC_UT + j];
+            const float mm = s_m[l * (RC_UT + 1) + j];
             const bool dm = rc_is_dormant(p.dormant, l);
             float o;
             if (p.ddgi) o = dm ? a : a + s_al[j] * (mm - a);
@@ -376,7 +431,7 @@ __global__ void __launch_bounds__(RC_TB) k_restored(rc_dcache c, int Np, int nl,
     if (i >= Np) return;
     c.n[i] = nmax; c.age[i] = 1e4f; c.pending[i] = 0;
     if (verify) {
-        c.pending[i] = 1; c.c0[i] = rc_est_of(c.Lc, i, nl, dormant);
+        c.pending[i] = 1; c.c0[i] = c.rsum[i];   // k_rowsum of the restored channels
         if (i == 0) c.npend[1] = Np;
     }
 }
@@ -385,34 +440,6 @@ __global__ void __launch_bounds__(RC_TB) k_restored(rc_dcache c, int Np, int nl,
 __global__ void __launch_bounds__(RC_TB) k_rescale(rc_dcache c, int Np, int nl, int l, float ratio) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < Np) c.Lc[(int64_t)nl * i + l] *= ratio;
-}
-
-// Per-block counts of the event kernels (blkcnt[block]) added in block order by one block into
-// *out (= the previous *out when add, else overwritten): no same-address atomics per entry.
-__device__ __forceinline__ void rc_block_count(bool flag, int* blkcnt) {
-    __shared__ int s_c[RC_TB / 64];
-    const unsigned long long b = __ballot(flag);
-    if ((threadIdx.x & 63) == 0) s_c[threadIdx.x >> 6] = __popcll(b);
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        int t = 0;
-        for (int k = 0; k < RC_TB / 64; ++k) t += s_c[k];
-        blkcnt[blockIdx.x] = t;
-    }
-}
-#define RC_CSB 1024
-__global__ void __launch_bounds__(RC_CSB) k_count_sum(const int* __restrict__ blkcnt, int nblk, int* out, int add) {
-    __shared__ long long s_w[RC_CSB / 64];
-    long long v = 0;
-    for (int j = threadIdx.x; j < nblk; j += RC_CSB) v += blkcnt[j];
-    v = rc_wave_sum_i64(v);
-    if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = v;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        long long t = 0;
-        for (int k = 0; k < RC_CSB / 64; ++k) t += s_w[k];
-        *out = (int)(add ? *out + t : t);
-    }
 }
 
 // Event invalidation from the transit-grid banks (cuda/poc_gpu.cu:642-660): an entry is touched when
@@ -432,27 +459,27 @@ __global__ void __launch_bounds__(RC_TB) k_touched(rc_dcache c, int Np, int nl, 
         t = hv != 0;
         if (t) {
             if (kind == 0) { c.n[i] = 0.f; c.age[i] = 0.f; }
-            else if (kind == 1) { c.pending[i] = 1; c.c0[i] = rc_est_of(c.Lc, i, nl, dormant); }
+            else if (kind == 1) { c.pending[i] = 1; c.c0[i] = c.rsum[i]; }   // k_rowsum of the channels
             if (kind >= 1 && dormant) c.stale[i] |= dormant;
         }
     }
     if (kind == 1) rc_block_count(t, blkcnt);
 }
 
-// light wakes from dormancy (cuda/poc_gpu.cu:663-674); verified entries counted per wave
-__global__ void __launch_bounds__(RC_TB) k_wake(rc_dcache c, int Np, int nl, int l, int verify, uint64_t dormant) {
+// light wakes from dormancy (cuda/poc_gpu.cu:663-674); with verify the entries' channel sums come from
+// k_rowsum (c.rsum) and the verified entries are counted per block
+__global__ void __launch_bounds__(RC_TB) k_wake(rc_dcache c, int Np, int nl, int l, int verify, uint64_t dormant, int* blkcnt) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     bool pend = false;
     if (i < Np && ((c.stale[i] >> l) & 1ull)) {
         if (verify) {
             c.pending[i] = 1;
-            c.c0[i] = rc_est_of(c.Lc, i, nl, dormant);
+            c.c0[i] = c.rsum[i];
             pend = true;
         } else { c.Lc[(int64_t)nl * i + l] = 0.f; c.n[i] = 0.f; c.age[i] = 0.f; }
         c.stale[i] &= ~(1ull << l);
     }
-    const unsigned long long b = __ballot(pend);
-    if ((threadIdx.x & 63) == 0 && b) atomicAdd(&c.npend[1], (int)__popcll(b));
+    if (verify) rc_block_count(pend, blkcnt);
 }
 
 // informed upper bound: shrink history by the true reference change (cuda/poc_gpu.cu:676-681)
@@ -537,11 +564,8 @@ __global__ void __launch_bounds__(RC_TB) k_snap_pack(rc_dcache c, int Np, int nl
     int before = 0;
     for (int k = 0; k < wv; ++k) before += s_w[k];
     const int slot = blockoff[blockIdx.x] + before + __popcll(m & ((1ull << lane) - 1ull));
-    const float* row = c.Lc + (int64_t)nl * i;
-    float L = 0.f;
-    for (int l = 0; l < nl; ++l) L += row[l];
-    rc_snaprec r;
-    r.index = (uint32_t)i; r.n = c.n[i]; r.key = ekeys[i]; r.radiance = L; r.age = c.age[i];
+    rc_snaprec r;   // radiance: the sum of all channels in light order (k_rowsum with no dormant channel)
+    r.index = (uint32_t)i; r.n = c.n[i]; r.key = ekeys[i]; r.radiance = c.rsum[i]; r.age = c.age[i];
     out[slot] = r;
 }
 // Exclusive prefix of n per-block counts in place, their total into *total (one block, any n): the
@@ -575,10 +599,11 @@ __global__ void __launch_bounds__(RC_SCB) k_scan_blocks(int* cnt, int n, int* to
 }
 
 // restore after a flush (rows already zero): record j < *countp, if its key matches its entry, gives the
-// entry its radiance in channel 0, its history and age; verify opens the verification window on it
-// (counted per wave). The record count is read on the device (the grid covers the most records).
+// entry its radiance in channel 0, its history and age; verify opens the verification window on it (its
+// channel sum is then the radiance, or 0 when channel 0 is dormant), counted per block. The record count
+// is read on the device (the grid covers the most records).
 __global__ void __launch_bounds__(RC_TB) k_snap_restore(rc_dcache c, int Np, int nl, const uint64_t* ekeys, const rc_snaprec* rec,
-                                                       const int* countp, int verify, uint64_t dormant) {
+                                                       const int* countp, int verify, uint64_t dormant, int* blkcnt) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     bool pend = false;
     if (j < *countp) {
@@ -587,11 +612,10 @@ __global__ void __launch_bounds__(RC_TB) k_snap_restore(rc_dcache c, int Np, int
         if (r.index < (uint32_t)Np && ekeys[i] == r.key) {
             c.Lc[(int64_t)nl * i] = r.radiance;
             c.n[i] = r.n; c.age[i] = r.age; c.pending[i] = 0;
-            if (verify) { c.pending[i] = 1; c.c0[i] = rc_est_of(c.Lc, i, nl, dormant); pend = true; }
+            if (verify) { c.pending[i] = 1; c.c0[i] = rc_is_dormant(dormant, 0) ? 0.f : 0.f + r.radiance; pend = true; }
         }
     }
-    const unsigned long long b = __ballot(pend);
-    if ((threadIdx.x & 63) == 0 && b) atomicAdd(&c.npend[1], (int)__popcll(b));
+    if (verify) rc_block_count(pend, blkcnt);
 }
 
 // Whole-cache flush (event to a new state, rtxgi_bench.patch:363-366): the light rows as 16-byte stores
@@ -1237,15 +1261,20 @@ hipError_t rc_launch_trace2(int trav, int deps, const rc_dscene* s, const rc_dst
     return hipGetLastError();
 }
 
+static void rc_rowsum(const rc_dcache* c, int Np, int nl, uint64_t dormant, hipStream_t stream) {
+    hipLaunchKernelGGL(k_rowsum, dim3((Np + RC_RSE - 1) / RC_RSE), dim3(RC_TB), 0, stream, (const float*)c->Lc, Np, nl, dormant, c->rsum);
+}
 hipError_t rc_launch_decide(const rc_dscene* s, const rc_dcache* c, const rc_params* p, hipStream_t stream) {
-    hipLaunchKernelGGL(k_decide, dim3(rc_grid(p->Np)), dim3(RC_TB), 0, stream, *s, *c, *p);
+    const int nblk = rc_grid(p->Np);
+    hipLaunchKernelGGL(k_decide, dim3(nblk), dim3(RC_TB), 0, stream, *s, *c, *p, c->blkcnt);
+    hipLaunchKernelGGL(k_count_sum, dim3(1), dim3(RC_CSB), 0, stream, (const int*)c->blkcnt, nblk, c->npend + 1, 1);
     return hipGetLastError();
 }
 hipError_t rc_launch_update(const rc_dscene* s, const rc_dstate* st, const rc_dcache* c, const rc_params* p, const uint64_t* lnew,
                             hipStream_t stream) {
     if (p->Np <= 0) return hipSuccess;
     dim3 g((p->Np + RC_UT - 1) / RC_UT), b(RC_TB);
-    size_t shm = (size_t)2 * p->nl * RC_UT * sizeof(float);
+    size_t shm = (size_t)p->nl * (2 * RC_UT + 1) * sizeof(float);
     const int ue = (RC_UT * p->nl + RC_TB - 1) / RC_TB;
     if (ue <= 4) hipLaunchKernelGGL(k_update<4>, g, b, shm, stream, *s, *st, *c, *p, lnew);
     else if (ue <= 9) hipLaunchKernelGGL(k_update<9>, g, b, shm, stream, *s, *st, *c, *p, lnew);
@@ -1266,6 +1295,7 @@ hipError_t rc_launch_sumpr(const rc_dcache* c, int Np, hipStream_t stream) {
     return hipGetLastError();
 }
 hipError_t rc_launch_restored(const rc_dcache* c, int Np, int nl, float nmax, uint64_t dormant, int verify, hipStream_t stream) {
+    if (verify) rc_rowsum(c, Np, nl, dormant, stream);
     hipLaunchKernelGGL(k_restored, dim3(rc_grid(Np)), dim3(RC_TB), 0, stream, *c, Np, nl, nmax, dormant, verify);
     return hipGetLastError();
 }
@@ -1275,12 +1305,16 @@ hipError_t rc_launch_rescale(const rc_dcache* c, int Np, int nl, int l, float ra
 }
 hipError_t rc_launch_touched(const rc_dcache* c, int Np, int nl, rc_vmask vm, int nw, int kind, uint64_t dormant, hipStream_t stream) {
     const int nblk = rc_grid(Np);
+    if (kind == 1) rc_rowsum(c, Np, nl, dormant, stream);
     hipLaunchKernelGGL(k_touched, dim3(nblk), dim3(RC_TB), 0, stream, *c, Np, nl, vm, nw, kind, dormant, c->blkcnt);
     if (kind == 1) hipLaunchKernelGGL(k_count_sum, dim3(1), dim3(RC_CSB), 0, stream, (const int*)c->blkcnt, nblk, c->npend + 1, 1);
     return hipGetLastError();
 }
 hipError_t rc_launch_wake(const rc_dcache* c, int Np, int nl, int l, int verify, uint64_t dormant, hipStream_t stream) {
-    hipLaunchKernelGGL(k_wake, dim3(rc_grid(Np)), dim3(RC_TB), 0, stream, *c, Np, nl, l, verify, dormant);
+    const int nblk = rc_grid(Np);
+    if (verify) rc_rowsum(c, Np, nl, dormant, stream);
+    hipLaunchKernelGGL(k_wake, dim3(nblk), dim3(RC_TB), 0, stream, *c, Np, nl, l, verify, dormant, c->blkcnt);
+    if (verify) hipLaunchKernelGGL(k_count_sum, dim3(1), dim3(RC_CSB), 0, stream, (const int*)c->blkcnt, nblk, c->npend + 1, 1);
     return hipGetLastError();
 }
 hipError_t rc_launch_oracle_method(const rc_dcache* c, int Np, const float* rold, const float* rnew, hipStream_t stream) {
@@ -1426,13 +1460,16 @@ hipError_t rc_launch_snap_count(const rc_dcache* c, int Np, int* blockcnt, hipSt
 }
 hipError_t rc_launch_snap_pack(const rc_dcache* c, int Np, int nl, const uint64_t* ekeys, const int* blockoff, rc_snaprec* out,
                                hipStream_t stream) {
+    rc_rowsum(c, Np, nl, 0ull, stream);
     hipLaunchKernelGGL(k_snap_pack, dim3(rc_grid(Np)), dim3(RC_TB), 0, stream, *c, Np, nl, ekeys, blockoff, out);
     return hipGetLastError();
 }
 hipError_t rc_launch_snap_restore(const rc_dcache* c, int Np, int nl, const uint64_t* ekeys, const rc_snaprec* rec, const int* count,
                                   int max_count, int verify, uint64_t dormant, hipStream_t stream) {
     if (max_count <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_snap_restore, dim3(rc_grid(max_count)), dim3(RC_TB), 0, stream, *c, Np, nl, ekeys, rec, count, verify, dormant);
+    const int nblk = rc_grid(max_count);
+    hipLaunchKernelGGL(k_snap_restore, dim3(nblk), dim3(RC_TB), 0, stream, *c, Np, nl, ekeys, rec, count, verify, dormant, c->blkcnt);
+    if (verify) hipLaunchKernelGGL(k_count_sum, dim3(1), dim3(RC_CSB), 0, stream, (const int*)c->blkcnt, nblk, c->npend + 1, 1);
     return hipGetLastError();
 }
 hipError_t rc_launch_scan_blocks(int* cnt, int n, int* total, hipStream_t stream) {

@@ -109,6 +109,7 @@ typedef struct rc_dcache {
     double* err;                 // [3] squared error, reference, valid count of the error pass
     double* errpart;             // [3][ceil(Np / 256)] per-block partials of the error pass
     int32_t* blkcnt;             // [ceil(Np / 256) + 1] per-block counts of the event kernels, then their total
+    float* rsum;                 // [Np] per-entry channel sums of the event kernels (k_rowsum)
 } rc_dcache;
 
 // Event voxels of the transit grid as a per-word bit mask (bit v & 31 of word v >> 5), built on the host
@@ -128,7 +129,7 @@ typedef struct rc_params {
     int32_t ramp;
     float rays;
     int32_t nl;
-    int32_t pad0;
+    int32_t mpitch;              // row pitch (entries) of the SoA sample means: Np rounded up to a multiple of 64
 } rc_params;
 
 // Pinhole camera of the query pass. Paths: primary hit, one-light NEE per explicit vertex,
