@@ -96,7 +96,11 @@ static hipError_t split_dst(const rc_dstate& base, const rc_dscene& ds, int trav
     rc_dstate d = base;
     d.ocap = trav == 2 ? cap : 0;
     d.oforce = force ? 1 : 0;
-    const size_t need = rc_ovf_words(&d, &ds, nthreads);
+    size_t need = rc_ovf_words(&d, &ds, nthreads);
+    if (need > ((size_t)1 << 30)) {   // beyond the 32-bit byte offsets of the global levels: every level in LDS
+        d.ocap = 0; d.oforce = 0;
+        need = 0;
+    }
     if (need > *words) {
         hipError_t e = hipStreamSynchronize(sync);
         if (e != hipSuccess) return e;

@@ -142,8 +142,13 @@ static inline size_t rc_shm(const rc_dstate& st, const rc_dscene& s, int trav, b
 static inline size_t rc_ovf_need(const rc_dstate& st, const rc_dscene& s, long long nthreads) {
     return st.ocap > 0 && st.ocap < s.wstack ? (size_t)nthreads * (size_t)(s.wstack - st.ocap) : 0;
 }
+// the split lane stacks address their global levels with 32-bit byte offsets (rc_lstk_s::off): a
+// launch whose global levels span more than 4 GiB cannot take them
+static inline bool rc_ovf_fits32(size_t words) { return words <= ((size_t)1 << 30); }
 // false when the state's global stack levels are too small for a wide-traversal launch of
-// nblocks blocks (the launchers then fail with hipErrorInvalidValue instead of running)
+// nblocks blocks, or too large for 32-bit offsets (the launchers then fail with
+// hipErrorInvalidValue instead of running)
 static inline bool rc_ovf_ok(const rc_dstate& st, const rc_dscene& s, int trav, long long nblocks) {
-    return trav != 2 || rc_ovf_need(st, s, nblocks * RC_TB) <= (st.ovf ? (size_t)st.ovf_words : 0);
+    const size_t need = rc_ovf_need(st, s, nblocks * RC_TB);
+    return trav != 2 || (need <= (st.ovf ? (size_t)st.ovf_words : 0) && rc_ovf_fits32(need));
 }

@@ -208,6 +208,27 @@ def test_radiance_vs_fp64_full_shape(torch_gpu, name):
     assert rl < 1e-4 and rr < 1e-4 and ri < 1e-4, (rl, rr, ri)
 
 
+def test_query_stack_offsets_past_32_bits(torch_gpu):
+    """1920x1080 at 16 spp with one LDS stack level: the global levels would span more than 4 GiB, past
+    the split stacks' 32-bit byte offsets, so the pass keeps every level in LDS; the image equals the
+    all-LDS pass's and a split that fits (16 LDS levels)"""
+    torch = torch_gpu
+    s, osc, flags = _shape(torch_gpu, "street1080")
+    cache = rea.Cache(s, rea.methods.plain())
+    cache.begin(0, 2)
+    cache.frame(0)
+    cam = s.camera()
+    cam.spp = 16
+    out = {}
+    for levels in (64, 1, 16):
+        cache.set_stack_levels(levels)
+        img = torch.full((cam.W * cam.H,), -1.0, dtype=torch.float32, device="cuda")
+        cache.render(img, cam, seed=3)
+        out[levels] = img.cpu().numpy().view(np.uint32).copy()
+    assert np.array_equal(out[1], out[64]) and np.array_equal(out[16], out[64])
+    _say("1080p x 16 spp: split stacks past 32-bit offsets fall back to LDS")
+
+
 # ---------------------------------------------------------------- adjoint at 64 emission scalars
 def test_adjoint_64_lights_4096_materials(torch_gpu):
     """63 lamps + sun = 64 emission scalars (the sun on bit 63 alone), 4,096 materials, 3 bounces x 4 rays:
