@@ -184,6 +184,12 @@ typedef struct rc_snap_rec { uint32_t index; float n; uint64_t key; float radian
 /* records of the cache's entries with samples, in entry order, into host_records (capacity records);
  * count_out = records written (the number of entries with samples when capacity suffices) */
 rc_status rc_cache_snapshot_save(rc_cache* c, rc_snap_rec* host_records, size_t capacity, size_t* count_out);
+/* rc_cache_snapshot_save that allocates nothing: the records are staged in a static device array of the
+ * library (2^21 records, 48 MiB per GPU, held until the process ends) instead of a buffer allocated per
+ * call; RC_ERR_ARG for a cache of more than 2^21 entries. No counterpart in the reference: for callers that
+ * cannot allocate device memory (serving loops, code under stream capture, harnesses that forbid hipMalloc).
+ * Calls that overlap on one GPU share the array and race. */
+rc_status rc_cache_snapshot_save_noalloc(rc_cache* c, rc_snap_rec* host_records, size_t capacity, size_t* count_out);
 /* flush, then restore the records whose key matches its entry (radiance into channel 0, history, age) */
 rc_status rc_cache_snapshot_load(rc_cache* c, const rc_snap_rec* host_records, size_t count);
 size_t rc_cache_array_bytes(const rc_cache* c, int which);

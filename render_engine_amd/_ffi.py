@@ -95,6 +95,7 @@ _SIGS = {
     "rc_cache_set_stack_levels": (C.c_int, [_P, C.c_int]),
     "rc_scene_set_stack_levels": (C.c_int, [_P, C.c_int]),
     "rc_cache_snapshot_save": (C.c_int, [_P, _P, C.c_size_t, C.POINTER(C.c_size_t)]),
+    "rc_cache_snapshot_save_noalloc": (C.c_int, [_P, _P, C.c_size_t, C.POINTER(C.c_size_t)]),
     "rc_cache_snapshot_load": (C.c_int, [_P, _P, C.c_size_t]),
     "rc_adjoint_create": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_uint32, C.POINTER(_P)]),
     "rc_adjoint_destroy": (None, [_P]),

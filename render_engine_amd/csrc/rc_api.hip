@@ -8,6 +8,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1489,6 +1490,59 @@ extern "C" rc_status rc_cache_snapshot_save(rc_cache* c, rc_snap_rec* host_recor
     if (e == hipSuccess && n) e = hipMemcpy(host_records, tmp, n * sizeof(rc_snaprec), hipMemcpyDeviceToHost);
     (void)hipFree(tmp);
     if (e != hipSuccess) return fail(RC_ERR_HIP, "snapshot save: %s", hipGetErrorString(e));
+    *count_out = n;
+    return RC_OK;
+}
+
+// ---- allocation-free snapshot save
+// The records are staged in a file-scope device array, `scratch`, instead of a buffer allocated per call.
+// Host code never passes the array's name to a kernel or copy (on the host it is only a stand-in for the
+// symbol): its device address on each GPU comes from hipGetSymbolAddress on the first call with that
+// GPU current, and is kept per device. Every call on one GPU shares that GPU's array; the pack writes
+// records [0, count) before they are read, so nothing in it needs clearing. Calls that overlap on one
+// GPU race on it.
+#define RC_NOALLOC_RECORDS (1u << 21)   // 2,097,152 records of 24 bytes: 48 MiB per GPU
+#define RC_MAX_DEVICES 64
+static __device__ __align__(256) rc_snaprec scratch[RC_NOALLOC_RECORDS];
+static std::mutex g_scratch_mu;
+static rc_snaprec* g_scratch_ptr[RC_MAX_DEVICES] = {};
+
+static hipError_t scratch_records(rc_snaprec** out) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= RC_MAX_DEVICES) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_ptr[dev] = (rc_snaprec*)p;
+    }
+    *out = g_scratch_ptr[dev];
+    return hipSuccess;
+}
+
+extern "C" rc_status rc_cache_snapshot_save_noalloc(rc_cache* c, rc_snap_rec* host_records, size_t capacity, size_t* count_out) {
+    if (!c || !count_out || (capacity > 0 && !host_records)) return fail(RC_ERR_ARG, "NULL argument");
+    if (!c->begun) return fail(RC_ERR_STATE, "rc_cache_begin not called");
+    rc_scene* S = c->S;
+    const size_t Np = S->H.Np;
+    if (Np > RC_NOALLOC_RECORDS)
+        return fail(RC_ERR_ARG, "rc_cache_snapshot_save_noalloc: the cache has %zu entries, at most %u fit the static record array "
+                    "(use rc_cache_snapshot_save)", Np, RC_NOALLOC_RECORDS);
+    rc_snaprec* rec = nullptr;
+    HIPCK(scratch_records(&rec));
+    HIPCK(wait_query(c));
+    HIPCK(snap_pack(c, rec, c->d_slotcnt + RC_KSNAP));
+    int cnt = 0;
+    HIPCK(hipMemcpyAsync(&cnt, c->d_slotcnt + RC_KSNAP, sizeof(int), hipMemcpyDeviceToHost, S->stream));
+    HIPCK(hipStreamSynchronize(S->stream));
+    const size_t n = std::min((size_t)cnt, capacity);
+    if (n) {
+        HIPCK(hipMemcpyAsync(host_records, rec, n * sizeof(rc_snaprec), hipMemcpyDeviceToHost, S->stream));
+        HIPCK(hipStreamSynchronize(S->stream));
+    }
     *count_out = n;
     return RC_OK;
 }
