@@ -157,9 +157,11 @@ rc_status rc_cache_frame(rc_cache* c, int state, int want_err, int timed, rc_fra
  * rays_out[2] = extension rays, shadow rays (may be NULL); ms_out = GPU time of the pass
  * (may be NULL; when set the call waits for the pass). The pass runs on the cache's own blocking
  * stream, after the work queued so far; the next rc_cache_frame's entry pass overlaps it.
+ * stream (may be NULL): the caller's stream; the pass then also starts after the work queued on it, and
+ * the work queued on it afterwards starts after the pass (so d_img may be written or read there).
  * rc_cache_sync, or a synchronisation of the device or of the legacy default stream, waits for it. */
 rc_status rc_cache_render(rc_cache* c, const rc_camera* cam, int no_cache, uint32_t seed, float* d_img, long long* rays_out,
-                          float* ms_out);
+                          float* ms_out, void* stream);
 /* Schedule of the camera pass (same image bit for bit): RC_RENDER_FUSED (default) runs each path
  * in one thread; RC_RENDER_STAGED (4-wide BVH scenes, trav 2) runs generate / trace / shade /
  * shadow-trace stages, the live rays of each 1024-path segment compacted between stages. The
@@ -283,8 +285,8 @@ enum { RC_MLP_W = 0, RC_MLP_TABLE, RC_MLP_GW, RC_MLP_GTABLE, RC_MLP_COUNT };
 rc_status rc_mlp_train_scene(rc_mlp* m, rc_cache* c, int n, uint32_t seed, float* loss_out, float* judge_out);
 /* camera pass over the cache's scene and state whose paths end on the MLP at bounce cam->cache_bounce:
  * with the cache's render mode RC_RENDER_FUSED one thread per path records its query point, then one
- * batched inference; RC_RENDER_STAGED runs the staged schedule (the same image); ms_out as rc_cache_render */
-rc_status rc_mlp_render(rc_mlp* m, rc_cache* c, const rc_camera* cam, uint32_t seed, float* d_img, float* ms_out);
+ * batched inference; RC_RENDER_STAGED runs the staged schedule (the same image); ms_out and stream as rc_cache_render */
+rc_status rc_mlp_render(rc_mlp* m, rc_cache* c, const rc_camera* cam, uint32_t seed, float* d_img, float* ms_out, void* stream);
 size_t rc_mlp_array_bytes(const rc_mlp* m, int which);
 rc_status rc_mlp_get(const rc_mlp* m, int which, void* host_dst, size_t bytes);
 rc_status rc_mlp_set(rc_mlp* m, int which, const void* host_src, size_t bytes);

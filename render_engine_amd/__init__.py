@@ -199,12 +199,14 @@ class Cache:
         return [self.frame(s, want_err).err for s in seq]
 
     def render(self, img, cam=None, no_cache=False, seed=7, count_rays=False, timed=False):
-        """query pass into img; returns (ext_rays, shadow_rays) when count_rays, ms when timed"""
+        """query pass into img, ordered after the work queued on torch's current stream and before the
+        work queued on it afterwards; returns (ext_rays, shadow_rays) when count_rays, ms when timed"""
+        import torch
         cam = cam or self.scene.camera()
         rays = (C.c_longlong * 2)() if count_rays else None
         ms = C.c_float(0.0) if timed else None
         check(lib.rc_cache_render(self._h, C.byref(cam), 1 if no_cache else 0, seed, _ptr(img), rays,
-                                  C.byref(ms) if timed else None))
+                                  C.byref(ms) if timed else None, _stream(torch.cuda.current_stream())))
         out = []
         if count_rays:
             out.append((rays[0], rays[1]))

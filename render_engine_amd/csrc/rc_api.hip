@@ -482,6 +482,7 @@ struct rc_cache {
     // event handling, a new run) first makes the scene stream wait for the last pass (ev_q).
     hipStream_t rs = nullptr;
     hipEvent_t ev_upd = nullptr, ev_q = nullptr;
+    hipEvent_t ev_in = nullptr, ev_out = nullptr;   // ordering against a caller's stream (render calls)
     bool q_pending = false;
     // R is double-buffered: a frame's update writes the buffer the camera pass two frames back
     // read (waiting for that pass's ev_r only), so it need not wait for the last camera pass.
@@ -508,6 +509,8 @@ struct rc_cache {
         if (ev_upd) (void)hipEventDestroy(ev_upd);
         if (ev_q) (void)hipEventDestroy(ev_q);
         if (ev_inv) (void)hipEventDestroy(ev_inv);
+        if (ev_in) (void)hipEventDestroy(ev_in);
+        if (ev_out) (void)hipEventDestroy(ev_out);
         if (h_inv) (void)hipHostFree(h_inv);
         for (hipEvent_t e : ev_r) if (e) (void)hipEventDestroy(e);
         for (void* q : owned) (void)hipFree(q);
@@ -558,6 +561,7 @@ extern "C" rc_status rc_cache_create(rc_scene* s, const rc_method* m, rc_cache**
     for (auto& e : c->ev) CK2(hipEventCreate(&e));
     CK2(hipStreamCreate(&c->rs));
     CK2(hipEventCreateWithFlags(&c->ev_upd, hipEventDisableTiming)); CK2(hipEventCreateWithFlags(&c->ev_q, hipEventDisableTiming));
+    CK2(hipEventCreateWithFlags(&c->ev_in, hipEventDisableTiming)); CK2(hipEventCreateWithFlags(&c->ev_out, hipEventDisableTiming));
     for (auto& e : c->ev_r) CK2(hipEventCreateWithFlags(&e, hipEventDisableTiming));
 #undef CK2
     *out = c;
@@ -903,7 +907,7 @@ extern "C" rc_status rc_cache_set_render_mode(rc_cache* c, int mode) {
 }
 
 extern "C" rc_status rc_cache_render(rc_cache* c, const rc_camera* cam, int no_cache, uint32_t seed, float* d_img, long long* rays_out,
-                                     float* ms_out) {
+                                     float* ms_out, void* stream) {
     if (!c || !cam || !d_img) return fail(RC_ERR_ARG, "NULL argument");
     if (!c->begun) return fail(RC_ERR_STATE, "rc_cache_begin not called");
     if (cam->W < 1 || cam->H < 1 || cam->spp < 1 || cam->bounces < 0 || cam->cache_bounce < 0 || (long long)cam->W * cam->H > (1ll << 30))
@@ -926,6 +930,10 @@ extern "C" rc_status rc_cache_render(rc_cache* c, const rc_camera* cam, int no_c
     hipStream_t qs = c->rs;
     HIPCK(hipEventRecord(c->ev_upd, st));
     HIPCK(hipStreamWaitEvent(qs, c->ev_upd, 0));
+    if (stream) {   // and after the caller's work on its stream (what wrote or reads d_img)
+        HIPCK(hipEventRecord(c->ev_in, (hipStream_t)stream));
+        HIPCK(hipStreamWaitEvent(qs, c->ev_in, 0));
+    }
     unsigned long long* ctr = nullptr;
     if (rays_out) { HIPCK(hipMemsetAsync(c->d_counters, 0, 16, qs)); ctr = c->d_counters; }
     if (ms_out) HIPCK(hipEventRecord(c->ev[0], qs));
@@ -936,6 +944,7 @@ extern "C" rc_status rc_cache_render(rc_cache* c, const rc_camera* cam, int no_c
         HIPCK(rc_launch_query(S->trav, &S->ds, &dq, no_cache ? nullptr : c->c.R, cam, seed, (uint32_t)c->f, c->dormant, d_img, ctr, qs));
     HIPCK(hipEventRecord(c->ev_q, qs));
     c->q_pending = true;
+    if (stream) HIPCK(hipStreamWaitEvent((hipStream_t)stream, c->ev_q, 0));   // the caller's later work sees d_img
     HIPCK(hipEventRecord(c->ev_r[c->rb], qs));   // the pass read R buffer rb
     c->r_pend[c->rb] = true;
     if (ms_out) {
@@ -1445,7 +1454,7 @@ extern "C" rc_status rc_mlp_train_scene(rc_mlp* m, rc_cache* c, int n, uint32_t 
     return RC_OK;
 }
 
-extern "C" rc_status rc_mlp_render(rc_mlp* m, rc_cache* c, const rc_camera* cam, uint32_t seed, float* d_img, float* ms_out) {
+extern "C" rc_status rc_mlp_render(rc_mlp* m, rc_cache* c, const rc_camera* cam, uint32_t seed, float* d_img, float* ms_out, void* stream) {
     if (!m || !c || !cam || !d_img) return fail(RC_ERR_ARG, "NULL argument");
     if (!c->begun) return fail(RC_ERR_STATE, "rc_cache_begin not called");
     if (cam->W < 1 || cam->H < 1 || cam->spp < 1 || cam->bounces < 0 || cam->cache_bounce < 0 || (long long)cam->W * cam->H * cam->spp > (1ll << 30))
@@ -1458,11 +1467,19 @@ extern "C" rc_status rc_mlp_render(rc_mlp* m, rc_cache* c, const rc_camera* cam,
     HIPCK(qwork_reserve(c, (long long)cam->W * cam->H * cam->spp));
     rc_dstate dk = S->dst[si];
     if (c->render_mode != RC_RENDER_STAGED) HIPCK(scene_dst(S, si, rc_query_mlp_threads(cam), &dk));
+    if (stream) {   // after the caller's work on its stream
+        HIPCK(hipEventRecord(c->ev_in, (hipStream_t)stream));
+        HIPCK(hipStreamWaitEvent(st, c->ev_in, 0));
+    }
     if (ms_out) HIPCK(hipEventRecord(c->ev[0], st));
     if (c->render_mode == RC_RENDER_STAGED)
         HIPCK(rc_launch_query_staged(&S->ds, &S->dst[si], nullptr, cam, seed, (uint32_t)c->f, c->dormant, d_img, nullptr, &c->qw, &m->d, st));
     else
         HIPCK(rc_launch_query_mlp(&S->ds, &dk, cam, seed, (uint32_t)c->f, c->dormant, d_img, &c->qw, &m->d, st));
+    if (stream) {   // the caller's later work sees d_img
+        HIPCK(hipEventRecord(c->ev_out, st));
+        HIPCK(hipStreamWaitEvent((hipStream_t)stream, c->ev_out, 0));
+    }
     if (ms_out) {
         HIPCK(hipEventRecord(c->ev[1], st));
         HIPCK(hipEventSynchronize(c->ev[1]));

@@ -37,6 +37,8 @@ size_t rc_ovf_words(const rc_dstate* st, const rc_dscene* s, long long nthreads)
 long long rc_grid_threads(long long n);
 hipError_t rc_launch_hash_insert(unsigned long long* HK, int* HV, uint32_t hmask, const unsigned long long* keys, const int* vals,
                                  int n, int* fail, hipStream_t stream);
+// empty slots (key 0) get value -1
+hipError_t rc_launch_hash_finish(const unsigned long long* HK, int* HV, uint32_t hsize, hipStream_t stream);
 hipError_t rc_launch_hash_query(const unsigned long long* HK, const int* HV, uint32_t hmask, const unsigned long long* keys, int n,
                                 int* out, hipStream_t stream);
 hipError_t rc_launch_hash_point(const rc_dscene* s, const float* H, const float* Nn, int n, int* out, hipStream_t stream);

@@ -1135,29 +1135,36 @@ __global__ void __launch_bounds__(RC_TB) k_q_pixels(rc_camera cam, rc_qwork w, f
 
 // ------------------------------------------------------------------------------ hash grid insert
 // Insert keys with 64-bit CAS and linear probing (same home slot mix64(key) & hmask and probe
-// order as the lookup, cuda/poc_gpu.cu:263-272). Lanes of a wave holding the same key elect
-// the lowest lane (ballot over the match mask) so each distinct key probes once per wave; the
-// value stored for a key is the minimum inserted (atomicMin), independent of arrival order.
-// fail[0] counts keys that found no slot within 64 probes.
+// order as the lookup, cuda/poc_gpu.cu:263-272). Wave-level dedup by a ballot loop over the wave's
+// distinct keys: the lowest lane still to be grouped names a key (a uniform shuffle), one ballot
+// gives the lanes holding it, and only a group of more than one lane reduces its values; the group's
+// lowest lane then inserts the group's minimum. Distinct keys cost one ballot each, duplicates cost
+// nothing more to probe. The value stored for a key is the minimum inserted (atomicMin), whatever the
+// arrival order; slots may differ from a sequential insertion, lookups do not. fail[0] counts keys
+// that found no slot within 64 probes.
 __global__ void __launch_bounds__(RC_TB) k_hash_insert(unsigned long long* HK, int* HV, uint32_t hmask,
                                                       const unsigned long long* keys, const int* vals, int n, int* fail) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63;
     bool active = i < n;
-    unsigned long long key = active ? keys[i] : 0ull;
-    int val = active ? vals[i] : 0x7fffffff;
+    const unsigned long long key = active ? keys[i] : 0ull;
+    const int val = active ? vals[i] : 0x7fffffff;
     active = active && key != 0ull;
-    // wave-level dedup: the lowest lane with a given key inserts the min value of its group
-    int lane = threadIdx.x & 63;
-    bool leader = active;
+    bool leader = false;
     int best = val;
-    for (int o = 0; o < 64; ++o) {
-        unsigned long long ko = __shfl(key, o, 64);
-        int vo = __shfl(val, o, 64);
-        bool ao = __shfl((int)active, o, 64) != 0;
-        if (active && ao && ko == key) {
-            if (o < lane) leader = false;
-            best = min(best, vo);
+    unsigned long long todo = __ballot(active);
+    while (todo) {
+        const int src = __ffsll((long long)todo) - 1;
+        const unsigned long long k = __shfl(key, src, 64);
+        const unsigned long long grp = __ballot(active && key == k);
+        if (__popcll(grp) > 1) {   // duplicates of k in this wave: their minimum value
+            int v = ((grp >> lane) & 1ull) ? val : 0x7fffffff;
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
+            if (lane == src) best = v;
         }
+        if (lane == src) leader = true;
+        todo &= ~grp;
     }
     if (!leader) return;
     uint32_t h = (uint32_t)rc_mix64(key) & hmask;
@@ -1167,6 +1174,12 @@ __global__ void __launch_bounds__(RC_TB) k_hash_insert(unsigned long long* HK, i
         h = (h + 1) & hmask;
     }
     atomicAdd(fail, 1);
+}
+
+// slots left empty by the insert: value -1 (the host table's empty value)
+__global__ void __launch_bounds__(RC_TB) k_hash_finish(const unsigned long long* HK, int* HV, uint32_t hsize) {
+    const uint32_t h = blockIdx.x * blockDim.x + threadIdx.x;
+    if (h < hsize && HK[h] == 0ull) HV[h] = -1;
 }
 
 __global__ void __launch_bounds__(RC_TB) k_hash_query(const unsigned long long* HK, const int* HV, uint32_t hmask,
@@ -1377,6 +1390,10 @@ hipError_t rc_launch_query(int trav, const rc_dscene* s, const rc_dstate* st, co
     else { if (trav == 0) RC_Q(0, false); else if (trav == 1) RC_Q(1, false); else RC_QW(false); }
 #undef RC_QW
 #undef RC_Q
+    return hipGetLastError();
+}
+hipError_t rc_launch_hash_finish(const unsigned long long* HK, int* HV, uint32_t hsize, hipStream_t stream) {
+    hipLaunchKernelGGL(k_hash_finish, dim3(rc_grid(hsize)), dim3(RC_TB), 0, stream, HK, HV, hsize);
     return hipGetLastError();
 }
 hipError_t rc_launch_hash_insert(unsigned long long* HK, int* HV, uint32_t hmask, const unsigned long long* keys, const int* vals,

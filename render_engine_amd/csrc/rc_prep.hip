@@ -12,6 +12,7 @@
 
 #include "rc_bvh.h"
 #include "rc_device.h"
+#include "rc_kernels.h"
 #include "rc_scene.h"
 
 namespace {
@@ -425,9 +426,46 @@ __global__ void __launch_bounds__(256) k_bvh_children(const BItem* items, int m,
 
 struct PrMax { __device__ __forceinline__ int operator()(int a, int b) const { return a > b ? a : b; } };
 
+__global__ void __launch_bounds__(256) k_iota(int* v, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) v[i] = i;
+}
+
 }  // namespace
 
 namespace rc {
+
+// The cache cells' hash table on the GPU (cuda/prep_mesh.cuh:356-366 builds it on the host): entry i's
+// key with value i, inserted with 64-bit CAS linear probing from mix64(key) & (size - 1), the same slot
+// function and probe order as the lookup. Slots may differ from the host's sequential insertion; every
+// key looks up to the same entry, and empty slots hold value -1 as on the host.
+std::string build_hash_device(const std::vector<uint64_t>& keys, int log2, std::vector<uint64_t>& HK, std::vector<int>& HV) {
+    const size_t HS = (size_t)1 << log2, n = keys.size();
+    if (HS > 0x80000000ull) return "hash table larger than 2^31 slots";
+    hipStream_t st = nullptr;
+    DevBuf B;
+    unsigned long long *dK, *dHK; int *dV, *dHV, *dF;
+    PK(B.get(&dK, std::max<size_t>(n, 1))); PK(B.get(&dV, std::max<size_t>(n, 1))); PK(B.get(&dHK, HS)); PK(B.get(&dHV, HS));
+    PK(B.get(&dF, 1));
+    PK(hipMemsetAsync(dHK, 0, HS * 8, st));
+    PK(hipMemsetAsync(dHV, 0x7f, HS * 4, st));   // 0x7f7f7f7f: above every entry index, lowered by atomicMin
+    PK(hipMemsetAsync(dF, 0, 4, st));
+    if (n) {
+        PK(hipMemcpyAsync(dK, keys.data(), n * 8, hipMemcpyHostToDevice, st));
+        hipLaunchKernelGGL(k_iota, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, dV, (int)n);
+        PK(hipGetLastError());
+        PK(rc_launch_hash_insert(dHK, dHV, (uint32_t)(HS - 1), dK, dV, (int)n, dF, st));
+    }
+    PK(rc_launch_hash_finish(dHK, dHV, (uint32_t)HS, st));
+    HK.resize(HS); HV.resize(HS);
+    int failed = 0;
+    PK(hipMemcpyAsync(HK.data(), dHK, HS * 8, hipMemcpyDeviceToHost, st));
+    PK(hipMemcpyAsync(HV.data(), dHV, HS * 4, hipMemcpyDeviceToHost, st));
+    PK(hipMemcpyAsync(&failed, dF, 4, hipMemcpyDeviceToHost, st));
+    PK(hipStreamSynchronize(st));
+    if (failed) return std::to_string(failed) + " cache keys found no hash slot within 64 probes";
+    return "";
+}
 
 std::string sample_cells_device(const Mesh& M, float h, const float lo[3], std::vector<uint64_t>& keys_out, std::vector<CellRec>& recs_out) {
     const int nt = M.nt();

@@ -386,7 +386,12 @@ std::string prepare(const SceneDesc& d, Scene& S) {
         if (d.table_log2 < log2) return "hash table too small for the cache cells (needs load <= 1/2)";
         log2 = d.table_log2;
     }
-    build_hash(keys, log2, S.HK, S.HV);
+    if (g_prep_device) {
+        std::string e = build_hash_device(keys, log2, S.HK, S.HV);
+        if (!e.empty()) return e;
+    } else {
+        build_hash(keys, log2, S.HK, S.HV);
+    }
     std::unordered_map<uint64_t, int> zone;
     for (int i = 0; i < S.npst; ++i) {
         const CellRec& c = recs[i];

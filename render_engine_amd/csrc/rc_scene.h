@@ -119,6 +119,9 @@ struct CellRec { float p[3]; float d2; float n[3]; double asum, wsum; int mat; }
 void sample_cells(const Mesh& M, float h, const float lo[3], std::vector<uint64_t>& keys, std::vector<CellRec>& recs);
 // The same cells computed on the GPU (rc_prep.hip), bit for bit; "" on success, else the error.
 std::string sample_cells_device(const Mesh& M, float h, const float lo[3], std::vector<uint64_t>& keys, std::vector<CellRec>& recs);
+// the cells' hash table built on the GPU (64-bit CAS inserts, ballot dedup): the host table's keys and
+// lookups, slots possibly elsewhere within their probe runs
+std::string build_hash_device(const std::vector<uint64_t>& keys, int log2, std::vector<uint64_t>& HK, std::vector<int>& HV);
 // Where prepare() builds the BVH and samples the cache cells: false (default) host, true GPU
 // (set by rc_scene_prep_device).
 extern bool g_prep_device;

@@ -100,7 +100,8 @@ class MLPCache:
         mode: fused paths recording their queries, or the staged pass; the same image); ms when timed"""
         cam = cam or cache.scene.camera()
         ms = C.c_float(0.0)
-        check(lib.rc_mlp_render(self._h, cache._h, C.byref(cam), seed, C.c_void_p(img.data_ptr()), C.byref(ms) if timed else None))
+        check(lib.rc_mlp_render(self._h, cache._h, C.byref(cam), seed, C.c_void_p(img.data_ptr()), C.byref(ms) if timed else None,
+                                C.c_void_p(torch.cuda.current_stream().cuda_stream)))
         return ms.value if timed else None
 
     def get(self, which):

@@ -132,6 +132,41 @@ def test_hash_insert_lookup(torch_gpu, street):
             k = (k + 1) & mask
 
 
+def test_hash_insert_duplicates_in_one_wave(torch_gpu):
+    """duplicate keys inside one wave (1 to 5 copies of a key among 64 lanes, in shuffled lanes, with
+    different values): the ballot groups them and the group's minimum value is stored; every key is
+    inserted once; the empty-key lanes (key 0) insert nothing"""
+    torch = torch_gpu
+    rng = np.random.default_rng(11)
+    distinct = rng.integers(1, 2**63, 6000, dtype=np.uint64)
+    keys, vals = [], []
+    for w in range(0, len(distinct), 20):              # 20 distinct keys per wave of 64 lanes
+        ks = distinct[w:w + 20]
+        reps = rng.integers(1, 6, len(ks))
+        lane_k = np.repeat(ks, reps)[:64]
+        lane_k = np.concatenate([lane_k, np.zeros(64 - len(lane_k), np.uint64)])
+        rng.shuffle(lane_k)
+        keys.append(lane_k)
+        vals.append(rng.integers(0, 1 << 30, 64).astype(np.int32))
+    keys, vals = np.concatenate(keys), np.concatenate(vals)
+    live = keys != 0
+    exp = {}
+    for k, v in zip(keys[live], vals[live]):
+        exp[int(k)] = min(exp.get(int(k), 1 << 31), int(v))
+    size = 1 << 15
+    HK = torch.zeros(size, dtype=torch.int64, device="cuda")
+    HV = torch.full((size,), 2**31 - 1, dtype=torch.int32, device="cuda")
+    fail = torch.zeros(1, dtype=torch.int32, device="cuda")
+    rea.hash_insert(HK, HV, torch.from_numpy(keys.view(np.int64)).cuda(), torch.from_numpy(vals).cuda(), fail)
+    q = np.array(sorted(exp), np.uint64)
+    out = torch.empty(len(q), dtype=torch.int32, device="cuda")
+    rea.hash_query(HK, HV, torch.from_numpy(q.view(np.int64)).cuda(), out)
+    assert int(fail.item()) == 0
+    assert np.array_equal(out.cpu().numpy(), np.array([exp[int(k)] for k in q], np.int32))
+    hk = HK.cpu().numpy().view(np.uint64)
+    assert np.count_nonzero(hk) == len(exp)
+
+
 RUN_METHODS = [
     ("plain", lambda: rea.methods.plain()),
     ("targeted", lambda: rea.methods.targeted()),
@@ -199,6 +234,32 @@ def test_query_bit_exact(torch_gpu, street):
         got = img.cpu().numpy()
         assert np.array_equal(got.view(np.uint32), exp.view(np.uint32)), f"no_cache={no_cache}: {np.sum(got != exp)} pixels differ"
         assert tuple(rays) == (er[0], er[1])
+
+
+def test_render_on_a_side_stream(torch_gpu, street):
+    """the camera pass is ordered against torch's current stream: a fill queued before it on a side
+    stream does not overwrite the image, a copy queued after it sees the whole image, and a clear
+    queued after it is not overwritten by the pass"""
+    torch = torch_gpu
+    scene, _ = street
+    cache = rea.Cache(scene, rea.methods.plain())
+    cache.begin(0, 3)
+    cache.frame(0)
+    cam = scene.camera()
+    ref = torch.zeros(cam.W * cam.H, dtype=torch.float32, device="cuda")
+    cache.render(ref, cam, seed=17)
+    torch.cuda.synchronize()
+    side = torch.cuda.Stream()
+    img = torch.empty_like(ref)
+    with torch.cuda.stream(side):
+        img.fill_(-1.0)
+        cache.render(img, cam, seed=17)
+        snap = img.clone()
+        img.zero_()
+    side.synchronize()
+    assert torch.equal(snap, ref)
+    torch.cuda.synchronize()
+    assert int(torch.count_nonzero(img)) == 0
 
 
 def test_query_stack_levels(torch_gpu, street):

@@ -1,7 +1,9 @@
 """Scene preparation on the GPU: the binned-SAH BVH (cuda/prep_mesh.cuh:42-114) and the cache cells
 sampled, reduced and Morton-ordered (:129-157) by the device path equal the host path's bit for
 bit, so every scene array built from them does too (nodes, triangle order, wide nodes, entry
-points, normals, albedos, materials, zones, hash table)."""
+points, normals, albedos, materials, zones). The hash table (:356-366) is built on the GPU with
+64-bit CAS inserts: the same keys, every key looking up to the same entry, empty slots -1, every
+key within 64 probes of its home slot with no empty slot in between; slots may differ."""
 import time
 
 import numpy as np
@@ -12,7 +14,43 @@ import render_engine_amd as rea
 pytestmark = pytest.mark.gpu
 
 INFO = ("Np", "npst", "nbox", "NR", "hsize", "nmat", "nst", "nnodes", "nwnodes", "bvh_depth", "wide_depth")
-ARRAYS = ("ALB", "REGION", "HK", "HV", "MAT", "MATALB", "ST", "STN", "BN", "WN", "ORDER")
+ARRAYS = ("ALB", "REGION", "MAT", "MATALB", "ST", "STN", "BN", "WN", "ORDER")
+M64 = np.uint64(0xFFFFFFFFFFFFFFFF)
+
+
+def mix64(x):
+    """the slot function (SplitMix64 finaliser), vectorised"""
+    x = np.asarray(x, np.uint64)
+    with np.errstate(over="ignore"):
+        x = x ^ (x >> np.uint64(30)); x = x * np.uint64(0xBF58476D1CE4E5B9)
+        x = x ^ (x >> np.uint64(27)); x = x * np.uint64(0x94D049BB133111EB)
+        x = x ^ (x >> np.uint64(31))
+    return x
+
+
+def _same_table(a, b):
+    HKa, HVa, HKb, HVb = a.array("HK"), a.array("HV"), b.array("HK"), b.array("HV")
+    assert HKa.shape == HKb.shape
+    oa, ob = HKa != 0, HKb != 0
+    ia, ib = np.argsort(HKa[oa]), np.argsort(HKb[ob])
+    assert np.array_equal(HKa[oa][ia], HKb[ob][ib]), "the tables hold different keys"
+    assert np.array_equal(HVa[oa][ia], HVb[ob][ib]), "a key looks up to another entry"
+    assert np.all(HVb[~ob] == -1)
+    size = len(HKb)
+    mask = np.uint64(size - 1)
+    slot = np.nonzero(ob)[0].astype(np.int64)
+    home = (mix64(HKb[ob]) & mask).astype(np.int64)
+    dist = (slot - home) % size
+    assert dist.max() < 64, "a key lies 64 or more probes from its home slot"
+    # start of the occupied run holding each slot (cyclic): the slot after the last empty one before it
+    empty = np.nonzero(~ob)[0]
+    assert len(empty) > 0
+    last_empty = np.full(size, -1, np.int64)
+    last_empty[empty] = empty
+    last_empty = np.maximum.accumulate(last_empty)
+    last_empty[last_empty < 0] = empty[-1] - size     # runs that wrap around the end
+    run = (slot - (last_empty[slot] + 1)) % size
+    assert np.all(dist <= run), "an empty slot lies between a key's home slot and its slot"
 
 
 def _both(make):
@@ -42,6 +80,7 @@ def _same(a, b):
         for name in ("P", "N"):
             x, y = a.array(name, st), b.array(name, st)
             assert np.array_equal(x.view(np.uint8), y.view(np.uint8)), (name, st)
+    _same_table(a, b)
 
 
 def test_prep_device_small_street(torch_gpu):
