@@ -535,17 +535,50 @@ void fine_events(Scene& S, int g) {
 
 // ------------------------------------------------------------------------------------ scene files
 namespace {
+// Reads bounded by the bytes left in the file: a count read from the file can never make a vector
+// larger than what remains to be read (a corrupt count fails the read instead of allocating).
 struct Reader {
     FILE* f = nullptr;
     bool ok = true;
-    template <class T> void rd(std::vector<T>& v, size_t n) {
-        v.resize(n);
-        if (!ok) return;
-        if (n && fread(v.data(), sizeof(T), n, f) != n) ok = false;
+    unsigned long long left = 0;
+    bool open(const char* path) {
+        f = fopen(path, "rb");
+        if (!f) return false;
+        if (fseek(f, 0, SEEK_END) != 0) return false;
+        const long n = ftell(f);
+        if (n < 0 || fseek(f, 0, SEEK_SET) != 0) return false;
+        left = (unsigned long long)n;
+        return true;
     }
-    template <class T> T one() { T x{}; if (ok && fread(&x, sizeof(T), 1, f) != 1) ok = false; return x; }
+    template <class T> void rd(std::vector<T>& v, size_t n) {
+        if (!ok || n > left / sizeof(T)) { ok = false; v.clear(); return; }
+        v.resize(n);
+        if (n && fread(v.data(), sizeof(T), n, f) != n) ok = false;
+        left -= (unsigned long long)n * sizeof(T);
+    }
+    template <class T> T one() {
+        T x{};
+        if (!ok || left < sizeof(T) || fread(&x, sizeof(T), 1, f) != 1) { ok = false; return x; }
+        left -= sizeof(T);
+        return x;
+    }
     ~Reader() { if (f) fclose(f); }
 };
+
+// a binary BVH read from a file: leaves inside the triangle range, inner nodes' children after the node
+// (so no cycle, and the wide collapse terminates) and inside the node range
+static std::string check_bvh(const std::vector<rc_bvh_node>& bn, int nst) {
+    const long long nn = (long long)bn.size();
+    for (long long k = 0; k < nn; ++k) {
+        const rc_bvh_node& b = bn[k];
+        if (b.cnt > 0) {
+            if (b.lf < 0 || (long long)b.lf + b.cnt > nst) return "BVH node " + std::to_string(k) + ": leaf triangles out of range";
+        } else if (b.cnt < 0 || b.lf <= k || (long long)b.lf + 1 >= nn) {
+            return "BVH node " + std::to_string(k) + ": children out of range or not after the node";
+        }
+    }
+    return "";
+}
 }  // namespace
 
 // fine 16x8x6 event voxels of a box scene -> transit voxels they overlap (cuda/poc_gpu.cu:720-735)
@@ -569,14 +602,14 @@ static std::vector<int> fold_to_transit(const std::vector<int>& fine, int VX, in
 
 std::string load_scene_file(const char* path, Scene& S, FileRefs& R) {
     Reader in;
-    in.f = fopen(path, "rb");
-    if (!in.f) return std::string("cannot open ") + path;
+    if (!in.open(path)) return std::string("cannot open ") + path;
     std::vector<int> h;
     in.rd(h, 8);
     if (!in.ok) return "truncated header";
     const int Np = h[0], Nq = h[1], Nt = h[2], NS = h[3], NL = h[4], NR = h[5], has_ref = h[6], fmt = h[7];
     if (fmt != 2 && fmt != 3) return "scene format " + std::to_string(fmt) + " not supported (2 or 3)";
-    if (Np <= 0 || Nq < 0 || Nt < 0 || NS <= 0 || NL <= 0 || NL > RC_MAX_LIGHTS || NR <= 0) return "bad header values";
+    if (Np <= 0 || Nq < 0 || Nt < 0 || NS <= 0 || NL <= 0 || NL > RC_MAX_LIGHTS || NR <= 0 || NR > Np) return "bad header values";
+    if (NS > 64) return "more than 64 states";
     S = Scene();
     S.vx = 8; S.vy = 4; S.vz = 4; S.nw = 4;
     S.Np = Np; S.NS = NS; S.nl = NL; S.NR = NR; S.Nq = Nq;
@@ -588,6 +621,8 @@ std::string load_scene_file(const char* path, Scene& S, FileRefs& R) {
         for (int a = 0; a < 3; ++a) { S.lo[a] = lo[a]; S.ext[a] = ext[a]; }
         S.h = hc[0];
         S.nst = m[0]; S.npst = m[3];
+        if (m[0] < 0 || m[1] < 0 || m[3] < 0 || m[3] > Np) return "bad mesh header counts";
+        if (m[0] > 0 && m[1] == 0) return "static triangles without a BVH";
         if (m[2] <= 0 || (m[2] & (m[2] - 1))) return "hash table size must be a power of two";
         in.rd(S.ST, 9 * (size_t)m[0]); in.rd(S.STN, 4 * (size_t)m[0]);
         std::vector<float> bn; in.rd(bn, 8 * (size_t)m[1]);
@@ -595,7 +630,11 @@ std::string load_scene_file(const char* path, Scene& S, FileRefs& R) {
         if (in.ok && m[1]) memcpy(S.BN.data(), bn.data(), bn.size() * 4);
         in.rd(S.HK, m[2]); in.rd(S.HV, m[2]);
         if (!in.ok) return "truncated mesh data";
-        build_wide(S.BN, S.WN, S.wide_depth);
+        std::string be = check_bvh(S.BN, S.nst);
+        if (!be.empty()) return be;
+        for (size_t k = 0; k < S.HK.size(); ++k)
+            if (S.HK[k] != 0ull && (S.HV[k] < 0 || S.HV[k] >= S.npst)) return "hash value out of the static entries' range";
+        if (!S.BN.empty()) build_wide(S.BN, S.WN, S.wide_depth);
         if (3 * S.wide_depth + 1 > RC_WSTACK) return "wide BVH too deep for the LDS traversal stack";
         if (S.WN.size() >= (1u << 24)) return "wide BVH has more than 2^24 nodes";
         if (S.nst >= (1 << 24)) return "more than 2^24 static triangles (the wide nodes' leaf entries hold 24-bit indices)";
@@ -607,6 +646,7 @@ std::string load_scene_file(const char* path, Scene& S, FileRefs& R) {
     in.rd(S.L, 3 * (size_t)NL); in.rd(S.A, Np); in.rd(S.region, Np);
     S.ltype.assign(NL, 0);
     if (!in.ok) return "truncated light / albedo / region data";
+    for (int i = 0; i < Np; ++i) if (S.region[i] < 0 || S.region[i] >= NR) return "entry region out of range";
     S.matalb = S.A;   // materials: the distinct entry albedos, sorted
     std::sort(S.matalb.begin(), S.matalb.end());
     S.matalb.erase(std::unique(S.matalb.begin(), S.matalb.end()), S.matalb.end());
@@ -856,8 +896,7 @@ std::string save_scene_file(const Scene& S, const FileRefs& R, const char* path)
 
 std::string read_msh(const char* path, Mesh& M) {
     Reader in;
-    in.f = fopen(path, "rb");
-    if (!in.f) return std::string("cannot open ") + path;
+    if (!in.open(path)) return std::string("cannot open ") + path;
     std::vector<char> magic;
     in.rd(magic, 4);
     const int nt = in.one<int>();
