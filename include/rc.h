@@ -117,7 +117,8 @@ rc_status rc_scene_box_room(uint64_t seed, int target_tris, rc_scene** out);
  * {base, night (sun 0), lamp 0 off, 12x10 m occluder inserted}; camera W x H x spp, bounces. */
 rc_status rc_scene_street(uint64_t seed, int nboxes, int target_tris, int nlamps, int sun, int target_points, int table_log2,
                           int W, int H, int spp, int bounces, rc_scene** out);
-/* Prototype scene file, format 2 (box scenes) or 3 (mesh scenes), uncompressed (cuda/poc_gpu.cu:751-801).
+/* Prototype scene file, format 2 (box scenes) or 3 (mesh scenes), plain or gzip-compressed (cuda/poc_gpu.cu:751-801;
+ * the .bin.gz scale scenes, cuda/run_cuda.sh:131).
  * Converged references stored in the file become the scene's references. */
 rc_status rc_scene_load(const char* path, rc_scene** out);
 /* Bench scene around an MSH1 mesh (the prep mode's scene, cuda/prep_mesh.cuh:303-453): 3 point lamps and a cube

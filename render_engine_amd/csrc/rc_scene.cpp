@@ -11,6 +11,8 @@
 #include <numeric>
 #include <unordered_map>
 
+#include <zlib.h>
+
 namespace rc {
 
 bool g_prep_device = false;
@@ -535,34 +537,41 @@ void fine_events(Scene& S, int g) {
 
 // ------------------------------------------------------------------------------------ scene files
 namespace {
-// Reads bounded by the bytes left in the file: a count read from the file can never make a vector
-// larger than what remains to be read (a corrupt count fails the read instead of allocating).
+// The whole file in memory (gzip-compressed files, such as the prototype's .bin.gz scale scenes,
+// decompressed on the way; plain files read as they are), then reads bounded by the bytes left: a count
+// read from the file can never make a vector larger than what remains (a corrupt count fails the read
+// instead of allocating).
 struct Reader {
-    FILE* f = nullptr;
+    std::vector<char> buf;
+    size_t pos = 0;
     bool ok = true;
-    unsigned long long left = 0;
     bool open(const char* path) {
-        f = fopen(path, "rb");
-        if (!f) return false;
-        if (fseek(f, 0, SEEK_END) != 0) return false;
-        const long n = ftell(f);
-        if (n < 0 || fseek(f, 0, SEEK_SET) != 0) return false;
-        left = (unsigned long long)n;
-        return true;
+        gzFile g = gzopen(path, "rb");
+        if (!g) return false;
+        char chunk[1 << 16];
+        int n;
+        while ((n = gzread(g, chunk, sizeof chunk)) > 0) {
+            if (buf.size() + (size_t)n > ((size_t)16 << 30)) { gzclose(g); return false; }   // 16 GiB: not a scene file
+            buf.insert(buf.end(), chunk, chunk + n);
+        }
+        const bool good = n == 0;
+        gzclose(g);
+        return good;
     }
+    size_t left() const { return buf.size() - pos; }
     template <class T> void rd(std::vector<T>& v, size_t n) {
-        if (!ok || n > left / sizeof(T)) { ok = false; v.clear(); return; }
+        if (!ok || n > left() / sizeof(T)) { ok = false; v.clear(); return; }
         v.resize(n);
-        if (n && fread(v.data(), sizeof(T), n, f) != n) ok = false;
-        left -= (unsigned long long)n * sizeof(T);
+        if (n) memcpy(v.data(), buf.data() + pos, n * sizeof(T));
+        pos += n * sizeof(T);
     }
     template <class T> T one() {
         T x{};
-        if (!ok || left < sizeof(T) || fread(&x, sizeof(T), 1, f) != 1) { ok = false; return x; }
-        left -= sizeof(T);
+        if (!ok || left() < sizeof(T)) { ok = false; return x; }
+        memcpy(&x, buf.data() + pos, sizeof(T));
+        pos += sizeof(T);
         return x;
     }
-    ~Reader() { if (f) fclose(f); }
 };
 
 // a binary BVH read from a file: leaves inside the triangle range, inner nodes' children after the node

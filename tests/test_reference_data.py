@@ -1,14 +1,15 @@
 """The runner and the radiance model against the reference's own data (CPU, via the oracle,
 which the GPU path matches bit for bit):
 
-* scene_main.bin (format 2, read by rc_scene_load) holds converged per-point references from the
+* scene_main.bin, scene_holdout.bin (the scene never used for tuning) and scene_stress.bin (8 lights,
+  6 states) (format 2, read by rc_scene_load) hold converged per-point references from the
   prototype's reference solver: the oracle's Jacobi solve on the same scene must agree within the
   references' self-noise;
-* reference/cuda_rtx4060/scen_main.jsonl holds the published error summaries of every method on
-  every scenario of scenarios_main.txt (tests/golden/scen_main_prepost.json): the runner must
-  reproduce them.
+* reference/cuda_rtx4060/scen_{main,holdout,stress}.jsonl hold the published error summaries of every
+  method on every scenario of scenarios_main.txt / scenarios_stress.txt
+  (tests/golden/scen_*_prepost.json): the runner must reproduce them.
 
-Needs the reference checkout (only the scene file is read; skipped where it is absent)."""
+Needs the reference checkout (only the scene files are read; skipped where they are absent)."""
 import json
 import os
 
@@ -21,14 +22,22 @@ from render_engine_amd import scenarios as sc
 
 REF = "/root/reference/cuda/scenes"
 MAIN = os.path.join(REF, "scene_main.bin")
-GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "scen_main_prepost.json")
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "scen_{}_prepost.json")
+SCENARIOS = {"main": "scenarios_main.txt", "holdout": "scenarios_main.txt", "stress": "scenarios_stress.txt"}
 pytestmark = pytest.mark.skipif(not os.path.exists(MAIN), reason="reference scene files not present")
+_SCENES = {}
+
+
+def _scene(name):
+    if name not in _SCENES:
+        s = rea.Scene.load(os.path.join(REF, f"scene_{name}.bin"))
+        _SCENES[name] = (s, po.OracleScene(s))
+    return _SCENES[name]
 
 
 @pytest.fixture(scope="module")
 def main_scene():
-    s = rea.Scene.load(MAIN)
-    return s, po.OracleScene(s)
+    return _scene("main")
 
 
 def test_scene_file_header(main_scene):
@@ -41,9 +50,31 @@ def test_scene_file_header(main_scene):
     assert len(s.array("EM", (1 * i.NS + 0) * i.nl + 0)) >= 1
 
 
-@pytest.mark.parametrize("state", [0, 3])
-def test_converged_solve_matches_stored_references(main_scene, state):
-    s, osc = main_scene
+def test_holdout_and_stress_headers():
+    h, _ = _scene("holdout")
+    st, _ = _scene("stress")
+    assert (h.info.Np, h.info.NS, h.info.nl, h.info.has_ref) == (10728, 5, 3, 1)
+    assert (st.info.Np, st.info.NS, st.info.nl, st.info.has_ref) == (11268, 6, 8, 1)
+    assert len(st.array("TQ", 0)) == 88
+
+
+@pytest.mark.parametrize("name,np_", [("43k", None), ("530k", 539987), ("1M", 1082802)])
+def test_scale_scenes_load(name, np_):
+    """the prototype's scale scenes (format 2, the 530k and 1M ones gzip-compressed; cuda/run_cuda.sh:131),
+    whose entry counts the reference's timing records state (reference/cuda_rtx4060/timing_*.jsonl)"""
+    path = os.path.join(REF, f"scene_scale_{name}.bin" + ("" if name == "43k" else ".gz"))
+    s = rea.Scene.load(path)
+    i = s.info
+    assert (i.NS, i.nl, i.nst, len(s.array("TQ", 0))) == (5, 3, 0, 58)
+    if np_:
+        assert i.Np == np_
+    P = s.array("P", 0).reshape(-1, 3)
+    assert np.all(np.isfinite(P)) and P.min() >= -0.5 and P.max() <= 8.5
+
+
+@pytest.mark.parametrize("name,state", [("main", 0), ("main", 3), ("holdout", 1), ("stress", 0), ("stress", 5)])
+def test_converged_solve_matches_stored_references(name, state):
+    s, osc = _scene(name)
     i = s.info
     ref, valid = s.array("REF", state), s.array("VALID", state).astype(bool)
     Pw = s.array("PW", state).reshape(-1, i.nl)
@@ -59,16 +90,23 @@ def test_converged_solve_matches_stored_references(main_scene, state):
     rel = np.sqrt(np.mean((e[valid] - ref[valid]) ** 2)) / ref[valid].mean()
     assert rel < 0.05, rel                                   # stored refs carry ~3 % self-noise
     assert abs(e[valid].sum() / ref[valid].sum() - 1) < 0.01
-    per_light = est[valid].sum(0) / Pw[valid].sum(0)
+    on = Pw[valid].sum(0) > 0                                  # lights switched off in this state carry nothing
+    per_light = est[valid].sum(0)[on] / Pw[valid].sum(0)[on]
     assert np.all(np.abs(per_light - 1) < 0.015), per_light
+    assert np.all(est[valid][:, ~on] == 0)
 
 
-@pytest.mark.parametrize("scen", ["R1 allumee-eteinte-rallumee", "I1 lampe eteinte", "I3 lampe allumee 1re fois", "I4 boite"])
-def test_runner_reproduces_published_curves(main_scene, scen):
-    s, osc = main_scene
+@pytest.mark.parametrize("name,scen", [("main", "R1 allumee-eteinte-rallumee"), ("main", "I1 lampe eteinte"),
+                                       ("main", "I3 lampe allumee 1re fois"), ("main", "I4 boite"),
+                                       ("holdout", "R2 allumee-eteinte-porte-rallumee"), ("holdout", "I2 porte"),
+                                       ("stress", "G1 mur effondre"), ("stress", "L2 quatre lampes eteintes puis rallumees"),
+                                       ("stress", "L3 trois lampes allumees 1re fois"),
+                                       ("stress", "M1 extinction, mur effondre dans le noir, rallumage")])
+def test_runner_reproduces_published_curves(name, scen):
+    s, osc = _scene(name)
     i = s.info
-    gold = json.load(open(GOLD))[scen]
-    parts = dict(sc.parse_scenarios(open(os.path.join(REF, "scenarios_main.txt")).read()))[scen]
+    gold = json.load(open(GOLD.format(name)))[scen]
+    parts = dict(sc.parse_scenarios(open(os.path.join(REF, SCENARIOS[name])).read()))[scen]
     seq = sc.expand(parts)
     last = len(seq) - parts[-1][1]
     NS = i.NS
