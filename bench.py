@@ -639,7 +639,7 @@ def bench_mlp(args, rea, scene, cfg, log):
                 c.begin(a, 777)
                 ml = M.MLPCache(M.config(list(info.lo), list(info.ext), log2_table=mc["log2_table"], base_res=mc["base_res"],
                                          max_res=mc["max_res"], max_batch=mc["batch"], lr=mc["lr"]))
-                bm, bg, bo, jj, fb = [], [], [], [], 0
+                bm, bg, bo, bq, jj, fb = [], [], [], [], [], 0
                 for f in range(200 + 60):
                     st = a if f < 200 else b
                     c.frame(st)
@@ -647,9 +647,14 @@ def bench_mlp(args, rea, scene, cfg, log):
                     if f >= 200:
                         ml.render(c, img, cam, seed=9000 + f)
                         em = img.cpu().numpy().reshape(cam.H, cam.W)
+                        ml.set_grid_fallback(0.5)   # the grid judges every query: off by > 50 % -> the grid's value
+                        ml.render(c, img, cam, seed=9000 + f)
+                        ml.set_grid_fallback(0.0)
+                        eq = img.cpu().numpy().reshape(cam.H, cam.W)
                         c.render(img, cam, seed=9000 + f)
                         eg = img.cpu().numpy().reshape(cam.H, cam.W)
                         bm.append(sc.region_bias(em, refs[b], region))
+                        bq.append(sc.region_bias(eq, refs[b], region))
                         bg.append(sc.region_bias(eg, refs[b], region))
                         use_grid = j > 0.25
                         fb += int(use_grid)
@@ -658,7 +663,11 @@ def bench_mlp(args, rea, scene, cfg, log):
                 qual[name] = {"region_pixels": int(region.sum()),
                               "mlp": {"frames_to_5pct": sc.frames_to_below(bm, 0.05), "bias_first_10": [round(x, 4) for x in bm[:10]]},
                               "grid_targeted": {"frames_to_5pct": sc.frames_to_below(bg, 0.05)},
-                              "mlp_with_grid_fallback": {"frames_to_5pct": sc.frames_to_below(bo, 0.05), "fallback_frames": fb},
+                              "mlp_with_grid_fallback": {"frames_to_5pct": sc.frames_to_below(bo, 0.05), "fallback_frames": fb,
+                                                         "judge": "global relative RMS of the training samples > 0.25: whole frame"},
+                              "mlp_with_per_query_grid_judge": {"frames_to_5pct": sc.frames_to_below(bq, 0.05),
+                                                                "bias_first_10": [round(x, 4) for x in bq[:10]],
+                                                                "judge": "per query: prediction off the grid's value by > 50 %"},
                               "judge_first_10": jj[:10]}
                 log(f"{name}: {json.dumps(qual[name])}")
         except Exception as e:

@@ -1251,6 +1251,7 @@ struct rc_mlp {
     rc_mlpdev d{};
     rc_mlpbuf b{};
     int step = 0;
+    float fallback = 0.f;   // per-query grid fallback threshold of the camera pass (0: none)
     // online-training workspace (max_batch samples)
     float4 *sP = nullptr, *sN = nullptr, *sD = nullptr, *hP = nullptr, *hN = nullptr, *hD = nullptr;
     float2* AE = nullptr;
@@ -1454,6 +1455,12 @@ extern "C" rc_status rc_mlp_train_scene(rc_mlp* m, rc_cache* c, int n, uint32_t 
     return RC_OK;
 }
 
+extern "C" rc_status rc_mlp_set_grid_fallback(rc_mlp* m, float thr) {
+    if (!m || !(thr >= 0.f)) return fail(RC_ERR_ARG, "rc_mlp_set_grid_fallback: threshold must be >= 0");
+    m->fallback = thr;
+    return RC_OK;
+}
+
 extern "C" rc_status rc_mlp_render(rc_mlp* m, rc_cache* c, const rc_camera* cam, uint32_t seed, float* d_img, float* ms_out, void* stream) {
     if (!m || !c || !cam || !d_img) return fail(RC_ERR_ARG, "NULL argument");
     if (!c->begun) return fail(RC_ERR_STATE, "rc_cache_begin not called");
@@ -1472,10 +1479,17 @@ extern "C" rc_status rc_mlp_render(rc_mlp* m, rc_cache* c, const rc_camera* cam,
         HIPCK(hipStreamWaitEvent(st, c->ev_in, 0));
     }
     if (ms_out) HIPCK(hipEventRecord(c->ev[0], st));
+    rc_mlpdev md = m->d;   // with the grid cache's resolved radiance as the per-query judge when a threshold is set
+    md.jR = m->fallback > 0.f ? c->c.R : nullptr;
+    md.jthr = m->fallback;
+    if (m->fallback > 0.f && !c->r_valid) {   // no frame yet: resolve the warm-start channels
+        HIPCK(rc_launch_resolve(&S->ds, &S->dst[si], c->c.Lc, c->c.R, c->dormant, st));
+        c->r_valid = true;
+    }
     if (c->render_mode == RC_RENDER_STAGED)
-        HIPCK(rc_launch_query_staged(&S->ds, &S->dst[si], nullptr, cam, seed, (uint32_t)c->f, c->dormant, d_img, nullptr, &c->qw, &m->d, st));
+        HIPCK(rc_launch_query_staged(&S->ds, &S->dst[si], nullptr, cam, seed, (uint32_t)c->f, c->dormant, d_img, nullptr, &c->qw, &md, st));
     else
-        HIPCK(rc_launch_query_mlp(&S->ds, &dk, cam, seed, (uint32_t)c->f, c->dormant, d_img, &c->qw, &m->d, st));
+        HIPCK(rc_launch_query_mlp(&S->ds, &dk, cam, seed, (uint32_t)c->f, c->dormant, d_img, &c->qw, &md, st));
     if (stream) {   // the caller's later work sees d_img
         HIPCK(hipEventRecord(c->ev_out, st));
         HIPCK(hipStreamWaitEvent((hipStream_t)stream, c->ev_out, 0));

@@ -927,7 +927,7 @@ __global__ void __launch_bounds__(RC_TB) k_query_mlp(rc_dscene s, rc_dstate st, 
         int e;
         if (rc_resolve_hit_n(s, st, H, d, q, e, n)) break;
         if (b == cam.cache_bounce) {
-            q0 = make_float4(H.x, H.y, H.z, 1.f); q1 = make_float4(n.x, n.y, n.z, 0.f); q2 = make_float4(-d.x, -d.y, -d.z, 0.f);
+            q0 = make_float4(H.x, H.y, H.z, 1.f); q1 = make_float4(n.x, n.y, n.z, __int_as_float(e)); q2 = make_float4(-d.x, -d.y, -d.z, 0.f);
             break;
         }
         const float A = s.Alb[e];
@@ -1049,7 +1049,7 @@ __global__ void __launch_bounds__(RC_QB) k_q_shade(rc_dscene s, rc_dstate st, co
                 float thr = w.thr[p];
                 if (mlp && b == cam.cache_bounce) {   // learned cache: query point, view direction back along the ray
                     ((float4*)w.mq0)[p] = make_float4(H.x, H.y, H.z, 1.f);
-                    ((float4*)w.mq1)[p] = make_float4(nrm.x, nrm.y, nrm.z, 0.f);
+                    ((float4*)w.mq1)[p] = make_float4(nrm.x, nrm.y, nrm.z, __int_as_float(e));   // w: the query's grid entry
                     ((float4*)w.mq2)[p] = make_float4(-d.x, -d.y, -d.z, 0.f);
                 } else if (R && b == cam.cache_bounce) {
                     w.rad[p] = w.rad[p] + thr * R[e];
@@ -1099,11 +1099,19 @@ __global__ void __launch_bounds__(RC_QB) k_q_shade(rc_dscene s, rc_dstate st, co
     if (k == 0) { w.seg[nblk + seg] = ns; w.seg[seg] = nb2; }
 }
 
-// the learned cache's prediction ends the paths that queried it (their last addition)
-__global__ void __launch_bounds__(RC_TB) k_q_mlp_add(rc_qwork w, int npath) {
+// the learned cache's prediction ends the paths that queried it (their last addition). With a judge
+// (jR), the grid cache's resolved radiance at the query's entry (stored in the query's normal row, w)
+// replaces a prediction that differs from it by more than jthr times it: the grid is the fallback
+// wherever it disagrees, e.g. in the region a light-off event just changed.
+__global__ void __launch_bounds__(RC_TB) k_q_mlp_add(rc_qwork w, int npath, const float* __restrict__ jR, float jthr) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= npath || ((const float4*)w.mq0)[p].w == 0.f) return;
-    w.rad[p] = w.rad[p] + w.thr[p] * fmaxf(w.mpred[p], 0.f);
+    float y = fmaxf(w.mpred[p], 0.f);
+    if (jR) {
+        const float r = jR[__float_as_int(((const float4*)w.mq1)[p].w)];
+        if (fabsf(y - r) > jthr * r) y = r;
+    }
+    w.rad[p] = w.rad[p] + w.thr[p] * y;
 }
 
 // rays in the lists (ray counters): sum of the segment counts
@@ -1440,7 +1448,7 @@ hipError_t rc_launch_query_mlp(const rc_dscene* s, const rc_dstate* st, const rc
     RC_LAUNCH_WIDE((k_query_mlp<2, true>), (k_query_mlp<2, false>), false, g, st, s, stream, *s, *st, *cam, seed, frame, dormant, *w, npath);
     hipError_t e = rc_launch_mlp_infer(mlp, (const float4*)w->mq0, (const float4*)w->mq1, (const float4*)w->mq2, npath, w->mpred, stream);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_q_mlp_add, dim3(rc_grid(npath)), dim3(RC_TB), 0, stream, *w, npath);
+    hipLaunchKernelGGL(k_q_mlp_add, dim3(rc_grid(npath)), dim3(RC_TB), 0, stream, *w, npath, mlp->jR, mlp->jthr);
     hipLaunchKernelGGL(k_q_pixels, dim3(rc_grid((long long)cam->W * cam->H)), dim3(RC_TB), 0, stream, *cam, *w, img);
     return hipGetLastError();
 }
@@ -1465,7 +1473,7 @@ hipError_t rc_launch_query_staged(const rc_dscene* s, const rc_dstate* st, const
     if (mlp) {
         hipError_t e = rc_launch_mlp_infer(mlp, (const float4*)w->mq0, (const float4*)w->mq1, (const float4*)w->mq2, npath, w->mpred, stream);
         if (e != hipSuccess) return e;
-        hipLaunchKernelGGL(k_q_mlp_add, dim3(rc_grid(npath)), dim3(RC_TB), 0, stream, *w, npath);
+        hipLaunchKernelGGL(k_q_mlp_add, dim3(rc_grid(npath)), dim3(RC_TB), 0, stream, *w, npath, mlp->jR, mlp->jthr);
     }
     hipLaunchKernelGGL(k_q_pixels, dim3(rc_grid((long long)cam->W * cam->H)), dim3(RC_TB), 0, stream, *cam, *w, img);
     return hipGetLastError();

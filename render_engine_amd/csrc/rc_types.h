@@ -186,6 +186,9 @@ typedef struct rc_mlpdev {
     int32_t log2T; uint32_t dense; // table size 2^log2T per level; bit lv: level lv indexed densely
     int32_t res[RC_MLP_LEVELS];    // grid resolution per level
     float lo[3], iext[3];          // position bounds (lo, 1 / extent)
+    const float* jR;               // camera pass: the grid cache's resolved radiance judging each query (NULL: none)
+    float jthr;                    // a prediction off the grid's value at its entry by more than jthr times it -> the grid's value
+    int32_t pad1;
 } rc_mlpdev;
 typedef struct rc_mlpbuf {
     void* actT; void* gzT;         // BF16 [6][64][cap] activations, [5][64][cap] pre-activation gradients

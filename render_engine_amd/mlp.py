@@ -95,6 +95,11 @@ class MLPCache:
         check(lib.rc_mlp_train_scene(self._h, cache._h, n, seed, C.byref(loss), C.byref(judge)))
         return loss.value, judge.value
 
+    def set_grid_fallback(self, thr):
+        """queries whose prediction differs from the grid cache's resolved radiance at their entry by more
+        than thr times it end on the grid's value (0: never)"""
+        check(lib.rc_mlp_set_grid_fallback(self._h, float(thr)))
+
     def render(self, cache, img, cam=None, seed=7, timed=False):
         """camera pass of the cache's scene ending on the MLP at cam.cache_bounce (the cache's render
         mode: fused paths recording their queries, or the staged pass; the same image); ms when timed"""

@@ -50,6 +50,40 @@ def test_mlp_render_constant_matches_grid(torch_gpu, street):
             assert np.array_equal(ga.view(np.uint32), gb.view(np.uint32)), f"state {st} mode {mode}: {np.sum(ga != gb)} pixels differ"
 
 
+def test_mlp_render_grid_fallback_per_query(torch_gpu, street):
+    """with the grid as per-query judge, a query whose prediction differs from the grid's resolved radiance
+    at its entry by more than the threshold ends on the grid's value: a network that predicts c1 everywhere
+    against a grid holding c1 on half of the entries and c0 far from it elsewhere gives the grid's own image
+    (both schedules); a threshold above the gap gives the network's image; 0 turns the judge off"""
+    torch = torch_gpu
+    c1, c0 = 0.625, 0.25
+    mlp = _mlp(street, log2_table=14, max_res=512, max_batch=4096)
+    W = np.zeros(M.NW, np.float32)
+    for L in range(M.HIDDEN):
+        W[L * 4096 + (38 if L == 0 else 0) * 64 + 0] = 1.0
+    W[M.HIDDEN * 4096] = c1
+    mlp.set(M.MLP_W, W)
+    cache = rea.Cache(street, rea.methods.plain())
+    cache.begin(0, 3)
+    cache.frame(0)
+    Np = street.info.Np
+    R = np.where(np.arange(Np) % 2 == 0, c1, c0).astype(np.float32)
+    cam = street.camera()
+    a = torch.zeros(cam.W * cam.H, dtype=torch.float32, device="cuda")
+    b = torch.zeros_like(a)
+    for mode in (rea.RENDER_FUSED, rea.RENDER_STAGED):
+        cache.set_render_mode(mode)
+        for thr, grid_R in ((0.5, R), (2.0, np.full(Np, c1, np.float32)), (0.0, np.full(Np, c1, np.float32))):
+            mlp.set_grid_fallback(thr)
+            cache.set("R", R)
+            mlp.render(cache, b, cam, seed=5)
+            cache.set("R", grid_R)
+            cache.render(a, cam, seed=5)
+            ga, gb = a.cpu().numpy(), b.cpu().numpy()
+            assert np.array_equal(ga.view(np.uint32), gb.view(np.uint32)), f"mode {mode} thr {thr}: {np.sum(ga != gb)} pixels differ"
+    mlp.set_grid_fallback(0.0)
+
+
 def test_mlp_online_training_against_the_grid(torch_gpu, street):
     torch = torch_gpu
     cache = rea.Cache(street, rea.methods.plain())
