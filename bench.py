@@ -720,7 +720,8 @@ def main():
               f"hash-key ranges, see the design notes; this build measures one GPU's share)", file=sys.stderr)
         sys.exit(2)
     cfg = CONFIGS[args.config]
-    tag = f"{cfg['kind']}{cfg['W']}x{cfg['H']}" if cfg["kind"] == "street" else "room"
+    tag = (f"{cfg['kind']}{cfg['W']}x{cfg['H']}" if cfg["kind"] == "street" else "room") + \
+        ("_adjoint" if "adjoint" in cfg else "") + ("_mlp" if "mlp" in cfg else "")
     if args.pmc_summary:
         summarize_pmc(args.pmc_summary, tag + ("_events" if args.events else ""), f"bench.py --config {args.config}"
                       + (" --events" if args.events else ""))

@@ -583,7 +583,9 @@ __device__ __forceinline__ void rc_rast_seg(const rc_dscene& s, f3 a, f3 b, uint
 }
 
 // Amanatides-Woo walk of segment a->b over a g^3 voxel grid on the scene bounds (the stepping rule
-// of rc_dda_reg), setting each crossed voxel's bit in V; a bit already set costs one load.
+// of rc_dda_reg), setting each crossed voxel's bit in V. Bits are gathered per 32-bit word while the
+// walk stays in it (x steps within a word), then the word costs one load and, for bits not yet set,
+// one atomicOr.
 __device__ __forceinline__ void rc_dda_volume_mark(const rc_dscene& s, int gx, int gy, int gz, f3 a, f3 b, uint32_t* V) {
     const float sx = gx / s.ext[0], sy = gy / s.ext[1], sz = gz / s.ext[2];
     float ax = (a.x - s.lo[0]) * sx, ay = (a.y - s.lo[1]) * sy, az = (a.z - s.lo[2]) * sz;
@@ -596,14 +598,19 @@ __device__ __forceinline__ void rc_dda_volume_mark(const rc_dscene& s, int gx, i
     float tmx = dx != 0 ? ((dx > 0 ? ((float)(ix + 1) - ax) : (ax - (float)ix)) * tdx) : 1e30f;
     float tmy = dy != 0 ? ((dy > 0 ? ((float)(iy + 1) - ay) : (ay - (float)iy)) * tdy) : 1e30f;
     float tmz = dz != 0 ? ((dz > 0 ? ((float)(iz + 1) - az) : (az - (float)iz)) * tdz) : 1e30f;
+    int64_t cw = -1;     // word whose bits are being gathered
+    uint32_t acc = 0u;
     for (int g = 0; g < gx + gy + gz; ++g) {
-        int64_t v = ((int64_t)iz * gy + iy) * gx + ix;
-        uint32_t bit = 1u << (v & 31);
-        uint32_t* w = V + (v >> 5);
-        if (!(*w & bit)) atomicOr(w, bit);
+        const int64_t v = ((int64_t)iz * gy + iy) * gx + ix;
+        if ((v >> 5) != cw) {
+            if (cw >= 0 && (V[cw] & acc) != acc) atomicOr(V + cw, acc);
+            cw = v >> 5; acc = 0u;
+        }
+        acc |= 1u << (v & 31);
         if (ix == ex && iy == ey && iz == ez) break;
         if (tmx < tmy && tmx < tmz) { ix += stx; tmx += tdx; if (ix < 0 || ix >= gx) break; }
         else if (tmy < tmz) { iy += sty; tmy += tdy; if (iy < 0 || iy >= gy) break; }
         else { iz += stz; tmz += tdz; if (iz < 0 || iz >= gz) break; }
     }
+    if ((V[cw] & acc) != acc) atomicOr(V + cw, acc);
 }

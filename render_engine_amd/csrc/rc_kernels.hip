@@ -68,12 +68,12 @@ __device__ __forceinline__ float rc_est_of(const float* Lc, int64_t i, int nl, u
 }
 
 // Per-entry sum of the light channels over the active lights, in light order (rc_est_of's sum, bit for
-// bit) for every entry: a block of RC_TB threads stages the contiguous Lc span of 64 entries in LDS with
-// coalesced loads (rows padded to an odd pitch: conflict-free row reads), then one thread per entry
-// adds its row. Replaces one lane per entry walking its own row, whose loads touch 64 rows at once.
-#define RC_RSE 64
+// bit) for every entry: a block of RC_TB threads stages the contiguous Lc span of its RC_TB entries in LDS
+// with coalesced loads (rows padded to an odd pitch: conflict-free row reads), then every thread adds one
+// entry's row. Replaces one lane per entry walking its own row, whose loads touch 64 rows at once.
+#define RC_RSE RC_TB
 __global__ void __launch_bounds__(RC_TB) k_rowsum(const float* __restrict__ Lc, int Np, int nl, uint64_t dormant, float* __restrict__ out) {
-    __shared__ float s_r[RC_RSE * (RC_MAX_LIGHTS + 1)];
+    extern __shared__ float s_r[];   // [RC_RSE][nl | 1]
     const int t = threadIdx.x, base = blockIdx.x * RC_RSE;
     const int T = min(RC_RSE, Np - base), pitch = nl | 1, nel = T * nl;
     const float* __restrict__ src = Lc + (int64_t)base * nl;
@@ -1283,7 +1283,8 @@ hipError_t rc_launch_trace2(int trav, int deps, const rc_dscene* s, const rc_dst
 }
 
 static void rc_rowsum(const rc_dcache* c, int Np, int nl, uint64_t dormant, hipStream_t stream) {
-    hipLaunchKernelGGL(k_rowsum, dim3((Np + RC_RSE - 1) / RC_RSE), dim3(RC_TB), 0, stream, (const float*)c->Lc, Np, nl, dormant, c->rsum);
+    hipLaunchKernelGGL(k_rowsum, dim3((Np + RC_RSE - 1) / RC_RSE), dim3(RC_TB), (size_t)RC_RSE * (nl | 1) * sizeof(float), stream,
+                       (const float*)c->Lc, Np, nl, dormant, c->rsum);
 }
 hipError_t rc_launch_decide(const rc_dscene* s, const rc_dcache* c, const rc_params* p, hipStream_t stream) {
     const int nblk = rc_grid(p->Np);

@@ -95,6 +95,33 @@ def test_hash_point_and_dda(torch_gpu, street):
     assert np.array_equal(dm.cpu().numpy().view(np.uint32), em)
 
 
+def test_dda_volume_bit_exact(torch_gpu, street):
+    """rc_dda_volume (Amanatides-Woo over a fine bit volume, bits gathered per word before the atomicOr)
+    against the oracle's walk: three volumes chosen per segment, a non-cubic 40 x 24 x 17 grid, segments
+    inside, crossing and leaving the bounds, axis-aligned and degenerate ones"""
+    torch = torch_gpu
+    scene, osc = street
+    i = scene.info
+    rng = np.random.default_rng(8)
+    n = 30000
+    lo, ext = np.array(list(i.lo)), np.array(list(i.ext))
+    A = (lo + ext * rng.uniform(-0.1, 1.1, (n, 3))).astype(np.float32)
+    B = (lo + ext * rng.uniform(-0.1, 1.1, (n, 3))).astype(np.float32)
+    B[:500] = A[:500]                                     # zero length
+    B[500:1500, 1:] = A[500:1500, 1:]                     # along x only
+    B[1500:2500, :2] = A[1500:2500, :2]                   # along z only
+    sv = rng.integers(0, 3, n).astype(np.int32)
+    gx, gy, gz = 40, 24, 17
+    wpv = (gx * gy * gz + 31) // 32
+    vol = torch.zeros(3 * wpv, dtype=torch.int32, device="cuda")
+    rea.dda_volume(scene, (gx, gy, gz), torch.from_numpy(A).cuda(), torch.from_numpy(B).cuda(), torch.from_numpy(sv).cuda(), vol, wpv)
+    exp = np.zeros(3 * wpv, np.uint32)
+    po.lib.orc_dda_volume(osc.state(0), gx, gy, gz, po.ptr(A), po.ptr(B), po.ptr(sv), n, po.ptr(exp), wpv)
+    got = vol.cpu().numpy().view(np.uint32)
+    assert np.array_equal(got, exp), f"{np.count_nonzero(got != exp)} words differ"
+    assert np.count_nonzero(exp) > 0.5 * len(exp)
+
+
 def test_hash_insert_lookup(torch_gpu, street):
     torch = torch_gpu
     scene, _ = street
