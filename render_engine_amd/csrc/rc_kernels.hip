@@ -68,10 +68,12 @@ __device__ __forceinline__ float rc_est_of(const float* Lc, int64_t i, int nl, u
 }
 
 // Per-entry sum of the light channels over the active lights, in light order (rc_est_of's sum, bit for
-// bit) for every entry: a block of RC_TB threads stages the contiguous Lc span of its RC_TB entries in LDS
-// with coalesced loads (rows padded to an odd pitch: conflict-free row reads), then every thread adds one
-// entry's row. Replaces one lane per entry walking its own row, whose loads touch 64 rows at once.
-#define RC_RSE RC_TB
+// bit) for every entry: a block of RC_TB threads stages the contiguous Lc span of 64 entries in LDS with
+// coalesced loads (rows padded to an odd pitch: conflict-free row reads), then one thread per entry adds
+// its row. Replaces one lane per entry walking its own row, whose loads touch 64 rows at once. (256
+// entries per block, every thread summing, measured 2.5 times slower: its 66 KB of LDS leave 8 waves
+// per CU for the serial sums.)
+#define RC_RSE 64
 __global__ void __launch_bounds__(RC_TB) k_rowsum(const float* __restrict__ Lc, int Np, int nl, uint64_t dormant, float* __restrict__ out) {
     extern __shared__ float s_r[];   // [RC_RSE][nl | 1]
     const int t = threadIdx.x, base = blockIdx.x * RC_RSE;
