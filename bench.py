@@ -702,7 +702,8 @@ def main():
     ap.add_argument("--steps", type=int, default=30)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--config", type=int, default=3, choices=sorted(CONFIGS))
-    ap.add_argument("--trav", type=int, default=2)
+    ap.add_argument("--trav", type=int, default=2, choices=[0, 1, 2, 3],
+                    help="0 binary (prototype order), 1 binary, 2 4-wide fp32 nodes, 3 camera pass on 4-wide quantized nodes")
     ap.add_argument("--render", choices=["staged", "fused"], default="fused",
                     help="camera-pass schedule (same image): staged ray lists or one thread per path")
     ap.add_argument("--stack-levels", type=int, default=0,
@@ -866,7 +867,7 @@ def main():
                                 f"plain cache; camera paths end on the cache at bounce {cb} (up to {cam.bounces} bounces "
                                 f"without a cache)")
                    if cfg["kind"] == "street" else f"box room {info.nst} tris, {info.Np} entries, {cam.W}x{cam.H}",
-                   "config_index": args.config, "traversal": ["binary-ref", "binary", "wide4"][args.trav],
+                   "config_index": args.config, "traversal": ["binary-ref", "binary", "wide4", "wide4-quantized"][args.trav],
                    "render": args.render if args.trav == 2 else "fused",
                    "stack_levels_lds": args.stack_levels or "auto"},
         "cache_mpts_s": round(info.Np * args.steps / elapsed / 1e6, 2),

@@ -134,7 +134,9 @@ rc_status rc_scene_info_get(const rc_scene* s, rc_scene_info* out);
 size_t rc_scene_array_bytes(const rc_scene* s, int which, int state);
 rc_status rc_scene_array(const rc_scene* s, int which, int state, void* dst, size_t bytes);
 /* Device upload of the scene and the per-state build (k_build, cuda/poc_gpu.cu:369-391).
- * trav: 0 = binary BVH in the prototype's order, 1 = binary closest-hit, 2 = 4-wide closest-hit. */
+ * trav: 0 = binary BVH in the prototype's order, 1 = binary closest-hit, 2 = 4-wide closest-hit, 3 = 2 with the
+ * camera pass and rc_trace walking the 4-wide nodes with quantized child boxes (64-byte nodes: per axis an
+ * origin, a power-of-two step and 8-bit planes; the same hits, the other kernels keep the fp32 nodes). */
 rc_status rc_scene_upload(rc_scene* s, int trav);
 rc_status rc_scene_set_trav(rc_scene* s, int trav);
 /* Fine voxel grid g^3 of the per-light volumes (before upload; 0 disables). Every lit shadow segment

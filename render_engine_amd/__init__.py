@@ -13,9 +13,9 @@ from ._ffi import A, CA, Camera, FrameStats, Method, RcError, SceneInfo, check, 
 from . import methods, scenarios  # noqa: F401
 
 __all__ = ["Scene", "Cache", "Method", "Camera", "RcError", "methods", "scenarios", "device_count", "trace", "hash_insert",
-           "hash_query", "hash_point", "dda_mark", "dda_volume", "TRAV_REF", "TRAV_BINARY", "TRAV_WIDE"]
+           "hash_query", "hash_point", "dda_mark", "dda_volume", "TRAV_REF", "TRAV_BINARY", "TRAV_WIDE", "TRAV_WIDE_Q"]
 
-TRAV_REF, TRAV_BINARY, TRAV_WIDE = 0, 1, 2
+TRAV_REF, TRAV_BINARY, TRAV_WIDE, TRAV_WIDE_Q = 0, 1, 2, 3   # 3: camera pass and trace() on quantized wide nodes
 RENDER_FUSED, RENDER_STAGED = 0, 1
 # compact snapshot record (rc_snap_rec, 24 bytes)
 SNAP_REC = np.dtype([("index", np.uint32), ("n", np.float32), ("key", np.uint64), ("radiance", np.float32), ("age", np.float32)])

@@ -58,6 +58,10 @@ template <class T> static hipError_t dalloc(T** d, size_t n, std::vector<void*>&
     return hipMemset(*d, 0, std::max<size_t>(n * sizeof(T), 16));
 }
 
+// traversal modes 2 and 3 share every kernel but the camera pass and rc_trace, which walk the
+// quantized nodes in mode 3; the others take mode 2 (fp32 wide nodes) there
+static inline int trav_fp(int trav) { return trav == 3 ? 2 : trav; }
+
 struct rc_scene {
     rc::Scene H;
     bool uploaded = false;
@@ -95,7 +99,7 @@ struct rc_scene {
 static hipError_t split_dst(const rc_dstate& base, const rc_dscene& ds, int trav, int cap, bool force, uint32_t** buf, size_t* words,
                             long long nthreads, hipStream_t sync, rc_dstate* out) {
     rc_dstate d = base;
-    d.ocap = trav == 2 ? cap : 0;
+    d.ocap = trav >= 2 ? cap : 0;
     d.oforce = force ? 1 : 0;
     size_t need = rc_ovf_words(&d, &ds, nthreads);
     if (need > ((size_t)1 << 30)) {   // beyond the 32-bit byte offsets of the global levels: every level in LDS
@@ -291,14 +295,14 @@ extern "C" rc_status rc_scene_set_fine_grid(rc_scene* s, int g) {
 }
 
 extern "C" rc_status rc_scene_set_trav(rc_scene* s, int trav) {
-    if (!s || trav < 0 || trav > 2) return fail(RC_ERR_ARG, "trav must be 0, 1 or 2");
+    if (!s || trav < 0 || trav > 3) return fail(RC_ERR_ARG, "trav must be 0, 1, 2 or 3");
     s->trav = trav;
     return RC_OK;
 }
 
 extern "C" rc_status rc_scene_upload(rc_scene* s, int trav) {
     if (!s) return fail(RC_ERR_ARG, "NULL scene");
-    if (trav < 0 || trav > 2) return fail(RC_ERR_ARG, "trav must be 0, 1 or 2");
+    if (trav < 0 || trav > 3) return fail(RC_ERR_ARG, "trav must be 0, 1, 2 or 3");
     if (s->uploaded) return fail(RC_ERR_STATE, "scene already uploaded");
     rc_status st = need_device();
     if (st != RC_OK) return st;
@@ -306,16 +310,28 @@ extern "C" rc_status rc_scene_upload(rc_scene* s, int trav) {
     rc::Scene& H = s->H;
     HIPCK(hipStreamCreate(&s->stream));
     rc_dscene& d = s->ds;
-    float *dST, *dST4, *dSTN, *dL, *dA; rc_bvh_node* dBN; rc_wnode* dWN; uint64_t* dHK; int *dHV, *dLT, *dR;
+    float *dST, *dST4, *dSTN, *dL, *dA; rc_bvh_node* dBN; rc_wnode* dWN; rc_qnode* dQN; uint64_t* dHK; int *dHV, *dLT, *dR;
     std::vector<float> st4((size_t)12 * H.nst, 0.f);
     for (int j = 0; j < H.nst; ++j)
         for (int r = 0; r < 3; ++r)
             for (int x = 0; x < 3; ++x) st4[12 * (size_t)j + 4 * r + x] = H.ST[9 * (size_t)j + 3 * r + x];
     HIPCK(dput(&dST, H.ST, s->owned)); HIPCK(dput(&dST4, st4, s->owned)); HIPCK(dput(&dSTN, H.STN, s->owned)); HIPCK(dput(&dBN, H.BN, s->owned));
+    {   // quantized wide nodes, step floor from the largest coordinate of the static triangles
+        float bound = 0.f;
+        for (size_t j = 0; j < (size_t)H.nst; ++j)   // v0, v0 + e1 and v0 + e2 of every triangle
+            for (int x = 0; x < 3; ++x) {
+                const float v = H.ST[9 * j + x], e1 = H.ST[9 * j + 3 + x], e2 = H.ST[9 * j + 6 + x];
+                bound = std::max(bound, std::max(std::fabs(v), std::max(std::fabs(v + e1), std::fabs(v + e2))));
+            }
+        for (int a = 0; a < 3; ++a) bound = std::max(bound, std::max(std::fabs(H.lo[a]), std::fabs(H.lo[a] + H.ext[a])));
+        std::vector<rc_qnode> qn;
+        rc::build_qnodes(H.WN, bound, qn);
+        HIPCK(dput(&dQN, qn, s->owned));
+    }
     HIPCK(dput(&dWN, H.WN, s->owned)); HIPCK(dput(&dHK, H.HK, s->owned)); HIPCK(dput(&dHV, H.HV, s->owned));
     HIPCK(dput(&dL, H.L, s->owned)); HIPCK(dput(&dLT, H.ltype, s->owned)); HIPCK(dput(&dA, H.A, s->owned));
     HIPCK(dput(&dR, H.region, s->owned));
-    d.ST = dST; d.ST4 = dST4; d.STN = dSTN; d.BN = dBN; d.WN = dWN; d.HK = dHK; d.HV = dHV; d.L = dL; d.ltype = dLT; d.Alb = dA; d.region = dR;
+    d.ST = dST; d.ST4 = dST4; d.STN = dSTN; d.BN = dBN; d.WN = dWN; d.QN = dQN; d.HK = dHK; d.HV = dHV; d.L = dL; d.ltype = dLT; d.Alb = dA; d.region = dR;
     d.hmask = H.HK.empty() ? 0u : (uint32_t)(H.HK.size() - 1); d.hcell = H.h;
     for (int a = 0; a < 3; ++a) { d.lo[a] = H.lo[a]; d.ext[a] = H.ext[a]; }
     d.sun_len = H.sun_len; d.nst = H.nst; d.nl = H.nl; d.Np = H.Np; d.NR = H.NR;
@@ -350,7 +366,7 @@ extern "C" rc_status rc_scene_upload(rc_scene* s, int trav) {
     for (int k = 0; k < H.NS; ++k) {
         rc_dstate dk;
         HIPCK(scene_dst(s, k, grid_threads(H.Np), &dk));
-        HIPCK(rc_launch_build(trav, &s->ds, &dk, s->stream));
+        HIPCK(rc_launch_build(trav_fp(trav), &s->ds, &dk, s->stream));
     }
     HIPCK(hipStreamSynchronize(s->stream));
     if (s->has_ref) {   // references read from a scene file
@@ -396,7 +412,7 @@ extern "C" rc_status rc_scene_reference(rc_scene* s, int K, int warm, int block,
                     uint32_t seed = (uint32_t)rc::mix64(0x51ull + 7919ull * si + 104729ull * c + 31ull * it);
                     rc_dstate dk;
                     TCK(scene_dst(s, si, grid_threads(Np), &dk));
-                    TCK(rc_launch_ref(s->trav, &s->ds, &dk, a, b, acc[c], closec, seed, K, it >= warm, c == 0, s->stream));
+                    TCK(rc_launch_ref(trav_fp(s->trav), &s->ds, &dk, a, b, acc[c], closec, seed, K, it >= warm, c == 0, s->stream));
                     std::swap(a, b);
                 }
                 if (a != chain[c]) TCK(hipMemcpyAsync(chain[c], a, nl * Np * 4, hipMemcpyDeviceToDevice, s->stream));
@@ -816,7 +832,7 @@ extern "C" rc_status rc_cache_frame(rc_cache* c, int si, int want_err, int timed
     bool track = M.deps && (c->f % M.track_every == 0);
     rc_dstate dk;
     HIPCK(scene_dst(S, si, grid_threads(p.Np), &dk));
-    HIPCK(rc_launch_trace2(S->trav, track ? 1 : 0, &S->ds, &dk, &d, &p, c->lnew, st));
+    HIPCK(rc_launch_trace2(trav_fp(S->trav), track ? 1 : 0, &S->ds, &dk, &d, &p, c->lnew, st));
     if (timed) HIPCK(hipEventRecord(c->ev[2], st));
     if (c->pend_left > 0) {
         HIPCK(hipMemsetAsync(d.npend + 1, 0, 4, st));
@@ -1028,7 +1044,7 @@ static hipStream_t pick(rc_scene* s, void* stream) { return stream ? (hipStream_
 extern "C" rc_status rc_trace(rc_scene* s, int state, const float* d_O, const float* d_D, int n, float* d_t, int* d_q, int trav, void* stream) {
     if (!s || n < 0 || (n > 0 && (!d_O || !d_D || !d_t || !d_q))) return fail(RC_ERR_ARG, "rc_trace: bad arguments");
     if (!s->uploaded) return fail(RC_ERR_STATE, "scene not uploaded");
-    if (state < 0 || state >= s->H.NS || trav < 0 || trav > 2) return fail(RC_ERR_ARG, "rc_trace: state / trav out of range");
+    if (state < 0 || state >= s->H.NS || trav < 0 || trav > 3) return fail(RC_ERR_ARG, "rc_trace: state / trav out of range");
     HIPCK(rc_launch_trace_rays(trav, &s->ds, &s->dst[state], d_O, d_D, n, d_t, d_q, pick(s, stream)));
     return RC_OK;
 }
@@ -1142,7 +1158,7 @@ extern "C" rc_status rc_adjoint_create(rc_scene* s, int state, int K, int bounce
     AK(hipMemcpyAsync(w.pmat, pmat.data(), Np * 4, hipMemcpyHostToDevice, s->stream));
     // the handle's kernels run on the caller's stream: a global stack buffer of their own
     AK(split_dst(s->dst[state], s->ds, s->trav, s->lcap, s->lforce, &a->ovf, &a->ovf_words, grid_threads(H.Np), s->stream, &a->dst));
-    AK(rc_launch_adj_unit(s->trav, &s->ds, &a->dst, w.G, w.SG, s->stream));
+    AK(rc_launch_adj_unit(trav_fp(s->trav), &s->ds, &a->dst, w.G, w.SG, s->stream));
     AK(hipStreamSynchronize(s->stream));
 #undef AK
     *out = a;
@@ -1154,7 +1170,7 @@ extern "C" void rc_adjoint_destroy(rc_adjoint* a) { delete a; }
 extern "C" rc_status rc_adjoint_forward(rc_adjoint* a, const float* d_I, const float* d_alb, float* d_R, void* stream) {
     if (!a || !d_I || !d_alb) return fail(RC_ERR_ARG, "NULL argument");
     rc_scene* s = a->S;
-    HIPCK(rc_launch_adj_forward(s->trav, &s->ds, &a->dst, &a->w, d_I, d_alb, a->K, a->B, a->seed, d_R, (hipStream_t)stream,
+    HIPCK(rc_launch_adj_forward(trav_fp(s->trav), &s->ds, &a->dst, &a->w, d_I, d_alb, a->K, a->B, a->seed, d_R, (hipStream_t)stream,
                                 a->timed ? a->tev : nullptr));
     a->forward_done = true;
     return a->timed ? adj_trace_time(a, &a->trace_ms[0]) : RC_OK;
@@ -1164,7 +1180,7 @@ extern "C" rc_status rc_adjoint_backward(rc_adjoint* a, const float* d_gR, float
     if (!a || !d_gR || !d_gI || !d_galb) return fail(RC_ERR_ARG, "NULL argument");
     if (!a->forward_done) return fail(RC_ERR_STATE, "rc_adjoint_backward before rc_adjoint_forward");
     rc_scene* s = a->S;
-    HIPCK(rc_launch_adj_backward(s->trav, &s->ds, &a->dst, &a->w, d_gR, a->K, a->B, a->seed, d_gI, d_galb, (hipStream_t)stream,
+    HIPCK(rc_launch_adj_backward(trav_fp(s->trav), &s->ds, &a->dst, &a->w, d_gR, a->K, a->B, a->seed, d_gI, d_galb, (hipStream_t)stream,
                                  a->timed ? a->tev : nullptr));
     a->backward_done = true;
     return a->timed ? adj_trace_time(a, &a->trace_ms[1]) : RC_OK;
@@ -1235,7 +1251,7 @@ static rc_status cache_grad_masks(rc_cache* c) {
         if (r != RC_OK) return r;
         uint64_t* g = nullptr;
         hipError_t e = dalloc(&g, Np, c->owned);
-        if (e == hipSuccess) e = rc_launch_adj_forward(S->trav, &S->ds, &a->dst, &a->w, dI, dalb, a->K, a->B, a->seed, nullptr, S->stream);
+        if (e == hipSuccess) e = rc_launch_adj_forward(trav_fp(S->trav), &S->ds, &a->dst, &a->w, dI, dalb, a->K, a->B, a->seed, nullptr, S->stream);
         if (e == hipSuccess) e = hipMemcpyAsync(g, a->w.SR, Np * 8, hipMemcpyDeviceToDevice, S->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(S->stream);
         rc_adjoint_destroy(a);
@@ -1437,7 +1453,7 @@ extern "C" rc_status rc_mlp_train_scene(rc_mlp* m, rc_cache* c, int n, uint32_t 
     }
     rc_dstate dk;
     HIPCK(scene_dst(S, si, grid_threads(n), &dk));
-    HIPCK(rc_launch_mlp_samples(S->trav, &S->ds, &dk, c->dormant, seed, n, m->sP, m->sN, m->sD, m->hP, m->hN, m->hD, m->AE, m->ent, st));
+    HIPCK(rc_launch_mlp_samples(trav_fp(S->trav), &S->ds, &dk, c->dormant, seed, n, m->sP, m->sN, m->sD, m->hP, m->hN, m->hD, m->AE, m->ent, st));
     HIPCK(rc_launch_mlp_infer(&m->d, m->hP, m->hN, m->hD, n, m->hy, st));
     HIPCK(rc_launch_mlp_targets(n, m->hP, m->hy, m->AE, m->tg, st));
     HIPCK(rc_launch_mlp_zero(m->b.loss, 1, st));
@@ -1467,7 +1483,7 @@ extern "C" rc_status rc_mlp_render(rc_mlp* m, rc_cache* c, const rc_camera* cam,
     if (cam->W < 1 || cam->H < 1 || cam->spp < 1 || cam->bounces < 0 || cam->cache_bounce < 0 || (long long)cam->W * cam->H * cam->spp > (1ll << 30))
         return fail(RC_ERR_ARG, "camera: bad size / spp / bounces");
     rc_scene* S = c->S;
-    if (S->trav != 2) return fail(RC_ERR_STATE, "the learned-cache camera pass runs on the 4-wide BVH (trav 2)");
+    if (S->trav < 2) return fail(RC_ERR_STATE, "the learned-cache camera pass runs on the 4-wide BVH (trav 2 or 3)");
     hipStream_t st = S->stream;
     const int si = c->prev_state;
     HIPCK(wait_query(c));   // the grid cache's camera pass may still use the staged workspace

@@ -267,6 +267,39 @@ __device__ __forceinline__ void rc_child_enter(const rc_wray& r, const rc_dscene
 #undef RC_SLAB
 }
 
+// The same for a quantized node (rc_qnode, byte offset node into QN): per axis the base
+// (o - ray origin) * inv and the step s * inv (exact: s is a power of two), the near and far
+// plane words picked by the direction's sign, then per child and plane one byte conversion and
+// one fma. The interval contains the fp32 box's (one step of slack on every side), so a box the
+// fp32 test enters is entered here too.
+__device__ __forceinline__ float4 rc_qrow(const rc_dscene& s, uint32_t off) {
+    return *(const float4*)((const char*)s.QN + off);
+}
+__device__ __forceinline__ int4 rc_qrow_i(const rc_dscene& s, uint32_t off) {
+    return *(const int4*)((const char*)s.QN + off);
+}
+__device__ __forceinline__ float rc_u8f(uint32_t w, int k) { return (float)((w >> (8 * k)) & 255u); }
+__device__ __forceinline__ void rc_child_enter_q(const rc_wray& r, const rc_dscene& s, uint32_t node, float tmax, int4 ch,
+                                                 float& t0, float& t1, float& t2, float& t3, int& e0, int& e1, int& e2, int& e3) {
+    const float4 a = rc_qrow(s, node), b = rc_qrow(s, node + 16u), c = rc_qrow(s, node + 32u);
+    const float bx = (a.x - r.o.x) * r.inv.x, by = (a.y - r.o.y) * r.inv.y, bz = (a.z - r.o.z) * r.inv.z;
+    const float sx = a.w * r.inv.x, sy = b.x * r.inv.y, sz = b.y * r.inv.z;
+    const uint32_t lx = __float_as_uint(b.z), ly = __float_as_uint(b.w), lz = __float_as_uint(c.x);
+    const uint32_t hx = __float_as_uint(c.y), hy = __float_as_uint(c.z), hz = __float_as_uint(c.w);
+    const bool px = r.nx == 0u, py = r.ny == 16u, pz = r.nz == 32u;   // direction component >= 0
+    const uint32_t nx = px ? lx : hx, fx = px ? hx : lx, ny = py ? ly : hy, fy = py ? hy : ly, nz = pz ? lz : hz, fz = pz ? hz : lz;
+#define RC_QSLAB(tc, k, cc, e)                                                                                       \
+    {                                                                                                               \
+        const float tn = fmaxf(fmaxf(fmaf(rc_u8f(nx, k), sx, bx), fmaf(rc_u8f(ny, k), sy, by)), fmaxf(fmaf(rc_u8f(nz, k), sz, bz), 0.f)); \
+        const float tf = fminf(fminf(fmaf(rc_u8f(fx, k), sx, bx), fmaf(rc_u8f(fy, k), sy, by)), fminf(fmaf(rc_u8f(fz, k), sz, bz), tmax)); \
+        const bool in_ = tn <= tf, leaf_ = ch.cc < 0;                                                               \
+        tc = in_ && !leaf_ ? tn : 1e30f;                                                                            \
+        e = in_ && leaf_ ? ch.cc : 0;                                                                               \
+    }
+    RC_QSLAB(t0, 0, x, e0) RC_QSLAB(t1, 1, y, e1) RC_QSLAB(t2, 2, z, e2) RC_QSLAB(t3, 3, w, e3)
+#undef RC_QSLAB
+}
+
 // four floats at a 4-byte aligned address, read by one 16-byte load
 typedef float rc_f4u __attribute__((ext_vector_type(4), aligned(4)));
 
@@ -321,7 +354,8 @@ struct rc_lstk_s {
 // and the hit itself does not depend on the order. A leaf longer than 127 triangles (only at
 // the builder's depth limit) is walked at once. The stack depth is bounded by the scene's
 // wstack levels (3 per wide level); the levels beyond lstk.cap live in global memory.
-template <class LS>
+// QN: the node steps read the quantized nodes (rc_qnode) instead of the fp32 ones.
+template <bool QN = false, class LS>
 __device__ __forceinline__ void rc_trace_wide_closest(f3 o, f3 d, const rc_dscene& s, float& tmin, int& q, const LS& lstk) {
     uint32_t* que = lstk.queue();
     const rc_wray r = rc_wray_make(o, d);
@@ -346,17 +380,18 @@ __device__ __forceinline__ void rc_trace_wide_closest(f3 o, f3 d, const rc_dscen
             continue;
         }
         if (!trav) continue;
-        const uint32_t node = (uint32_t)cur << 7;
-        const int4 ch = rc_wrow_i(s, node + 96u);
+        const uint32_t node = QN ? (uint32_t)cur << 6 : (uint32_t)cur << 7;
+        const int4 ch = QN ? rc_qrow_i(s, node + 48u) : rc_wrow_i(s, node + 96u);
         float t0, t1, t2, t3;
         int e0, e1, e2, e3;
-        rc_child_enter(r, s, node, tmin, ch, t0, t1, t2, t3, e0, e1, e2, e3);
+        if (QN) rc_child_enter_q(r, s, node, tmin, ch, t0, t1, t2, t3, e0, e1, e2, e3);
+        else rc_child_enter(r, s, node, tmin, ch, t0, t1, t2, t3, e0, e1, e2, e3);
         int i0 = ch.x, i1 = ch.y, i2 = ch.z, i3 = ch.w;
 #define RC_QUEUE(e, cw)                                                                         \
         if (e != 0) {                                                                           \
             if (e & 127) { que[nq * 256] = (uint32_t)e & 0x7fffffffu; ++nq; }                   \
-            else {   /* more than 127 triangles: their end in row 7 */                          \
-                const int a_ = (int)(((uint32_t)e >> 7) & 0xffffffu), b_ = rc_wrow_i(s, node + 112u).cw; \
+            else {   /* more than 127 triangles: their end in row 7 of the fp32 node */         \
+                const int a_ = (int)(((uint32_t)e >> 7) & 0xffffffu), b_ = rc_wrow_i(s, ((uint32_t)cur << 7) + 112u).cw; \
                 for (int j = a_; j < b_; ++j) {                                                 \
                     const float t = rc_tri_static4(r.o, r.d, (const float4*)s.ST4 + 3 * (int64_t)j); \
                     if (rc_closer(t, j, tmin, q)) { tmin = t; q = -2 - j; }                     \
@@ -394,7 +429,7 @@ __device__ __forceinline__ void rc_trace_wide_closest(f3 o, f3 d, const rc_dscen
 // near-first, so blockers next to the shading point, the likely ones, are found early. Leaves
 // are postponed as in the closest-hit walk (queued per lane, walked by the wave together); a
 // lane stops at its first blocker. The answer does not depend on the order.
-template <class LS>
+template <bool QN = false, class LS>
 __device__ __forceinline__ bool rc_occluded_wide(f3 o, f3 d, const rc_dscene& s, float tlim, const LS& lstk) {
     uint32_t* que = lstk.queue();
     const rc_wray r = rc_wray_make(o, d);
@@ -421,17 +456,18 @@ __device__ __forceinline__ bool rc_occluded_wide(f3 o, f3 d, const rc_dscene& s,
             continue;
         }
         if (!trav) continue;
-        const uint32_t node = (uint32_t)cur << 7;
-        const int4 ch = rc_wrow_i(s, node + 96u);
+        const uint32_t node = QN ? (uint32_t)cur << 6 : (uint32_t)cur << 7;
+        const int4 ch = QN ? rc_qrow_i(s, node + 48u) : rc_wrow_i(s, node + 96u);
         float t0, t1, t2, t3;
         int e0, e1, e2, e3;
-        rc_child_enter(r, s, node, tlim, ch, t0, t1, t2, t3, e0, e1, e2, e3);
+        if (QN) rc_child_enter_q(r, s, node, tlim, ch, t0, t1, t2, t3, e0, e1, e2, e3);
+        else rc_child_enter(r, s, node, tlim, ch, t0, t1, t2, t3, e0, e1, e2, e3);
         int i0 = ch.x, i1 = ch.y, i2 = ch.z, i3 = ch.w;
 #define RC_QUEUE(e, cw)                                                                         \
         if (e != 0) {                                                                           \
             if (e & 127) { que[nq * 256] = (uint32_t)e & 0x7fffffffu; ++nq; }                   \
-            else {   /* more than 127 triangles: their end in row 7 */                          \
-                const int a_ = (int)(((uint32_t)e >> 7) & 0xffffffu), b_ = rc_wrow_i(s, node + 112u).cw; \
+            else {   /* more than 127 triangles: their end in row 7 of the fp32 node */         \
+                const int a_ = (int)(((uint32_t)e >> 7) & 0xffffffu), b_ = rc_wrow_i(s, ((uint32_t)cur << 7) + 112u).cw; \
                 for (int j = a_; j < b_ && !blocked; ++j) {                                     \
                     const float t = rc_tri_static4(r.o, r.d, (const float4*)s.ST4 + 3 * (int64_t)j); \
                     if (t > 1e-4f && t < tlim) blocked = true;                                  \

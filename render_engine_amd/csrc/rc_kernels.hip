@@ -758,8 +758,8 @@ __device__ __forceinline__ float rc_nee(const rc_dscene& s, const rc_dstate& st,
     if (s.ltype[l] == 1) val = (float)nl * (I * c);
     else { const float rr = fmaxf(r, 0.05f); val = (float)nl * (I * c / (rr * rr)); }
     const float tlim = r - 1e-3f;
-    if (TRAV == 2) {   // visibility only: any hit closer than r - 1e-3 blocks the light
-        if (rc_occluded_lds(O, D, sh, st.nt, tlim) || (s.nst > 0 && rc_occluded_wide(O, D, s, tlim, lstk))) return 0.f;
+    if (TRAV >= 2) {   // visibility only: any hit closer than r - 1e-3 blocks the light
+        if (rc_occluded_lds(O, D, sh, st.nt, tlim) || (s.nst > 0 && rc_occluded_wide<TRAV == 3>(O, D, s, tlim, lstk))) return 0.f;
     } else {
         float t; int q;
         rc_trace_full<TRAV>(O, D, sh, shq, st.nt, s, t, q, lstk);
@@ -1253,7 +1253,8 @@ hipError_t rc_launch_trace_rays(int trav, const rc_dscene* s, const rc_dstate* s
     size_t shm = rc_shm(*st, *s, trav, false);
     if (trav == 0) hipLaunchKernelGGL((k_trace_rays<0, false>), g, b, shm, stream, *s, *st, O, D, n, t, q);
     else if (trav == 1) hipLaunchKernelGGL((k_trace_rays<1, false>), g, b, shm, stream, *s, *st, O, D, n, t, q);
-    else RC_LAUNCH_WIDE((k_trace_rays<2, true>), (k_trace_rays<2, false>), false, g, st, s, stream, *s, *st, O, D, n, t, q);
+    else if (trav == 2) RC_LAUNCH_WIDE((k_trace_rays<2, true>), (k_trace_rays<2, false>), false, g, st, s, stream, *s, *st, O, D, n, t, q);
+    else RC_LAUNCH_WIDE((k_trace_rays<3, true>), (k_trace_rays<3, false>), false, g, st, s, stream, *s, *st, O, D, n, t, q);
     return hipGetLastError();
 }
 
@@ -1397,8 +1398,11 @@ hipError_t rc_launch_query(int trav, const rc_dscene* s, const rc_dstate* st, co
 #define RC_Q(T_, P_) hipLaunchKernelGGL((k_query<T_, P_, false>), g, b, shm, stream, *s, *st, R, *cam, seed, frame, dormant, img, counters)
 #define RC_QW(P_) RC_LAUNCH_WIDE((k_query<2, P_, true>), (k_query<2, P_, false>), true, g, st, s, stream, *s, *st, R, *cam, seed, frame, dormant, \
                                  img, counters)
-    if (per_path) { if (trav == 0) RC_Q(0, true); else if (trav == 1) RC_Q(1, true); else RC_QW(true); }
-    else { if (trav == 0) RC_Q(0, false); else if (trav == 1) RC_Q(1, false); else RC_QW(false); }
+#define RC_QW3(P_) RC_LAUNCH_WIDE((k_query<3, P_, true>), (k_query<3, P_, false>), true, g, st, s, stream, *s, *st, R, *cam, seed, frame, \
+                                  dormant, img, counters)
+    if (per_path) { if (trav == 0) RC_Q(0, true); else if (trav == 1) RC_Q(1, true); else if (trav == 2) RC_QW(true); else RC_QW3(true); }
+    else { if (trav == 0) RC_Q(0, false); else if (trav == 1) RC_Q(1, false); else if (trav == 2) RC_QW(false); else RC_QW3(false); }
+#undef RC_QW3
 #undef RC_QW
 #undef RC_Q
     return hipGetLastError();

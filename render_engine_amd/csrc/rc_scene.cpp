@@ -194,6 +194,47 @@ void build_wide(const std::vector<rc_bvh_node>& bn, std::vector<rc_wnode>& wn, i
     }
 }
 
+void build_qnodes(const std::vector<rc_wnode>& wn, float bound, std::vector<rc_qnode>& qn) {
+    qn.assign(wn.size(), rc_qnode{});
+    // smallest step: 2^-18 of the scene's extent rounded up to a power of two (a step is then far
+    // above the rounding error of either slab test for rays starting inside a few scene radii)
+    const int kmin = std::max(-60, (int)std::ceil(std::log2(std::max(bound, 1e-6f))) - 18);
+    for (size_t i = 0; i < wn.size(); ++i) {
+        const rc_wnode& w = wn[i];
+        rc_qnode& q = qn[i];
+        const float* lo[3] = {w.lox, w.loy, w.loz};
+        const float* hi[3] = {w.hix, w.hiy, w.hiz};
+        float org[3] = {0.f, 0.f, 0.f}, step[3] = {1.f, 1.f, 1.f};
+        uint32_t qlo[3] = {0u, 0u, 0u}, qhi[3] = {0u, 0u, 0u};
+        for (int a = 0; a < 3; ++a) {
+            double mn = INFINITY, mx = -INFINITY;
+            for (int c = 0; c < 4; ++c)
+                if (w.child[c] != 0) { mn = std::min(mn, (double)lo[a][c]); mx = std::max(mx, (double)hi[a][c]); }
+            if (!(mn <= mx)) { for (int c = 0; c < 4; ++c) qlo[a] |= 255u << (8 * c); continue; }
+            for (int k = kmin;; ++k) {
+                const double s = std::ldexp(1.0, k);
+                if ((mx - mn) > 250.0 * s) continue;
+                float o = (float)(mn - s);
+                if ((double)o > mn - s) o = std::nextafter(o, -INFINITY);
+                uint32_t l = 0, h = 0;
+                bool fits = true;
+                for (int c = 0; c < 4; ++c) {
+                    if (w.child[c] == 0) { l |= 255u << (8 * c); continue; }
+                    const double a0 = std::floor(((double)lo[a][c] - o) / s) - 1.0, a1 = std::ceil(((double)hi[a][c] - o) / s) + 1.0;
+                    if (a0 < 0.0 || a1 > 255.0) { fits = false; break; }
+                    l |= (uint32_t)a0 << (8 * c); h |= (uint32_t)a1 << (8 * c);
+                }
+                if (!fits) continue;
+                org[a] = o; step[a] = (float)s; qlo[a] = l; qhi[a] = h;
+                break;
+            }
+        }
+        q.ox = org[0]; q.oy = org[1]; q.oz = org[2]; q.sx = step[0]; q.sy = step[1]; q.sz = step[2];
+        q.qlox = qlo[0]; q.qloy = qlo[1]; q.qloz = qlo[2]; q.qhix = qhi[0]; q.qhiy = qhi[1]; q.qhiz = qhi[2];
+        for (int c = 0; c < 4; ++c) q.child[c] = w.child[c];
+    }
+}
+
 // ------------------------------------------------------------------------------------ cells
 void sample_cells(const Mesh& M, float h, const float lo[3], std::vector<uint64_t>& keys_out, std::vector<CellRec>& recs_out) {
     std::unordered_map<uint64_t, CellRec> cells;

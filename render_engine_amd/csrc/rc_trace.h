@@ -24,12 +24,12 @@ __device__ __forceinline__ double rc_wave_sum_f64(double v) {
 __device__ __forceinline__ long long rc_fx(float v, float scale) { return (long long)(v * scale); }
 
 // Static geometry per traversal mode: 0 binary in the prototype's order, 1 binary closest-hit,
-// 2 wide closest-hit (LDS stack).
+// 2 wide closest-hit (LDS stack), 3 the same over the quantized nodes.
 template <int TRAV, class LS>
 __device__ __forceinline__ void rc_trace_static(f3 o, f3 d, const rc_dscene& s, float& tmin, int& q, const LS& lstk) {
     if (TRAV == 0) rc_trace_bvh_ref(o, d, s, tmin, q);
     else if (TRAV == 1) rc_trace_bvh_closest(o, d, s, tmin, q);
-    else rc_trace_wide_closest(o, d, s, tmin, q, lstk);
+    else rc_trace_wide_closest<TRAV == 3>(o, d, s, tmin, q, lstk);
 }
 
 // Dynamic triangles from LDS, then the static BVH carrying tmin (cuda/poc_gpu.cu:340-344).
@@ -136,7 +136,7 @@ template <> struct rc_lstk_sel<false> {
 static inline size_t rc_shm(const rc_dstate& st, const rc_dscene& s, int trav, bool split = true) {
     size_t tri = (((size_t)st.nt * 10 + 3) & ~(size_t)3) * sizeof(float);
     const int lv = split && st.ocap > 0 && st.ocap < s.wstack ? st.ocap : s.wstack;
-    return tri + (trav == 2 ? (size_t)(lv + RC_WQ) * 256 * sizeof(uint32_t) : 0);
+    return tri + (trav >= 2 ? (size_t)(lv + RC_WQ) * 256 * sizeof(uint32_t) : 0);
 }
 // global stack words a launch of nthreads threads needs (0: every level in LDS)
 static inline size_t rc_ovf_need(const rc_dstate& st, const rc_dscene& s, long long nthreads) {

@@ -36,6 +36,21 @@ typedef struct rc_wnode {
     int32_t child[4];                       // 16 B
     int32_t cend[4];                        // 16 B
 } rc_wnode;
+
+// The same 4-wide node with quantized child boxes (64 B = one line; node i of QN is node i of WN):
+// per axis an origin o and a power-of-two step s, and per child and axis the planes o + q s with
+// 8-bit q (byte c of each q word = child c). The quantized box contains the fp32 box with at
+// least one step to spare on every side, and the step is at least 2^-18 of the scene's largest
+// coordinate, so the slab test on it is conservative with respect to the fp32 test of the exact
+// box despite both tests' rounding (the closest hit is never culled: same hits). Empty slots
+// hold q lo = 255, q hi = 0 on every axis (never entered). child[] as in rc_wnode; leaves of more
+// than 127 triangles read their end from WN.
+typedef struct rc_qnode {
+    float ox, oy, oz, sx;                   // row 0
+    float sy, sz; uint32_t qlox, qloy;      // row 1
+    uint32_t qloz, qhix, qhiy, qhiz;        // row 2
+    int32_t child[4];                       // row 3
+} rc_qnode;
 #define RC_WSTACK 48          // most LDS stack levels per lane (48 KB per 256-lane block)
 #define RC_WQ 7               // queued leaf ranges per lane of the wide closest-hit traversal (LDS levels)
 #define RC_LCAP 16            // LDS levels of the camera pass's lane stack (deeper levels: global overflow)
@@ -72,6 +87,7 @@ typedef struct rc_dscene {
     const float* STN;        // [4 nst] geometric normal + flags (w bit 0 = double sided)
     const rc_bvh_node* BN;   // binary BVH nodes
     const rc_wnode* WN;      // 4-wide BVH nodes
+    const rc_qnode* QN;      // the same nodes with quantized child boxes (traversal mode 3)
     const uint64_t* HK;      // hash keys (0 = empty)
     const int32_t* HV;       // entry of each key
     const float* L;          // [3 NL] light positions (point) or unit direction to the light (sun)

@@ -56,7 +56,8 @@ def test_trace_bit_exact(torch_gpu, which, request):
         O, D = _rays(scene, 20000, 10 + st, st)
         t0, q0 = osc.trace(O, D, st, trav=0)
         t1, q1 = osc.trace(O, D, st, trav=1)
-        for trav, (te, qe) in ((0, (t0, q0)), (1, (t1, q1)), (2, (t1, q1))):
+        t3, q3 = osc.trace(O, D, st, trav=3)   # brute force: the quantized nodes' reference
+        for trav, (te, qe) in ((0, (t0, q0)), (1, (t1, q1)), (2, (t1, q1)), (3, (t3, q3))):
             tg, qg = _gpu_trace(torch_gpu, scene, O, D, trav, st)
             assert np.array_equal(qg, qe), f"trav {trav} state {st}: {np.sum(qg != qe)} hit ids differ"
             assert np.array_equal(tg.view(np.uint32), te.view(np.uint32)), f"trav {trav} state {st}: t differs"
@@ -311,6 +312,45 @@ def test_query_stack_levels(torch_gpu, street):
         assert rays == out[64][1], levels
     with pytest.raises(rea.RcError):
         cache.set_stack_levels(0)
+
+
+@pytest.mark.parametrize("which", ["room", "street"])
+def test_query_quantized_nodes(torch_gpu, which, request):
+    """The camera pass over the quantized wide nodes (traversal mode 3) against the oracle's camera
+    pass with brute-force closest hits (every triangle tested), bit for bit with the same ray counts, in
+    every state, with and without the cache, with the lane stacks in LDS and nearly all in global memory
+    (1 LDS level). The quantized boxes hold one step of slack on every side, so no box holding the
+    closest hit is culled; the fp32 slab test can cull one at a grazing edge (one path of the room's
+    524,288 in state 1: the fp32 modes then follow the oracle's fp32 traversal instead)."""
+    torch = torch_gpu
+    scene, osc = request.getfixturevalue(which)
+    cache = rea.Cache(scene, rea.methods.plain())
+    cache.begin(0, 5)
+    cam = scene.camera()
+    npix = cam.W * cam.H
+    try:
+        scene.set_trav(rea.TRAV_WIDE_Q)
+        for st in range(scene.info.NS):
+            cache.frame(st)
+            R = cache.get("R")
+            for spp, nb, cb in ((4, 3, 1), (1, 3, 5), (2, 1, 1)):
+                cam.spp, cam.bounces, cam.cache_bounce = spp, nb, cb
+                for no_cache in (False, True):
+                    exp = np.zeros(npix, np.float32)
+                    er = (C.c_longlong * 2)()
+                    po.lib.orc_k_query(osc.state(st), 3, None if no_cache else po.ptr(R), C.byref(cam), 21 + st, st + 1, 0,
+                                       po.ptr(exp), er, 0, npix)
+                    for levels in (64, 1):
+                        cache.set_stack_levels(levels)
+                        img = torch.full((npix,), -1.0, dtype=torch.float32, device="cuda")
+                        rays = cache.render(img, cam, no_cache=no_cache, seed=21 + st, count_rays=True)
+                        got = img.cpu().numpy()
+                        tag = f"state {st} spp {spp} bounces {nb} cache_bounce {cb} no_cache {no_cache} levels {levels}"
+                        assert np.array_equal(got.view(np.uint32), exp.view(np.uint32)), \
+                            f"{tag}: {np.sum(got != exp)} pixels differ"
+                        assert tuple(rays) == (er[0], er[1]), tag
+    finally:
+        scene.set_trav(rea.TRAV_WIDE)
 
 
 @pytest.mark.parametrize("which", ["room", "street"])
