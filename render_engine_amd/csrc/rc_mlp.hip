@@ -99,34 +99,47 @@ __global__ void __launch_bounds__(MROWS * MWAVES) k_mlp_fwd(rc_mlpdev m, const f
         if (TRAIN) for (int k = 0; k < MW; ++k) actT[(int64_t)k * ldT + base + lane] = row[k];
     }
     __syncthreads();
+    // Each layer's weight operands are loaded once into registers (8 x 16 bytes per lane) and serve
+    // the wave's 4 row tiles; the products run as D = W A^T, so a lane's 4 results are 4 consecutive
+    // output features of one sample: one 8-byte LDS store per column tile. (Operands reloaded from
+    // memory for every product, and D = A W^T with 2-byte stores: 1.7x slower.)
     const __bf16* Wt = (const __bf16*)m.Wt;
     for (int L = 0; L < RC_MLP_HIDDEN; ++L) {
         const __bf16* W = Wt + (size_t)L * MW * MW;
+        bf16x8 wop[2][4];
+#pragma unroll
+        for (int ks = 0; ks < 2; ++ks)
+#pragma unroll
+            for (int ct = 0; ct < 4; ++ct) wop[ks][ct] = rc_opnd(W, MW, ct * 16, ks * 32);
+#pragma unroll
         for (int rt = 0; rt < MROWS / 16; ++rt) {
+            const bf16x8 a0 = rc_opnd(act, MLD, rt * 16, 0), a1 = rc_opnd(act, MLD, rt * 16, 32);
             f32x4 acc[4];
-            for (int ct = 0; ct < 4; ++ct) acc[ct] = f32x4{0.f, 0.f, 0.f, 0.f};
-            for (int ks = 0; ks < 2; ++ks) {
-                const bf16x8 a = rc_opnd(act, MLD, rt * 16, ks * 32);
-                for (int ct = 0; ct < 4; ++ct) acc[ct] = rc_mfma(a, rc_opnd(W, MW, ct * 16, ks * 32), acc[ct]);
-            }
+#pragma unroll
+            for (int ct = 0; ct < 4; ++ct) acc[ct] = rc_mfma(wop[1][ct], a1, rc_mfma(wop[0][ct], a0, f32x4{0.f, 0.f, 0.f, 0.f}));
+            // lane: features ct * 16 + 4 g + i of sample rt * 16 + c16
+#pragma unroll
             for (int ct = 0; ct < 4; ++ct) {
                 __bf16 h[4];
-                for (int i = 0; i < 4; ++i) {
-                    h[i] = rc_bf(fmaxf(acc[ct][i], 0.f));
-                    act[(rt * 16 + 4 * g + i) * MLD + ct * 16 + c16] = h[i];
-                }
+#pragma unroll
+                for (int i = 0; i < 4; ++i) h[i] = rc_bf(fmaxf(acc[ct][i], 0.f));
+                typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+                *(bf16x4*)(act + (rt * 16 + c16) * MLD + ct * 16 + 4 * g) = bf16x4{h[0], h[1], h[2], h[3]};
                 if (TRAIN) {
-                    __bf16* dst = actT + ((int64_t)(L + 1) * MW + ct * 16 + c16) * ldT + base + rt * 16 + 4 * g;
-                    for (int i = 0; i < 4; ++i) dst[i] = h[i];
+                    __bf16* dst = actT + ((int64_t)(L + 1) * MW + ct * 16 + 4 * g) * ldT + base + rt * 16 + c16;
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) dst[(int64_t)i * ldT] = h[i];
                 }
             }
         }
         __syncthreads();
     }
     const __bf16* W5 = Wt + (size_t)RC_MLP_HIDDEN * MW * MW;   // [16][64], output column 0
+    const bf16x8 w50 = rc_opnd(W5, MW, 0, 0), w51 = rc_opnd(W5, MW, 0, 32);
+#pragma unroll
     for (int rt = 0; rt < MROWS / 16; ++rt) {
-        f32x4 acc = f32x4{0.f, 0.f, 0.f, 0.f};
-        for (int ks = 0; ks < 2; ++ks) acc = rc_mfma(rc_opnd(act, MLD, rt * 16, ks * 32), rc_opnd(W5, MW, 0, ks * 32), acc);
+        f32x4 acc = rc_mfma(rc_opnd(act, MLD, rt * 16, 0), w50, f32x4{0.f, 0.f, 0.f, 0.f});
+        acc = rc_mfma(rc_opnd(act, MLD, rt * 16, 32), w51, acc);
         if (c16 == 0)
             for (int i = 0; i < 4; ++i) {
                 const int s = base + rt * 16 + 4 * g + i;
