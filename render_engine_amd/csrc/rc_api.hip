@@ -1297,7 +1297,7 @@ extern "C" rc_status rc_mlp_create(const rc_mlp_config* c, rc_mlp** out) {
     for (int a = 0; a < 3; ++a) { d.lo[a] = c->lo[a]; d.iext[a] = 1.f / c->ext[a]; }
     rc_mlpbuf& b = m->b;
     b.cap = rc_mlp_pad(c->max_batch);
-    const size_t cap = b.cap, nsplit = (cap + 1023) / 1024;
+    const size_t cap = b.cap, nsplit = (cap + 255) / 256;   // weight-gradient splits of 256 samples (rc_launch_mlp_grad)
 #define MK(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) { delete m; return fail(RC_ERR_HIP, "%s: %s", #x, hipGetErrorString(e_)); } } while (0)
     MK(dalloc(&d.table, nt, m->owned)); MK(dalloc(&d.gtable, nt, m->owned));
     MK(dalloc(&d.Wm, (size_t)RC_MLP_NW, m->owned)); MK(dalloc(&d.gW, (size_t)RC_MLP_NW, m->owned));

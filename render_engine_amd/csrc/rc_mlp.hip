@@ -199,12 +199,17 @@ __global__ void __launch_bounds__(MROWS * MWAVES) k_mlp_bwd(rc_mlpdev m, const f
     __syncthreads();
     for (int L = RC_MLP_HIDDEN - 1; L >= 0; --L) {
         const __bf16* W = Wk + (size_t)L * MW * MW;   // W_L[k][j]
+        bf16x8 wop[2][4];   // the layer's weight operands, loaded once for the wave's 4 row tiles
+#pragma unroll
+        for (int ks = 0; ks < 2; ++ks)
+#pragma unroll
+            for (int ct = 0; ct < 4; ++ct) wop[ks][ct] = rc_opnd(W, MW, ct * 16, ks * 32);
         for (int rt = 0; rt < MROWS / 16; ++rt) {
             f32x4 acc[4];
             for (int ct = 0; ct < 4; ++ct) acc[ct] = f32x4{0.f, 0.f, 0.f, 0.f};
             for (int ks = 0; ks < 2; ++ks) {
                 const bf16x8 a = rc_opnd(gs, MLD, rt * 16, ks * 32);
-                for (int ct = 0; ct < 4; ++ct) acc[ct] = rc_mfma(a, rc_opnd(W, MW, ct * 16, ks * 32), acc[ct]);
+                for (int ct = 0; ct < 4; ++ct) acc[ct] = rc_mfma(a, wop[ks][ct], acc[ct]);
             }
             const int s0 = base + rt * 16 + 4 * g;
             for (int ct = 0; ct < 4; ++ct) {
@@ -228,7 +233,9 @@ __global__ void __launch_bounds__(MROWS * MWAVES) k_mlp_bwd(rc_mlpdev m, const f
 
 // Weight gradients, split over the batch: dW_L[k][j] = sum_b a_L[b][k] g_z_{L+1}[b][j] for L = 0..4
 // (P = actT[L], Q = gzT[L], both sample-contiguous), dW_5[k][0] = sum_b a_5[b][k] dy[b].
-// Block (split, L): wave w computes rows k = 16w..16w+15; partial sums part[L][split][k][j].
+// Block (split, L): wave w computes rows k = 16w..16w+15; partial sums part[L][split][k][j]. For
+// dW_5 the block's 4 waves sum quarters of the split (thread k + 64 q: samples q, q + 4, ...) and
+// wave 0 adds the quarters in order (a fixed order: the same sums every run).
 __global__ void __launch_bounds__(MROWS * MWAVES) k_mlp_dw(const __bf16* actT, const __bf16* gzT, const float* dy, int ldT, int chunk,
                                                            float* part) {
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, g = lane >> 4, c16 = lane & 15;
@@ -247,26 +254,37 @@ __global__ void __launch_bounds__(MROWS * MWAVES) k_mlp_dw(const __bf16* actT, c
         for (int ct = 0; ct < 4; ++ct)
             for (int i = 0; i < 4; ++i) out[(wv * 16 + 4 * g + i) * MW + ct * 16 + c16] = acc[ct][i];
     } else {
-        const int k = threadIdx.x;
-        if (k < MW) {
-            const __bf16* a5 = actT + ((size_t)RC_MLP_HIDDEN * MW + k) * ldT;
-            float s = 0.f;
-            for (int b = b0; b < b1; ++b) s += rc_fb(a5[b]) * dy[b];
-            for (int j = 0; j < MW; ++j) out[k * MW + j] = j == 0 ? s : 0.f;
+        __shared__ float s_q[4][MW];
+        const int k = threadIdx.x & 63, q = threadIdx.x >> 6;
+        const __bf16* a5 = actT + ((size_t)RC_MLP_HIDDEN * MW + k) * ldT;
+        float s = 0.f;
+        for (int b = b0 + q; b < b1; b += 4) s += rc_fb(a5[b]) * dy[b];
+        s_q[q][k] = s;
+        __syncthreads();
+        if (q == 0) {
+            const float t = ((s_q[0][k] + s_q[1][k]) + s_q[2][k]) + s_q[3][k];
+            for (int j = 0; j < MW; ++j) out[k * MW + j] = j == 0 ? t : 0.f;
         }
     }
 }
 
 // gW (master layout: layers 0..4 [64][64], layer 5 [64][16]) = sum over splits, in split order
+// (a block = 64 weights x 4 quarters of the splits: each thread adds its quarter in split order, then
+// the quarters are added in order; the same sums every run)
 __global__ void __launch_bounds__(RC_TB) k_mlp_dw_sum(const float* part, int nsplit, float* gW) {
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= RC_MLP_NW) return;
-    int L, k, j;
-    if (e < RC_MLP_HIDDEN * MW * MW) { L = e / (MW * MW); k = (e / MW) % MW; j = e % MW; }
-    else { L = RC_MLP_HIDDEN; k = (e - RC_MLP_HIDDEN * MW * MW) / 16; j = (e - RC_MLP_HIDDEN * MW * MW) % 16; }
+    __shared__ float s_q[4][64];
+    const int e = blockIdx.x * 64 + (threadIdx.x & 63), q = threadIdx.x >> 6;
     float s = 0.f;
-    for (int sp = 0; sp < nsplit; ++sp) s += part[((size_t)L * nsplit + sp) * MW * MW + k * MW + j];
-    gW[e] = s;
+    if (e < RC_MLP_NW) {
+        int L, k, j;
+        if (e < RC_MLP_HIDDEN * MW * MW) { L = e / (MW * MW); k = (e / MW) % MW; j = e % MW; }
+        else { L = RC_MLP_HIDDEN; k = (e - RC_MLP_HIDDEN * MW * MW) / 16; j = (e - RC_MLP_HIDDEN * MW * MW) % 16; }
+        const int q0 = q * nsplit / 4, q1 = (q + 1) * nsplit / 4;
+        for (int sp = q0; sp < q1; ++sp) s += part[((size_t)L * nsplit + sp) * MW * MW + k * MW + j];
+    }
+    s_q[q][threadIdx.x & 63] = s;
+    __syncthreads();
+    if (q == 0 && e < RC_MLP_NW) gW[e] = ((s_q[0][threadIdx.x] + s_q[1][threadIdx.x]) + s_q[2][threadIdx.x]) + s_q[3][threadIdx.x];
 }
 
 // hash-table gradient: each sample's feature gradients spread over its 8 corners per level;
@@ -441,10 +459,10 @@ hipError_t rc_launch_mlp_grad(const rc_mlpdev* m, const rc_mlpbuf* b, const floa
     if (t) hipLaunchKernelGGL(k_mlp_loss, dim3(rc_grid(npad)), dim3(RC_TB), 0, st, (const float*)b->y, t, n, npad, b->dy, loss);
     const float* dy = t ? (const float*)b->dy : dy_in;
     hipLaunchKernelGGL(k_mlp_bwd, dim3(nb), dim3(MROWS * MWAVES), 0, st, *m, dy, (const __bf16*)actT, npad, gzT, b->gX);
-    const int chunk = 1024, nsplit = (npad + chunk - 1) / chunk;
+    const int chunk = 256, nsplit = (npad + chunk - 1) / chunk;
     hipLaunchKernelGGL(k_mlp_dw, dim3(nsplit, RC_MLP_HIDDEN + 1), dim3(MROWS * MWAVES), 0, st, (const __bf16*)actT, (const __bf16*)gzT, dy,
                        npad, chunk, b->part);
-    hipLaunchKernelGGL(k_mlp_dw_sum, dim3(rc_grid(RC_MLP_NW)), dim3(RC_TB), 0, st, (const float*)b->part, nsplit, m->gW);
+    hipLaunchKernelGGL(k_mlp_dw_sum, dim3((RC_MLP_NW + 63) / 64), dim3(RC_TB), 0, st, (const float*)b->part, nsplit, m->gW);
     hipLaunchKernelGGL(k_mlp_enc_bwd, dim3(rc_grid((long long)n * RC_MLP_LEVELS)), dim3(RC_TB), 0, st, *m, P, n, (const float*)b->gX);
     return hipGetLastError();
 }
