@@ -69,22 +69,32 @@ __device__ __forceinline__ float rc_est_of(const float* Lc, int64_t i, int nl, u
 
 // Per-entry sum of the light channels over the active lights, in light order (rc_est_of's sum, bit for
 // bit) for every entry: a block of RC_TB threads stages the contiguous Lc span of 64 entries in LDS with
-// coalesced loads (rows padded to an odd pitch: conflict-free row reads), then one thread per entry adds
-// its row. Replaces one lane per entry walking its own row, whose loads touch 64 rows at once. (256
-// entries per block, every thread summing, measured 2.5 times slower: its 66 KB of LDS leave 8 waves
-// per CU for the serial sums.)
+// coalesced 16-byte loads (the span starts at a multiple of 256 nl bytes; rows padded to an odd pitch:
+// conflict-free row reads), then one thread per entry adds its row. Replaces one lane per entry walking
+// its own row, whose loads touch 64 rows at once. (256 entries per block, every thread summing,
+// measured 2.5 times slower: its 66 KB of LDS leave 8 waves per CU for the serial sums; 4-byte loads:
+// 103 us against 1M entries x 65 lights.)
 #define RC_RSE 64
 __global__ void __launch_bounds__(RC_TB) k_rowsum(const float* __restrict__ Lc, int Np, int nl, uint64_t dormant, float* __restrict__ out) {
     extern __shared__ float s_r[];   // [RC_RSE][nl | 1]
     const int t = threadIdx.x, base = blockIdx.x * RC_RSE;
-    const int T = min(RC_RSE, Np - base), pitch = nl | 1, nel = T * nl;
+    const int T = min(RC_RSE, Np - base), pitch = nl | 1, nel = T * nl, nv = nel >> 2;
     const float* __restrict__ src = Lc + (int64_t)base * nl;
-    const int dq = RC_TB / nl, dr = RC_TB - dq * nl;
-    int j = t / nl, l = t - (t / nl) * nl;
-    for (int e = t; e < nel; e += RC_TB) {
-        s_r[j * pitch + l] = src[e];
-        j += dq; l += dr;
-        if (l >= nl) { l -= nl; ++j; }
+    {   // element 4 v + q of the span is light l of entry j (rows of nl)
+        const int dq = 4 * RC_TB / nl, dr = 4 * RC_TB - dq * nl;
+        int j = 4 * t / nl, l = 4 * t - (4 * t / nl) * nl;
+        for (int v = t; v < nv; v += RC_TB) {
+            const float4 x = ((const float4*)src)[v];
+            int jj = j, ll = l;
+            s_r[jj * pitch + ll] = x.x; if (++ll == nl) { ll = 0; ++jj; }
+            s_r[jj * pitch + ll] = x.y; if (++ll == nl) { ll = 0; ++jj; }
+            s_r[jj * pitch + ll] = x.z; if (++ll == nl) { ll = 0; ++jj; }
+            s_r[jj * pitch + ll] = x.w;
+            j += dq; l += dr;
+            if (l >= nl) { l -= nl; ++j; }
+        }
+        const int e = 4 * nv + t;   // the last nel % 4 elements (a partial last block)
+        if (e < nel) s_r[(e / nl) * pitch + e % nl] = src[e];
     }
     __syncthreads();
     if (t < T) {
@@ -214,6 +224,19 @@ __device__ __forceinline__ void rc_block_count(bool flag, int* blkcnt) {
     __shared__ int s_c[RC_TB / 64];
     const unsigned long long b = __ballot(flag);
     if ((threadIdx.x & 63) == 0) s_c[threadIdx.x >> 6] = __popcll(b);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int t = 0;
+        for (int k = 0; k < RC_TB / 64; ++k) t += s_c[k];
+        blkcnt[blockIdx.x] = t;
+    }
+}
+// the same with a count per thread
+__device__ __forceinline__ void rc_block_add(int v, int* blkcnt) {
+    __shared__ int s_c[RC_TB / 64];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    if ((threadIdx.x & 63) == 0) s_c[threadIdx.x >> 6] = v;
     __syncthreads();
     if (threadIdx.x == 0) {
         int t = 0;
@@ -603,21 +626,35 @@ __global__ void __launch_bounds__(RC_SCB) k_scan_blocks(int* cnt, int n, int* to
 // restore after a flush (rows already zero): record j < *countp, if its key matches its entry, gives the
 // entry its radiance in channel 0, its history and age; verify opens the verification window on it (its
 // channel sum is then the radiance, or 0 when channel 0 is dormant), counted per block. The record count
-// is read on the device (the grid covers the most records).
+// is read on the device (the grid covers the most records). A thread takes RC_SRR records a block
+// apart, their loads issued together: the key check waits on the record's index, so one record per
+// thread left the kernel waiting on two dependent loads at a time.
+#define RC_SRR 4
 __global__ void __launch_bounds__(RC_TB) k_snap_restore(rc_dcache c, int Np, int nl, const uint64_t* ekeys, const rc_snaprec* rec,
                                                        const int* countp, int verify, uint64_t dormant, int* blkcnt) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    bool pend = false;
-    if (j < *countp) {
-        const rc_snaprec r = rec[j];
-        const int i = (int)r.index;
-        if (r.index < (uint32_t)Np && ekeys[i] == r.key) {
-            c.Lc[(int64_t)nl * i] = r.radiance;
-            c.n[i] = r.n; c.age[i] = r.age; c.pending[i] = 0;
-            if (verify) { c.pending[i] = 1; c.c0[i] = rc_is_dormant(dormant, 0) ? 0.f : 0.f + r.radiance; pend = true; }
+    const int cnt = *countp;
+    const int j0 = blockIdx.x * (RC_TB * RC_SRR) + threadIdx.x;
+    rc_snaprec r[RC_SRR];
+    uint64_t key[RC_SRR];
+#pragma unroll
+    for (int u = 0; u < RC_SRR; ++u) {
+        const int j = j0 + u * RC_TB;
+        if (j < cnt) r[u] = rec[j];
+        else r[u].index = 0xffffffffu;
+    }
+#pragma unroll
+    for (int u = 0; u < RC_SRR; ++u) key[u] = r[u].index < (uint32_t)Np ? ekeys[r[u].index] : ~r[u].key;
+    int npend = 0;
+#pragma unroll
+    for (int u = 0; u < RC_SRR; ++u) {
+        if (r[u].index < (uint32_t)Np && key[u] == r[u].key) {
+            const int i = (int)r[u].index;
+            c.Lc[(int64_t)nl * i] = r[u].radiance;
+            c.n[i] = r[u].n; c.age[i] = r[u].age; c.pending[i] = 0;
+            if (verify) { c.pending[i] = 1; c.c0[i] = rc_is_dormant(dormant, 0) ? 0.f : 0.f + r[u].radiance; ++npend; }
         }
     }
-    if (verify) rc_block_count(pend, blkcnt);
+    if (verify) rc_block_add(npend, blkcnt);
 }
 
 // Whole-cache flush (event to a new state, rtxgi_bench.patch:363-366): the light rows as 16-byte stores
@@ -1499,7 +1536,7 @@ hipError_t rc_launch_snap_pack(const rc_dcache* c, int Np, int nl, const uint64_
 hipError_t rc_launch_snap_restore(const rc_dcache* c, int Np, int nl, const uint64_t* ekeys, const rc_snaprec* rec, const int* count,
                                   int max_count, int verify, uint64_t dormant, hipStream_t stream) {
     if (max_count <= 0) return hipSuccess;
-    const int nblk = rc_grid(max_count);
+    const int nblk = (max_count + RC_TB * RC_SRR - 1) / (RC_TB * RC_SRR);
     hipLaunchKernelGGL(k_snap_restore, dim3(nblk), dim3(RC_TB), 0, stream, *c, Np, nl, ekeys, rec, count, verify, dormant, c->blkcnt);
     if (verify) hipLaunchKernelGGL(k_count_sum, dim3(1), dim3(RC_CSB), 0, stream, (const int*)c->blkcnt, nblk, c->npend + 1, 1);
     return hipGetLastError();
