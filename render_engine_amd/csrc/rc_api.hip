@@ -316,16 +316,9 @@ extern "C" rc_status rc_scene_upload(rc_scene* s, int trav) {
         for (int r = 0; r < 3; ++r)
             for (int x = 0; x < 3; ++x) st4[12 * (size_t)j + 4 * r + x] = H.ST[9 * (size_t)j + 3 * r + x];
     HIPCK(dput(&dST, H.ST, s->owned)); HIPCK(dput(&dST4, st4, s->owned)); HIPCK(dput(&dSTN, H.STN, s->owned)); HIPCK(dput(&dBN, H.BN, s->owned));
-    {   // quantized wide nodes, step floor from the largest coordinate of the static triangles
-        float bound = 0.f;
-        for (size_t j = 0; j < (size_t)H.nst; ++j)   // v0, v0 + e1 and v0 + e2 of every triangle
-            for (int x = 0; x < 3; ++x) {
-                const float v = H.ST[9 * j + x], e1 = H.ST[9 * j + 3 + x], e2 = H.ST[9 * j + 6 + x];
-                bound = std::max(bound, std::max(std::fabs(v), std::max(std::fabs(v + e1), std::fabs(v + e2))));
-            }
-        for (int a = 0; a < 3; ++a) bound = std::max(bound, std::max(std::fabs(H.lo[a]), std::fabs(H.lo[a] + H.ext[a])));
+    {   // quantized wide nodes
         std::vector<rc_qnode> qn;
-        rc::build_qnodes(H.WN, bound, qn);
+        rc::build_qnodes(H.WN, rc::quant_step_floor(H.BN, H.lo, H.ext), qn);
         HIPCK(dput(&dQN, qn, s->owned));
     }
     HIPCK(dput(&dWN, H.WN, s->owned)); HIPCK(dput(&dHK, H.HK, s->owned)); HIPCK(dput(&dHV, H.HV, s->owned));

@@ -142,6 +142,15 @@ void build_bvh(const Mesh& M, std::vector<rc_bvh_node>& nodes, std::vector<int>&
     }
 }
 
+float quant_step_floor(const std::vector<rc_bvh_node>& bn, const float lo[3], const float ext[3]) {
+    float b = 0.f;
+    for (int a = 0; a < 3; ++a) {
+        b = std::max(b, std::max(std::fabs(lo[a]), std::fabs(lo[a] + ext[a])));
+        if (!bn.empty()) b = std::max(b, std::max(std::fabs(bn[0].lo[a]), std::fabs(bn[0].hi[a])));
+    }
+    return std::ldexp(1.f, std::max(-60, (int)std::ceil(std::log2(std::max(b, 1e-6f))) - 18));
+}
+
 void build_wide(const std::vector<rc_bvh_node>& bn, std::vector<rc_wnode>& wn, int& depth_out) {
     wn.clear();
     depth_out = 0;
@@ -194,11 +203,11 @@ void build_wide(const std::vector<rc_bvh_node>& bn, std::vector<rc_wnode>& wn, i
     }
 }
 
-void build_qnodes(const std::vector<rc_wnode>& wn, float bound, std::vector<rc_qnode>& qn) {
+void build_qnodes(const std::vector<rc_wnode>& wn, float min_step, std::vector<rc_qnode>& qn) {
     qn.assign(wn.size(), rc_qnode{});
-    // smallest step: 2^-18 of the scene's extent rounded up to a power of two (a step is then far
-    // above the rounding error of either slab test for rays starting inside a few scene radii)
-    const int kmin = std::max(-60, (int)std::ceil(std::log2(std::max(bound, 1e-6f))) - 18);
+    // smallest step: quant_step_floor (a step is then far above the rounding error of either slab
+    // test for rays starting inside a few scene radii)
+    const int kmin = std::max(-60, (int)std::ceil(std::log2(std::max(min_step, 1e-18f))));
     for (size_t i = 0; i < wn.size(); ++i) {
         const rc_wnode& w = wn[i];
         rc_qnode& q = qn[i];

@@ -112,8 +112,11 @@ std::string build_bvh_device(const Mesh& M, std::vector<rc_bvh_node>& nodes, std
 // Collapse of the binary BVH into 4-wide nodes (children: up to 4 descendants, leaves keep
 // their triangle ranges).
 void build_wide(const std::vector<rc_bvh_node>& bn, std::vector<rc_wnode>& wn, int& depth);
-// quantized copy of the wide nodes (rc_qnode); bound = the largest absolute coordinate of the scene
-void build_qnodes(const std::vector<rc_wnode>& wn, float bound, std::vector<rc_qnode>& qn);
+// the smallest quantization step of a scene's quantized nodes: 2^-18 of its largest absolute coordinate
+// (the BVH root box and the scene bounds), rounded up to a power of two
+float quant_step_floor(const std::vector<rc_bvh_node>& bn, const float lo[3], const float ext[3]);
+// quantized copy of the wide nodes (rc_qnode); min_step: the smallest quantization step
+void build_qnodes(const std::vector<rc_wnode>& wn, float min_step, std::vector<rc_qnode>& qn);
 
 struct CellRec { float p[3]; float d2; float n[3]; double asum, wsum; int mat; };
 // Cache-cell sampling on the triangles (cuda/prep_mesh.cuh:129-157): returns cells sorted by
