@@ -519,14 +519,24 @@ def bench_adjoint(args, rea, scene, cfg, log):
             fb, fwork, cpu = adjoint_cpu(scene, adj, a, args.cpu_seconds, rays_step, log)
         except Exception as e:  # the baseline is reported, not required
             log(f"cpu leg failed: {e!r}")
-    roof = None
-    if fb:
-        ach = fb / k_fwd / 1e6
-        roof = {"bound": "hbm", "achieved": ach, "peak": 8000.0, "unit": "GB/s", "frac": ach / 8000.0, "traffic": None,
-                "kernel": "k_adj_fwd",
-                "basis": "per entry 36 B (P, N, L and S out) + per ray 32 B/binary node + 36 B/triangle + 8 B/hash probe + "
-                         "per hit 16 B (A, E, SG; 28 B with the previous bounce's L, S), oracle-counted, mean over the bounces",
-                "work": fwork}
+    # roofline of the forward bounce: VALU issue (rocprofv3 SQ_INSTS_VALU per launch of this workload over the
+    # in-run launch time), the counter HBM traffic and the oracle-counted touched bytes beside it
+    prof = pmc_profile(f"street{scene.camera().W}x{scene.camera().H}_adjoint")
+    kc = kernel_counters(prof, "k_adj_fwd")
+    valu = kc.get("SQ_INSTS_VALU") if kc else None
+    traffic = kc.get("traffic_bytes") if kc else None
+    ach = valu / (k_fwd * 1e-3) / 1e9 if valu else None
+    roof = {"bound": "valu", "achieved": ach, "peak": VALU_PEAK, "unit": "G wave-instructions/s",
+            "frac": ach / VALU_PEAK if ach else None, "traffic": traffic, "kernel": "k_adj_fwd",
+            "valu_per_launch": valu, "hbm_GB_s": traffic / k_fwd / 1e6 if traffic else None,
+            "hbm_frac": traffic / k_fwd / 1e6 / 8000.0 if traffic else None,
+            "touched_bytes": fb, "touched_frac_of_hbm_peak": fb / k_fwd / 1e6 / 8000.0 if fb else None,
+            "counters": f"profiles/pmc_street{scene.camera().W}x{scene.camera().H}_adjoint.json" if prof else None,
+            "basis": "SQ_INSTS_VALU per launch (rocprofv3, this workload) / in-run launch time; peak = 256 CUs x 1 VALU "
+                     "wave-instruction per cycle x 2.4 GHz. traffic = FETCH_SIZE + WRITE_SIZE per launch (copy-calibrated); "
+                     "touched bytes = per entry 36 B (P, N, L and S out) + per ray 32 B/binary node + 36 B/triangle + 8 B/hash "
+                     "probe + per hit 16 B (A, E, SG; 28 B with the previous bounce's L, S), oracle-counted, mean over the bounces",
+            "work": fwork if fb else None}
     qual = None
     if not args.no_quality:
         try:
