@@ -301,7 +301,10 @@ __global__ void __launch_bounds__(RC_TB) k_decide(rc_dscene s, rc_dcache c, rc_p
 // RC_UT, so every row's tile is 256-byte aligned), so the block waits for memory once; the per-entry
 // factors are computed by the first wave meanwhile, then the blend runs over the span in flat
 // order (coalesced Lc2 stores). The means are staged in LDS at pitch RC_UT + 1: the flat order's
-// lanes read consecutive lights of one entry, which then fall in distinct banks.
+// lanes read consecutive lights of one entry, which then fall in distinct banks. The resolved terms
+// A (Edir + Lc) then take the means' place in LDS (one more barrier) for the per-entry sums: half the
+// LDS of separate arrays, so the registers (5 blocks per CU at 65 lights) rather than LDS (4) bound the
+// blocks in flight (237 -> 231 us at 1M entries x 65 lights).
 // UE: channel elements per thread (ceil(RC_UT * nl / RC_TB) at most), instantiated for a few
 // light counts so that small ones keep few registers (more waves in flight)
 template <int UE>
@@ -314,7 +317,7 @@ __global__ void __launch_bounds__(RC_TB) k_update(rc_dscene s, rc_dstate st, rc_
     const int t = threadIdx.x;
     const int nel = T * nl;
     float* s_m = sm;
-    float* s_v = sm + nl * (RC_UT + 1);
+    float* s_v = sm;   // after the blend
     const int64_t g0 = (int64_t)base * nl;
     const float* __restrict__ Lc = c.Lc + g0;
     const float* __restrict__ Ed = st.Edir + g0;
@@ -388,10 +391,16 @@ __global__ void __launch_bounds__(RC_TB) k_update(rc_dscene s, rc_dstate st, rc_
             }
             Lc2[e] = o;
             const float lv = dm ? 0.f : o;
-            s_v[e] = s_A[j] * (re[u] + lv);
+            ra[u] = s_A[j] * (re[u] + lv);
         }
         j += dq; l += dr;
         if (l >= nl) { l -= nl; ++j; }
+    }
+    __syncthreads();   // every mean read: the terms overwrite them
+#pragma unroll
+    for (int u = 0; u < UE; ++u) {
+        const int e = t + u * RC_TB;
+        if (e < nel) s_v[e] = ra[u];
     }
     __syncthreads();
     if (t < T) {
@@ -1336,7 +1345,7 @@ hipError_t rc_launch_update(const rc_dscene* s, const rc_dstate* st, const rc_dc
                             hipStream_t stream) {
     if (p->Np <= 0) return hipSuccess;
     dim3 g((p->Np + RC_UT - 1) / RC_UT), b(RC_TB);
-    size_t shm = (size_t)p->nl * (2 * RC_UT + 1) * sizeof(float);
+    size_t shm = (size_t)p->nl * (RC_UT + 1) * sizeof(float);
     const int ue = (RC_UT * p->nl + RC_TB - 1) / RC_TB;
     if (ue <= 4) hipLaunchKernelGGL(k_update<4>, g, b, shm, stream, *s, *st, *c, *p, lnew);
     else if (ue <= 9) hipLaunchKernelGGL(k_update<9>, g, b, shm, stream, *s, *st, *c, *p, lnew);
