@@ -727,8 +727,8 @@ def main():
     ap.add_argument("--quiet", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only: --gpus {args.gpus} is not supported (the path shards by screen tiles and "
-              f"hash-key ranges, see the design notes; this build measures one GPU's share)", file=sys.stderr)
+        print(f"bench.py runs on one GPU only: --gpus {args.gpus} is not supported (the path would shard the camera "
+              f"pass by screen tiles and the cache entries by Morton ranges; this measures one GPU's share)", file=sys.stderr)
         sys.exit(2)
     cfg = CONFIGS[args.config]
     tag = (f"{cfg['kind']}{cfg['W']}x{cfg['H']}" if cfg["kind"] == "street" else "room") + \
