@@ -824,6 +824,32 @@ def main():
         "k_query": {"ms": t_q, "rays": (n_ext + n_sh) / args.steps, "mrays_s": (n_ext + n_sh) / args.steps / t_q / 1e3},
     }
     log(f"kernels: {json.dumps(kernels)}")
+    # 4) the steady-state frame cost of each method (no event in the timed frames): what dependency
+    # tracking and snapshots cost every frame, beside the plain cache timed above
+    method_frames = {}
+    for mname, mk in (("plain", rea.methods.plain), ("flush_snapshots", rea.methods.flush_snapshots),
+                      ("targeted", rea.methods.targeted), ("targeted_grad", rea.methods.targeted_grad)):
+        mc = rea.Cache(scene, mk())
+        mc.begin(0, seed)
+        for f in range(args.warmup):
+            mc.frame(0)
+            mc.render(img, cam, seed=seed + f)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        nfr = max(5, args.steps // 2)
+        for f in range(nfr):
+            mc.frame(0)
+            mc.render(img, cam, seed=seed + args.warmup + f)
+        torch.cuda.synchronize()
+        ms_frame = (time.perf_counter() - t0) * 1e3 / nfr
+        tr, up = [], []
+        for f in range(nfr):
+            fs = mc.frame(0, timed=True)
+            tr.append(fs.ms_trace); up.append(fs.ms_update)
+        method_frames[mname] = {"ms_per_frame": round(ms_frame, 4), "entry_pass_ms": round(float(np.mean(tr)), 4),
+                                "update_ms": round(float(np.mean(up)), 4)}
+        del mc
+    log(f"method frames: {json.dumps(method_frames)}")
     # CPU leg (the C oracle on the host cores): the algorithmic bytes of the query pass counted on the
     # same paths (nodes / triangles / probes / resolved reads), the FP64 twins, the baseline timing
     cpu, qb, fp64 = None, None, None
@@ -885,6 +911,7 @@ def main():
         "scene_prep_s": round(prep_s, 3),
         "rays_per_frame": {"entry": n_entry / args.steps, "query_ext": n_ext / args.steps, "query_shadow": n_sh / args.steps},
         "kernels": kernels,
+        "method_frames": method_frames,
         "roofline": roof,
         "cpu_baseline": cpu,
         "fp64_relrms": fp64,
