@@ -4,6 +4,7 @@
 // the plain / targeted / flush + compact-snapshot methods over an event, the camera pass and the adjoint
 // on the room, and then reads every scene file named on the command line (corrupt files must fail
 // cleanly). Exit status 0 when every step behaves; the sanitizers abort on the first fault.
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -96,6 +97,42 @@ static void run_oracle(const rc::Scene& S) {
                    dalb.data());
 }
 
+// quantized wide nodes: every child box decoded in exact (double) arithmetic holds its fp32 box with at
+// least one step of slack on each side, steps are powers of two at or above the floor, empty slots and
+// children are as in the fp32 nodes
+static void check_qnodes(const rc::Scene& S) {
+    std::vector<rc_qnode> qn;
+    const float floor_step = rc::quant_step_floor(S.BN, S.lo, S.ext);
+    rc::build_qnodes(S.WN, floor_step, qn);
+    CHECK(qn.size() == S.WN.size(), "quantized node count");
+    long long boxes = 0;
+    for (size_t i = 0; i < qn.size(); ++i) {
+        const rc_wnode& w = S.WN[i];
+        const rc_qnode& q = qn[i];
+        const float o[3] = {q.ox, q.oy, q.oz}, st[3] = {q.sx, q.sy, q.sz};
+        const uint32_t ql[3] = {q.qlox, q.qloy, q.qloz}, qh[3] = {q.qhix, q.qhiy, q.qhiz};
+        const float* lo[3] = {w.lox, w.loy, w.loz};
+        const float* hi[3] = {w.hix, w.hiy, w.hiz};
+        for (int a = 0; a < 3; ++a) {
+            int e2 = 0;
+            CHECK(std::frexp((double)st[a], &e2) == 0.5 && st[a] >= floor_step, "node %zu axis %d: step %g not a power of two >= %g", i,
+                  a, st[a], floor_step);
+        }
+        for (int c = 0; c < 4; ++c) {
+            CHECK(q.child[c] == w.child[c], "node %zu child %d", i, c);
+            for (int a = 0; a < 3; ++a) {
+                const uint32_t l = (ql[a] >> (8 * c)) & 255u, h = (qh[a] >> (8 * c)) & 255u;
+                if (w.child[c] == 0) { CHECK(l == 255u && h == 0u, "node %zu empty slot %d", i, c); continue; }
+                const double dl = (double)o[a] + (double)l * st[a], dh = (double)o[a] + (double)h * st[a];
+                CHECK(dl <= (double)lo[a][c] - st[a] && dh >= (double)hi[a][c] + st[a], "node %zu child %d axis %d: [%g, %g] vs [%g, %g]",
+                      i, c, a, dl, dh, lo[a][c], hi[a][c]);
+                ++boxes;
+            }
+        }
+    }
+    CHECK(boxes > 0, "no quantized boxes checked");
+}
+
 int main(int argc, char** argv) {
     rc::Scene room, street;
     std::string e = rc::prepare(rc::gen_box_room(1, 600), room);
@@ -111,6 +148,8 @@ int main(int argc, char** argv) {
     CHECK(e.empty() && back.Np == room.Np && back.nst == room.nst && back.BN.size() == room.BN.size(), "reload: %s", e.c_str());
     run_oracle(room);
     run_oracle(street);
+    check_qnodes(room);
+    check_qnodes(street);
     for (int k = 1; k < argc; ++k) {   // scene files given by the caller: read or refused, never a fault
         rc::Scene S; rc::FileRefs R;
         e = rc::load_scene_file(argv[k], S, R);
