@@ -1,7 +1,8 @@
 """The host scene preparation (render_engine_amd/csrc/rc_scene.cpp) and the CPU oracle built with
 AddressSanitizer and UndefinedBehaviorSanitizer (tests/native): the driver prepares the procedural scenes,
 round-trips a scene file, runs the oracle's build, runner frames over an event, camera pass and adjoint,
-then reads truncated and corrupt scene files (CPU only; needs g++ with the sanitizer runtimes)."""
+checks the quantized wide nodes' boxes against the fp32 ones in exact arithmetic, then reads truncated and
+corrupt scene files (CPU only; needs g++ with the sanitizer runtimes)."""
 import os
 import shutil
 import subprocess
