@@ -201,6 +201,24 @@ size_t rc_cache_array_bytes(const rc_cache* c, int which);
 rc_status rc_cache_get(const rc_cache* c, int which, void* host_dst, size_t bytes);
 rc_status rc_cache_set(rc_cache* c, int which, const void* host_src, size_t bytes);
 rc_status rc_cache_sync(rc_cache* c);
+/* The event block of the transition from the cache's current state to to_state, run now instead of at the
+ * start of the next rc_cache_frame (which then runs none): snapshot of the state being left, rescale /
+ * dormancy / wake, emitter and occluder invalidation, targeted invalidation, flush, snapshot restore,
+ * eventboost (cuda/poc_gpu.cu:856-943; rtxgi_bench.patch:355-366, 1028-1050). invalidated_out: entries
+ * the targeted invalidation reset (waits for it). Before the first frame only the state changes, as in
+ * rc_cache_frame. */
+rc_status rc_cache_event(rc_cache* c, int to_state, int* invalidated_out);
+/* Dependency-targeted invalidation with an explicit invalidation set, the resolve-time test of the SDK
+ * bench (rtxgi_bench.patch:368-386, 1028-1050) on the cache's recorded masks: an entry whose light masks
+ * (both banks) meet light_mask (bit min(l, 63) per light), or whose transit masks (both banks) hold one of
+ * the nvox transit-grid voxels, or that recorded no dependency at all, restarts from zero. Meaningful for
+ * methods that record dependencies (rc_method.deps); the resolved radiance follows at the next frame's
+ * update or rc_cache_resolve. invalidated_out: entries reset (waits for the count). */
+rc_status rc_cache_invalidate(rc_cache* c, unsigned long long light_mask, const int* voxels, int nvox, int* invalidated_out);
+/* Resolved radiance R = A sum_l (Edir_l + Lc_l) of every entry from the current channels (the value the
+ * camera pass reads; the resolve of SHaRC's SharcResolveEntry, rtxgi_bench.patch:1044, and the cache
+ * read of cuda/poc_gpu.cu:470-478), e.g. after rc_cache_set of the channels. */
+rc_status rc_cache_resolve(rc_cache* c);
 
 /* ---- kernel level (device pointers) ---- */
 /* closest hit of n rays: t (inf on a miss), q (>= 0 quad of a dynamic triangle, -2-j static j,

@@ -82,6 +82,7 @@ def _load():
     lib.orc_k_update.argtypes = [C.POINTER(OrcScene), C.POINTER(OrcCache), C.POINTER(OrcParams), _P]
     lib.orc_k_ref.argtypes = [C.POINTER(OrcScene), C.c_int, _P, _P, _P, _P, C.c_uint32, C.c_int, C.c_int, C.c_int]
     lib.orc_k_resolve.argtypes = [C.POINTER(OrcScene), _P, _P, C.c_uint64]
+    lib.orc_k_invalidate.argtypes = [C.POINTER(OrcCache), C.c_int, C.c_int, C.c_uint64, _P, C.c_int, C.c_int]
     lib.orc_count_enable.argtypes = [C.c_int]
     lib.orc_count_get.argtypes = [_P]
     lib.orc_k_query.argtypes = [C.POINTER(OrcScene), C.c_int, _P, _P, C.c_uint32, C.c_uint32, C.c_uint64, _P, _P, C.c_int, C.c_int]

@@ -254,6 +254,25 @@ class Cache:
     def sync(self):
         check(lib.rc_cache_sync(self._h))
 
+    def event(self, to_state):
+        """the event block of the transition to to_state now (the next frame in to_state runs none);
+        returns the entries the targeted invalidation reset"""
+        n = C.c_int(0)
+        check(lib.rc_cache_event(self._h, int(to_state), C.byref(n)))
+        return n.value
+
+    def invalidate(self, light_mask=0, voxels=()):
+        """targeted invalidation on the recorded masks: entries depending on a light of light_mask (bit
+        min(l, 63)) or a transit voxel of voxels, or on nothing recorded, restart; returns their count"""
+        v = np.ascontiguousarray(voxels, np.int32)
+        n = C.c_int(0)
+        check(lib.rc_cache_invalidate(self._h, int(light_mask), v.ctypes.data_as(C.POINTER(C.c_int)), len(v), C.byref(n)))
+        return n.value
+
+    def resolve(self):
+        """resolved radiance R = A sum_l (Edir_l + Lc_l) of every entry from the current channels"""
+        check(lib.rc_cache_resolve(self._h))
+
 
 # ---------------------------------------------------------------- kernel level (torch device tensors)
 def trace(scene, O, D, t, q, state=0, trav=TRAV_WIDE, stream=None):

@@ -91,6 +91,9 @@ _SIGS = {
     "rc_cache_get": (C.c_int, [_P, C.c_int, _P, C.c_size_t]),
     "rc_cache_set": (C.c_int, [_P, C.c_int, _P, C.c_size_t]),
     "rc_cache_sync": (C.c_int, [_P]),
+    "rc_cache_event": (C.c_int, [_P, C.c_int, C.POINTER(C.c_int)]),
+    "rc_cache_invalidate": (C.c_int, [_P, C.c_ulonglong, C.POINTER(C.c_int), C.c_int, C.POINTER(C.c_int)]),
+    "rc_cache_resolve": (C.c_int, [_P]),
     "rc_cache_set_render_mode": (C.c_int, [_P, C.c_int]),
     "rc_cache_set_stack_levels": (C.c_int, [_P, C.c_int]),
     "rc_scene_set_stack_levels": (C.c_int, [_P, C.c_int]),

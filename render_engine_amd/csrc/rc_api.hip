@@ -869,6 +869,57 @@ extern "C" rc_status rc_cache_frame(rc_cache* c, int si, int want_err, int timed
     return RC_OK;
 }
 
+// ---- the event block, targeted invalidation and the resolve as calls of their own
+
+extern "C" rc_status rc_cache_event(rc_cache* c, int to_state, int* invalidated_out) {
+    if (!c) return fail(RC_ERR_ARG, "NULL cache");
+    if (!c->begun) return fail(RC_ERR_STATE, "rc_cache_begin not called");
+    if (to_state < 0 || to_state >= c->S->H.NS) return fail(RC_ERR_ARG, "state %d out of range", to_state);
+    if (invalidated_out) *invalidated_out = 0;
+    if (to_state == c->prev_state) return RC_OK;
+    if (c->f > 0) {   // as rc_cache_frame: before the first frame a state change runs no event block
+        HIPCK(wait_query(c));
+        rc_status r = run_event(c, c->prev_state, to_state);
+        if (r != RC_OK) return r;
+        if (c->inv_pending) {
+            HIPCK(hipEventSynchronize(c->ev_inv));
+            if (invalidated_out) *invalidated_out = *c->h_inv;
+            c->inv_pending = false;
+        }
+    }
+    c->prev_state = to_state;
+    return RC_OK;
+}
+
+extern "C" rc_status rc_cache_invalidate(rc_cache* c, unsigned long long light_mask, const int* voxels, int nvox, int* invalidated_out) {
+    if (!c) return fail(RC_ERR_ARG, "NULL cache");
+    if (!c->begun) return fail(RC_ERR_STATE, "rc_cache_begin not called");
+    rc_scene* S = c->S;
+    const rc::Scene& H = S->H;
+    if (nvox < 0 || (nvox > 0 && !voxels)) return fail(RC_ERR_ARG, "rc_cache_invalidate: bad voxel list");
+    std::vector<int> V(voxels, voxels + nvox);
+    for (int v : V)
+        if (v < 0 || v >= 32 * H.nw) return fail(RC_ERR_ARG, "rc_cache_invalidate: voxel %d outside the transit grid (%d voxels)", v, 32 * H.nw);
+    const int Np = H.Np;
+    hipStream_t st = S->stream;
+    int* cnt = c->c.blkcnt + (Np + 255) / 256;
+    HIPCK(rc_launch_invalidate(&c->c, Np, H.nl, (uint64_t)light_mask, nullptr, vox_mask(V), H.nw, nullptr, nullptr, cnt, st));
+    HIPCK(hipMemcpyAsync(c->h_inv, cnt, sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPCK(hipStreamSynchronize(st));
+    if (invalidated_out) *invalidated_out = *c->h_inv;
+    return RC_OK;
+}
+
+extern "C" rc_status rc_cache_resolve(rc_cache* c) {
+    if (!c) return fail(RC_ERR_ARG, "NULL cache");
+    if (!c->begun) return fail(RC_ERR_STATE, "rc_cache_begin not called");
+    rc_scene* S = c->S;
+    HIPCK(wait_query(c));   // the last camera pass may still read R
+    HIPCK(rc_launch_resolve(&S->ds, &S->dst[c->prev_state], c->c.Lc, c->c.R, c->dormant, S->stream));
+    c->r_valid = true;
+    return RC_OK;
+}
+
 // Staged camera-pass workspace for npath paths (grown on demand, kept between calls).
 static hipError_t qwork_reserve(rc_cache* c, long long npath) {
     if (npath <= c->qw_paths) return hipSuccess;

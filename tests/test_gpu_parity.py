@@ -314,6 +314,64 @@ def test_query_stack_levels(torch_gpu, street):
         cache.set_stack_levels(0)
 
 
+@pytest.mark.parametrize("mk", [rea.methods.targeted, rea.methods.flush_snapshots, lambda: dict(rea.methods.timing_list())["v4-A2C suivi 1/4"]])
+def test_event_call_equals_the_frames_event(torch_gpu, street, mk):
+    """rc_cache_event run between frames gives, frame by frame, the arrays and reset counts of the event
+    block that rc_cache_frame runs itself on a state change"""
+    scene, _ = street
+    seq = [0] * 21 + [3] * 2 + [0] * 2 + [2] * 2 + [1] * 2
+    a, b = rea.Cache(scene, mk()), rea.Cache(scene, mk())
+    a.begin(seq[0], 13)
+    b.begin(seq[0], 13)
+    names = ("N", "KK", "R", "AGE", "LC", "M", "NPEND", "SUMPR")
+    for f, st in enumerate(seq):
+        inv_a = a.event(st) if f > 0 and seq[f - 1] != st else 0
+        fa = a.frame(st)
+        fb = b.frame(st)
+        assert fa.event == 0
+        assert inv_a == fb.invalidated, f"frame {f}: {inv_a} vs {fb.invalidated}"
+        for arr in names + (("LCUR", "CUR") if a.method.deps else ()):
+            ga, gb = a.get(arr), b.get(arr)
+            if ga.dtype == np.float32:
+                ga, gb = ga.view(np.uint32), gb.view(np.uint32)
+            assert np.array_equal(ga, gb), f"frame {f} state {st}: {arr} differs in {np.sum(ga != gb)} entries"
+
+
+def test_invalidate_and_resolve_calls_against_oracle(torch_gpu, street):
+    """rc_cache_invalidate with an explicit light set and voxel list resets exactly the oracle's entries
+    (orc_k_invalidate on the same recorded masks), and rc_cache_resolve gives the oracle's resolved radiance"""
+    scene, osc = street
+    m = rea.methods.targeted()
+    cache = rea.Cache(scene, m)
+    orun = po.OracleRunner(osc, m)
+    cache.begin(0, 21)
+    orun.begin(0, 21)
+    for _ in range(4):
+        cache.frame(0)
+        orun.frame(0)
+    i = scene.info
+    for mask, vox in ((1 << 2, []), (0, [5, 77, 100]), ((1 << 0) | (1 << 63), [3])):
+        n = cache.invalidate(mask, vox)
+        c = po.lib.orc_runner_cache(orun.h)
+        npend = C.cast(c.contents.npend, C.POINTER(C.c_int32))
+        before = int(npend[1])
+        v = np.asarray(vox, np.int32)
+        po.lib.orc_k_invalidate(c, i.Np, i.nl, mask, po.ptr(v) if len(v) else None, len(v), i.nw)
+        assert n == int(npend[1]) - before and n > 0, (mask, vox, n)
+        for arr in ("LC", "N", "AGE"):
+            g, e = cache.get(arr), orun.get(arr)
+            if g.dtype == np.float32:
+                g, e = g.view(np.uint32), e.view(np.uint32)
+            assert np.array_equal(g, e), f"{arr} after invalidating {mask:#x} {vox}: {np.sum(g != e)} entries differ"
+    cache.resolve()
+    R = np.zeros(i.Np, np.float32)
+    Lc = orun.get("LC")
+    po.lib.orc_k_resolve(osc.state(0), po.ptr(Lc), po.ptr(R), 0)
+    assert np.array_equal(cache.get("R").view(np.uint32), R.view(np.uint32))
+    with pytest.raises(rea.RcError):
+        cache.invalidate(0, [32 * i.nw])
+
+
 @pytest.mark.parametrize("which", ["room", "street"])
 def test_query_quantized_nodes(torch_gpu, which, request):
     """The camera pass over the quantized wide nodes (traversal mode 3) against the oracle's camera
