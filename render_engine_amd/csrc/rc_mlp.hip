@@ -40,6 +40,15 @@ __device__ __forceinline__ uint32_t rc_mlp_index(const rc_mlpdev& m, int lv, uin
     return (x ^ (y * 2654435761u) ^ (z * 805459861u)) & ((1u << m.log2T) - 1u);
 }
 
+// Orders one wave's LDS writes before its later LDS reads. The forward and backward passes keep
+// each wave's activations in a tile of its own, so no other wave waits: a wave's LDS instructions
+// complete in issue order, and the fences keep the compiler from moving accesses across this point.
+__device__ __forceinline__ void rc_wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // corner grid of level lv for the normalised position x: base corner and fractions
 __device__ __forceinline__ void rc_mlp_cell(const rc_mlpdev& m, int lv, f3 x, uint32_t& ix, uint32_t& iy, uint32_t& iz, f3& fr) {
     const float r = (float)m.res[lv];
@@ -98,7 +107,7 @@ __global__ void __launch_bounds__(MROWS * MWAVES) k_mlp_fwd(rc_mlpdev m, const f
         else for (int k = 0; k < MW; ++k) row[k] = rc_bf(0.f);
         if (TRAIN) for (int k = 0; k < MW; ++k) actT[(int64_t)k * ldT + base + lane] = row[k];
     }
-    __syncthreads();
+    rc_wave_sync();
     // Each layer's weight operands are loaded once into registers (8 x 16 bytes per lane) and serve
     // the wave's 4 row tiles; the products run as D = W A^T, so a lane's 4 results are 4 consecutive
     // output features of one sample: one 8-byte LDS store per column tile. (Operands reloaded from
@@ -132,7 +141,7 @@ __global__ void __launch_bounds__(MROWS * MWAVES) k_mlp_fwd(rc_mlpdev m, const f
                 }
             }
         }
-        __syncthreads();
+        rc_wave_sync();
     }
     const __bf16* W5 = Wt + (size_t)RC_MLP_HIDDEN * MW * MW;   // [16][64], output column 0
     const bf16x8 w50 = rc_opnd(W5, MW, 0, 0), w51 = rc_opnd(W5, MW, 0, 32);
@@ -196,7 +205,7 @@ __global__ void __launch_bounds__(MROWS * MWAVES) k_mlp_bwd(rc_mlpdev m, const f
             gzT[((int64_t)(RC_MLP_HIDDEN - 1) * MW + k) * ldT + s] = v;
         }
     }
-    __syncthreads();
+    rc_wave_sync();
     for (int L = RC_MLP_HIDDEN - 1; L >= 0; --L) {
         const __bf16* W = Wk + (size_t)L * MW * MW;   // W_L[k][j]
         bf16x8 wop[2][4];   // the layer's weight operands, loaded once for the wave's 4 row tiles
@@ -227,7 +236,7 @@ __global__ void __launch_bounds__(MROWS * MWAVES) k_mlp_bwd(rc_mlpdev m, const f
                 }
             }
         }
-        __syncthreads();
+        rc_wave_sync();
     }
 }
 
