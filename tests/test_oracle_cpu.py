@@ -48,6 +48,80 @@ def test_hash32_is_a_bijection_and_rnd_uniform():
     assert np.all((u * 16777216.0) == np.floor(u * 16777216.0))   # 24-bit grid
 
 
+M32 = 0xFFFFFFFF
+
+
+def _lowbias32_inverse(x):
+    # the published inverse of lowbias32 (C. Wellons, "Prospecting for Hash Functions", 2018,
+    # hash-prospector: lowbias32_r); it undoes hash32 only if every constant and shift matches
+    x ^= x >> 16
+    x = (x * 0x43021123) & M32
+    x ^= (x >> 15) ^ (x >> 30)
+    x = (x * 0x1D69E2A5) & M32
+    x ^= x >> 16
+    return x
+
+
+def test_hash32_against_published_inverse():
+    rng = np.random.default_rng(7)
+    xs = [0, 1, 2, 0x7FFFFFFF, 0x80000000, M32] + [int(v) for v in rng.integers(0, 1 << 32, 20000, dtype=np.uint64)]
+    for x in xs:
+        assert _lowbias32_inverse(L.orc_hash32(x)) == x
+
+
+def _fmix32(h, c1):
+    # MurmurHash3's 32-bit finaliser with its second multiplier 0xc2b2ae35 (Appleby, SMHasher)
+    h ^= h >> 16; h = (h * c1) & M32; h ^= h >> 13; h = (h * 0xC2B2AE35) & M32; h ^= h >> 16
+    return h
+
+
+def test_seed_chain_constants():
+    # per-entry seed hash32(seed * G ^ hash32(frame * C ^ hash32(i + 1))) (cuda/poc_gpu.cu:442):
+    # peel it with the published inverse of hash32 to read G and C back out of the oracle
+    inv = _lowbias32_inverse
+    G = inv(L.orc_seed(1, 0, 0)) ^ L.orc_hash32(L.orc_hash32(1))
+    C_ = inv(inv(L.orc_seed(0, 1, 0))) ^ L.orc_hash32(1)
+    # G = floor(2^32 / golden ratio) (Knuth's multiplicative hashing constant)
+    assert G == int(math.floor(2**32 * (math.sqrt(5.0) - 1.0) / 2.0)) == 0x9E3779B9
+    # C is MurmurHash3's fmix32 multiplier: with it the published MurmurHash3_x86_32 vectors of
+    # the empty key come out ("" seed 1 -> 0x514E28B7, seed 0xffffffff -> 0x81F16F39)
+    assert _fmix32(1, C_) == 0x514E28B7 and _fmix32(M32, C_) == 0x81F16F39
+    # and the chain is the composition for other arguments too
+    for seed, frame, i in ((3, 17, 999), (0xDEADBEEF, 123456, 0), (7, 0, 1 << 20)):
+        want = L.orc_hash32(((seed * G) & M32) ^ L.orc_hash32(((frame * C_) & M32) ^ L.orc_hash32((i + 1) & M32)))
+        assert L.orc_seed(seed, frame, i) == want
+
+
+def _pcg_rxs_m_xs_inverse(w):
+    # inverse of PCG's 32-bit RXS-M-XS output permutation, with the multiplier's published
+    # inverse 2897767785 (pcg_random.hpp, mcg_unmultiplier<uint32_t>) for 277803737
+    w ^= w >> 22
+    w = (w * 2897767785) & M32
+    s = w
+    sh = (s >> 28) + 4          # the top 4 bits are untouched by the shift of at least 4
+    x = s
+    for _ in range(8):          # undo x ^= x >> sh
+        x = s ^ (x >> sh)
+    return x
+
+
+def test_rnd_against_pcg():
+    # state step: PCG's default 32-bit LCG (multiplier 747796405, increment 2891336453,
+    # pcg_random.hpp default_multiplier / default_increment<uint32_t>); full period by Hull-Dobell
+    # (c odd, a - 1 divisible by 4, modulus a power of two)
+    a, c = 747796405, 2891336453
+    assert c % 2 == 1 and (a - 1) % 4 == 0
+    rng = np.random.default_rng(11)
+    for s0 in [0, 1, M32] + [int(v) for v in rng.integers(0, 1 << 32, 3000, dtype=np.uint64)]:
+        s = C.c_uint32(s0)
+        u = L.orc_rnd(C.byref(s))
+        assert s.value == (s0 * a + c) & M32
+        # output: 24 high bits of RXS-M-XS(state); one of the 256 completions of the low byte
+        # must invert, through the published unmultiplier, to the state itself
+        hi = int(u * 16777216.0)
+        assert any(_pcg_rxs_m_xs_inverse((hi << 8) | b) == s.value for b in range(256))
+
+
 def test_sincos_against_libm():
     xs = np.linspace(0.0, 2 * math.pi, 100001, dtype=np.float32)
     s, c = C.c_float(), C.c_float()
