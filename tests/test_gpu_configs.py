@@ -317,7 +317,7 @@ def test_mlp_full_shape(torch_gpu):
     dy = rng.normal(size=n).astype(np.float32) / n
     mlp.zero_grad()
     mlp.backward(Pt, Nt, Dt, torch.from_numpy(dy).cuda())
-    gWr, gTr = backward_ref(cfg, W, acts, corners, dy, None)
+    gWr, gTr = backward_ref(cfg, W, acts, corners, dy)
     assert rel(mlp.get(M.MLP_GW), gWr) < 2e-2
     assert rel(mlp.get(M.MLP_GTABLE), gTr) < 2e-2
     mlp.zero_grad()

@@ -1,99 +1,29 @@
-"""Hash-encoded MLP cache kernels (BF16 MFMA, FP32 accumulation) against a float32 restatement
-of the same network on the CPU with the same BF16 roundings: forward outputs, weight and table
-gradients, the Adam step, and training that fits a smooth function. Tolerances: the encoding is
-exact (same operation order, no contraction); each MFMA layer differs from the CPU sum in
-accumulation order only, which can flip a BF16 rounding, so outputs and gradients are compared
-by relative L2 (2e-2)."""
+"""Hash-encoded MLP cache kernels (BF16 MFMA, FP32 accumulation) against the oracle's restatement
+of the same network (oracle/mlp_oracle.py, pinned on the CPU by tests/test_mlp_oracle_cpu.py) with
+the same BF16 roundings: forward outputs, weight and table gradients, the Adam step, and training
+that fits a smooth function. Tolerances: the encoding is exact (same operation order, no
+contraction); each MFMA layer differs from the CPU sum in accumulation order only, which can flip
+a BF16 rounding, so outputs and gradients are compared by relative L2 (2e-2)."""
 import numpy as np
 import pytest
 
+import mlp_oracle as MO
 import render_engine_amd.mlp as M
 
 pytestmark = pytest.mark.gpu
 
 
-def bf(x):
-    import torch
-    return torch.from_numpy(np.ascontiguousarray(x, np.float32)).to(torch.bfloat16).float().numpy()
-
-
 def encode_ref(cfg, table, P, N, D):
-    res, dense = M.level_resolutions(cfg)
-    T = 1 << cfg.log2_table
-    tb = table.reshape(M.LEVELS, T, 2)
-    lo = np.array(list(cfg.lo), np.float32)
-    iext = (np.float32(1) / np.array(list(cfg.ext), np.float32)).astype(np.float32)
-    x = np.clip((P[:, :3] - lo) * iext, np.float32(0), np.float32(1)).astype(np.float32)
-    n = len(P)
-    X = np.zeros((n, 64), np.float32)
-    corners = []
-    for lv in range(M.LEVELS):
-        r = np.float32(res[lv])
-        s = (x * r).astype(np.float32)
-        f = np.minimum(np.floor(s), r - np.float32(1)).astype(np.float32)
-        i = f.astype(np.uint32)
-        fr = (s - f).astype(np.float32)
-        f0 = np.zeros(n, np.float32); f1 = np.zeros(n, np.float32)
-        cl = []
-        for c in range(8):
-            cx, cy, cz = i[:, 0] + np.uint32(c & 1), i[:, 1] + np.uint32((c >> 1) & 1), i[:, 2] + np.uint32(c >> 2)
-            wx = fr[:, 0] if c & 1 else np.float32(1) - fr[:, 0]
-            wy = fr[:, 1] if (c >> 1) & 1 else np.float32(1) - fr[:, 1]
-            wz = fr[:, 2] if c >> 2 else np.float32(1) - fr[:, 2]
-            w = ((wx * wy).astype(np.float32) * wz).astype(np.float32)
-            if dense[lv]:
-                r1 = np.uint32(res[lv] + 1)
-                idx = cx + r1 * (cy + r1 * cz)
-            else:
-                idx = (cx ^ (cy * np.uint32(2654435761)) ^ (cz * np.uint32(805459861))) & np.uint32(T - 1)
-            f0 = (f0 + (w * tb[lv, idx, 0]).astype(np.float32)).astype(np.float32)
-            f1 = (f1 + (w * tb[lv, idx, 1]).astype(np.float32)).astype(np.float32)
-            cl.append((idx, w))
-        X[:, 2 * lv], X[:, 2 * lv + 1] = f0, f1
-        corners.append(cl)
-    X[:, 32:35] = np.float32(0.5) * (N[:, :3] + np.float32(1))
-    X[:, 35:38] = np.float32(0.5) * (D[:, :3] + np.float32(1))
-    X[:, 38] = 1
-    return bf(X), corners
-
-
-def split_w(W):
-    Ws = [W[L * 4096:(L + 1) * 4096].reshape(64, 64) for L in range(M.HIDDEN)]
-    W5 = W[M.HIDDEN * 4096:].reshape(64, 16)
-    return Ws, W5
+    return MO.encode(table, P, N, D, np.array(list(cfg.lo), np.float32), np.array(list(cfg.ext), np.float32), cfg.log2_table,
+                     cfg.base_res, cfg.max_res)
 
 
 def forward_ref(W, X):
-    Ws, W5 = split_w(bf(W))
-    acts = [X]
-    a = X
-    for L in range(M.HIDDEN):
-        a = bf(np.maximum(a @ Ws[L], 0))
-        acts.append(a)
-    return (a @ W5)[:, 0], acts
+    return MO.forward(W, X)
 
 
-def backward_ref(cfg, W, acts, corners, dy, nt):
-    Ws, W5 = split_w(bf(W))
-    gW = np.zeros_like(W)
-    a5 = acts[-1]
-    gW[M.HIDDEN * 4096:].reshape(64, 16)[:, 0] = a5.T @ dy
-    gz = bf(np.where(a5 > 0, dy[:, None] * W5[:, 0][None, :], 0))
-    gX = None
-    for L in range(M.HIDDEN - 1, -1, -1):
-        gW[L * 4096:(L + 1) * 4096] = (acts[L].T.astype(np.float64) @ gz.astype(np.float64)).ravel()
-        ga = gz @ Ws[L].T
-        if L > 0:
-            gz = bf(np.where(acts[L] > 0, ga, 0))
-        else:
-            gX = ga[:, :32]
-    T = 1 << cfg.log2_table
-    gt = np.zeros((M.LEVELS, T, 2))
-    for lv in range(M.LEVELS):
-        for idx, w in corners[lv]:
-            np.add.at(gt[lv, :, 0], idx, w.astype(np.float64) * gX[:, 2 * lv])
-            np.add.at(gt[lv, :, 1], idx, w.astype(np.float64) * gX[:, 2 * lv + 1])
-    return gW, gt.ravel()
+def backward_ref(cfg, W, acts, corners, dy):
+    return MO.backward(W, acts, corners, dy, cfg.log2_table)
 
 
 def _inputs(torch, n, seed, lo, ext):
@@ -131,6 +61,7 @@ def test_mlp_forward_matches_reference(torch_gpu, small, n):
     X, _ = encode_ref(cfg, mlp.get(M.MLP_TABLE), P, N, D)
     yr, _ = forward_ref(mlp.get(M.MLP_W), X)
     assert np.all(np.isfinite(y))
+    print(f"[mlp] forward n={n}: relative L2 {_rel(y, yr):.2e}", flush=True)
     assert _rel(y, yr) < 2e-2, _rel(y, yr)
 
 
@@ -146,7 +77,8 @@ def test_mlp_gradients_match_reference(torch_gpu, small):
     W = mlp.get(M.MLP_W)
     X, corners = encode_ref(cfg, mlp.get(M.MLP_TABLE), P, N, D)
     _, acts = forward_ref(W, X)
-    gWr, gTr = backward_ref(cfg, W, acts, corners, dy, None)
+    gWr, gTr = backward_ref(cfg, W, acts, corners, dy)
+    print(f"[mlp] gradients: weights {_rel(gW, gWr):.2e}, table {_rel(gT, gTr):.2e}", flush=True)
     for L in range(M.HIDDEN):
         sl = slice(L * 4096, (L + 1) * 4096)
         assert _rel(gW[sl], gWr[sl]) < 2e-2, (L, _rel(gW[sl], gWr[sl]))
