@@ -135,6 +135,29 @@ def cpu_baseline(osc, scene, seconds, log):
                        f"same scene, state 0", fps=frames / t_tot)
 
 
+def mlp_cpu_baseline(mlp, cfg, P, N, D, seconds, log):
+    """The learned cache's oracle (numpy, one thread) on the host: encoding + network inference of
+    the same queries with the same table and weights, timed per batch."""
+    import numpy as np
+    import mlp_oracle as MO
+    import render_engine_amd.mlp as M
+    from threadpoolctl import threadpool_limits
+
+    table, W = mlp.get(M.MLP_TABLE), mlp.get(M.MLP_W)
+    lo, ext = np.array(list(cfg.lo), np.float32), np.array(list(cfg.ext), np.float32)
+    n, done, t_tot = len(P), 0, 0.0
+    with threadpool_limits(limits=1):
+        while done == 0 or (t_tot < seconds and done < 8 * n):
+            t0 = time.perf_counter()
+            X, _ = MO.encode(table, P, N, D, lo, ext, cfg.log2_table, cfg.base_res, cfg.max_res)
+            MO.forward(W, X)
+            t_tot += time.perf_counter() - t0
+            done += n
+    log(f"mlp cpu baseline: {done} queries in {t_tot:.1f} s on 1 thread")
+    return dict(value=done / t_tot / 1e6, unit="Mqueries/s (encoding + network inference only, no path tracing)", cores=1, kind="port",
+                sample=f"{done} queries ({n} cache-entry positions and normals) through the numpy oracle, one thread")
+
+
 def reference_images(rea, scene, cam, log, ref_spp_rounds=16):
     """converged per-state references (two-chain Jacobi solve), then per state the camera image at
     ref_spp_rounds x spp through the converged cache"""
@@ -631,6 +654,10 @@ def bench_mlp(args, rea, scene, cfg, log):
     e1.record()
     e1.synchronize()
     ms_inf = e0.elapsed_time(e1) / reps
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        cpu = mlp_cpu_baseline(mlp, mlp.cfg, P.cpu().numpy(), Nn.cpu().numpy(), Nn.cpu().numpy(), min(args.cpu_seconds, 8.0), log)
     flop_per_query = 2 * (M.HIDDEN * 64 * 64 + 64 * 16)
     tflops = B * flop_per_query / (ms_inf * 1e-3) / 1e12
     log(f"frame {np.mean(fr):.3f} ms, train {np.mean(tr):.3f} ms, render {np.mean(rd):.3f} ms, infer {ms_inf:.4f} ms "
@@ -700,7 +727,8 @@ def bench_mlp(args, rea, scene, cfg, log):
                      "kernel": "k_mlp_fwd (inference, 65536 queries)",
                      "basis": f"{flop_per_query} FLOP per query (5 x 64x64 + 64x16 multiply-adds)"},
         "judge_relrms": {"after_warmup": round(float(judges[args.warmup + 199]), 4), "timed_mean": round(float(np.mean(judges[-args.steps:])), 4)},
-        "cpu_baseline": None,
+        "cpu_baseline": cpu,
+        "gpu_inference_over_cpu": round(B / (ms_inf * 1e-3) / 1e6 / cpu["value"], 1) if cpu else None,
         "light_off": qual,
     }
     print(json.dumps(out))

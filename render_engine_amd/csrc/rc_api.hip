@@ -1356,6 +1356,13 @@ extern "C" rc_status rc_mlp_create(const rc_mlp_config* c, rc_mlp** out) {
     MK(dalloc(&b.mW, (size_t)RC_MLP_NW, m->owned)); MK(dalloc(&b.vW, (size_t)RC_MLP_NW, m->owned));
     MK(dalloc(&b.mT, nt, m->owned)); MK(dalloc(&b.vT, nt, m->owned)); MK(dalloc(&b.loss, 1, m->owned));
     {
+        const size_t nbk = (size_t)RC_MLP_LEVELS * (T >> std::min(RC_EB_SB, c->log2_table));
+        b.epairs = (int64_t)std::min<size_t>(cap, RC_EB_CHUNK) * RC_EB_PAIRS;
+        MK(dalloc(&b.ecnt, nbk + 1, m->owned)); MK(dalloc(&b.ecur, nbk + 1, m->owned));
+        MK(dalloc(&b.ebase, nbk + 1, m->owned)); MK(dalloc(&b.epart, nbk + 1, m->owned));
+        MK(dalloc(&b.eslot, (size_t)b.epairs, m->owned)); MK(dalloc(&b.eval, 2 * (size_t)b.epairs, m->owned));
+    }
+    {
         const size_t nb = (size_t)c->max_batch;
         MK(dalloc(&m->sP, nb, m->owned)); MK(dalloc(&m->sN, nb, m->owned)); MK(dalloc(&m->sD, nb, m->owned));
         MK(dalloc(&m->hP, nb, m->owned)); MK(dalloc(&m->hN, nb, m->owned)); MK(dalloc(&m->hD, nb, m->owned));

@@ -296,48 +296,182 @@ __global__ void __launch_bounds__(RC_TB) k_mlp_dw_sum(const float* part, int nsp
     if (q == 0 && e < RC_MLP_NW) gW[e] = ((s_q[0][threadIdx.x] + s_q[1][threadIdx.x]) + s_q[2][threadIdx.x]) + s_q[3][threadIdx.x];
 }
 
-// hash-table gradient: each sample's feature gradients spread over its 8 corners per level;
-// threads run level-major (consecutive threads: consecutive samples of one level), so the
-// atomics of a wave stay within one level's table. Training samples come in entry order,
-// which is spatially sorted, so at the coarse levels neighbouring lanes share corners: lanes
-// holding the same corner address in a run of consecutive lanes add their values first (a
-// segmented scan by shuffles) and only the run's last lane issues the two atomics.
-__global__ void __launch_bounds__(RC_TB) k_mlp_enc_bwd(rc_mlpdev m, const float4* P, int n, const float* gX) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+// ------------------------------------------------------------------------------ table gradient
+// The gradient of each level's table is every sample's two feature gradients spread over its 8
+// corners with the trilinear weights. One device-scope float atomic per corner into the 64 MB
+// gradient (random addresses) runs at the atomic rate past the L2s; instead the updates are grouped
+// by table slice (2^RC_EB_SB slots) and summed in LDS:
+//   k_eb_count    per block (one level, RC_EB_SPT x 256 samples): corner pairs per slice, counted in
+//                 LDS and added to the per-slice totals;
+//   k_eb_scan     slice starts, and the parts (at most RC_EB_PMAX pairs each) each slice is summed in;
+//   k_eb_scatter  the same pairs again, each block reserving its range of every slice with one
+//                 atomic per slice, stored as (local slot, g0, g1);
+//   k_eb_sum      one block per part: its pairs added into the slice's sums in LDS, then the slice's
+//                 gradient read, added to and written back with 16-byte accesses; a slice summed in
+//                 several parts (coarse levels, where many samples share few corners) adds its
+//                 nonzero sums with atomics instead.
+// Within a wave, lanes hold consecutive samples of one level. Training samples come in entry order,
+// which is spatially sorted, so at the coarse levels neighbouring lanes share corners: lanes holding
+// the same slot in a run of consecutive lanes add their values first (a segmented scan by shuffles)
+// and only the run's last lane emits the pair. Batches above RC_EB_CHUNK samples run in rounds.
+#define RC_EB_SPT 4
+#define RC_EB_PMAX 16384
+
+// a block's sweep over its samples of the round at level lv; emit(slot, v0, v1) for every pair
+template <class F>
+__device__ __forceinline__ void rc_eb_sweep(const rc_mlpdev& m, int lv, const float4* P, const float* gX, int s0, int nc, F&& emit) {
     const int lane = threadIdx.x & 63;
-    const bool live = t < n * RC_MLP_LEVELS;
-    const int lv = live ? t / n : 0, s = live ? t % n : 0;
-    float g0 = 0.f, g1 = 0.f;
-    if (live) { g0 = gX[(int64_t)s * (2 * RC_MLP_LEVELS) + 2 * lv]; g1 = gX[(int64_t)s * (2 * RC_MLP_LEVELS) + 2 * lv + 1]; }
-    const bool act = live && (g0 != 0.f || g1 != 0.f);
-    if (!__any(act)) return;   // wave-uniform exit
-    uint32_t ix = 0, iy = 0, iz = 0;
-    f3 fr = mk3(0.f, 0.f, 0.f);
-    if (act) rc_mlp_cell(m, lv, rc_mlp_norm(m, P[s]), ix, iy, iz, fr);
-    const size_t T = (size_t)1 << m.log2T;
     const unsigned long long below = (1ull << lane) - 1ull;
-    for (int c = 0; c < 8; ++c) {
-        const uint32_t cx = ix + (c & 1), cy = iy + ((c >> 1) & 1), cz = iz + (c >> 2);
-        const float w = ((c & 1) ? fr.x : 1.f - fr.x) * (((c >> 1) & 1) ? fr.y : 1.f - fr.y) * ((c >> 2) ? fr.z : 1.f - fr.z);
-        const long long idx = act ? (long long)((size_t)lv * T + rc_mlp_index(m, lv, cx, cy, cz)) : -1 - lane;   // inactive: own run
-        float v0 = act ? w * g0 : 0.f, v1 = act ? w * g1 : 0.f;
-        const long long prev = __shfl_up(idx, 1, 64);
-        const unsigned long long heads = __ballot(lane == 0 || prev != idx);
-        const int seg = __popcll(heads & (below | (1ull << lane)));   // run number of this lane
-#pragma unroll
-        for (int d = 1; d < 64; d <<= 1) {
-            const float o0 = __shfl_up(v0, d, 64), o1 = __shfl_up(v1, d, 64);
-            const int os = __shfl_up(seg, d, 64);
-            if (lane >= d && os == seg) { v0 += o0; v1 += o1; }
+    for (int k = 0; k < RC_EB_SPT; ++k) {
+        const int s = (blockIdx.x * RC_EB_SPT + k) * RC_TB + threadIdx.x;   // sample of the round
+        const bool live = s < nc;
+        float g0 = 0.f, g1 = 0.f;
+        if (live) {
+            const float* g = gX + (int64_t)(s0 + s) * (2 * RC_MLP_LEVELS) + 2 * lv;
+            g0 = g[0]; g1 = g[1];
         }
-        const bool tail = lane == 63 || ((heads >> (lane + 1)) & 1ull);
-        if (act && tail) {
-            float* gt = m.gtable + 2 * idx;
-            unsafeAtomicAdd(gt, v0);
-            unsafeAtomicAdd(gt + 1, v1);
+        const bool act = live && (g0 != 0.f || g1 != 0.f);
+        if (!__any(act)) continue;   // wave-uniform
+        uint32_t ix = 0, iy = 0, iz = 0;
+        f3 fr = mk3(0.f, 0.f, 0.f);
+        if (act) rc_mlp_cell(m, lv, rc_mlp_norm(m, P[s0 + s]), ix, iy, iz, fr);
+        for (int c = 0; c < 8; ++c) {
+            const uint32_t cx = ix + (c & 1), cy = iy + ((c >> 1) & 1), cz = iz + (c >> 2);
+            const float w = ((c & 1) ? fr.x : 1.f - fr.x) * (((c >> 1) & 1) ? fr.y : 1.f - fr.y) * ((c >> 2) ? fr.z : 1.f - fr.z);
+            const int idx = act ? (int)rc_mlp_index(m, lv, cx, cy, cz) : -1 - lane;   // slots < 2^24; inactive: a run of its own
+            float v0 = act ? w * g0 : 0.f, v1 = act ? w * g1 : 0.f;
+            const int prev = __shfl_up(idx, 1, 64);
+            const unsigned long long heads = __ballot(lane == 0 || prev != idx);
+            const int seg = __popcll(heads & (below | (1ull << lane)));   // run number of this lane
+#pragma unroll
+            for (int d = 1; d < 64; d <<= 1) {
+                const float o0 = __shfl_up(v0, d, 64), o1 = __shfl_up(v1, d, 64);
+                const int os = __shfl_up(seg, d, 64);
+                if (lane >= d && os == seg) { v0 += o0; v1 += o1; }
+            }
+            const bool tail = lane == 63 || ((heads >> (lane + 1)) & 1ull);
+            if (act && tail) emit((uint32_t)idx, v0, v1);
         }
     }
 }
+
+__global__ void __launch_bounds__(RC_TB) k_eb_zero(uint32_t* x, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = 0u;
+}
+
+__global__ void __launch_bounds__(RC_TB) k_eb_count(rc_mlpdev m, const float4* P, const float* gX, int s0, int nc, uint32_t* cnt) {
+    extern __shared__ uint32_t s_ecnt[];   // [nsl] pairs per slice
+    const int lv = blockIdx.y, sb = min(RC_EB_SB, m.log2T), nsl = 1 << (m.log2T - sb);
+    for (int j = threadIdx.x; j < nsl; j += RC_TB) s_ecnt[j] = 0u;
+    __syncthreads();
+    rc_eb_sweep(m, lv, P, gX, s0, nc, [&](uint32_t slot, float, float) { atomicAdd(&s_ecnt[slot >> sb], 1u); });
+    __syncthreads();
+    for (int j = threadIdx.x; j < nsl; j += RC_TB)
+        if (s_ecnt[j]) atomicAdd(&cnt[lv * nsl + j], s_ecnt[j]);
+}
+
+// exclusive scans of the slice counts (pair starts, base[nbk] = total) and of their part counts
+// (part[nbk] = total parts); clears the cursors of the scatter
+__global__ void __launch_bounds__(1024) k_eb_scan(const uint32_t* cnt, int nbk, uint32_t* base, uint32_t* part, uint32_t* cur) {
+    __shared__ uint32_t s_a[16], s_p[16];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int per = (nbk + 1023) / 1024, j0 = min(nbk, t * per), j1 = min(nbk, j0 + per);
+    uint32_t a = 0u, pc = 0u;
+    for (int j = j0; j < j1; ++j) { const uint32_t c = cnt[j]; a += c; pc += (c + RC_EB_PMAX - 1) / RC_EB_PMAX; }
+    uint32_t ia = a, ip = pc;
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t oa = __shfl_up(ia, d, 64), op = __shfl_up(ip, d, 64);
+        if (lane >= d) { ia += oa; ip += op; }
+    }
+    if (lane == 63) { s_a[w] = ia; s_p[w] = ip; }
+    __syncthreads();
+    if (t == 0) {
+        uint32_t x = 0u, y = 0u;
+        for (int k = 0; k < 16; ++k) { const uint32_t u = s_a[k], v = s_p[k]; s_a[k] = x; s_p[k] = y; x += u; y += v; }
+    }
+    __syncthreads();
+    uint32_t ea = s_a[w] + ia - a, ep = s_p[w] + ip - pc;
+    for (int j = j0; j < j1; ++j) {
+        const uint32_t c = cnt[j];
+        base[j] = ea; part[j] = ep; cur[j] = 0u;
+        ea += c; ep += (c + RC_EB_PMAX - 1) / RC_EB_PMAX;
+    }
+    if (t == 1023) { base[nbk] = ea; part[nbk] = ep; }
+}
+
+__global__ void __launch_bounds__(RC_TB) k_eb_scatter(rc_mlpdev m, const float4* P, const float* gX, int s0, int nc, const uint32_t* base,
+                                                      uint32_t* cur, uint16_t* eslot, float2* eval) {
+    extern __shared__ uint32_t s_erank[];   // [nsl] counts, then ranks; [nsl] the block's start in every slice
+    const int lv = blockIdx.y, sb = min(RC_EB_SB, m.log2T), nsl = 1 << (m.log2T - sb);
+    uint32_t* s_off = s_erank + nsl;
+    const uint32_t* bb = base + lv * nsl;
+    for (int j = threadIdx.x; j < nsl; j += RC_TB) s_erank[j] = 0u;
+    __syncthreads();
+    rc_eb_sweep(m, lv, P, gX, s0, nc, [&](uint32_t slot, float, float) { atomicAdd(&s_erank[slot >> sb], 1u); });
+    __syncthreads();
+    for (int j = threadIdx.x; j < nsl; j += RC_TB) {
+        const uint32_t c = s_erank[j];
+        s_off[j] = c ? bb[j] + atomicAdd(&cur[lv * nsl + j], c) : 0u;
+        s_erank[j] = 0u;
+    }
+    __syncthreads();
+    const uint32_t lmask = (1u << sb) - 1u;
+    rc_eb_sweep(m, lv, P, gX, s0, nc, [&](uint32_t slot, float v0, float v1) {
+        const uint32_t sl = slot >> sb, pos = s_off[sl] + atomicAdd(&s_erank[sl], 1u);
+        if (pos < bb[sl + 1]) {   // the two sweeps emit the same pairs: always inside the slice's range
+            eslot[pos] = (uint16_t)(slot & lmask);
+            eval[pos] = make_float2(v0, v1);
+        }
+    });
+}
+
+__global__ void __launch_bounds__(RC_TB) k_eb_sum(rc_mlpdev m, int nbk, const uint32_t* base, const uint32_t* part, const uint16_t* eslot,
+                                                  const float2* eval) {
+    extern __shared__ float4 s_eacc4[];    // [2^sb] x 2 sums
+    __shared__ int s_b;
+    float* s_acc = (float*)s_eacc4;
+    const int sb = min(RC_EB_SB, m.log2T), nsl = 1 << (m.log2T - sb), S = 1 << sb;
+    const uint32_t blk = blockIdx.x;
+    if (blk >= part[nbk]) return;          // the grid is an upper bound on the parts
+    if (threadIdx.x == 0) {                // the slice: the last b with part[b] <= blk
+        int lo = 0, hi = nbk - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (part[mid] <= blk) lo = mid; else hi = mid - 1;
+        }
+        s_b = lo;
+    }
+    for (int j = threadIdx.x; j < 2 * S; j += RC_TB) s_acc[j] = 0.f;
+    __syncthreads();
+    const int b = s_b;
+    const uint32_t np = part[b + 1] - part[b], k = blk - part[b];
+    const uint32_t p0 = base[b] + k * RC_EB_PMAX, p1 = min(base[b + 1], p0 + RC_EB_PMAX);
+    for (uint32_t p = p0 + threadIdx.x; p < p1; p += RC_TB) {
+        const uint32_t loc = eslot[p];
+        const float2 v = eval[p];
+        atomicAdd(&s_acc[2 * loc], v.x);
+        atomicAdd(&s_acc[2 * loc + 1], v.y);
+    }
+    __syncthreads();
+    const int lv = b / nsl, sl = b - lv * nsl;
+    float* g = m.gtable + (((size_t)lv << m.log2T) + ((size_t)sl << sb)) * 2;
+    if (np == 1) {
+        float4* g4 = (float4*)g;
+        for (int j = threadIdx.x; j < S / 2; j += RC_TB) {
+            float4 x = g4[j];
+            const float4 a = s_eacc4[j];
+            x.x += a.x; x.y += a.y; x.z += a.z; x.w += a.w;
+            g4[j] = x;
+        }
+    } else {
+        for (int j = threadIdx.x; j < 2 * S; j += RC_TB) {
+            const float a = s_acc[j];
+            if (a != 0.f) unsafeAtomicAdd(&g[j], a);
+        }
+    }
+}
+
 
 // Adam (torch.optim.Adam's formula): m, v moments, bias corrections c1 = 1 - b1^t, c2 = 1 - b2^t;
 // the gradient is cleared for the next step
@@ -457,6 +591,25 @@ hipError_t rc_launch_mlp_infer(const rc_mlpdev* m, const float4* P, const float4
     return hipGetLastError();
 }
 
+// table gradient of n samples added to m->gtable, in rounds of RC_EB_CHUNK samples
+static hipError_t rc_mlp_table_grad(const rc_mlpdev* m, const rc_mlpbuf* b, const float4* P, int n, hipStream_t st) {
+    const int sb = std::min(RC_EB_SB, m->log2T), nsl = 1 << (m->log2T - sb), nbk = RC_MLP_LEVELS * nsl;
+    const size_t lds_bucket = 2 * (size_t)nsl * sizeof(uint32_t), lds_sum = ((size_t)2 << sb) * sizeof(float);
+    for (int s0 = 0; s0 < n; s0 += RC_EB_CHUNK) {
+        const int nc = std::min(RC_EB_CHUNK, n - s0);
+        const dim3 g((nc + RC_EB_SPT * RC_TB - 1) / (RC_EB_SPT * RC_TB), RC_MLP_LEVELS);
+        hipLaunchKernelGGL(k_eb_zero, dim3((nbk + RC_TB - 1) / RC_TB), dim3(RC_TB), 0, st, b->ecnt, nbk);
+        hipLaunchKernelGGL(k_eb_count, g, dim3(RC_TB), lds_bucket / 2, st, *m, P, (const float*)b->gX, s0, nc, b->ecnt);
+        hipLaunchKernelGGL(k_eb_scan, dim3(1), dim3(1024), 0, st, (const uint32_t*)b->ecnt, nbk, b->ebase, b->epart, b->ecur);
+        hipLaunchKernelGGL(k_eb_scatter, g, dim3(RC_TB), lds_bucket, st, *m, P, (const float*)b->gX, s0, nc, (const uint32_t*)b->ebase,
+                           b->ecur, b->eslot, (float2*)b->eval);
+        const long long maxparts = nbk + ((long long)nc * RC_EB_PAIRS + RC_EB_PMAX - 1) / RC_EB_PMAX;
+        hipLaunchKernelGGL(k_eb_sum, dim3((unsigned)maxparts), dim3(RC_TB), lds_sum, st, *m, nbk, (const uint32_t*)b->ebase,
+                           (const uint32_t*)b->epart, (const uint16_t*)b->eslot, (const float2*)b->eval);
+    }
+    return hipGetLastError();
+}
+
 // forward (train) + backward: gradients of sum_i dy_i y_i added to m->gW (overwritten) and m->gtable (accumulated)
 hipError_t rc_launch_mlp_grad(const rc_mlpdev* m, const rc_mlpbuf* b, const float4* P, const float4* N, const float4* D, const float* t,
                               const float* dy_in, int n, float* loss, hipStream_t st) {
@@ -472,8 +625,7 @@ hipError_t rc_launch_mlp_grad(const rc_mlpdev* m, const rc_mlpbuf* b, const floa
     hipLaunchKernelGGL(k_mlp_dw, dim3(nsplit, RC_MLP_HIDDEN + 1), dim3(MROWS * MWAVES), 0, st, (const __bf16*)actT, (const __bf16*)gzT, dy,
                        npad, chunk, b->part);
     hipLaunchKernelGGL(k_mlp_dw_sum, dim3((RC_MLP_NW + 63) / 64), dim3(RC_TB), 0, st, (const float*)b->part, nsplit, m->gW);
-    hipLaunchKernelGGL(k_mlp_enc_bwd, dim3(rc_grid((long long)n * RC_MLP_LEVELS)), dim3(RC_TB), 0, st, *m, P, n, (const float*)b->gX);
-    return hipGetLastError();
+    return rc_mlp_table_grad(m, b, P, n, st);
 }
 
 hipError_t rc_launch_mlp_adam(const rc_mlpdev* m, const rc_mlpbuf* b, float lr, float b1, float b2, float eps, int step, hipStream_t st) {

@@ -212,7 +212,17 @@ typedef struct rc_mlpbuf {
     float* mW; float* vW; float* mT; float* vT;    // Adam moments (weights, table)
     float* loss;                   // [1]
     int32_t cap, pad0;             // padded batch capacity
+    // table-gradient scatter by table slice (2^RC_EB_SB slots each): per-slice pair counts, cursors,
+    // starts (nbk + 1 each, nbk = 16 levels x slices), part starts, and the pairs of one round
+    // (local slot, the two feature gradients), RC_EB_PAIRS per sample of the round
+    uint32_t* ecnt; uint32_t* ecur; uint32_t* ebase; uint32_t* epart;
+    uint16_t* eslot; float* eval;
+    int64_t epairs;                // pair capacity of one round
 } rc_mlpbuf;
+// table slices of 4096 slots: a slice's 2 x 4096 float sums fit in 32 KB of LDS
+#define RC_EB_SB 12
+#define RC_EB_CHUNK (1 << 16)      // samples per round of the scatter
+#define RC_EB_PAIRS (8 * 16)       // at most 8 corners x 16 levels per sample
 
 // compact snapshot record, 24 bytes (the C ABI's rc_snap_rec)
 typedef struct rc_snaprec { uint32_t index; float n; uint64_t key; float radiance; float age; } rc_snaprec;
