@@ -1360,6 +1360,7 @@ extern "C" rc_status rc_mlp_create(const rc_mlp_config* c, rc_mlp** out) {
         b.epairs = (int64_t)std::min<size_t>(cap, RC_EB_CHUNK) * RC_EB_PAIRS;
         MK(dalloc(&b.ecnt, nbk + 1, m->owned)); MK(dalloc(&b.ecur, nbk + 1, m->owned));
         MK(dalloc(&b.ebase, nbk + 1, m->owned)); MK(dalloc(&b.epart, nbk + 1, m->owned));
+        MK(dalloc(&b.epmap, nbk + (size_t)(b.epairs + RC_EB_PMAX - 1) / RC_EB_PMAX, m->owned));
         MK(dalloc(&b.eslot, (size_t)b.epairs, m->owned)); MK(dalloc(&b.eval, 2 * (size_t)b.epairs, m->owned));
     }
     {

@@ -315,7 +315,6 @@ __global__ void __launch_bounds__(RC_TB) k_mlp_dw_sum(const float* part, int nsp
 // the same slot in a run of consecutive lanes add their values first (a segmented scan by shuffles)
 // and only the run's last lane emits the pair. Batches above RC_EB_CHUNK samples run in rounds.
 #define RC_EB_SPT 4
-#define RC_EB_PMAX 16384
 
 // a block's sweep over its samples of the round at level lv; emit(slot, v0, v1) for every pair
 template <class F>
@@ -372,8 +371,9 @@ __global__ void __launch_bounds__(RC_TB) k_eb_count(rc_mlpdev m, const float4* P
 }
 
 // exclusive scans of the slice counts (pair starts, base[nbk] = total) and of their part counts
-// (part[nbk] = total parts); clears the cursors of the scatter
-__global__ void __launch_bounds__(1024) k_eb_scan(const uint32_t* cnt, int nbk, uint32_t* base, uint32_t* part, uint32_t* cur) {
+// (part[nbk] = total parts), the slice of every part (pmap); clears the cursors of the scatter
+__global__ void __launch_bounds__(1024) k_eb_scan(const uint32_t* cnt, int nbk, uint32_t* base, uint32_t* part, uint32_t* cur,
+                                                  uint32_t* pmap) {
     __shared__ uint32_t s_a[16], s_p[16];
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const int per = (nbk + 1023) / 1024, j0 = min(nbk, t * per), j1 = min(nbk, j0 + per);
@@ -393,9 +393,10 @@ __global__ void __launch_bounds__(1024) k_eb_scan(const uint32_t* cnt, int nbk, 
     __syncthreads();
     uint32_t ea = s_a[w] + ia - a, ep = s_p[w] + ip - pc;
     for (int j = j0; j < j1; ++j) {
-        const uint32_t c = cnt[j];
+        const uint32_t c = cnt[j], np = (c + RC_EB_PMAX - 1) / RC_EB_PMAX;
         base[j] = ea; part[j] = ep; cur[j] = 0u;
-        ea += c; ep += (c + RC_EB_PMAX - 1) / RC_EB_PMAX;
+        for (uint32_t k = 0; k < np; ++k) pmap[ep + k] = (uint32_t)j;   // the slice of every part
+        ea += c; ep += np;
     }
     if (t == 1023) { base[nbk] = ea; part[nbk] = ep; }
 }
@@ -426,25 +427,16 @@ __global__ void __launch_bounds__(RC_TB) k_eb_scatter(rc_mlpdev m, const float4*
     });
 }
 
-__global__ void __launch_bounds__(RC_TB) k_eb_sum(rc_mlpdev m, int nbk, const uint32_t* base, const uint32_t* part, const uint16_t* eslot,
-                                                  const float2* eval) {
+__global__ void __launch_bounds__(RC_TB) k_eb_sum(rc_mlpdev m, int nbk, const uint32_t* base, const uint32_t* part, const uint32_t* pmap,
+                                                  const uint16_t* eslot, const float2* eval) {
     extern __shared__ float4 s_eacc4[];    // [2^sb] x 2 sums
-    __shared__ int s_b;
     float* s_acc = (float*)s_eacc4;
     const int sb = min(RC_EB_SB, m.log2T), nsl = 1 << (m.log2T - sb), S = 1 << sb;
     const uint32_t blk = blockIdx.x;
     if (blk >= part[nbk]) return;          // the grid is an upper bound on the parts
-    if (threadIdx.x == 0) {                // the slice: the last b with part[b] <= blk
-        int lo = 0, hi = nbk - 1;
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (part[mid] <= blk) lo = mid; else hi = mid - 1;
-        }
-        s_b = lo;
-    }
+    const int b = (int)pmap[blk];
     for (int j = threadIdx.x; j < 2 * S; j += RC_TB) s_acc[j] = 0.f;
     __syncthreads();
-    const int b = s_b;
     const uint32_t np = part[b + 1] - part[b], k = blk - part[b];
     const uint32_t p0 = base[b] + k * RC_EB_PMAX, p1 = min(base[b + 1], p0 + RC_EB_PMAX);
     for (uint32_t p = p0 + threadIdx.x; p < p1; p += RC_TB) {
@@ -600,12 +592,12 @@ static hipError_t rc_mlp_table_grad(const rc_mlpdev* m, const rc_mlpbuf* b, cons
         const dim3 g((nc + RC_EB_SPT * RC_TB - 1) / (RC_EB_SPT * RC_TB), RC_MLP_LEVELS);
         hipLaunchKernelGGL(k_eb_zero, dim3((nbk + RC_TB - 1) / RC_TB), dim3(RC_TB), 0, st, b->ecnt, nbk);
         hipLaunchKernelGGL(k_eb_count, g, dim3(RC_TB), lds_bucket / 2, st, *m, P, (const float*)b->gX, s0, nc, b->ecnt);
-        hipLaunchKernelGGL(k_eb_scan, dim3(1), dim3(1024), 0, st, (const uint32_t*)b->ecnt, nbk, b->ebase, b->epart, b->ecur);
+        hipLaunchKernelGGL(k_eb_scan, dim3(1), dim3(1024), 0, st, (const uint32_t*)b->ecnt, nbk, b->ebase, b->epart, b->ecur, b->epmap);
         hipLaunchKernelGGL(k_eb_scatter, g, dim3(RC_TB), lds_bucket, st, *m, P, (const float*)b->gX, s0, nc, (const uint32_t*)b->ebase,
                            b->ecur, b->eslot, (float2*)b->eval);
         const long long maxparts = nbk + ((long long)nc * RC_EB_PAIRS + RC_EB_PMAX - 1) / RC_EB_PMAX;
         hipLaunchKernelGGL(k_eb_sum, dim3((unsigned)maxparts), dim3(RC_TB), lds_sum, st, *m, nbk, (const uint32_t*)b->ebase,
-                           (const uint32_t*)b->epart, (const uint16_t*)b->eslot, (const float2*)b->eval);
+                           (const uint32_t*)b->epart, (const uint32_t*)b->epmap, (const uint16_t*)b->eslot, (const float2*)b->eval);
     }
     return hipGetLastError();
 }

@@ -213,9 +213,10 @@ typedef struct rc_mlpbuf {
     float* loss;                   // [1]
     int32_t cap, pad0;             // padded batch capacity
     // table-gradient scatter by table slice (2^RC_EB_SB slots each): per-slice pair counts, cursors,
-    // starts (nbk + 1 each, nbk = 16 levels x slices), part starts, and the pairs of one round
-    // (local slot, the two feature gradients), RC_EB_PAIRS per sample of the round
-    uint32_t* ecnt; uint32_t* ecur; uint32_t* ebase; uint32_t* epart;
+    // starts (nbk + 1 each, nbk = 16 levels x slices), part starts, the slice of every part (nbk +
+    // epairs / RC_EB_PMAX at most), and the pairs of one round (local slot, the two feature
+    // gradients), RC_EB_PAIRS per sample of the round
+    uint32_t* ecnt; uint32_t* ecur; uint32_t* ebase; uint32_t* epart; uint32_t* epmap;
     uint16_t* eslot; float* eval;
     int64_t epairs;                // pair capacity of one round
 } rc_mlpbuf;
@@ -223,6 +224,7 @@ typedef struct rc_mlpbuf {
 #define RC_EB_SB 12
 #define RC_EB_CHUNK (1 << 16)      // samples per round of the scatter
 #define RC_EB_PAIRS (8 * 16)       // at most 8 corners x 16 levels per sample
+#define RC_EB_PMAX 16384           // pairs summed by one block
 
 // compact snapshot record, 24 bytes (the C ABI's rc_snap_rec)
 typedef struct rc_snaprec { uint32_t index; float n; uint64_t key; float radiance; float age; } rc_snaprec;
