@@ -302,6 +302,14 @@ def kernel_counters(prof, prefix):
     return None
 
 
+def pmc_traffic(tag, kname, grid):
+    """HBM bytes per launch (copy-calibrated FETCH_SIZE + WRITE_SIZE) of kernel kname's dispatches of grid
+    threads in the committed counter summary of this workload, or None"""
+    prof = pmc_profile(tag)
+    e = ((prof or {}).get("kernels_by_grid", {}).get(kname) or {}).get(str(grid))
+    return e.get("traffic_bytes") if e else None
+
+
 VALU_PEAK = 256 * 2.4   # G wave-instructions/s: 256 CUs x one VALU wave-instruction per CU per cycle x 2.4 GHz
 
 
@@ -372,28 +380,34 @@ def summarize_pmc(dirs, tag, workload):
             for r in rows:
                 k = r.get("Kernel_Name", "")
                 ctr, val = r.get("Counter_Name"), float(r.get("Counter_Value", 0) or 0)
-                key = (os.path.basename(d.rstrip("/")), k, ctr, r.get("Dispatch_Id"))
+                key = (os.path.basename(d.rstrip("/")), k, ctr, r.get("Dispatch_Id"), r.get("Grid_Size"))
                 per.setdefault(key, 0.0)
                 per[key] += val   # counters of one dispatch may come in several rows (per dimension)
-    kernels = {}
-    for (d, k, ctr, disp), v in per.items():
+    kernels, by_grid = {}, {}
+    for (d, k, ctr, disp, grid), v in per.items():
         name = k.split("(")[0].replace("void ", "").strip()
         if d.startswith("cal_"):
             cal.setdefault(ctr, []).append(v)
             continue
         kernels.setdefault(name, {}).setdefault(ctr, []).append(v)
+        by_grid.setdefault(name, {}).setdefault(str(int(float(grid or 0))), {}).setdefault(ctr, []).append(v)
     copy_kib = 256 * 1024
     factor = {c: copy_kib / max(v) for c, v in cal.items()}   # the copy kernel: the largest dispatch of the calibration run
     out = {"workload": workload, "copy_bytes": copy_kib * 1024.0, "fetch_factor": factor.get("FETCH_SIZE"),
-           "write_factor": factor.get("WRITE_SIZE"), "kernels": {}}
-    for name, ctrs in kernels.items():
+           "write_factor": factor.get("WRITE_SIZE"), "kernels": {}, "kernels_by_grid": {}}
+
+    def entry(ctrs):
         e = {c: float(np.mean(v)) for c, v in ctrs.items()}
         e["dispatches"] = max(len(v) for v in ctrs.values())
         if "FETCH_SIZE" in e and "WRITE_SIZE" in e and factor.get("FETCH_SIZE") and factor.get("WRITE_SIZE"):
             e["fetch_bytes"] = e["FETCH_SIZE"] * 1024 * factor["FETCH_SIZE"]
             e["write_bytes"] = e["WRITE_SIZE"] * 1024 * factor["WRITE_SIZE"]
             e["traffic_bytes"] = e["fetch_bytes"] + e["write_bytes"]
-        out["kernels"][name] = e
+        return e
+    for name, ctrs in kernels.items():
+        out["kernels"][name] = entry(ctrs)
+        if len(by_grid.get(name, {})) > 1:   # a kernel launched at several sizes: per dispatch size too
+            out["kernels_by_grid"][name] = {g: entry(c) for g, c in by_grid[name].items()}
     path = os.path.join(ROOT, "profiles", f"pmc_{tag}.json")
     json.dump(out, open(path, "w"), indent=1, sort_keys=True)
     print(path)
@@ -626,7 +640,7 @@ def bench_mlp(args, rea, scene, cfg, log):
         judges.append(j)
     log(f"judge after warm-up: {judges[-1]:.3f} (first {judges[0]:.3f})")
     # timed frames
-    tr, rd, fr = [], [], []
+    tr, rd, fr, inf = [], [], [], []
     torch.cuda.synchronize()
     t_start = time.perf_counter()
     for f in range(args.steps):
@@ -637,7 +651,7 @@ def bench_mlp(args, rea, scene, cfg, log):
         _, j = mlp.train_scene(cache, n=B, seed=1000 + f)
         t2 = time.perf_counter()
         ms = mlp.render(cache, qimg, qcam, seed=2000 + f, timed=True)
-        fr.append((t1 - t0) * 1e3); tr.append((t2 - t1) * 1e3); rd.append(ms)
+        fr.append((t1 - t0) * 1e3); tr.append((t2 - t1) * 1e3); rd.append(ms); inf.append(mlp.last_infer_ms())
         judges.append(j)
     elapsed = time.perf_counter() - t_start
     # inference throughput of the MFMA network on one full batch (positions of random cache entries)
@@ -660,8 +674,11 @@ def bench_mlp(args, rea, scene, cfg, log):
         cpu = mlp_cpu_baseline(mlp, mlp.cfg, P.cpu().numpy(), Nn.cpu().numpy(), Nn.cpu().numpy(), min(args.cpu_seconds, 8.0), log)
     flop_per_query = 2 * (M.HIDDEN * 64 * 64 + 64 * 16)
     tflops = B * flop_per_query / (ms_inf * 1e-3) / 1e12
-    log(f"frame {np.mean(fr):.3f} ms, train {np.mean(tr):.3f} ms, render {np.mean(rd):.3f} ms, infer {ms_inf:.4f} ms "
-        f"({tflops:.1f} TFLOP/s), judge {np.mean(judges[-args.steps:]):.3f}")
+    nq = qcam.W * qcam.H * qcam.spp
+    ms_inf_r = float(np.mean(inf))
+    tflops_r = nq * flop_per_query / (ms_inf_r * 1e-3) / 1e12
+    log(f"frame {np.mean(fr):.3f} ms, train {np.mean(tr):.3f} ms, render {np.mean(rd):.3f} ms (its inference {ms_inf_r:.4f} ms, "
+        f"{tflops_r:.1f} TFLOP/s), infer {B} {ms_inf:.4f} ms ({tflops:.1f} TFLOP/s), judge {np.mean(judges[-args.steps:]):.3f}")
     qual = None
     if not args.no_quality:
         try:
@@ -722,10 +739,13 @@ def bench_mlp(args, rea, scene, cfg, log):
                                f"table, 5 x 64 ReLU, {B} training samples per frame, queried at {qcam.W}x{qcam.H}",
                    "config_index": args.config},
         "kernels": {"grid_frame_ms": float(np.mean(fr)), "mlp_train_ms": float(np.mean(tr)), "mlp_render_1080p_ms": float(np.mean(rd)),
-                    "mlp_infer_65536_ms": ms_inf, "step_ms": step_ms},
-        "roofline": {"bound": "mfma", "achieved": tflops, "peak": 2500.0, "unit": "TFLOP/s", "frac": tflops / 2500.0, "traffic": None,
-                     "kernel": "k_mlp_fwd (inference, 65536 queries)",
-                     "basis": f"{flop_per_query} FLOP per query (5 x 64x64 + 64x16 multiply-adds)"},
+                    "mlp_render_inference_ms": ms_inf_r, "mlp_infer_65536_ms": ms_inf, "mlp_infer_65536_tflops": tflops, "step_ms": step_ms},
+        "roofline": {"bound": "mfma", "achieved": tflops_r, "peak": 2500.0, "unit": "TFLOP/s", "frac": tflops_r / 2500.0,
+                     "traffic": pmc_traffic(f"street{cfg['W']}x{cfg['H']}_mlp", "k_mlp_fwd<false>", nq),
+                     "kernel": f"k_mlp_fwd (the batched inference of the {qcam.W}x{qcam.H} pass's {nq} recorded queries)",
+                     "basis": f"{flop_per_query} FLOP per query (5 x 64x64 + 64x16 multiply-adds) / the inference's time inside the "
+                              f"render (HIP events on its stream, rc_mlp_last_infer_ms); traffic = FETCH_SIZE + WRITE_SIZE per launch "
+                              f"of that dispatch size (copy-calibrated)"},
         "judge_relrms": {"after_warmup": round(float(judges[args.warmup + 199]), 4), "timed_mean": round(float(np.mean(judges[-args.steps:])), 4)},
         "cpu_baseline": cpu,
         "gpu_inference_over_cpu": round(B / (ms_inf * 1e-3) / 1e6 / cpu["value"], 1) if cpu else None,

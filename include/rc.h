@@ -308,6 +308,9 @@ rc_status rc_mlp_train_scene(rc_mlp* m, rc_cache* c, int n, uint32_t seed, float
  * with the cache's render mode RC_RENDER_FUSED one thread per path records its query point, then one
  * batched inference; RC_RENDER_STAGED runs the staged schedule (the same image); ms_out and stream as rc_cache_render */
 rc_status rc_mlp_render(rc_mlp* m, rc_cache* c, const rc_camera* cam, uint32_t seed, float* d_img, float* ms_out, void* stream);
+/* time of the batched inference inside the last rc_mlp_render called with ms_out (HIP events on the stream it ran
+ * on; 0 before). No counterpart in the reference: a measurement accessor of the learned-cache extension. */
+rc_status rc_mlp_last_infer_ms(const rc_mlp* m, float* ms_out);
 /* The grid cache as per-query judge and fallback of rc_mlp_render: a query whose prediction differs from the
  * cache's resolved radiance at its entry by more than thr times that value ends on the grid's value instead
  * (thr = 0: never, the default). No counterpart in the reference: the learned-cache extension's fallback. */

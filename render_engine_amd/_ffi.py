@@ -121,6 +121,7 @@ _SIGS = {
     "rc_mlp_train_scene": (C.c_int, [_P, _P, C.c_int, C.c_uint32, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
     "rc_mlp_render": (C.c_int, [_P, _P, C.POINTER(Camera), C.c_uint32, _P, C.POINTER(C.c_float), _P]),
     "rc_mlp_set_grid_fallback": (C.c_int, [_P, C.c_float]),
+    "rc_mlp_last_infer_ms": (C.c_int, [_P, C.POINTER(C.c_float)]),
     "rc_trace": (C.c_int, [_P, C.c_int, _P, _P, C.c_int, _P, _P, C.c_int, _P]),
     "rc_hash_insert": (C.c_int, [_P, _P, C.c_uint32, _P, _P, C.c_int, _P, _P]),
     "rc_hash_query": (C.c_int, [_P, _P, C.c_uint32, _P, C.c_int, _P, _P]),

@@ -1312,13 +1312,18 @@ struct rc_mlp {
     rc_mlpbuf b{};
     int step = 0;
     float fallback = 0.f;   // per-query grid fallback threshold of the camera pass (0: none)
+    hipEvent_t iev[2]{};    // around the batched inference of a timed rc_mlp_render
+    float infer_ms = 0.f;   // its time
     // online-training workspace (max_batch samples)
     float4 *sP = nullptr, *sN = nullptr, *sD = nullptr, *hP = nullptr, *hN = nullptr, *hD = nullptr;
     float2* AE = nullptr;
     float *hy = nullptr, *tg = nullptr, *jacc = nullptr;
     int* ent = nullptr;
     std::vector<void*> owned;
-    ~rc_mlp() { for (void* p : owned) (void)hipFree(p); }
+    ~rc_mlp() {
+        for (hipEvent_t e : iev) if (e) (void)hipEventDestroy(e);
+        for (void* p : owned) (void)hipFree(p);
+    }
 };
 
 extern "C" rc_status rc_mlp_create(const rc_mlp_config* c, rc_mlp** out) {
@@ -1554,10 +1559,13 @@ extern "C" rc_status rc_mlp_render(rc_mlp* m, rc_cache* c, const rc_camera* cam,
         HIPCK(rc_launch_resolve(&S->ds, &S->dst[si], c->c.Lc, c->c.R, c->dormant, st));
         c->r_valid = true;
     }
+    if (ms_out && !m->iev[0])
+        for (hipEvent_t& e : m->iev) HIPCK(hipEventCreate(&e));
+    const hipEvent_t* iev = ms_out ? m->iev : nullptr;
     if (c->render_mode == RC_RENDER_STAGED)
-        HIPCK(rc_launch_query_staged(&S->ds, &S->dst[si], nullptr, cam, seed, (uint32_t)c->f, c->dormant, d_img, nullptr, &c->qw, &md, st));
+        HIPCK(rc_launch_query_staged(&S->ds, &S->dst[si], nullptr, cam, seed, (uint32_t)c->f, c->dormant, d_img, nullptr, &c->qw, &md, st, iev));
     else
-        HIPCK(rc_launch_query_mlp(&S->ds, &dk, cam, seed, (uint32_t)c->f, c->dormant, d_img, &c->qw, &md, st));
+        HIPCK(rc_launch_query_mlp(&S->ds, &dk, cam, seed, (uint32_t)c->f, c->dormant, d_img, &c->qw, &md, st, iev));
     if (stream) {   // the caller's later work sees d_img
         HIPCK(hipEventRecord(c->ev_out, st));
         HIPCK(hipStreamWaitEvent((hipStream_t)stream, c->ev_out, 0));
@@ -1566,7 +1574,14 @@ extern "C" rc_status rc_mlp_render(rc_mlp* m, rc_cache* c, const rc_camera* cam,
         HIPCK(hipEventRecord(c->ev[1], st));
         HIPCK(hipEventSynchronize(c->ev[1]));
         *ms_out = ms_between(c->ev[0], c->ev[1]);
+        m->infer_ms = ms_between(m->iev[0], m->iev[1]);
     }
+    return RC_OK;
+}
+
+extern "C" rc_status rc_mlp_last_infer_ms(const rc_mlp* m, float* ms_out) {
+    if (!m || !ms_out) return fail(RC_ERR_ARG, "NULL argument");
+    *ms_out = m->infer_ms;
     return RC_OK;
 }
 

@@ -47,11 +47,12 @@ hipError_t rc_launch_dda_volume(const rc_dscene* s, int gx, int gy, int gz, cons
                                 uint32_t* vol, long long words_per_vol, hipStream_t stream);
 hipError_t rc_launch_vol_lights(int nl, const uint32_t* Va, const uint32_t* Vb, long long words_per_vol, const int* vox, int nvox,
                                 unsigned long long* out, hipStream_t stream);
+// infer_ev (may be NULL): two events recorded on the stream around the batched MLP inference
 hipError_t rc_launch_query_mlp(const rc_dscene* s, const rc_dstate* st, const rc_camera* cam, uint32_t seed, uint32_t frame, uint64_t dormant,
-                               float* img, const rc_qwork* w, const rc_mlpdev* mlp, hipStream_t stream);
+                               float* img, const rc_qwork* w, const rc_mlpdev* mlp, hipStream_t stream, const hipEvent_t* infer_ev = nullptr);
 hipError_t rc_launch_query_staged(const rc_dscene* s, const rc_dstate* st, const float* R, const rc_camera* cam, uint32_t seed,
                                   uint32_t frame, uint64_t dormant, float* img, unsigned long long* counters, const rc_qwork* w,
-                                  const rc_mlpdev* mlp, hipStream_t stream);
+                                  const rc_mlpdev* mlp, hipStream_t stream, const hipEvent_t* infer_ev = nullptr);
 hipError_t rc_launch_adj_unit(int trav, const rc_dscene* s, const rc_dstate* st, float* G, uint64_t* SG, hipStream_t stream);
 hipError_t rc_launch_adj_forward(int trav, const rc_dscene* s, const rc_dstate* st, const rc_adjbuf* w, const float* I,
                                  const float* alb, int K, int B, uint32_t seed, float* R, hipStream_t stream,

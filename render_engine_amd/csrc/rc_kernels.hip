@@ -1494,20 +1494,22 @@ hipError_t rc_launch_vol_lights(int nl, const uint32_t* Va, const uint32_t* Vb, 
 }
 
 hipError_t rc_launch_query_mlp(const rc_dscene* s, const rc_dstate* st, const rc_camera* cam, uint32_t seed, uint32_t frame, uint64_t dormant,
-                               float* img, const rc_qwork* w, const rc_mlpdev* mlp, hipStream_t stream) {
+                               float* img, const rc_qwork* w, const rc_mlpdev* mlp, hipStream_t stream, const hipEvent_t* infer_ev) {
     const int npath = cam->W * cam->H * cam->spp;
     if (npath <= 0) return hipSuccess;
     const dim3 g((unsigned)(rc_query_mlp_threads(cam) / RC_TB));
     RC_LAUNCH_WIDE((k_query_mlp<2, true>), (k_query_mlp<2, false>), false, g, st, s, stream, *s, *st, *cam, seed, frame, dormant, *w, npath);
+    if (infer_ev) (void)hipEventRecord(infer_ev[0], stream);
     hipError_t e = rc_launch_mlp_infer(mlp, (const float4*)w->mq0, (const float4*)w->mq1, (const float4*)w->mq2, npath, w->mpred, stream);
     if (e != hipSuccess) return e;
+    if (infer_ev) (void)hipEventRecord(infer_ev[1], stream);
     hipLaunchKernelGGL(k_q_mlp_add, dim3(rc_grid(npath)), dim3(RC_TB), 0, stream, *w, npath, mlp->jR, mlp->jthr);
     hipLaunchKernelGGL(k_q_pixels, dim3(rc_grid((long long)cam->W * cam->H)), dim3(RC_TB), 0, stream, *cam, *w, img);
     return hipGetLastError();
 }
 hipError_t rc_launch_query_staged(const rc_dscene* s, const rc_dstate* st, const float* R, const rc_camera* cam, uint32_t seed,
                                   uint32_t frame, uint64_t dormant, float* img, unsigned long long* counters, const rc_qwork* w,
-                                  const rc_mlpdev* mlp, hipStream_t stream) {
+                                  const rc_mlpdev* mlp, hipStream_t stream, const hipEvent_t* infer_ev) {
     const int npath = cam->W * cam->H * cam->spp;
     const size_t shm = rc_shm(*st, *s, 2, false);   // the stage kernels keep every stack level in LDS
     const int nblk = (npath + RC_QB - 1) / RC_QB;
@@ -1524,8 +1526,10 @@ hipError_t rc_launch_query_staged(const rc_dscene* s, const rc_dstate* st, const
         hipLaunchKernelGGL(k_q_trace<true>, tgrid, dim3(RC_TB), shm, stream, *s, *st, *w, (const int*)w->sidx, (const int*)w->seg + nblk, npath);
     }
     if (mlp) {
+        if (infer_ev) (void)hipEventRecord(infer_ev[0], stream);
         hipError_t e = rc_launch_mlp_infer(mlp, (const float4*)w->mq0, (const float4*)w->mq1, (const float4*)w->mq2, npath, w->mpred, stream);
         if (e != hipSuccess) return e;
+        if (infer_ev) (void)hipEventRecord(infer_ev[1], stream);
         hipLaunchKernelGGL(k_q_mlp_add, dim3(rc_grid(npath)), dim3(RC_TB), 0, stream, *w, npath, mlp->jR, mlp->jthr);
     }
     hipLaunchKernelGGL(k_q_pixels, dim3(rc_grid((long long)cam->W * cam->H)), dim3(RC_TB), 0, stream, *cam, *w, img);

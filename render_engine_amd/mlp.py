@@ -109,6 +109,12 @@ class MLPCache:
                                 C.c_void_p(torch.cuda.current_stream().cuda_stream)))
         return ms.value if timed else None
 
+    def last_infer_ms(self):
+        """time of the batched inference inside the last render(..., timed=True) (HIP events on its stream)"""
+        ms = C.c_float(0.0)
+        check(lib.rc_mlp_last_infer_ms(self._h, C.byref(ms)))
+        return ms.value
+
     def get(self, which):
         nb = lib.rc_mlp_array_bytes(self._h, which)
         out = np.empty(nb // 4, np.float32)
