@@ -867,7 +867,7 @@ def main():
     upd_bytes = update_bytes_per_entry(info.nl) * info.Np
     kernels = {
         "k_trace2": {"ms": t_tr, "rays": n_entry / args.steps, "mrays_s": n_entry / args.steps / t_tr / 1e3},
-        "k_update": {"ms": t_up, "entries": info.Np, "alg_bytes": upd_bytes, "GB_s": upd_bytes / t_up / 1e6,
+        "k_update": {"ms": t_up, "entries": info.Np, "mpts_s": info.Np / t_up / 1e3, "alg_bytes": upd_bytes, "GB_s": upd_bytes / t_up / 1e6,
                      "hbm_frac": upd_bytes / t_up / 1e6 / 8000.0},
         "k_query": {"ms": t_q, "rays": (n_ext + n_sh) / args.steps, "mrays_s": (n_ext + n_sh) / args.steps / t_q / 1e3},
     }
