@@ -120,6 +120,30 @@ def test_mlp_table_gradient_rounds_and_accumulation(torch_gpu):
     assert np.all(mlp.get(M.MLP_GTABLE) == 0)
 
 
+@pytest.mark.parametrize("log2T,n", [(10, 1), (10, 37), (11, 300)])
+def test_mlp_table_gradient_small_tables(torch_gpu, log2T, n):
+    """tables smaller than one 4,096-slot slice (one slice of 2^log2T slots per level) and batches of
+    a few samples: the table gradient matches the oracle's"""
+    torch = torch_gpu
+    lo, ext = np.array([-1.0, 0.0, 2.0], np.float32), np.array([3.0, 5.0, 2.0], np.float32)
+    cfg = M.config(lo, ext, log2_table=log2T, base_res=4, max_res=128, max_batch=512, seed=6)
+    mlp = M.MLPCache(cfg)
+    rng = np.random.default_rng(20 + n)
+    mlp.set(M.MLP_TABLE, rng.uniform(-1, 1, M.LEVELS * 2 * (1 << log2T)).astype(np.float32))
+    P, N, D, (Pt, Nt, Dt) = _inputs(torch, n, 30 + n, lo, ext)
+    dy = rng.normal(size=n).astype(np.float32)
+    mlp.zero_grad()
+    mlp.backward(Pt, Nt, Dt, torch.from_numpy(dy).cuda())
+    gT = mlp.get(M.MLP_GTABLE)
+    W = mlp.get(M.MLP_W)
+    X, corners = encode_ref(cfg, mlp.get(M.MLP_TABLE), P, N, D)
+    _, acts = forward_ref(W, X)
+    _, gTr = backward_ref(cfg, W, acts, corners, dy)
+    assert np.count_nonzero(gTr) > 0
+    assert _rel(gT, gTr) < 2e-2, _rel(gT, gTr)
+    mlp.zero_grad()
+
+
 def test_mlp_adam_step(torch_gpu, small):
     torch = torch_gpu
     cfg = M.config(small[0].lo, small[0].ext, log2_table=12, base_res=4, max_res=256, max_batch=4096, seed=9, lr=1e-2)

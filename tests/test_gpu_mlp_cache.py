@@ -45,9 +45,10 @@ def test_mlp_render_constant_matches_grid(torch_gpu, street):
             a = torch.zeros(cam.W * cam.H, dtype=torch.float32, device="cuda")
             b = torch.zeros_like(a)
             cache.render(a, cam, seed=21)
-            mlp.render(cache, b, cam, seed=21)
+            ms = mlp.render(cache, b, cam, seed=21, timed=True)
             ga, gb = a.cpu().numpy(), b.cpu().numpy()
             assert np.array_equal(ga.view(np.uint32), gb.view(np.uint32)), f"state {st} mode {mode}: {np.sum(ga != gb)} pixels differ"
+            assert 0.0 < mlp.last_infer_ms() < ms   # the batched inference, timed inside the timed pass
 
 
 def test_mlp_render_grid_fallback_per_query(torch_gpu, street):
