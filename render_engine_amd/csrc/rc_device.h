@@ -351,9 +351,11 @@ struct rc_lstk_s {
 // queued (or one queue could overflow), so the triangle loop runs with most lanes busy
 // instead of after each node step for the few lanes that entered a leaf. Culling uses the
 // current tmin, which may lag the queued hits: more node steps at worst, never a missed hit,
-// and the hit itself does not depend on the order. A leaf longer than 127 triangles (only at
-// the builder's depth limit) is walked at once. The stack depth is bounded by the scene's
-// wstack levels (3 per wide level); the levels beyond lstk.cap live in global memory.
+// and the hit itself does not depend on the order. Leaf children hold at most RC_WLEAF_MAX
+// triangles (the wide collapse splits longer binary leaves), so a node step only queues them: no
+// triangle loop inside the node step, whose live values would raise the register pressure of the
+// whole traversal. The stack depth is bounded by the scene's wstack levels (3 per wide level); the
+// levels beyond lstk.cap live in global memory.
 // QN: the node steps read the quantized nodes (rc_qnode) instead of the fp32 ones.
 template <bool QN = false, class LS>
 __device__ __forceinline__ void rc_trace_wide_closest(f3 o, f3 d, const rc_dscene& s, float& tmin, int& q, const LS& lstk) {
@@ -387,18 +389,9 @@ __device__ __forceinline__ void rc_trace_wide_closest(f3 o, f3 d, const rc_dscen
         if (QN) rc_child_enter_q(r, s, node, tmin, ch, t0, t1, t2, t3, e0, e1, e2, e3);
         else rc_child_enter(r, s, node, tmin, ch, t0, t1, t2, t3, e0, e1, e2, e3);
         int i0 = ch.x, i1 = ch.y, i2 = ch.z, i3 = ch.w;
-#define RC_QUEUE(e, cw)                                                                         \
-        if (e != 0) {                                                                           \
-            if (e & 127) { que[nq * 256] = (uint32_t)e & 0x7fffffffu; ++nq; }                   \
-            else {   /* more than 127 triangles: their end in row 7 of the fp32 node */         \
-                const int a_ = (int)(((uint32_t)e >> 7) & 0xffffffu), b_ = rc_wrow_i(s, ((uint32_t)cur << 7) + 112u).cw; \
-                for (int j = a_; j < b_; ++j) {                                                 \
-                    const float t = rc_tri_static4(r.o, r.d, (const float4*)s.ST4 + 3 * (int64_t)j); \
-                    if (rc_closer(t, j, tmin, q)) { tmin = t; q = -2 - j; }                     \
-                }                                                                               \
-            }                                                                                   \
-        }
-        RC_QUEUE(e0, x) RC_QUEUE(e1, y) RC_QUEUE(e2, z) RC_QUEUE(e3, w)
+#define RC_QUEUE(e) \
+        if (e != 0) { que[nq * 256] = (uint32_t)e & 0x7fffffffu; ++nq; }
+        RC_QUEUE(e0) RC_QUEUE(e1) RC_QUEUE(e2) RC_QUEUE(e3)
 #undef RC_QUEUE
         // sorting network on (t, node): ascending entry distance, misses (1e30) last
         rc_cswap(t0, i0, t1, i1); rc_cswap(t2, i2, t3, i3);
@@ -463,20 +456,10 @@ __device__ __forceinline__ bool rc_occluded_wide(f3 o, f3 d, const rc_dscene& s,
         if (QN) rc_child_enter_q(r, s, node, tlim, ch, t0, t1, t2, t3, e0, e1, e2, e3);
         else rc_child_enter(r, s, node, tlim, ch, t0, t1, t2, t3, e0, e1, e2, e3);
         int i0 = ch.x, i1 = ch.y, i2 = ch.z, i3 = ch.w;
-#define RC_QUEUE(e, cw)                                                                         \
-        if (e != 0) {                                                                           \
-            if (e & 127) { que[nq * 256] = (uint32_t)e & 0x7fffffffu; ++nq; }                   \
-            else {   /* more than 127 triangles: their end in row 7 of the fp32 node */         \
-                const int a_ = (int)(((uint32_t)e >> 7) & 0xffffffu), b_ = rc_wrow_i(s, ((uint32_t)cur << 7) + 112u).cw; \
-                for (int j = a_; j < b_ && !blocked; ++j) {                                     \
-                    const float t = rc_tri_static4(r.o, r.d, (const float4*)s.ST4 + 3 * (int64_t)j); \
-                    if (t > 1e-4f && t < tlim) blocked = true;                                  \
-                }                                                                               \
-            }                                                                                   \
-        }
-        RC_QUEUE(e0, x) RC_QUEUE(e1, y) RC_QUEUE(e2, z) RC_QUEUE(e3, w)
+#define RC_QUEUE(e) \
+        if (e != 0) { que[nq * 256] = (uint32_t)e & 0x7fffffffu; ++nq; }
+        RC_QUEUE(e0) RC_QUEUE(e1) RC_QUEUE(e2) RC_QUEUE(e3)
 #undef RC_QUEUE
-        if (blocked) { cur = -1; nq = 0; continue; }
         rc_cswap(t0, i0, t1, i1); rc_cswap(t2, i2, t3, i3);
         rc_cswap(t0, i0, t2, i2); rc_cswap(t1, i1, t3, i3);
         rc_cswap(t1, i1, t2, i2);

@@ -876,10 +876,11 @@ __host__ __device__ inline void rc_query_tile_of(const rc_qtile& t, int w, int& 
 // Camera pass, one thread per path (spp divides 64: the spp paths of a pixel sit in adjacent
 // lanes and are summed in sample order by the pixel's first lane; waves take pixel tiles), or
 // one thread per pixel looping over its samples (any spp). Both sum ((r0 + r1) + r2) + ... and
-// divide by spp. Five waves per SIMD (96 VGPRs, 4 registers spilled to scratch) measured 9 %
-// faster than four waves without spills: the extra wave hides more of the traversal's latency.
+// divide by spp. Six waves per SIMD (80 VGPRs, 1 register spilled to scratch): the traversal is
+// latency bound as much as issue bound, and more waves hide more of its latency (1920x1080: 2.85 ms
+// at five waves without spills, 2.70 ms at six).
 template <int TRAV, bool PER_PATH, bool SPLIT>
-__global__ void __launch_bounds__(RC_TB) __attribute__((amdgpu_waves_per_eu(5))) k_query(rc_dscene s, rc_dstate st, const float* R, rc_camera cam, uint32_t seed,
+__global__ void __launch_bounds__(RC_TB) __attribute__((amdgpu_waves_per_eu(6))) k_query(rc_dscene s, rc_dstate st, const float* R, rc_camera cam, uint32_t seed,
                                                 uint32_t frame, uint64_t dormant, float* img, unsigned long long* counters) {
     extern __shared__ float sh[];
     int* shq = (int*)(sh + 9 * st.nt);

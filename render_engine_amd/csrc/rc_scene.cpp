@@ -154,28 +154,38 @@ float quant_step_floor(const std::vector<rc_bvh_node>& bn, const float lo[3], co
 void build_wide(const std::vector<rc_bvh_node>& bn, std::vector<rc_wnode>& wn, int& depth_out) {
     wn.clear();
     depth_out = 0;
-    struct Job { int bnode, wslot, depth; };
+    // a job is a binary node (first < 0) or a range of more than RC_WLEAF_MAX triangles of the binary leaf
+    // bnode (first .. first + count - 1), split over wide children with the leaf's box
+    struct Job { int bnode, first, count, wslot, depth; };
+    struct Kid { int bnode, first, count; };   // first < 0: binary node bnode
     std::vector<Job> jobs;
     wn.push_back(rc_wnode{});
-    jobs.push_back({0, 0, 1});
+    jobs.push_back({0, -1, 0, 0, 1});
     while (!jobs.empty()) {
         Job jb = jobs.back(); jobs.pop_back();
         depth_out = std::max(depth_out, jb.depth);
-        std::vector<int> kids;
-        if (bn[jb.bnode].cnt > 0) kids.push_back(jb.bnode);
-        else { kids.push_back(bn[jb.bnode].lf); kids.push_back(bn[jb.bnode].lf + 1); }
-        while (kids.size() < 4) {   // open the inner child with the largest surface area
-            int best = -1; float ba = -1.f;
-            for (size_t k = 0; k < kids.size(); ++k) {
-                const rc_bvh_node& c = bn[kids[k]];
-                if (c.cnt > 0) continue;
-                float a = rc_bvh_area(c.lo, c.hi);
-                if (a > ba) { ba = a; best = (int)k; }
+        std::vector<Kid> kids;
+        if (jb.first >= 0) {   // a long leaf's range: four near-equal chunks
+            const int q = (jb.count + 3) / 4;
+            for (int a = jb.first; a < jb.first + jb.count; a += q) kids.push_back({jb.bnode, a, std::min(q, jb.first + jb.count - a)});
+        } else if (bn[jb.bnode].cnt > 0) {
+            kids.push_back({jb.bnode, -1, 0});
+        } else {
+            kids.push_back({bn[jb.bnode].lf, -1, 0});
+            kids.push_back({bn[jb.bnode].lf + 1, -1, 0});
+            while (kids.size() < 4) {   // open the inner child with the largest surface area
+                int best = -1; float ba = -1.f;
+                for (size_t k = 0; k < kids.size(); ++k) {
+                    const rc_bvh_node& c = bn[kids[k].bnode];
+                    if (c.cnt > 0) continue;
+                    float a = rc_bvh_area(c.lo, c.hi);
+                    if (a > ba) { ba = a; best = (int)k; }
+                }
+                if (best < 0) break;
+                int b = kids[best].bnode;
+                kids[best] = {bn[b].lf, -1, 0};
+                kids.insert(kids.begin() + best + 1, Kid{bn[b].lf + 1, -1, 0});
             }
-            if (best < 0) break;
-            int b = kids[best];
-            kids[best] = bn[b].lf;
-            kids.insert(kids.begin() + best + 1, bn[b].lf + 1);
         }
         rc_wnode w;
         for (int c = 0; c < 4; ++c) {
@@ -185,17 +195,19 @@ void build_wide(const std::vector<rc_bvh_node>& bn, std::vector<rc_wnode>& wn, i
         }
         std::vector<Job> sub;
         for (size_t k = 0; k < kids.size(); ++k) {
-            const rc_bvh_node& c = bn[kids[k]];
+            const rc_bvh_node& c = bn[kids[k].bnode];
             w.lox[k] = c.lo[0]; w.loy[k] = c.lo[1]; w.loz[k] = c.lo[2];
             w.hix[k] = c.hi[0]; w.hiy[k] = c.hi[1]; w.hiz[k] = c.hi[2];
-            if (c.cnt > 0) {   // leaf: the traversal's queue entry with the top bit set; a count above 127 reads as 0
-                w.child[k] = (int32_t)(0x80000000u | (uint32_t)c.lf << 7 | (uint32_t)(c.cnt <= 127 ? c.cnt : 0));
-                w.cend[k] = c.lf + c.cnt;
-            } else {
+            const bool leaf = kids[k].first >= 0 || c.cnt > 0;
+            const int first = kids[k].first >= 0 ? kids[k].first : c.lf, count = kids[k].first >= 0 ? kids[k].count : c.cnt;
+            if (leaf && count <= RC_WLEAF_MAX) {   // the traversal's queue entry with the top bit set
+                w.child[k] = (int32_t)(0x80000000u | (uint32_t)first << 7 | (uint32_t)count);
+                w.cend[k] = first + count;
+            } else {   // an inner node, or a longer leaf split over a node of its own
                 int slot = (int)wn.size();
                 wn.push_back(rc_wnode{});
                 w.child[k] = slot; w.cend[k] = 0;
-                sub.push_back({kids[k], slot, jb.depth + 1});
+                sub.push_back(leaf ? Job{kids[k].bnode, first, count, slot, jb.depth + 1} : Job{kids[k].bnode, -1, 0, slot, jb.depth + 1});
             }
         }
         wn[jb.wslot] = w;

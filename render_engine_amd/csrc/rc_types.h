@@ -26,10 +26,12 @@ typedef struct rc_bvh_node {
 
 // 4-wide BVH node (128 B = two 64-byte lines): child boxes as fp32 SoA so the four slab
 // tests of a node read whole float4s. child[c] < 0: leaf, child[c] = 1 << 31 | first << 7 |
-// count (triangles first .. first + count - 1, same order and indices as the binary BVH; count
-// 0 when above 127, the end then in cend[c]; cend[c] = first + count for every leaf); child[c] >
-// 0: inner node child[c] (cend 0); child[c] == 0: empty slot (box lo = +inf, hi = -inf: never
-// entered). The traversal reads rows 0-6 only (row 7 for leaves of more than 127 triangles).
+// count (triangles first .. first + count - 1, same order and indices as the binary BVH, count
+// 1..RC_WLEAF_MAX: a longer binary leaf is split over a wide node of its own whose children are
+// chunks of its range with its box; cend[c] = first + count); child[c] > 0: inner node child[c]
+// (cend 0); child[c] == 0: empty slot (box lo = +inf, hi = -inf: never entered). The traversal
+// reads rows 0-6 only.
+#define RC_WLEAF_MAX 127
 typedef struct rc_wnode {
     float lox[4], loy[4], loz[4], hix[4];   // 64 B
     float hiy[4], hiz[4];                   // 32 B
@@ -43,8 +45,7 @@ typedef struct rc_wnode {
 // least one step to spare on every side, and the step is at least 2^-18 of the scene's largest
 // coordinate, so the slab test on it is conservative with respect to the fp32 test of the exact
 // box despite both tests' rounding (the closest hit is never culled: same hits). Empty slots
-// hold q lo = 255, q hi = 0 on every axis (never entered). child[] as in rc_wnode; leaves of more
-// than 127 triangles read their end from WN.
+// hold q lo = 255, q hi = 0 on every axis (never entered). child[] as in rc_wnode.
 typedef struct rc_qnode {
     float ox, oy, oz, sx;                   // row 0
     float sy, sz; uint32_t qlox, qloy;      // row 1

@@ -24,6 +24,29 @@ def street(torch_gpu):
     return s, po.OracleScene(s)
 
 
+_CAMERAS = {}
+
+
+@pytest.fixture(scope="module")
+def room_longleaf(torch_gpu, tmp_path_factory):
+    """the room loaded from a file whose binary BVH root is one leaf holding every static triangle: the
+    wide collapse splits it into nodes of chunks of at most 127 triangles"""
+    path = str(tmp_path_factory.mktemp("longleaf") / "room.bin")
+    room = rea.Scene.box_room(seed=1, target_tris=2000)
+    room.save(path)
+    _CAMERAS["room_longleaf"] = room.camera()   # scene files hold no camera
+    b = np.fromfile(path, np.uint8)
+    m = b[60:76].view(np.int32)   # header int[8], lo[3], ext[3], h, then nst, nnodes, table size, entries
+    nst = int(m[0])
+    bn = 76 + 36 * nst + 16 * nst
+    node = b[bn:bn + 32].view(np.int32)
+    node[3], node[7] = 0, nst
+    b.tofile(path)
+    s = rea.Scene.load(path).upload(trav=rea.TRAV_WIDE)
+    assert s.info.nst == nst > 127
+    return s, po.OracleScene(s)
+
+
 def _rays(scene, n, seed, st=0):
     rng = np.random.default_rng(seed)
     i = scene.info
@@ -49,7 +72,7 @@ def _gpu_trace(torch, scene, O, D, trav, st=0):
     return t.cpu().numpy(), q.cpu().numpy()
 
 
-@pytest.mark.parametrize("which", ["room", "street"])
+@pytest.mark.parametrize("which", ["room", "street", "room_longleaf"])
 def test_trace_bit_exact(torch_gpu, which, request):
     scene, osc = request.getfixturevalue(which)
     for st in range(scene.info.NS):
@@ -239,9 +262,12 @@ def test_runner_bit_exact(torch_gpu, which, name, mk, request):
             assert np.array_equal(g, e), f"{name} frame {f} state {st}: {arr} differs in {np.sum(g != e)} entries"
 
 
-def test_query_bit_exact(torch_gpu, street):
+@pytest.mark.parametrize("which", ["street", "room_longleaf"])
+def test_query_bit_exact(torch_gpu, which, request):
+    """frames of the plain cache, then the camera pass with and without the cache, bit for bit (on the room
+    whose BVH is one long leaf too: the entry pass's and camera pass's walks over the split wide nodes)"""
     torch = torch_gpu
-    scene, osc = street
+    scene, osc = request.getfixturevalue(which)
     m = rea.methods.plain()
     cache = rea.Cache(scene, m)
     orun = po.OracleRunner(osc, m)
@@ -250,7 +276,9 @@ def test_query_bit_exact(torch_gpu, street):
     for _ in range(3):
         cache.frame(0)
         orun.frame(0)
-    cam = scene.camera()
+        for name in ("LC", "N", "KK"):
+            assert np.array_equal(cache.get(name), orun.get(name)), f"cache array {name} differs"
+    cam = _CAMERAS.get(which) or scene.camera()
     img = torch.zeros(cam.W * cam.H, dtype=torch.float32, device="cuda")
     for no_cache in (0, 1):
         rays = cache.render(img, cam, no_cache=bool(no_cache), seed=4, count_rays=True)

@@ -260,8 +260,8 @@ def test_bvh_structure(street):
                 assert np.all(lo[c] >= lo[k]) and np.all(hi[c] <= hi[k])
     assert np.all(seen == 1)
     # 4-wide nodes: every triangle in exactly one leaf range (leaf child = 1 << 31 | first << 7 |
-    # count, count 0 above 127, end in the last row), inner children reached once, empty slots
-    # (child 0) carry the never-entered box (lo = +inf, hi = -inf)
+    # count, count 1..127, end in the last row), inner children reached once, empty slots (child 0)
+    # carry the never-entered box (lo = +inf, hi = -inf)
     wn = scene.array("WN").reshape(-1, 32)
     wf = wn.view(np.float32)
     ch, ce = wn[:, 24:28].view(np.uint32), wn[:, 28:32].view(np.int32)
@@ -274,7 +274,7 @@ def test_bvh_structure(street):
             e = int(ch[k, c])
             if e >> 31:
                 first, count = (e >> 7) & 0xFFFFFF, e & 127
-                assert ce[k, c] > first and (count == ce[k, c] - first if count else ce[k, c] - first > 127)
+                assert 1 <= count <= 127 and ce[k, c] == first + count
                 wseen[first:ce[k, c]] += 1
                 v0 = ST[first:ce[k, c], 0:3]
                 assert np.all(v0 >= wlo[k, :, c] - 1e-5) and np.all(v0 <= whi[k, :, c] + 1e-5)

@@ -121,3 +121,43 @@ def test_bad_mesh_files(tmp_path):
     open(p, "wb").write(b"MS")
     with pytest.raises(rea.RcError):
         rea.Scene.mesh(p, 1000)
+
+
+def _long_leaf_file(tmp_path, b):
+    """the room file with its binary BVH's root turned into one leaf holding every static triangle"""
+    L = _layout(b)
+    c = b.copy()
+    node = c[L["bn"]:L["bn"] + 32].view(np.int32)
+    node[3], node[7] = 0, L["nst"]   # lf, cnt
+    p = str(tmp_path / "longleaf.bin")
+    c.tofile(p)
+    return p, L["nst"]
+
+
+def test_long_leaf_split_over_wide_nodes(room_file, tmp_path):
+    """a binary leaf of more than 127 triangles becomes a wide node of its own (recursively) whose
+    leaf children are chunks of at most 127 of its triangles, each triangle in exactly one chunk,
+    every chunk with the leaf's box"""
+    path, b = room_file
+    p, nst = _long_leaf_file(tmp_path, b)
+    assert nst > 4 * 127
+    s = rea.Scene.load(p)
+    wn = s.array("WN").reshape(-1, 32)
+    wf = wn.view(np.float32)
+    ch, ce = wn[:, 24:28].view(np.uint32), wn[:, 28:32].view(np.int32)
+    root_lo = b[_layout(b)["bn"]:_layout(b)["bn"] + 12].view(np.float32)
+    seen = np.zeros(nst, np.int32)
+    reached = np.zeros(len(wn), np.int32)
+    for k in range(len(wn)):
+        for c in range(4):
+            e = int(ch[k, c])
+            if e >> 31:
+                first, count = (e >> 7) & 0xFFFFFF, e & 127
+                assert 1 <= count <= 127 and ce[k, c] == first + count
+                assert np.array_equal(wf[k, [c, 4 + c, 8 + c]], root_lo)
+                seen[first:first + count] += 1
+            elif e != 0:
+                reached[e] += 1
+    assert np.all(seen == 1)
+    assert reached[0] == 0 and np.all(reached[1:] == 1)
+    assert s.info.wide_depth >= 3
