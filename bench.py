@@ -310,7 +310,12 @@ def pmc_traffic(tag, kname, grid):
     return e.get("traffic_bytes") if e else None
 
 
-VALU_PEAK = 256 * 2.4   # G wave-instructions/s: 256 CUs x one VALU wave-instruction per CU per cycle x 2.4 GHz
+# G wave-instructions/s: the published FP32 vector rate (157.3 TFLOP/s = 256 CUs x 128 FMA lanes per cycle x
+# 2.4 GHz) is two wave64 VALU instructions per CU per cycle; render_engine_amd/valu_probe (independent FP32
+# multiply / add / max chains) issues 1125 G/s of them on the part (profiles/valu_issue_ceiling.md)
+VALU_PEAK = 256 * 2 * 2.4
+VALU_BASIS = ("peak = 256 CUs x 2 VALU wave-instructions per cycle x 2.4 GHz (the published FP32 vector rate, 128 FMA "
+              "lanes per CU and cycle; the VALU probe reaches 0.92 of it)")
 
 
 def bench_events(args, rea, scene, log):
@@ -569,8 +574,8 @@ def bench_adjoint(args, rea, scene, cfg, log):
             "hbm_frac": traffic / k_fwd / 1e6 / 8000.0 if traffic else None,
             "touched_bytes": fb, "touched_frac_of_hbm_peak": fb / k_fwd / 1e6 / 8000.0 if fb else None,
             "counters": f"profiles/pmc_street{scene.camera().W}x{scene.camera().H}_adjoint.json" if prof else None,
-            "basis": "SQ_INSTS_VALU per launch (rocprofv3, this workload) / in-run launch time; peak = 256 CUs x 1 VALU "
-                     "wave-instruction per cycle x 2.4 GHz. traffic = FETCH_SIZE + WRITE_SIZE per launch (copy-calibrated); "
+            "basis": "SQ_INSTS_VALU per launch (rocprofv3, this workload) / in-run launch time; " + VALU_BASIS + ". traffic = "
+                     "FETCH_SIZE + WRITE_SIZE per launch (copy-calibrated); "
                      "touched bytes = per entry 36 B (P, N, L and S out) + per ray 32 B/binary node + 36 B/triangle + 8 B/hash "
                      "probe + per hit 16 B (A, E, SG; 28 B with the previous bounce's L, S), oracle-counted, mean over the bounces",
             "work": fwork if fb else None}
@@ -923,8 +928,8 @@ def main():
             "hbm_frac": traffic / t_q / 1e6 / 8000.0 if traffic else None,
             "touched_bytes": qb, "touched_frac_of_hbm_peak": qb / t_q / 1e6 / 8000.0 if qb else None,
             "counters": f"profiles/pmc_{tag}.json" if prof else None,
-            "basis": "SQ_INSTS_VALU per launch (rocprofv3, this workload) / in-run launch time; peak = 256 CUs x 1 VALU "
-                     "wave-instruction per cycle x 2.4 GHz. traffic = FETCH_SIZE + WRITE_SIZE per launch (copy-calibrated); "
+            "basis": "SQ_INSTS_VALU per launch (rocprofv3, this workload) / in-run launch time; " + VALU_BASIS + ". traffic = "
+                     "FETCH_SIZE + WRITE_SIZE per launch (copy-calibrated); "
                      "touched bytes = 32 B/binary node + 36 B/triangle + 8 B/hash probe + 4 B/resolved read, oracle-counted"}
     qual = None
     if not args.no_quality:
