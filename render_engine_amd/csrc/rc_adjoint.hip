@@ -28,7 +28,8 @@ template <int TRAV, bool SPLIT>
 __global__ void __launch_bounds__(RC_TB) k_adj_unit(rc_dscene s, rc_dstate st, float* G, uint64_t* SG) {
     extern __shared__ float sh[];
     int* shq = (int*)(sh + 9 * st.nt);
-    const auto lstk = rc_lstk_sel<SPLIT>::make(sh, st, s);
+    auto lstk = rc_lstk_sel<SPLIT>::make(sh, st, s);
+    lstk.qwalk = RC_WQ_L - 4;   // launched with RC_WQ_L queue levels (RC_ADJ_T)
     rc_load_dyn(st, sh, shq);
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= s.Np) return;
@@ -181,7 +182,8 @@ __global__ void __launch_bounds__(RC_TB) __attribute__((amdgpu_waves_per_eu(5)))
                                                   const float* Lin, const uint64_t* Sin, float* Lout, uint64_t* Sout, uint32_t seed, int K) {
     extern __shared__ float sh[];
     int* shq = (int*)(sh + 9 * st.nt);
-    const auto lstk = rc_lstk_sel<SPLIT>::make(sh, st, s);
+    auto lstk = rc_lstk_sel<SPLIT>::make(sh, st, s);
+    lstk.qwalk = RC_WQ_L - 4;   // launched with RC_WQ_L queue levels (RC_ADJ_T)
     rc_load_dyn(st, sh, shq);
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= s.Np) return;
@@ -230,7 +232,8 @@ __global__ void __launch_bounds__(RC_TB) __attribute__((amdgpu_waves_per_eu(5)))
     extern __shared__ float sh[];
     __shared__ rc_adj_rays R;
     int* shq = (int*)(sh + 9 * st.nt);
-    const auto lstk = rc_lstk_sel<SPLIT>::make(sh, st, s);
+    auto lstk = rc_lstk_sel<SPLIT>::make(sh, st, s);
+    lstk.qwalk = RC_WQ_L - 4;   // launched with RC_WQ_L queue levels (RC_ADJ_T)
     rc_load_dyn(st, sh, shq);
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     // an entry of zero weight contributes nothing: its rays are not traced
@@ -338,7 +341,7 @@ __global__ void __launch_bounds__(RC_TB) k_adj_dalb(int nmat, const int* moff, c
     do {                                                                                            \
         if (trav == 0) hipLaunchKernelGGL((KER<0, false>), g, b, shm, stream, __VA_ARGS__);        \
         else if (trav == 1) hipLaunchKernelGGL((KER<1, false>), g, b, shm, stream, __VA_ARGS__);   \
-        else RC_LAUNCH_WIDE((KER<2, true>), (KER<2, false>), false, g, st, s, stream, __VA_ARGS__);       \
+        else RC_LAUNCH_WIDE_Q((KER<2, true>), (KER<2, false>), false, RC_WQ_L, g, st, s, stream, __VA_ARGS__); \
     } while (0)
 
 hipError_t rc_launch_adj_unit(int trav, const rc_dscene* s, const rc_dstate* st, float* G, uint64_t* SG, hipStream_t stream) {

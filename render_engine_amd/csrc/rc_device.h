@@ -304,14 +304,15 @@ __device__ __forceinline__ void rc_child_enter_q(const rc_wray& r, const rc_dsce
 typedef float rc_f4u __attribute__((ext_vector_type(4), aligned(4)));
 
 // Lane stack and leaf queue of the wide traversal: LDS levels one column per lane (256 entries
-// apart), the RC_WQ queue levels after the first cap levels. rc_lstk_l keeps every stack level in
-// LDS (cap = wstack); rc_lstk_s keeps the first cap levels in LDS and the deeper ones in global
+// apart), the queue levels (RC_WQ, or RC_WQ_L for the kernels launched with them) after the first
+// cap levels. rc_lstk_l keeps every stack level in LDS (cap = wstack); rc_lstk_s keeps the first cap levels in LDS and the deeper ones in global
 // memory (uniform base < 4 GiB; level cap + k of thread g at word k * ostride + g, g = the
 // thread's index in the grid), which only a ray nested deeper than cap touches. The stack never
 // holds more than wstack entries (at most 3 pushes per wide level).
 struct rc_lstk_l {
     uint32_t* col;        // this lane's LDS column
     int cap;
+    int qwalk;            // a lane holding more queued leaf ranges than this starts the wave's leaf walk (queue levels - 4)
     __device__ __forceinline__ void push(int sp, uint32_t e) const { col[sp * 256] = e; }
     __device__ __forceinline__ uint32_t pop(int sp) const { return col[sp * 256]; }
     __device__ __forceinline__ uint32_t* queue() const { return col + (size_t)cap * 256; }
@@ -321,6 +322,7 @@ struct rc_lstk_s {
     uint32_t* ovf;
     uint32_t ostride;
     int cap;
+    int qwalk;            // as rc_lstk_l::qwalk
     // byte offset of global level sp (32-bit: a scalar base plus one vector offset per access)
     __device__ __forceinline__ uint32_t off(int sp) const {
         return ((uint32_t)(sp - cap) * ostride + blockIdx.x * blockDim.x + threadIdx.x) * 4u;
@@ -344,11 +346,12 @@ struct rc_lstk_s {
 
 // 4-wide BVH closest hit of the static geometry, order-independent rule, with postponed
 // leaves. A node step tests the node's four child boxes, queues the triangle ranges of the
-// entered leaf children (one LDS column per lane, RC_WQ levels after the stack; entries
+// entered leaf children (one LDS column per lane, RC_WQ or RC_WQ_L levels after the stack; entries
 // first << 7 | count) and goes on with the nearest entered inner child, the others pushed
 // far-to-near (LDS stack, 32-bit entries node << 8 | code(t)), or with a pop. The wave walks
 // the queued triangles of all its lanes together once every lane still traversing has some
-// queued (or one queue could overflow), so the triangle loop runs with most lanes busy
+// queued (or one lane holds more than lstk.qwalk, the queue levels - 4, so that the next node step's
+// four children still fit), so the triangle loop runs with most lanes busy
 // instead of after each node step for the few lanes that entered a leaf. Culling uses the
 // current tmin, which may lag the queued hits: more node steps at worst, never a missed hit,
 // and the hit itself does not depend on the order. Leaf children hold at most RC_WLEAF_MAX
@@ -365,7 +368,7 @@ __device__ __forceinline__ void rc_trace_wide_closest(f3 o, f3 d, const rc_dscen
     for (;;) {
         const bool trav = cur >= 0, pend = nq > 0;
         if (!__any(trav || pend)) break;
-        if (__any(pend) && (__all(pend || !trav) || __any(nq > RC_WQ - 4))) {
+        if (__any(pend) && (__all(pend || !trav) || __any(nq > lstk.qwalk))) {
             int k = 0, a = 0, b = 0;
             for (;;) {
                 if (a >= b) {
@@ -431,7 +434,7 @@ __device__ __forceinline__ bool rc_occluded_wide(f3 o, f3 d, const rc_dscene& s,
     for (;;) {
         const bool trav = cur >= 0, pend = nq > 0;
         if (!__any(trav || pend)) break;
-        if (__any(pend) && (__all(pend || !trav) || __any(nq > RC_WQ - 4))) {
+        if (__any(pend) && (__all(pend || !trav) || __any(nq > lstk.qwalk))) {
             int k = 0, a = 0, b = 0;
             for (;;) {
                 if (a >= b) {

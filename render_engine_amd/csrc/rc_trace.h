@@ -55,6 +55,7 @@ __device__ __forceinline__ rc_lstk_l rc_lstk_lds(float* sh, int nt, const rc_dsc
     rc_lstk_l k;
     k.col = rc_lstack(sh, nt) + (threadIdx.x & 255);
     k.cap = s.wstack;
+    k.qwalk = RC_WQ - 4;
     return k;
 }
 // the state's split for this launch: the first min(wstack, ocap) levels in LDS, deeper ones in
@@ -64,6 +65,7 @@ __device__ __forceinline__ rc_lstk_s rc_lstk_of(float* sh, const rc_dstate& st, 
     k.col = rc_lstack(sh, st.nt) + (threadIdx.x & 255);
     k.cap = st.ocap > 0 ? min(s.wstack, st.ocap) : s.wstack;
     k.ovf = st.ovf; k.ostride = gridDim.x * blockDim.x;
+    k.qwalk = RC_WQ - 4;
     return k;
 }
 
@@ -112,9 +114,11 @@ static inline bool rc_use_split(const void* ks, const void* kl, size_t shm_s, si
 // tie_split); returns
 // hipErrorInvalidValue from the enclosing launcher when the split one runs and the state's global
 // levels are too small for the grid.
-#define RC_LAUNCH_WIDE(KS, KL, TIE, grid, st, s, stream, ...)                                                    \
+#define RC_LAUNCH_WIDE(KS, KL, TIE, grid, st, s, stream, ...) RC_LAUNCH_WIDE_Q(KS, KL, TIE, RC_WQ, grid, st, s, stream, __VA_ARGS__)
+// the same with WQ queue levels per lane
+#define RC_LAUNCH_WIDE_Q(KS, KL, TIE, WQ, grid, st, s, stream, ...)                                             \
     do {                                                                                                    \
-        const size_t shm_s_ = rc_shm(*(st), *(s), 2, true), shm_l_ = rc_shm(*(st), *(s), 2, false);        \
+        const size_t shm_s_ = rc_shm(*(st), *(s), 2, true, WQ), shm_l_ = rc_shm(*(st), *(s), 2, false, WQ); \
         if (((st)->oforce && (st)->ocap > 0) || rc_use_split((const void*)KS, (const void*)KL, shm_s_, shm_l_, TIE)) { \
             if (!rc_ovf_ok(*(st), *(s), 2, (grid).x)) return hipErrorInvalidValue;                          \
             hipLaunchKernelGGL(KS, grid, dim3(RC_TB), shm_s_, stream, __VA_ARGS__);                         \
@@ -133,10 +137,11 @@ template <> struct rc_lstk_sel<false> {
 };
 
 // (split: a kernel whose lane stacks follow the state's split, rc_lstk_of; else all levels in LDS)
-static inline size_t rc_shm(const rc_dstate& st, const rc_dscene& s, int trav, bool split = true) {
+// wq: the kernel's queue levels (RC_WQ, or RC_WQ_L for a kernel whose lane stacks set qwalk = RC_WQ_L - 4)
+static inline size_t rc_shm(const rc_dstate& st, const rc_dscene& s, int trav, bool split = true, int wq = RC_WQ) {
     size_t tri = (((size_t)st.nt * 10 + 3) & ~(size_t)3) * sizeof(float);
     const int lv = split && st.ocap > 0 && st.ocap < s.wstack ? st.ocap : s.wstack;
-    return tri + (trav >= 2 ? (size_t)(lv + RC_WQ) * 256 * sizeof(uint32_t) : 0);
+    return tri + (trav >= 2 ? (size_t)(lv + wq) * 256 * sizeof(uint32_t) : 0);
 }
 // global stack words a launch of nthreads threads needs (0: every level in LDS)
 static inline size_t rc_ovf_need(const rc_dstate& st, const rc_dscene& s, long long nthreads) {

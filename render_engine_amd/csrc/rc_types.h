@@ -53,7 +53,11 @@ typedef struct rc_qnode {
     int32_t child[4];                       // row 3
 } rc_qnode;
 #define RC_WSTACK 48          // most LDS stack levels per lane (48 KB per 256-lane block)
-#define RC_WQ 7               // queued leaf ranges per lane of the wide closest-hit traversal (LDS levels)
+// Queued leaf ranges per lane of the wide traversals (LDS levels after the stack): RC_WQ, the wave
+// walks the queued leaves once a lane holds 3 (1920x1080 frame +0.8 % against 7 levels walked at 4);
+// the adjoint's kernels keep RC_WQ_L = 7 walked at 4 (their short rays lose 0.5-1 % walking at 3)
+#define RC_WQ 6
+#define RC_WQ_L 7
 #define RC_LCAP 16            // LDS levels of the camera pass's lane stack (deeper levels: global overflow)
 
 // Per-state device view of the scene (cuda/poc_gpu.cu:222-237).
