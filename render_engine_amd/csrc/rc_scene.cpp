@@ -151,9 +151,70 @@ float quant_step_floor(const std::vector<rc_bvh_node>& bn, const float lo[3], co
     return std::ldexp(1.f, std::max(-60, (int)std::ceil(std::log2(std::max(b, 1e-6f))) - 18));
 }
 
+// Slot assignment of the 4-wide collapse by dynamic programming over the binary tree (surface-area
+// cost of the wide nodes; the leaves are the binary leaves whatever the choice, so their cost is
+// fixed): S[n][k-1] = least cost of representing subtree n by at most k children of its parent,
+// either n itself (a leaf, or a wide node: area(n) + the best 4-slot split of its two children) or
+// j slots for its left child and k - j for its right one. split[n][k-1] = the j of that split (0:
+// n itself). Children follow their parent in the array (the builder's breadth-first order). Against
+// opening the largest-area child first: 1.1-1.3 % less wide-node area cost on the streets, the
+// 1920x1080 camera pass 0.5 % faster, the same hits.
+struct WideDP {
+    std::vector<std::array<float, 4>> S;
+    std::vector<std::array<uint8_t, 4>> split;
+};
+static bool wide_dp(const std::vector<rc_bvh_node>& bn, WideDP& dp) {
+    const size_t nb = bn.size();
+    dp.S.assign(nb, {0.f, 0.f, 0.f, 0.f});
+    dp.split.assign(nb, {0, 0, 0, 0});
+    for (size_t i = nb; i-- > 0;) {
+        const rc_bvh_node& n = bn[i];
+        std::array<float, 4>& S = dp.S[i];
+        std::array<uint8_t, 4>& sp = dp.split[i];
+        if (n.cnt > 0) { S = {0.f, 0.f, 0.f, 0.f}; continue; }   // leaf: its cost is the same in every collapse
+        const size_t L = (size_t)n.lf, R = L + 1;
+        if (L <= i || R >= nb) return false;
+        const std::array<float, 4>&SL = dp.S[L], &SR = dp.S[R];
+        float d4 = std::numeric_limits<float>::infinity();
+        for (int j = 1; j <= 3; ++j) d4 = std::min(d4, SL[j - 1] + SR[4 - j - 1]);
+        S[0] = rc_bvh_area(n.lo, n.hi) + d4; sp[0] = 0;
+        for (int k = 2; k <= 4; ++k) {
+            S[k - 1] = S[k - 2]; sp[k - 1] = sp[k - 2];
+            for (int j = 1; j < k; ++j) {
+                const float c = SL[j - 1] + SR[k - j - 1];
+                if (c < S[k - 1]) { S[k - 1] = c; sp[k - 1] = (uint8_t)j; }
+            }
+        }
+        // where k slots gain nothing over k - 1, sp repeats the k - 1 choice; wide_collect then hands
+        // the right child one slot more, which S (non-increasing in k) prices no higher
+    }
+    return true;
+}
+// the binary nodes that take the k slots of subtree n (k >= 1)
+static void wide_collect(const std::vector<rc_bvh_node>& bn, const WideDP& dp, int n, int k, std::vector<int>& out) {
+    const int j = bn[n].cnt > 0 ? 0 : dp.split[n][k - 1];
+    if (j == 0) { out.push_back(n); return; }
+    wide_collect(bn, dp, bn[n].lf, j, out);
+    wide_collect(bn, dp, bn[n].lf + 1, k - j, out);
+}
+// the 4-slot split of inner node n's two children
+static void wide_children(const std::vector<rc_bvh_node>& bn, const WideDP& dp, int n, std::vector<int>& out) {
+    const int L = bn[n].lf, R = L + 1;
+    int bj = 1;
+    float best = std::numeric_limits<float>::infinity();
+    for (int j = 1; j <= 3; ++j) {
+        const float c = dp.S[L][j - 1] + dp.S[R][4 - j - 1];
+        if (c < best) { best = c; bj = j; }
+    }
+    wide_collect(bn, dp, L, bj, out);
+    wide_collect(bn, dp, R, 4 - bj, out);
+}
+
 void build_wide(const std::vector<rc_bvh_node>& bn, std::vector<rc_wnode>& wn, int& depth_out) {
     wn.clear();
     depth_out = 0;
+    WideDP dp;
+    const bool use_dp = wide_dp(bn, dp);   // else (a child before its parent) the greedy rule below
     // a job is a binary node (first < 0) or a range of more than RC_WLEAF_MAX triangles of the binary leaf
     // bnode (first .. first + count - 1), split over wide children with the leaf's box
     struct Job { int bnode, first, count, wslot, depth; };
@@ -170,6 +231,10 @@ void build_wide(const std::vector<rc_bvh_node>& bn, std::vector<rc_wnode>& wn, i
             for (int a = jb.first; a < jb.first + jb.count; a += q) kids.push_back({jb.bnode, a, std::min(q, jb.first + jb.count - a)});
         } else if (bn[jb.bnode].cnt > 0) {
             kids.push_back({jb.bnode, -1, 0});
+        } else if (use_dp) {
+            std::vector<int> ch;
+            wide_children(bn, dp, jb.bnode, ch);
+            for (int c : ch) kids.push_back({c, -1, 0});
         } else {
             kids.push_back({bn[jb.bnode].lf, -1, 0});
             kids.push_back({bn[jb.bnode].lf + 1, -1, 0});

@@ -109,8 +109,8 @@ void fine_events(Scene& S, int g);
 void build_bvh(const Mesh& M, std::vector<rc_bvh_node>& nodes, std::vector<int>& order, int& depth);
 // The same tree built on the GPU (rc_prep.hip), bit for bit; "" on success, else the error.
 std::string build_bvh_device(const Mesh& M, std::vector<rc_bvh_node>& nodes, std::vector<int>& order, int& depth);
-// Collapse of the binary BVH into 4-wide nodes (children: up to 4 descendants, leaves keep
-// their triangle ranges; a leaf of more than RC_WLEAF_MAX triangles becomes a node of its own whose
+// Collapse of the binary BVH into 4-wide nodes (children: up to 4 descendants chosen by least
+// surface-area cost over the binary tree, leaves keep their triangle ranges; a leaf of more than RC_WLEAF_MAX triangles becomes a node of its own whose
 // children are near-equal chunks of its range, recursively, each with the leaf's box).
 void build_wide(const std::vector<rc_bvh_node>& bn, std::vector<rc_wnode>& wn, int& depth);
 // the smallest quantization step of a scene's quantized nodes: 2^-18 of its largest absolute coordinate
