@@ -287,6 +287,68 @@ def test_bvh_structure(street):
     assert reached[0] == 0 and np.all(reached[1:] == 1)
 
 
+
+def _area(lo, hi):
+    d = np.maximum(np.asarray(hi, np.float64) - np.asarray(lo, np.float64), 0.0)
+    return 2.0 * (d[0] * d[1] + d[1] * d[2] + d[2] * d[0])
+
+
+@pytest.mark.parametrize("which", ["room", "street"])
+def test_wide_collapse_is_area_optimal(which, request):
+    """the 4-wide collapse's total node area (sum over wide nodes of the area of the union of their
+    child boxes) equals the least one over every way of cutting the binary tree into nodes of 2..4
+    children, found here by enumerating each node's child sets (repeatedly replacing an inner
+    member by its two children while fewer than 4), and is at most the largest-area-first rule's"""
+    scene, _ = request.getfixturevalue(which)
+    bn = scene.array("BN").reshape(-1, 8)
+    lo, hi = bn[:, 0:3].view(np.float32).astype(np.float64), bn[:, 4:7].view(np.float32).astype(np.float64)
+    lf, cnt = bn[:, 3].astype(np.int64), bn[:, 7].astype(np.int64)
+    A = [_area(lo[k], hi[k]) for k in range(len(bn))]
+    W = {}
+
+    def sets(n):   # every child set of a wide node for inner node n
+        out, todo = set(), [frozenset((int(lf[n]), int(lf[n]) + 1))]
+        while todo:
+            f = todo.pop()
+            if f in out:
+                continue
+            out.add(f)
+            if len(f) < 4:
+                for m in f:
+                    if cnt[m] == 0:
+                        todo.append((f - {m}) | {int(lf[m]), int(lf[m]) + 1})
+        return out
+
+    for n in range(len(bn) - 1, -1, -1):   # children follow their parent
+        if cnt[n] == 0:
+            W[n] = A[n] + min(sum(W[m] for m in f if cnt[m] == 0) for f in sets(n))
+    best = W[0]
+    # largest-area-first: open the inner member with the largest area while fewer than 4
+    def greedy(n):
+        f = [int(lf[n]), int(lf[n]) + 1]
+        while len(f) < 4:
+            inner = [m for m in f if cnt[m] == 0]
+            if not inner:
+                break
+            m = max(inner, key=lambda x: A[x])
+            i = f.index(m)
+            f[i:i + 1] = [int(lf[m]), int(lf[m]) + 1]
+        return A[n] + sum(greedy(m) for m in f if cnt[m] == 0)
+    g = greedy(0)
+    wn = scene.array("WN").reshape(-1, 32)
+    wf = wn.view(np.float32).astype(np.float64)
+    ch = wn[:, 24:28]
+    got = 0.0
+    for k in range(len(wn)):
+        m = ch[k] != 0
+        klo = [wf[k, 4 * a:4 * a + 4][m].min() for a in range(3)]
+        khi = [wf[k, 12 + 4 * a:16 + 4 * a][m].max() for a in range(3)]
+        got += _area(klo, khi)
+    print(f"[wide] {which}: {len(wn)} wide nodes, area cost {got:.6g}, optimum {best:.6g}, largest-area-first {g:.6g}")
+    assert abs(got - best) <= 1e-5 * best
+    assert got <= g * (1 + 1e-6)
+
+
 # ---------------------------------------------------------------- DDA
 def _exact_voxels(osc_state, info, a, b):
     """voxels whose closed box meets the segment (slab test in double precision)"""
