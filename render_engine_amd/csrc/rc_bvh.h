@@ -1,6 +1,6 @@
 // Split rule of the binned-SAH BVH build, shared by the host builder (rc_scene.cpp) and the
 // device builder (rc_prep.hip) so that both make the same decisions from the same numbers:
-// 32 centroid bins on each of the three axes, the cheapest split over all of them (axes in
+// 64 centroid bins on each of the three axes, the cheapest split over all of them (axes in
 // order, bins in order, strict '<'), leaves of at most RC_BVH_LEAF triangles or when the SAH
 // says so, a middle split (no reordering) for clusters of equal centroids, depth <= 56
 // (cuda/prep_mesh.cuh:42-114 bins 16 along the longest axis only).
@@ -11,7 +11,7 @@
 #define RC_HD
 #endif
 
-#define RC_BVH_NB 32
+#define RC_BVH_NB 64
 #define RC_BVH_LEAF 2
 
 struct rc_bins {   // one axis: triangle count and box of each centroid bin

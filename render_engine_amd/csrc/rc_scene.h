@@ -101,7 +101,7 @@ struct Scene {
 // disappears between states a and b (both placements), for every ordered pair.
 void fine_events(Scene& S, int g);
 
-// Binned-SAH binary BVH over the mesh (split rule in rc_bvh.h: 32 bins on each of the three
+// Binned-SAH binary BVH over the mesh (split rule in rc_bvh.h: 64 bins on each of the three
 // axes, leaves of <= 2 triangles or when SAH says so, a middle split for equal centroids,
 // depth <= 56; the reference bins 16 along the longest axis, cuda/prep_mesh.cuh:42-114), built
 // breadth first with stable partitions. Hits do not depend on the tree's shape: the
